@@ -1,0 +1,168 @@
+"""Benchmark of the per-sample path-tracing hot path on one MI355X.
+
+Workload (one step = one frame): the triangle-heavy single frame -- a
+noise-displaced icosphere of 1,310,720 triangles in rough glass (IOR 1.5,
+alpha 0.05) on a 2-triangle copper plane (alpha 0.3), lit by a 1024x512
+synthetic sky+sun environment; 1024x1024 pixels, 64 samples per pixel
+(67,108,864 path samples), Sobol sampler, max depth 16, thin-lens depth of
+field with lens diameter 0.02, the reference's NEE estimator (4 light samples
+per non-specular vertex) with Russian roulette from vertex 3.
+
+Prints one JSON line.  `value` is path samples per second over K timed frames
+(scene resident in HBM).  `roofline` is for the kernel with the largest share
+of device time, from algorithmic bytes (HIP-event-timed launches, counters of
+an identical counting frame).  `cpu_baseline` times the CPU oracle (the C
+restatement of the reference's estimator, OpenMP) on the host cores over a
+bounded slice of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpath-samples/s on 1 MI355X; relMSE vs 4096-spp reference at equal time"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (layout constants of pathtracer_amd/csrc/pt_scene.h)
+B_EXT_RAY = 4 + 24 + 8        # queue index + ray (o, d) read + hit (t, prim) write
+B_SHADOW_RAY = 44             # shadow-queue entry (o, d, tmax, contribution, slot)
+B_NODE = 64                   # BVH2 node: both child boxes + child links
+B_PRIM = 48                   # triangle record (v0, e1, e2)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(spec, cfg, target_s=15.0):
+    from oracle import oracle_py as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    o = O.OracleScene(spec)
+    build_s = time.perf_counter() - t0
+    total = cfg.width * cfg.height * cfg.spp_x * cfg.spp_y
+    # spread the slice over the frame: a band of rows in the middle of the image
+    start = (total // 2) // (cfg.spp_x * cfg.spp_y) * (cfg.spp_x * cfg.spp_y)
+    n = 4096
+    t = 0.0
+    while True:
+        t1 = time.perf_counter()
+        o.render_range(cfg, start, n, threads)
+        t = time.perf_counter() - t1
+        if t > 1.0 or n >= 1 << 24:
+            break
+        n *= 4
+    n_final = max(n, int(n * target_s / max(t, 1e-3)))
+    n_final = min(n_final, total - start)
+    t1 = time.perf_counter()
+    o.render_range(cfg, start, n_final, threads)
+    t = time.perf_counter() - t1
+    return {"value": n_final / t / 1e6, "unit": "Mpath-samples/s", "cores": threads, "kind": "port",
+            "sample": f"{n_final} consecutive path samples (pixel-major) from the middle of the same frame, "
+                      f"{t:.1f} s on {threads} OpenMP threads; oracle BVH/scene build {build_s:.1f} s untimed"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--estimator", type=int, default=0, help="0: reference NEE, 1: NEE+MIS")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--small", action="store_true", help="256x256 quick check (not a bench line)")
+    ap.add_argument("--save", default="", help="write the film as PFM/PNG with this path prefix")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path would shard by pixel tiles with one film reduction; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from pathtracer_amd import _lib as L, render as R, scenes
+
+    t0 = time.perf_counter()
+    if args.small:
+        spec = scenes.glass_hdri(width=256, height=256, spp_x=4, spp_y=4)
+    else:
+        spec = scenes.glass_hdri()
+    log(f"scene generated in {time.perf_counter() - t0:.1f} s: {spec.n_triangles} triangles")
+    t0 = time.perf_counter()
+    scene = R.Scene(spec)
+    log(f"BVH built and uploaded in {time.perf_counter() - t0:.1f} s: {scene.info.bvh_nodes} nodes, "
+        f"depth {scene.info.bvh_depth}")
+    cfg = scene.cfg(estimator=args.estimator)
+    total = cfg.width * cfg.height * cfg.spp_x * cfg.spp_y
+
+    # counting frame (untimed): traversal counters for the roofline
+    ccfg = scene.cfg(estimator=args.estimator, flags=L.RENDER_COUNTERS)
+    cst = scene.render_device(ccfg)
+    log(f"counting frame: {cst['segments']} extension rays, {cst['shadow_rays']} shadow rays")
+
+    tcfg = scene.cfg(estimator=args.estimator, flags=L.RENDER_TIMING)
+    for i in range(args.warmup):
+        scene.render_device(tcfg)
+        log(f"warmup {i} done")
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    t1 = time.perf_counter()
+    stats = []
+    for i in range(args.steps):
+        stats.append(scene.render_device(tcfg))
+        log(f"step {i}: {stats[-1]['ms_total']:.1f} ms")
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    dt = time.perf_counter() - t1
+    value = total * args.steps / dt / 1e6
+
+    # roofline of the dominant kernel (HIP-event times of the timed frames)
+    ms = {k: sum(s[k] for s in stats) / args.steps for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate",
+                                                              "ms_film")}
+    launches = stats[0]["extend_launches"]
+    ext_bytes = cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE + cst["prim_tests"] * B_PRIM
+    sh_bytes = cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_NODE + \
+        cst["shadow_prim_tests"] * B_PRIM
+    if ms["ms_extend"] >= ms["ms_shadow"]:
+        kname, kbytes, kms = "k_extend (closest-hit traversal)", ext_bytes, ms["ms_extend"]
+    else:
+        kname, kbytes, kms = "k_shadow (any-hit traversal)", sh_bytes, ms["ms_shadow"]
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+                "launches_per_frame": launches, "bytes_per_launch": int(kbytes / max(launches, 1)),
+                "ms_per_launch": round(kms / max(launches, 1), 4)}
+
+    res = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "glass_hdri: 1.31M-tri displaced rough-glass icosphere + copper plane + 1024x512 "
+                               "sky/sun HDRI, thin-lens DoF" if not args.small else "glass_hdri small",
+                   "width": cfg.width, "height": cfg.height, "spp": cfg.spp_x * cfg.spp_y,
+                   "path_samples_per_step": total, "max_depth": cfg.max_depth, "sampler": "sobol",
+                   "estimator": ["reference_nee", "nee_mis"][args.estimator], "rr_depth": cfg.rr_depth,
+                   "triangles": spec.n_triangles},
+        "mrays_per_s": round((cst["segments"] + cst["shadow_rays"]) * args.steps / dt / 1e6, 1),
+        "kernel_ms_per_step": {k: round(v, 2) for k, v in ms.items()},
+        "counters_per_frame": {k: cst[k] for k in ("segments", "shadow_rays", "node_visits", "prim_tests",
+                                                    "shadow_node_visits", "shadow_prim_tests", "nan_samples")},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if args.save:
+        from pathtracer_amd import film_io
+        import numpy as np
+        img = np.empty((cfg.height, cfg.width, 3), np.float32)
+        L.check(L.lib.pt_film_read(scene.h, L.ptr(img)), "pt_film_read")
+        film_io.write_pfm(args.save + ".pfm", img)
+        film_io.write_png(args.save + ".png", img)
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(spec, cfg)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,262 @@
+/*
+ * pathtracer_amd.h -- C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * The reference renderer (747745124/PathTracer) has no FFI layer: its
+ * boundary is a set of C++ abstract classes bound at compile time.  Each
+ * entry point below replaces one of those interfaces; the comment on each
+ * names the reference file:line it stands in for.  A source-compatible C++
+ * scene API that lowers onto these calls lives in
+ * include/pathtracer_amd/scene.hpp.
+ *
+ * Conventions
+ *   - Every call returns a pt_status (0 = PT_OK).  Nothing throws across the
+ *     ABI; pt_last_error() returns a thread-local message for the last
+ *     failing call on this thread.
+ *   - Host buffers are caller-owned.  Device buffers are owned by pt_scene.
+ *   - All arrays are plain C arrays; vectors are packed float triples.
+ *   - A pt_scene is used by one host thread at a time (thread-compatible,
+ *     not thread-safe), and owns one HIP stream.
+ */
+#ifndef PATHTRACER_AMD_H
+#define PATHTRACER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pt_status;
+enum {
+  PT_OK = 0,
+  PT_ERR_INVALID = 1,     /* bad argument / invalid scene state            */
+  PT_ERR_DEVICE = 2,      /* HIP runtime error                             */
+  PT_ERR_NOMEM = 3,       /* host or device allocation failed              */
+  PT_ERR_UNSUPPORTED = 4, /* feature not available                         */
+  PT_ERR_NO_DEVICE = 5    /* no gfx950 device visible                      */
+};
+
+/* ------------------------------------------------------------------ */
+/* Materials  (reference: class Material, material/material.hpp:15-77) */
+/* ------------------------------------------------------------------ */
+enum pt_mat_kind {
+  PT_MAT_LAMBERTIAN = 1,     /* material.hpp:79-137        p: albedo[3]          */
+  PT_MAT_EMITTER = 2,        /* material.hpp:467-506       p: color[3], intensity */
+  PT_MAT_CONDUCTOR = 3,      /* conductor.hpp:4-145        p: eta[3], k[3], ax, ay */
+  PT_MAT_MFDIELECTRIC = 4,   /* dielectric.hpp:5-167       p: eta, ax, ay        */
+  PT_MAT_THINDIELECTRIC = 5, /* thinDielectric.hpp:3-70    p: eta, split(0/1)    */
+  PT_MAT_PHONG = 6,          /* material.hpp:303-465       p: diffuse[3], specular[3], ambient[3], shininess */
+  PT_MAT_DISNEY = 7,         /* disneyPrincipledBSDF.hpp:8-99 p: baseColor[3], specTrans, metallic,
+                                subsurface, specular, roughness, specularTint, anisotropic, sheen,
+                                sheenTint, clearcoat, clearcoatGloss, eta   (constructor order) */
+  PT_MAT_HAIR = 8,           /* hairMarsch.hpp:3-158       p: sigma_a[3], h, eta, beta_m, beta_n, alpha */
+  PT_MAT_KAJIYA_KAY = 9      /* no reference code (README.md:74 lists it)  p: diffuse[3], specular[3], exponent */
+};
+
+#define PT_MAT_NPARAMS 30
+typedef struct pt_material_desc {
+  uint32_t kind;              /* pt_mat_kind                                */
+  uint32_t flags;             /* reserved, 0                                */
+  float p[PT_MAT_NPARAMS];    /* kind-specific parameters, see pt_mat_kind  */
+} pt_material_desc;           /* 128 bytes                                  */
+
+/* Primitive flags.  The reference's wrappers change HitRecord::is_inside:
+ * FlipFace toggles it (objectTransform.hpp:21); Translate/Rotate/Scale re-run
+ * set_normal on an already ray-opposed normal, which forces it true
+ * (objectTransform.hpp:60,156; objectTransform.cpp:132,172,211). */
+enum {
+  PT_PRIM_FLIP_FACE = 1u << 0,
+  PT_PRIM_FORCE_INSIDE = 1u << 1
+};
+
+/* ------------------------------------------------------------------ */
+/* Camera  (reference: PerspectiveCamera, base/camera.cpp:7-56)        */
+/* ------------------------------------------------------------------ */
+typedef struct pt_camera_desc {
+  float fov;          /* vertical field of view, radians (camera.cpp:12)        */
+  float aspect;       /* width / height                                         */
+  float focal_dist;   /* image-plane distance                                   */
+  float aperture;     /* reference semantics: lens half-width = focal_dist/aperture/2,
+                         lens point uniform on a square (camera.cpp:44-50);
+                         0 or +inf = pinhole                                     */
+  float up[3];
+  float front[3];
+  float pos[3];
+  float pos1[3];      /* camera position at shutter close (motion blur)         */
+  float shutter;      /* 0 = static camera; >0: position = lerp(pos,pos1,time)   */
+  uint32_t normalize_dir; /* 1: camera rays carry unit directions               */
+} pt_camera_desc;
+
+/* ------------------------------------------------------------------ */
+/* Samplers  (reference: HaltonSampler, sampler/sampler.hpp:16-83)     */
+/* ------------------------------------------------------------------ */
+enum pt_sampler_kind {
+  PT_SAMPLER_HALTON_OWEN = 0,  /* RandomStrategy::Owen  (default, sampler.hpp:67-71)   */
+  PT_SAMPLER_HALTON_PERM = 1,  /* RandomStrategy::PermuteDigits (sampler.hpp:60-65)     */
+  PT_SAMPLER_HALTON_NONE = 2,  /* RandomStrategy::None  (sampler.hpp:57-58)            */
+  PT_SAMPLER_SOBOL = 3,        /* Owen-scrambled Sobol; no reference code (README.md:98) */
+  PT_SAMPLER_STRATIFIED = 4    /* per-pixel jittered strata with permuted strata order  */
+};
+
+/* Integrator estimators (reference: getRayColor, integrator/maxdepth_nee.hpp:9-256) */
+enum pt_estimator {
+  PT_EST_REFERENCE_NEE = 0, /* the reference's estimator: N light samples per
+                               non-specular vertex, emitter hits excluded after
+                               non-specular vertices, env radiance also taken on
+                               BSDF misses (maxdepth_nee.hpp:26-37,91-255)       */
+  PT_EST_NEE_MIS = 1        /* one light sample + BSDF sample under the power
+                               heuristic; environment sampled by its 2-D CDF     */
+};
+
+typedef struct pt_render_cfg {
+  uint32_t width, height;
+  uint32_t spp_x, spp_y;   /* samples per pixel = spp_x*spp_y (framebuffer.cpp:131-145) */
+  uint32_t max_depth;      /* getRayColor(max_depth): vertices 0..max_depth-1 shade  */
+  uint32_t sampler;        /* pt_sampler_kind                                        */
+  uint32_t estimator;      /* pt_estimator                                           */
+  uint32_t rr_depth;       /* first vertex index with Russian roulette; 0 = off      */
+  uint32_t light_samples;  /* NEE samples per vertex for PT_EST_REFERENCE_NEE (4 = 2x2) */
+  uint32_t seed;           /* seeds the counter-based random stream                   */
+  float bg[3];             /* background radiance when there is no environment        */
+  uint32_t spp_begin;      /* render samples [spp_begin, spp_begin+spp_count)         */
+  uint32_t spp_count;      /* 0 = all samples                                         */
+  uint32_t wavefront;      /* path slots in flight; 0 = automatic                     */
+  uint32_t flags;          /* PT_RENDER_* bits                                        */
+} pt_render_cfg;
+
+enum {
+  PT_RENDER_COUNTERS = 1u << 0,   /* collect traversal counters into pt_stats       */
+  PT_RENDER_NO_SORT = 1u << 1,    /* disable material-key sorting of hits           */
+  PT_RENDER_ACCUMULATE = 1u << 2, /* add into the film instead of clearing it        */
+  PT_RENDER_KEEP_SAMPLES = 1u << 3, /* keep per-sample radiance (pt_film_read_samples) */
+  PT_RENDER_TIMING = 1u << 4      /* HIP-event timing of every kernel launch        */
+};
+
+typedef struct pt_stats {
+  uint64_t paths;          /* path samples traced                                */
+  uint64_t segments;       /* extension (closest-hit) rays                        */
+  uint64_t shadow_rays;    /* shadow (any-hit) rays                               */
+  uint64_t node_visits;    /* BVH node-pair fetches, extension rays               */
+  uint64_t prim_tests;     /* primitive tests, extension rays                     */
+  uint64_t shadow_node_visits;
+  uint64_t shadow_prim_tests;
+  uint64_t nan_samples;    /* path samples discarded for NaN/Inf radiance        */
+  uint64_t extend_launches;
+  double ms_total;         /* wall time of the render call (device work)         */
+  double ms_generate, ms_extend, ms_shade, ms_shadow, ms_film; /* HIP-event sums   */
+} pt_stats;
+
+/* ------------------------------------------------------------------ */
+/* Scene  (reference: SceneInfo, render/renderManager.hpp:35-158;       */
+/*         ObjectList::addObject objectList.hpp:73)                     */
+/* ------------------------------------------------------------------ */
+typedef struct pt_scene pt_scene;
+
+pt_status pt_scene_create(pt_scene **out);
+void pt_scene_destroy(pt_scene *scene);
+
+/* Material constructors (material/ headers).  Returns the material id.        */
+pt_status pt_scene_add_material(pt_scene *scene, const pt_material_desc *desc,
+                                uint32_t *out_id);
+/* TriangleMesh(MeshData, material)  (primitives/triangleMesh.cpp:5-26).
+ * pos: nverts*3 floats; idx: ntris*3 int32 (MeshData::indices).  No
+ * per-vertex normals: shading uses the geometric normal, meshBVH.cpp:121.   */
+pt_status pt_scene_add_mesh(pt_scene *scene, const float *pos, uint32_t nverts,
+                            const int32_t *idx, uint32_t ntris,
+                            uint32_t material, uint32_t flags);
+/* Sphere(center, radius, material)  (base/primitive.hpp:81-203)           */
+pt_status pt_scene_add_sphere(pt_scene *scene, const float center[3],
+                              float radius, uint32_t material, uint32_t flags);
+/* AARectangle<Axis>(k, d0min, d0max, d1min, d1max, material)
+ * (base/primitive.hpp:205-273).  axis: 0=X, 1=Y, 2=Z.                      */
+pt_status pt_scene_add_rect(pt_scene *scene, int axis, float k, float d0min,
+                            float d0max, float d1min, float d1max,
+                            uint32_t material, uint32_t flags);
+/* EnvironmentLight(hdri, scale)  (light/envLight.hpp:4-92).  rgb holds
+ * w*h texels, texel (x,y) at rgb[(y*w+x)*3], y=0 being the direction -Y
+ * (the reference stores texels[x][y] after a vertical flip, texture.cpp:39-42). */
+pt_status pt_scene_set_env(pt_scene *scene, const float *rgb, uint32_t w,
+                           uint32_t h, float scale);
+pt_status pt_scene_set_camera(pt_scene *scene, const pt_camera_desc *cam);
+
+typedef struct pt_build_cfg {
+  uint32_t leaf_size;   /* max primitives per BVH leaf (0 = 4)                */
+  uint32_t sah_bins;    /* SAH bins per axis (0 = 16)                         */
+  uint32_t lds_levels;  /* top BVH levels staged in LDS (0 = none)            */
+  uint32_t flags;
+} pt_build_cfg;
+
+/* Builds the BVH, discovers emissive rects/spheres as lights
+ * (base/lightDiscovery.hpp:5-81) and uploads everything to the device.      */
+pt_status pt_scene_commit(pt_scene *scene, const pt_build_cfg *cfg);
+
+typedef struct pt_scene_info {
+  uint32_t n_materials, n_triangles, n_spheres, n_rects, n_lights;
+  uint32_t bvh_nodes, bvh_depth;
+  uint64_t device_bytes;
+  double build_ms;
+} pt_scene_info;
+pt_status pt_scene_get_info(const pt_scene *scene, pt_scene_info *info);
+
+/* SceneInfo::render (renderManager.hpp:51-158): renders into the scene's
+ * device film.  The film holds per-pixel means, row j=0 at the bottom
+ * (FrameBuffer::setPixelColor(j,i), renderManager.hpp:149).                */
+pt_status pt_render(pt_scene *scene, const pt_render_cfg *cfg, pt_stats *stats);
+/* Copies the film (width*height*3 floats, row-major, j=0 first) to host.   */
+pt_status pt_film_read(pt_scene *scene, float *out_rgb);
+/* Per-sample radiance of the last render when PT_RENDER keeps it
+ * (n = width*height*spp*3 floats), ordered pixel-major then sample.        */
+pt_status pt_film_read_samples(pt_scene *scene, float *out_rgb, size_t n);
+
+/* ------------------------------------------------------------------ */
+/* Parity / test entry points (batched, executed on the device).        */
+/* ------------------------------------------------------------------ */
+
+/* HaltonSampler::SampleDimension (sampler.hpp:54-73) and the other
+ * samplers.  For Halton kinds: out[i] = value of dimension dim[i] at
+ * sequence index index[i].  For Sobol/stratified: index[i] is the sample
+ * number inside pixel[i], spp is the samples per pixel.                   */
+pt_status pt_sample(uint32_t sampler, uint32_t seed, uint32_t spp,
+                    const uint32_t *dim, const uint64_t *index,
+                    const uint32_t *pixel, size_t n, float *out);
+
+/* BVHNode::intersect (primitives/bvh.hpp:27-41) over the committed scene.
+ * rays: n*8 floats (ox,oy,oz,dx,dy,dz,tmin,tmax).  prim: primitive id
+ * (triangles first in insertion order, then spheres, then rects) or -1.   */
+pt_status pt_intersect(pt_scene *scene, const float *rays, size_t n,
+                       int any_hit, int32_t *prim, float *t);
+
+/* Material::f and Material::scatter_pdf(wo,wi,rec) for one material over n
+ * tuples.  wo, wi, nrm: n*3; is_inside: n bytes (HitRecord::is_inside).     */
+pt_status pt_bsdf_eval(const pt_material_desc *mat, const float *wo,
+                       const float *wi, const float *nrm,
+                       const uint8_t *is_inside, size_t n, float *f,
+                       float *pdf);
+/* Material::scatter with explicit samples (uc, u2).  Outputs: wi (n*3),
+ * attenuation (n*3), pdf (ScatterRecord::pdf_val), flags (BxDFFlags of the
+ * sampled lobe), ok (scatter returned true).  aux: one extra uniform per
+ * tuple for lobe choices the reference draws from rand_num (Phong).       */
+pt_status pt_bsdf_sample(const pt_material_desc *mat, const float *wo,
+                         const float *nrm, const uint8_t *is_inside,
+                         const float *uc, const float *u2, const float *aux,
+                         size_t n, float *wi, float *f, float *pdf,
+                         uint32_t *flags, uint8_t *ok);
+
+/* Light::get_sample / get_normal_at / pdf_value for light `light` of a
+ * committed scene (light/light.hpp:17-306, light/envLight.hpp:20-76).
+ * u2: n*2; ref: n*3 shading points.  Outputs p (n*3 point, or dir*1e5 for
+ * the environment), nrm (n*3), pdf (solid-angle pdf of pdf_value(ref, p-ref)),
+ * le (n*3, L_emit towards ref).                                             */
+pt_status pt_light_sample(pt_scene *scene, uint32_t light, const float *u2,
+                          const float *ref, size_t n, float *p, float *nrm,
+                          float *pdf, float *le);
+
+pt_status pt_device_synchronize(void);
+const char *pt_last_error(void);
+const char *pt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PATHTRACER_AMD_H */

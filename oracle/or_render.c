@@ -1,0 +1,416 @@
+/*
+ * or_render.c -- oracle restatement of the reference's render driver and
+ * NEE integrator, plus the NEE+MIS estimator the GPU path also offers.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ *   SceneInfo::render      render/renderManager.hpp:51-158
+ *   getRayColor (NEE)      integrator/maxdepth_nee.hpp:9-256
+ *   PerspectiveCamera      base/camera.cpp:7-56
+ *   FrameBuffer offsets    base/framebuffer.cpp:131-145, utils/random.hpp:47-63
+ *
+ * Random numbers the reference draws from gl::rand_num (lens, light choice,
+ * NEE jitter, Phong lobe) come from a counter-based hash keyed by (global
+ * sample index, vertex slot, seed), the same stream the GPU path uses, so a
+ * path sample is reproducible on both sides.  The Halton sample index of
+ * path sample k of pixel (i, j) is 1 + k + spp*(j + H*i): the order of the
+ * reference's single-threaded loops (renderManager.hpp:99-115).
+ *
+ * Pinning (tests/test_oracle_render.py): white-furnace closed form (an
+ * enclosing Lambertian sphere of albedo a lit by a uniform environment
+ * converges to the geometric series), a diffuse plane under a quad light
+ * against the analytic point-to-rectangle form factor, and estimator
+ * agreement (reference NEE vs NEE+MIS) where the reference is unbiased.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+#include "oracle.h"
+#include "or_scene.h"
+
+enum { SLOT_LENS_X = 0, SLOT_LENS_Y = 1, SLOT_TIME = 2, SLOT_AUX = 4, SLOT_RR = 5,
+       SLOT_MIS_LIGHT = 6, SLOT_NEE_OFF = 8, SLOT_NEE_LIGHT = 40 };
+static inline uint32_t slot(uint32_t d, uint32_t purpose) { return d * 64u + purpose; }
+
+typedef struct {
+  const or_scene *s;
+  const pt_render_cfg *cfg;
+  uint32_t spp, lx, ly, ns;
+  float *offx, *offy;
+} rctx;
+
+static void grid_factor(uint32_t n, uint32_t *a, uint32_t *b) {
+  uint32_t x = 1;
+  for (uint32_t f = 1; f * f <= n; ++f)
+    if (n % f == 0) x = f;
+  *a = x;
+  *b = n / x;
+}
+
+/* FrameBuffer::setOffsets (framebuffer.cpp:131-145): one jittered set per
+ * frame, shared by every pixel. */
+static void make_offsets(rctx *c) {
+  uint32_t sx = c->cfg->spp_x, sy = c->cfg->spp_y;
+  c->offx = (float *)malloc(sizeof(float) * c->spp);
+  c->offy = (float *)malloc(sizeof(float) * c->spp);
+  for (uint32_t i = 0; i < sx; ++i)
+    for (uint32_t j = 0; j < sy; ++j) {
+      uint32_t k = i * sy + j;
+      float a0 = (float)i / (float)sx, a1 = (float)(i + 1) / (float)sx;
+      float b0 = (float)j / (float)sy, b1 = (float)(j + 1) / (float)sy;
+      c->offx[k] = (float)(a0 + (a1 - a0) * (double)or_rng(0xffffffffu - k, 0u, c->cfg->seed ^ 0x5bd1e995u));
+      c->offy[k] = (float)(b0 + (b1 - b0) * (double)or_rng(0xffffffffu - k, 1u, c->cfg->seed ^ 0x5bd1e995u));
+    }
+}
+
+/* PerspectiveCamera::generateRay (camera.cpp:42-56), plus a shutter time
+ * that moves the camera from pos to pos1 (motion blur; zero shutter is the
+ * reference's static camera). */
+static void camera_ray(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint32_t g, v3 *o,
+                       v3 *d) {
+  const or_scene *s = c->s;
+  const pt_camera_desc *cam = &s->cam;
+  float W = (float)c->cfg->width, H = (float)c->cfg->height;
+  float u = ((float)i + c->offx[k]) / W;
+  float v = ((float)j + c->offy[k]) / H;
+  float radius = (cam->aperture > 0.f && isfinite(cam->aperture)) ? cam->focal_dist / cam->aperture * 0.5f : 0.f;
+  float lu = -radius + (radius - -radius) * or_rng(g, slot(0, SLOT_LENS_X), c->cfg->seed);
+  float lv = -radius + (radius - -radius) * or_rng(g, slot(0, SLOT_LENS_Y), c->cfg->seed);
+  v3 pos = V3(cam->pos[0], cam->pos[1], cam->pos[2]);
+  if (cam->shutter > 0.f) {
+    float t = or_rng(g, slot(0, SLOT_TIME), c->cfg->seed) * cam->shutter;
+    v3 p1 = V3(cam->pos1[0], cam->pos1[1], cam->pos1[2]);
+    pos = vadd(pos, vscale(vsub(p1, pos), t));
+  }
+  v3 front = V3(cam->front[0], cam->front[1], cam->front[2]);
+  v3 lookat = vneg(vnorm(front));
+  v3 bl = vsub(vsub(pos, vdivs(s->cam_horiz, 2.0f)), vdivs(s->cam_vert, 2.0f));
+  bl = vsub(bl, vscale(lookat, cam->focal_dist));
+  v3 ps = vadd(pos, vadd(vscale(s->cam_h, lu), vscale(s->cam_v, lv)));
+  v3 dir = vsub(vadd(vadd(bl, vscale(s->cam_horiz, u)), vscale(s->cam_vert, v)), ps);
+  if (cam->normalize_dir) dir = vnorm(dir);
+  *o = ps;
+  *d = dir;
+}
+
+int or_camera_rays(const or_scene *s, const pt_render_cfg *cfg, const uint32_t *pixel,
+                   const uint32_t *k, size_t n, float *rays) {
+  rctx c;
+  memset(&c, 0, sizeof(c));
+  c.s = s;
+  c.cfg = cfg;
+  c.spp = cfg->spp_x * cfg->spp_y;
+  make_offsets(&c);
+  for (size_t q = 0; q < n; ++q) {
+    uint32_t i = pixel[q] % cfg->width, j = pixel[q] / cfg->width;
+    uint32_t g = 1u + k[q] + c.spp * (j + cfg->height * i);
+    v3 o, d;
+    camera_ray(&c, i, j, k[q], g, &o, &d);
+    float *r = &rays[8 * q];
+    r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = d.x; r[4] = d.y; r[5] = d.z;
+    r[6] = 1e-4f; r[7] = 1e5f;
+  }
+  free(c.offx);
+  free(c.offy);
+  return 0;
+}
+
+static v3 miss_radiance(const or_scene *s, const pt_render_cfg *cfg, v3 d) {
+  if (s->env) return or_env_lookup(s, vnorm(d));
+  return V3(cfg->bg[0], cfg->bg[1], cfg->bg[2]);
+}
+
+/* The NEE loop of getRayColor (maxdepth_nee.hpp:90-232). */
+static v3 nee_reference(const rctx *c, const or_hit *h, v3 wo, const pt_material_desc *m,
+                        uint32_t g, uint32_t d, uint64_t *cnt) {
+  const or_scene *s = c->s;
+  v3 lt = vs(0.f);
+  int N = s->n_lights;
+  if (N <= 0 || c->ns == 0) return lt;
+  for (uint32_t si = 0; si < c->ns; ++si) {
+    uint32_t ii = si / c->ly, jj = si % c->ly; /* getOffsets(LX, LY) (random.hpp:47-63) */
+    float a0 = (float)ii / (float)c->lx, a1 = (float)(ii + 1) / (float)c->lx;
+    float b0 = (float)jj / (float)c->ly, b1 = (float)(jj + 1) / (float)c->ly;
+    float ox = (float)(a0 + (a1 - a0) * (double)or_rng(g, slot(d, SLOT_NEE_OFF + 2 * si), c->cfg->seed));
+    float oy = (float)(b0 + (b1 - b0) * (double)or_rng(g, slot(d, SLOT_NEE_OFF + 2 * si + 1), c->cfg->seed));
+    /* LightList::uniform_get (lightList.hpp:31-35) */
+    int li = (int)(or_rng(g, slot(d, SLOT_NEE_LIGHT + si), c->cfg->seed) * (float)N) % N;
+    const or_light *L = &s->lights[li];
+    v3 wi, le;
+    float pdf, fac;
+    int vis;
+    if (L->type == OR_LIGHT_ENV) {
+      v3 nl;
+      v3 lp = or_light_point(s, L, ox, oy, &nl);
+      wi = vnorm(vsub(lp, h->p));
+      if (cnt) cnt[2]++;
+      vis = !or_scene_intersect(s, h->p, wi, 0.001f, FLT_MAX, 1, NULL, NULL);
+      if (vis) {
+        le = or_env_lookup(s, wi);
+        pdf = (1.0f / (float)N) * (float)(1.f / (4.f * M_PI));
+        fac = fmaxf(0.f, vdot(h->n, wi));
+      } else {
+        le = vs(0.f); pdf = 1.f; fac = 0.f;
+      }
+    } else {
+      v3 nl;
+      v3 lp = or_light_point(s, L, ox, oy, &nl);
+      v3 ld = vsub(lp, h->p);
+      float len = vlen(ld);
+      ld = vdivs(ld, len);
+      wi = ld;
+      if (len < 0.001f) {
+        vis = 0;
+      } else {
+        if (cnt) cnt[2]++;
+        vis = !or_scene_intersect(s, h->p, ld, 0.001f, len - 0.001f, 1, NULL, NULL);
+      }
+      if (vis) {
+        le = (vdot(nl, vneg(wi)) > 0.f) ? L->le : vs(0.f);
+        pdf = (L->area <= 1e-7f) ? 0.f : 1.f / (L->area * (float)N);
+        float noi = fmaxf(0.f, vdot(h->n, wi));
+        float nol = fmaxf(0.f, vdot(nl, vneg(wi)));
+        float d2 = len * len;
+        fac = (d2 < 1e-7f) ? 0.f : (nol * noi) / d2;
+      } else {
+        le = vs(0.f); pdf = 1.f; fac = 0.f;
+      }
+    }
+    if (vis && pdf > 1e-7f) {
+      v3 f = or_bsdf_f(m, wo, wi, h);
+      lt = vadd(lt, vdivs(vscale(vmul(f, le), fac), pdf));
+    }
+  }
+  return vdivs(lt, (float)c->ns);
+}
+
+/* Solid-angle pdf with which the NEE+MIS estimator samples point p on
+ * light li from o. */
+static float light_pdf_sa(const or_scene *s, int li, v3 o, v3 p) {
+  const or_light *L = &s->lights[li];
+  if (L->type == OR_LIGHT_ENV) return or_env_pdf_dir(s, vsub(p, o));
+  v3 w = vsub(p, o);
+  float d2 = vdot(w, w);
+  v3 nl = (L->type == OR_LIGHT_QUAD) ? L->n : vnorm(vsub(p, L->c));
+  float c = fabsf(vdot(nl, vnorm(w)));
+  if (!(c > 0.f) || !(L->area > 0.f)) return 0.f;
+  return d2 / (L->area * c);
+}
+
+static inline float power2(float a, float b) { return (a * a) / (a * a + b * b); }
+
+/* One path sample.  Returns the radiance. */
+static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t *cnt) {
+  const or_scene *s = c->s;
+  const pt_render_cfg *cfg = c->cfg;
+  uint32_t g = 1u + k + c->spp * (j + cfg->height * i);
+  uint32_t pixel = j * cfg->width + i;
+  v3 ro, rd;
+  camera_ray(c, i, j, k, g, &ro, &rd);
+  v3 L = vs(0.f), beta = vs(1.f);
+  or_hit h;
+  int have = 0, hit = 0;
+  const uint32_t D = cfg->max_depth;
+  const int mis = cfg->estimator == PT_EST_NEE_MIS;
+  int prev_spec = 1;
+  float prev_pdf = 0.f;
+  v3 prev_p = ro;
+  const int N = s->n_lights;
+  if (cnt) cnt[0]++;
+  for (uint32_t d = 0;; ++d) {
+    if (!mis && d >= D) break; /* max_depth == 0 returns 0 (maxdepth_nee.hpp:16-17) */
+    if (!have) {
+      if (cnt) cnt[1]++;
+      hit = or_scene_intersect(s, ro, rd, 1e-4f, 1e5f, 0, &h, cnt ? cnt + 4 : NULL);
+    }
+    have = 0;
+    if (!hit) {
+      if (!mis) {
+        L = vadd(L, vmul(beta, miss_radiance(s, cfg, rd)));
+      } else if (prev_spec) {
+        if (d < D || d == 0) L = vadd(L, vmul(beta, miss_radiance(s, cfg, rd)));
+      } else if (s->env) {
+        float pl = or_env_pdf_dir(s, rd) / (float)N;
+        L = vadd(L, vscale(vmul(beta, or_env_lookup(s, rd)), power2(prev_pdf, pl)));
+      } else {
+        L = vadd(L, vmul(beta, miss_radiance(s, cfg, rd)));
+      }
+      break;
+    }
+    const pt_material_desc *m = &s->mats[h.mat];
+    if (mis && or_is_emitter(m)) {
+      int li = or_prim_light(s, h.prim);
+      if (prev_spec || li < 0) {
+        if (d < D) L = vadd(L, vmul(beta, or_emit(m, rd, &h)));
+      } else {
+        const or_light *Lg = &s->lights[li];
+        v3 le = (vdot(h.ng_light, vneg(vnorm(rd))) > 0.f) ? Lg->le : vs(0.f);
+        float pl = light_pdf_sa(s, li, prev_p, h.p) / (float)N;
+        L = vadd(L, vscale(vmul(beta, le), power2(prev_pdf, pl)));
+      }
+      break;
+    }
+    if (mis && d >= D) break;
+    float uc = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d);
+    float u0 = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d + 1);
+    float u1 = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d + 2);
+    float aux = or_rng(g, slot(d, SLOT_AUX), cfg->seed);
+    or_srec sr;
+    if (!or_scatter(m, rd, &h, uc, u0, u1, aux, &sr)) {
+      L = vadd(L, vmul(beta, or_emit(m, rd, &h)));
+      break;
+    }
+    if (sr.flags & OR_SPECULAR) {
+      /* maxdepth_nee.hpp:73-78 (and the split-ray case taken stochastically) */
+      beta = vmul(beta, sr.att);
+      ro = h.p;
+      rd = sr.wi;
+      prev_spec = 1;
+    } else {
+      v3 wo = vneg(vnorm(rd));
+      v3 wi = vnorm(sr.wi);
+      float cs = fabsf(vdot(h.n, wi));
+      float pdf = sr.pdf;
+      if (!mis) {
+        L = vadd(L, vmul(beta, nee_reference(c, &h, wo, m, g, d, cnt)));
+        if (cnt) cnt[1]++;
+        hit = or_scene_intersect(s, h.p, wi, 1e-4f, 1e5f, 0, &h, cnt ? cnt + 4 : NULL);
+        have = 1;
+        if (hit && or_is_emitter(&s->mats[h.mat])) break; /* maxdepth_nee.hpp:246-247 */
+      } else if (N > 0) {
+        int li = (int)(or_rng(g, slot(d, SLOT_MIS_LIGHT), cfg->seed) * (float)N) % N;
+        const or_light *Lg = &s->lights[li];
+        float e0 = or_rng(g, slot(d, SLOT_NEE_OFF), cfg->seed);
+        float e1 = or_rng(g, slot(d, SLOT_NEE_OFF + 1), cfg->seed);
+        v3 lwi, le = vs(0.f);
+        float pl = 0.f;
+        int vis = 0;
+        if (Lg->type == OR_LIGHT_ENV) {
+          lwi = or_env_sample_dir(s, e0, e1, &pl);
+          if (pl > 0.f) {
+            if (cnt) cnt[2]++;
+            vis = !or_scene_intersect(s, h.p, lwi, 0.001f, FLT_MAX, 1, NULL, NULL);
+            le = or_env_lookup(s, lwi);
+          }
+        } else {
+          v3 nl;
+          v3 lp = or_light_point(s, Lg, e0, e1, &nl);
+          v3 ld = vsub(lp, h.p);
+          float len = vlen(ld);
+          lwi = vdivs(ld, len);
+          float nol = vdot(nl, vneg(lwi));
+          if (len >= 0.001f && nol > 0.f && Lg->area > 0.f) {
+            pl = len * len / (Lg->area * nol);
+            if (cnt) cnt[2]++;
+            vis = !or_scene_intersect(s, h.p, lwi, 0.001f, len - 0.001f, 1, NULL, NULL);
+            le = Lg->le;
+          }
+        }
+        if (vis && pl > 0.f) {
+          pl /= (float)N;
+          v3 f = or_bsdf_f(m, wo, lwi, &h);
+          float pb = or_bsdf_pdf(m, wo, lwi, &h);
+          float w = power2(pl, pb);
+          float cl = fabsf(vdot(h.n, lwi));
+          L = vadd(L, vscale(vmul(beta, vmul(f, le)), cl / pl * w));
+        }
+      }
+      beta = vscale(vmul(beta, sr.att), cs / pdf);
+      prev_pdf = pdf;
+      prev_p = h.p;
+      prev_spec = 0;
+      ro = h.p;
+      rd = wi;
+    }
+    if (cfg->rr_depth && d >= cfg->rr_depth) {
+      float q = fminf(1.f, vmaxc(beta));
+      if (!(or_rng(g, slot(d, SLOT_RR), cfg->seed) < q)) break;
+      beta = vdivs(beta, q);
+    }
+  }
+  return L;
+}
+
+static int setup(rctx *c, const or_scene *s, const pt_render_cfg *cfg) {
+  memset(c, 0, sizeof(*c));
+  if (!s->committed || !s->has_cam) return -1;
+  c->s = s;
+  c->cfg = cfg;
+  c->spp = cfg->spp_x * cfg->spp_y;
+  c->ns = cfg->light_samples;
+  grid_factor(c->ns ? c->ns : 1, &c->lx, &c->ly);
+  make_offsets(c);
+  return 0;
+}
+
+int or_render(const or_scene *s, const pt_render_cfg *cfg, float *out_rgb, float *out_var,
+              float *out_samples, uint64_t *counters, int threads) {
+  rctx c;
+  if (setup(&c, s, cfg)) return -1;
+  uint32_t W = cfg->width, H = cfg->height;
+  uint32_t k0 = cfg->spp_begin, kn = cfg->spp_count ? cfg->spp_count : c.spp - k0;
+  uint64_t tot[8] = {0};
+  int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+  {
+    uint64_t cnt[8] = {0};
+#pragma omp for schedule(dynamic, 16)
+    for (long long p = 0; p < (long long)W * H; ++p) {
+      uint32_t i = (uint32_t)(p % W), j = (uint32_t)(p / W);
+      double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+      for (uint32_t k = k0; k < k0 + kn; ++k) {
+        v3 Lp = trace_path(&c, i, j, k, counters ? cnt : NULL);
+        if (!v3_isfinite(Lp)) {
+          cnt[3]++;
+          Lp = vs(0.f);
+        }
+        if (out_samples) {
+          float *o = &out_samples[3 * ((size_t)p * kn + (k - k0))];
+          o[0] = Lp.x; o[1] = Lp.y; o[2] = Lp.z;
+        }
+        sum[0] += Lp.x; sum[1] += Lp.y; sum[2] += Lp.z;
+        sq[0] += (double)Lp.x * Lp.x; sq[1] += (double)Lp.y * Lp.y; sq[2] += (double)Lp.z * Lp.z;
+      }
+      for (int ch = 0; ch < 3; ++ch) {
+        double mean = sum[ch] / kn;
+        out_rgb[3 * p + ch] = (float)mean;
+        if (out_var) out_var[3 * p + ch] = kn > 1 ? (float)((sq[ch] - kn * mean * mean) / (kn - 1)) : 0.f;
+      }
+    }
+#pragma omp critical(or_render_cnt)
+    for (int q = 0; q < 8; ++q) tot[q] += cnt[q];
+  }
+  if (counters) memcpy(counters, tot, sizeof(tot));
+  free(c.offx);
+  free(c.offy);
+  return 0;
+}
+
+double or_render_range(const or_scene *s, const pt_render_cfg *cfg, uint64_t first,
+                       uint64_t count, int threads, uint64_t *counters) {
+  rctx c;
+  if (setup(&c, s, cfg)) return -1.0;
+  uint32_t W = cfg->width;
+  double lum = 0.0;
+  uint64_t tot[8] = {0};
+  int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt) reduction(+ : lum)
+  {
+    uint64_t cnt[8] = {0};
+#pragma omp for schedule(dynamic, 64)
+    for (long long q = 0; q < (long long)count; ++q) {
+      uint64_t sidx = first + (uint64_t)q;
+      uint64_t p = sidx / c.spp;
+      uint32_t k = (uint32_t)(sidx % c.spp);
+      uint32_t i = (uint32_t)(p % W), j = (uint32_t)(p / W);
+      v3 Lp = trace_path(&c, i, j, k, counters ? cnt : NULL);
+      if (v3_isfinite(Lp)) lum += (double)(Lp.x + Lp.y + Lp.z);
+    }
+#pragma omp critical(or_range_cnt)
+    for (int z = 0; z < 8; ++z) tot[z] += cnt[z];
+  }
+  if (counters) memcpy(counters, tot, sizeof(tot));
+  free(c.offx);
+  free(c.offy);
+  return lum;
+}

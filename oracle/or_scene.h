@@ -1,0 +1,72 @@
+/*
+ * or_scene.h -- internal scene layout of the CPU oracle.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ */
+#ifndef OR_SCENE_H
+#define OR_SCENE_H
+#include "oracle.h"
+
+typedef struct { v3 p[3]; int mesh; } or_tri;
+typedef struct {
+  v3 bmin, bmax;
+  int left, right;
+  int leaf_first, leaf_count;
+} or_mnode;
+typedef struct {
+  int first_tri, n_tris;
+  uint32_t mat, flags, seq;
+  or_mnode *nodes;
+  int n_nodes, cap_nodes;
+  int *leaf_ids;
+  int n_leaf, cap_leaf;
+} or_mesh;
+typedef struct { v3 c; float r; uint32_t mat, flags, seq; int light; } or_sphere;
+typedef struct {
+  int axis;
+  float k, d0min, d0max, d1min, d1max;
+  uint32_t mat, flags, seq;
+  int light;
+} or_rect;
+enum { OR_LIGHT_QUAD = 1, OR_LIGHT_SPHERE = 2, OR_LIGHT_ENV = 3 };
+typedef struct {
+  int type;
+  v3 v[4], n; /* quad */
+  v3 c;       /* sphere */
+  float r, area;
+  v3 le;      /* texture colour * intensity */
+} or_light;
+
+struct or_scene {
+  pt_material_desc *mats;
+  int n_mats, cap_mats;
+  or_tri *tris;
+  int n_tris, cap_tris;
+  or_mesh *meshes;
+  int n_meshes, cap_meshes;
+  or_sphere *spheres;
+  int n_spheres, cap_spheres;
+  or_rect *rects;
+  int n_rects, cap_rects;
+  or_light *lights;
+  int n_lights;
+  uint32_t seq;
+  float *env;
+  int env_w, env_h;
+  float env_scale;
+  float *env_func, *env_cdf, *env_marg, *env_mcdf;
+  float env_integral;
+  pt_camera_desc cam;
+  int has_cam;
+  v3 cam_h, cam_v, cam_horiz, cam_vert, cam_bl_rel;
+  int committed;
+};
+
+int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, int any,
+                       or_hit *h, uint64_t *cnt);
+int or_prim_light(const or_scene *s, int prim);
+v3 or_env_lookup(const or_scene *s, v3 dir);
+v3 or_uniform_sphere(float u1, float u2);
+v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf);
+float or_env_pdf_dir(const or_scene *s, v3 dir);
+v3 or_light_point(const or_scene *s, const or_light *L, float u, float v, v3 *nrm);
+#endif

@@ -1,0 +1,381 @@
+// pt_device.h -- traversal, primitive tests, lights, environment and camera
+// on the device.
+#pragma once
+#include "pt_scene.h"
+#include "pt_bsdf.h"
+
+namespace pt {
+
+// ---------------------------------------------------------------------
+// Sobol direction numbers (Joe & Kuo 2008: d=1 van der Corput; d=2..4 with
+// (s, a, m) = (1,0,{1}), (2,1,{1,3}), (3,1,{1,3,1})), built at compile time.
+// ---------------------------------------------------------------------
+constexpr SobolTable make_sobol() {
+  SobolTable t{};
+  const int S[4] = {0, 1, 2, 3};
+  const int A[4] = {0, 0, 1, 1};
+  const uint32_t M[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 3, 0}, {1, 3, 1}};
+  for (int k = 0; k < 32; ++k) t.v[0][k] = 1u << (31 - k);
+  for (int d = 1; d < 4; ++d) {
+    int s = S[d];
+    for (int k = 0; k < 32; ++k) {
+      if (k < s) {
+        t.v[d][k] = M[d][k] << (31 - k);
+      } else {
+        uint32_t v = t.v[d][k - s] ^ (t.v[d][k - s] >> s);
+        for (int j = 1; j < s; ++j)
+          if ((A[d] >> (s - 1 - j)) & 1) v ^= t.v[d][k - j];
+        t.v[d][k] = v;
+      }
+    }
+  }
+  return t;
+}
+__constant__ static const SobolTable kSobol = make_sobol();
+
+// One sampler dimension of path sample k of `pixel` (Halton index g).
+PT_D float sample_dim(const DevScene &s, uint32_t sampler, uint32_t seed, uint32_t spp_x, uint32_t spp_y,
+                      uint32_t pixel, uint32_t k, uint32_t g, uint32_t dim) {
+  switch (sampler) {
+    case PT_SAMPLER_SOBOL:
+      return sobol_owen(kSobol, k, dim, pt_pixel_seed(pixel, seed));
+    case PT_SAMPLER_STRATIFIED:
+      return stratified(pt_pixel_seed(pixel, seed), k, spp_x, spp_y, dim);
+    case PT_SAMPLER_HALTON_NONE:
+      return halton_plain(dim, g);
+    case PT_SAMPLER_HALTON_PERM:
+      if (dim >= 25u) return halton_plain(dim, g);  // sampler.hpp:61-63
+      return halton_perm((int)dim, g, s.perm, s.perm_off);
+    default: {
+      uint32_t hash = (uint32_t)mix_bits((uint64_t)(1u + (dim << 4)));
+      return halton_owen((int)(dim % 25u), g, hash);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Primitive tests.  Triangles: MeshBVHNode::hitTriangle (meshBVH.cpp:85-113)
+// on the ray direction as given; spheres and rects normalise the direction
+// (primitive.hpp:132, primitive.cpp:38).  All bounds inclusive.  Compiled
+// without FMA contraction so that every hit decision is the one the
+// reference's unfused FP32 arithmetic makes.
+// ---------------------------------------------------------------------
+#pragma clang fp contract(off)
+PT_D bool hit_tri(float4 A, float4 B, float4 C, f3 o, f3 d, float tmin, float tmax, float &t) {
+  f3 p0 = mk3(A.x, A.y, A.z), e1 = mk3(B.x, B.y, B.z), e2 = mk3(C.x, C.y, C.z);
+  f3 P = cross3(d, e2);
+  float det = dot3(e1, P);
+  if (fabsf(det) < 1e-8f) return false;
+  float inv = 1.0f / det;
+  f3 T = sub3(o, p0);
+  float u = dot3(T, P) * inv;
+  if (u < 0.f || u > 1.f) return false;
+  f3 Q = cross3(T, e1);
+  float v = dot3(d, Q) * inv;
+  if (v < 0.f || u + v > 1.f) return false;
+  float tt = dot3(e2, Q) * inv;
+  if (tt < tmin || tt > tmax) return false;
+  t = tt;
+  return true;
+}
+PT_D bool hit_sphere(float4 A, float4 B, f3 o, f3 d, float tmin, float tmax, float &t) {
+  f3 dn = norm3(d);
+  f3 oc = sub3(o, mk3(A.x, A.y, A.z));
+  float r = B.x;
+  float a = dot3(dn, dn);
+  float b = 2.f * dot3(dn, oc);
+  float c = dot3(oc, oc) - r * r;
+  float disc = b * b - 4.f * a * c;
+  if (disc < 0.f) return false;
+  float delta = sqrtf(disc);
+  float tt = (-b - delta) / (2.f * a);
+  if (tt < tmin || tt > tmax) {
+    tt = (-b + delta) / (2.f * a);
+    if (tt < tmin || tt > tmax) return false;
+  }
+  t = tt;
+  return true;
+}
+PT_D bool hit_rect(float4 A, float4 B, f3 o, f3 d, float tmin, float tmax, float &t) {
+  f3 dn = norm3(d);
+  int ax = (int)B.z;
+  int b0 = (ax == 0) ? 1 : 0, b1 = (ax == 2) ? 1 : 2;
+  float tt = (A.x - comp3(o, ax)) / comp3(dn, ax);
+  if (tt < tmin || tt > tmax) return false;
+  float d0 = comp3(o, b0) + tt * comp3(dn, b0);
+  float d1 = comp3(o, b1) + tt * comp3(dn, b1);
+  if (d0 < A.y || d0 > A.z || d1 < B.x || d1 > B.y) return false;
+  t = tt;
+  return true;
+}
+
+PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, float tmax, float &t, uint32_t &id) {
+  float4 A = prims[3 * p];
+  float4 B = prims[3 * p + 1];
+  uint32_t tag = __float_as_uint(A.w);
+  id = tag & 0x3fffffffu;
+  uint32_t type = tag >> 30;
+  if (type == PRIM_TRI) {
+    float4 C = prims[3 * p + 2];
+    return hit_tri(A, B, C, o, d, tmin, tmax, t);
+  }
+  if (type == PRIM_SPHERE) return hit_sphere(A, B, o, d, tmin, tmax, t);
+  return hit_rect(A, B, o, d, tmin, tmax, t);
+}
+#pragma clang fp contract(fast)
+
+// ---------------------------------------------------------------------
+// BVH2 traversal, ordered (near child first), stack in registers/scratch.
+// ANY: stop at the first primitive hit in [tmin, tmax] (shadow rays).
+// ---------------------------------------------------------------------
+#define PT_STACK 64
+template <bool ANY, bool COUNT>
+PT_D bool traverse(const DevScene &s, f3 o, f3 d, float tmin, float tmax, float &t_out, int &prim_out,
+                   uint32_t &n_nodes, uint32_t &n_prims) {
+  const float ix = 1.f / d.x, iy = 1.f / d.y, iz = 1.f / d.z;
+  const float oix = o.x * ix, oiy = o.y * iy, oiz = o.z * iz;
+  float tb = tmax;
+  int best = -1;
+  int stack[PT_STACK];
+  int sp = 0;
+  int cur = s.root;
+  while (true) {
+    if (cur >= 0) {
+      const float4 *nd = s.nodes + 4 * cur;
+      float4 a = nd[0], b = nd[1], c = nd[2];
+      int4 ch = *reinterpret_cast<const int4 *>(nd + 3);
+      if (COUNT) ++n_nodes;
+      float l0x = fmaf(a.x, ix, -oix), h0x = fmaf(a.y, ix, -oix);
+      float l0y = fmaf(a.z, iy, -oiy), h0y = fmaf(a.w, iy, -oiy);
+      float l0z = fmaf(c.x, iz, -oiz), h0z = fmaf(c.y, iz, -oiz);
+      float l1x = fmaf(b.x, ix, -oix), h1x = fmaf(b.y, ix, -oix);
+      float l1y = fmaf(b.z, iy, -oiy), h1y = fmaf(b.w, iy, -oiy);
+      float l1z = fmaf(c.z, iz, -oiz), h1z = fmaf(c.w, iz, -oiz);
+      float n0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), tmin));
+      float f0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tb));
+      float n1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), tmin));
+      float f1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tb));
+      bool hit0 = n0 <= f0, hit1 = n1 <= f1;
+      if (hit0 && hit1) {
+        int nearc = ch.x, farc = ch.y;
+        if (n1 < n0) { nearc = ch.y; farc = ch.x; }
+        stack[sp++] = farc;
+        cur = nearc;
+        continue;
+      }
+      if (hit0) { cur = ch.x; continue; }
+      if (hit1) { cur = ch.y; continue; }
+    } else {
+      uint32_t st = leaf_start(cur), cn = leaf_count(cur);
+      for (uint32_t p = st; p < st + cn; ++p) {
+        float t;
+        uint32_t id;
+        if (COUNT) ++n_prims;
+        if (hit_prim(s.prims, p, o, d, tmin, tb, t, id)) {
+          tb = t;
+          best = (int)id;
+          if (ANY) {
+            t_out = t;
+            prim_out = best;
+            return true;
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    cur = stack[--sp];
+  }
+  t_out = tb;
+  prim_out = best;
+  return best >= 0;
+}
+
+// ---------------------------------------------------------------------
+// Shading geometry.  HitRecord::set_normal (materialBase.hpp:48-52) and
+// the wrapper flags (FlipFace toggles is_inside; Translate/Rotate/Scale
+// force it true).
+// ---------------------------------------------------------------------
+struct SurfHit {
+  f3 p;
+  f3 ng;  // geometric normal before flipping (light-facing for rects)
+  ShadePoint sp;
+  uint32_t mat;
+  int light;
+};
+PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
+  SurfHit h;
+  uint2 info = s.prim_info[prim];
+  float4 g = s.prim_geom[prim];
+  uint32_t type = (info.x >> 24) & 3u, fl = info.x >> 28;
+  h.mat = info.x & 0xffffffu;
+  h.light = (int)info.y;
+  f3 n;
+  if (type == PRIM_TRI) {
+    h.p = add3(o, scl3(d, t));
+    n = mk3(g.x, g.y, g.z);
+  } else if (type == PRIM_SPHERE) {
+    h.p = add3(o, scl3(norm3(d), t));
+    n = norm3(sub3(h.p, mk3(g.x, g.y, g.z)));
+  } else {
+    h.p = add3(o, scl3(norm3(d), t));
+    int ax = (int)g.x;
+    n = ax == 0 ? mk3(1.f, 0.f, 0.f) : (ax == 1 ? mk3(0.f, 1.f, 0.f) : mk3(0.f, 0.f, 1.f));
+  }
+  h.ng = n;
+  bool inside = dot3(d, n) < 0.f;
+  h.sp.n = inside ? n : neg3(n);
+  if (fl & PT_PRIM_FLIP_FACE) inside = !inside;
+  if (fl & PT_PRIM_FORCE_INSIDE) inside = true;
+  h.sp.inside = inside;
+  return h;
+}
+
+// ---------------------------------------------------------------------
+// Environment.  HDRITexture::getTexelColor (texture.hpp:105-115) and the
+// bilinear ImageTexture lookup with index clamp to size-2 taken after the
+// fractions (texture.cpp:111-142).
+// ---------------------------------------------------------------------
+PT_D f3 env_texel(const DevScene &s, uint32_t x, uint32_t y) {
+  const float *c = s.env + 3 * ((size_t)y * s.env_w + x);
+  return mk3(c[0], c[1], c[2]);
+}
+PT_D f3 env_lookup(const DevScene &s, f3 dir) {
+  f3 dn = norm3(dir);
+  float theta = acosf(-dn.y);
+  float phi = atan2f(-dn.z, dn.x) + PT_PI;
+  float u = phi / (2.f * PT_PI), v = theta / PT_PI;
+  float fx = u * (float)s.env_w, fy = v * (float)s.env_h;
+  uint32_t x = (uint32_t)fx, y = (uint32_t)fy;
+  float x1 = fx - (float)x, y1 = fy - (float)y;
+  x = min(x, (uint32_t)(s.env_w - 2));
+  y = min(y, (uint32_t)(s.env_h - 2));
+  float x2 = 1.f - x1, y2 = 1.f - y1;
+  f3 c1 = env_texel(s, x, y), c2 = env_texel(s, x + 1, y);
+  f3 c3 = env_texel(s, x, y + 1), c4 = env_texel(s, x + 1, y + 1);
+  f3 c12 = add3(scl3(c1, x2), scl3(c2, x1));
+  f3 c34 = add3(scl3(c3, x2), scl3(c4, x1));
+  return scl3(add3(scl3(c12, y2), scl3(c34, y1)), s.env_scale);
+}
+// uniformSampleSphere (utils/random.hpp:7-15)
+PT_D f3 uniform_sphere(float u1, float u2) {
+  float z = 1.f - 2.f * u1;
+  float r = sqrtf(fmaxf(0.f, 1.f - z * z));
+  float ph = 2.f * PT_PI * u2;
+  return mk3(r * cosf(ph), r * sinf(ph), z);
+}
+PT_D uint32_t find_interval(const float *cdf, uint32_t n, float u) {
+  uint32_t lo = 0, hi = n;
+  while (lo + 1 < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (cdf[mid] <= u) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+// 2-D piecewise-constant environment sampling (the reference samples the
+// environment uniformly, texture.hpp:120-131; this is the MIS estimator's).
+PT_D f3 env_sample(const DevScene &s, float u0, float u1, float &pdf) {
+  uint32_t W = s.env_w, H = s.env_h;
+  if (!(s.env_integral > 0.f)) {
+    pdf = 1.f / (4.f * PT_PI);
+    return uniform_sphere(u0, u1);
+  }
+  uint32_t y = find_interval(s.env_mcdf, H, u1);
+  float dy = u1 - s.env_mcdf[y], wy = s.env_mcdf[y + 1] - s.env_mcdf[y];
+  if (wy > 0.f) dy /= wy;
+  const float *cdf = s.env_cdf + (size_t)y * (W + 1);
+  uint32_t x = find_interval(cdf, W, u0);
+  float dx = u0 - cdf[x], wx = cdf[x + 1] - cdf[x];
+  if (wx > 0.f) dx /= wx;
+  float u = ((float)x + dx) / (float)W, v = ((float)y + dy) / (float)H;
+  float th = v * PT_PI, ph = u * 2.f * PT_PI;
+  float st = sinf(th);
+  float f = s.env_func[(size_t)y * W + x];
+  pdf = (st > 0.f) ? f / s.env_integral / (2.f * PT_PI * PT_PI * st) : 0.f;
+  return mk3(-st * cosf(ph), -cosf(th), st * sinf(ph));
+}
+PT_D float env_pdf(const DevScene &s, f3 dir) {
+  if (!(s.env_integral > 0.f)) return 1.f / (4.f * PT_PI);
+  f3 dn = norm3(dir);
+  float th = acosf(clampf_(-dn.y, -1.f, 1.f));
+  float ph = atan2f(-dn.z, dn.x) + PT_PI;
+  float u = ph / (2.f * PT_PI), v = th / PT_PI;
+  int x = min(max((int)(u * s.env_w), 0), s.env_w - 1);
+  int y = min(max((int)(v * s.env_h), 0), s.env_h - 1);
+  float st = sinf(th);
+  if (!(st > 0.f)) return 0.f;
+  return s.env_func[(size_t)y * s.env_w + x] / s.env_integral / (2.f * PT_PI * PT_PI * st);
+}
+
+// Light::get_sample + get_normal_at (light.hpp:131-147, 254-271;
+// envLight.hpp:20-37,51-54)
+PT_D f3 light_point(const DevLight &L, float u, float v, f3 &nrm) {
+  if (L.type == LIGHT_QUAD) {
+    nrm = L.n;
+    return add3(add3(L.v0, scl3(L.e1, u)), scl3(L.e2, v));
+  }
+  if (L.type == LIGHT_SPHERE) {
+    float th = 2.f * PT_PI * u;
+    float ph = acosf(2.f * v - 1.f);
+    f3 dir = mk3(sinf(ph) * cosf(th), sinf(ph) * sinf(th), cosf(ph));
+    f3 p = add3(L.c, scl3(dir, L.r));
+    nrm = norm3(sub3(p, L.c));
+    return p;
+  }
+  f3 p = scl3(uniform_sphere(u, v), 1e5f);
+  nrm = neg3(norm3(p));
+  return p;
+}
+// Solid-angle pdf of the MIS estimator's light sampling for point p of
+// light L seen from o (area lights: uniform area sampling).
+PT_D float light_pdf_sa(const DevScene &s, const DevLight &L, f3 o, f3 p) {
+  if (L.type == LIGHT_ENV) return env_pdf(s, sub3(p, o));
+  f3 w = sub3(p, o);
+  float d2 = dot3(w, w);
+  f3 nl = (L.type == LIGHT_QUAD) ? L.n : norm3(sub3(p, L.c));
+  float c = fabsf(dot3(nl, norm3(w)));
+  if (!(c > 0.f) || !(L.area > 0.f)) return 0.f;
+  return d2 / (L.area * c);
+}
+
+// ---------------------------------------------------------------------
+// PerspectiveCamera::generateRay (camera.cpp:42-56): lens point uniform on
+// the square [-r, r]^2 (r = focalDist/aperture/2), direction towards the
+// image-plane point; a shutter time moves the camera (motion blur).
+// ---------------------------------------------------------------------
+PT_D void camera_ray(const DevCamera &c, float u, float v, float r0, float r1, float rt, f3 &o, f3 &d) {
+  float lu = -c.radius + (c.radius - -c.radius) * r0;
+  float lv = -c.radius + (c.radius - -c.radius) * r1;
+  f3 pos = c.pos;
+  if (c.shutter > 0.f) pos = add3(pos, scl3(sub3(c.pos1, pos), rt * c.shutter));
+  f3 bl = sub3(sub3(pos, divs3(c.horiz, 2.f)), divs3(c.vert, 2.f));
+  bl = sub3(bl, scl3(c.lookat, c.focal_dist));
+  f3 ps = add3(pos, add3(scl3(c.h, lu), scl3(c.v, lv)));
+  f3 dir = sub3(add3(add3(bl, scl3(c.horiz, u)), scl3(c.vert, v)), ps);
+  if (c.normalize_dir) dir = norm3(dir);
+  o = ps;
+  d = dir;
+}
+
+// Wave-aggregated queue append: one atomic per wave.  Must be reached by
+// every lane of the wave.
+PT_D uint32_t wave_append(uint32_t *counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  uint32_t lane = __lane_id();
+  uint32_t base = 0;
+  if (mask) {
+    uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+    uint32_t total = (uint32_t)__popcll(mask);
+    if (lane == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, (int)leader);
+  }
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+PT_D void wave_add_u64(unsigned long long *ctr, uint32_t v) {
+  // wave reduction then one atomic
+  unsigned long long x = v;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  if (__lane_id() == 0 && x) atomicAdd(ctr, x);
+}
+
+}  // namespace pt

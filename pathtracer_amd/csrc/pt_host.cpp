@@ -1,0 +1,1148 @@
+// pt_host.cpp -- host side of the MI355X path tracer: scene assembly, BVH
+// build, light discovery, device upload, wavefront orchestration and the C
+// ABI of include/pathtracer_amd.h.
+//
+// Reference interfaces replaced (file:line):
+//   SceneInfo::render                 render/renderManager.hpp:51-158
+//   ObjectList / BVHNode              base/objectList.hpp:9-150, primitives/bvh.hpp:8-102
+//   TriangleMesh / MeshBVHNode        primitives/triangleMesh.cpp:5-56, meshBVH.cpp:6-143
+//   discover_emissive_objects_as_lights  base/lightDiscovery.hpp:5-81
+//   FrameBuffer offsets               base/framebuffer.cpp:131-145
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pathtracer_amd.h"
+#include "pt_launch.h"
+
+using namespace pt;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+static pt_status fail(pt_status st, const std::string &msg) {
+  g_err = msg;
+  return st;
+}
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      return fail(PT_ERR_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(_e));      \
+  } while (0)
+
+// ------------------------------------------------------------------ device buffers
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;  // owns a device allocation: never copied
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    if (n == 0) n = 256;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+static hipError_t upload(DevBuf &b, const void *src, size_t n) {
+  hipError_t e = b.alloc(n);
+  if (e != hipSuccess) return e;
+  if (n) return hipMemcpy(b.p, src, n, hipMemcpyHostToDevice);
+  return hipSuccess;
+}
+
+// ------------------------------------------------------------------ host-side scene records
+struct HTri { f3 p[3]; uint32_t mesh; };
+struct HMesh { uint32_t first, count, mat, flags, seq; };
+struct HSphere { f3 c; float r; uint32_t mat, flags, seq; int light; };
+struct HRect { int axis; float k, d0min, d0max, d1min, d1max; uint32_t mat, flags, seq; int light; };
+
+struct pt_scene {
+  std::vector<pt_material_desc> mats;
+  std::vector<HTri> tris;
+  std::vector<HMesh> meshes;
+  std::vector<HSphere> spheres;
+  std::vector<HRect> rects;
+  uint32_t seq = 0;
+  std::vector<float> env;
+  uint32_t env_w = 0, env_h = 0;
+  float env_scale = 1.f;
+  pt_camera_desc cam{};
+  bool has_cam = false;
+  bool committed = false;
+
+  // device
+  hipStream_t stream = nullptr;
+  int device = -1;
+  DevBuf d_nodes, d_prims, d_info, d_geom, d_mats, d_lights, d_env, d_env_func, d_env_cdf, d_env_mcdf;
+  DevBuf d_perm, d_perm_off;
+  uint32_t perm_seed = 0xffffffffu;
+  bool perm_valid = false;
+  DevScene ds{};
+  std::vector<DevLight> lights;
+  pt_scene_info info{};
+  // wavefront
+  DevBuf w_buf, w_counts, w_stats, d_off;
+  Wave wave{};
+  uint32_t wave_cap = 0;
+  DevBuf film, samples;
+  uint32_t film_w = 0, film_h = 0;
+  uint64_t film_spp = 0;
+  size_t samples_n = 0;
+  uint32_t n_cu = 256;
+};
+
+static bool g_has_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return false;
+  return n > 0;
+}
+
+static pt_status ensure_stream(pt_scene *s) {
+  if (s->stream) return PT_OK;
+  if (!g_has_device()) return fail(PT_ERR_NO_DEVICE, "no HIP device visible");
+  HIP_TRY(hipGetDevice(&s->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, s->device));
+  s->n_cu = (uint32_t)std::max(1, prop.multiProcessorCount);
+  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ C ABI: scene building
+extern "C" const char *pt_last_error(void) { return g_err.c_str(); }
+extern "C" const char *pt_version(void) { return "pathtracer_amd 0.1 (gfx950 wavefront)"; }
+
+extern "C" pt_status pt_scene_create(pt_scene **out) {
+  if (!out) return fail(PT_ERR_INVALID, "pt_scene_create: out is NULL");
+  try {
+    *out = new pt_scene();
+  } catch (...) {
+    return fail(PT_ERR_NOMEM, "pt_scene_create: out of host memory");
+  }
+  return PT_OK;
+}
+
+extern "C" void pt_scene_destroy(pt_scene *s) {
+  if (!s) return;
+  if (s->stream) {
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamDestroy(s->stream);
+  }
+  delete s;
+}
+
+static bool valid_kind(uint32_t k) { return k >= PT_MAT_LAMBERTIAN && k <= PT_MAT_KAJIYA_KAY; }
+
+extern "C" pt_status pt_scene_add_material(pt_scene *s, const pt_material_desc *d, uint32_t *out_id) {
+  if (!s || !d) return fail(PT_ERR_INVALID, "pt_scene_add_material: NULL argument");
+  if (!valid_kind(d->kind)) return fail(PT_ERR_INVALID, "pt_scene_add_material: unknown material kind");
+  if (s->mats.size() >= (1u << 24)) return fail(PT_ERR_INVALID, "too many materials");
+  s->mats.push_back(*d);
+  if (out_id) *out_id = (uint32_t)s->mats.size() - 1;
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_add_mesh(pt_scene *s, const float *pos, uint32_t nv, const int32_t *idx, uint32_t nt,
+                                       uint32_t mat, uint32_t flags) {
+  if (!s || (!pos && nv) || (!idx && nt)) return fail(PT_ERR_INVALID, "pt_scene_add_mesh: NULL argument");
+  if (mat >= s->mats.size()) return fail(PT_ERR_INVALID, "pt_scene_add_mesh: material id out of range");
+  HMesh m{(uint32_t)s->tris.size(), nt, mat, flags, s->seq++};
+  s->tris.reserve(s->tris.size() + nt);
+  for (uint32_t t = 0; t < nt; ++t) {
+    HTri tr;
+    for (int c = 0; c < 3; ++c) {
+      int32_t v = idx[3 * t + c];
+      if (v < 0 || (uint32_t)v >= nv) {
+        s->tris.resize(m.first);
+        return fail(PT_ERR_INVALID, "pt_scene_add_mesh: vertex index out of range");
+      }
+      tr.p[c] = mk3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
+    }
+    tr.mesh = (uint32_t)s->meshes.size();
+    s->tris.push_back(tr);
+  }
+  s->meshes.push_back(m);
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_add_sphere(pt_scene *s, const float c[3], float r, uint32_t mat, uint32_t flags) {
+  if (!s || !c) return fail(PT_ERR_INVALID, "pt_scene_add_sphere: NULL argument");
+  if (mat >= s->mats.size()) return fail(PT_ERR_INVALID, "pt_scene_add_sphere: material id out of range");
+  if (!(r > 0.f) || !std::isfinite(r)) return fail(PT_ERR_INVALID, "pt_scene_add_sphere: radius must be > 0");
+  s->spheres.push_back(HSphere{mk3(c[0], c[1], c[2]), r, mat, flags, s->seq++, -1});
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_add_rect(pt_scene *s, int axis, float k, float d0min, float d0max, float d1min,
+                                       float d1max, uint32_t mat, uint32_t flags) {
+  if (!s) return fail(PT_ERR_INVALID, "pt_scene_add_rect: NULL scene");
+  if (axis < 0 || axis > 2) return fail(PT_ERR_INVALID, "pt_scene_add_rect: axis must be 0, 1 or 2");
+  if (mat >= s->mats.size()) return fail(PT_ERR_INVALID, "pt_scene_add_rect: material id out of range");
+  s->rects.push_back(HRect{axis, k, d0min, d0max, d1min, d1max, mat, flags, s->seq++, -1});
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_set_env(pt_scene *s, const float *rgb, uint32_t w, uint32_t h, float scale) {
+  if (!s || !rgb) return fail(PT_ERR_INVALID, "pt_scene_set_env: NULL argument");
+  if (w < 2 || h < 2) return fail(PT_ERR_INVALID, "pt_scene_set_env: texture must be at least 2x2");
+  s->env.assign(rgb, rgb + 3 * (size_t)w * h);
+  s->env_w = w;
+  s->env_h = h;
+  s->env_scale = scale;
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_set_camera(pt_scene *s, const pt_camera_desc *c) {
+  if (!s || !c) return fail(PT_ERR_INVALID, "pt_scene_set_camera: NULL argument");
+  if (!(c->fov > 0.f) || !(c->aspect > 0.f) || !(c->focal_dist > 0.f))
+    return fail(PT_ERR_INVALID, "pt_scene_set_camera: fov, aspect and focal_dist must be > 0");
+  s->cam = *c;
+  s->has_cam = true;
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ BVH build (binned SAH)
+namespace {
+
+struct BRef {
+  float lo[3], hi[3];
+  uint32_t prim;  // index into the unified primitive list
+};
+
+struct Builder {
+  std::vector<BRef> refs;
+  std::vector<float4> nodes;  // 4 per node
+  std::vector<uint32_t> order;
+  uint32_t leaf_size = 4, bins = 16;
+  std::atomic<uint32_t> max_depth{0};
+  std::atomic<uint32_t> node_count{0};
+
+  static void bounds(const BRef *r, size_t n, float lo[3], float hi[3]) {
+    for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+    for (size_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], r[i].lo[a]);
+        hi[a] = std::max(hi[a], r[i].hi[a]);
+      }
+  }
+  static float area(const float lo[3], const float hi[3]) {
+    float dx = std::max(0.f, hi[0] - lo[0]), dy = std::max(0.f, hi[1] - lo[1]), dz = std::max(0.f, hi[2] - lo[2]);
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+  }
+  static constexpr size_t kLeaf = ~(size_t)0;
+  // Split [b, e): returns the split position (b < m < e), or kLeaf.
+  size_t split(size_t b, size_t e, uint32_t depth) {
+    size_t n = e - b;
+    if (n <= 1) return kLeaf;
+    BRef *r = refs.data();
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = b; i < e; ++i)
+      for (int a = 0; a < 3; ++a) {
+        float c = 0.5f * (r[i].lo[a] + r[i].hi[a]);
+        clo[a] = std::min(clo[a], c);
+        chi[a] = std::max(chi[a], c);
+      }
+    if (depth > 40) {  // bound the depth: median split on the widest centroid axis
+      if (n <= leaf_size) return kLeaf;
+      int ax = 0;
+      for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+      size_t m = b + n / 2;
+      std::nth_element(r + b, r + m, r + e, [ax](const BRef &x, const BRef &y) {
+        return x.lo[ax] + x.hi[ax] < y.lo[ax] + y.hi[ax];
+      });
+      return m;
+    }
+    float plo[3], phi[3];
+    bounds(r + b, n, plo, phi);
+    float parea = area(plo, phi);
+    const uint32_t NB = bins;
+    float best_cost = INFINITY;
+    int best_axis = -1;
+    uint32_t best_bin = 0;
+    for (int a = 0; a < 3; ++a) {
+      float ext = chi[a] - clo[a];
+      if (!(ext > 0.f)) continue;
+      float scale = (float)NB / ext;
+      uint32_t cnt[64] = {0};
+      float blo[64][3], bhi[64][3];
+      for (uint32_t k = 0; k < NB; ++k)
+        for (int c = 0; c < 3; ++c) { blo[k][c] = INFINITY; bhi[k][c] = -INFINITY; }
+      for (size_t i = b; i < e; ++i) {
+        float c = 0.5f * (r[i].lo[a] + r[i].hi[a]);
+        uint32_t k = std::min(NB - 1, (uint32_t)((c - clo[a]) * scale));
+        ++cnt[k];
+        for (int q = 0; q < 3; ++q) {
+          blo[k][q] = std::min(blo[k][q], r[i].lo[q]);
+          bhi[k][q] = std::max(bhi[k][q], r[i].hi[q]);
+        }
+      }
+      float rarea[64];
+      uint32_t rcnt[64];
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      uint32_t acc = 0;
+      for (int k = (int)NB - 1; k > 0; --k) {
+        for (int q = 0; q < 3; ++q) { lo[q] = std::min(lo[q], blo[k][q]); hi[q] = std::max(hi[q], bhi[k][q]); }
+        acc += cnt[k];
+        rcnt[k] = acc;
+        rarea[k] = area(lo, hi);
+      }
+      for (int q = 0; q < 3; ++q) { lo[q] = INFINITY; hi[q] = -INFINITY; }
+      acc = 0;
+      for (uint32_t k = 0; k + 1 < NB; ++k) {
+        for (int q = 0; q < 3; ++q) { lo[q] = std::min(lo[q], blo[k][q]); hi[q] = std::max(hi[q], bhi[k][q]); }
+        acc += cnt[k];
+        if (acc == 0 || rcnt[k + 1] == 0) continue;
+        float cost = area(lo, hi) * (float)acc + rarea[k + 1] * (float)rcnt[k + 1];
+        if (cost < best_cost) { best_cost = cost; best_axis = a; best_bin = k; }
+      }
+    }
+    // SAH: traversal cost 1, intersection cost 1 (relative)
+    float leaf_cost = (float)n;
+    float split_cost = (parea > 0.f) ? 1.f + best_cost / parea : INFINITY;
+    if (n <= leaf_size && !(split_cost < leaf_cost)) return kLeaf;
+    if (best_axis < 0) {
+      if (n <= 16) return kLeaf;
+      return b + n / 2;  // all centroids coincide: split by index
+    }
+    float ext = chi[best_axis] - clo[best_axis];
+    float scale = (float)NB / ext;
+    int ax = best_axis;
+    float c0 = clo[ax];
+    BRef *mid = std::partition(r + b, r + e, [&](const BRef &x) {
+      float c = 0.5f * (x.lo[ax] + x.hi[ax]);
+      return std::min(NB - 1, (uint32_t)((c - c0) * scale)) <= best_bin;
+    });
+    size_t m = (size_t)(mid - r);
+    if (m == b || m == e) return b + n / 2;
+    return m;
+  }
+
+  static void pad(float lo[3], float hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+      float p = 1e-6f * (std::fabs(lo[a]) + std::fabs(hi[a])) + 1e-12f;
+      lo[a] -= p;
+      hi[a] += p;
+    }
+  }
+
+  // Writes node `ni` for range [b, e) already known to be split at m.
+  void write_node(uint32_t ni, size_t b, size_t m, size_t e, int c0, int c1) {
+    float l0[3], h0[3], l1[3], h1[3];
+    bounds(refs.data() + b, m - b, l0, h0);
+    bounds(refs.data() + m, e - m, l1, h1);
+    pad(l0, h0);
+    pad(l1, h1);
+    float4 *nd = &nodes[4 * (size_t)ni];
+    nd[0] = make_float4(l0[0], h0[0], l0[1], h0[1]);
+    nd[1] = make_float4(l1[0], h1[0], l1[1], h1[1]);
+    nd[2] = make_float4(l0[2], h0[2], l1[2], h1[2]);
+    int4 ch = make_int4(c0, c1, 0, 0);
+    std::memcpy(&nd[3], &ch, sizeof(int4));
+  }
+
+  int build(size_t b, size_t e, uint32_t depth, bool parallel) {
+    uint32_t md = max_depth.load(std::memory_order_relaxed);
+    while (depth > md && !max_depth.compare_exchange_weak(md, depth)) {
+    }
+    size_t m = split(b, e, depth);
+    if (m == kLeaf) return leaf_code((uint32_t)b, (uint32_t)(e - b));
+    uint32_t ni = node_count.fetch_add(1);
+    int c0, c1;
+    if (parallel && (e - b) > 200000) {
+      int r0 = 0;
+      std::thread th([&] { r0 = build(b, m, depth + 1, depth < 3); });
+      c1 = build(m, e, depth + 1, depth < 3);
+      th.join();
+      c0 = r0;
+    } else {
+      c0 = build(b, m, depth + 1, false);
+      c1 = build(m, e, depth + 1, false);
+    }
+    write_node(ni, b, m, e, c0, c1);
+    return (int)ni;
+  }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ murmur / digit permutations
+// gl::pbrt::murmurHash64A (utils/hash.hpp:11-56), for DigitPermutation tables.
+static uint64_t murmur64a(const unsigned char *key, size_t len, uint64_t seed) {
+  const uint64_t m = 0xc6a4a7935bd1e995ull;
+  const int r = 47;
+  uint64_t h = seed ^ (len * m);
+  const unsigned char *end = key + 8 * (len / 8);
+  while (key != end) {
+    uint64_t k;
+    std::memcpy(&k, key, 8);
+    key += 8;
+    k *= m;
+    k ^= k >> r;
+    k *= m;
+    h ^= k;
+    h *= m;
+  }
+  switch (len & 7) {
+    case 7: h ^= uint64_t(key[6]) << 48; [[fallthrough]];
+    case 6: h ^= uint64_t(key[5]) << 40; [[fallthrough]];
+    case 5: h ^= uint64_t(key[4]) << 32; [[fallthrough]];
+    case 4: h ^= uint64_t(key[3]) << 24; [[fallthrough]];
+    case 3: h ^= uint64_t(key[2]) << 16; [[fallthrough]];
+    case 2: h ^= uint64_t(key[1]) << 8; [[fallthrough]];
+    case 1: h ^= uint64_t(key[0]); h *= m;
+  }
+  h ^= h >> r;
+  h *= m;
+  h ^= h >> r;
+  return h;
+}
+
+static const int kHostPrimes[25] = {2,  3,  5,  7,  11, 13, 17, 19, 23, 29, 31, 37, 41,
+                                    43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97};
+
+// DigitPermutation (sampler/digitPerm.hpp:12-56) for all 25 bases.
+static pt_status ensure_perm(pt_scene *s, uint32_t seed) {
+  if (s->perm_valid && s->perm_seed == seed) return PT_OK;
+  std::vector<uint16_t> tab;
+  std::vector<uint32_t> off(25);
+  for (int bi = 0; bi < 25; ++bi) {
+    int base = kHostPrimes[bi];
+    off[bi] = (uint32_t)tab.size();
+    int nd = 0;
+    double inv = 1.0 / (double)base, ibm = 1.0;
+    while (1.0 - (base - 1) * ibm < 1.0) { ++nd; ibm *= inv; }
+    for (int di = 0; di < nd; ++di) {
+      unsigned char buf[12];
+      int32_t b32 = base, d32 = di;
+      std::memcpy(buf, &b32, 4);
+      std::memcpy(buf + 4, &d32, 4);
+      std::memcpy(buf + 8, &seed, 4);
+      uint64_t dseed = murmur64a(buf, 12, 0);
+      for (int dv = 0; dv < base; ++dv)
+        tab.push_back((uint16_t)permutation_element((uint32_t)dv, (uint32_t)base, (uint32_t)dseed));
+    }
+  }
+  HIP_TRY(upload(s->d_perm, tab.data(), tab.size() * sizeof(uint16_t)));
+  HIP_TRY(upload(s->d_perm_off, off.data(), off.size() * sizeof(uint32_t)));
+  s->perm_seed = seed;
+  s->perm_valid = true;
+  s->ds.perm = s->d_perm.as<uint16_t>();
+  s->ds.perm_off = s->d_perm_off.as<uint32_t>();
+  s->ds.perm_seed = seed;
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ commit
+static float texel_max(const pt_scene *s, uint32_t x, uint32_t y) {
+  const float *c = &s->env[3 * ((size_t)y * s->env_w + x)];
+  return std::max(c[0], std::max(c[1], c[2]));
+}
+
+extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
+  if (!s) return fail(PT_ERR_INVALID, "pt_scene_commit: NULL scene");
+  size_t nprim = s->tris.size() + s->spheres.size() + s->rects.size();
+  if (nprim == 0) return fail(PT_ERR_INVALID, "pt_scene_commit: the scene has no objects");  // renderManager.hpp:64-68
+  if (nprim >= (1u << 27)) return fail(PT_ERR_INVALID, "pt_scene_commit: too many primitives");
+  pt_status st = ensure_stream(s);
+  if (st != PT_OK) return st;
+  auto t0 = std::chrono::steady_clock::now();
+
+  const uint32_t ntri = (uint32_t)s->tris.size(), nsph = (uint32_t)s->spheres.size();
+  // -- lights (insertion order; environment last)
+  s->lights.clear();
+  for (uint32_t q = 0; q < s->seq; ++q) {
+    for (auto &r : s->rects) {
+      if (r.seq != q) continue;
+      r.light = -1;
+      const pt_material_desc &md = s->mats[r.mat];
+      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f)) continue;
+      f3 v[4];
+      float k = r.k;
+      if (r.axis == 0) {
+        v[0] = mk3(k, r.d0min, r.d1min); v[1] = mk3(k, r.d0max, r.d1min); v[3] = mk3(k, r.d0min, r.d1max);
+      } else if (r.axis == 1) {
+        v[0] = mk3(r.d0min, k, r.d1min); v[1] = mk3(r.d0max, k, r.d1min); v[3] = mk3(r.d0min, k, r.d1max);
+      } else {
+        v[0] = mk3(r.d0min, r.d1min, k); v[1] = mk3(r.d0max, r.d1min, k); v[3] = mk3(r.d0min, r.d1max, k);
+      }
+      DevLight L{};
+      L.type = LIGHT_QUAD;
+      L.v0 = v[0];
+      L.e1 = sub3(v[1], v[0]);
+      L.e2 = sub3(v[3], v[0]);
+      L.n = norm3(cross3(L.e1, L.e2));
+      L.area = len3(cross3(L.e1, L.e2));
+      L.le = scl3(mk3(md.p[0], md.p[1], md.p[2]), md.p[3]);
+      r.light = (int)s->lights.size();
+      s->lights.push_back(L);
+    }
+    for (auto &sp : s->spheres) {
+      if (sp.seq != q) continue;
+      sp.light = -1;
+      const pt_material_desc &md = s->mats[sp.mat];
+      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f)) continue;
+      DevLight L{};
+      L.type = LIGHT_SPHERE;
+      L.c = sp.c;
+      L.r = sp.r;
+      L.area = (float)(4 * M_PI * sp.r * sp.r);
+      L.le = scl3(mk3(md.p[0], md.p[1], md.p[2]), md.p[3]);
+      sp.light = (int)s->lights.size();
+      s->lights.push_back(L);
+    }
+  }
+  if (!s->env.empty()) {
+    DevLight L{};
+    L.type = LIGHT_ENV;
+    s->lights.push_back(L);
+  }
+
+  // -- primitive references and per-primitive info
+  Builder B;
+  B.leaf_size = (cfg && cfg->leaf_size) ? std::min<uint32_t>(cfg->leaf_size, 16u) : 4u;
+  B.bins = (cfg && cfg->sah_bins) ? std::min<uint32_t>(std::max<uint32_t>(cfg->sah_bins, 2u), 64u) : 16u;
+  B.refs.resize(nprim);
+  std::vector<uint2> info(nprim);
+  std::vector<float4> geom(nprim);
+  for (uint32_t i = 0; i < ntri; ++i) {
+    const HTri &t = s->tris[i];
+    BRef &r = B.refs[i];
+    for (int a = 0; a < 3; ++a) {
+      r.lo[a] = std::min(comp3(t.p[0], a), std::min(comp3(t.p[1], a), comp3(t.p[2], a)));
+      r.hi[a] = std::max(comp3(t.p[0], a), std::max(comp3(t.p[1], a), comp3(t.p[2], a)));
+    }
+    r.prim = i;
+    const HMesh &m = s->meshes[t.mesh];
+    info[i] = make_uint2(m.mat | (PRIM_TRI << 24) | ((m.flags & 3u) << 28), 0xffffffffu);
+    f3 e1 = sub3(t.p[1], t.p[0]), e2 = sub3(t.p[2], t.p[0]);
+    f3 ng = norm3(cross3(e1, e2));
+    geom[i] = make_float4(ng.x, ng.y, ng.z, 0.f);
+  }
+  for (uint32_t i = 0; i < nsph; ++i) {
+    const HSphere &sp = s->spheres[i];
+    BRef &r = B.refs[ntri + i];
+    r.lo[0] = sp.c.x - sp.r; r.lo[1] = sp.c.y - sp.r; r.lo[2] = sp.c.z - sp.r;
+    r.hi[0] = sp.c.x + sp.r; r.hi[1] = sp.c.y + sp.r; r.hi[2] = sp.c.z + sp.r;
+    r.prim = ntri + i;
+    info[ntri + i] = make_uint2(sp.mat | (PRIM_SPHERE << 24) | ((sp.flags & 3u) << 28), (uint32_t)sp.light);
+    geom[ntri + i] = make_float4(sp.c.x, sp.c.y, sp.c.z, sp.r);
+  }
+  for (uint32_t i = 0; i < s->rects.size(); ++i) {
+    const HRect &rc = s->rects[i];
+    uint32_t id = ntri + nsph + i;
+    BRef &r = B.refs[id];
+    int a = rc.axis, b0 = (a == 0) ? 1 : 0, b1 = (a == 2) ? 1 : 2;
+    r.lo[a] = rc.k; r.hi[a] = rc.k;
+    r.lo[b0] = rc.d0min; r.hi[b0] = rc.d0max;
+    r.lo[b1] = rc.d1min; r.hi[b1] = rc.d1max;
+    r.prim = id;
+    info[id] = make_uint2(rc.mat | (PRIM_RECT << 24) | ((rc.flags & 3u) << 28), (uint32_t)rc.light);
+    geom[id] = make_float4((float)a, rc.k, 0.f, 0.f);
+  }
+
+  // -- BVH
+  B.nodes.resize(4 * std::max<size_t>(1, 2 * nprim));
+  int root = B.build(0, nprim, 0, true);
+  uint32_t nn = B.node_count.load();
+  if (root < 0) {  // the whole scene is one leaf: wrap it in a root with both children on it
+    uint32_t ni = nn++;
+    B.write_node(ni, 0, nprim, nprim, root, root);
+    // write_node bounded [0,nprim) and [nprim,nprim); give child 1 the same box
+    B.nodes[4 * (size_t)ni + 1] = B.nodes[4 * (size_t)ni + 0];
+    float4 z = B.nodes[4 * (size_t)ni + 2];
+    B.nodes[4 * (size_t)ni + 2] = make_float4(z.x, z.y, z.x, z.y);
+    // the root must be node 0: swap it into place
+    if (ni != 0) return fail(PT_ERR_INVALID, "internal: leaf root not at node 0");
+    root = (int)ni;
+  }
+  B.nodes.resize(4 * (size_t)nn);
+
+  // -- primitives in leaf order
+  std::vector<float4> prims(3 * nprim);
+  for (size_t k = 0; k < nprim; ++k) {
+    uint32_t id = B.refs[k].prim;
+    float4 *P = &prims[3 * k];
+    if (id < ntri) {
+      const HTri &t = s->tris[id];
+      f3 e1 = sub3(t.p[1], t.p[0]), e2 = sub3(t.p[2], t.p[0]);  // meshBVH.cpp:93-94
+      uint32_t tag = id | (PRIM_TRI << 30);
+      P[0] = make_float4(t.p[0].x, t.p[0].y, t.p[0].z, __builtin_bit_cast(float, tag));
+      P[1] = make_float4(e1.x, e1.y, e1.z, 0.f);
+      P[2] = make_float4(e2.x, e2.y, e2.z, 0.f);
+    } else if (id < ntri + nsph) {
+      const HSphere &sp = s->spheres[id - ntri];
+      uint32_t tag = id | (PRIM_SPHERE << 30);
+      P[0] = make_float4(sp.c.x, sp.c.y, sp.c.z, __builtin_bit_cast(float, tag));
+      P[1] = make_float4(sp.r, 0.f, 0.f, 0.f);
+      P[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      const HRect &rc = s->rects[id - ntri - nsph];
+      uint32_t tag = id | ((uint32_t)PRIM_RECT << 30);
+      P[0] = make_float4(rc.k, rc.d0min, rc.d0max, __builtin_bit_cast(float, tag));
+      P[1] = make_float4(rc.d1min, rc.d1max, (float)rc.axis, 0.f);
+      P[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+
+  // -- environment tables (2-D piecewise-constant; same construction as the oracle)
+  std::vector<float> efunc, ecdf, emarg, emcdf;
+  float eint = 0.f;
+  if (!s->env.empty()) {
+    uint32_t W = s->env_w, H = s->env_h;
+    efunc.resize((size_t)W * H);
+    ecdf.resize((size_t)(W + 1) * H);
+    emarg.resize(H);
+    emcdf.resize(H + 1);
+    for (uint32_t y = 0; y < H; ++y) {
+      uint32_t y0 = y < H - 2 ? y : H - 2;
+      float st = sinf((float)M_PI * ((float)y + 0.5f) / (float)H);
+      for (uint32_t x = 0; x < W; ++x) {
+        uint32_t x0 = x < W - 2 ? x : W - 2;
+        float v = std::max(std::max(texel_max(s, x0, y0), texel_max(s, x0 + 1, y0)),
+                           std::max(texel_max(s, x0, y0 + 1), texel_max(s, x0 + 1, y0 + 1)));
+        efunc[(size_t)y * W + x] = v * st;
+      }
+    }
+    for (uint32_t y = 0; y < H; ++y) {
+      float *cdf = &ecdf[(size_t)y * (W + 1)];
+      const float *f = &efunc[(size_t)y * W];
+      double acc = 0.0;
+      cdf[0] = 0.f;
+      for (uint32_t x = 0; x < W; ++x) {
+        acc += (double)f[x] / W;
+        cdf[x + 1] = (float)acc;
+      }
+      emarg[y] = (float)acc;
+      if (acc > 0.0)
+        for (uint32_t x = 1; x <= W; ++x) cdf[x] = (float)((double)cdf[x] / acc);
+      else
+        for (uint32_t x = 1; x <= W; ++x) cdf[x] = (float)x / (float)W;
+    }
+    double acc = 0.0;
+    emcdf[0] = 0.f;
+    for (uint32_t y = 0; y < H; ++y) {
+      acc += (double)emarg[y] / H;
+      emcdf[y + 1] = (float)acc;
+    }
+    eint = (float)acc;
+    for (uint32_t y = 1; y <= H; ++y) emcdf[y] = acc > 0.0 ? (float)((double)emcdf[y] / acc) : (float)y / H;
+  }
+
+  // -- upload
+  HIP_TRY(upload(s->d_nodes, B.nodes.data(), B.nodes.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_prims, prims.data(), prims.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_info, info.data(), info.size() * sizeof(uint2)));
+  HIP_TRY(upload(s->d_geom, geom.data(), geom.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_mats, s->mats.data(), s->mats.size() * sizeof(pt_material_desc)));
+  if (s->lights.empty())
+    HIP_TRY(s->d_lights.alloc(sizeof(DevLight)));
+  else
+    HIP_TRY(upload(s->d_lights, s->lights.data(), s->lights.size() * sizeof(DevLight)));
+  if (!s->env.empty()) {
+    HIP_TRY(upload(s->d_env, s->env.data(), s->env.size() * sizeof(float)));
+    HIP_TRY(upload(s->d_env_func, efunc.data(), efunc.size() * sizeof(float)));
+    HIP_TRY(upload(s->d_env_cdf, ecdf.data(), ecdf.size() * sizeof(float)));
+    HIP_TRY(upload(s->d_env_mcdf, emcdf.data(), emcdf.size() * sizeof(float)));
+  }
+  DevScene &ds = s->ds;
+  ds.nodes = s->d_nodes.as<float4>();
+  ds.prims = s->d_prims.as<float4>();
+  ds.prim_info = s->d_info.as<uint2>();
+  ds.prim_geom = s->d_geom.as<float4>();
+  ds.mats = s->d_mats.as<pt_material_desc>();
+  ds.lights = s->d_lights.as<DevLight>();
+  ds.n_lights = (int)s->lights.size();
+  ds.root = root;
+  ds.has_env = !s->env.empty();
+  ds.env = ds.has_env ? s->d_env.as<float>() : nullptr;
+  ds.env_w = (int)s->env_w;
+  ds.env_h = (int)s->env_h;
+  ds.env_scale = s->env_scale;
+  ds.env_func = ds.has_env ? s->d_env_func.as<float>() : nullptr;
+  ds.env_cdf = ds.has_env ? s->d_env_cdf.as<float>() : nullptr;
+  ds.env_mcdf = ds.has_env ? s->d_env_mcdf.as<float>() : nullptr;
+  ds.env_integral = eint;
+  s->perm_valid = false;
+  st = ensure_perm(s, 0u);
+  if (st != PT_OK) return st;
+
+  auto t1 = std::chrono::steady_clock::now();
+  pt_scene_info &in = s->info;
+  in.n_materials = (uint32_t)s->mats.size();
+  in.n_triangles = ntri;
+  in.n_spheres = nsph;
+  in.n_rects = (uint32_t)s->rects.size();
+  in.n_lights = (uint32_t)s->lights.size();
+  in.bvh_nodes = nn;
+  in.bvh_depth = B.max_depth;
+  in.device_bytes = s->d_nodes.bytes + s->d_prims.bytes + s->d_info.bytes + s->d_geom.bytes + s->d_env.bytes +
+                    s->d_env_func.bytes + s->d_env_cdf.bytes;
+  in.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  s->committed = true;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_get_info(const pt_scene *s, pt_scene_info *info) {
+  if (!s || !info) return fail(PT_ERR_INVALID, "pt_scene_get_info: NULL argument");
+  *info = s->info;
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ wavefront buffers
+static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
+  if (s->wave_cap >= cap && s->w_buf.p) return PT_OK;
+  const size_t per_slot_f = 21;  // 4-byte words per slot (the arrays taken below)
+  const size_t per_shadow = 11;  // 4-byte words per shadow-queue entry
+  size_t scap = 4 * (size_t)cap;
+  size_t bytes = (per_slot_f * cap + per_shadow * scap) * 4 + (per_slot_f + per_shadow) * 256;
+  HIP_TRY(s->w_buf.alloc(bytes));
+  HIP_TRY(s->w_counts.alloc(64 * sizeof(uint32_t)));
+  HIP_TRY(s->w_stats.alloc(CNT_N * sizeof(unsigned long long)));
+  char *p = s->w_buf.as<char>();
+  auto take = [&](size_t n) {
+    char *r = p;
+    p += ((n * 4 + 255) / 256) * 256;
+    return r;
+  };
+  Wave &w = s->wave;
+  w.ox = (float *)take(cap); w.oy = (float *)take(cap); w.oz = (float *)take(cap);
+  w.dx = (float *)take(cap); w.dy = (float *)take(cap); w.dz = (float *)take(cap);
+  w.t = (float *)take(cap); w.prim = (int *)take(cap);
+  w.bx = (float *)take(cap); w.by = (float *)take(cap); w.bz = (float *)take(cap);
+  w.lx = (float *)take(cap); w.ly = (float *)take(cap); w.lz = (float *)take(cap);
+  w.ppdf = (float *)take(cap);
+  w.px = (float *)take(cap); w.py = (float *)take(cap); w.pz = (float *)take(cap);
+  w.flags = (uint32_t *)take(cap);
+  w.q[0] = (uint32_t *)take(cap);
+  w.q[1] = (uint32_t *)take(cap);
+  w.sox = (float *)take(scap); w.soy = (float *)take(scap); w.soz = (float *)take(scap);
+  w.sdx = (float *)take(scap); w.sdy = (float *)take(scap); w.sdz = (float *)take(scap);
+  w.stmax = (float *)take(scap);
+  w.scr = (float *)take(scap); w.scg = (float *)take(scap); w.scb = (float *)take(scap);
+  w.sslot = (uint32_t *)take(scap);
+  if ((size_t)(p - s->w_buf.as<char>()) > s->w_buf.bytes) return fail(PT_ERR_INVALID, "internal: wave layout overflow");
+  w.counts = s->w_counts.as<uint32_t>();
+  w.stats = s->w_stats.as<unsigned long long>();
+  w.cap = cap;
+  w.shadow_cap = (uint32_t)scap;
+  s->wave_cap = cap;
+  return PT_OK;
+}
+
+static DevCamera make_camera(const pt_camera_desc &c) {
+  // PerspectiveCamera::PerspectiveCamera (camera.cpp:7-35)
+  DevCamera d{};
+  float theta = c.fov / 2.f;
+  float hh = tanf(theta);
+  float ph = 1.0f * hh * 2.0f;
+  float pw = ph * c.aspect;
+  f3 up = mk3(c.up[0], c.up[1], c.up[2]);
+  f3 front = mk3(c.front[0], c.front[1], c.front[2]);
+  d.lookat = neg3(norm3(front));
+  d.h = norm3(cross3(up, d.lookat));
+  d.v = norm3(cross3(d.lookat, d.h));
+  d.horiz = scl3(scl3(d.h, pw), c.focal_dist);
+  d.vert = scl3(scl3(d.v, ph), c.focal_dist);
+  d.pos = mk3(c.pos[0], c.pos[1], c.pos[2]);
+  d.pos1 = mk3(c.pos1[0], c.pos1[1], c.pos1[2]);
+  d.focal_dist = c.focal_dist;
+  d.radius = (c.aperture > 0.f && std::isfinite(c.aperture)) ? c.focal_dist / c.aperture * 0.5f : 0.f;
+  d.shutter = c.shutter;
+  d.normalize_dir = (int)c.normalize_dir;
+  return d;
+}
+
+static void grid_factor(uint32_t n, uint32_t &a, uint32_t &b) {
+  uint32_t x = 1;
+  for (uint32_t f = 1; f * f <= n; ++f)
+    if (n % f == 0) x = f;
+  a = x;
+  b = n / x;
+}
+
+// ------------------------------------------------------------------ render
+extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *stats) {
+  if (!s || !cfg) return fail(PT_ERR_INVALID, "pt_render: NULL argument");
+  if (!s->committed) return fail(PT_ERR_INVALID, "pt_render: scene not committed");
+  if (!s->has_cam) return fail(PT_ERR_INVALID, "pt_render: camera is not initialized");  // renderManager.hpp:58-62
+  if (cfg->width == 0 || cfg->height == 0 || cfg->spp_x == 0 || cfg->spp_y == 0)
+    return fail(PT_ERR_INVALID, "pt_render: width, height, spp_x, spp_y must be > 0");
+  if (cfg->estimator > PT_EST_NEE_MIS) return fail(PT_ERR_INVALID, "pt_render: unknown estimator");
+  if (cfg->sampler > PT_SAMPLER_STRATIFIED) return fail(PT_ERR_INVALID, "pt_render: unknown sampler");
+  if (cfg->estimator == PT_EST_REFERENCE_NEE && cfg->light_samples > 4)
+    return fail(PT_ERR_UNSUPPORTED, "pt_render: at most 4 light samples per vertex");
+  const uint32_t spp = cfg->spp_x * cfg->spp_y;
+  if (cfg->spp_begin >= spp) return fail(PT_ERR_INVALID, "pt_render: spp_begin out of range");
+  const uint32_t sppc = cfg->spp_count ? std::min(cfg->spp_count, spp - cfg->spp_begin) : spp - cfg->spp_begin;
+  const uint64_t npix = (uint64_t)cfg->width * cfg->height;
+  if (npix * spp >= (1ull << 32)) return fail(PT_ERR_INVALID, "pt_render: width*height*spp must be < 2^32");
+  const uint64_t total = npix * sppc;
+  pt_status st = ensure_stream(s);
+  if (st != PT_OK) return st;
+  if (cfg->sampler == PT_SAMPLER_HALTON_PERM) {
+    st = ensure_perm(s, cfg->seed);
+    if (st != PT_OK) return st;
+  }
+  uint32_t want = cfg->wavefront ? cfg->wavefront : (1u << 21);
+  uint64_t chunk_pix = std::max<uint64_t>(1, want / sppc);
+  chunk_pix = std::min(chunk_pix, npix);
+  uint32_t cap = (uint32_t)(chunk_pix * sppc);
+  st = ensure_wave(s, cap);
+  if (st != PT_OK) return st;
+
+  // film
+  if (s->film_w != cfg->width || s->film_h != cfg->height || !(cfg->flags & PT_RENDER_ACCUMULATE)) {
+    HIP_TRY(s->film.alloc(npix * 3 * sizeof(float)));
+    HIP_TRY(hipMemsetAsync(s->film.p, 0, npix * 3 * sizeof(float), s->stream));
+    s->film_w = cfg->width;
+    s->film_h = cfg->height;
+    s->film_spp = 0;
+  }
+  float *samples = nullptr;
+  if (cfg->flags & PT_RENDER_KEEP_SAMPLES) {
+    s->samples_n = (size_t)total * 3;
+    HIP_TRY(s->samples.alloc(s->samples_n * sizeof(float)));
+    samples = s->samples.as<float>();
+  }
+  // per-frame jittered pixel offsets (FrameBuffer::setOffsets, framebuffer.cpp:131-145)
+  std::vector<float> off(2 * spp);
+  for (uint32_t i = 0; i < cfg->spp_x; ++i)
+    for (uint32_t j = 0; j < cfg->spp_y; ++j) {
+      uint32_t k = i * cfg->spp_y + j;
+      float a0 = (float)i / (float)cfg->spp_x, a1 = (float)(i + 1) / (float)cfg->spp_x;
+      float b0 = (float)j / (float)cfg->spp_y, b1 = (float)(j + 1) / (float)cfg->spp_y;
+      off[k] = (float)(a0 + (a1 - a0) * (double)pt_rng(0xffffffffu - k, 0u, cfg->seed ^ 0x5bd1e995u));
+      off[spp + k] = (float)(b0 + (b1 - b0) * (double)pt_rng(0xffffffffu - k, 1u, cfg->seed ^ 0x5bd1e995u));
+    }
+  HIP_TRY(s->d_off.alloc(off.size() * sizeof(float)));
+  HIP_TRY(hipMemcpyAsync(s->d_off.p, off.data(), off.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemsetAsync(s->w_stats.p, 0, CNT_N * sizeof(unsigned long long), s->stream));
+
+  RenderParams rp{};
+  rp.width = cfg->width;
+  rp.height = cfg->height;
+  rp.spp_x = cfg->spp_x;
+  rp.spp_y = cfg->spp_y;
+  rp.spp = spp;
+  rp.spp_begin = cfg->spp_begin;
+  rp.spp_count = sppc;
+  rp.max_depth = cfg->max_depth;
+  rp.sampler = cfg->sampler;
+  rp.estimator = cfg->estimator;
+  rp.rr_depth = cfg->rr_depth;
+  rp.light_samples = cfg->light_samples;
+  grid_factor(std::max(1u, cfg->light_samples), rp.lx, rp.ly);
+  rp.seed = cfg->seed;
+  rp.bg[0] = cfg->bg[0]; rp.bg[1] = cfg->bg[1]; rp.bg[2] = cfg->bg[2];
+  DevCamera cam = make_camera(s->cam);
+  const bool count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
+  const bool timing = (cfg->flags & PT_RENDER_TIMING) != 0;
+  const uint32_t grid_cap = s->n_cu * 16;
+  const uint32_t nverts = cfg->estimator == PT_EST_NEE_MIS ? cfg->max_depth + 1 : cfg->max_depth;
+
+  struct Ev { hipEvent_t a, b; int kind; };
+  std::vector<Ev> evs;
+  auto ev_begin = [&](int kind) -> int {
+    if (!timing) return -1;
+    Ev e;
+    (void)hipEventCreate(&e.a);
+    (void)hipEventCreate(&e.b);
+    e.kind = kind;
+    (void)hipEventRecord(e.a, s->stream);
+    evs.push_back(e);
+    return (int)evs.size() - 1;
+  };
+  auto ev_end = [&](int i) {
+    if (i >= 0) (void)hipEventRecord(evs[i].b, s->stream);
+  };
+  hipEvent_t tstart, tstop;
+  HIP_TRY(hipEventCreate(&tstart));
+  HIP_TRY(hipEventCreate(&tstop));
+  HIP_TRY(hipEventRecord(tstart, s->stream));
+  uint64_t launches = 0;
+  for (uint64_t first = 0; first < total; first += cap) {
+    uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - first);
+    rp.chunk_first = first;
+    rp.chunk_paths = n;
+    int e = ev_begin(0);
+    launch_generate(s->stream, s->ds, cam, rp, s->wave, s->d_off.as<float>(), s->d_off.as<float>() + spp);
+    launch_set_counts(s->stream, s->wave.counts, n, 0, 0);
+    ev_end(e);
+    int qi = 0;
+    for (uint32_t d = 0; d < nverts; ++d) {
+      e = ev_begin(1);
+      launch_extend(s->stream, s->ds, s->wave, qi, n, grid_cap, count);
+      ev_end(e);
+      ++launches;
+      e = ev_begin(2);
+      launch_clear_count(s->stream, s->wave.counts, (uint32_t)(qi ^ 1));
+      launch_clear_count(s->stream, s->wave.counts, (uint32_t)CNT_SHADOW);
+      launch_shade(s->stream, s->ds, rp, s->wave, qi, d, n, grid_cap);
+      ev_end(e);
+      e = ev_begin(3);
+      launch_shadow(s->stream, s->ds, s->wave, 4u * n, grid_cap, count);
+      ev_end(e);
+      qi ^= 1;
+    }
+    e = ev_begin(4);
+    launch_film(s->stream, rp, s->wave, s->film.as<float>(), samples, s->wave.stats);
+    ev_end(e);
+  }
+  HIP_TRY(hipEventRecord(tstop, s->stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->film_spp += sppc;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, tstart, tstop);
+    stats->ms_total = ms;
+    unsigned long long c[CNT_N];
+    HIP_TRY(hipMemcpy(c, s->w_stats.p, sizeof(c), hipMemcpyDeviceToHost));
+    stats->paths = total;
+    stats->segments = c[CNT_SEGMENTS];
+    stats->shadow_rays = c[CNT_SHADOW_RAYS];
+    stats->node_visits = c[CNT_NODES];
+    stats->prim_tests = c[CNT_PRIMS];
+    stats->shadow_node_visits = c[CNT_SNODES];
+    stats->shadow_prim_tests = c[CNT_SPRIMS];
+    stats->nan_samples = c[CNT_NAN];
+    stats->extend_launches = launches;
+    for (auto &ev : evs) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, ev.a, ev.b);
+      double *dst[5] = {&stats->ms_generate, &stats->ms_extend, &stats->ms_shade, &stats->ms_shadow, &stats->ms_film};
+      *dst[ev.kind] += t;
+    }
+  }
+  for (auto &ev : evs) {
+    (void)hipEventDestroy(ev.a);
+    (void)hipEventDestroy(ev.b);
+  }
+  (void)hipEventDestroy(tstart);
+  (void)hipEventDestroy(tstop);
+  return PT_OK;
+}
+
+extern "C" pt_status pt_film_read(pt_scene *s, float *out) {
+  if (!s || !out) return fail(PT_ERR_INVALID, "pt_film_read: NULL argument");
+  if (!s->film.p || s->film_spp == 0) return fail(PT_ERR_INVALID, "pt_film_read: nothing rendered");
+  size_t n = (size_t)s->film_w * s->film_h * 3;
+  HIP_TRY(hipMemcpy(out, s->film.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  float inv = 1.f / (float)s->film_spp;
+  for (size_t i = 0; i < n; ++i) out[i] *= inv;  // FrameBuffer average (renderManager.hpp:146-150)
+  return PT_OK;
+}
+
+extern "C" pt_status pt_film_read_samples(pt_scene *s, float *out, size_t n) {
+  if (!s || !out) return fail(PT_ERR_INVALID, "pt_film_read_samples: NULL argument");
+  if (!s->samples.p || n != s->samples_n) return fail(PT_ERR_INVALID, "pt_film_read_samples: size mismatch");
+  HIP_TRY(hipMemcpy(out, s->samples.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ parity entry points
+namespace {
+// Temporary device buffers of one parity call; each buffer lives in its own
+// heap node so that growing the list never moves or frees a live allocation.
+struct Scratch {
+  std::vector<std::unique_ptr<DevBuf>> bufs;
+  void *put(const void *src, size_t n) {
+    bufs.push_back(std::make_unique<DevBuf>());
+    DevBuf &b = *bufs.back();
+    if (b.alloc(n) != hipSuccess) return nullptr;
+    if (src && hipMemcpy(b.p, src, n, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return b.p;
+  }
+};
+pt_scene *g_aux_scene() {
+  static pt_scene *s = nullptr;
+  if (!s) s = new pt_scene();
+  return s;
+}
+}  // namespace
+
+#define NEED(ptr)                                                                 \
+  do {                                                                            \
+    if (!(ptr)) return fail(PT_ERR_DEVICE, "device allocation or copy failed");   \
+  } while (0)
+
+extern "C" pt_status pt_sample(uint32_t sampler, uint32_t seed, uint32_t spp, const uint32_t *dim,
+                               const uint64_t *index, const uint32_t *pixel, size_t n, float *out) {
+  if (!dim || !index || !out) return fail(PT_ERR_INVALID, "pt_sample: NULL argument");
+  if (sampler > PT_SAMPLER_STRATIFIED) return fail(PT_ERR_INVALID, "pt_sample: unknown sampler");
+  if (n == 0) return PT_OK;
+  for (size_t i = 0; i < n; ++i)
+    if (index[i] >= (1ull << 32)) return fail(PT_ERR_INVALID, "pt_sample: index must be < 2^32");
+  pt_scene *s = g_aux_scene();
+  pt_status st = ensure_stream(s);
+  if (st != PT_OK) return st;
+  if (sampler == PT_SAMPLER_HALTON_PERM) {
+    st = ensure_perm(s, seed);
+    if (st != PT_OK) return st;
+  }
+  uint32_t sx = 1, sy = std::max(1u, spp);
+  grid_factor(std::max(1u, spp), sx, sy);
+  Scratch sc;
+  auto *d_dim = (uint32_t *)sc.put(dim, n * 4);
+  auto *d_idx = (uint64_t *)sc.put(index, n * 8);
+  auto *d_pix = pixel ? (uint32_t *)sc.put(pixel, n * 4) : nullptr;
+  auto *d_out = (float *)sc.put(nullptr, n * 4);
+  NEED(d_dim); NEED(d_idx); NEED(d_out);
+  if (pixel) NEED(d_pix);
+  launch_sample(s->stream, s->ds, sampler, seed, sx, sy, d_dim, d_idx, d_pix, n, d_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(out, d_out, n * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+extern "C" pt_status pt_intersect(pt_scene *s, const float *rays, size_t n, int any_hit, int32_t *prim, float *t) {
+  if (!s || !rays || !prim || !t) return fail(PT_ERR_INVALID, "pt_intersect: NULL argument");
+  if (!s->committed) return fail(PT_ERR_INVALID, "pt_intersect: scene not committed");
+  if (n == 0) return PT_OK;
+  Scratch sc;
+  auto *d_r = (float *)sc.put(rays, n * 32);
+  auto *d_p = (int32_t *)sc.put(nullptr, n * 4);
+  auto *d_t = (float *)sc.put(nullptr, n * 4);
+  NEED(d_r); NEED(d_p); NEED(d_t);
+  launch_intersect(s->stream, s->ds, d_r, n, any_hit, d_p, d_t);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(prim, d_p, n * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(t, d_t, n * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+extern "C" pt_status pt_bsdf_eval(const pt_material_desc *m, const float *wo, const float *wi, const float *nrm,
+                                  const uint8_t *inside, size_t n, float *f, float *pdf) {
+  if (!m || !wo || !wi || !nrm || !f || !pdf) return fail(PT_ERR_INVALID, "pt_bsdf_eval: NULL argument");
+  if (!valid_kind(m->kind)) return fail(PT_ERR_INVALID, "pt_bsdf_eval: unknown material kind");
+  if (n == 0) return PT_OK;
+  pt_scene *s = g_aux_scene();
+  pt_status st = ensure_stream(s);
+  if (st != PT_OK) return st;
+  Scratch sc;
+  auto *a = (float *)sc.put(wo, n * 12);
+  auto *b = (float *)sc.put(wi, n * 12);
+  auto *c = (float *)sc.put(nrm, n * 12);
+  auto *d = inside ? (uint8_t *)sc.put(inside, n) : nullptr;
+  auto *of = (float *)sc.put(nullptr, n * 12);
+  auto *op = (float *)sc.put(nullptr, n * 4);
+  NEED(a); NEED(b); NEED(c); NEED(of); NEED(op);
+  if (inside) NEED(d);
+  launch_bsdf_eval(s->stream, *m, a, b, c, d, n, of, op);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(f, of, n * 12, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(pdf, op, n * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+extern "C" pt_status pt_bsdf_sample(const pt_material_desc *m, const float *wo, const float *nrm,
+                                    const uint8_t *inside, const float *uc, const float *u2, const float *aux,
+                                    size_t n, float *wi, float *f, float *pdf, uint32_t *flags, uint8_t *ok) {
+  if (!m || !wo || !nrm || !uc || !u2 || !wi || !f || !pdf || !flags || !ok)
+    return fail(PT_ERR_INVALID, "pt_bsdf_sample: NULL argument");
+  if (!valid_kind(m->kind)) return fail(PT_ERR_INVALID, "pt_bsdf_sample: unknown material kind");
+  if (n == 0) return PT_OK;
+  pt_scene *s = g_aux_scene();
+  pt_status st = ensure_stream(s);
+  if (st != PT_OK) return st;
+  Scratch sc;
+  auto *a = (float *)sc.put(wo, n * 12);
+  auto *c = (float *)sc.put(nrm, n * 12);
+  auto *d = inside ? (uint8_t *)sc.put(inside, n) : nullptr;
+  auto *e = (float *)sc.put(uc, n * 4);
+  auto *g = (float *)sc.put(u2, n * 8);
+  auto *x = aux ? (float *)sc.put(aux, n * 4) : nullptr;
+  auto *owi = (float *)sc.put(nullptr, n * 12);
+  auto *of = (float *)sc.put(nullptr, n * 12);
+  auto *op = (float *)sc.put(nullptr, n * 4);
+  auto *ofl = (uint32_t *)sc.put(nullptr, n * 4);
+  auto *ook = (uint8_t *)sc.put(nullptr, n);
+  NEED(a); NEED(c); NEED(e); NEED(g); NEED(owi); NEED(of); NEED(op); NEED(ofl); NEED(ook);
+  if (inside) NEED(d);
+  if (aux) NEED(x);
+  launch_bsdf_sample(s->stream, *m, a, c, d, e, g, x, n, owi, of, op, ofl, ook);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(wi, owi, n * 12, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(f, of, n * 12, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(pdf, op, n * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(flags, ofl, n * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ok, ook, n, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+extern "C" pt_status pt_light_sample(pt_scene *s, uint32_t light, const float *u2, const float *ref, size_t n,
+                                     float *p, float *nrm, float *pdf, float *le) {
+  if (!s || !u2 || !ref || !p || !nrm || !pdf || !le) return fail(PT_ERR_INVALID, "pt_light_sample: NULL argument");
+  if (!s->committed) return fail(PT_ERR_INVALID, "pt_light_sample: scene not committed");
+  if (light >= s->lights.size()) return fail(PT_ERR_INVALID, "pt_light_sample: light index out of range");
+  if (n == 0) return PT_OK;
+  Scratch sc;
+  auto *a = (float *)sc.put(u2, n * 8);
+  auto *b = (float *)sc.put(ref, n * 12);
+  auto *op = (float *)sc.put(nullptr, n * 12);
+  auto *on = (float *)sc.put(nullptr, n * 12);
+  auto *od = (float *)sc.put(nullptr, n * 4);
+  auto *ol = (float *)sc.put(nullptr, n * 12);
+  NEED(a); NEED(b); NEED(op); NEED(on); NEED(od); NEED(ol);
+  launch_light_sample(s->stream, s->ds, light, a, b, n, op, on, od, ol);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(p, op, n * 12, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(nrm, on, n * 12, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(pdf, od, n * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(le, ol, n * 12, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+extern "C" pt_status pt_device_synchronize(void) {
+  if (!g_has_device()) return fail(PT_ERR_NO_DEVICE, "no HIP device visible");
+  HIP_TRY(hipDeviceSynchronize());
+  return PT_OK;
+}

@@ -1,0 +1,31 @@
+// pt_launch.h -- host-callable kernel launchers (defined in pt_kernels.hip).
+#pragma once
+#include <algorithm>
+#include "pt_scene.h"
+
+namespace pt {
+
+void launch_generate(hipStream_t st, const DevScene &s, const DevCamera &c, const RenderParams &rp, const Wave &w,
+                     const float *offx, const float *offy);
+void launch_set_counts(hipStream_t st, uint32_t *counts, uint32_t q0, uint32_t q1, uint32_t sh);
+void launch_clear_count(hipStream_t st, uint32_t *counts, uint32_t idx);
+void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uint32_t max_n, uint32_t grid_cap,
+                   bool count);
+void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
+                  uint32_t max_n, uint32_t grid_cap);
+void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count);
+void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
+                 unsigned long long *stats);
+void launch_sample(hipStream_t st, const DevScene &s, uint32_t sampler, uint32_t seed, uint32_t sx, uint32_t sy,
+                   const uint32_t *dim, const uint64_t *index, const uint32_t *pixel, size_t n, float *out);
+void launch_intersect(hipStream_t st, const DevScene &s, const float *rays, size_t n, int any, int32_t *prim,
+                      float *t);
+void launch_bsdf_eval(hipStream_t st, const pt_material_desc &m, const float *wo, const float *wi, const float *nrm,
+                      const uint8_t *inside, size_t n, float *f, float *pdf);
+void launch_bsdf_sample(hipStream_t st, const pt_material_desc &m, const float *wo, const float *nrm,
+                        const uint8_t *inside, const float *uc, const float *u2, const float *aux, size_t n, float *wi,
+                        float *f, float *pdf, uint32_t *flags, uint8_t *ok);
+void launch_light_sample(hipStream_t st, const DevScene &s, uint32_t light, const float *u2, const float *ref,
+                         size_t n, float *p, float *nrm, float *pdf, float *le);
+
+}  // namespace pt

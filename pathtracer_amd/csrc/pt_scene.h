@@ -1,0 +1,105 @@
+// pt_scene.h -- device-resident scene and wavefront layout shared by the
+// host orchestration (pt_host.cpp) and the kernels (pt_kernels.hip).
+//
+// HBM layout (all SoA, 256-byte aligned allocations):
+//   nodes      BVH2 nodes, 64 B each: the boxes of both children plus their
+//              indices, so one fetch decides both children (4 x float4).
+//   prims      primitives in BVH leaf order, 48 B each (3 x float4):
+//                triangle  (v0.xyz, id|type) (e1.xyz, -) (e2.xyz, -)
+//                sphere    (c.xyz,  id|type) (r, -, -, -) (-)
+//                rect      (k, d0min, d0max, id|type) (d1min, d1max, axis, -) (-)
+//              e1 = p1-p0, e2 = p2-p0 are formed on the host exactly as
+//              MeshBVHNode::hitTriangle forms them (meshBVH.cpp:93-94).
+//   prim_info  per global primitive id: x = material | type<<24 | flags<<28,
+//              y = light index (or -1).
+//   prim_geom  per global primitive id: triangle geometric normal,
+//              sphere centre+radius, rect axis.
+#pragma once
+#include <stdint.h>
+#include "pt_math.h"
+#include "pt_sampler.h"
+#include "../../include/pathtracer_amd.h"
+
+namespace pt {
+
+enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
+enum { LIGHT_QUAD = 1, LIGHT_SPHERE = 2, LIGHT_ENV = 3 };
+
+// A child reference: >= 0 internal node index; < 0 leaf ~(start<<4 | count-1).
+PT_HD int leaf_code(uint32_t start, uint32_t count) { return ~(int)((start << 4) | (count - 1)); }
+PT_HD uint32_t leaf_start(int c) { return ((uint32_t)~c) >> 4; }
+PT_HD uint32_t leaf_count(int c) { return (((uint32_t)~c) & 15u) + 1u; }
+
+struct DevLight {
+  int type;
+  float area;
+  f3 v0, e1, e2, n;  // quad: get_sample = v0 + u e1 + v e2 (light.hpp:131-138)
+  f3 c;              // sphere
+  float r;
+  f3 le;             // texture colour * intensity
+};
+
+struct DevScene {
+  const float4 *nodes;
+  const float4 *prims;
+  const uint2 *prim_info;
+  const float4 *prim_geom;
+  const pt_material_desc *mats;
+  const DevLight *lights;
+  int n_lights;
+  int root;  // child code of the root (node 0 or a single leaf)
+  // environment (texels: (y*w + x)*3)
+  const float *env;
+  int env_w, env_h;
+  float env_scale;
+  const float *env_func, *env_cdf, *env_mcdf;
+  float env_integral;
+  int has_env;
+  // Halton permutation tables
+  const uint16_t *perm;
+  const uint32_t *perm_off;
+  uint32_t perm_seed;
+};
+
+struct DevCamera {
+  f3 pos, pos1, h, v, horiz, vert, lookat;
+  float focal_dist, radius, shutter;
+  int normalize_dir;
+};
+
+// Per-render constants.
+struct RenderParams {
+  uint32_t width, height, spp_x, spp_y, spp, spp_begin, spp_count;
+  uint32_t max_depth, sampler, estimator, rr_depth, light_samples, lx, ly, seed;
+  float bg[3];
+  uint64_t chunk_first;  // first path number of the chunk (pixel-major order)
+  uint32_t chunk_paths;
+};
+
+// Wavefront buffers for `cap` path slots.
+struct Wave {
+  // per slot
+  float *ox, *oy, *oz, *dx, *dy, *dz;
+  float *t;
+  int *prim;
+  float *bx, *by, *bz;  // throughput
+  float *lx, *ly, *lz;  // radiance
+  float *ppdf;          // pdf of the BSDF sample that led here (MIS)
+  float *px, *py, *pz;  // previous vertex (MIS)
+  uint32_t *flags;      // bit0: previous vertex specular (or camera)
+  // queues
+  uint32_t *q[2];
+  uint32_t *counts;            // [0],[1] ray queues, [2] shadow queue
+  unsigned long long *stats;   // CNT_* traversal / path counters
+  // shadow queue (entries)
+  float *sox, *soy, *soz, *sdx, *sdy, *sdz, *stmax, *scr, *scg, *scb;
+  uint32_t *sslot;
+  uint32_t cap, shadow_cap;
+};
+
+enum {
+  CNT_Q0 = 0, CNT_Q1 = 1, CNT_SHADOW = 2, CNT_SEGMENTS = 3, CNT_SHADOW_RAYS = 4,
+  CNT_NODES = 5, CNT_PRIMS = 6, CNT_SNODES = 7, CNT_SPRIMS = 8, CNT_NAN = 9, CNT_N = 16
+};
+
+}  // namespace pt

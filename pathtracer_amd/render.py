@@ -1,0 +1,142 @@
+"""GPU renderer: lowers a SceneSpec onto the C ABI (the role of SceneInfo,
+render/renderManager.hpp:35-158) and runs the wavefront kernels."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .scenes import SceneSpec, camera_desc, render_cfg
+
+
+class Scene:
+    """A committed device scene."""
+
+    def __init__(self, spec: SceneSpec, leaf_size: int = 4, sah_bins: int = 16):
+        self.spec = spec
+        h = ctypes.c_void_p()
+        L.check(L.lib.pt_scene_create(ctypes.byref(h)), "pt_scene_create")
+        self.h = h
+        self._keep = []
+        for kind, params in spec.materials:
+            mid = ctypes.c_uint32()
+            L.check(L.lib.pt_scene_add_material(h, ctypes.byref(L.material(kind, params)), ctypes.byref(mid)),
+                    "pt_scene_add_material")
+        for kind, i in spec.order:
+            if kind == "mesh":
+                pos, idx, mat, flags = spec.meshes[i]
+                L.check(L.lib.pt_scene_add_mesh(h, L.ptr(pos), pos.shape[0], L.ptr(idx), idx.shape[0], mat, flags),
+                        "pt_scene_add_mesh")
+            elif kind == "sphere":
+                c, r, mat, flags = spec.spheres[i]
+                ca = np.asarray(c, np.float32)
+                L.check(L.lib.pt_scene_add_sphere(h, L.ptr(ca), r, mat, flags), "pt_scene_add_sphere")
+            else:
+                axis, k, a0, a1, b0, b1, mat, flags = spec.rects[i]
+                L.check(L.lib.pt_scene_add_rect(h, axis, k, a0, a1, b0, b1, mat, flags), "pt_scene_add_rect")
+        if spec.env is not None:
+            rgb, scale = spec.env
+            rgb = np.ascontiguousarray(rgb, np.float32)
+            L.check(L.lib.pt_scene_set_env(h, L.ptr(rgb), rgb.shape[1], rgb.shape[0], scale), "pt_scene_set_env")
+        cam = camera_desc(**spec.camera)
+        L.check(L.lib.pt_scene_set_camera(h, ctypes.byref(cam)), "pt_scene_set_camera")
+        b = L.BuildCfg()
+        b.leaf_size, b.sah_bins = leaf_size, sah_bins
+        L.check(L.lib.pt_scene_commit(h, ctypes.byref(b)), "pt_scene_commit")
+        info = L.SceneInfo()
+        L.check(L.lib.pt_scene_get_info(h, ctypes.byref(info)), "pt_scene_get_info")
+        self.info = info
+
+    def close(self):
+        if self.h:
+            L.lib.pt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cfg(self, **overrides) -> L.RenderCfg:
+        kw = dict(self.spec.render)
+        kw.update(overrides)
+        return render_cfg(**kw)
+
+    def render(self, cfg: L.RenderCfg | None = None, **overrides):
+        """Returns (image (h, w, 3) per-pixel mean, stats dict)."""
+        cfg = cfg or self.cfg(**overrides)
+        st = L.Stats()
+        L.check(L.lib.pt_render(self.h, ctypes.byref(cfg), ctypes.byref(st)), "pt_render")
+        img = np.empty((cfg.height, cfg.width, 3), np.float32)
+        L.check(L.lib.pt_film_read(self.h, L.ptr(img)), "pt_film_read")
+        return img, st.as_dict()
+
+    def render_device(self, cfg: L.RenderCfg) -> dict:
+        st = L.Stats()
+        L.check(L.lib.pt_render(self.h, ctypes.byref(cfg), ctypes.byref(st)), "pt_render")
+        return st.as_dict()
+
+    def samples(self, cfg: L.RenderCfg) -> np.ndarray:
+        spp = cfg.spp_x * cfg.spp_y
+        sppc = cfg.spp_count or (spp - cfg.spp_begin)
+        n = cfg.width * cfg.height * sppc * 3
+        out = np.empty(n, np.float32)
+        L.check(L.lib.pt_film_read_samples(self.h, L.ptr(out), n), "pt_film_read_samples")
+        return out.reshape(cfg.height * cfg.width, sppc, 3)
+
+    def intersect(self, rays: np.ndarray, any_hit: bool = False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        L.check(L.lib.pt_intersect(self.h, L.ptr(rays), n, int(any_hit), L.ptr(prim), L.ptr(t)), "pt_intersect")
+        return prim, t
+
+    def light_sample(self, light: int, u2: np.ndarray, ref: np.ndarray):
+        u2 = np.ascontiguousarray(u2, np.float32)
+        ref = np.ascontiguousarray(ref, np.float32)
+        n = u2.shape[0]
+        p, nrm, le = (np.empty((n, 3), np.float32) for _ in range(3))
+        pdf = np.empty(n, np.float32)
+        L.check(L.lib.pt_light_sample(self.h, light, L.ptr(u2), L.ptr(ref), n, L.ptr(p), L.ptr(nrm), L.ptr(pdf),
+                                      L.ptr(le)), "pt_light_sample")
+        return p, nrm, pdf, le
+
+
+def sample(sampler: int, seed: int, dims, index, pixel=None, spp: int = 1) -> np.ndarray:
+    dims = np.ascontiguousarray(dims, np.uint32)
+    index = np.ascontiguousarray(index, np.uint64)
+    pix = None if pixel is None else np.ascontiguousarray(pixel, np.uint32)
+    out = np.empty(dims.shape[0], np.float32)
+    L.check(L.lib.pt_sample(sampler, seed, spp, L.ptr(dims), L.ptr(index), L.ptr(pix), dims.shape[0], L.ptr(out)),
+            "pt_sample")
+    return out
+
+
+def bsdf_eval(kind: int, params, wo, wi, nrm, inside=None):
+    m = L.material(kind, params)
+    wo, wi, nrm = (np.ascontiguousarray(a, np.float32) for a in (wo, wi, nrm))
+    n = wo.shape[0]
+    ins = None if inside is None else np.ascontiguousarray(inside, np.uint8)
+    f = np.empty((n, 3), np.float32)
+    pdf = np.empty(n, np.float32)
+    L.check(L.lib.pt_bsdf_eval(ctypes.byref(m), L.ptr(wo), L.ptr(wi), L.ptr(nrm), L.ptr(ins), n, L.ptr(f),
+                               L.ptr(pdf)), "pt_bsdf_eval")
+    return f, pdf
+
+
+def bsdf_sample(kind: int, params, wo, nrm, uc, u2, aux=None, inside=None):
+    m = L.material(kind, params)
+    wo, nrm, uc, u2 = (np.ascontiguousarray(a, np.float32) for a in (wo, nrm, uc, u2))
+    n = wo.shape[0]
+    ins = None if inside is None else np.ascontiguousarray(inside, np.uint8)
+    ax = None if aux is None else np.ascontiguousarray(aux, np.float32)
+    wi, f = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32)
+    pdf = np.empty(n, np.float32)
+    flags = np.empty(n, np.uint32)
+    ok = np.empty(n, np.uint8)
+    L.check(L.lib.pt_bsdf_sample(ctypes.byref(m), L.ptr(wo), L.ptr(nrm), L.ptr(ins), L.ptr(uc), L.ptr(u2), L.ptr(ax),
+                                 n, L.ptr(wi), L.ptr(f), L.ptr(pdf), L.ptr(flags), L.ptr(ok)), "pt_bsdf_sample")
+    return wi, f, pdf, flags, ok

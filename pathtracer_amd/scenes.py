@@ -1,0 +1,267 @@
+"""Scene descriptions and the seeded synthetic scenes of the benchmark workloads.
+
+A SceneSpec is plain data (numpy arrays and parameter lists).  It is fed
+unchanged to the GPU renderer (pathtracer_amd.render) and, in tests and the
+CPU-baseline leg of bench.py, to the CPU oracle, so both sides see the same
+scene.
+
+Workloads (sizes as stated in each builder):
+  cornell()       256x256, 16 spp Halton, depth 8: Cornell box of 20 triangles,
+                  gold-conductor icosphere (1,280 triangles), Lambertian sphere,
+                  0.5x0.5 ceiling light of radiance 15.
+  glass_hdri()    1024x1024, 64 spp Sobol, depth 16: noise-displaced icosphere of
+                  1,310,720 triangles, rough glass (IOR 1.5, alpha 0.05), copper
+                  ground (alpha 0.3), 1024x512 synthetic sky + sun environment,
+                  thin-lens depth of field (lens diameter 0.02).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+GOLD_ETA, GOLD_K = (0.14, 0.43, 1.38), (4.54, 2.455, 1.914)  # commonMaterials.hpp:43-47
+COPPER_ETA, COPPER_K = (0.200438, 0.924033, 1.10221), (3.91295, 2.45285, 2.14219)
+
+
+@dataclasses.dataclass
+class SceneSpec:
+    materials: list = dataclasses.field(default_factory=list)   # (kind, params)
+    meshes: list = dataclasses.field(default_factory=list)      # (pos f32 (n,3), idx i32 (m,3), mat, flags)
+    spheres: list = dataclasses.field(default_factory=list)     # (center(3), r, mat, flags)
+    rects: list = dataclasses.field(default_factory=list)       # (axis, k, d0min, d0max, d1min, d1max, mat, flags)
+    env: tuple | None = None                                     # (rgb f32 (h,w,3) row y=0 is direction -Y, scale)
+    camera: dict = dataclasses.field(default_factory=dict)
+    render: dict = dataclasses.field(default_factory=dict)
+    order: list = dataclasses.field(default_factory=list)        # insertion order of ("mesh"|"sphere"|"rect", index)
+    name: str = ""
+
+    def add_material(self, kind, params) -> int:
+        self.materials.append((kind, list(params)))
+        return len(self.materials) - 1
+
+    def add_mesh(self, pos, idx, mat, flags=0):
+        self.meshes.append((np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(idx, np.int32), mat, flags))
+        self.order.append(("mesh", len(self.meshes) - 1))
+
+    def add_sphere(self, c, r, mat, flags=0):
+        self.spheres.append((tuple(float(x) for x in c), float(r), mat, flags))
+        self.order.append(("sphere", len(self.spheres) - 1))
+
+    def add_rect(self, axis, k, d0min, d0max, d1min, d1max, mat, flags=0):
+        self.rects.append((axis, k, d0min, d0max, d1min, d1max, mat, flags))
+        self.order.append(("rect", len(self.rects) - 1))
+
+    @property
+    def n_triangles(self):
+        return sum(m[1].shape[0] for m in self.meshes)
+
+
+def render_cfg(**kw) -> L.RenderCfg:
+    c = L.RenderCfg()
+    defaults = dict(width=64, height=64, spp_x=1, spp_y=1, max_depth=8, sampler=L.SAMPLER_HALTON_OWEN,
+                    estimator=L.EST_REFERENCE_NEE, rr_depth=0, light_samples=4, seed=0,
+                    bg=(0.0, 0.0, 0.0), spp_begin=0, spp_count=0, wavefront=0, flags=0)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        if k == "bg":
+            for i in range(3):
+                c.bg[i] = float(v[i])
+        else:
+            setattr(c, k, int(v))
+    return c
+
+
+def camera_desc(fov_deg, aspect, focal_dist, aperture, up, front, pos, pos1=None, shutter=0.0,
+                normalize_dir=1) -> L.CameraDesc:
+    c = L.CameraDesc()
+    c.fov = math.radians(fov_deg)
+    c.aspect = aspect
+    c.focal_dist = focal_dist
+    c.aperture = aperture
+    for i in range(3):
+        c.up[i], c.front[i], c.pos[i] = up[i], front[i], pos[i]
+        c.pos1[i] = (pos1 if pos1 is not None else pos)[i]
+    c.shutter = shutter
+    c.normalize_dir = normalize_dir
+    return c
+
+
+# ---------------------------------------------------------------- geometry helpers
+
+def quad_tris(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3)."""
+    pos = np.array([p0, p1, p2, p3], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return pos, idx
+
+
+def merge(parts):
+    pos, idx, off = [], [], 0
+    for p, i in parts:
+        pos.append(p)
+        idx.append(i + off)
+        off += p.shape[0]
+    return np.concatenate(pos).astype(np.float32), np.concatenate(idx).astype(np.int32)
+
+
+def icosphere(subdiv: int):
+    """Unit icosphere: 20*4^subdiv triangles, shared vertices."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t],
+                  [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                  [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+                  [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid])
+        nf = f.shape[0]
+        a = base + inv[:nf]
+        b = base + inv[nf:2 * nf]
+        c = base + inv[2 * nf:]
+        f = np.concatenate([np.stack([f[:, 0], a, c], 1), np.stack([f[:, 1], b, a], 1),
+                            np.stack([f[:, 2], c, b], 1), np.stack([a, b, c], 1)])
+    return v, f
+
+
+def value_noise(p: np.ndarray, seed: int, octaves: int = 4, freq: float = 2.0) -> np.ndarray:
+    """Seeded 3-D value noise in [-1, 1] (lattice values, smoothstep trilinear), octaves summed
+    with weights 1/2^i and normalised."""
+    rng = np.random.default_rng(seed)
+    lattice = rng.uniform(-1.0, 1.0, size=(64, 64, 64))
+    out = np.zeros(p.shape[0])
+    norm = 0.0
+    for o in range(octaves):
+        q = p * (freq * 2 ** o) + 17.0 * o
+        i0 = np.floor(q).astype(np.int64)
+        fr = q - i0
+        w = fr * fr * (3 - 2 * fr)
+        acc = np.zeros(p.shape[0])
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    val = lattice[(i0[:, 0] + dx) & 63, (i0[:, 1] + dy) & 63, (i0[:, 2] + dz) & 63]
+                    wt = (w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1]) * \
+                         (w[:, 2] if dz else 1 - w[:, 2])
+                    acc += wt * val
+        amp = 0.5 ** o
+        out += amp * acc
+        norm += amp
+    return out / norm
+
+
+def displaced_icosphere(subdiv, radius, amp, seed, center=(0, 0, 0)):
+    v, f = icosphere(subdiv)
+    r = radius * (1.0 + amp * value_noise(v, seed))
+    pos = v * r[:, None] + np.asarray(center, np.float64)
+    return pos.astype(np.float32), f.astype(np.int32)
+
+
+def sky_hdri(w=1024, h=512, seed=1, zenith=0.3, horizon=1.0, sun_peak=5e4, sun_width_deg=0.5):
+    """Synthetic equirect sky: gradient from horizon to zenith over the upper
+    hemisphere, dim ground, plus a Gaussian sun lobe.  Texel (x, y) is
+    direction theta = (y+0.5)/h*pi from -Y, phi = (x+0.5)/w*2pi with
+    dir = (-sin th cos ph, -cos th, sin th sin ph) (texture.hpp:105-115)."""
+    rng = np.random.default_rng(seed)
+    y = (np.arange(h) + 0.5) / h * np.pi
+    x = (np.arange(w) + 0.5) / w * 2 * np.pi
+    th, ph = np.meshgrid(y, x, indexing="ij")
+    d = np.stack([-np.sin(th) * np.cos(ph), -np.cos(th), np.sin(th) * np.sin(ph)], -1)
+    elev = np.clip(d[..., 1], -1, 1)
+    sky = np.where(elev >= 0, horizon + (zenith - horizon) * elev, 0.25 * horizon)
+    tint = np.array([0.85, 0.92, 1.0])
+    rgb = sky[..., None] * tint
+    az = rng.uniform(0, 2 * np.pi)
+    el = np.radians(rng.uniform(25, 50))
+    sun = np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
+    ang = np.arccos(np.clip(d @ sun, -1, 1))
+    sig = np.radians(sun_width_deg)
+    rgb = rgb + (sun_peak * np.exp(-0.5 * (ang / sig) ** 2))[..., None] * np.array([1.0, 0.95, 0.85])
+    return rgb.astype(np.float32)
+
+
+# ---------------------------------------------------------------- workloads
+
+def cornell(width=256, height=256, spp_x=4, spp_y=4, max_depth=8, rr_depth=3,
+            sampler=L.SAMPLER_HALTON_OWEN, estimator=L.EST_REFERENCE_NEE, ico_subdiv=3) -> SceneSpec:
+    """Cornell box, side 2.0, made of 10 quads (20 triangles)."""
+    s = SceneSpec(name="cornell")
+    white = s.add_material(L.MAT_LAMBERTIAN, [0.73, 0.73, 0.73])
+    red = s.add_material(L.MAT_LAMBERTIAN, [0.65, 0.05, 0.05])
+    green = s.add_material(L.MAT_LAMBERTIAN, [0.12, 0.45, 0.15])
+    light = s.add_material(L.MAT_EMITTER, [1.0, 1.0, 1.0, 15.0])
+    gold = s.add_material(L.MAT_CONDUCTOR, [*GOLD_ETA, *GOLD_K, 0.2, 0.2])
+    diffuse = s.add_material(L.MAT_LAMBERTIAN, [0.8, 0.8, 0.8])
+    a, b = -1.0, 1.0
+    # each wall in two halves
+    white_parts = [
+        quad_tris((a, 0, a), (0, 0, a), (0, 0, b), (a, 0, b)), quad_tris((0, 0, a), (b, 0, a), (b, 0, b), (0, 0, b)),
+        quad_tris((a, 2, a), (a, 2, b), (0, 2, b), (0, 2, a)), quad_tris((0, 2, a), (0, 2, b), (b, 2, b), (b, 2, a)),
+        quad_tris((a, 0, a), (a, 1, a), (b, 1, a), (b, 0, a)), quad_tris((a, 1, a), (a, 2, a), (b, 2, a), (b, 1, a)),
+    ]
+    red_parts = [quad_tris((a, 0, a), (a, 0, 0), (a, 2, 0), (a, 2, a)),
+                 quad_tris((a, 0, 0), (a, 0, b), (a, 2, b), (a, 2, 0))]
+    green_parts = [quad_tris((b, 0, a), (b, 2, a), (b, 2, 0), (b, 0, 0)),
+                   quad_tris((b, 0, 0), (b, 2, 0), (b, 2, b), (b, 0, b))]
+    s.add_mesh(*merge(white_parts), white)
+    s.add_mesh(*merge(red_parts), red)
+    s.add_mesh(*merge(green_parts), green)
+    s.add_rect(1, 1.999, -0.25, 0.25, -0.25, 0.25, light)
+    v, f = icosphere(ico_subdiv)
+    s.add_mesh((v * 0.35 + np.array([-0.4, 0.35, -0.3])).astype(np.float32), f.astype(np.int32), gold)
+    s.add_sphere((0.45, 0.3, 0.3), 0.3, diffuse)
+    s.camera = dict(fov_deg=40.0, aspect=width / height, focal_dist=1.0, aperture=0.0, up=(0, 1, 0),
+                    front=(0, 0, -1), pos=(0, 1.0, 3.9))
+    s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=max_depth, sampler=sampler,
+                    estimator=estimator, rr_depth=rr_depth, light_samples=4, seed=7, bg=(0, 0, 0))
+    return s
+
+
+def glass_hdri(width=1024, height=1024, spp_x=8, spp_y=8, max_depth=16, subdiv=8, sampler=L.SAMPLER_SOBOL,
+               estimator=L.EST_REFERENCE_NEE, rr_depth=3, env_w=1024, env_h=512, seed=1) -> SceneSpec:
+    """Noise-displaced rough-glass icosphere on a copper plane under a sky."""
+    s = SceneSpec(name="glass_hdri")
+    glass = s.add_material(L.MAT_MFDIELECTRIC, [1.5, 0.05, 0.05])
+    copper = s.add_material(L.MAT_CONDUCTOR, [*COPPER_ETA, *COPPER_K, 0.3, 0.3])
+    pos, idx = displaced_icosphere(subdiv, 1.0, 0.08, seed)
+    s.add_mesh(pos, idx, glass)
+    g = 20.0
+    y0 = -1.1
+    s.add_mesh(*quad_tris((-g, y0, -g), (-g, y0, g), (g, y0, g), (g, y0, -g)), copper)
+    s.env = (sky_hdri(env_w, env_h, seed), 1.0)
+    focal = 4.0
+    s.camera = dict(fov_deg=35.0, aspect=width / height, focal_dist=focal, aperture=focal / 0.02,
+                    up=(0, 1, 0), front=(0, -0.1, -1), pos=(0, 0.4, 4.0))
+    s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=max_depth, sampler=sampler,
+                    estimator=estimator, rr_depth=rr_depth, light_samples=4, seed=11, bg=(0, 0, 0))
+    return s
+
+
+def furnace(albedo=0.5, env_value=1.0, max_depth=64, width=8, height=8, spp_x=8, spp_y=8) -> SceneSpec:
+    """Lambertian sphere of albedo a under a uniform environment of radiance E.
+    A convex object sees only the environment, so every camera ray that hits
+    the sphere has expectation a*E under an unbiased estimator, and 2*a*E
+    under the reference estimator, which counts the environment both through
+    NEE and through the BSDF ray that misses (maxdepth_nee.hpp:26-37, 107-145);
+    rays that miss see E.  Closed-form check of both estimators."""
+    s = SceneSpec(name="furnace")
+    m = s.add_material(L.MAT_LAMBERTIAN, [albedo] * 3)
+    s.add_sphere((0, 0, 0), 1.0, m)
+    s.env = (np.full((16, 32, 3), env_value, np.float32), 1.0)
+    s.camera = dict(fov_deg=20.0, aspect=width / height, focal_dist=1.0, aperture=0.0, up=(0, 1, 0),
+                    front=(0, 0, -1), pos=(0, 0, 5.0))
+    s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=max_depth,
+                    sampler=L.SAMPLER_HALTON_OWEN, estimator=L.EST_NEE_MIS, rr_depth=0, light_samples=4, seed=3,
+                    bg=(0, 0, 0))
+    return s
