@@ -499,3 +499,33 @@ int or_bsdf_sample_batch(const pt_material_desc *m, const float *wo, const float
   }
   return 0;
 }
+
+/* ---------------- exported helpers for closed-form tests ---------------- */
+float or_fresnel_dielectric(float ci, float eta) { return fresnel_dielectric(ci, eta); }
+void or_fresnel_complex(float ci, const float *eta, const float *k, float *out) {
+  v3 r = fresnel_complex(ci, V3(eta[0], eta[1], eta[2]), V3(k[0], k[1], k[2]));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+float or_trd_D(float ax, float ay, const float *wm) {
+  trd_t d = {ax, ay};
+  return trd_D(d, V3(wm[0], wm[1], wm[2]));
+}
+float or_trd_Dw(float ax, float ay, const float *w, const float *wm) {
+  trd_t d = {ax, ay};
+  return trd_Dw(d, V3(w[0], w[1], w[2]), V3(wm[0], wm[1], wm[2]));
+}
+float or_trd_G1(float ax, float ay, const float *w) {
+  trd_t d = {ax, ay};
+  return trd_G1(d, V3(w[0], w[1], w[2]));
+}
+void or_trd_sample_wm(float ax, float ay, const float *w, float u0, float u1, float *out) {
+  trd_t d = {ax, ay};
+  v3 m = trd_sample_wm(d, V3(w[0], w[1], w[2]), u0, u1);
+  out[0] = m.x; out[1] = m.y; out[2] = m.z;
+}
+int or_refract(const float *wi, const float *n, float eta, float *wt, float *etap) {
+  v3 t = vs(0.f);
+  int ok = pbrt_refract(V3(wi[0], wi[1], wi[2]), V3(n[0], n[1], n[2]), eta, etap, &t);
+  wt[0] = t.x; wt[1] = t.y; wt[2] = t.z;
+  return ok;
+}

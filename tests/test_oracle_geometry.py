@@ -1,0 +1,167 @@
+"""Pins the oracle's geometry, lights and camera (CPU only).
+
+Moller-Trumbore known answer of tests/test_mesh_io.cpp:26-32; the median-split
+mesh BVH (meshBVH.cpp:6-83) against brute force over every primitive; sphere
+and quad intersections against their closed forms; QuadLight / SphereLight
+sample and pdf closed forms (light.hpp:119-306); the environment CDF pdf
+integrates to 1 and matches its sampler; camera rays at zero aperture pass
+through the pinhole.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from pathtracer_amd import scenes
+from tests.helpers import random_dirs, unit
+
+
+def _scene_with(mesh=None, spheres=(), rects=(), env=None):
+    s = scenes.SceneSpec()
+    m = s.add_material(1, [0.5, 0.5, 0.5])
+    e = s.add_material(2, [1.0, 0.9, 0.8, 7.0])
+    if mesh is not None:
+        s.add_mesh(mesh[0], mesh[1], m)
+    for c, r, emissive in spheres:
+        s.add_sphere(c, r, e if emissive else m)
+    for args, emissive in rects:
+        s.add_rect(*args, e if emissive else m)
+    if env is not None:
+        s.env = (env, 1.0)
+    s.camera = dict(fov_deg=40, aspect=1, focal_dist=1, aperture=0, up=(0, 1, 0), front=(0, 0, -1), pos=(0, 0, 5))
+    return s
+
+
+def test_mesh_io_known_answer():
+    # tests/data/minimal.obj: (0,0,0), (1,0,0), (0,1,0); ray (.25,.25,-1) + t(0,0,1)
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    idx = np.array([[0, 1, 2]], np.int32)
+    o = O.OracleScene(_scene_with((pos, idx)))
+    prim, t = o.intersect(np.array([[0.25, 0.25, -1, 0, 0, 1, 0.001, 100.0]], np.float32))
+    assert prim[0] == 0
+    assert t[0] == 1.0  # hit point (0.25, 0.25, 0)
+
+
+def test_bvh_matches_brute_force():
+    pos, idx = scenes.displaced_icosphere(3, 1.0, 0.1, 4)
+    spec = _scene_with((pos, idx), spheres=[((1.5, 0, 0), 0.4, False)],
+                       rects=[((1, -1.1, -2, 2, -2, 2), False), ((0, -1.5, -1, 1, -1, 1), False)])
+    o = O.OracleScene(spec)
+    rng = np.random.default_rng(2)
+    n = 40_000
+    org = rng.uniform(-3, 3, size=(n, 3))
+    tgt = rng.uniform(-1.2, 1.2, size=(n, 3))
+    d = unit(tgt - org)
+    rays = np.concatenate([org, d, np.full((n, 1), 1e-4), rng.uniform(0.5, 10, size=(n, 1))], 1).astype(np.float32)
+    p1, t1 = o.intersect(rays)
+    p2, t2 = o.intersect(rays, brute=True)
+    assert np.array_equal(p1 >= 0, p2 >= 0)
+    h = p1 >= 0
+    assert np.allclose(t1[h], t2[h], rtol=1e-6)
+    assert np.mean(p1[h] == p2[h]) > 0.9999   # differing ids only on ties
+    a, _ = o.intersect(rays, any_hit=True)
+    assert np.array_equal(a >= 0, p2 >= 0)
+
+
+def test_sphere_and_rect_closed_forms():
+    o = O.OracleScene(_scene_with(spheres=[((0, 0, 0), 1.0, False)], rects=[((2, -3.0, -1, 1, -1, 1), False)]))
+    rng = np.random.default_rng(4)
+    n = 2000
+    org = np.repeat(np.array([[0.0, 0.0, 5.0]]), n, 0)
+    d = unit(np.concatenate([rng.uniform(-0.15, 0.15, (n, 2)), -np.ones((n, 1))], 1))
+    rays = np.concatenate([org, d, np.full((n, 1), 1e-4), np.full((n, 1), 1e5)], 1).astype(np.float32)
+    prim, t = o.intersect(rays)
+    # |o + t d| = 1 for sphere hits (first root), z = -3 plane for rect hits
+    b = np.sum(org * d, 1)
+    c = np.sum(org * org, 1) - 1.0
+    disc = b * b - c
+    hit_s = disc >= 0
+    t_exp = -b - np.sqrt(np.maximum(disc, 0))
+    assert np.all(prim[hit_s] == 0)
+    assert np.allclose(t[hit_s], t_exp[hit_s], rtol=1e-5)
+    r = ~hit_s & (prim == 1)
+    assert np.allclose(t[r], (5.0 + 3.0) / -d[r, 2], rtol=1e-5)
+
+
+def test_quad_light_closed_forms():
+    # AARectangle<Y> at y=2, x in [-0.5,0.5], z in [-0.25,0.25]: QuadLight normal is -Y
+    spec = _scene_with(rects=[((1, 2.0, -0.5, 0.5, -0.25, 0.25), True)])
+    o = O.OracleScene(spec)
+    rng = np.random.default_rng(6)
+    n = 1000
+    u2 = rng.random((n, 2), np.float32)
+    ref = np.repeat(np.array([[0.1, 0.0, 0.2]], np.float32), n, 0)
+    p, nrm, pdf, le = o.light_sample(0, u2, ref)
+    assert np.allclose(p[:, 0], -0.5 + u2[:, 0], atol=1e-6)
+    assert np.allclose(p[:, 2], -0.25 + 0.5 * u2[:, 1], atol=1e-6)
+    assert np.allclose(nrm, [0, -1, 0])
+    d2 = np.sum((p - ref) ** 2, 1)
+    cos = np.abs(p[:, 1] - ref[:, 1]) / np.sqrt(d2)
+    assert np.allclose(pdf, d2 / (0.5 * cos), rtol=1e-5)   # area 0.5
+    assert np.allclose(le, [7.0, 6.3, 5.6], rtol=1e-6)      # lit from below (one-sided)
+
+
+def test_sphere_light_closed_forms():
+    spec = _scene_with(spheres=[((0, 3, 0), 0.5, True)])
+    o = O.OracleScene(spec)
+    rng = np.random.default_rng(7)
+    n = 200_000
+    u2 = rng.random((n, 2), np.float32)
+    ref = np.zeros((n, 3), np.float32)
+    p, nrm, pdf, le = o.light_sample(0, u2, ref)
+    assert np.allclose(np.linalg.norm(p - [0, 3, 0], axis=1), 0.5, atol=1e-5)
+    # uniform over the whole sphere: the mean point is the centre
+    assert np.allclose(p.mean(0), [0, 3, 0], atol=5e-3)
+    cone = 1 / (2 * np.pi * (1 - np.sqrt(1 - 0.25 / 9)))
+    assert np.allclose(pdf, cone, rtol=1e-5)                 # SphereLight::pdf_value: cone pdf
+    facing = np.sum(nrm * (ref - p), 1) > 0
+    assert np.all(np.any(le[facing] > 0, 1)) and np.all(le[~facing] == 0)
+
+
+def test_env_cdf_pdf_normalised_and_matches_sampler():
+    env = scenes.sky_hdri(64, 32, seed=2, sun_peak=50.0, sun_width_deg=8.0)
+    o = O.OracleScene(_scene_with(spheres=[((0, 0, 0), 0.1, False)], env=env))
+    nt, nph = 400, 800
+    th = (np.arange(nt) + 0.5) / nt * np.pi
+    ph = (np.arange(nph) + 0.5) / nph * 2 * np.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    d = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3).astype(np.float32)
+    dw = (np.sin(T) * (np.pi / nt) * (2 * np.pi / nph)).reshape(-1)
+    pdf = o.env_pdf(d[::5])
+    assert abs(np.sum(pdf * dw[::5]) * 5 - 1.0) < 1e-2
+    rng = np.random.default_rng(8)
+    dirs, ps = o.env_sample(rng.random((100_000, 2), np.float32))
+    assert np.allclose(np.linalg.norm(dirs, axis=1), 1, atol=1e-5)
+    assert np.allclose(o.env_pdf(dirs[:500]), ps[:500], rtol=1e-3)
+    # E[1/pdf] = 4*pi
+    assert abs(np.mean(1 / ps) - 4 * np.pi) < 0.03 * 4 * np.pi
+
+
+def test_pinhole_camera_rays():
+    spec = scenes.cornell(width=16, height=16)
+    o = O.OracleScene(spec)
+    cfg = o.cfg()
+    pix = np.arange(256, dtype=np.uint32)
+    rays = o.camera_rays(cfg, pix, np.zeros(256, np.uint32))
+    assert np.allclose(rays[:, :3], [0, 1.0, 3.9], atol=1e-6)         # zero aperture: all from the pinhole
+    assert np.allclose(np.linalg.norm(rays[:, 3:6], axis=1), 1, atol=1e-6)
+    # the image plane spans the vertical field of view (40 degrees)
+    vy = rays[:, 4] / -rays[:, 5]
+    assert abs(vy.max() - np.tan(np.radians(20)) * (15.9 / 16 * 2 - 1)) < 0.05
+
+
+def test_thin_lens_focus():
+    # points on the focal plane are sharp: every lens sample of one pixel meets there
+    spec = scenes.glass_hdri(width=8, height=8, spp_x=4, spp_y=4, subdiv=1, env_w=16, env_h=8)
+    spec.camera = dict(fov_deg=30, aspect=1, focal_dist=4.0, aperture=4.0 / 0.5, up=(0, 1, 0), front=(0, 0, -1),
+                       pos=(0, 0, 4.0))
+    o = O.OracleScene(spec)
+    cfg = o.cfg()
+    k = np.arange(16, dtype=np.uint32)
+    rays = o.camera_rays(cfg, np.full(16, 27, np.uint32), k)
+    # lens radius focal/aperture/2 = 0.25: origins spread on the lens square
+    assert np.ptp(rays[:, 0]) > 0.05 and np.all(np.abs(rays[:, :2]) <= 0.25 + 1e-6)
+    t = 4.0 / -rays[:, 5]
+    hit = rays[:, :3] + t[:, None] * rays[:, 3:6]
+    # k differs, so the pixel jitter differs; hits stay within one pixel footprint of each other
+    pix = 2 * 4.0 * np.tan(np.radians(15)) / 8
+    assert np.ptp(hit[:, 0]) <= pix * 1.01 and np.ptp(hit[:, 1]) <= pix * 1.01
