@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (layout constants of pathtracer_amd/csrc/pt_scene.h)
 B_EXT_RAY = 4 + 24 + 8        # queue index + ray (o, d) read + hit (t, prim) write
 B_SHADOW_RAY = 44             # shadow-queue entry (o, d, tmax, contribution, slot)
-B_NODE = 64                   # BVH2 node: both child boxes + child links
+B_NODE = 112                  # BVH4 node: four child boxes (SoA) + four child codes
 B_PRIM = 48                   # triangle record (v0, e1, e2)
 
 

@@ -397,7 +397,7 @@ static int kk_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, floa
   return 1;
 }
 
-/* Disney and hair live in or_bsdf_disney.c */
+/* Disney and hair: same translation unit, so they share the helpers above */
 int or_disney_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, float uc,
                       float u0, float u1, or_srec *s);
 v3 or_disney_f(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h);
@@ -406,6 +406,7 @@ int or_hair_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, float 
                     float u0, float u1, or_srec *s);
 v3 or_hair_f(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h);
 float or_hair_pdf(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h);
+#include "or_bsdf_disney.c"
 
 /* ---------------- dispatch (Material virtuals, material.hpp:15-77) -------- */
 int or_is_emitter(const pt_material_desc *m) { return m->kind == PT_MAT_EMITTER; }

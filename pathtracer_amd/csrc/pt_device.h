@@ -66,7 +66,7 @@ PT_D bool hit_tri(float4 A, float4 B, float4 C, f3 o, f3 d, float tmin, float tm
   f3 P = cross3(d, e2);
   float det = dot3(e1, P);
   if (fabsf(det) < 1e-8f) return false;
-  float inv = 1.0f / det;
+  float inv = rcp_rn(det);
   f3 T = sub3(o, p0);
   float u = dot3(T, P) * inv;
   if (u < 0.f || u > 1.f) return false;
@@ -79,7 +79,7 @@ PT_D bool hit_tri(float4 A, float4 B, float4 C, f3 o, f3 d, float tmin, float tm
   return true;
 }
 PT_D bool hit_sphere(float4 A, float4 B, f3 o, f3 d, float tmin, float tmax, float &t) {
-  f3 dn = norm3(d);
+  f3 dn = norm3_rn(d);
   f3 oc = sub3(o, mk3(A.x, A.y, A.z));
   float r = B.x;
   float a = dot3(dn, dn);
@@ -87,20 +87,20 @@ PT_D bool hit_sphere(float4 A, float4 B, f3 o, f3 d, float tmin, float tmax, flo
   float c = dot3(oc, oc) - r * r;
   float disc = b * b - 4.f * a * c;
   if (disc < 0.f) return false;
-  float delta = sqrtf(disc);
-  float tt = (-b - delta) / (2.f * a);
+  float delta = sqrt_rn(disc);
+  float tt = div_rn(-b - delta, 2.f * a);
   if (tt < tmin || tt > tmax) {
-    tt = (-b + delta) / (2.f * a);
+    tt = div_rn(-b + delta, 2.f * a);
     if (tt < tmin || tt > tmax) return false;
   }
   t = tt;
   return true;
 }
 PT_D bool hit_rect(float4 A, float4 B, f3 o, f3 d, float tmin, float tmax, float &t) {
-  f3 dn = norm3(d);
+  f3 dn = norm3_rn(d);
   int ax = (int)B.z;
   int b0 = (ax == 0) ? 1 : 0, b1 = (ax == 2) ? 1 : 2;
-  float tt = (A.x - comp3(o, ax)) / comp3(dn, ax);
+  float tt = div_rn(A.x - comp3(o, ax), comp3(dn, ax));
   if (tt < tmin || tt > tmax) return false;
   float d0 = comp3(o, b0) + tt * comp3(dn, b0);
   float d1 = comp3(o, b1) + tt * comp3(dn, b1);
@@ -125,46 +125,73 @@ PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, floa
 #pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------
-// BVH2 traversal, ordered (near child first), stack in registers/scratch.
-// ANY: stop at the first primitive hit in [tmin, tmax] (shadow rays).
+// BVH4 traversal.  One 128-byte node per step: the four child boxes are
+// tested together, the nearest hit child is visited next and the others are
+// pushed far-to-near with their entry distance, so a pop whose box starts
+// beyond the current closest hit is skipped.  ANY: stop at the first
+// primitive hit in [tmin, tmax] (shadow rays); no ordering needed.
 // ---------------------------------------------------------------------
-#define PT_STACK 64
+#define PT_CSWAP(ta, ca, tb_, cb)          \
+  if (tb_ < ta) {                         \
+    float _t = ta; ta = tb_; tb_ = _t;    \
+    int _c = ca; ca = cb; cb = _c;        \
+  }
 template <bool ANY, bool COUNT>
 PT_D bool traverse(const DevScene &s, f3 o, f3 d, float tmin, float tmax, float &t_out, int &prim_out,
                    uint32_t &n_nodes, uint32_t &n_prims) {
-  const float ix = 1.f / d.x, iy = 1.f / d.y, iz = 1.f / d.z;
+  const float ix = rcp_fast(d.x), iy = rcp_fast(d.y), iz = rcp_fast(d.z);
   const float oix = o.x * ix, oiy = o.y * iy, oiz = o.z * iz;
   float tb = tmax;
   int best = -1;
-  int stack[PT_STACK];
+  int stack_c[PT_STACK4];
+  float stack_t[PT_STACK4];
   int sp = 0;
   int cur = s.root;
   while (true) {
     if (cur >= 0) {
-      const float4 *nd = s.nodes + 4 * cur;
-      float4 a = nd[0], b = nd[1], c = nd[2];
-      int4 ch = *reinterpret_cast<const int4 *>(nd + 3);
+      const float4 *nd = s.nodes + 8 * (size_t)cur;
+      const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+      const int4 ch = *reinterpret_cast<const int4 *>(nd + 6);
       if (COUNT) ++n_nodes;
-      float l0x = fmaf(a.x, ix, -oix), h0x = fmaf(a.y, ix, -oix);
-      float l0y = fmaf(a.z, iy, -oiy), h0y = fmaf(a.w, iy, -oiy);
-      float l0z = fmaf(c.x, iz, -oiz), h0z = fmaf(c.y, iz, -oiz);
-      float l1x = fmaf(b.x, ix, -oix), h1x = fmaf(b.y, ix, -oix);
-      float l1y = fmaf(b.z, iy, -oiy), h1y = fmaf(b.w, iy, -oiy);
-      float l1z = fmaf(c.z, iz, -oiz), h1z = fmaf(c.w, iz, -oiz);
-      float n0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), tmin));
-      float f0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tb));
-      float n1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), tmin));
-      float f1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tb));
-      bool hit0 = n0 <= f0, hit1 = n1 <= f1;
-      if (hit0 && hit1) {
-        int nearc = ch.x, farc = ch.y;
-        if (n1 < n0) { nearc = ch.y; farc = ch.x; }
-        stack[sp++] = farc;
-        cur = nearc;
+      float tn[4], tf[4];
+      const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float a0 = fmaf(lxs[k], ix, -oix), a1 = fmaf(hxs[k], ix, -oix);
+        float b0 = fmaf(lys[k], iy, -oiy), b1 = fmaf(hys[k], iy, -oiy);
+        float c0 = fmaf(lzs[k], iz, -oiz), c1 = fmaf(hzs[k], iz, -oiz);
+        tn[k] = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
+        tf[k] = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tb));
+      }
+      const int cs[4] = {ch.x, ch.y, ch.z, ch.w};
+      float t0 = INFINITY, t1 = INFINITY, t2 = INFINITY, t3 = INFINITY;
+      int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      int nh = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        bool h = tn[k] <= tf[k] && cs[k] != kEmptyChild;
+        float tk = h ? tn[k] : INFINITY;
+        if (k == 0) { t0 = tk; c0 = cs[0]; }
+        if (k == 1) { t1 = tk; c1 = cs[1]; }
+        if (k == 2) { t2 = tk; c2 = cs[2]; }
+        if (k == 3) { t3 = tk; c3 = cs[3]; }
+        nh += h ? 1 : 0;
+      }
+      if (nh > 0) {
+        // sorting network on the entry distance; misses (inf) sink to the end
+        PT_CSWAP(t0, c0, t1, c1);
+        PT_CSWAP(t2, c2, t3, c3);
+        PT_CSWAP(t0, c0, t2, c2);
+        PT_CSWAP(t1, c1, t3, c3);
+        PT_CSWAP(t1, c1, t2, c2);
+        if (nh >= 4) { stack_c[sp] = c3; stack_t[sp] = t3; ++sp; }
+        if (nh >= 3) { stack_c[sp] = c2; stack_t[sp] = t2; ++sp; }
+        if (nh >= 2) { stack_c[sp] = c1; stack_t[sp] = t1; ++sp; }
+        cur = c0;
         continue;
       }
-      if (hit0) { cur = ch.x; continue; }
-      if (hit1) { cur = ch.y; continue; }
     } else {
       uint32_t st = leaf_start(cur), cn = leaf_count(cur);
       for (uint32_t p = st; p < st + cn; ++p) {
@@ -182,13 +209,23 @@ PT_D bool traverse(const DevScene &s, f3 o, f3 d, float tmin, float tmax, float 
         }
       }
     }
-    if (sp == 0) break;
-    cur = stack[--sp];
+    // pop the next subtree whose box starts no farther than the closest hit
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (ANY || stack_t[sp] <= tb) {
+        cur = stack_c[sp];
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
   }
   t_out = tb;
   prim_out = best;
   return best >= 0;
 }
+#undef PT_CSWAP
 
 // ---------------------------------------------------------------------
 // Shading geometry.  HitRecord::set_normal (materialBase.hpp:48-52) and
@@ -351,7 +388,7 @@ PT_D void camera_ray(const DevCamera &c, float u, float v, float r0, float r1, f
   bl = sub3(bl, scl3(c.lookat, c.focal_dist));
   f3 ps = add3(pos, add3(scl3(c.h, lu), scl3(c.v, lv)));
   f3 dir = sub3(add3(add3(bl, scl3(c.horiz, u)), scl3(c.vert, v)), ps);
-  if (c.normalize_dir) dir = norm3(dir);
+  if (c.normalize_dir) dir = norm3_rn(dir);
   o = ps;
   d = dir;
 }

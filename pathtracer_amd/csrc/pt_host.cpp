@@ -407,6 +407,82 @@ struct Builder {
 
 }  // namespace
 
+// ------------------------------------------------------------------ BVH4 collapse
+// Each 4-wide node takes the two children of a binary node and repeatedly
+// opens its largest-area internal child until it has four children.  Node
+// layout (8 x float4 = 128 bytes, one cache line): lo.x, hi.x, lo.y, hi.y,
+// lo.z, hi.z of the four children (SoA), then the four child codes, then
+// padding.  Parents precede children, so the root is node 0.
+namespace {
+struct Child {
+  int code;
+  float lo[3], hi[3];
+};
+float child_area(const Child &c) {
+  float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+void bvh2_children(const std::vector<float4> &n2, int node, Child out[2]) {
+  const float4 *nd = &n2[4 * (size_t)node];
+  int4 ch;
+  std::memcpy(&ch, &nd[3], sizeof(int4));
+  out[0] = Child{ch.x, {nd[0].x, nd[0].z, nd[2].x}, {nd[0].y, nd[0].w, nd[2].y}};
+  out[1] = Child{ch.y, {nd[1].x, nd[1].z, nd[2].z}, {nd[1].y, nd[1].w, nd[2].w}};
+}
+int collapse_node(const std::vector<float4> &n2, int node2, std::vector<float4> &n4, uint32_t depth,
+                  uint32_t &max_depth) {
+  if (depth > max_depth) max_depth = depth;
+  Child ch[4];
+  Child two[2];
+  bvh2_children(n2, node2, two);
+  int n = 2;
+  ch[0] = two[0];
+  ch[1] = two[1];
+  while (n < 4) {
+    int best = -1;
+    float ba = -1.f;
+    for (int i = 0; i < n; ++i)
+      if (ch[i].code >= 0 && child_area(ch[i]) > ba) {
+        ba = child_area(ch[i]);
+        best = i;
+      }
+    if (best < 0) break;
+    bvh2_children(n2, ch[best].code, two);
+    ch[best] = two[0];
+    ch[n++] = two[1];
+  }
+  int idx = (int)(n4.size() / 8);
+  n4.resize(n4.size() + 8);
+  int codes[4];
+  for (int i = 0; i < 4; ++i) {
+    if (i >= n) codes[i] = kEmptyChild;
+    else if (ch[i].code >= 0) codes[i] = collapse_node(n2, ch[i].code, n4, depth + 1, max_depth);
+    else codes[i] = ch[i].code;
+  }
+  float4 *nd = &n4[8 * (size_t)idx];
+  float v[6][4];
+  for (int i = 0; i < 4; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      // empty slots get an inverted box; traversal also checks the code
+      v[2 * a][i] = i < n ? ch[i].lo[a] : 1.f;
+      v[2 * a + 1][i] = i < n ? ch[i].hi[a] : -1.f;
+    }
+  }
+  for (int r = 0; r < 6; ++r) nd[r] = make_float4(v[r][0], v[r][1], v[r][2], v[r][3]);
+  int4 cc = make_int4(codes[0], codes[1], codes[2], codes[3]);
+  std::memcpy(&nd[6], &cc, sizeof(int4));
+  nd[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  return idx;
+}
+}  // namespace
+
+static void collapse_bvh4(const std::vector<float4> &n2, std::vector<float4> &n4, uint32_t &max_depth) {
+  n4.clear();
+  n4.reserve(n2.size());
+  max_depth = 0;
+  collapse_node(n2, 0, n4, 1, max_depth);
+}
+
 // ------------------------------------------------------------------ murmur / digit permutations
 // gl::pbrt::murmurHash64A (utils/hash.hpp:11-56), for DigitPermutation tables.
 static uint64_t murmur64a(const unsigned char *key, size_t len, uint64_t seed) {
@@ -599,6 +675,13 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   }
   B.nodes.resize(4 * (size_t)nn);
 
+  // -- collapse the binary tree into 4-wide nodes
+  std::vector<float4> nodes4;
+  uint32_t depth4 = 0;
+  collapse_bvh4(B.nodes, nodes4, depth4);
+  if (3 * depth4 + 3 > PT_STACK4)
+    return fail(PT_ERR_UNSUPPORTED, "pt_scene_commit: BVH too deep for the traversal stack");
+
   // -- primitives in leaf order
   std::vector<float4> prims(3 * nprim);
   for (size_t k = 0; k < nprim; ++k) {
@@ -671,7 +754,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   }
 
   // -- upload
-  HIP_TRY(upload(s->d_nodes, B.nodes.data(), B.nodes.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_nodes, nodes4.data(), nodes4.size() * sizeof(float4)));
   HIP_TRY(upload(s->d_prims, prims.data(), prims.size() * sizeof(float4)));
   HIP_TRY(upload(s->d_info, info.data(), info.size() * sizeof(uint2)));
   HIP_TRY(upload(s->d_geom, geom.data(), geom.size() * sizeof(float4)));
@@ -715,8 +798,8 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   in.n_spheres = nsph;
   in.n_rects = (uint32_t)s->rects.size();
   in.n_lights = (uint32_t)s->lights.size();
-  in.bvh_nodes = nn;
-  in.bvh_depth = B.max_depth;
+  in.bvh_nodes = (uint32_t)(nodes4.size() / 8);
+  in.bvh_depth = depth4;
   in.device_bytes = s->d_nodes.bytes + s->d_prims.bytes + s->d_info.bytes + s->d_geom.bytes + s->d_env.bytes +
                     s->d_env_func.bytes + s->d_env_cdf.bytes;
   in.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -825,7 +908,9 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
     st = ensure_perm(s, cfg->seed);
     if (st != PT_OK) return st;
   }
-  uint32_t want = cfg->wavefront ? cfg->wavefront : (1u << 21);
+  // 8M path slots in flight by default (~2.2 GB of wavefront state): enough rays
+  // in the late, sparse bounces to keep 256 CUs busy.
+  uint32_t want = cfg->wavefront ? cfg->wavefront : (1u << 23);
   uint64_t chunk_pix = std::max<uint64_t>(1, want / sppc);
   chunk_pix = std::min(chunk_pix, npix);
   uint32_t cap = (uint32_t)(chunk_pix * sppc);
@@ -879,7 +964,7 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   DevCamera cam = make_camera(s->cam);
   const bool count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
   const bool timing = (cfg->flags & PT_RENDER_TIMING) != 0;
-  const uint32_t grid_cap = s->n_cu * 16;
+  const uint32_t grid_cap = s->n_cu * 32;  // 8 waves per CU of 256-thread blocks, x4
   const uint32_t nverts = cfg->estimator == PT_EST_NEE_MIS ? cfg->max_depth + 1 : cfg->max_depth;
 
   struct Ev { hipEvent_t a, b; int kind; };

@@ -28,8 +28,8 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s, DevCamera cam, Ren
   uint32_t k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
   uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
   uint32_t g = 1u + k + rp.spp * (pj + rp.height * pi);
-  float u = ((float)pi + offx[k]) / (float)rp.width;
-  float v = ((float)pj + offy[k]) / (float)rp.height;
+  float u = div_rn((float)pi + offx[k], (float)rp.width);
+  float v = div_rn((float)pj + offy[k], (float)rp.height);
   f3 o, d;
   camera_ray(cam, u, v, pt_rng(g, rslot(0, SLOT_LENS_X), rp.seed), pt_rng(g, rslot(0, SLOT_LENS_Y), rp.seed),
              pt_rng(g, rslot(0, SLOT_TIME), rp.seed), o, d);
@@ -220,8 +220,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave
               float inv_ns = 1.f / (float)rp.light_samples;
               for (uint32_t si = 0; si < rp.light_samples && si < 4u; ++si) {
                 uint32_t ii = si / rp.ly, jj = si % rp.ly;
-                float a0 = (float)ii / (float)rp.lx, a1 = (float)(ii + 1u) / (float)rp.lx;
-                float b0 = (float)jj / (float)rp.ly, b1 = (float)(jj + 1u) / (float)rp.ly;
+                float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
+                float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
                 float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
                 float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
                 int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;

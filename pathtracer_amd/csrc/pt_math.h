@@ -19,6 +19,22 @@ struct f3 {
   float x, y, z;
 };
 
+// Division and square root whose results must be bit-identical to IEEE
+// arithmetic (sampler values, camera ray set-up, primitive hit decisions).
+// The kernels are compiled with correctly rounded division and square root,
+// so these are the plain operators; they mark the places that rely on it.
+PT_HD float div_rn(float a, float b) { return a / b; }
+PT_HD float rcp_rn(float a) { return 1.0f / a; }
+PT_HD float sqrt_rn(float a) { return sqrtf(a); }
+// Hardware reciprocal (about 1 ulp): for the BVH slab test only.
+PT_HD float rcp_fast(float a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(a);
+#else
+  return 1.0f / a;
+#endif
+}
+
 PT_HD f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
 PT_HD f3 s3(float s) { return mk3(s, s, s); }
 PT_HD f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -30,6 +46,10 @@ PT_HD f3 neg3(f3 a) { return mk3(-a.x, -a.y, -a.z); }
 PT_HD float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 PT_HD float len3(f3 a) { return sqrtf(dot3(a, a)); }
 PT_HD f3 norm3(f3 a) { float l = len3(a); return mk3(a.x / l, a.y / l, a.z / l); }
+PT_HD f3 norm3_rn(f3 a) {
+  float l = sqrt_rn(dot3(a, a));
+  return mk3(div_rn(a.x, l), div_rn(a.y, l), div_rn(a.z, l));
+}
 PT_HD f3 cross3(f3 a, f3 b) {
   return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }

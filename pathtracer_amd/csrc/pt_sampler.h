@@ -182,9 +182,9 @@ PT_HD float stratified(uint32_t pixel_seed, uint32_t k, uint32_t spp_x, uint32_t
   uint32_t key_dim = (r == 2u) ? dim - 1u : dim;
   uint32_t s = permutation_element(k % spp, spp, pt_hash32(pixel_seed ^ pt_hash32(key_dim)));
   float j = pt_rng(pixel_seed ^ 0x2c1b3c6du, k, dim);
-  if (r == 0u) return ((float)s + j) / (float)spp;
-  if (r == 1u) return ((float)(s % spp_x) + j) / (float)spp_x;
-  return ((float)(s / spp_x) + j) / (float)spp_y;
+  if (r == 0u) return div_rn((float)s + j, (float)spp);
+  if (r == 1u) return div_rn((float)(s % spp_x) + j, (float)spp_x);
+  return div_rn((float)(s / spp_x) + j, (float)spp_y);
 }
 
 }  // namespace pt

@@ -25,7 +25,10 @@ namespace pt {
 enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
 enum { LIGHT_QUAD = 1, LIGHT_SPHERE = 2, LIGHT_ENV = 3 };
 
-// A child reference: >= 0 internal node index; < 0 leaf ~(start<<4 | count-1).
+// A child reference: >= 0 internal node index; < 0 leaf ~(start<<4 | count-1);
+// kEmptyChild marks an unused slot of a 4-wide node.
+constexpr int kEmptyChild = (int)0x80000000;
+#define PT_STACK4 96
 PT_HD int leaf_code(uint32_t start, uint32_t count) { return ~(int)((start << 4) | (count - 1)); }
 PT_HD uint32_t leaf_start(int c) { return ((uint32_t)~c) >> 4; }
 PT_HD uint32_t leaf_count(int c) { return (((uint32_t)~c) & 15u) + 1u; }

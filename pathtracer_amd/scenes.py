@@ -248,6 +248,44 @@ def glass_hdri(width=1024, height=1024, spp_x=8, spp_y=8, max_depth=16, subdiv=8
     return s
 
 
+EUMELANIN_SIGMA_A = (0.419, 0.697, 1.37)  # absorption per unit eumelanin concentration
+
+
+def disney_hair_grid(width=1024, height=1024, spp_x=8, spp_y=16, max_depth=12, seed=3,
+                     sampler=L.SAMPLER_STRATIFIED, estimator=L.EST_REFERENCE_NEE, rr_depth=3) -> SceneSpec:
+    """Material-divergence workload: an 8x8 grid of analytic spheres with Disney
+    principled BSDFs whose parameters are drawn uniform in [0,1], a ninth row of
+    8 Marschner-hair spheres (sigma_a from eumelanin in [0,1.5], longitudinal
+    roughness in [0.1,0.6], azimuthal roughness 0.3, IOR 1.55), one 4x4 quad
+    light of radiance 8.  72 materials, 72 spheres, 1 light."""
+    rng = np.random.default_rng(seed)
+    s = SceneSpec(name="disney_hair_grid")
+    r = 0.42
+    for row in range(8):
+        for col in range(8):
+            base = rng.uniform(0, 1, 3)
+            metallic, roughness, specular, spec_tint, sheen, clearcoat, gloss, subsurface, aniso, spec_trans = \
+                rng.uniform(0, 1, 10)
+            m = s.add_material(L.MAT_DISNEY, [*base, spec_trans, metallic, subsurface, specular, roughness,
+                                              spec_tint, aniso, sheen, 0.5, clearcoat, gloss, 1.5])
+            s.add_sphere((col - 3.5, row + 0.5, 0.0), r, m)
+    for col in range(8):
+        mel = rng.uniform(0, 1.5)
+        beta_m = rng.uniform(0.1, 0.6)
+        m = s.add_material(L.MAT_HAIR, [mel * EUMELANIN_SIGMA_A[0], mel * EUMELANIN_SIGMA_A[1],
+                                        mel * EUMELANIN_SIGMA_A[2], 0.0, 1.55, beta_m, 0.3, math.radians(2.0)])
+        s.add_sphere((col - 3.5, 8.5, 0.0), r, m)
+    light = s.add_material(L.MAT_EMITTER, [1.0, 1.0, 1.0, 8.0])
+    s.add_rect(1, 11.0, -2.0, 2.0, 2.0, 6.0, light)          # 4x4, facing down, in front of the grid
+    floor = s.add_material(L.MAT_LAMBERTIAN, [0.5, 0.5, 0.5])
+    s.add_rect(1, 0.0, -30, 30, -30, 30, floor)
+    s.camera = dict(fov_deg=45.0, aspect=width / height, focal_dist=1.0, aperture=0.0, up=(0, 1, 0),
+                    front=(0, 0, -1), pos=(0, 4.5, 12.0))
+    s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=max_depth, sampler=sampler,
+                    estimator=estimator, rr_depth=rr_depth, light_samples=4, seed=17, bg=(0.02, 0.02, 0.03))
+    return s
+
+
 def furnace(albedo=0.5, env_value=1.0, max_depth=64, width=8, height=8, spp_x=8, spp_y=8) -> SceneSpec:
     """Lambertian sphere of albedo a under a uniform environment of radiance E.
     A convex object sees only the environment, so every camera ray that hits

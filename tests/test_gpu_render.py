@@ -57,6 +57,15 @@ def test_glass_hdri_per_sample(gpu):
         assert _sample_agreement(gs, os_) > 0.99
 
 
+def test_disney_hair_grid_per_sample(gpu):
+    from pathtracer_amd import scenes
+    spec = scenes.disney_hair_grid(width=48, height=48, spp_x=2, spp_y=2, max_depth=6)
+    for est in (0, 1):
+        _, gs, _, os_, st = _per_sample(spec, estimator=est, rr_depth=0)
+        assert _sample_agreement(gs, os_) > 0.99
+        assert st["nan_samples"] == 0
+
+
 @pytest.mark.parametrize("estimator,expect", [(0, 1.0), (1, 0.5)])
 def test_white_furnace(gpu, estimator, expect):
     from pathtracer_amd import render as R, scenes
