@@ -3,17 +3,20 @@
 Workload (one step = one frame): the triangle-heavy single frame -- a
 noise-displaced icosphere of 1,310,720 triangles in rough glass (IOR 1.5,
 alpha 0.05) on a 2-triangle copper plane (alpha 0.3), lit by a 1024x512
-synthetic sky+sun environment; 1024x1024 pixels, 64 samples per pixel
-(67,108,864 path samples), Sobol sampler, max depth 16, thin-lens depth of
-field with lens diameter 0.02, the reference's NEE estimator (4 light samples
-per non-specular vertex) with Russian roulette from vertex 3.
+synthetic sky+sun environment sampled through its 2D piecewise-constant CDF;
+1024x1024 pixels, 64 samples per pixel (67,108,864 path samples), Sobol
+sampler, max depth 16, thin-lens depth of field with lens diameter 0.02,
+NEE+MIS with Russian roulette from vertex 3.  The reference's own estimator
+(4 uniform light samples per non-specular vertex, no MIS) is timed on the
+same frame as a second line item.
 
 Prints one JSON line.  `value` is path samples per second over K timed frames
 (scene resident in HBM).  `roofline` is for the kernel with the largest share
 of device time, from algorithmic bytes (HIP-event-timed launches, counters of
-an identical counting frame).  `cpu_baseline` times the CPU oracle (the C
-restatement of the reference's estimator, OpenMP) on the host cores over a
-bounded slice of the same frame.
+an identical counting frame).  `quality` is the relMSE of a 64-spp render
+against a 4096-spp render of the same camera at 128x128.  `cpu_baseline`
+times the CPU oracle (the C restatement of the reference's estimators,
+OpenMP) on the host cores over a bounded slice of the same frame.
 """
 from __future__ import annotations
 
@@ -33,6 +36,13 @@ B_EXT_RAY = 4 + 24 + 8        # queue index + ray (o, d) read + hit (t, prim) wr
 B_SHADOW_RAY = 44             # shadow-queue entry (o, d, tmax, contribution, slot)
 B_NODE = 112                  # BVH4 node: four child boxes (SoA) + four child codes
 B_PRIM = 48                   # triangle record (v0, e1, e2)
+# shading, per extension segment: queue index 4, ray 24, hit 8, throughput /
+# radiance / flags 28, primitive info + geometry 24, material 32, MIS vertex 16
+# read; radiance, throughput, ray, flags, MIS vertex, queue index 72 written
+B_SHADE_SEG = 136 + 72
+# shading, per shadow ray written: the queue entry
+B_SHADE_SHADOW = B_SHADOW_RAY
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_glass_hdri.txt")
 
 
 def log(msg):
@@ -67,12 +77,49 @@ def cpu_baseline(spec, cfg, target_s=15.0):
                       f"{t:.1f} s on {threads} OpenMP threads; oracle BVH/scene build {build_s:.1f} s untimed"}
 
 
+def read_traffic(kernel, estimator):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary committed beside the profiles (None when absent or for another
+    kernel or estimator)."""
+    try:
+        kv = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = v.strip()
+        if kv.get("kernel") != kernel or int(kv.get("estimator", -1)) != estimator:
+            return None
+        return int(float(kv["hbm_bytes_per_launch"]))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def quality(scene, estimator, res=128, spp_side=8, ref_side=64):
+    """relMSE of a 64-spp frame against a 4096-spp frame (another seed) of the
+    same camera at res x res: mean over pixels and channels of
+    (x - r)^2 / (r^2 + 1e-2)."""
+    import numpy as np
+    t0 = time.perf_counter()
+    ref, _ = scene.render(scene.cfg(estimator=estimator, width=res, height=res, spp_x=ref_side, spp_y=ref_side,
+                                    seed=101))
+    t_ref = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    img, _ = scene.render(scene.cfg(estimator=estimator, width=res, height=res, spp_x=spp_side, spp_y=spp_side,
+                                    seed=7))
+    t_img = time.perf_counter() - t0
+    rel = float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)))
+    return {"relmse": rel, "spp": spp_side * spp_side, "ref_spp": ref_side * ref_side,
+            "resolution": f"{res}x{res}", "ms_render": round(t_img * 1e3, 1), "ms_ref": round(t_ref * 1e3, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--estimator", type=int, default=0, help="0: reference NEE, 1: NEE+MIS")
+    ap.add_argument("--estimator", type=int, default=1, help="0: reference NEE, 1: NEE+MIS (default)")
+    ap.add_argument("--no-quality", action="store_true", help="skip the relMSE measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--small", action="store_true", help="256x256 quick check (not a bench line)")
     ap.add_argument("--save", default="", help="write the film as PFM/PNG with this path prefix")
@@ -117,21 +164,40 @@ def main():
     dt = time.perf_counter() - t1
     value = total * args.steps / dt / 1e6
 
+    # the other estimator on the same frame (one warm-up, one timed frame)
+    other = 1 - args.estimator
+    ocfg = scene.cfg(estimator=other, flags=L.RENDER_TIMING)
+    scene.render_device(ocfg)
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    t2 = time.perf_counter()
+    scene.render_device(ocfg)
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    odt = time.perf_counter() - t2
+    other_res = {"estimator": ["reference_nee", "nee_mis"][other], "value": round(total / odt / 1e6, 2),
+                 "ms_per_step": round(odt * 1e3, 2)}
+    log(f"other estimator: {other_res}")
+
     # roofline of the dominant kernel (HIP-event times of the timed frames)
     ms = {k: sum(s[k] for s in stats) / args.steps for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate",
                                                               "ms_film")}
     launches = stats[0]["extend_launches"]
-    ext_bytes = cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE + cst["prim_tests"] * B_PRIM
-    sh_bytes = cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_NODE + \
-        cst["shadow_prim_tests"] * B_PRIM
-    if ms["ms_extend"] >= ms["ms_shadow"]:
-        kname, kbytes, kms = "k_extend (closest-hit traversal)", ext_bytes, ms["ms_extend"]
-    else:
-        kname, kbytes, kms = "k_shadow (any-hit traversal)", sh_bytes, ms["ms_shadow"]
+    kernels = {
+        "k_extend": ("closest-hit traversal", cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE +
+                     cst["prim_tests"] * B_PRIM, ms["ms_extend"]),
+        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_NODE +
+                     cst["shadow_prim_tests"] * B_PRIM, ms["ms_shadow"]),
+        "k_shade": ("materials, lights, NEE, queue compaction", cst["segments"] * B_SHADE_SEG +
+                    cst["shadow_rays"] * B_SHADE_SHADOW, ms["ms_shade"]),
+    }
+    per_kernel = {k: {"what": w, "gb_per_frame": round(b / 1e9, 3), "ms_per_frame": round(t, 2),
+                      "achieved_gbs": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else 0.0}
+                  for k, (w, b, t) in kernels.items()}
+    kname = max(kernels, key=lambda k: kernels[k][2])
+    _, kbytes, kms = kernels[kname]
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
-                "launches_per_frame": launches, "bytes_per_launch": int(kbytes / max(launches, 1)),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(kname, args.estimator),
+                "kernel": kname, "launches_per_frame": launches, "bytes_per_launch": int(kbytes / max(launches, 1)),
                 "ms_per_launch": round(kms / max(launches, 1), 4)}
 
     res = {
@@ -149,8 +215,14 @@ def main():
         "counters_per_frame": {k: cst[k] for k in ("segments", "shadow_rays", "node_visits", "prim_tests",
                                                     "shadow_node_visits", "shadow_prim_tests", "nan_samples")},
         "roofline": roofline,
+        "kernels": per_kernel,
+        "other_estimator": other_res,
+        "quality": None,
         "cpu_baseline": None,
     }
+    if not args.no_quality and not args.small:
+        res["quality"] = quality(scene, args.estimator)
+        log(f"quality: {res['quality']}")
     if args.save:
         from pathtracer_amd import film_io
         import numpy as np

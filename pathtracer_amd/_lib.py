@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpathtracer_amd.so")
+# PATHTRACER_AMD_LIB selects another build of the same library (e.g. one
+# compiled with different occupancy targets).
+LIB_PATH = os.environ.get("PATHTRACER_AMD_LIB") or os.path.join(_HERE, "libpathtracer_amd.so")
 
 # pt_mat_kind
 MAT_LAMBERTIAN, MAT_EMITTER, MAT_CONDUCTOR, MAT_MFDIELECTRIC = 1, 2, 3, 4

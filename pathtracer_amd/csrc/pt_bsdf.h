@@ -382,47 +382,56 @@ PT_D bool kk_scatter(const pt_material_desc &m, f3 wo, const ShadePoint &h, floa
 namespace pt {
 
 // ---- dispatch ----
+// MS is the set of material kinds compiled into the caller (bit k = kind k):
+// a kernel instantiated for a scene's kinds carries no register pressure from
+// the materials that scene does not use.  Kinds outside MS scatter nothing.
+constexpr uint32_t kMatAll = 0xffffffffu;
+constexpr uint32_t kMatBasic = (1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_EMITTER) | (1u << PT_MAT_CONDUCTOR) |
+                               (1u << PT_MAT_MFDIELECTRIC) | (1u << PT_MAT_THINDIELECTRIC) |
+                               (1u << PT_MAT_PHONG) | (1u << PT_MAT_KAJIYA_KAY);
+#define PT_IN(MS, K) (((MS) >> (K)) & 1u)
+
+template <uint32_t MS = kMatAll>
 PT_D bool bsdf_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float uc, float u0, float u1,
                        float aux, Scatter &s) {
   s.flags = 0;
   s.pdf = 0.f;
   s.att = s3(0.f);
   s.wi = s3(0.f);
-  switch (m.kind) {
-    case PT_MAT_LAMBERTIAN: return lambert_scatter(m, h, u0, u1, s);
-    case PT_MAT_CONDUCTOR: return conductor_scatter(m, wo, h, u0, u1, s);
-    case PT_MAT_MFDIELECTRIC: return mfdiel_scatter(m, wo, h, uc, u0, u1, s);
-    case PT_MAT_THINDIELECTRIC: return thin_scatter(m, wo, h, uc, s);
-    case PT_MAT_PHONG: return phong_scatter(m, wo, h, u0, u1, aux, s);
-    case PT_MAT_DISNEY: return disney_scatter(m, wo, h, uc, u0, u1, s);
-    case PT_MAT_HAIR: return hair_scatter(m, wo, h, uc, u0, u1, s);
-    case PT_MAT_KAJIYA_KAY: return kk_scatter(m, wo, h, u0, u1, s);
-    default: return false;
-  }
+  const uint32_t k = m.kind;
+  if (PT_IN(MS, PT_MAT_LAMBERTIAN) && k == PT_MAT_LAMBERTIAN) return lambert_scatter(m, h, u0, u1, s);
+  if (PT_IN(MS, PT_MAT_CONDUCTOR) && k == PT_MAT_CONDUCTOR) return conductor_scatter(m, wo, h, u0, u1, s);
+  if (PT_IN(MS, PT_MAT_MFDIELECTRIC) && k == PT_MAT_MFDIELECTRIC) return mfdiel_scatter(m, wo, h, uc, u0, u1, s);
+  if (PT_IN(MS, PT_MAT_THINDIELECTRIC) && k == PT_MAT_THINDIELECTRIC) return thin_scatter(m, wo, h, uc, s);
+  if (PT_IN(MS, PT_MAT_PHONG) && k == PT_MAT_PHONG) return phong_scatter(m, wo, h, u0, u1, aux, s);
+  if (PT_IN(MS, PT_MAT_DISNEY) && k == PT_MAT_DISNEY) return disney_scatter(m, wo, h, uc, u0, u1, s);
+  if (PT_IN(MS, PT_MAT_HAIR) && k == PT_MAT_HAIR) return hair_scatter(m, wo, h, uc, u0, u1, s);
+  if (PT_IN(MS, PT_MAT_KAJIYA_KAY) && k == PT_MAT_KAJIYA_KAY) return kk_scatter(m, wo, h, u0, u1, s);
+  return false;
 }
+template <uint32_t MS = kMatAll>
 PT_D f3 bsdf_f(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
-  switch (m.kind) {
-    case PT_MAT_LAMBERTIAN: return scl3(mp3(m, 0), 1.f / PT_PI);
-    case PT_MAT_CONDUCTOR: return conductor_f(m, wo, wi, h);
-    case PT_MAT_MFDIELECTRIC: return mfdiel_f(m, wo, wi, h);
-    case PT_MAT_PHONG: return phong_f(m, wo, wi, h);
-    case PT_MAT_DISNEY: return disney_f(m, wo, wi, h);
-    case PT_MAT_HAIR: return hair_f(m, wo, wi, h);
-    case PT_MAT_KAJIYA_KAY: return kk_f(m, wo, wi, h);
-    default: return s3(0.f);
-  }
+  const uint32_t k = m.kind;
+  if (PT_IN(MS, PT_MAT_LAMBERTIAN) && k == PT_MAT_LAMBERTIAN) return scl3(mp3(m, 0), 1.f / PT_PI);
+  if (PT_IN(MS, PT_MAT_CONDUCTOR) && k == PT_MAT_CONDUCTOR) return conductor_f(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_MFDIELECTRIC) && k == PT_MAT_MFDIELECTRIC) return mfdiel_f(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_PHONG) && k == PT_MAT_PHONG) return phong_f(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_DISNEY) && k == PT_MAT_DISNEY) return disney_f(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_HAIR) && k == PT_MAT_HAIR) return hair_f(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_KAJIYA_KAY) && k == PT_MAT_KAJIYA_KAY) return kk_f(m, wo, wi, h);
+  return s3(0.f);
 }
+template <uint32_t MS = kMatAll>
 PT_D float bsdf_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
-  switch (m.kind) {
-    case PT_MAT_LAMBERTIAN: return cosine_pdf(h.n, wi);
-    case PT_MAT_CONDUCTOR: return conductor_pdf(m, wo, wi, h);
-    case PT_MAT_MFDIELECTRIC: return mfdiel_pdf(m, wo, wi, h);
-    case PT_MAT_PHONG: return phong_pdf(m, wo, wi, h);
-    case PT_MAT_DISNEY: return disney_pdf(m, wo, wi, h);
-    case PT_MAT_HAIR: return hair_pdf(m, wo, wi, h);
-    case PT_MAT_KAJIYA_KAY: return cosine_pdf(h.n, wi);
-    default: return 0.f;
-  }
+  const uint32_t k = m.kind;
+  if (PT_IN(MS, PT_MAT_LAMBERTIAN) && k == PT_MAT_LAMBERTIAN) return cosine_pdf(h.n, wi);
+  if (PT_IN(MS, PT_MAT_CONDUCTOR) && k == PT_MAT_CONDUCTOR) return conductor_pdf(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_MFDIELECTRIC) && k == PT_MAT_MFDIELECTRIC) return mfdiel_pdf(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_PHONG) && k == PT_MAT_PHONG) return phong_pdf(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_DISNEY) && k == PT_MAT_DISNEY) return disney_pdf(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_HAIR) && k == PT_MAT_HAIR) return hair_pdf(m, wo, wi, h);
+  if (PT_IN(MS, PT_MAT_KAJIYA_KAY) && k == PT_MAT_KAJIYA_KAY) return cosine_pdf(h.n, wi);
+  return 0.f;
 }
 // DiffuseEmitter::emit (material.hpp:487-493)
 PT_D f3 bsdf_emit(const pt_material_desc &m, f3 ray_d, const ShadePoint &h) {

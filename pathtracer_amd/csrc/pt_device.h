@@ -125,27 +125,75 @@ PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, floa
 #pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------
+// Traversal stack.  Entries [0, lo) live in a private array; with K > 0 the
+// newest entries [lo, sp) (at most K) live in an LDS ring at slot i % K, laid
+// out slot-major ([K][PT_BLOCK]) so that the lanes of a wave always touch
+// distinct banks whatever their depths.  K = 0 keeps everything private.
+// The counters are plain locals: kept in a struct beside the arrays they
+// were placed in scratch memory too.
+// ---------------------------------------------------------------------
+#define PT_BLOCK 256
+// LDS words per block for a stack of K entries (codes, plus distances); at
+// least one word so that the declaration stays valid for K = 0.
+#define PT_TRAV_LDS(K, WITH_T) ((K) > 0 ? (K) * PT_BLOCK * ((WITH_T) ? 2 : 1) : 1)
+
+// ---------------------------------------------------------------------
 // BVH4 traversal.  One 128-byte node per step: the four child boxes are
 // tested together, the nearest hit child is visited next and the others are
 // pushed far-to-near with their entry distance, so a pop whose box starts
 // beyond the current closest hit is skipped.  ANY: stop at the first
 // primitive hit in [tmin, tmax] (shadow rays); no ordering needed.
+// `lds` is the calling block's PT_TRAV_LDS(K, !ANY) words of LDS.
 // ---------------------------------------------------------------------
 #define PT_CSWAP(ta, ca, tb_, cb)          \
   if (tb_ < ta) {                         \
     float _t = ta; ta = tb_; tb_ = _t;    \
     int _c = ca; ca = cb; cb = _c;        \
   }
-template <bool ANY, bool COUNT>
-PT_D bool traverse(const DevScene &s, f3 o, f3 d, float tmin, float tmax, float &t_out, int &prim_out,
+template <bool ANY, bool COUNT, int K>
+PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tmax, float &t_out, int &prim_out,
                    uint32_t &n_nodes, uint32_t &n_prims) {
   const float ix = rcp_fast(d.x), iy = rcp_fast(d.y), iz = rcp_fast(d.z);
   const float oix = o.x * ix, oiy = o.y * iy, oiz = o.z * iz;
   float tb = tmax;
   int best = -1;
-  int stack_c[PT_STACK4];
-  float stack_t[PT_STACK4];
-  int sp = 0;
+  constexpr bool WT = !ANY;
+  int pc[PT_STACK4];
+  float ptv[WT ? PT_STACK4 : 1];
+  int *const lc = lds + threadIdx.x;
+  float *const lt = reinterpret_cast<float *>(lds + K * PT_BLOCK) + threadIdx.x;
+  int sp = 0, lo = 0;
+  auto push = [&](int c, float t) {
+    if constexpr (K > 0) {
+      if (sp - lo == K) {  // ring full: the oldest entry goes to the private array
+        const int s0 = (lo & (K - 1)) * PT_BLOCK;
+        pc[lo] = lc[s0];
+        if (WT) ptv[lo] = lt[s0];
+        ++lo;
+      }
+      const int s1 = (sp & (K - 1)) * PT_BLOCK;
+      lc[s1] = c;
+      if (WT) lt[s1] = t;
+    } else {
+      pc[sp] = c;
+      if (WT) ptv[sp] = t;
+    }
+    ++sp;
+  };
+  auto pop = [&](int &c, float &t) {
+    --sp;
+    if constexpr (K > 0) {
+      if (sp >= lo) {
+        const int s1 = (sp & (K - 1)) * PT_BLOCK;
+        c = lc[s1];
+        t = WT ? lt[s1] : 0.f;
+        return;
+      }
+      lo = sp;
+    }
+    c = pc[sp];
+    t = WT ? ptv[sp] : 0.f;
+  };
   int cur = s.root;
   while (true) {
     if (cur >= 0) {
@@ -166,18 +214,19 @@ PT_D bool traverse(const DevScene &s, f3 o, f3 d, float tmin, float tmax, float 
         tf[k] = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tb));
       }
       const int cs[4] = {ch.x, ch.y, ch.z, ch.w};
-      float t0 = INFINITY, t1 = INFINITY, t2 = INFINITY, t3 = INFINITY;
-      int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      float t0, t1, t2, t3;
+      int c0 = cs[0], c1 = cs[1], c2 = cs[2], c3 = cs[3];
       int nh = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        bool h = tn[k] <= tf[k] && cs[k] != kEmptyChild;
-        float tk = h ? tn[k] : INFINITY;
-        if (k == 0) { t0 = tk; c0 = cs[0]; }
-        if (k == 1) { t1 = tk; c1 = cs[1]; }
-        if (k == 2) { t2 = tk; c2 = cs[2]; }
-        if (k == 3) { t3 = tk; c3 = cs[3]; }
-        nh += h ? 1 : 0;
+      {
+        bool h0 = tn[0] <= tf[0] && cs[0] != kEmptyChild;
+        bool h1 = tn[1] <= tf[1] && cs[1] != kEmptyChild;
+        bool h2 = tn[2] <= tf[2] && cs[2] != kEmptyChild;
+        bool h3 = tn[3] <= tf[3] && cs[3] != kEmptyChild;
+        t0 = h0 ? tn[0] : INFINITY;
+        t1 = h1 ? tn[1] : INFINITY;
+        t2 = h2 ? tn[2] : INFINITY;
+        t3 = h3 ? tn[3] : INFINITY;
+        nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
       }
       if (nh > 0) {
         // sorting network on the entry distance; misses (inf) sink to the end
@@ -186,9 +235,9 @@ PT_D bool traverse(const DevScene &s, f3 o, f3 d, float tmin, float tmax, float 
         PT_CSWAP(t0, c0, t2, c2);
         PT_CSWAP(t1, c1, t3, c3);
         PT_CSWAP(t1, c1, t2, c2);
-        if (nh >= 4) { stack_c[sp] = c3; stack_t[sp] = t3; ++sp; }
-        if (nh >= 3) { stack_c[sp] = c2; stack_t[sp] = t2; ++sp; }
-        if (nh >= 2) { stack_c[sp] = c1; stack_t[sp] = t1; ++sp; }
+        if (nh >= 4) push(c3, t3);
+        if (nh >= 3) push(c2, t2);
+        if (nh >= 2) push(c1, t1);
         cur = c0;
         continue;
       }
@@ -212,9 +261,11 @@ PT_D bool traverse(const DevScene &s, f3 o, f3 d, float tmin, float tmax, float 
     // pop the next subtree whose box starts no farther than the closest hit
     bool found = false;
     while (sp > 0) {
-      --sp;
-      if (ANY || stack_t[sp] <= tb) {
-        cur = stack_c[sp];
+      int c;
+      float te;
+      pop(c, te);
+      if (ANY || te <= tb) {
+        cur = c;
         found = true;
         break;
       }

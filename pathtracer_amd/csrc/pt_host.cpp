@@ -777,6 +777,8 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.mats = s->d_mats.as<pt_material_desc>();
   ds.lights = s->d_lights.as<DevLight>();
   ds.n_lights = (int)s->lights.size();
+  ds.mat_kinds = 0u;
+  for (const pt_material_desc &m : s->mats) ds.mat_kinds |= 1u << m.kind;
   ds.root = root;
   ds.has_env = !s->env.empty();
   ds.env = ds.has_env ? s->d_env.as<float>() : nullptr;

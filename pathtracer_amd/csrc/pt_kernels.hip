@@ -55,8 +55,24 @@ __global__ void k_clear_u32(uint32_t *p, uint32_t idx) {
 }
 
 // ------------------------------------------------------------------ extend
+// Traversal-stack entries kept in LDS per lane (the rest in private memory):
+// 8 (code, distance) pairs for closest hit, 16 codes for any hit -- 64 bytes
+// per lane, 16 KB per block, which leaves occupancy to the register budget.
+// Measured on the 1.3M-triangle glass scene against the private stack alone
+// (K = 0): closest hit 9% and any hit 4% faster.
+#ifndef PT_K_EXT
+#define PT_K_EXT 8
+#endif
+#ifndef PT_K_SHADOW
+#define PT_K_SHADOW 16
+#endif
+
+#ifndef PT_TRAV_WAVES
+#define PT_TRAV_WAVES 8
+#endif
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_extend(DevScene s, Wave w, int qi) {
+__global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_extend(DevScene s, Wave w, int qi) {
+  __shared__ int lds[PT_TRAV_LDS(PT_K_EXT, true)];
   const uint32_t n = w.counts[qi];
   const uint32_t *queue = w.q[qi];
   uint32_t nn = 0, np = 0, segs = 0;
@@ -68,7 +84,7 @@ __global__ void __launch_bounds__(256) k_extend(DevScene s, Wave w, int qi) {
       f3 d = mk3(w.dx[slot], w.dy[slot], w.dz[slot]);
       float t;
       int prim;
-      traverse<false, COUNT>(s, o, d, 1e-4f, 1e5f, t, prim, nn, np);
+      traverse<false, COUNT, PT_K_EXT>(s, lds, o, d, 1e-4f, 1e5f, t, prim, nn, np);
       w.t[slot] = t;
       w.prim[slot] = prim;
       ++segs;
@@ -83,7 +99,8 @@ __global__ void __launch_bounds__(256) k_extend(DevScene s, Wave w, int qi) {
 
 // ------------------------------------------------------------------ shadow
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_shadow(DevScene s, Wave w) {
+__global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_shadow(DevScene s, Wave w) {
+  __shared__ int lds[PT_TRAV_LDS(PT_K_SHADOW, false)];
   const uint32_t n = w.counts[CNT_SHADOW];
   uint32_t nn = 0, np = 0, rays = 0;
   for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
@@ -93,7 +110,7 @@ __global__ void __launch_bounds__(256) k_shadow(DevScene s, Wave w) {
       f3 d = mk3(w.sdx[i], w.sdy[i], w.sdz[i]);
       float t;
       int prim;
-      bool occ = traverse<true, COUNT>(s, o, d, 0.001f, w.stmax[i], t, prim, nn, np);
+      bool occ = traverse<true, COUNT, PT_K_SHADOW>(s, lds, o, d, 0.001f, w.stmax[i], t, prim, nn, np);
       ++rays;
       if (!occ) {
         uint32_t slot = w.sslot[i];
@@ -111,11 +128,18 @@ __global__ void __launch_bounds__(256) k_shadow(DevScene s, Wave w) {
 }
 
 // ------------------------------------------------------------------ shade
-struct ShadowRec {
-  f3 d, c;
-  float tmax;
-  bool valid;
-};
+// Appends one shadow ray.  Called from divergent code: the compiler turns the
+// uniform-address atomic into one atomic per wave (lane-ordered offsets).
+PT_D void emit_shadow(const Wave &w, f3 p, f3 d, float tmax, f3 c, uint32_t slot) {
+  uint32_t idx = atomicAdd(&w.counts[CNT_SHADOW], 1u);
+  if (idx < w.shadow_cap) {
+    w.sox[idx] = p.x; w.soy[idx] = p.y; w.soz[idx] = p.z;
+    w.sdx[idx] = d.x; w.sdy[idx] = d.y; w.sdz[idx] = d.z;
+    w.stmax[idx] = tmax;
+    w.scr[idx] = c.x; w.scg[idx] = c.y; w.scb[idx] = c.z;
+    w.sslot[idx] = slot;
+  }
+}
 
 PT_D f3 miss_radiance(const DevScene &s, const RenderParams &rp, f3 d) {
   if (s.has_env) return env_lookup(s, d);
@@ -124,8 +148,13 @@ PT_D f3 miss_radiance(const DevScene &s, const RenderParams &rp, f3 d) {
 PT_D float power_w(float a, float b) { return (a * a) / (a * a + b * b); }
 
 // EST 0: getRayColor (maxdepth_nee.hpp:9-256).  EST 1: NEE + MIS.
-template <int EST>
-__global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave w, int qi, uint32_t d) {
+// Occupancy target of the shading kernel (waves per SIMD): the register
+// allocator spills the rarely used material state to reach it.
+#ifndef PT_SHADE_WAVES
+#define PT_SHADE_WAVES 4
+#endif
+template <int EST, uint32_t MS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHADE_WAVES))) k_shade(DevScene s, RenderParams rp, Wave w, int qi, uint32_t d) {
   const uint32_t n = w.counts[qi];
   const uint32_t *qin = w.q[qi];
   uint32_t *qout = w.q[qi ^ 1];
@@ -138,9 +167,6 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave
     bool cont = false;
     uint32_t slot = 0;
     f3 p = s3(0.f), wi = s3(0.f);
-    ShadowRec sh[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sh[k].valid = false;
     if (active) {
       slot = qin[i];
       uint64_t q = rp.chunk_first + slot;
@@ -203,7 +229,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave
         float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
         f3 wo = neg3(norm3(dir));
         Scatter sc;
-        if (!bsdf_scatter(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
+        if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
           L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
           done = true;
         } else if (sc.flags & BX_SPECULAR) {
@@ -228,20 +254,21 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave
                 const DevLight &Lg = s.lights[li];
                 f3 nl;
                 f3 lp = light_point(Lg, ox, oy, nl);
-                f3 c = s3(0.f);
+                f3 c = s3(0.f), lwi = s3(0.f);
+                float tmax = 0.f;
+                bool valid = false;
                 if (Lg.type == LIGHT_ENV) {
-                  f3 lwi = norm3(sub3(lp, p));
+                  lwi = norm3(sub3(lp, p));
                   float pdf = (1.f / (float)N) * (1.f / (4.f * PT_PI));
                   float fac = fmaxf(0.f, dot3(h.sp.n, lwi));
-                  f3 f = bsdf_f(*mat, wo, lwi, h.sp);
+                  f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
                   c = divs3(scl3(mul3(f, env_lookup(s, lwi)), fac), pdf);
-                  sh[si].d = lwi;
-                  sh[si].tmax = 3.402823466e+38f;
-                  sh[si].valid = true;
+                  tmax = 3.402823466e+38f;
+                  valid = true;
                 } else {
                   f3 ld = sub3(lp, p);
                   float len = len3(ld);
-                  f3 lwi = divs3(ld, len);
+                  lwi = divs3(ld, len);
                   if (len >= 0.001f) {
                     f3 le = (dot3(nl, neg3(lwi)) > 0.f) ? Lg.le : s3(0.f);
                     float pdf = (Lg.area <= 1e-7f) ? 0.f : 1.f / (Lg.area * (float)N);
@@ -250,17 +277,18 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave
                     float d2 = len * len;
                     float fac = (d2 < 1e-7f) ? 0.f : (nol * noi) / d2;
                     if (pdf > 1e-7f) {
-                      f3 f = bsdf_f(*mat, wo, lwi, h.sp);
+                      f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
                       c = divs3(scl3(mul3(f, le), fac), pdf);
-                      sh[si].d = lwi;
-                      sh[si].tmax = len - 0.001f;
-                      sh[si].valid = true;
+                      tmax = len - 0.001f;
+                      valid = true;
                     }
                   }
                 }
-                sh[si].c = scl3(mul3(beta, c), inv_ns);
-                if (!(sh[si].c.x > 0.f || sh[si].c.y > 0.f || sh[si].c.z > 0.f) && finite3(sh[si].c))
-                  sh[si].valid = false;
+                c = scl3(mul3(beta, c), inv_ns);
+                // zero contributions are dropped; non-finite ones are traced so the film's
+                // NaN guard sees them
+                if (valid && ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)))
+                  emit_shadow(w, p, lwi, tmax, c, slot);
               }
             }
           } else if (N > 0) {
@@ -289,13 +317,11 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave
             }
             if (pl > 0.f) {
               pl /= (float)N;
-              f3 f = bsdf_f(*mat, wo, lwi, h.sp);
-              float pb = bsdf_pdf(*mat, wo, lwi, h.sp);
+              f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+              float pb = bsdf_pdf<MS>(*mat, wo, lwi, h.sp);
               float cl = fabsf(dot3(h.sp.n, lwi));
-              sh[0].c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
-              sh[0].d = lwi;
-              sh[0].tmax = tmax;
-              sh[0].valid = (sh[0].c.x > 0.f || sh[0].c.y > 0.f || sh[0].c.z > 0.f) || !finite3(sh[0].c);
+              f3 c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
+              if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)) emit_shadow(w, p, lwi, tmax, c, slot);
             }
           }
           beta = scl3(mul3(beta, sc.att), cs / sc.pdf);
@@ -323,18 +349,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderParams rp, Wave
         w.flags[slot] = fl;
       }
     }
-    // converged: append shadow rays and the continuation
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      uint32_t idx = wave_append(&w.counts[CNT_SHADOW], sh[k].valid);
-      if (sh[k].valid && idx < w.shadow_cap) {
-        w.sox[idx] = p.x; w.soy[idx] = p.y; w.soz[idx] = p.z;
-        w.sdx[idx] = sh[k].d.x; w.sdy[idx] = sh[k].d.y; w.sdz[idx] = sh[k].d.z;
-        w.stmax[idx] = sh[k].tmax;
-        w.scr[idx] = sh[k].c.x; w.scg[idx] = sh[k].c.y; w.scb[idx] = sh[k].c.z;
-        w.sslot[idx] = slot;
-      }
-    }
+    // converged: append the continuation
     uint32_t idx = wave_append(cout, cont);
     if (cont) qout[idx] = slot;
   }
@@ -377,7 +392,11 @@ __global__ void k_sample(DevScene s, uint32_t sampler, uint32_t seed, uint32_t s
   out[i] = sample_dim(s, sampler, seed, sx, sy, px, (uint32_t)index[i], (uint32_t)index[i], dim[i]);
 }
 
-__global__ void k_intersect(DevScene s, const float *rays, size_t n, int any, int32_t *prim, float *tout) {
+__global__ void __launch_bounds__(PT_BLOCK) k_intersect(DevScene s, const float *rays, size_t n, int any,
+                                                        int32_t *prim, float *tout) {
+  __shared__ int lds[PT_TRAV_LDS(PT_K_EXT, true) > PT_TRAV_LDS(PT_K_SHADOW, false)
+                         ? PT_TRAV_LDS(PT_K_EXT, true)
+                         : PT_TRAV_LDS(PT_K_SHADOW, false)];
   size_t i = gtid();
   if (i >= n) return;
   const float *r = rays + 8 * i;
@@ -386,9 +405,9 @@ __global__ void k_intersect(DevScene s, const float *rays, size_t n, int any, in
   uint32_t a = 0, b = 0;
   bool hit;
   if (any)
-    hit = traverse<true, false>(s, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
+    hit = traverse<true, false, PT_K_SHADOW>(s, lds, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
   else
-    hit = traverse<false, false>(s, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
+    hit = traverse<false, false, PT_K_EXT>(s, lds, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
   prim[i] = hit ? (any ? 1 : pr) : -1;
   tout[i] = hit ? (any ? 0.f : t) : __int_as_float(0x7f800000);
 }
@@ -473,26 +492,35 @@ void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uin
   uint32_t g = std::min(blocks_for(max_n), grid_cap);
   if (g == 0) g = 1;
   if (count)
-    hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(256), 0, st, s, w, qi);
+    hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi);
   else
-    hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(256), 0, st, s, w, qi);
+    hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi);
 }
 void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
                   uint32_t max_n, uint32_t grid_cap) {
   uint32_t g = std::min(blocks_for(max_n), grid_cap);
   if (g == 0) g = 1;
-  if (rp.estimator == PT_EST_NEE_MIS)
-    hipLaunchKernelGGL(k_shade<1>, dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
-  else
-    hipLaunchKernelGGL(k_shade<0>, dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+  // the basic material set when the scene uses nothing else (fewer registers)
+  const bool basic = (s.mat_kinds & ~kMatBasic) == 0u;
+  if (rp.estimator == PT_EST_NEE_MIS) {
+    if (basic)
+      hipLaunchKernelGGL((k_shade<1, kMatBasic>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+    else
+      hipLaunchKernelGGL((k_shade<1, kMatAll>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+  } else {
+    if (basic)
+      hipLaunchKernelGGL((k_shade<0, kMatBasic>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+    else
+      hipLaunchKernelGGL((k_shade<0, kMatAll>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+  }
 }
 void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count) {
   uint32_t g = std::min(blocks_for(max_n), grid_cap);
   if (g == 0) g = 1;
   if (count)
-    hipLaunchKernelGGL(k_shadow<true>, dim3(g), dim3(256), 0, st, s, w);
+    hipLaunchKernelGGL(k_shadow<true>, dim3(g), dim3(PT_BLOCK), 0, st, s, w);
   else
-    hipLaunchKernelGGL(k_shadow<false>, dim3(g), dim3(256), 0, st, s, w);
+    hipLaunchKernelGGL(k_shadow<false>, dim3(g), dim3(PT_BLOCK), 0, st, s, w);
 }
 void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
                  unsigned long long *stats) {
@@ -506,7 +534,7 @@ void launch_sample(hipStream_t st, const DevScene &s, uint32_t sampler, uint32_t
 }
 void launch_intersect(hipStream_t st, const DevScene &s, const float *rays, size_t n, int any, int32_t *prim,
                       float *t) {
-  hipLaunchKernelGGL(k_intersect, dim3(blocks_for(n)), dim3(256), 0, st, s, rays, n, any, prim, t);
+  hipLaunchKernelGGL(k_intersect, dim3(blocks_for(n)), dim3(PT_BLOCK), 0, st, s, rays, n, any, prim, t);
 }
 void launch_bsdf_eval(hipStream_t st, const pt_material_desc &m, const float *wo, const float *wi, const float *nrm,
                       const uint8_t *inside, size_t n, float *f, float *pdf) {

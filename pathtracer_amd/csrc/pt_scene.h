@@ -50,6 +50,7 @@ struct DevScene {
   const pt_material_desc *mats;
   const DevLight *lights;
   int n_lights;
+  uint32_t mat_kinds;  // bit k set when some material has kind k
   int root;  // child code of the root (node 0 or a single leaf)
   // environment (texels: (y*w + x)*3)
   const float *env;
