@@ -64,10 +64,16 @@ typedef struct pt_material_desc {
 /* Primitive flags.  The reference's wrappers change HitRecord::is_inside:
  * FlipFace toggles it (objectTransform.hpp:21); Translate/Rotate/Scale re-run
  * set_normal on an already ray-opposed normal, which forces it true
- * (objectTransform.hpp:60,156; objectTransform.cpp:132,172,211). */
+ * (objectTransform.hpp:60,156; objectTransform.cpp:132,172,211).  Applied in
+ * this order: FORCE_INSIDE sets it true, then FLIP_FACE toggles it, so a
+ * FlipFace around a transformed object (both flags) yields false.
+ * NOT_LIGHT keeps an emissive rect/sphere out of light discovery, which in
+ * the reference only looks at top-level objects (lightDiscovery.hpp:10-60):
+ * emitters inside a Box or a nested list are hit but never sampled. */
 enum {
   PT_PRIM_FLIP_FACE = 1u << 0,
-  PT_PRIM_FORCE_INSIDE = 1u << 1
+  PT_PRIM_FORCE_INSIDE = 1u << 1,
+  PT_PRIM_NOT_LIGHT = 1u << 2
 };
 
 /* ------------------------------------------------------------------ */
@@ -160,11 +166,21 @@ void pt_scene_destroy(pt_scene *scene);
 pt_status pt_scene_add_material(pt_scene *scene, const pt_material_desc *desc,
                                 uint32_t *out_id);
 /* TriangleMesh(MeshData, material)  (primitives/triangleMesh.cpp:5-26).
- * pos: nverts*3 floats; idx: ntris*3 int32 (MeshData::indices).  No
- * per-vertex normals: shading uses the geometric normal, meshBVH.cpp:121.   */
+ * pos: nverts*3 floats; idx: ntris*3 int32 (MeshData::indices).  Without
+ * per-vertex normals shading uses the geometric normal, meshBVH.cpp:121.    */
 pt_status pt_scene_add_mesh(pt_scene *scene, const float *pos, uint32_t nverts,
                             const int32_t *idx, uint32_t ntris,
                             uint32_t material, uint32_t flags);
+/* TriangleMesh with per-corner shading normals (MeshData::normals and
+ * normalIndices, mesh_io/meshData.hpp:6-13): the hit interpolates
+ * (1-u-v) n0 + u n1 + v n2 with the hit's barycentrics, normalises it and
+ * turns it to the geometric normal's side (meshBVH.cpp:116-130).
+ * nrm: nnormals*3 floats; nidx: ntris*3 int32.                             */
+pt_status pt_scene_add_mesh_normals(pt_scene *scene, const float *pos, uint32_t nverts,
+                                    const int32_t *idx, uint32_t ntris,
+                                    const float *nrm, uint32_t nnormals,
+                                    const int32_t *nidx, uint32_t material,
+                                    uint32_t flags);
 /* Sphere(center, radius, material)  (base/primitive.hpp:81-203)           */
 pt_status pt_scene_add_sphere(pt_scene *scene, const float center[3],
                               float radius, uint32_t material, uint32_t flags);

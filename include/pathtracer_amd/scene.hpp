@@ -1,0 +1,799 @@
+// scene.hpp -- source-compatible scene API over the C ABI of pathtracer_amd.h.
+//
+// The reference renderer's scenes are C++ functions that build a SceneInfo
+// out of std::make_shared<Sphere>(...), Lambertian, Conductor, Translate,
+// Rotate<Axis::Y>, loadOBJMesh(...) and so on (e.g. scenes/scenes_hdri.hpp)
+// and call SceneInfo::render (render/renderManager.hpp:51-158).  This header
+// declares the same class names and constructor signatures.  The objects only
+// record parameters; SceneInfo::render lowers them onto pt_scene_* calls and
+// renders on the GPU.  A scene function written for the reference compiles
+// against this header unchanged when it uses the classes below.
+//
+// Lowering rules (the reference's run-time behaviour, resolved once at scene
+// build time):
+//   * Translate / Rotate<Axis> / Scale are flattened into world-space
+//     geometry.  Rotate<Axis::A>(obj, angle) maps an object point p to the
+//     world point R_A(angle)^T p (objectTransform.cpp:93-213 rotates the ray
+//     by R_A); Scale(obj, s) maps p to s*p (objectTransform.hpp:121-146);
+//     Translate(obj, o) to p + o (objectTransform.hpp:43-60).  A rect that a
+//     rotation leaves non-axis-aligned becomes two triangles.
+//   * HitRecord::is_inside: every Translate/Rotate/Scale re-runs set_normal
+//     on a ray-opposed normal, which forces it true; FlipFace toggles it
+//     (objectTransform.hpp:21).  The chain is folded from the innermost
+//     wrapper outwards into PT_PRIM_FORCE_INSIDE / PT_PRIM_FLIP_FACE.
+//   * Light discovery (lightDiscovery.hpp:5-81) only looks at top-level
+//     objects whose underlying shape is an AARectangle or a Sphere; emissive
+//     primitives anywhere else (inside a Box or a nested ObjectList) are
+//     lowered with PT_PRIM_NOT_LIGHT.  The reference builds the light of a
+//     wrapped top-level emitter from the untransformed shape; here the light
+//     follows the transformed geometry.
+//   * Materials are lowered once each, in order of first use.
+//
+// Supported: Lambertian and DiffuseEmitter (solid colours), Conductor,
+// MFDielectric, ThinDielectric, Phong, DisneyPrincipledBSDF, HairMarschner,
+// KajiyaKay; Sphere, AARectangle<Axis> (XY/XZ/YZRectangle), Box,
+// TriangleMesh (with MeshData normals), loadOBJMesh, ObjectList, FlipFace,
+// Translate, Rotate<Axis>, Scale; PerspectiveCamera; EnvironmentLight from
+// .hdr or .pfm files.  Image textures, media and curves are not.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <optional>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <variant>
+#include <vector>
+
+#include "../pathtracer_amd.h"
+#include "image_io.hpp"
+
+// ------------------------------------------------------------------ gl::vec
+namespace gl {
+
+struct vec2 {
+  float e[2] = {0.f, 0.f};
+  vec2() = default;
+  explicit vec2(float s) : e{s, s} {}
+  vec2(float a, float b) : e{a, b} {}
+  float u() const { return e[0]; }
+  float v() const { return e[1]; }
+  float x() const { return e[0]; }
+  float y() const { return e[1]; }
+  float &operator[](int i) { return e[i]; }
+  float operator[](int i) const { return e[i]; }
+};
+
+struct vec3 {
+  float e[3] = {0.f, 0.f, 0.f};
+  vec3() = default;
+  vec3(float s) : e{s, s, s} {}  // implicit, as gl::vec3(0.7f) and colour constants use it
+  vec3(float a, float b, float c) : e{a, b, c} {}
+  float x() const { return e[0]; }
+  float y() const { return e[1]; }
+  float z() const { return e[2]; }
+  float r() const { return e[0]; }
+  float g() const { return e[1]; }
+  float b() const { return e[2]; }
+  float &operator[](int i) { return e[i]; }
+  float operator[](int i) const { return e[i]; }
+  vec3 operator-() const { return vec3(-e[0], -e[1], -e[2]); }
+  vec3 &operator+=(const vec3 &o) { e[0] += o.e[0]; e[1] += o.e[1]; e[2] += o.e[2]; return *this; }
+  vec3 &operator*=(float s) { e[0] *= s; e[1] *= s; e[2] *= s; return *this; }
+  float length() const { return std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]); }
+  vec3 normalize() const { float l = length(); return vec3(e[0] / l, e[1] / l, e[2] / l); }
+};
+inline vec3 operator+(const vec3 &a, const vec3 &b) { return vec3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+inline vec3 operator-(const vec3 &a, const vec3 &b) { return vec3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline vec3 operator*(const vec3 &a, const vec3 &b) { return vec3(a[0] * b[0], a[1] * b[1], a[2] * b[2]); }
+inline vec3 operator*(const vec3 &a, float s) { return vec3(a[0] * s, a[1] * s, a[2] * s); }
+inline vec3 operator*(float s, const vec3 &a) { return a * s; }
+inline vec3 operator/(const vec3 &a, float s) { return vec3(a[0] / s, a[1] / s, a[2] / s); }
+inline float dot(const vec3 &a, const vec3 &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline vec3 cross(const vec3 &a, const vec3 &b) {
+  return vec3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+inline vec3 normalize(const vec3 &a) { return a.normalize(); }
+inline float to_radian(float deg) { return deg * 3.14159265358979323846f / 180.f; }
+
+}  // namespace gl
+
+enum class Axis { X = 0, Y = 1, Z = 2 };
+
+// ------------------------------------------------------------------ lowering
+namespace ptamd {
+
+// What a scene lowers to: the C ABI calls, in order.  SceneInfo::render uses
+// the ABI sink below; tests can record the calls instead.
+struct SceneSink {
+  virtual ~SceneSink() = default;
+  virtual uint32_t material(const pt_material_desc &d) = 0;
+  virtual void mesh(const std::vector<float> &pos, const std::vector<int32_t> &idx, const std::vector<float> *nrm,
+                    const std::vector<int32_t> *nidx, uint32_t mat, uint32_t flags) = 0;
+  virtual void sphere(const float c[3], float r, uint32_t mat, uint32_t flags) = 0;
+  virtual void rect(int axis, float k, float d0min, float d0max, float d1min, float d1max, uint32_t mat,
+                    uint32_t flags) = 0;
+};
+
+inline void check(pt_status st, const char *what) {
+  if (st != PT_OK) throw std::runtime_error(std::string(what) + ": " + pt_last_error());
+}
+
+struct AbiSink : SceneSink {
+  pt_scene *s;
+  explicit AbiSink(pt_scene *scene) : s(scene) {}
+  uint32_t material(const pt_material_desc &d) override {
+    uint32_t id = 0;
+    check(pt_scene_add_material(s, &d, &id), "pt_scene_add_material");
+    return id;
+  }
+  void mesh(const std::vector<float> &pos, const std::vector<int32_t> &idx, const std::vector<float> *nrm,
+            const std::vector<int32_t> *nidx, uint32_t mat, uint32_t flags) override {
+    const uint32_t nv = (uint32_t)(pos.size() / 3), nt = (uint32_t)(idx.size() / 3);
+    if (nrm)
+      check(pt_scene_add_mesh_normals(s, pos.data(), nv, idx.data(), nt, nrm->data(), (uint32_t)(nrm->size() / 3),
+                                      nidx->data(), mat, flags),
+            "pt_scene_add_mesh_normals");
+    else
+      check(pt_scene_add_mesh(s, pos.data(), nv, idx.data(), nt, mat, flags), "pt_scene_add_mesh");
+  }
+  void sphere(const float c[3], float r, uint32_t mat, uint32_t flags) override {
+    check(pt_scene_add_sphere(s, c, r, mat, flags), "pt_scene_add_sphere");
+  }
+  void rect(int axis, float k, float d0min, float d0max, float d1min, float d1max, uint32_t mat,
+            uint32_t flags) override {
+    check(pt_scene_add_rect(s, axis, k, d0min, d0max, d1min, d1max, mat, flags), "pt_scene_add_rect");
+  }
+};
+
+// Object-to-world map p -> M p + t (rotations and uniform scale only).
+struct Xform {
+  float m[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  float t[3] = {0, 0, 0};
+  float scale = 1.f;
+  bool identity_rotation = true;
+  gl::vec3 point(const gl::vec3 &p) const {
+    gl::vec3 r;
+    for (int i = 0; i < 3; ++i) r[i] = m[i][0] * p[0] + m[i][1] * p[1] + m[i][2] * p[2] + t[i];
+    return r;
+  }
+  gl::vec3 dir(const gl::vec3 &n) const {  // normals: rotation part only (uniform scale)
+    gl::vec3 r;
+    for (int i = 0; i < 3; ++i) r[i] = (m[i][0] * n[0] + m[i][1] * n[1] + m[i][2] * n[2]) / scale;
+    return r;
+  }
+  // this o inner: apply `inner` first
+  Xform then_outer_of(const Xform &inner) const {
+    Xform r;
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) r.m[i][j] = m[i][0] * inner.m[0][j] + m[i][1] * inner.m[1][j] + m[i][2] * inner.m[2][j];
+      r.t[i] = m[i][0] * inner.t[0] + m[i][1] * inner.t[1] + m[i][2] * inner.t[2] + t[i];
+    }
+    r.scale = scale * inner.scale;
+    r.identity_rotation = identity_rotation && inner.identity_rotation;
+    return r;
+  }
+};
+
+// Wrapper chain from the outermost object inwards: 'F' FlipFace, 'T' transform.
+struct Chain {
+  std::string w;
+  uint32_t flags() const {
+    std::optional<bool> forced;
+    bool flip = false;
+    for (auto it = w.rbegin(); it != w.rend(); ++it) {  // innermost first
+      if (*it == 'T') {
+        forced = true;
+        flip = false;
+      } else if (forced) {
+        forced = !*forced;
+      } else {
+        flip = !flip;
+      }
+    }
+    if (forced) return *forced ? (uint32_t)PT_PRIM_FORCE_INSIDE : (uint32_t)(PT_PRIM_FORCE_INSIDE | PT_PRIM_FLIP_FACE);
+    return flip ? (uint32_t)PT_PRIM_FLIP_FACE : 0u;
+  }
+};
+
+struct Lowering;
+}  // namespace ptamd
+
+// ------------------------------------------------------------------ textures
+class Texture2D {
+ public:
+  virtual ~Texture2D() = default;
+  virtual gl::vec3 constant_color() const {
+    throw std::runtime_error("only constant-colour textures are supported by the GPU materials");
+  }
+};
+class ConstantTexture : public Texture2D {
+ public:
+  explicit ConstantTexture(const gl::vec3 &c) : c(c) {}
+  gl::vec3 constant_color() const override { return c; }
+  gl::vec3 c;
+};
+using SolidColor = ConstantTexture;
+using ColorVariant = std::variant<gl::vec3, std::shared_ptr<Texture2D>>;
+
+namespace ptamd {
+inline gl::vec3 color_of(const ColorVariant &v) {
+  if (std::holds_alternative<gl::vec3>(v)) return std::get<gl::vec3>(v);
+  const auto &t = std::get<std::shared_ptr<Texture2D>>(v);
+  if (!t) throw std::runtime_error("null texture");
+  return t->constant_color();
+}
+}  // namespace ptamd
+
+// ------------------------------------------------------------------ materials
+class Material {
+ public:
+  virtual ~Material() = default;
+  virtual pt_material_desc desc() const = 0;
+  virtual bool is_emitter() const { return false; }
+
+ protected:
+  static pt_material_desc make(uint32_t kind, std::initializer_list<float> p) {
+    pt_material_desc d{};
+    d.kind = kind;
+    int i = 0;
+    for (float x : p) d.p[i++] = x;
+    return d;
+  }
+};
+
+// material.hpp:79-137
+class Lambertian : public Material {
+ public:
+  Lambertian(const ColorVariant &a) : albedo(ptamd::color_of(a)) {}
+  pt_material_desc desc() const override { return make(PT_MAT_LAMBERTIAN, {albedo[0], albedo[1], albedo[2]}); }
+  gl::vec3 albedo;
+};
+// material.hpp:467-506
+class DiffuseEmitter : public Material {
+ public:
+  DiffuseEmitter(const ColorVariant &a, float intensity = 1.0f) : color(ptamd::color_of(a)), intensity(intensity) {}
+  pt_material_desc desc() const override {
+    return make(PT_MAT_EMITTER, {color[0], color[1], color[2], intensity});
+  }
+  bool is_emitter() const override { return true; }
+  gl::vec3 color;
+  float intensity;
+};
+// conductor.hpp:4-145
+class Conductor : public Material {
+ public:
+  Conductor(const gl::vec3 &eta, const gl::vec3 &k, float alpha_x, float alpha_y)
+      : eta(eta), k(k), ax(alpha_x), ay(alpha_y) {}
+  pt_material_desc desc() const override {
+    return make(PT_MAT_CONDUCTOR, {eta[0], eta[1], eta[2], k[0], k[1], k[2], ax, ay});
+  }
+  gl::vec3 eta, k;
+  float ax, ay;
+};
+// dielectric.hpp:5-167
+class MFDielectric : public Material {
+ public:
+  MFDielectric(float eta, float alpha_x, float alpha_y) : eta(eta), ax(alpha_x), ay(alpha_y) {}
+  pt_material_desc desc() const override { return make(PT_MAT_MFDIELECTRIC, {eta, ax, ay}); }
+  float eta, ax, ay;
+};
+// thinDielectric.hpp:3-70
+class ThinDielectric : public Material {
+ public:
+  ThinDielectric(float eta, bool use_split_ray = false) : eta(eta), split(use_split_ray) {}
+  pt_material_desc desc() const override { return make(PT_MAT_THINDIELECTRIC, {eta, split ? 1.f : 0.f}); }
+  float eta;
+  bool split;
+};
+// material.hpp:303-465
+class Phong : public Material {
+ public:
+  Phong(const gl::vec3 &diffuse, const gl::vec3 &specular, const gl::vec3 &ambient, float shininess)
+      : diffuse(diffuse), specular(specular), ambient(ambient), shininess(shininess) {}
+  pt_material_desc desc() const override {
+    return make(PT_MAT_PHONG, {diffuse[0], diffuse[1], diffuse[2], specular[0], specular[1], specular[2], ambient[0],
+                               ambient[1], ambient[2], shininess});
+  }
+  gl::vec3 diffuse, specular, ambient;
+  float shininess;
+};
+// disneyPrincipledBSDF.hpp:34-63 (constructor argument order)
+class DisneyPrincipledBSDF : public Material {
+ public:
+  DisneyPrincipledBSDF(const gl::vec3 &baseColor, float specular_transmission, float metallic, float subsurface,
+                       float specular, float roughness, float specular_tint, float anisotropic, float sheen,
+                       float sheen_tint, float clearcoat, float clearcoat_gloss, float eta)
+      : p{baseColor[0], baseColor[1], baseColor[2], specular_transmission, metallic, subsurface, specular,
+          roughness, specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss, eta} {}
+  pt_material_desc desc() const override {
+    pt_material_desc d{};
+    d.kind = PT_MAT_DISNEY;
+    for (int i = 0; i < 15; ++i) d.p[i] = p[i];
+    return d;
+  }
+  float p[15];
+};
+// hairMarsch.hpp:7-9
+class HairMarschner : public Material {
+ public:
+  HairMarschner(const gl::vec3 &sigma_a, float h, float eta, float beta_m, float beta_n, float alpha)
+      : sigma_a(sigma_a), h(h), eta(eta), beta_m(beta_m), beta_n(beta_n), alpha(alpha) {}
+  pt_material_desc desc() const override {
+    return make(PT_MAT_HAIR, {sigma_a[0], sigma_a[1], sigma_a[2], h, eta, beta_m, beta_n, alpha});
+  }
+  gl::vec3 sigma_a;
+  float h, eta, beta_m, beta_n, alpha;
+};
+// No class in the reference (README.md:74 names the model): diffuse +
+// specular around the tangent, exponent p.
+class KajiyaKay : public Material {
+ public:
+  KajiyaKay(const gl::vec3 &diffuse, const gl::vec3 &specular, float exponent)
+      : diffuse(diffuse), specular(specular), exponent(exponent) {}
+  pt_material_desc desc() const override {
+    return make(PT_MAT_KAJIYA_KAY, {diffuse[0], diffuse[1], diffuse[2], specular[0], specular[1], specular[2],
+                                    exponent});
+  }
+  gl::vec3 diffuse, specular;
+  float exponent;
+};
+
+// ------------------------------------------------------------------ lowering state
+namespace ptamd {
+struct Lowering {
+  SceneSink &sink;
+  std::map<const Material *, uint32_t> ids;
+  explicit Lowering(SceneSink &s) : sink(s) {}
+  uint32_t mat(const std::shared_ptr<Material> &m) {
+    if (!m) throw std::runtime_error("object without a material");
+    auto it = ids.find(m.get());
+    if (it != ids.end()) return it->second;
+    uint32_t id = sink.material(m->desc());
+    ids[m.get()] = id;
+    return id;
+  }
+};
+}  // namespace ptamd
+
+// ------------------------------------------------------------------ geometry
+class Hittable {
+ public:
+  virtual ~Hittable() = default;
+  // Lowers this object with the object-to-world map X and wrapper chain C.
+  // `light_ok`: an emissive rect/sphere reached here may be discovered as a light.
+  virtual void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const = 0;
+  virtual std::shared_ptr<Material> get_material() const { return nullptr; }
+};
+
+namespace ptamd {
+inline uint32_t light_flags(const std::shared_ptr<Material> &m, bool light_ok) {
+  return (m && m->is_emitter() && !light_ok) ? (uint32_t)PT_PRIM_NOT_LIGHT : 0u;
+}
+}  // namespace ptamd
+
+// primitive.hpp:81-203
+class Sphere : public Hittable {
+ public:
+  Sphere(const gl::vec3 &center, float radius) : center(center), radius(radius) {}
+  Sphere(const gl::vec3 &center, float radius, std::shared_ptr<Material> material)
+      : center(center), radius(radius), material(std::move(material)) {}
+  void setMaterial(std::shared_ptr<Material> m) { material = std::move(m); }
+  std::shared_ptr<Material> get_material() const override { return material; }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const override {
+    if (!(X.scale > 0.f)) throw std::runtime_error("Scale must be positive");
+    gl::vec3 c = X.point(center);
+    float cc[3] = {c[0], c[1], c[2]};
+    L.sink.sphere(cc, radius * X.scale, L.mat(material), C.flags() | ptamd::light_flags(material, light_ok));
+  }
+  gl::vec3 center;
+  float radius;
+  std::shared_ptr<Material> material;
+};
+
+// primitive.hpp:205-273; d0/d1 are the two other axes in increasing order
+template <Axis axis>
+class AARectangle : public Hittable {
+ public:
+  AARectangle(float k, float d0_min, float d0_max, float d1_min, float d1_max)
+      : _d0_min(d0_min), _d0_max(d0_max), _d1_min(d1_min), _d1_max(d1_max), _k(k) {}
+  AARectangle(float k, float d0_min, float d0_max, float d1_min, float d1_max, std::shared_ptr<Material> material)
+      : _d0_min(d0_min), _d0_max(d0_max), _d1_min(d1_min), _d1_max(d1_max), _k(k), material(std::move(material)) {}
+  std::shared_ptr<Material> get_material() const override { return material; }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const override {
+    if (!(X.scale > 0.f)) throw std::runtime_error("Scale must be positive");
+    const int a = (int)axis, b0 = (a == 0) ? 1 : 0, b1 = (a == 2) ? 1 : 2;
+    const uint32_t mat = L.mat(material);
+    const uint32_t flags = C.flags() | ptamd::light_flags(material, light_ok);
+    if (X.identity_rotation) {
+      const float s = X.scale;
+      L.sink.rect(a, _k * s + X.t[a], _d0_min * s + X.t[b0], _d0_max * s + X.t[b0], _d1_min * s + X.t[b1],
+                  _d1_max * s + X.t[b1], mat, flags);
+      return;
+    }
+    // a rotated rect: two triangles over the same corners
+    std::vector<float> pos;
+    const float d0[2] = {_d0_min, _d0_max}, d1[2] = {_d1_min, _d1_max};
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        gl::vec3 p;
+        p[a] = _k;
+        p[b0] = d0[i];
+        p[b1] = d1[j];
+        gl::vec3 w = X.point(p);
+        pos.insert(pos.end(), {w[0], w[1], w[2]});
+      }
+    std::vector<int32_t> idx = {0, 1, 3, 0, 3, 2};
+    L.sink.mesh(pos, idx, nullptr, nullptr, mat, flags | (uint32_t)PT_PRIM_NOT_LIGHT);
+  }
+  float _d0_min, _d0_max, _d1_min, _d1_max, _k;
+  std::shared_ptr<Material> material;
+};
+using XYRectangle = AARectangle<Axis::Z>;
+using XZRectangle = AARectangle<Axis::Y>;
+using YZRectangle = AARectangle<Axis::X>;
+
+// mesh_io/meshData.hpp:6-13
+struct MeshData {
+  std::vector<gl::vec3> positions;
+  std::vector<gl::vec3> normals;
+  std::vector<gl::vec2> uvs;
+  std::vector<std::array<int, 3>> indices;
+  std::vector<std::array<int, 3>> normalIndices;
+  std::vector<std::array<int, 3>> uvIndices;
+};
+
+// primitives/triangleMesh.cpp:5-56.  Shading normals are used when every
+// face has three valid normal indices.
+class TriangleMesh : public Hittable {
+ public:
+  TriangleMesh(MeshData &&data, std::shared_ptr<Material> mat) : mesh(std::move(data)), material(std::move(mat)) {}
+  std::shared_ptr<Material> get_material() const override { return material; }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool) const override {
+    if (!(X.scale > 0.f)) throw std::runtime_error("Scale must be positive");
+    std::vector<float> pos;
+    pos.reserve(mesh.positions.size() * 3);
+    for (const auto &p : mesh.positions) {
+      gl::vec3 w = X.point(p);
+      pos.insert(pos.end(), {w[0], w[1], w[2]});
+    }
+    std::vector<int32_t> idx;
+    idx.reserve(mesh.indices.size() * 3);
+    for (const auto &f : mesh.indices) idx.insert(idx.end(), {f[0], f[1], f[2]});
+    bool smooth = !mesh.normals.empty() && mesh.normalIndices.size() == mesh.indices.size();
+    if (smooth)
+      for (const auto &f : mesh.normalIndices)
+        for (int c = 0; c < 3; ++c)
+          if (f[c] < 0 || (size_t)f[c] >= mesh.normals.size()) smooth = false;
+    const uint32_t mat = L.mat(material);
+    // meshes are never discovered as lights (lightDiscovery.hpp:62-66)
+    const uint32_t flags = C.flags() | (uint32_t)PT_PRIM_NOT_LIGHT;
+    if (!smooth) {
+      L.sink.mesh(pos, idx, nullptr, nullptr, mat, flags);
+      return;
+    }
+    std::vector<float> nrm;
+    for (const auto &n : mesh.normals) {
+      gl::vec3 w = X.dir(n);
+      nrm.insert(nrm.end(), {w[0], w[1], w[2]});
+    }
+    std::vector<int32_t> nidx;
+    for (const auto &f : mesh.normalIndices) nidx.insert(nidx.end(), {f[0], f[1], f[2]});
+    L.sink.mesh(pos, idx, &nrm, &nidx, mat, flags);
+  }
+  MeshData mesh;
+  std::shared_ptr<Material> material;
+};
+
+// base/objectList.hpp:9-105
+class ObjectList : public Hittable {
+ public:
+  ObjectList() = default;
+  void addObject(std::shared_ptr<Hittable> object) { objects.push_back(std::move(object)); }
+  void clear() { objects.clear(); }
+  std::vector<std::shared_ptr<Hittable>> getLists() const { return objects; }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool) const override {
+    for (const auto &o : objects) o->lower(L, X, C, false);  // nested lists hold no lights
+  }
+  std::vector<std::shared_ptr<Hittable>> objects;
+};
+
+// primitives/box.hpp:5-53: six rects with the box's material
+class Box : public Hittable {
+ public:
+  Box(gl::vec3 min_xyz, gl::vec3 max_xyz, std::shared_ptr<Material> material = nullptr)
+      : min_xyz(min_xyz), max_xyz(max_xyz), material(std::move(material)) {}
+  std::shared_ptr<Material> get_material() const override { return material; }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool) const override {
+    const float x0 = min_xyz[0], x1 = max_xyz[0], y0 = min_xyz[1], y1 = max_xyz[1], z0 = min_xyz[2],
+                z1 = max_xyz[2];
+    YZRectangle(x1, y0, y1, z0, z1, material).lower(L, X, C, false);
+    YZRectangle(x0, y0, y1, z0, z1, material).lower(L, X, C, false);
+    XZRectangle(y1, x0, x1, z0, z1, material).lower(L, X, C, false);
+    XZRectangle(y0, x0, x1, z0, z1, material).lower(L, X, C, false);
+    XYRectangle(z1, x0, x1, y0, y1, material).lower(L, X, C, false);
+    XYRectangle(z0, x0, x1, y0, y1, material).lower(L, X, C, false);
+  }
+  gl::vec3 min_xyz, max_xyz;
+  std::shared_ptr<Material> material;
+};
+
+// utils/objectTransform.hpp:5-181
+class FlipFace : public Hittable {
+ public:
+  FlipFace(std::shared_ptr<Hittable> object) : object(std::move(object)) {}
+  std::shared_ptr<Material> get_material() const override { return object->get_material(); }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const override {
+    ptamd::Chain c2{C.w + "F"};
+    object->lower(L, X, c2, light_ok);
+  }
+  std::shared_ptr<Hittable> object;
+};
+class Translate : public Hittable {
+ public:
+  Translate(std::shared_ptr<Hittable> object, const gl::vec3 &offset) : object(std::move(object)), offset(offset) {}
+  std::shared_ptr<Material> get_material() const override { return object->get_material(); }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const override {
+    ptamd::Xform inner;
+    for (int i = 0; i < 3; ++i) inner.t[i] = offset[i];
+    object->lower(L, X.then_outer_of(inner), ptamd::Chain{C.w + "T"}, light_ok);
+  }
+  std::shared_ptr<Hittable> object;
+  gl::vec3 offset;
+};
+template <Axis axis>
+class Rotate : public Hittable {
+ public:
+  Rotate(std::shared_ptr<Hittable> object, float angle)
+      : object(std::move(object)), angle(angle), sin_theta(std::sin(angle)), cos_theta(std::cos(angle)) {}
+  std::shared_ptr<Material> get_material() const override { return object->get_material(); }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const override {
+    // the reference rotates rays by R (objectTransform.cpp:93-213): for Y,
+    // x' = c x - s z, z' = s x + c z; geometry therefore maps by R^T
+    const float c = cos_theta, s = sin_theta;
+    int i = 0, j = 0;
+    if (axis == Axis::X) { i = 1; j = 2; }
+    if (axis == Axis::Y) { i = 0; j = 2; }
+    if (axis == Axis::Z) { i = 0; j = 1; }
+    ptamd::Xform inner;
+    inner.m[i][i] = c;
+    inner.m[i][j] = s;
+    inner.m[j][i] = -s;
+    inner.m[j][j] = c;
+    inner.identity_rotation = (s == 0.f && c == 1.f);
+    object->lower(L, X.then_outer_of(inner), ptamd::Chain{C.w + "T"}, light_ok);
+  }
+  std::shared_ptr<Hittable> object;
+  float angle, sin_theta, cos_theta;
+};
+class Scale : public Hittable {
+ public:
+  Scale(std::shared_ptr<Hittable> obj, float s) : object(std::move(obj)), scale(s) {}
+  std::shared_ptr<Material> get_material() const override { return object->get_material(); }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const override {
+    ptamd::Xform inner;
+    for (int i = 0; i < 3; ++i) inner.m[i][i] = scale;
+    inner.scale = scale;
+    object->lower(L, X.then_outer_of(inner), ptamd::Chain{C.w + "T"}, light_ok);
+  }
+  std::shared_ptr<Hittable> object;
+  float scale;
+};
+
+// mesh_io/meshLoader.cpp:7-58: positions, normals, uvs and triangular faces
+// (any other face size is an error) with per-corner indices.
+inline std::shared_ptr<TriangleMesh> loadOBJMesh(const std::string &path, std::shared_ptr<Material> mat) {
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  MeshData d;
+  std::string line;
+  auto fix = [](long i, size_t n) -> int {  // 1-based, negative = relative to the end
+    if (i > 0) return (int)(i - 1);
+    if (i < 0) return (int)((long)n + i);
+    return -1;
+  };
+  size_t lineno = 0;
+  while (std::getline(f, line)) {
+    ++lineno;
+    std::istringstream is(line);
+    std::string tag;
+    if (!(is >> tag) || tag[0] == '#') continue;
+    if (tag == "v") {
+      float x = 0, y = 0, z = 0;
+      is >> x >> y >> z;
+      d.positions.emplace_back(x, y, z);
+    } else if (tag == "vn") {
+      float x = 0, y = 0, z = 0;
+      is >> x >> y >> z;
+      d.normals.emplace_back(x, y, z);
+    } else if (tag == "vt") {
+      float u = 0, v = 0;
+      is >> u >> v;
+      d.uvs.emplace_back(u, v);
+    } else if (tag == "f") {
+      std::array<int, 3> vi{}, ti{}, ni{};
+      std::string tok;
+      int k = 0;
+      while (is >> tok) {
+        if (k == 3) throw std::runtime_error(path + ":" + std::to_string(lineno) + ": non-triangle face in OBJ");
+        long a = 0, b = 0, c = 0;
+        size_t s1 = tok.find('/');
+        a = std::stol(tok.substr(0, s1));
+        if (s1 != std::string::npos) {
+          size_t s2 = tok.find('/', s1 + 1);
+          std::string t = tok.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1);
+          if (!t.empty()) b = std::stol(t);
+          if (s2 != std::string::npos && s2 + 1 < tok.size()) c = std::stol(tok.substr(s2 + 1));
+        }
+        vi[k] = fix(a, d.positions.size());
+        ti[k] = fix(b, d.uvs.size());
+        ni[k] = fix(c, d.normals.size());
+        ++k;
+      }
+      if (k != 3) throw std::runtime_error(path + ":" + std::to_string(lineno) + ": non-triangle face in OBJ");
+      for (int c = 0; c < 3; ++c)
+        if (vi[c] < 0 || (size_t)vi[c] >= d.positions.size())
+          throw std::runtime_error(path + ":" + std::to_string(lineno) + ": vertex index out of range");
+      d.indices.push_back(vi);
+      d.uvIndices.push_back(ti);
+      d.normalIndices.push_back(ni);
+    }
+  }
+  return std::make_shared<TriangleMesh>(std::move(d), std::move(mat));
+}
+
+// ------------------------------------------------------------------ camera, environment
+// base/camera.hpp:16-35, camera.cpp:7-56.  fov in radians; `aperture` with
+// the reference's meaning (lens half-width focalDist/aperture/2).
+class PerspectiveCamera {
+ public:
+  PerspectiveCamera(float fov, float aspect, float focalDist, float aperture, gl::vec3 up, gl::vec3 front,
+                    gl::vec3 position)
+      : fov(fov), aspect(aspect), focalDist(focalDist), aperture(aperture), up(up), front(front), position(position),
+        position1(position) {}
+  pt_camera_desc desc() const {
+    pt_camera_desc c{};
+    c.fov = fov;
+    c.aspect = aspect;
+    c.focal_dist = focalDist;
+    c.aperture = aperture;
+    for (int i = 0; i < 3; ++i) {
+      c.up[i] = up[i];
+      c.front[i] = front[i];
+      c.pos[i] = position[i];
+      c.pos1[i] = position1[i];
+    }
+    c.shutter = shutter;
+    c.normalize_dir = normalize_directions ? 1u : 0u;
+    return c;
+  }
+  float fov, aspect, focalDist, aperture;
+  gl::vec3 up, front, position;
+  // beyond the reference: motion blur (position at shutter close) and the
+  // unit-length camera directions that keep t in one unit for every shape
+  gl::vec3 position1;
+  float shutter = 0.f;
+  bool normalize_directions = true;
+};
+
+// light/envLight.hpp:4-92 with HDRITexture (texture.hpp:97-132)
+class EnvironmentLight {
+ public:
+  EnvironmentLight(const std::string &hdri_filepath, float intensity_scale = 1.0f) : scale(intensity_scale) {
+    ptamd::image_io::Image img = ptamd::image_io::read_env(hdri_filepath);
+    width = img.width;
+    height = img.height;
+    rgb = std::move(img.rgb);
+  }
+  // an in-memory map in the pt_scene_set_env layout
+  EnvironmentLight(std::vector<float> texels, uint32_t w, uint32_t h, float intensity_scale = 1.0f)
+      : rgb(std::move(texels)), width(w), height(h), scale(intensity_scale) {}
+  std::vector<float> rgb;
+  uint32_t width = 0, height = 0;
+  float scale = 1.f;
+};
+
+// ------------------------------------------------------------------ SceneInfo
+// render/renderManager.hpp:35-158.  The render settings the reference takes
+// from config.hpp (MAX_RAY_DEPTH 10, 2x2 light samples, GAMMA 2.0, the
+// maxdepth_nee integrator, the Halton sampler) are fields here.
+struct SceneInfo {
+  std::shared_ptr<PerspectiveCamera> camera = nullptr;
+  ObjectList objects;
+  std::shared_ptr<EnvironmentLight> environment_light = nullptr;
+  gl::vec3 bg_color = gl::vec3(0.7f, 0.8f, 1.0f);
+  bool use_bvh = true;
+  unsigned _width = 800;
+  unsigned _height = 800;
+  unsigned spp_x = 2;
+  unsigned spp_y = 2;
+  float _gamma = 1.0f;
+  // config.hpp settings
+  unsigned max_depth = 10;
+  unsigned light_samples = 4;
+  float write_gamma = 2.0f;
+  uint32_t sampler = PT_SAMPLER_HALTON_OWEN;
+  uint32_t estimator = PT_EST_REFERENCE_NEE;
+  uint32_t rr_depth = 0;
+  uint32_t seed = 0;
+
+  SceneInfo() = default;
+
+  // Lowers the scene into `sink` (materials, then objects in order).
+  void lower(ptamd::SceneSink &sink) const {
+    ptamd::Lowering L(sink);
+    for (const auto &o : objects.objects) o->lower(L, ptamd::Xform{}, ptamd::Chain{}, true);
+  }
+
+  pt_render_cfg render_cfg() const {
+    pt_render_cfg c{};
+    c.width = _width;
+    c.height = _height;
+    c.spp_x = spp_x;
+    c.spp_y = spp_y;
+    c.max_depth = max_depth;
+    c.sampler = sampler;
+    c.estimator = estimator;
+    c.rr_depth = rr_depth;
+    c.light_samples = light_samples;
+    c.seed = seed;
+    for (int i = 0; i < 3; ++i) c.bg[i] = bg_color[i];
+    return c;
+  }
+
+  // Builds the device scene.  The caller owns the returned handle.
+  pt_scene *build() const {
+    if (!camera) throw std::runtime_error("Camera is not initialized!");  // renderManager.hpp:58-62
+    if (objects.objects.empty()) throw std::runtime_error("No objects in the scene!");
+    pt_scene *s = nullptr;
+    ptamd::check(pt_scene_create(&s), "pt_scene_create");
+    try {
+      ptamd::AbiSink sink(s);
+      lower(sink);
+      if (environment_light)
+        ptamd::check(pt_scene_set_env(s, environment_light->rgb.data(), environment_light->width,
+                                      environment_light->height, environment_light->scale),
+                     "pt_scene_set_env");
+      pt_camera_desc cd = camera->desc();
+      ptamd::check(pt_scene_set_camera(s, &cd), "pt_scene_set_camera");
+      ptamd::check(pt_scene_commit(s, nullptr), "pt_scene_commit");
+    } catch (...) {
+      pt_scene_destroy(s);
+      throw;
+    }
+    return s;
+  }
+
+  // Renders the frame and writes it as PNG (gamma `write_gamma`, vertical
+  // flip, FrameBuffer::writeToFile); a path ending in .pfm gets the FP32
+  // film instead.  Returns the film (height x width x 3, row 0 at the bottom).
+  std::vector<float> render(const std::string &out_path = "./output.png", bool show_progress = true) {
+    pt_scene *s = build();
+    std::vector<float> film((size_t)_width * _height * 3);
+    try {
+      pt_render_cfg cfg = render_cfg();
+      pt_stats st{};
+      ptamd::check(pt_render(s, &cfg, &st), "pt_render");
+      ptamd::check(pt_film_read(s, film.data()), "pt_film_read");
+      if (show_progress)
+        std::printf("rendered %ux%u, %u spp in %.1f ms (%llu path samples)\n", _width, _height, spp_x * spp_y,
+                    st.ms_total, (unsigned long long)st.paths);
+    } catch (...) {
+      pt_scene_destroy(s);
+      throw;
+    }
+    pt_scene_destroy(s);
+    if (!out_path.empty()) {
+      if (out_path.size() > 4 && out_path.substr(out_path.size() - 4) == ".pfm")
+        ptamd::image_io::write_pfm(out_path, film.data(), _width, _height);
+      else
+        ptamd::image_io::write_png(out_path, film.data(), _width, _height, write_gamma);
+    }
+    return film;
+  }
+};

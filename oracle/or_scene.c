@@ -55,6 +55,14 @@ int or_scene_add_material(or_scene *s, const pt_material_desc *d) {
 
 int or_scene_add_mesh(or_scene *s, const float *pos, uint32_t nv, const int32_t *idx,
                       uint32_t nt, uint32_t mat, uint32_t flags) {
+  return or_scene_add_mesh_normals(s, pos, nv, idx, nt, NULL, 0, NULL, mat, flags);
+}
+
+/* TriangleMesh(MeshData, material) (primitives/triangleMesh.cpp:5-26), with
+ * optional MeshData::normals / normalIndices (mesh_io/meshData.hpp:6-13). */
+int or_scene_add_mesh_normals(or_scene *s, const float *pos, uint32_t nv, const int32_t *idx,
+                              uint32_t nt, const float *nrm, uint32_t nn, const int32_t *nidx,
+                              uint32_t mat, uint32_t flags) {
   GROW(s->meshes, s->n_meshes, s->cap_meshes, or_mesh);
   or_mesh *m = &s->meshes[s->n_meshes];
   memset(m, 0, sizeof(*m));
@@ -70,6 +78,14 @@ int or_scene_add_mesh(or_scene *s, const float *pos, uint32_t nv, const int32_t 
       int32_t vi = idx[3 * t + c];
       if (vi < 0 || (uint32_t)vi >= nv) return -1;
       tr->p[c] = V3(pos[3 * vi], pos[3 * vi + 1], pos[3 * vi + 2]);
+    }
+    tr->smooth = nidx != NULL;
+    if (nidx) {
+      for (int c = 0; c < 3; ++c) {
+        int32_t ni = nidx[3 * t + c];
+        if (ni < 0 || (uint32_t)ni >= nn) return -1;
+        tr->n[c] = V3(nrm[3 * ni], nrm[3 * ni + 1], nrm[3 * ni + 2]);
+      }
     }
     tr->mesh = s->n_meshes;
   }
@@ -253,7 +269,7 @@ int or_scene_commit(or_scene *s) {
       or_rect *r = &s->rects[i];
       if (r->seq != q) continue;
       const pt_material_desc *md = &s->mats[r->mat];
-      if (md->kind != PT_MAT_EMITTER || !(md->p[3] > 0.f)) continue;
+      if (md->kind != PT_MAT_EMITTER || !(md->p[3] > 0.f) || (r->flags & PT_PRIM_NOT_LIGHT)) continue;
       or_light *L = &s->lights[s->n_lights];
       L->type = OR_LIGHT_QUAD;
       /* QuadLight(AARectangle<Axis>) vertex order (light.hpp:71-108) */
@@ -280,7 +296,7 @@ int or_scene_commit(or_scene *s) {
       or_sphere *sp = &s->spheres[i];
       if (sp->seq != q) continue;
       const pt_material_desc *md = &s->mats[sp->mat];
-      if (md->kind != PT_MAT_EMITTER || !(md->p[3] > 0.f)) continue;
+      if (md->kind != PT_MAT_EMITTER || !(md->p[3] > 0.f) || (sp->flags & PT_PRIM_NOT_LIGHT)) continue;
       or_light *L = &s->lights[s->n_lights];
       L->type = OR_LIGHT_SPHERE;
       L->c = sp->c;
@@ -305,13 +321,14 @@ int or_scene_num_lights(const or_scene *s) { return s->n_lights; }
 /* ---------------- intersection ---------------- */
 
 /* HitRecord::set_normal (material/materialBase.hpp:48-52) plus the
- * wrapper semantics (FlipFace toggles is_inside, objectTransform.hpp:21;
- * Translate/Rotate/Scale force it true). */
+ * wrapper semantics: Translate/Rotate/Scale force is_inside true
+ * (objectTransform.hpp:60,156; objectTransform.cpp:132,172,211), then a
+ * FlipFace outside them toggles it (objectTransform.hpp:21). */
 static void set_normal(or_hit *h, v3 d, v3 n, uint32_t flags) {
   h->inside = vdot(d, n) < 0.f;
   h->n = h->inside ? n : vneg(n);
-  if (flags & PT_PRIM_FLIP_FACE) h->inside = !h->inside;
   if (flags & PT_PRIM_FORCE_INSIDE) h->inside = 1;
+  if (flags & PT_PRIM_FLIP_FACE) h->inside = !h->inside;
 }
 
 /* AABB::intersect (utils/aabb.hpp:21-42): Kensler slab on the raw direction */
@@ -456,7 +473,20 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
     v3 ng = vnorm(vcross(e1, e2));
     h->p = vadd(o, vscale(d, best));
     const or_mesh *m = &s->meshes[tr->mesh];
-    set_normal(h, d, ng, m->flags);
+    v3 ns = ng;
+    if (tr->smooth) {
+      /* MeshBVHNode::hitTriangle (meshBVH.cpp:95-130): barycentrics of the
+       * hit, interpolated normal, turned to the geometric normal's side */
+      v3 P = vcross(d, e2);
+      float inv = 1.0f / vdot(e1, P);
+      v3 T = vsub(o, tr->p[0]);
+      float u = vdot(T, P) * inv;
+      v3 Q = vcross(T, e1);
+      float v = vdot(d, Q) * inv;
+      ns = vnorm(vadd(vadd(vscale(tr->n[0], 1.f - u - v), vscale(tr->n[1], u)), vscale(tr->n[2], v)));
+      if (vdot(ns, ng) < 0.f) ns = vneg(ns);
+    }
+    set_normal(h, d, ns, m->flags);
     h->mat = m->mat;
     h->prim = idx;
   } else if (kind == 1) {
@@ -504,6 +534,22 @@ int or_intersect_batch(const or_scene *s, const float *rays, size_t n, int any_h
       prim[i] = hit ? h.prim : -1;
       t[i] = hit ? h.t : INFINITY;
     }
+  }
+  return 0;
+}
+
+/* Closest hit's shading normal (turned against the ray, HitRecord::set_normal)
+ * and is_inside, for rays that hit; zeros and -1 for misses. */
+int or_surface_batch(const or_scene *s, const float *rays, size_t n, float *nrm, int32_t *inside) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long i = 0; i < (long long)n; ++i) {
+    const float *r = &rays[8 * i];
+    or_hit h;
+    int hit = or_scene_intersect(s, V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]), r[6], r[7], 0, &h, NULL);
+    nrm[3 * i] = hit ? h.n.x : 0.f;
+    nrm[3 * i + 1] = hit ? h.n.y : 0.f;
+    nrm[3 * i + 2] = hit ? h.n.z : 0.f;
+    inside[i] = hit ? h.inside : -1;
   }
   return 0;
 }

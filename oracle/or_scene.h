@@ -6,7 +6,7 @@
 #define OR_SCENE_H
 #include "oracle.h"
 
-typedef struct { v3 p[3]; int mesh; } or_tri;
+typedef struct { v3 p[3]; int mesh; int smooth; v3 n[3]; } or_tri; /* n: corner normals when smooth */
 typedef struct {
   v3 bmin, bmax;
   int left, right;

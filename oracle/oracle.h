@@ -82,6 +82,9 @@ void or_scene_destroy(or_scene *s);
 int or_scene_add_material(or_scene *s, const pt_material_desc *d);
 int or_scene_add_mesh(or_scene *s, const float *pos, uint32_t nv, const int32_t *idx,
                       uint32_t nt, uint32_t mat, uint32_t flags);
+int or_scene_add_mesh_normals(or_scene *s, const float *pos, uint32_t nv, const int32_t *idx,
+                              uint32_t nt, const float *nrm, uint32_t nn, const int32_t *nidx,
+                              uint32_t mat, uint32_t flags);
 int or_scene_add_sphere(or_scene *s, const float *c, float r, uint32_t mat, uint32_t flags);
 int or_scene_add_rect(or_scene *s, int axis, float k, float d0min, float d0max, float d1min,
                       float d1max, uint32_t mat, uint32_t flags);
@@ -89,6 +92,7 @@ int or_scene_set_env(or_scene *s, const float *rgb, uint32_t w, uint32_t h, floa
 int or_scene_set_camera(or_scene *s, const pt_camera_desc *c);
 int or_scene_commit(or_scene *s);
 int or_scene_num_lights(const or_scene *s);
+int or_surface_batch(const or_scene *s, const float *rays, size_t n, float *nrm, int32_t *inside);
 int or_intersect_batch(const or_scene *s, const float *rays, size_t n, int any_hit,
                        int32_t *prim, float *t);
 int or_brute_intersect_batch(const or_scene *s, const float *rays, size_t n,

@@ -43,6 +43,7 @@ def _load():
         "or_scene_destroy": (None, [_P]),
         "or_scene_add_material": (ctypes.c_int, [_P, _P]),
         "or_scene_add_mesh": (ctypes.c_int, [_P, _P, u32, _P, u32, u32, u32]),
+        "or_scene_add_mesh_normals": (ctypes.c_int, [_P, _P, u32, _P, u32, _P, u32, _P, u32, u32]),
         "or_scene_add_sphere": (ctypes.c_int, [_P, _P, f32, u32, u32]),
         "or_scene_add_rect": (ctypes.c_int, [_P, ctypes.c_int, f32, f32, f32, f32, f32, u32, u32]),
         "or_scene_set_env": (ctypes.c_int, [_P, _P, u32, u32, f32]),
@@ -51,6 +52,7 @@ def _load():
         "or_scene_num_lights": (ctypes.c_int, [_P]),
         "or_intersect_batch": (ctypes.c_int, [_P, _P, sz, ctypes.c_int, _P, _P]),
         "or_brute_intersect_batch": (ctypes.c_int, [_P, _P, sz, _P, _P]),
+        "or_surface_batch": (ctypes.c_int, [_P, _P, sz, _P, _P]),
         "or_light_sample_batch": (ctypes.c_int, [_P, u32, _P, _P, sz, _P, _P, _P, _P]),
         "or_env_sample_batch": (ctypes.c_int, [_P, _P, sz, _P, _P]),
         "or_env_pdf": (f32, [_P, _P]),
@@ -95,7 +97,13 @@ class OracleScene:
         for kind, i in spec.order:
             if kind == "mesh":
                 pos, idx, mat, flags = spec.meshes[i]
-                if lib.or_scene_add_mesh(self.h, ptr(pos), pos.shape[0], ptr(idx), idx.shape[0], mat, flags) < 0:
+                if i in spec.mesh_normals:
+                    nrm, nidx = spec.mesh_normals[i]
+                    rc = lib.or_scene_add_mesh_normals(self.h, ptr(pos), pos.shape[0], ptr(idx), idx.shape[0],
+                                                       ptr(nrm), nrm.shape[0], ptr(nidx), mat, flags)
+                else:
+                    rc = lib.or_scene_add_mesh(self.h, ptr(pos), pos.shape[0], ptr(idx), idx.shape[0], mat, flags)
+                if rc < 0:
                     raise ValueError("bad mesh")
             elif kind == "sphere":
                 c, r, mat, flags = spec.spheres[i]
@@ -153,6 +161,15 @@ class OracleScene:
         else:
             lib.or_intersect_batch(self.h, ptr(rays), n, int(any_hit), ptr(prim), ptr(t))
         return prim, t
+
+    def surface(self, rays):
+        """Shading normal (against the ray) and is_inside (-1: miss) of the closest hits."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        nrm = np.empty((n, 3), np.float32)
+        inside = np.empty(n, np.int32)
+        lib.or_surface_batch(self.h, ptr(rays), n, ptr(nrm), ptr(inside))
+        return nrm, inside
 
     def camera_rays(self, cfg, pixel, k):
         pixel = np.ascontiguousarray(pixel, np.uint32)

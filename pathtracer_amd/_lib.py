@@ -26,7 +26,7 @@ EST_REFERENCE_NEE, EST_NEE_MIS = 0, 1
 # render flags
 RENDER_COUNTERS, RENDER_NO_SORT, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
-PRIM_FLIP_FACE, PRIM_FORCE_INSIDE = 1, 2
+PRIM_FLIP_FACE, PRIM_FORCE_INSIDE, PRIM_NOT_LIGHT = 1, 2, 4
 NPARAMS = 30
 
 
@@ -80,7 +80,7 @@ class SceneInfo(ctypes.Structure):
 
 # every symbol include/pathtracer_amd.h declares
 EXPORTED = [
-    "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_mesh",
+    "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_mesh", "pt_scene_add_mesh_normals",
     "pt_scene_add_sphere", "pt_scene_add_rect", "pt_scene_set_env", "pt_scene_set_camera",
     "pt_scene_commit", "pt_scene_get_info", "pt_render", "pt_film_read", "pt_film_read_samples",
     "pt_sample", "pt_intersect", "pt_bsdf_eval", "pt_bsdf_sample", "pt_light_sample",
@@ -103,6 +103,7 @@ def _load():
         "pt_scene_destroy": (None, [_P]),
         "pt_scene_add_material": (i32, [_P, ctypes.POINTER(MaterialDesc), ctypes.POINTER(u32)]),
         "pt_scene_add_mesh": (i32, [_P, _P, u32, _P, u32, u32, u32]),
+        "pt_scene_add_mesh_normals": (i32, [_P, _P, u32, _P, u32, _P, u32, _P, u32, u32]),
         "pt_scene_add_sphere": (i32, [_P, _P, ctypes.c_float, u32, u32]),
         "pt_scene_add_rect": (i32, [_P, ctypes.c_int] + [ctypes.c_float] * 5 + [u32, u32]),
         "pt_scene_set_env": (i32, [_P, _P, u32, u32, ctypes.c_float]),

@@ -290,17 +290,39 @@ struct SurfHit {
   uint32_t mat;
   int light;
 };
+// Interpolated shading normal of a triangle hit (meshBVH.cpp:116-130): the
+// barycentrics (u, v) of MeshBVHNode::hitTriangle recomputed from the same
+// ray with the same arithmetic as hit_tri, then normalize((1-u-v) n0 + u n1 +
+// v n2), turned to the geometric normal's side.
+#pragma clang fp contract(off)
+PT_D f3 smooth_normal(const float4 *r, f3 o, f3 d, f3 ng) {
+  const float4 a = r[0], b = r[1], c = r[2], e = r[3], f = r[4];
+  f3 p0 = mk3(a.x, a.y, a.z), e1 = mk3(b.x, b.y, b.z), e2 = mk3(c.x, c.y, c.z);
+  f3 P = cross3(d, e2);
+  float inv = rcp_rn(dot3(e1, P));
+  f3 T = sub3(o, p0);
+  float u = dot3(T, P) * inv;
+  f3 Q = cross3(T, e1);
+  float v = dot3(d, Q) * inv;
+  f3 n0 = mk3(a.w, b.w, c.w), n1 = mk3(e.x, e.y, e.z), n2 = mk3(e.w, f.x, f.y);
+  f3 ns = norm3_rn(add3(add3(scl3(n0, 1.f - u - v), scl3(n1, u)), scl3(n2, v)));
+  return dot3(ns, ng) < 0.f ? neg3(ns) : ns;
+}
+#pragma clang fp contract(fast)
+
 PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
   SurfHit h;
   uint2 info = s.prim_info[prim];
   float4 g = s.prim_geom[prim];
   uint32_t type = (info.x >> 24) & 3u, fl = info.x >> 28;
   h.mat = info.x & 0xffffffu;
-  h.light = (int)info.y;
+  h.light = (type == PRIM_TRI) ? -1 : (int)info.y;
   f3 n;
   if (type == PRIM_TRI) {
     h.p = add3(o, scl3(d, t));
     n = mk3(g.x, g.y, g.z);
+    h.ng = n;
+    if (info.y != 0xffffffffu) n = smooth_normal(s.tri_smooth + 5 * (size_t)info.y, o, d, n);
   } else if (type == PRIM_SPHERE) {
     h.p = add3(o, scl3(norm3(d), t));
     n = norm3(sub3(h.p, mk3(g.x, g.y, g.z)));
@@ -309,11 +331,11 @@ PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
     int ax = (int)g.x;
     n = ax == 0 ? mk3(1.f, 0.f, 0.f) : (ax == 1 ? mk3(0.f, 1.f, 0.f) : mk3(0.f, 0.f, 1.f));
   }
-  h.ng = n;
+  if (type != PRIM_TRI) h.ng = n;
   bool inside = dot3(d, n) < 0.f;
   h.sp.n = inside ? n : neg3(n);
-  if (fl & PT_PRIM_FLIP_FACE) inside = !inside;
   if (fl & PT_PRIM_FORCE_INSIDE) inside = true;
+  if (fl & PT_PRIM_FLIP_FACE) inside = !inside;
   h.sp.inside = inside;
   return h;
 }

@@ -87,7 +87,7 @@ static hipError_t upload(DevBuf &b, const void *src, size_t n) {
 }
 
 // ------------------------------------------------------------------ host-side scene records
-struct HTri { f3 p[3]; uint32_t mesh; };
+struct HTri { f3 p[3]; uint32_t mesh; int smooth; };  // smooth: index into pt_scene::normals3 or -1
 struct HMesh { uint32_t first, count, mat, flags, seq; };
 struct HSphere { f3 c; float r; uint32_t mat, flags, seq; int light; };
 struct HRect { int axis; float k, d0min, d0max, d1min, d1max; uint32_t mat, flags, seq; int light; };
@@ -95,6 +95,7 @@ struct HRect { int axis; float k, d0min, d0max, d1min, d1max; uint32_t mat, flag
 struct pt_scene {
   std::vector<pt_material_desc> mats;
   std::vector<HTri> tris;
+  std::vector<f3> normals3;  // three corner normals per smooth triangle
   std::vector<HMesh> meshes;
   std::vector<HSphere> spheres;
   std::vector<HRect> rects;
@@ -109,7 +110,7 @@ struct pt_scene {
   // device
   hipStream_t stream = nullptr;
   int device = -1;
-  DevBuf d_nodes, d_prims, d_info, d_geom, d_mats, d_lights, d_env, d_env_func, d_env_cdf, d_env_mcdf;
+  DevBuf d_nodes, d_prims, d_info, d_geom, d_smooth, d_mats, d_lights, d_env, d_env_func, d_env_cdf, d_env_mcdf;
   DevBuf d_perm, d_perm_off;
   uint32_t perm_seed = 0xffffffffu;
   bool perm_valid = false;
@@ -179,28 +180,54 @@ extern "C" pt_status pt_scene_add_material(pt_scene *s, const pt_material_desc *
   return PT_OK;
 }
 
-extern "C" pt_status pt_scene_add_mesh(pt_scene *s, const float *pos, uint32_t nv, const int32_t *idx, uint32_t nt,
-                                       uint32_t mat, uint32_t flags) {
-  if (!s || (!pos && nv) || (!idx && nt)) return fail(PT_ERR_INVALID, "pt_scene_add_mesh: NULL argument");
-  if (mat >= s->mats.size()) return fail(PT_ERR_INVALID, "pt_scene_add_mesh: material id out of range");
-  HMesh m{(uint32_t)s->tris.size(), nt, mat, flags, s->seq++};
-  s->tris.reserve(s->tris.size() + nt);
+static pt_status add_mesh(pt_scene *s, const char *fn, const float *pos, uint32_t nv, const int32_t *idx, uint32_t nt,
+                          const float *nrm, uint32_t nn, const int32_t *nidx, uint32_t mat, uint32_t flags) {
+  const std::string f(fn);
+  if (!s || (!pos && nv) || (!idx && nt) || (nidx && !nrm && nt)) return fail(PT_ERR_INVALID, f + ": NULL argument");
+  if (mat >= s->mats.size()) return fail(PT_ERR_INVALID, f + ": material id out of range");
+  const size_t first = s->tris.size(), first_n = s->normals3.size();
+  auto undo = [&](const char *msg) {
+    s->tris.resize(first);
+    s->normals3.resize(first_n);
+    return fail(PT_ERR_INVALID, f + msg);
+  };
+  HMesh m{(uint32_t)first, nt, mat, flags, 0};
+  s->tris.reserve(first + nt);
   for (uint32_t t = 0; t < nt; ++t) {
     HTri tr;
     for (int c = 0; c < 3; ++c) {
       int32_t v = idx[3 * t + c];
-      if (v < 0 || (uint32_t)v >= nv) {
-        s->tris.resize(m.first);
-        return fail(PT_ERR_INVALID, "pt_scene_add_mesh: vertex index out of range");
-      }
+      if (v < 0 || (uint32_t)v >= nv) return undo(": vertex index out of range");
       tr.p[c] = mk3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
     }
     tr.mesh = (uint32_t)s->meshes.size();
+    tr.smooth = -1;
+    if (nidx) {
+      tr.smooth = (int)(s->normals3.size() / 3);
+      for (int c = 0; c < 3; ++c) {
+        int32_t v = nidx[3 * t + c];
+        if (v < 0 || (uint32_t)v >= nn) return undo(": normal index out of range");
+        s->normals3.push_back(mk3(nrm[3 * v], nrm[3 * v + 1], nrm[3 * v + 2]));
+      }
+    }
     s->tris.push_back(tr);
   }
+  m.seq = s->seq++;
   s->meshes.push_back(m);
   s->committed = false;
   return PT_OK;
+}
+
+extern "C" pt_status pt_scene_add_mesh(pt_scene *s, const float *pos, uint32_t nv, const int32_t *idx, uint32_t nt,
+                                       uint32_t mat, uint32_t flags) {
+  return add_mesh(s, "pt_scene_add_mesh", pos, nv, idx, nt, nullptr, 0, nullptr, mat, flags);
+}
+
+extern "C" pt_status pt_scene_add_mesh_normals(pt_scene *s, const float *pos, uint32_t nv, const int32_t *idx,
+                                               uint32_t nt, const float *nrm, uint32_t nn, const int32_t *nidx,
+                                               uint32_t mat, uint32_t flags) {
+  if (!nidx && nt) return fail(PT_ERR_INVALID, "pt_scene_add_mesh_normals: NULL argument");
+  return add_mesh(s, "pt_scene_add_mesh_normals", pos, nv, idx, nt, nrm, nn, nidx, mat, flags);
 }
 
 extern "C" pt_status pt_scene_add_sphere(pt_scene *s, const float c[3], float r, uint32_t mat, uint32_t flags) {
@@ -573,7 +600,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
       if (r.seq != q) continue;
       r.light = -1;
       const pt_material_desc &md = s->mats[r.mat];
-      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f)) continue;
+      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f) || (r.flags & PT_PRIM_NOT_LIGHT)) continue;
       f3 v[4];
       float k = r.k;
       if (r.axis == 0) {
@@ -598,7 +625,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
       if (sp.seq != q) continue;
       sp.light = -1;
       const pt_material_desc &md = s->mats[sp.mat];
-      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f)) continue;
+      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f) || (sp.flags & PT_PRIM_NOT_LIGHT)) continue;
       DevLight L{};
       L.type = LIGHT_SPHERE;
       L.c = sp.c;
@@ -622,6 +649,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   B.refs.resize(nprim);
   std::vector<uint2> info(nprim);
   std::vector<float4> geom(nprim);
+  std::vector<float4> smooth;  // 5 float4 per triangle with shading normals
   for (uint32_t i = 0; i < ntri; ++i) {
     const HTri &t = s->tris[i];
     BRef &r = B.refs[i];
@@ -631,8 +659,19 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     }
     r.prim = i;
     const HMesh &m = s->meshes[t.mesh];
-    info[i] = make_uint2(m.mat | (PRIM_TRI << 24) | ((m.flags & 3u) << 28), 0xffffffffu);
     f3 e1 = sub3(t.p[1], t.p[0]), e2 = sub3(t.p[2], t.p[0]);
+    uint32_t sm = 0xffffffffu;
+    if (t.smooth >= 0) {
+      // p0, e1, e2 for the barycentrics of the hit, then the three corner normals
+      sm = (uint32_t)(smooth.size() / 5);
+      const f3 *n = &s->normals3[3 * (size_t)t.smooth];
+      smooth.push_back(make_float4(t.p[0].x, t.p[0].y, t.p[0].z, n[0].x));
+      smooth.push_back(make_float4(e1.x, e1.y, e1.z, n[0].y));
+      smooth.push_back(make_float4(e2.x, e2.y, e2.z, n[0].z));
+      smooth.push_back(make_float4(n[1].x, n[1].y, n[1].z, n[2].x));
+      smooth.push_back(make_float4(n[2].y, n[2].z, 0.f, 0.f));
+    }
+    info[i] = make_uint2(m.mat | (PRIM_TRI << 24) | ((m.flags & 3u) << 28), sm);
     f3 ng = norm3(cross3(e1, e2));
     geom[i] = make_float4(ng.x, ng.y, ng.z, 0.f);
   }
@@ -758,6 +797,10 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   HIP_TRY(upload(s->d_prims, prims.data(), prims.size() * sizeof(float4)));
   HIP_TRY(upload(s->d_info, info.data(), info.size() * sizeof(uint2)));
   HIP_TRY(upload(s->d_geom, geom.data(), geom.size() * sizeof(float4)));
+  if (smooth.empty())
+    HIP_TRY(s->d_smooth.alloc(sizeof(float4)));
+  else
+    HIP_TRY(upload(s->d_smooth, smooth.data(), smooth.size() * sizeof(float4)));
   HIP_TRY(upload(s->d_mats, s->mats.data(), s->mats.size() * sizeof(pt_material_desc)));
   if (s->lights.empty())
     HIP_TRY(s->d_lights.alloc(sizeof(DevLight)));
@@ -774,6 +817,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.prims = s->d_prims.as<float4>();
   ds.prim_info = s->d_info.as<uint2>();
   ds.prim_geom = s->d_geom.as<float4>();
+  ds.tri_smooth = s->d_smooth.as<float4>();
   ds.mats = s->d_mats.as<pt_material_desc>();
   ds.lights = s->d_lights.as<DevLight>();
   ds.n_lights = (int)s->lights.size();
@@ -802,7 +846,8 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   in.n_lights = (uint32_t)s->lights.size();
   in.bvh_nodes = (uint32_t)(nodes4.size() / 8);
   in.bvh_depth = depth4;
-  in.device_bytes = s->d_nodes.bytes + s->d_prims.bytes + s->d_info.bytes + s->d_geom.bytes + s->d_env.bytes +
+  in.device_bytes = s->d_nodes.bytes + s->d_prims.bytes + s->d_info.bytes + s->d_geom.bytes + s->d_smooth.bytes +
+                    s->d_env.bytes +
                     s->d_env_func.bytes + s->d_env_cdf.bytes;
   in.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   s->committed = true;

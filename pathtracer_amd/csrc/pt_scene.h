@@ -11,7 +11,10 @@
 //              e1 = p1-p0, e2 = p2-p0 are formed on the host exactly as
 //              MeshBVHNode::hitTriangle forms them (meshBVH.cpp:93-94).
 //   prim_info  per global primitive id: x = material | type<<24 | flags<<28,
-//              y = light index (or -1).
+//              y = light index (spheres, rects) or shading-normal record
+//              (triangles), -1 for none.
+//   tri_smooth per triangle with shading normals, 80 B: the vertex p0 and
+//              edges e1, e2 (for the hit's barycentrics) and three normals.
 //   prim_geom  per global primitive id: triangle geometric normal,
 //              sphere centre+radius, rect axis.
 #pragma once
@@ -47,6 +50,7 @@ struct DevScene {
   const float4 *prims;
   const uint2 *prim_info;
   const float4 *prim_geom;
+  const float4 *tri_smooth;  // per smooth triangle: (p0,n0.x) (e1,n0.y) (e2,n0.z) (n1,n2.x) (n2.y,n2.z,-,-)
   const pt_material_desc *mats;
   const DevLight *lights;
   int n_lights;

@@ -26,8 +26,14 @@ class Scene:
         for kind, i in spec.order:
             if kind == "mesh":
                 pos, idx, mat, flags = spec.meshes[i]
-                L.check(L.lib.pt_scene_add_mesh(h, L.ptr(pos), pos.shape[0], L.ptr(idx), idx.shape[0], mat, flags),
-                        "pt_scene_add_mesh")
+                if i in spec.mesh_normals:
+                    nrm, nidx = spec.mesh_normals[i]
+                    L.check(L.lib.pt_scene_add_mesh_normals(h, L.ptr(pos), pos.shape[0], L.ptr(idx), idx.shape[0],
+                                                            L.ptr(nrm), nrm.shape[0], L.ptr(nidx), mat, flags),
+                            "pt_scene_add_mesh_normals")
+                else:
+                    L.check(L.lib.pt_scene_add_mesh(h, L.ptr(pos), pos.shape[0], L.ptr(idx), idx.shape[0], mat,
+                                                    flags), "pt_scene_add_mesh")
             elif kind == "sphere":
                 c, r, mat, flags = spec.spheres[i]
                 ca = np.asarray(c, np.float32)

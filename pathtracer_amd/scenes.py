@@ -37,15 +37,19 @@ class SceneSpec:
     camera: dict = dataclasses.field(default_factory=dict)
     render: dict = dataclasses.field(default_factory=dict)
     order: list = dataclasses.field(default_factory=list)        # insertion order of ("mesh"|"sphere"|"rect", index)
+    mesh_normals: dict = dataclasses.field(default_factory=dict)  # mesh index -> (normals f32 (k,3), nidx i32 (m,3))
     name: str = ""
 
     def add_material(self, kind, params) -> int:
         self.materials.append((kind, list(params)))
         return len(self.materials) - 1
 
-    def add_mesh(self, pos, idx, mat, flags=0):
+    def add_mesh(self, pos, idx, mat, flags=0, normals=None, nidx=None):
         self.meshes.append((np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(idx, np.int32), mat, flags))
         self.order.append(("mesh", len(self.meshes) - 1))
+        if normals is not None:
+            self.mesh_normals[len(self.meshes) - 1] = (np.ascontiguousarray(normals, np.float32),
+                                                       np.ascontiguousarray(nidx, np.int32))
 
     def add_sphere(self, c, r, mat, flags=0):
         self.spheres.append((tuple(float(x) for x in c), float(r), mat, flags))
@@ -76,9 +80,9 @@ def render_cfg(**kw) -> L.RenderCfg:
 
 
 def camera_desc(fov_deg, aspect, focal_dist, aperture, up, front, pos, pos1=None, shutter=0.0,
-                normalize_dir=1) -> L.CameraDesc:
+                normalize_dir=1, fov_rad=None) -> L.CameraDesc:
     c = L.CameraDesc()
-    c.fov = math.radians(fov_deg)
+    c.fov = math.radians(fov_deg) if fov_rad is None else fov_rad
     c.aspect = aspect
     c.focal_dist = focal_dist
     c.aperture = aperture
@@ -194,8 +198,9 @@ def sky_hdri(w=1024, h=512, seed=1, zenith=0.3, horizon=1.0, sun_peak=5e4, sun_w
 # ---------------------------------------------------------------- workloads
 
 def cornell(width=256, height=256, spp_x=4, spp_y=4, max_depth=8, rr_depth=3,
-            sampler=L.SAMPLER_HALTON_OWEN, estimator=L.EST_REFERENCE_NEE, ico_subdiv=3) -> SceneSpec:
-    """Cornell box, side 2.0, made of 10 quads (20 triangles)."""
+            sampler=L.SAMPLER_HALTON_OWEN, estimator=L.EST_REFERENCE_NEE, ico_subdiv=3, smooth=False) -> SceneSpec:
+    """Cornell box, side 2.0, made of 10 quads (20 triangles).  smooth: the
+    icosphere carries per-vertex shading normals (the unit sphere's)."""
     s = SceneSpec(name="cornell")
     white = s.add_material(L.MAT_LAMBERTIAN, [0.73, 0.73, 0.73])
     red = s.add_material(L.MAT_LAMBERTIAN, [0.65, 0.05, 0.05])
@@ -219,7 +224,8 @@ def cornell(width=256, height=256, spp_x=4, spp_y=4, max_depth=8, rr_depth=3,
     s.add_mesh(*merge(green_parts), green)
     s.add_rect(1, 1.999, -0.25, 0.25, -0.25, 0.25, light)
     v, f = icosphere(ico_subdiv)
-    s.add_mesh((v * 0.35 + np.array([-0.4, 0.35, -0.3])).astype(np.float32), f.astype(np.int32), gold)
+    s.add_mesh((v * 0.35 + np.array([-0.4, 0.35, -0.3])).astype(np.float32), f.astype(np.int32), gold,
+               normals=v.astype(np.float32) if smooth else None, nidx=f.astype(np.int32) if smooth else None)
     s.add_sphere((0.45, 0.3, 0.3), 0.3, diffuse)
     s.camera = dict(fov_deg=40.0, aspect=width / height, focal_dist=1.0, aperture=0.0, up=(0, 1, 0),
                     front=(0, 0, -1), pos=(0, 1.0, 3.9))

@@ -1,0 +1,143 @@
+// scene_api_check.cpp -- drives include/pathtracer_amd/scene.hpp the way a
+// scene function written for the reference would, for the tests:
+//   scene_api_check dump   <obj> <out.txt>   lowering only (no device): one line per ABI call
+//   scene_api_check render <obj> <out.pfm>   SceneInfo::render on the GPU (FP32 film)
+//   scene_api_check hdr    <in.hdr> <out.pfm>  EnvironmentLight texels, written back as PFM
+//   scene_api_check png    <out.png>         a fixed 4x3 film through write_png
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/pathtracer_amd/scene.hpp"
+
+using namespace gl;
+
+// A scene in the reference's style (compare scenes/scenes_hdri.hpp:5-60).
+static SceneInfo test_scene(const std::string &obj) {
+  SceneInfo scene;
+  auto white = std::make_shared<Lambertian>(vec3(0.73f, 0.73f, 0.73f));
+  auto red = std::make_shared<Lambertian>(vec3(0.65f, 0.05f, 0.05f));
+  auto light = std::make_shared<DiffuseEmitter>(vec3(1.f), 15.f);
+  auto gold = std::make_shared<Conductor>(vec3(0.14f, 0.43f, 1.38f), vec3(4.54f, 2.455f, 1.914f), 0.2f, 0.2f);
+  auto glass = std::make_shared<MFDielectric>(1.5f, 0.1f, 0.1f);
+  auto disney = std::make_shared<DisneyPrincipledBSDF>(vec3(0.8f, 0.3f, 0.2f), 0.f, 0.5f, 0.f, 0.5f, 0.4f, 0.f, 0.f,
+                                                       0.f, 0.5f, 0.f, 0.f, 1.5f);
+  scene.objects.addObject(std::make_shared<YZRectangle>(2.f, 0.f, 2.f, 0.f, 2.f, red));
+  scene.objects.addObject(std::make_shared<FlipFace>(std::make_shared<YZRectangle>(0.f, 0.f, 2.f, 0.f, 2.f, white)));
+  scene.objects.addObject(std::make_shared<XZRectangle>(0.f, 0.f, 2.f, 0.f, 2.f, white));
+  scene.objects.addObject(std::make_shared<XZRectangle>(2.f, 0.f, 2.f, 0.f, 2.f, white));
+  scene.objects.addObject(std::make_shared<XYRectangle>(0.f, 0.f, 2.f, 0.f, 2.f, white));
+  scene.objects.addObject(std::make_shared<XZRectangle>(1.99f, 0.75f, 1.25f, 0.75f, 1.25f, light));
+  // a box rotated about Y then translated: rects become triangles
+  std::shared_ptr<Hittable> box = std::make_shared<Box>(vec3(0.f), vec3(0.5f, 0.9f, 0.5f), white);
+  box = std::make_shared<Rotate<Axis::Y>>(box, to_radian(15.f));
+  box = std::make_shared<Translate>(box, vec3(1.1f, 0.f, 0.9f));
+  scene.objects.addObject(box);
+  // a sphere scaled then translated, and a flipped translated sphere
+  std::shared_ptr<Hittable> ball = std::make_shared<Sphere>(vec3(0.f), 1.f, gold);
+  ball = std::make_shared<Scale>(ball, 0.3f);
+  ball = std::make_shared<Translate>(ball, vec3(0.5f, 0.3f, 0.6f));
+  scene.objects.addObject(ball);
+  scene.objects.addObject(
+      std::make_shared<FlipFace>(std::make_shared<Translate>(std::make_shared<Sphere>(vec3(0.f), 0.15f, disney),
+                                                             vec3(1.5f, 1.2f, 0.4f))));
+  // an emitter in a nested list is hit but never sampled
+  auto nested = std::make_shared<ObjectList>();
+  nested->addObject(std::make_shared<XYRectangle>(1.98f, 0.2f, 0.4f, 1.5f, 1.7f, light));
+  scene.objects.addObject(nested);
+  // an OBJ mesh with shading normals, scaled, rotated about X and translated
+  std::shared_ptr<Hittable> mesh = loadOBJMesh(obj, glass);
+  mesh = std::make_shared<Scale>(mesh, 0.25f);
+  mesh = std::make_shared<Rotate<Axis::X>>(mesh, to_radian(30.f));
+  mesh = std::make_shared<Translate>(mesh, vec3(0.6f, 1.2f, 1.2f));
+  scene.objects.addObject(mesh);
+
+  scene.camera = std::make_shared<PerspectiveCamera>(to_radian(40.f), 1.f, 1.f, 1e30f, vec3(0.f, 1.f, 0.f),
+                                                     vec3(0.f, 0.f, -1.f), vec3(1.f, 1.f, 4.2f));
+  scene._width = 64;
+  scene._height = 64;
+  scene.spp_x = 2;
+  scene.spp_y = 2;
+  scene.max_depth = 6;
+  scene.bg_color = vec3(0.f);
+  return scene;
+}
+
+struct RecordingSink : ptamd::SceneSink {
+  FILE *f;
+  uint32_t n = 0;
+  explicit RecordingSink(FILE *f) : f(f) {}
+  uint32_t material(const pt_material_desc &d) override {
+    std::fprintf(f, "material %u", d.kind);
+    for (int i = 0; i < 15; ++i) std::fprintf(f, " %.9g", d.p[i]);
+    std::fprintf(f, "\n");
+    return n++;
+  }
+  void mesh(const std::vector<float> &pos, const std::vector<int32_t> &idx, const std::vector<float> *nrm,
+            const std::vector<int32_t> *nidx, uint32_t mat, uint32_t flags) override {
+    std::fprintf(f, "mesh %u %u %zu %zu %zu", mat, flags, pos.size() / 3, idx.size() / 3, nrm ? nrm->size() / 3 : 0);
+    for (float x : pos) std::fprintf(f, " %.9g", x);
+    for (int32_t x : idx) std::fprintf(f, " %d", x);
+    if (nrm) {
+      for (float x : *nrm) std::fprintf(f, " %.9g", x);
+      for (int32_t x : *nidx) std::fprintf(f, " %d", x);
+    }
+    std::fprintf(f, "\n");
+  }
+  void sphere(const float c[3], float r, uint32_t mat, uint32_t flags) override {
+    std::fprintf(f, "sphere %u %u %.9g %.9g %.9g %.9g\n", mat, flags, c[0], c[1], c[2], r);
+  }
+  void rect(int axis, float k, float d0min, float d0max, float d1min, float d1max, uint32_t mat,
+            uint32_t flags) override {
+    std::fprintf(f, "rect %u %u %d %.9g %.9g %.9g %.9g %.9g\n", mat, flags, axis, k, d0min, d0max, d1min, d1max);
+  }
+};
+
+int main(int argc, char **argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: scene_api_check dump|render|hdr|png ...\n");
+    return 2;
+  }
+  const std::string mode = argv[1];
+  try {
+    if (mode == "dump" && argc == 4) {
+      SceneInfo s = test_scene(argv[2]);
+      FILE *f = std::fopen(argv[3], "w");
+      if (!f) return 1;
+      RecordingSink sink(f);
+      s.lower(sink);
+      pt_camera_desc c = s.camera->desc();
+      std::fprintf(f, "camera %.9g %.9g %.9g %.9g", c.fov, c.aspect, c.focal_dist, c.aperture);
+      for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.up[i]);
+      for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.front[i]);
+      for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.pos[i]);
+      std::fprintf(f, "\n");
+      std::fclose(f);
+      return 0;
+    }
+    if (mode == "render" && argc == 4) {
+      SceneInfo s = test_scene(argv[2]);
+      s.render(argv[3], false);
+      return 0;
+    }
+    if (mode == "hdr" && argc == 4) {
+      EnvironmentLight env(argv[2], 1.f);
+      ptamd::image_io::write_pfm(argv[3], env.rgb.data(), env.width, env.height);
+      return 0;
+    }
+    if (mode == "png" && argc == 3) {
+      float film[4 * 3 * 3];
+      for (int i = 0; i < 36; ++i) film[i] = (float)i / 35.f;
+      film[5] = -1.f;
+      film[7] = std::nanf("");
+      film[8] = 7.f;
+      ptamd::image_io::write_png(argv[2], film, 4, 3, 2.0f);
+      return 0;
+    }
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  std::fprintf(stderr, "bad arguments\n");
+  return 2;
+}

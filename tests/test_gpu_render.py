@@ -41,6 +41,15 @@ def test_cornell_per_sample(gpu, estimator):
     assert st["nan_samples"] == 0
 
 
+def test_cornell_smooth_normals_per_sample(gpu):
+    """Per-corner shading normals (pt_scene_add_mesh_normals) on the gold icosphere."""
+    from pathtracer_amd import scenes
+    spec = scenes.cornell(width=40, height=40, spp_x=2, spp_y=2, smooth=True)
+    _, gs, _, os_, st = _per_sample(spec, rr_depth=0)
+    assert _sample_agreement(gs, os_) > 0.995
+    assert st["nan_samples"] == 0
+
+
 @pytest.mark.parametrize("sampler", [1, 2, 3, 4])
 def test_cornell_per_sample_samplers(gpu, sampler):
     from pathtracer_amd import scenes

@@ -165,3 +165,77 @@ def test_thin_lens_focus():
     # k differs, so the pixel jitter differs; hits stay within one pixel footprint of each other
     pix = 2 * 4.0 * np.tan(np.radians(15)) / 8
     assert np.ptp(hit[:, 0]) <= pix * 1.01 and np.ptp(hit[:, 1]) <= pix * 1.01
+
+
+def _smooth_scene(pos, idx, nrm, nidx, flags=0):
+    s = scenes.SceneSpec()
+    m = s.add_material(1, [0.5, 0.5, 0.5])
+    s.add_mesh(pos, idx, m, flags, normals=nrm, nidx=nidx)
+    s.camera = dict(fov_deg=40, aspect=1, focal_dist=1, aperture=0, up=(0, 1, 0), front=(0, 0, -1), pos=(0, 0, 5))
+    return s
+
+
+def test_smooth_normal_interpolation_corners():
+    """MeshBVHNode::hitTriangle (meshBVH.cpp:116-130): the shading normal is
+    normalize((1-u-v) n0 + u n1 + v n2) with u along p1-p0 and v along p2-p0,
+    turned to the face normal's side, then against the ray (set_normal).  Rays
+    hit the triangle at chosen barycentrics from both sides."""
+    p = np.array([[0, 0, 0], [2, 0, 0], [0, 2, 0]], np.float32)
+    nrm = unit([[0.2, 0.1, 1.0], [1.0, -0.3, 0.8], [-0.6, 1.0, 0.7]])
+    o = O.OracleScene(_smooth_scene(p, np.array([[0, 1, 2]], np.int32), nrm, np.array([[0, 1, 2]], np.int32)))
+    rng = np.random.default_rng(11)
+    n = 2000
+    u = rng.uniform(0.02, 0.96, n)
+    v = rng.uniform(0.02, 0.98, n) * (1 - u)
+    hit = p[0] + u[:, None] * (p[1] - p[0]) + v[:, None] * (p[2] - p[0])
+    side = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    org = hit + side[:, None] * np.array([0.1, 0.2, 3.0])
+    d = unit(hit - org)
+    rays = np.concatenate([org, d, np.full((n, 1), 1e-4), np.full((n, 1), 100.0)], 1).astype(np.float32)
+    got, inside = o.surface(rays)
+    ns = (1 - u - v)[:, None] * nrm[0] + u[:, None] * nrm[1] + v[:, None] * nrm[2]
+    ns = ns / np.linalg.norm(ns, axis=1, keepdims=True)           # all on the +z side here
+    want = np.where((np.sum(d * ns, axis=1) < 0)[:, None], ns, -ns)
+    np.testing.assert_allclose(got, want, atol=2e-5)
+    assert np.array_equal(inside == 1, np.sum(d * ns, axis=1) < 0)
+
+
+def test_smooth_normals_equal_to_face_normals_change_nothing():
+    pos, idx = scenes.displaced_icosphere(2, 1.0, 0.1, 3)
+    e1 = pos[idx[:, 1]] - pos[idx[:, 0]]
+    e2 = pos[idx[:, 2]] - pos[idx[:, 0]]
+    fn = unit(np.cross(e1, e2))
+    nidx = np.arange(3 * len(idx), dtype=np.int32).reshape(-1, 3)
+    flat = O.OracleScene(_scene_with((pos, idx)))
+    smooth = O.OracleScene(_smooth_scene(pos, idx, np.repeat(fn, 3, axis=0), nidx))
+    rng = np.random.default_rng(12)
+    n = 5000
+    org = rng.uniform(-3, 3, size=(n, 3))
+    d = unit(rng.uniform(-0.8, 0.8, size=(n, 3)) - org)
+    rays = np.concatenate([org, d, np.full((n, 1), 1e-4), np.full((n, 1), 100.0)], 1).astype(np.float32)
+    a, ia = flat.surface(rays)
+    b, ib = smooth.surface(rays)
+    assert np.array_equal(ia, ib)
+    np.testing.assert_allclose(a, b, atol=1e-6)
+
+
+def test_smooth_icosphere_approaches_the_sphere_normal():
+    v, f = scenes.icosphere(3)
+    v = v.astype(np.float32)
+    o_s = O.OracleScene(_smooth_scene(v, f.astype(np.int32), v, f.astype(np.int32)))
+    o_f = O.OracleScene(_scene_with((v, f.astype(np.int32))))
+    rng = np.random.default_rng(13)
+    n = 4000
+    org = unit(rng.normal(size=(n, 3))) * 4.0
+    d = unit(rng.normal(scale=0.2, size=(n, 3)) - org)
+    rays = np.concatenate([org, d, np.full((n, 1), 1e-4), np.full((n, 1), 100.0)], 1).astype(np.float32)
+    prim, t = o_s.intersect(rays)
+    h = prim >= 0
+    p = rays[h, :3] + t[h, None] * rays[h, 3:6]
+    true_n = unit(p)
+    ns, _ = o_s.surface(rays[h])
+    nf, _ = o_f.surface(rays[h])
+    err_s = np.linalg.norm(ns - true_n, axis=1)   # rays arrive from outside: the normal turned against
+    err_f = np.linalg.norm(nf - true_n, axis=1)   # the ray is the outward one
+    assert err_s.max() < 0.02
+    assert err_s.mean() < 0.25 * err_f.mean()
