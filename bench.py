@@ -32,14 +32,14 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpath-samples/s on 1 MI355X; relMSE vs 4096-spp reference at equal time"
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (layout constants of pathtracer_amd/csrc/pt_scene.h)
-B_EXT_RAY = 4 + 24 + 8        # queue index + ray (o, d) read + hit (t, prim) write
+B_EXT_RAY = 24 + 8            # ray (o, d) read + hit (t, prim) write, unit stride
 B_SHADOW_RAY = 44             # shadow-queue entry (o, d, tmax, contribution, slot)
 B_NODE = 112                  # BVH4 node: four child boxes (SoA) + four child codes
 B_PRIM = 48                   # triangle record (v0, e1, e2)
-# shading, per extension segment: queue index 4, ray 24, hit 8, throughput /
-# radiance / flags 28, primitive info + geometry 24, material 32, MIS vertex 16
-# read; radiance, throughput, ray, flags, MIS vertex, queue index 72 written
-B_SHADE_SEG = 136 + 72
+# shading, per extension segment: the live-path record (ray 24, hit 8,
+# throughput 12, flags 4, MIS pdf + vertex 16, slot 4 = 68) plus primitive
+# info + geometry 24 and material 32 read; the survivor's record (68) written
+B_SHADE_SEG = 68 + 24 + 32 + 68
 # shading, per shadow ray written: the queue entry
 B_SHADE_SHADOW = B_SHADOW_RAY
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_glass_hdri.txt")

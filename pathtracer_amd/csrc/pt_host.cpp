@@ -863,12 +863,22 @@ extern "C" pt_status pt_scene_get_info(const pt_scene *s, pt_scene_info *info) {
 // ------------------------------------------------------------------ wavefront buffers
 static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
   if (s->wave_cap >= cap && s->w_buf.p) return PT_OK;
-  const size_t per_slot_f = 21;  // 4-byte words per slot (the arrays taken below)
+  const size_t per_state = 17;   // 4-byte words per entry of one live-path buffer
+  const size_t per_slot = 3;     // radiance per slot
   const size_t per_shadow = 11;  // 4-byte words per shadow-queue entry
-  size_t scap = 4 * (size_t)cap;
-  size_t bytes = (per_slot_f * cap + per_shadow * scap) * 4 + (per_slot_f + per_shadow) * 256;
+  const size_t n_arrays = 2 * per_state + per_slot + per_shadow;
+  // Partition p of a live-path buffer starts with at most R * ceil(n / (R P))
+  // paths (runs of R slots dealt round-robin) and never grows; its shadow
+  // rays, at most 4 per path, stay in partition p.
+  const uint64_t run = PT_PART_RUN;
+  const uint64_t part_cap = ((uint64_t)cap + run * PT_PARTS - 1) / (run * PT_PARTS) * run;
+  const uint64_t shadow_part_cap = 4 * part_cap;
+  if (shadow_part_cap * PT_PARTS >= (1ull << 32)) return fail(PT_ERR_INVALID, "pt_render: wavefront too large");
+  const size_t lcap = (size_t)(part_cap * PT_PARTS);
+  const size_t scap = (size_t)(shadow_part_cap * PT_PARTS);
+  size_t bytes = (2 * per_state * lcap + per_slot * (size_t)cap + per_shadow * scap) * 4 + n_arrays * 256;
   HIP_TRY(s->w_buf.alloc(bytes));
-  HIP_TRY(s->w_counts.alloc(64 * sizeof(uint32_t)));
+  HIP_TRY(s->w_counts.alloc(PT_COUNTER_WORDS * sizeof(uint32_t)));
   HIP_TRY(s->w_stats.alloc(CNT_N * sizeof(unsigned long long)));
   char *p = s->w_buf.as<char>();
   auto take = [&](size_t n) {
@@ -877,16 +887,18 @@ static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
     return r;
   };
   Wave &w = s->wave;
-  w.ox = (float *)take(cap); w.oy = (float *)take(cap); w.oz = (float *)take(cap);
-  w.dx = (float *)take(cap); w.dy = (float *)take(cap); w.dz = (float *)take(cap);
-  w.t = (float *)take(cap); w.prim = (int *)take(cap);
-  w.bx = (float *)take(cap); w.by = (float *)take(cap); w.bz = (float *)take(cap);
+  for (int b = 0; b < 2; ++b) {
+    PathState &ps = w.st[b];
+    ps.ox = (float *)take(lcap); ps.oy = (float *)take(lcap); ps.oz = (float *)take(lcap);
+    ps.dx = (float *)take(lcap); ps.dy = (float *)take(lcap); ps.dz = (float *)take(lcap);
+    ps.t = (float *)take(lcap); ps.prim = (int *)take(lcap);
+    ps.bx = (float *)take(lcap); ps.by = (float *)take(lcap); ps.bz = (float *)take(lcap);
+    ps.ppdf = (float *)take(lcap);
+    ps.px = (float *)take(lcap); ps.py = (float *)take(lcap); ps.pz = (float *)take(lcap);
+    ps.flags = (uint32_t *)take(lcap);
+    ps.slot = (uint32_t *)take(lcap);
+  }
   w.lx = (float *)take(cap); w.ly = (float *)take(cap); w.lz = (float *)take(cap);
-  w.ppdf = (float *)take(cap);
-  w.px = (float *)take(cap); w.py = (float *)take(cap); w.pz = (float *)take(cap);
-  w.flags = (uint32_t *)take(cap);
-  w.q[0] = (uint32_t *)take(cap);
-  w.q[1] = (uint32_t *)take(cap);
   w.sox = (float *)take(scap); w.soy = (float *)take(scap); w.soz = (float *)take(scap);
   w.sdx = (float *)take(scap); w.sdy = (float *)take(scap); w.sdz = (float *)take(scap);
   w.stmax = (float *)take(scap);
@@ -896,7 +908,8 @@ static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
   w.counts = s->w_counts.as<uint32_t>();
   w.stats = s->w_stats.as<unsigned long long>();
   w.cap = cap;
-  w.shadow_cap = (uint32_t)scap;
+  w.part_cap = (uint32_t)part_cap;
+  w.shadow_part_cap = (uint32_t)shadow_part_cap;
   s->wave_cap = cap;
   return PT_OK;
 }
@@ -955,7 +968,7 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
     st = ensure_perm(s, cfg->seed);
     if (st != PT_OK) return st;
   }
-  // 8M path slots in flight by default (~2.2 GB of wavefront state): enough rays
+  // 8M path slots in flight by default (~2.7 GB of wavefront state): enough rays
   // in the late, sparse bounces to keep 256 CUs busy.
   uint32_t want = cfg->wavefront ? cfg->wavefront : (1u << 23);
   uint64_t chunk_pix = std::max<uint64_t>(1, want / sppc);
@@ -1040,7 +1053,6 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
     rp.chunk_paths = n;
     int e = ev_begin(0);
     launch_generate(s->stream, s->ds, cam, rp, s->wave, s->d_off.as<float>(), s->d_off.as<float>() + spp);
-    launch_set_counts(s->stream, s->wave.counts, n, 0, 0);
     ev_end(e);
     int qi = 0;
     for (uint32_t d = 0; d < nverts; ++d) {
@@ -1049,8 +1061,6 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
       ev_end(e);
       ++launches;
       e = ev_begin(2);
-      launch_clear_count(s->stream, s->wave.counts, (uint32_t)(qi ^ 1));
-      launch_clear_count(s->stream, s->wave.counts, (uint32_t)CNT_SHADOW);
       launch_shade(s->stream, s->ds, rp, s->wave, qi, d, n, grid_cap);
       ev_end(e);
       e = ev_begin(3);

@@ -3,12 +3,15 @@
 // One render is a sequence of chunks of path samples (pixel-major order,
 // whole pixels per chunk).  Per chunk:
 //   k_generate                       camera rays for every slot
-//   per vertex d:  k_extend          closest hit of the active-ray queue
-//                  k_shade<EST>      material, emission, NEE shadow rays,
-//                                    continuation rays (queue compaction)
+//   per vertex d:  k_extend          closest hit of the live paths
+//                  k_shade<EST, MS>  material, emission, NEE shadow rays;
+//                                    survivors are appended (compacted) to
+//                                    the other live-path buffer
 //                  k_shadow          any-hit of the shadow queue, adds the
 //                                    unoccluded contributions
 //   k_film                           per-pixel sums (fixed order)
+// Live paths are stored densely, so extension and shading read and write
+// them with unit stride; only radiance is addressed by slot.
 // The estimator is getRayColor of integrator/maxdepth_nee.hpp:9-256
 // unrolled into a loop (EST 0), or NEE+MIS (EST 1).
 #include "pt_device.h"
@@ -19,10 +22,30 @@ namespace pt {
 __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
 // ------------------------------------------------------------------ generate
+// Camera rays for every slot of the chunk.  Slots are dealt to the live-path
+// partitions in runs of R = 64 (one wave's worth, a pixel's samples at 64
+// spp): partition p takes runs p, p + P, p + 2P, ..., so every partition holds
+// an even mix of the chunk's pixels while each wave starts coherent.  Thread
+// t writes entry t % m of partition t / m (m = R * ceil(n / (R P))), so the
+// writes are unit-stride.  Also zeroes the radiance and sets the counters.
 __global__ void __launch_bounds__(256) k_generate(DevScene s, DevCamera cam, RenderParams rp, Wave w,
                                                   const float *offx, const float *offy) {
-  uint32_t slot = gtid();
-  if (slot >= rp.chunk_paths) return;
+  const uint32_t t = gtid(), n = rp.chunk_paths;
+  const uint32_t per = (n + PT_PART_RUN * PT_PARTS - 1) / (PT_PART_RUN * PT_PARTS) * PT_PART_RUN;
+  if (t < n) {
+    w.lx[t] = 0.f; w.ly[t] = 0.f; w.lz[t] = 0.f;
+  }
+  if (t >= per * PT_PARTS) return;
+  const uint32_t part = t / per, l = t % per;
+  if (l == 0) {
+    const uint32_t full = n / (PT_PART_RUN * PT_PARTS), rest = n - full * PT_PART_RUN * PT_PARTS;
+    const uint32_t extra = rest > part * PT_PART_RUN ? min(rest - part * PT_PART_RUN, (uint32_t)PT_PART_RUN) : 0u;
+    *w.live(0, part) = full * PT_PART_RUN + extra;
+    *w.live(1, part) = 0u;
+  }
+  const uint32_t slot = ((l / PT_PART_RUN) * PT_PARTS + part) * PT_PART_RUN + l % PT_PART_RUN;
+  if (slot >= n) return;
+  const uint32_t e = part * w.part_cap + l;
   uint64_t q = rp.chunk_first + slot;
   uint32_t pixel = (uint32_t)(q / rp.spp_count);
   uint32_t k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
@@ -33,25 +56,14 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s, DevCamera cam, Ren
   f3 o, d;
   camera_ray(cam, u, v, pt_rng(g, rslot(0, SLOT_LENS_X), rp.seed), pt_rng(g, rslot(0, SLOT_LENS_Y), rp.seed),
              pt_rng(g, rslot(0, SLOT_TIME), rp.seed), o, d);
-  w.ox[slot] = o.x; w.oy[slot] = o.y; w.oz[slot] = o.z;
-  w.dx[slot] = d.x; w.dy[slot] = d.y; w.dz[slot] = d.z;
-  w.bx[slot] = 1.f; w.by[slot] = 1.f; w.bz[slot] = 1.f;
-  w.lx[slot] = 0.f; w.ly[slot] = 0.f; w.lz[slot] = 0.f;
-  w.flags[slot] = 1u;
-  w.ppdf[slot] = 0.f;
-  w.px[slot] = o.x; w.py[slot] = o.y; w.pz[slot] = o.z;
-  w.q[0][slot] = slot;
-}
-
-__global__ void k_set_counts(uint32_t *counts, uint32_t q0, uint32_t q1, uint32_t sh) {
-  if (threadIdx.x == 0) {
-    counts[0] = q0;
-    counts[1] = q1;
-    counts[2] = sh;
-  }
-}
-__global__ void k_clear_u32(uint32_t *p, uint32_t idx) {
-  if (threadIdx.x == 0) p[idx] = 0u;
+  const PathState &ps = w.st[0];
+  ps.ox[e] = o.x; ps.oy[e] = o.y; ps.oz[e] = o.z;
+  ps.dx[e] = d.x; ps.dy[e] = d.y; ps.dz[e] = d.z;
+  ps.bx[e] = 1.f; ps.by[e] = 1.f; ps.bz[e] = 1.f;
+  ps.flags[e] = 1u;
+  ps.ppdf[e] = 0.f;
+  ps.px[e] = o.x; ps.py[e] = o.y; ps.pz[e] = o.z;
+  ps.slot[e] = slot;
 }
 
 // ------------------------------------------------------------------ extend
@@ -73,20 +85,26 @@ __global__ void k_clear_u32(uint32_t *p, uint32_t idx) {
 template <bool COUNT>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_extend(DevScene s, Wave w, int qi) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_EXT, true)];
-  const uint32_t n = w.counts[qi];
-  const uint32_t *queue = w.q[qi];
+  const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
+  const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
+  const PathState &ps = w.st[qi];
+  // the shading that follows appends to the other buffer and the shadow queue
+  if (blockIdx.x == 0 && threadIdx.x < PT_PARTS) {
+    *w.live(qi ^ 1, threadIdx.x) = 0u;
+    *w.shadow(threadIdx.x) = 0u;
+  }
   uint32_t nn = 0, np = 0, segs = 0;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+  for (uint32_t i0 = (blockIdx.x / PT_PARTS) * blockDim.x; i0 < n; i0 += nb * blockDim.x) {
     uint32_t i = i0 + threadIdx.x;
     if (i < n) {
-      uint32_t slot = queue[i];
-      f3 o = mk3(w.ox[slot], w.oy[slot], w.oz[slot]);
-      f3 d = mk3(w.dx[slot], w.dy[slot], w.dz[slot]);
+      i += base;
+      f3 o = mk3(ps.ox[i], ps.oy[i], ps.oz[i]);
+      f3 d = mk3(ps.dx[i], ps.dy[i], ps.dz[i]);
       float t;
       int prim;
       traverse<false, COUNT, PT_K_EXT>(s, lds, o, d, 1e-4f, 1e5f, t, prim, nn, np);
-      w.t[slot] = t;
-      w.prim[slot] = prim;
+      ps.t[i] = t;
+      ps.prim[i] = prim;
       ++segs;
     }
   }
@@ -101,11 +119,15 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
 template <bool COUNT>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_shadow(DevScene s, Wave w) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_SHADOW, false)];
-  const uint32_t n = w.counts[CNT_SHADOW];
+  // blocks b, b + P, b + 2P, ... share partition p = b % P (the grid is a multiple of P)
+  const uint32_t p = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
+  const uint32_t n = min(*w.shadow(p), w.shadow_part_cap);
+  const uint32_t base = p * w.shadow_part_cap;
   uint32_t nn = 0, np = 0, rays = 0;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+  for (uint32_t i0 = (blockIdx.x / PT_PARTS) * blockDim.x; i0 < n; i0 += nb * blockDim.x) {
     uint32_t i = i0 + threadIdx.x;
     if (i < n) {
+      i += base;
       f3 o = mk3(w.sox[i], w.soy[i], w.soz[i]);
       f3 d = mk3(w.sdx[i], w.sdy[i], w.sdz[i]);
       float t;
@@ -128,11 +150,16 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
 }
 
 // ------------------------------------------------------------------ shade
-// Appends one shadow ray.  Called from divergent code: the compiler turns the
-// uniform-address atomic into one atomic per wave (lane-ordered offsets).
+// Appends one shadow ray to this block's partition of the shadow queue.
+// Called from divergent code: the compiler turns the uniform-address atomic
+// into one atomic per wave (lane-ordered offsets).  A partition holds 4
+// entries per live-path entry of the same partition, so the capacity check
+// never drops a ray.
 PT_D void emit_shadow(const Wave &w, f3 p, f3 d, float tmax, f3 c, uint32_t slot) {
-  uint32_t idx = atomicAdd(&w.counts[CNT_SHADOW], 1u);
-  if (idx < w.shadow_cap) {
+  const uint32_t part = blockIdx.x % PT_PARTS;
+  uint32_t idx = atomicAdd(w.shadow(part), 1u);
+  if (idx < w.shadow_part_cap) {
+    idx += part * w.shadow_part_cap;
     w.sox[idx] = p.x; w.soy[idx] = p.y; w.soz[idx] = p.z;
     w.sdx[idx] = d.x; w.sdy[idx] = d.y; w.sdz[idx] = d.z;
     w.stmax[idx] = tmax;
@@ -155,32 +182,38 @@ PT_D float power_w(float a, float b) { return (a * a) / (a * a + b * b); }
 #endif
 template <int EST, uint32_t MS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHADE_WAVES))) k_shade(DevScene s, RenderParams rp, Wave w, int qi, uint32_t d) {
-  const uint32_t n = w.counts[qi];
-  const uint32_t *qin = w.q[qi];
-  uint32_t *qout = w.q[qi ^ 1];
-  uint32_t *cout = &w.counts[qi ^ 1];
+  const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
+  const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
+  const PathState &in = w.st[qi];
+  const PathState &out = w.st[qi ^ 1];
+  uint32_t *cout = w.live(qi ^ 1, part);
   const uint32_t D = rp.max_depth;
   const int N = s.n_lights;
-  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+  for (uint32_t i0 = (blockIdx.x / PT_PARTS) * blockDim.x; i0 < n; i0 += nb * blockDim.x) {
     uint32_t i = i0 + threadIdx.x;
     bool active = i < n;
+    i += base;
     bool cont = false;
     uint32_t slot = 0;
-    f3 p = s3(0.f), wi = s3(0.f);
+    f3 p = s3(0.f), wi = s3(0.f), beta = s3(0.f), pprev = s3(0.f);
+    float ppdf = 0.f;
+    uint32_t fl = 0u;
     if (active) {
-      slot = qin[i];
+      slot = in.slot[i];
       uint64_t q = rp.chunk_first + slot;
       uint32_t pixel = (uint32_t)(q / rp.spp_count);
       uint32_t k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
       uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
       uint32_t g = 1u + k + rp.spp * (pj + rp.height * pi);
-      f3 o = mk3(w.ox[slot], w.oy[slot], w.oz[slot]);
-      f3 dir = mk3(w.dx[slot], w.dy[slot], w.dz[slot]);
-      float t = w.t[slot];
-      int prim = w.prim[slot];
-      f3 beta = mk3(w.bx[slot], w.by[slot], w.bz[slot]);
-      f3 L = mk3(w.lx[slot], w.ly[slot], w.lz[slot]);
-      uint32_t fl = w.flags[slot];
+      f3 o = mk3(in.ox[i], in.oy[i], in.oz[i]);
+      f3 dir = mk3(in.dx[i], in.dy[i], in.dz[i]);
+      float t = in.t[i];
+      int prim = in.prim[i];
+      beta = mk3(in.bx[i], in.by[i], in.bz[i]);
+      fl = in.flags[i];
+      ppdf = in.ppdf[i];
+      pprev = mk3(in.px[i], in.py[i], in.pz[i]);
+      f3 L = s3(0.f);  // radiance this vertex adds (emission); NEE goes through shadow rays
       bool prev_spec = fl & 1u;
       bool done = false;
       if (prim < 0) {
@@ -190,7 +223,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
           if (d < D) L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
         } else if (s.has_env) {
           float pl = env_pdf(s, dir) / (float)N;
-          L = add3(L, scl3(mul3(beta, env_lookup(s, dir)), power_w(w.ppdf[slot], pl)));
+          L = add3(L, scl3(mul3(beta, env_lookup(s, dir)), power_w(ppdf, pl)));
         } else {
           L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
         }
@@ -214,8 +247,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
             // normal follows its vertex order, the sphere's points outwards
             f3 nl = (Lg.type == LIGHT_QUAD) ? Lg.n : h.ng;
             f3 le = (dot3(nl, neg3(norm3(dir))) > 0.f) ? Lg.le : s3(0.f);
-            float pl = light_pdf_sa(s, Lg, mk3(w.px[slot], w.py[slot], w.pz[slot]), h.p) / (float)N;
-            L = add3(L, scl3(mul3(beta, le), power_w(w.ppdf[slot], pl)));
+            float pl = light_pdf_sa(s, Lg, pprev, h.p) / (float)N;
+            L = add3(L, scl3(mul3(beta, le), power_w(ppdf, pl)));
           }
           done = true;
         }
@@ -331,8 +364,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
             cont = d + 1u < D;  // the emitter check of the next hit happens at the next vertex
           } else {
             cont = d + 1u <= D;
-            w.ppdf[slot] = sc.pdf;
-            w.px[slot] = p.x; w.py[slot] = p.y; w.pz[slot] = p.z;
+            ppdf = sc.pdf;
+            pprev = p;
           }
         }
         if (cont && rp.rr_depth && d >= rp.rr_depth) {
@@ -341,17 +374,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
           else beta = divs3(beta, qs);
         }
       }
-      w.lx[slot] = L.x; w.ly[slot] = L.y; w.lz[slot] = L.z;
-      if (cont) {
-        w.bx[slot] = beta.x; w.by[slot] = beta.y; w.bz[slot] = beta.z;
-        w.ox[slot] = p.x; w.oy[slot] = p.y; w.oz[slot] = p.z;
-        w.dx[slot] = wi.x; w.dy[slot] = wi.y; w.dz[slot] = wi.z;
-        w.flags[slot] = fl;
+      // this path's slot is written by no other lane; this vertex's shadow rays
+      // add to it only after this kernel
+      if (L.x != 0.f || L.y != 0.f || L.z != 0.f || !finite3(L)) {
+        w.lx[slot] += L.x;
+        w.ly[slot] += L.y;
+        w.lz[slot] += L.z;
       }
     }
-    // converged: append the continuation
-    uint32_t idx = wave_append(cout, cont);
-    if (cont) qout[idx] = slot;
+    // converged: append the continuing paths to the same partition of the other buffer
+    uint32_t j = base + wave_append(cout, cont);
+    if (cont) {
+      out.ox[j] = p.x; out.oy[j] = p.y; out.oz[j] = p.z;
+      out.dx[j] = wi.x; out.dy[j] = wi.y; out.dz[j] = wi.z;
+      out.bx[j] = beta.x; out.by[j] = beta.y; out.bz[j] = beta.z;
+      out.flags[j] = fl;
+      out.ppdf[j] = ppdf;
+      out.px[j] = pprev.x; out.py[j] = pprev.y; out.pz[j] = pprev.z;
+      out.slot[j] = slot;
+    }
   }
 }
 
@@ -476,21 +517,24 @@ __global__ void k_light_sample(DevScene s, uint32_t light, const float *u2, cons
 
 // ------------------------------------------------------------------ launchers
 static inline uint32_t blocks_for(size_t n, uint32_t bs = 256) { return (uint32_t)((n + bs - 1) / bs); }
+// Grid of a partitioned kernel: enough blocks for max_n entries spread over the
+// partitions, at most grid_cap, and a multiple of PT_PARTS so that every
+// partition gets the same number of blocks.
+static inline uint32_t part_grid(size_t max_n, uint32_t grid_cap) {
+  uint32_t per_part = blocks_for((max_n + PT_PARTS - 1) / PT_PARTS);
+  uint32_t g = std::min<uint64_t>((uint64_t)per_part * PT_PARTS, grid_cap);
+  return std::max<uint32_t>(PT_PARTS, g / PT_PARTS * PT_PARTS);
+}
 
 void launch_generate(hipStream_t st, const DevScene &s, const DevCamera &c, const RenderParams &rp, const Wave &w,
                      const float *offx, const float *offy) {
-  hipLaunchKernelGGL(k_generate, dim3(blocks_for(rp.chunk_paths)), dim3(256), 0, st, s, c, rp, w, offx, offy);
-}
-void launch_set_counts(hipStream_t st, uint32_t *counts, uint32_t q0, uint32_t q1, uint32_t sh) {
-  hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, counts, q0, q1, sh);
-}
-void launch_clear_count(hipStream_t st, uint32_t *counts, uint32_t idx) {
-  hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, counts, idx);
+  const size_t per = (rp.chunk_paths + PT_PART_RUN * PT_PARTS - 1) / (PT_PART_RUN * PT_PARTS) * PT_PART_RUN;
+  const size_t threads = std::max<size_t>(per * PT_PARTS, rp.chunk_paths);
+  hipLaunchKernelGGL(k_generate, dim3(blocks_for(threads)), dim3(256), 0, st, s, c, rp, w, offx, offy);
 }
 void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uint32_t max_n, uint32_t grid_cap,
                    bool count) {
-  uint32_t g = std::min(blocks_for(max_n), grid_cap);
-  if (g == 0) g = 1;
+  uint32_t g = part_grid(max_n, grid_cap);
   if (count)
     hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi);
   else
@@ -498,8 +542,7 @@ void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uin
 }
 void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
                   uint32_t max_n, uint32_t grid_cap) {
-  uint32_t g = std::min(blocks_for(max_n), grid_cap);
-  if (g == 0) g = 1;
+  uint32_t g = part_grid(max_n, grid_cap);
   // the basic material set when the scene uses nothing else (fewer registers)
   const bool basic = (s.mat_kinds & ~kMatBasic) == 0u;
   if (rp.estimator == PT_EST_NEE_MIS) {
@@ -515,8 +558,7 @@ void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, con
   }
 }
 void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count) {
-  uint32_t g = std::min(blocks_for(max_n), grid_cap);
-  if (g == 0) g = 1;
+  uint32_t g = part_grid(max_n, grid_cap);
   if (count)
     hipLaunchKernelGGL(k_shadow<true>, dim3(g), dim3(PT_BLOCK), 0, st, s, w);
   else

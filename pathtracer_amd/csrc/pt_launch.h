@@ -7,8 +7,6 @@ namespace pt {
 
 void launch_generate(hipStream_t st, const DevScene &s, const DevCamera &c, const RenderParams &rp, const Wave &w,
                      const float *offx, const float *offy);
-void launch_set_counts(hipStream_t st, uint32_t *counts, uint32_t q0, uint32_t q1, uint32_t sh);
-void launch_clear_count(hipStream_t st, uint32_t *counts, uint32_t idx);
 void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uint32_t max_n, uint32_t grid_cap,
                    bool count);
 void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
