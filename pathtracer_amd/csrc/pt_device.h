@@ -276,6 +276,164 @@ PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tm
   prim_out = best;
   return best >= 0;
 }
+
+// ---------------------------------------------------------------------
+// Persistent traversal with per-lane ray refill (the "while-while with
+// dynamic fetch" of Aila & Laine, HPG 2009, on 64-wide waves).  A wave owns
+// the contiguous ray range [begin, end) and runs one traversal step (one
+// node or one leaf) per iteration; lanes whose rays are finished take the
+// next rays of the range once REFILL of them are idle, so lanes do not idle
+// while the wave's longest ray finishes.  The per-ray results equal traverse<>'s:
+// the same node order, the same culling and the same primitive tests.
+// fetch(i, o, d, tmin, tmax) loads ray i; finish(i, hit, t, prim) stores it.
+// begin, end must be the same in every lane of the wave.
+// ---------------------------------------------------------------------
+template <bool ANY, bool COUNT, int K, int REFILL, class Fetch, class Finish>
+PT_D void trace_range(const DevScene &s, int *lds, uint32_t begin, uint32_t end, Fetch fetch, Finish finish,
+                      uint32_t &n_nodes, uint32_t &n_prims) {
+  constexpr bool WT = !ANY;
+  int pc[PT_STACK4];
+  float ptv[WT ? PT_STACK4 : 1];
+  int *const lc = lds + threadIdx.x;
+  float *const lt = reinterpret_cast<float *>(lds + K * PT_BLOCK) + threadIdx.x;
+  const uint32_t lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t next = begin;
+  int ray = -1;
+  f3 o = s3(0.f), d = s3(0.f);
+  float ix = 0.f, iy = 0.f, iz = 0.f, oix = 0.f, oiy = 0.f, oiz = 0.f, tmin = 0.f, tb = 0.f;
+  int best = -1, cur = 0, sp = 0, lo = 0;
+  auto push = [&](int c, float t) {
+    if constexpr (K > 0) {
+      if (sp - lo == K) {
+        const int s0 = (lo & (K - 1)) * PT_BLOCK;
+        pc[lo] = lc[s0];
+        if (WT) ptv[lo] = lt[s0];
+        ++lo;
+      }
+      const int s1 = (sp & (K - 1)) * PT_BLOCK;
+      lc[s1] = c;
+      if (WT) lt[s1] = t;
+    } else {
+      pc[sp] = c;
+      if (WT) ptv[sp] = t;
+    }
+    ++sp;
+  };
+  auto pop = [&](int &c, float &t) {
+    --sp;
+    if constexpr (K > 0) {
+      if (sp >= lo) {
+        const int s1 = (sp & (K - 1)) * PT_BLOCK;
+        c = lc[s1];
+        t = WT ? lt[s1] : 0.f;
+        return;
+      }
+      lo = sp;
+    }
+    c = pc[sp];
+    t = WT ? ptv[sp] : 0.f;
+  };
+  while (true) {
+    // refill the lanes that have no ray once at least REFILL of them are idle
+    // (or all are): waiting for a group keeps the new rays of a wave adjacent
+    // (wave-uniform bookkeeping)
+    const unsigned long long idle = __ballot(ray < 0);
+    const int n_idle = __popcll(idle);
+    if (next < end && (n_idle >= REFILL || n_idle == 64)) {
+      const uint32_t mine = next + (uint32_t)__popcll(idle & below);
+      if (ray < 0 && mine < end) {
+        ray = (int)mine;
+        float tmax;
+        fetch(mine, o, d, tmin, tmax);
+        ix = rcp_fast(d.x); iy = rcp_fast(d.y); iz = rcp_fast(d.z);
+        oix = o.x * ix; oiy = o.y * iy; oiz = o.z * iz;
+        tb = tmax;
+        best = -1;
+        cur = s.root;
+        sp = 0;
+        lo = 0;
+      }
+      next = min(end, next + (uint32_t)__popcll(idle));
+    }
+    if (__ballot(ray >= 0) == 0ull) break;
+    if (ray < 0) continue;
+    bool need_pop = true;
+    bool done = false;
+    if (cur >= 0) {
+      const float4 *nd = s.nodes + 8 * (size_t)cur;
+      const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+      const int4 ch = *reinterpret_cast<const int4 *>(nd + 6);
+      if (COUNT) ++n_nodes;
+      float tn[4], tf[4];
+      const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float a0 = fmaf(lxs[k], ix, -oix), a1 = fmaf(hxs[k], ix, -oix);
+        float b0 = fmaf(lys[k], iy, -oiy), b1 = fmaf(hys[k], iy, -oiy);
+        float c0 = fmaf(lzs[k], iz, -oiz), c1 = fmaf(hzs[k], iz, -oiz);
+        tn[k] = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
+        tf[k] = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tb));
+      }
+      const int cs[4] = {ch.x, ch.y, ch.z, ch.w};
+      int c0 = cs[0], c1 = cs[1], c2 = cs[2], c3 = cs[3];
+      const bool h0 = tn[0] <= tf[0] && cs[0] != kEmptyChild;
+      const bool h1 = tn[1] <= tf[1] && cs[1] != kEmptyChild;
+      const bool h2 = tn[2] <= tf[2] && cs[2] != kEmptyChild;
+      const bool h3 = tn[3] <= tf[3] && cs[3] != kEmptyChild;
+      float t0 = h0 ? tn[0] : INFINITY, t1 = h1 ? tn[1] : INFINITY;
+      float t2 = h2 ? tn[2] : INFINITY, t3 = h3 ? tn[3] : INFINITY;
+      const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+      if (nh > 0) {
+        PT_CSWAP(t0, c0, t1, c1);
+        PT_CSWAP(t2, c2, t3, c3);
+        PT_CSWAP(t0, c0, t2, c2);
+        PT_CSWAP(t1, c1, t3, c3);
+        PT_CSWAP(t1, c1, t2, c2);
+        if (nh >= 4) push(c3, t3);
+        if (nh >= 3) push(c2, t2);
+        if (nh >= 2) push(c1, t1);
+        cur = c0;
+        need_pop = false;
+      }
+    } else {
+      const uint32_t st = leaf_start(cur), cn = leaf_count(cur);
+      for (uint32_t q = st; q < st + cn; ++q) {
+        float t;
+        uint32_t id;
+        if (COUNT) ++n_prims;
+        if (hit_prim(s.prims, q, o, d, tmin, tb, t, id)) {
+          tb = t;
+          best = (int)id;
+          if (ANY) {
+            done = true;
+            break;
+          }
+        }
+      }
+    }
+    if (need_pop && !done) {
+      bool found = false;
+      while (sp > 0) {
+        int c;
+        float te;
+        pop(c, te);
+        if (ANY || te <= tb) {
+          cur = c;
+          found = true;
+          break;
+        }
+      }
+      if (!found) done = true;
+    }
+    if (done) {
+      finish((uint32_t)ray, best >= 0, tb, best);
+      ray = -1;
+    }
+  }
+}
 #undef PT_CSWAP
 
 // ---------------------------------------------------------------------

@@ -1024,7 +1024,13 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   DevCamera cam = make_camera(s->cam);
   const bool count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
   const bool timing = (cfg->flags & PT_RENDER_TIMING) != 0;
-  const uint32_t grid_cap = s->n_cu * 32;  // 8 waves per CU of 256-thread blocks, x4
+  const uint32_t grid_cap = s->n_cu * 32;  // shading: 4 blocks of 256 threads per CU resident, x8
+#ifndef PT_TRAV_BLOCKS_PER_CU
+#define PT_TRAV_BLOCKS_PER_CU 8
+#endif
+  // traversal: one resident round (8 waves per SIMD = 8 blocks per CU), each wave
+  // refilling its lanes from its share of the rays
+  const uint32_t trav_cap = std::max<uint32_t>(PT_PARTS, s->n_cu * PT_TRAV_BLOCKS_PER_CU / PT_PARTS * PT_PARTS);
   const uint32_t nverts = cfg->estimator == PT_EST_NEE_MIS ? cfg->max_depth + 1 : cfg->max_depth;
 
   struct Ev { hipEvent_t a, b; int kind; };
@@ -1057,14 +1063,14 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
     int qi = 0;
     for (uint32_t d = 0; d < nverts; ++d) {
       e = ev_begin(1);
-      launch_extend(s->stream, s->ds, s->wave, qi, n, grid_cap, count);
+      launch_extend(s->stream, s->ds, s->wave, qi, n, trav_cap, count);
       ev_end(e);
       ++launches;
       e = ev_begin(2);
       launch_shade(s->stream, s->ds, rp, s->wave, qi, d, n, grid_cap);
       ev_end(e);
       e = ev_begin(3);
-      launch_shadow(s->stream, s->ds, s->wave, 4u * n, grid_cap, count);
+      launch_shadow(s->stream, s->ds, s->wave, 4u * n, trav_cap, count);
       ev_end(e);
       qi ^= 1;
     }
