@@ -79,8 +79,8 @@ def cpu_baseline(spec, cfg, target_s=15.0):
 
 def read_traffic(kernel, estimator):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary committed beside the profiles (None when absent or for another
-    kernel or estimator)."""
+    summary committed beside the profiles (None when absent, or measured for
+    another estimator)."""
     try:
         kv = {}
         with open(TRAFFIC_FILE) as f:
@@ -88,9 +88,9 @@ def read_traffic(kernel, estimator):
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
                     kv[k.strip()] = v.strip()
-        if kv.get("kernel") != kernel or int(kv.get("estimator", -1)) != estimator:
+        if int(kv.get("estimator", -1)) != estimator:
             return None
-        return int(float(kv["hbm_bytes_per_launch"]))
+        return int(float(kv[f"{kernel}.hbm_bytes_per_launch"]))
     except (OSError, KeyError, ValueError):
         return None
 
