@@ -111,8 +111,14 @@ enum pt_estimator {
                                non-specular vertex, emitter hits excluded after
                                non-specular vertices, env radiance also taken on
                                BSDF misses (maxdepth_nee.hpp:26-37,91-255)       */
-  PT_EST_NEE_MIS = 1        /* one light sample + BSDF sample under the power
+  PT_EST_NEE_MIS = 1,       /* one light sample + BSDF sample under the power
                                heuristic; environment sampled by its 2-D CDF     */
+  PT_EST_NAIVE = 2,         /* maxdepth_naive.hpp:9-75: no NEE; each bounce samples
+                               a uniform mixture of light_samples shape pdfs and
+                               the BSDF; misses and the depth limit return bg     */
+  PT_EST_ROULETTE = 3       /* roulette_naive.hpp:9-78: the same mixture with
+                               Russian roulette q = clamp(max(f), 0.5, 1) at every
+                               vertex; the depth limit returns 0                  */
 };
 
 typedef struct pt_render_cfg {

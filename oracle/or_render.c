@@ -29,7 +29,7 @@
 #include "or_scene.h"
 
 enum { SLOT_LENS_X = 0, SLOT_LENS_Y = 1, SLOT_TIME = 2, SLOT_AUX = 4, SLOT_RR = 5,
-       SLOT_MIS_LIGHT = 6, SLOT_NEE_OFF = 8, SLOT_NEE_LIGHT = 40 };
+       SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_NEE_LIGHT = 40 };
 static inline uint32_t slot(uint32_t d, uint32_t purpose) { return d * 64u + purpose; }
 
 typedef struct {
@@ -199,8 +199,103 @@ static float light_pdf_sa(const or_scene *s, int li, v3 o, v3 p) {
 
 static inline float power2(float a, float b) { return (a * a) / (a * a + b * b); }
 
+/* One path sample of the mixture-pdf integrators.  getRayColor of
+ * maxdepth_naive.hpp:9-75 (roulette = 0) and roulette_naive.hpp:9-78
+ * (roulette = 1), unrolled:
+ *   - a miss returns bg_color (the environment light is not consulted);
+ *   - the depth limit returns bg_color (naive) or 0 (roulette);
+ *   - Material::emit is added at every hit (DiffuseEmitter: two-sided);
+ *   - roulette: q = clamp(maxComponent(attenuation), 0.5, 1) right after a
+ *     successful scatter, the path ends when rand > q, else f /= q;
+ *   - specular lobes: throughput *= attenuation;
+ *   - otherwise wi is drawn from MixedPDF (mixedPDF.hpp) of light_samples
+ *     HittablePDFs towards uniformly chosen area lights (the
+ *     ObjectList(LightList) shapes, objectList.hpp:15-60) plus the
+ *     material's pdf, equal weights: throughput *= f cos / pdf_mix.
+ * The environment light is not among the light objects (it is appended to
+ * the LightList after them, renderManager.hpp:88-95).  Without area lights
+ * the mixture is the material's pdf alone (the reference would index an
+ * empty list). */
+static v3 trace_path_naive(const rctx *c, uint32_t i, uint32_t j, uint32_t k, int roulette, uint64_t *cnt) {
+  const or_scene *s = c->s;
+  const pt_render_cfg *cfg = c->cfg;
+  uint32_t g = 1u + k + c->spp * (j + cfg->height * i);
+  uint32_t pixel = j * cfg->width + i;
+  v3 ro, rd;
+  camera_ray(c, i, j, k, g, &ro, &rd);
+  v3 L = vs(0.f), beta = vs(1.f);
+  const v3 bg = V3(cfg->bg[0], cfg->bg[1], cfg->bg[2]);
+  const uint32_t D = cfg->max_depth;
+  const int NA = s->n_lights - (s->env ? 1 : 0);  /* area lights come first */
+  const uint32_t M = cfg->light_samples < 4u ? cfg->light_samples : 4u;
+  if (cnt) cnt[0]++;
+  for (uint32_t d = 0;; ++d) {
+    if (d >= D) {
+      if (!roulette) L = vadd(L, vmul(beta, bg));
+      break;
+    }
+    or_hit h;
+    if (cnt) cnt[1]++;
+    if (!or_scene_intersect(s, ro, rd, 1e-4f, 1e5f, 0, &h, cnt ? cnt + 4 : NULL)) {
+      L = vadd(L, vmul(beta, bg));
+      break;
+    }
+    const pt_material_desc *m = &s->mats[h.mat];
+    float uc = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d);
+    float u0 = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d + 1);
+    float u1 = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d + 2);
+    float aux = or_rng(g, slot(d, SLOT_AUX), cfg->seed);
+    or_srec sr;
+    if (!or_scatter(m, rd, &h, uc, u0, u1, aux, &sr)) {
+      L = vadd(L, vmul(beta, or_emit(m, rd, &h)));
+      break;
+    }
+    float q = 1.f;
+    if (roulette) {
+      q = fminf(fmaxf(vmaxc(sr.att), 0.5f), 1.f);
+      if (or_rng(g, slot(d, SLOT_RR), cfg->seed) > q) break;
+    }
+    L = vadd(L, vmul(beta, or_emit(m, rd, &h)));
+    if (sr.flags & OR_SPECULAR) {
+      beta = vmul(beta, vdivs(sr.att, q));
+      ro = h.p;
+      rd = sr.wi;
+      continue;
+    }
+    v3 wo = vneg(vnorm(rd));
+    /* the light objects of the mixture, then one component */
+    int li[4] = {0, 0, 0, 0};
+    const uint32_t nl = NA > 0 ? M : 0u;
+    for (uint32_t si = 0; si < nl; ++si)
+      li[si] = (int)(or_rng(g, slot(d, SLOT_NEE_LIGHT + si), cfg->seed) * (float)NA) % NA;
+    const uint32_t ncomp = nl + 1u;
+    uint32_t comp = (uint32_t)floorf(or_rng(g, slot(d, SLOT_MIX), cfg->seed) * (float)ncomp) % ncomp;
+    v3 wi;
+    if (comp < nl) {
+      float e0 = or_rng(g, slot(d, SLOT_NEE_OFF), cfg->seed);
+      float e1 = or_rng(g, slot(d, SLOT_NEE_OFF + 1), cfg->seed);
+      wi = vnorm(or_shape_sample(s, &s->lights[li[comp]], h.p, e0, e1));
+    } else {
+      wi = vnorm(sr.wi);
+    }
+    float pdf = 0.f;
+    const float wgt = 1.f / (float)ncomp;
+    for (uint32_t si = 0; si < nl; ++si) pdf += wgt * or_shape_pdf(s, &s->lights[li[si]], h.p, wi);
+    pdf += wgt * or_bsdf_pdf(m, wo, wi, &h);
+    v3 f = vdivs(or_bsdf_f(m, wo, wi, &h), q);
+    float cs = fabsf(vdot(h.n, wi));
+    beta = vscale(vmul(beta, f), cs / pdf);
+    ro = h.p;
+    rd = wi;
+    if (!(beta.x != 0.f || beta.y != 0.f || beta.z != 0.f)) break;  /* nothing more can arrive */
+  }
+  return L;
+}
+
 /* One path sample.  Returns the radiance. */
 static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t *cnt) {
+  if (c->cfg->estimator == PT_EST_NAIVE || c->cfg->estimator == PT_EST_ROULETTE)
+    return trace_path_naive(c, i, j, k, c->cfg->estimator == PT_EST_ROULETTE, cnt);
   const or_scene *s = c->s;
   const pt_render_cfg *cfg = c->cfg;
   uint32_t g = 1u + k + c->spp * (j + cfg->height * i);

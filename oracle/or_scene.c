@@ -290,6 +290,7 @@ int or_scene_commit(or_scene *s) {
       L->n = vnorm(vcross(e1, e2));
       L->area = vlen(vcross(e1, e2));
       L->le = vscale(V3(md->p[0], md->p[1], md->p[2]), md->p[3]);
+      L->src = i;
       r->light = s->n_lights++;
     }
     for (int i = 0; i < s->n_spheres; ++i) {
@@ -303,6 +304,7 @@ int or_scene_commit(or_scene *s) {
       L->r = sp->r;
       L->area = (float)(4 * M_PI * sp->r * sp->r);
       L->le = vscale(V3(md->p[0], md->p[1], md->p[2]), md->p[3]);
+      L->src = i;
       sp->light = s->n_lights++;
     }
   }
@@ -678,6 +680,54 @@ int or_env_sample_batch(const or_scene *s, const float *u2, size_t n, float *dir
     dir[3 * i] = d.x; dir[3 * i + 1] = d.y; dir[3 * i + 2] = d.z;
   }
   return 0;
+}
+
+/* Hittable::pdf_value of the shape behind an area light, from o towards the
+ * unit direction wi: AARectangle (primitive.hpp:244-256) d^2/(|cos| A) with
+ * the hit's t, Sphere (primitive.hpp:172-184) 1/(2 pi (1 - cos theta_max));
+ * 0 when the ray misses the shape (default tmin 1e-4, tmax 1e4). */
+float or_shape_pdf(const or_scene *s, const or_light *L, v3 o, v3 wi) {
+  float t;
+  if (L->type == OR_LIGHT_QUAD) {
+    const or_rect *r = &s->rects[L->src];
+    if (!rect_hit(r, o, wi, 1e-4f, 1e4f, &t)) return 0.f;
+    float area = (r->d0max - r->d0min) * (r->d1max - r->d1min);
+    float d2 = t * t * vdot(wi, wi);
+    v3 n = r->axis == 0 ? V3(1, 0, 0) : (r->axis == 1 ? V3(0, 1, 0) : V3(0, 0, 1));
+    float cosine = fabsf(vdot(wi, n) / vlen(wi));
+    return d2 / (cosine * area);
+  }
+  if (L->type == OR_LIGHT_SPHERE) {
+    const or_sphere *sp = &s->spheres[L->src];
+    if (!sphere_hit(sp, o, wi, 1e-4f, 1e4f, &t)) return 0.f;
+    v3 oc = vsub(sp->c, o);
+    float ctm = sqrtf(1.f - sp->r * sp->r / vdot(oc, oc));
+    return (float)(1.0 / (2.0 * M_PI * (double)(1.f - ctm)));
+  }
+  return 0.f;
+}
+
+/* Hittable::get_sample of that shape from o (not normalised): a uniform
+ * point of the rect minus o (primitive.cpp:125-146, C_rand(a, b) = a + (b-a) u),
+ * or a direction in the cone the sphere subtends (primitive.hpp:186-192,
+ * randomToSphere random.hpp:36-44, OrthoBasis around c - o). */
+v3 or_shape_sample(const or_scene *s, const or_light *L, v3 o, float u, float v) {
+  if (L->type == OR_LIGHT_QUAD) {
+    const or_rect *r = &s->rects[L->src];
+    float a = r->d0min + (r->d0max - r->d0min) * u;
+    float b = r->d1min + (r->d1max - r->d1min) * v;
+    v3 pt = r->axis == 0 ? V3(r->k, a, b) : (r->axis == 1 ? V3(a, r->k, b) : V3(a, b, r->k));
+    return vsub(pt, o);
+  }
+  const or_sphere *sp = &s->spheres[L->src];
+  v3 dir = vsub(sp->c, o);
+  float d2 = vdot(dir, dir);
+  float z = 1.f + v * (sqrtf(1.f - sp->r * sp->r / d2) - 1.f);
+  float phi = 2.f * (float)M_PI * u;
+  float x = cosf(phi) * sqrtf(1.f - z * z);
+  float y = sinf(phi) * sqrtf(1.f - z * z);
+  onb_t b = onb_make(vnorm(dir));
+  return onb_to_world(&b, V3(x, y, z));
 }
 
 /* Light::get_sample / get_normal_at / L_emit (light/light.hpp:131-165,

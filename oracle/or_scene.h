@@ -34,6 +34,7 @@ typedef struct {
   v3 c;       /* sphere */
   float r, area;
   v3 le;      /* texture colour * intensity */
+  int src;    /* the rect (quad) or sphere it was discovered from */
 } or_light;
 
 struct or_scene {
@@ -69,4 +70,6 @@ v3 or_uniform_sphere(float u1, float u2);
 v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf);
 float or_env_pdf_dir(const or_scene *s, v3 dir);
 v3 or_light_point(const or_scene *s, const or_light *L, float u, float v, v3 *nrm);
+float or_shape_pdf(const or_scene *s, const or_light *L, v3 o, v3 wi);
+v3 or_shape_sample(const or_scene *s, const or_light *L, v3 o, float u, float v);
 #endif

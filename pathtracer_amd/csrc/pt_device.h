@@ -574,6 +574,51 @@ PT_D float env_pdf(const DevScene &s, f3 dir) {
   return s.env_func[(size_t)y * s.env_w + x] / s.env_integral / (2.f * PT_PI * PT_PI * st);
 }
 
+// Hittable::pdf_value of the shape behind an area light, from o towards the
+// unit direction wi (the mixture estimators): AARectangle (primitive.hpp:
+// 244-256) d^2/(|cos| A) with the hit's t, Sphere (primitive.hpp:172-184)
+// 1/(2 pi (1 - cos theta_max)); 0 when the ray misses the shape.
+PT_D float shape_pdf(const DevLight &L, f3 o, f3 wi) {
+  float t;
+  if (L.type == LIGHT_QUAD) {
+    if (!hit_rect(make_float4(L.k, L.d0min, L.d0max, 0.f), make_float4(L.d1min, L.d1max, (float)L.axis, 0.f), o, wi,
+                  1e-4f, 1e4f, t))
+      return 0.f;
+    float area = (L.d0max - L.d0min) * (L.d1max - L.d1min);
+    float d2 = t * t * dot3(wi, wi);
+    f3 n = L.axis == 0 ? mk3(1.f, 0.f, 0.f) : (L.axis == 1 ? mk3(0.f, 1.f, 0.f) : mk3(0.f, 0.f, 1.f));
+    float cosine = fabsf(dot3(wi, n) / len3(wi));
+    return d2 / (cosine * area);
+  }
+  if (L.type == LIGHT_SPHERE) {
+    if (!hit_sphere(make_float4(L.c.x, L.c.y, L.c.z, 0.f), make_float4(L.r, 0.f, 0.f, 0.f), o, wi, 1e-4f, 1e4f, t))
+      return 0.f;
+    f3 oc = sub3(L.c, o);
+    float ctm = sqrtf(1.f - L.r * L.r / dot3(oc, oc));
+    return (float)(1.0 / (2.0 * 3.14159265358979323846 * (double)(1.f - ctm)));
+  }
+  return 0.f;
+}
+// Hittable::get_sample of that shape from o (not normalised): a uniform
+// point of the rect minus o (primitive.cpp:125-146), or a direction in the
+// cone the sphere subtends (primitive.hpp:186-192, random.hpp:36-44).
+PT_D f3 shape_sample(const DevLight &L, f3 o, float u, float v) {
+  if (L.type == LIGHT_QUAD) {
+    float a = L.d0min + (L.d0max - L.d0min) * u;
+    float b = L.d1min + (L.d1max - L.d1min) * v;
+    f3 pt = L.axis == 0 ? mk3(L.k, a, b) : (L.axis == 1 ? mk3(a, L.k, b) : mk3(a, b, L.k));
+    return sub3(pt, o);
+  }
+  f3 dir = sub3(L.c, o);
+  float d2 = dot3(dir, dir);
+  float z = 1.f + v * (sqrtf(1.f - L.r * L.r / d2) - 1.f);
+  float phi = 2.f * PT_PI * u;
+  float x = cosf(phi) * sqrtf(1.f - z * z);
+  float y = sinf(phi) * sqrtf(1.f - z * z);
+  Frame b = make_frame(norm3(dir));
+  return to_world(b, mk3(x, y, z));
+}
+
 // Light::get_sample + get_normal_at (light.hpp:131-147, 254-271;
 // envLight.hpp:20-37,51-54)
 PT_D f3 light_point(const DevLight &L, float u, float v, f3 &nrm) {

@@ -618,6 +618,9 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
       L.n = norm3(cross3(L.e1, L.e2));
       L.area = len3(cross3(L.e1, L.e2));
       L.le = scl3(mk3(md.p[0], md.p[1], md.p[2]), md.p[3]);
+      L.axis = r.axis;
+      L.k = r.k;
+      L.d0min = r.d0min; L.d0max = r.d0max; L.d1min = r.d1min; L.d1max = r.d1max;
       r.light = (int)s->lights.size();
       s->lights.push_back(L);
     }
@@ -952,7 +955,7 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   if (!s->has_cam) return fail(PT_ERR_INVALID, "pt_render: camera is not initialized");  // renderManager.hpp:58-62
   if (cfg->width == 0 || cfg->height == 0 || cfg->spp_x == 0 || cfg->spp_y == 0)
     return fail(PT_ERR_INVALID, "pt_render: width, height, spp_x, spp_y must be > 0");
-  if (cfg->estimator > PT_EST_NEE_MIS) return fail(PT_ERR_INVALID, "pt_render: unknown estimator");
+  if (cfg->estimator > PT_EST_ROULETTE) return fail(PT_ERR_INVALID, "pt_render: unknown estimator");
   if (cfg->sampler > PT_SAMPLER_STRATIFIED) return fail(PT_ERR_INVALID, "pt_render: unknown sampler");
   if (cfg->estimator == PT_EST_REFERENCE_NEE && cfg->light_samples > 4)
     return fail(PT_ERR_UNSUPPORTED, "pt_render: at most 4 light samples per vertex");
@@ -1070,7 +1073,8 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
       launch_shade(s->stream, s->ds, rp, s->wave, qi, d, n, grid_cap);
       ev_end(e);
       e = ev_begin(3);
-      launch_shadow(s->stream, s->ds, s->wave, 4u * n, trav_cap, count);
+      if (cfg->estimator <= PT_EST_NEE_MIS)  // the mixture estimators trace no shadow rays
+        launch_shadow(s->stream, s->ds, s->wave, 4u * n, trav_cap, count);
       ev_end(e);
       qi ^= 1;
     }

@@ -239,162 +239,231 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
       f3 L = s3(0.f);  // radiance this vertex adds (emission); NEE goes through shadow rays
       bool prev_spec = fl & 1u;
       bool done = false;
-      if (prim < 0) {
-        if (EST == 0) {
-          L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
-        } else if (prev_spec) {
-          if (d < D) L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
-        } else if (s.has_env) {
-          float pl = env_pdf(s, dir) / (float)N;
-          L = add3(L, scl3(mul3(beta, env_lookup(s, dir)), power_w(ppdf, pl)));
+      if constexpr (EST >= PT_EST_NAIVE) {
+        // maxdepth_naive.hpp:9-75 / roulette_naive.hpp:9-78: no NEE; misses and
+        // the naive depth limit return bg_color (the environment is not consulted)
+        const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
+        const int NA = N - (s.has_env ? 1 : 0);  // area lights come first
+        (void)prev_spec;
+        if (prim < 0) {
+          L = add3(L, mul3(beta, bg));
         } else {
-          L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
-        }
-        done = true;
-      }
-      SurfHit h;
-      const pt_material_desc *mat = nullptr;
-      if (!done) {
-        h = surface(s, prim, o, dir, t);
-        mat = &s.mats[h.mat];
-        if (mat->kind == PT_MAT_EMITTER) {
-          if (EST == 0) {
-            // A non-specular predecessor excludes emitter hits (maxdepth_nee.hpp:246-247);
-            // camera and specular chains take DiffuseEmitter::emit.
-            if (prev_spec) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
-          } else if (prev_spec || h.light < 0) {
-            if (d < D) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+          SurfHit h = surface(s, prim, o, dir, t);
+          const pt_material_desc *mat = &s.mats[h.mat];
+          p = h.p;
+          float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
+          float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
+          float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
+          float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
+          f3 wo = neg3(norm3(dir));
+          Scatter sc;
+          if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
+            L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
           } else {
-            const DevLight &Lg = s.lights[h.light];
-            // one-sided QuadLight/SphereLight::L_emit (light.hpp:157-165, 296-303): the quad's
-            // normal follows its vertex order, the sphere's points outwards
-            f3 nl = (Lg.type == LIGHT_QUAD) ? Lg.n : h.ng;
-            f3 le = (dot3(nl, neg3(norm3(dir))) > 0.f) ? Lg.le : s3(0.f);
-            float pl = light_pdf_sa(s, Lg, pprev, h.p) / (float)N;
-            L = add3(L, scl3(mul3(beta, le), power_w(ppdf, pl)));
+            float q = 1.f;
+            bool alive = true;
+            if (EST == PT_EST_ROULETTE) {
+              q = fminf(fmaxf(maxc3(sc.att), 0.5f), 1.f);
+              alive = !(pt_rng(g, rslot(d, SLOT_RR), rp.seed) > q);
+            }
+            if (alive) {
+              if (sc.flags & BX_SPECULAR) {
+                beta = mul3(beta, divs3(sc.att, q));
+                wi = sc.wi;
+              } else {
+                // MixedPDF of light_samples HittablePDFs and the material's pdf
+                int li[4] = {0, 0, 0, 0};
+                const uint32_t nl = NA > 0 ? min(rp.light_samples, 4u) : 0u;
+                for (uint32_t si = 0; si < nl; ++si)
+                  li[si] = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)NA) % NA;
+                const uint32_t ncomp = nl + 1u;
+                const uint32_t comp =
+                    (uint32_t)floorf(pt_rng(g, rslot(d, SLOT_MIX), rp.seed) * (float)ncomp) % ncomp;
+                if (comp < nl) {
+                  float e0 = pt_rng(g, rslot(d, SLOT_NEE_OFF), rp.seed);
+                  float e1 = pt_rng(g, rslot(d, SLOT_NEE_OFF + 1u), rp.seed);
+                  int lc = li[0];
+                  for (uint32_t si = 1; si < nl; ++si)
+                    if (si == comp) lc = li[si];
+                  wi = norm3(shape_sample(s.lights[lc], p, e0, e1));
+                } else {
+                  wi = norm3(sc.wi);
+                }
+                const float wgt = 1.f / (float)ncomp;
+                float pdf = 0.f;
+                for (uint32_t si = 0; si < nl; ++si) pdf += wgt * shape_pdf(s.lights[li[si]], p, wi);
+                pdf += wgt * bsdf_pdf<MS>(*mat, wo, wi, h.sp);
+                f3 f = divs3(bsdf_f<MS>(*mat, wo, wi, h.sp), q);
+                float cs = fabsf(dot3(h.sp.n, wi));
+                beta = scl3(mul3(beta, f), cs / pdf);
+              }
+              if (d + 1u < D) {
+                cont = beta.x != 0.f || beta.y != 0.f || beta.z != 0.f;
+              } else if (EST == PT_EST_NAIVE) {
+                L = add3(L, mul3(beta, bg));  // getRayColor(.., 0) returns bg_color
+              }
+              fl = 0u;
+            }
+          }
+        }
+      } else {
+        if (prim < 0) {
+          if (EST == 0) {
+            L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
+          } else if (prev_spec) {
+            if (d < D) L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
+          } else if (s.has_env) {
+            float pl = env_pdf(s, dir) / (float)N;
+            L = add3(L, scl3(mul3(beta, env_lookup(s, dir)), power_w(ppdf, pl)));
+          } else {
+            L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
           }
           done = true;
         }
-      }
-      if (!done && d >= D) done = true;
-      if (!done) {
-        p = h.p;
-        float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
-        float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
-        float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
-        float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
-        f3 wo = neg3(norm3(dir));
-        Scatter sc;
-        if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
-          L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
-          done = true;
-        } else if (sc.flags & BX_SPECULAR) {
-          beta = mul3(beta, sc.att);
-          wi = sc.wi;
-          cont = d + 1u < D;
-          fl = 1u;
-        } else {
-          f3 wn = norm3(sc.wi);
-          float cs = fabsf(dot3(h.sp.n, wn));
-          if (EST == 0) {
-            // NEE with light_samples jittered offsets (maxdepth_nee.hpp:90-232)
-            if (N > 0) {
-              float inv_ns = 1.f / (float)rp.light_samples;
-              for (uint32_t si = 0; si < rp.light_samples && si < 4u; ++si) {
-                uint32_t ii = si / rp.ly, jj = si % rp.ly;
-                float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
-                float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
-                float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
-                float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
-                int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
-                const DevLight &Lg = s.lights[li];
-                f3 nl;
-                f3 lp = light_point(Lg, ox, oy, nl);
-                f3 c = s3(0.f), lwi = s3(0.f);
-                float tmax = 0.f;
-                bool valid = false;
-                if (Lg.type == LIGHT_ENV) {
-                  lwi = norm3(sub3(lp, p));
-                  float pdf = (1.f / (float)N) * (1.f / (4.f * PT_PI));
-                  float fac = fmaxf(0.f, dot3(h.sp.n, lwi));
-                  f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
-                  c = divs3(scl3(mul3(f, env_lookup(s, lwi)), fac), pdf);
-                  tmax = 3.402823466e+38f;
-                  valid = true;
-                } else {
-                  f3 ld = sub3(lp, p);
-                  float len = len3(ld);
-                  lwi = divs3(ld, len);
-                  if (len >= 0.001f) {
-                    f3 le = (dot3(nl, neg3(lwi)) > 0.f) ? Lg.le : s3(0.f);
-                    float pdf = (Lg.area <= 1e-7f) ? 0.f : 1.f / (Lg.area * (float)N);
-                    float noi = fmaxf(0.f, dot3(h.sp.n, lwi));
-                    float nol = fmaxf(0.f, dot3(nl, neg3(lwi)));
-                    float d2 = len * len;
-                    float fac = (d2 < 1e-7f) ? 0.f : (nol * noi) / d2;
-                    if (pdf > 1e-7f) {
-                      f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
-                      c = divs3(scl3(mul3(f, le), fac), pdf);
-                      tmax = len - 0.001f;
-                      valid = true;
+        SurfHit h;
+        const pt_material_desc *mat = nullptr;
+        if (!done) {
+          h = surface(s, prim, o, dir, t);
+          mat = &s.mats[h.mat];
+          if (mat->kind == PT_MAT_EMITTER) {
+            if (EST == 0) {
+              // A non-specular predecessor excludes emitter hits (maxdepth_nee.hpp:246-247);
+              // camera and specular chains take DiffuseEmitter::emit.
+              if (prev_spec) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+            } else if (prev_spec || h.light < 0) {
+              if (d < D) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+            } else {
+              const DevLight &Lg = s.lights[h.light];
+              // one-sided QuadLight/SphereLight::L_emit (light.hpp:157-165, 296-303): the quad's
+              // normal follows its vertex order, the sphere's points outwards
+              f3 nl = (Lg.type == LIGHT_QUAD) ? Lg.n : h.ng;
+              f3 le = (dot3(nl, neg3(norm3(dir))) > 0.f) ? Lg.le : s3(0.f);
+              float pl = light_pdf_sa(s, Lg, pprev, h.p) / (float)N;
+              L = add3(L, scl3(mul3(beta, le), power_w(ppdf, pl)));
+            }
+            done = true;
+          }
+        }
+        if (!done && d >= D) done = true;
+        if (!done) {
+          p = h.p;
+          float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
+          float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
+          float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
+          float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
+          f3 wo = neg3(norm3(dir));
+          Scatter sc;
+          if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
+            L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+            done = true;
+          } else if (sc.flags & BX_SPECULAR) {
+            beta = mul3(beta, sc.att);
+            wi = sc.wi;
+            cont = d + 1u < D;
+            fl = 1u;
+          } else {
+            f3 wn = norm3(sc.wi);
+            float cs = fabsf(dot3(h.sp.n, wn));
+            if (EST == 0) {
+              // NEE with light_samples jittered offsets (maxdepth_nee.hpp:90-232)
+              if (N > 0) {
+                float inv_ns = 1.f / (float)rp.light_samples;
+                for (uint32_t si = 0; si < rp.light_samples && si < 4u; ++si) {
+                  uint32_t ii = si / rp.ly, jj = si % rp.ly;
+                  float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
+                  float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
+                  float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
+                  float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
+                  int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
+                  const DevLight &Lg = s.lights[li];
+                  f3 nl;
+                  f3 lp = light_point(Lg, ox, oy, nl);
+                  f3 c = s3(0.f), lwi = s3(0.f);
+                  float tmax = 0.f;
+                  bool valid = false;
+                  if (Lg.type == LIGHT_ENV) {
+                    lwi = norm3(sub3(lp, p));
+                    float pdf = (1.f / (float)N) * (1.f / (4.f * PT_PI));
+                    float fac = fmaxf(0.f, dot3(h.sp.n, lwi));
+                    f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+                    c = divs3(scl3(mul3(f, env_lookup(s, lwi)), fac), pdf);
+                    tmax = 3.402823466e+38f;
+                    valid = true;
+                  } else {
+                    f3 ld = sub3(lp, p);
+                    float len = len3(ld);
+                    lwi = divs3(ld, len);
+                    if (len >= 0.001f) {
+                      f3 le = (dot3(nl, neg3(lwi)) > 0.f) ? Lg.le : s3(0.f);
+                      float pdf = (Lg.area <= 1e-7f) ? 0.f : 1.f / (Lg.area * (float)N);
+                      float noi = fmaxf(0.f, dot3(h.sp.n, lwi));
+                      float nol = fmaxf(0.f, dot3(nl, neg3(lwi)));
+                      float d2 = len * len;
+                      float fac = (d2 < 1e-7f) ? 0.f : (nol * noi) / d2;
+                      if (pdf > 1e-7f) {
+                        f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+                        c = divs3(scl3(mul3(f, le), fac), pdf);
+                        tmax = len - 0.001f;
+                        valid = true;
+                      }
                     }
                   }
+                  c = scl3(mul3(beta, c), inv_ns);
+                  // zero contributions are dropped; non-finite ones are traced so the film's
+                  // NaN guard sees them
+                  if (valid && ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)))
+                    emit_shadow(w, p, lwi, tmax, c, slot);
                 }
-                c = scl3(mul3(beta, c), inv_ns);
-                // zero contributions are dropped; non-finite ones are traced so the film's
-                // NaN guard sees them
-                if (valid && ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)))
-                  emit_shadow(w, p, lwi, tmax, c, slot);
+              }
+            } else if (N > 0) {
+              int li = (int)(pt_rng(g, rslot(d, SLOT_MIS_LIGHT), rp.seed) * (float)N) % N;
+              const DevLight &Lg = s.lights[li];
+              float e0 = pt_rng(g, rslot(d, SLOT_NEE_OFF), rp.seed);
+              float e1 = pt_rng(g, rslot(d, SLOT_NEE_OFF + 1u), rp.seed);
+              f3 lwi = s3(0.f), le = s3(0.f);
+              float pl = 0.f, tmax = 0.f;
+              if (Lg.type == LIGHT_ENV) {
+                lwi = env_sample(s, e0, e1, pl);
+                le = env_lookup(s, lwi);
+                tmax = 3.402823466e+38f;
+              } else {
+                f3 nl;
+                f3 lp = light_point(Lg, e0, e1, nl);
+                f3 ld = sub3(lp, p);
+                float len = len3(ld);
+                lwi = divs3(ld, len);
+                float nol = dot3(nl, neg3(lwi));
+                if (len >= 0.001f && nol > 0.f && Lg.area > 0.f) {
+                  pl = len * len / (Lg.area * nol);
+                  le = Lg.le;
+                  tmax = len - 0.001f;
+                }
+              }
+              if (pl > 0.f) {
+                pl /= (float)N;
+                f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+                float pb = bsdf_pdf<MS>(*mat, wo, lwi, h.sp);
+                float cl = fabsf(dot3(h.sp.n, lwi));
+                f3 c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
+                if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)) emit_shadow(w, p, lwi, tmax, c, slot);
               }
             }
-          } else if (N > 0) {
-            int li = (int)(pt_rng(g, rslot(d, SLOT_MIS_LIGHT), rp.seed) * (float)N) % N;
-            const DevLight &Lg = s.lights[li];
-            float e0 = pt_rng(g, rslot(d, SLOT_NEE_OFF), rp.seed);
-            float e1 = pt_rng(g, rslot(d, SLOT_NEE_OFF + 1u), rp.seed);
-            f3 lwi = s3(0.f), le = s3(0.f);
-            float pl = 0.f, tmax = 0.f;
-            if (Lg.type == LIGHT_ENV) {
-              lwi = env_sample(s, e0, e1, pl);
-              le = env_lookup(s, lwi);
-              tmax = 3.402823466e+38f;
+            beta = scl3(mul3(beta, sc.att), cs / sc.pdf);
+            wi = wn;
+            fl = 0u;
+            if (EST == 0) {
+              cont = d + 1u < D;  // the emitter check of the next hit happens at the next vertex
             } else {
-              f3 nl;
-              f3 lp = light_point(Lg, e0, e1, nl);
-              f3 ld = sub3(lp, p);
-              float len = len3(ld);
-              lwi = divs3(ld, len);
-              float nol = dot3(nl, neg3(lwi));
-              if (len >= 0.001f && nol > 0.f && Lg.area > 0.f) {
-                pl = len * len / (Lg.area * nol);
-                le = Lg.le;
-                tmax = len - 0.001f;
-              }
-            }
-            if (pl > 0.f) {
-              pl /= (float)N;
-              f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
-              float pb = bsdf_pdf<MS>(*mat, wo, lwi, h.sp);
-              float cl = fabsf(dot3(h.sp.n, lwi));
-              f3 c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
-              if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)) emit_shadow(w, p, lwi, tmax, c, slot);
+              cont = d + 1u <= D;
+              ppdf = sc.pdf;
+              pprev = p;
             }
           }
-          beta = scl3(mul3(beta, sc.att), cs / sc.pdf);
-          wi = wn;
-          fl = 0u;
-          if (EST == 0) {
-            cont = d + 1u < D;  // the emitter check of the next hit happens at the next vertex
-          } else {
-            cont = d + 1u <= D;
-            ppdf = sc.pdf;
-            pprev = p;
+          if (cont && rp.rr_depth && d >= rp.rr_depth) {
+            float qs = fminf(1.f, maxc3(beta));
+            if (!(pt_rng(g, rslot(d, SLOT_RR), rp.seed) < qs)) cont = false;
+            else beta = divs3(beta, qs);
           }
-        }
-        if (cont && rp.rr_depth && d >= rp.rr_depth) {
-          float qs = fminf(1.f, maxc3(beta));
-          if (!(pt_rng(g, rslot(d, SLOT_RR), rp.seed) < qs)) cont = false;
-          else beta = divs3(beta, qs);
         }
       }
       // this path's slot is written by no other lane; this vertex's shadow rays
@@ -563,21 +632,23 @@ void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uin
   else
     hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi);
 }
+template <int EST>
+static void launch_shade_est(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi,
+                             uint32_t d, uint32_t g) {
+  // the basic material set when the scene uses nothing else (fewer registers)
+  if ((s.mat_kinds & ~kMatBasic) == 0u)
+    hipLaunchKernelGGL((k_shade<EST, kMatBasic>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+  else
+    hipLaunchKernelGGL((k_shade<EST, kMatAll>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+}
 void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
                   uint32_t max_n, uint32_t grid_cap) {
   uint32_t g = part_grid(max_n, grid_cap);
-  // the basic material set when the scene uses nothing else (fewer registers)
-  const bool basic = (s.mat_kinds & ~kMatBasic) == 0u;
-  if (rp.estimator == PT_EST_NEE_MIS) {
-    if (basic)
-      hipLaunchKernelGGL((k_shade<1, kMatBasic>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
-    else
-      hipLaunchKernelGGL((k_shade<1, kMatAll>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
-  } else {
-    if (basic)
-      hipLaunchKernelGGL((k_shade<0, kMatBasic>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
-    else
-      hipLaunchKernelGGL((k_shade<0, kMatAll>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
+  switch (rp.estimator) {
+    case PT_EST_NEE_MIS: launch_shade_est<PT_EST_NEE_MIS>(st, s, rp, w, qi, d, g); break;
+    case PT_EST_NAIVE: launch_shade_est<PT_EST_NAIVE>(st, s, rp, w, qi, d, g); break;
+    case PT_EST_ROULETTE: launch_shade_est<PT_EST_ROULETTE>(st, s, rp, w, qi, d, g); break;
+    default: launch_shade_est<PT_EST_REFERENCE_NEE>(st, s, rp, w, qi, d, g); break;
   }
 }
 void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count) {

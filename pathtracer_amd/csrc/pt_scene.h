@@ -43,6 +43,9 @@ struct DevLight {
   f3 c;              // sphere
   float r;
   f3 le;             // texture colour * intensity
+  // the AARectangle a quad light was discovered from (its shape pdf/sample)
+  int axis;
+  float k, d0min, d0max, d1min, d1max;
 };
 
 struct DevScene {

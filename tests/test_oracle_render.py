@@ -7,6 +7,12 @@
   estimator, which counts the environment through NEE and again through the
   BSDF ray that misses (maxdepth_nee.hpp:26-37, 107-145).
 * The reference estimator and NEE+MIS agree on an area-lit scene.
+* The mixture-pdf integrators (maxdepth_naive.hpp, roulette_naive.hpp): the
+  same form factor through BSDF/shape-pdf mixture sampling (pins the rect
+  pdf d^2/(cos A), the mixture weights and the roulette 1/q), the depth limit
+  (bg_color for naive, 0 for roulette) and agreement with the reference
+  estimator on a scene lit by a sphere light (pins the cone pdf
+  1/(2 pi (1 - cos theta_max)) and its sampler).
 """
 import numpy as np
 import pytest
@@ -75,3 +81,86 @@ def test_russian_roulette_keeps_expectation():
     a, _, _, _ = o.render(o.cfg(rr_depth=0))
     b, _, _, _ = o.render(o.cfg(rr_depth=2, seed=5))
     assert np.allclose(a.mean((0, 1)), b.mean((0, 1)), rtol=0.03)
+
+
+@pytest.mark.parametrize("estimator", [2, 3])
+def test_mixture_estimators_form_factor(estimator):
+    spec, expect = floor_under_light()
+    o = O.OracleScene(spec)
+    img, var, _, _ = o.render(o.cfg(estimator=estimator, max_depth=2, spp_x=128, spp_y=128), want_var=True)
+    se = np.sqrt(var[0, 0, 0] / (128 * 128))
+    assert abs(img[0, 0, 0] - expect) < max(4 * se, 2e-3 * expect)
+
+
+def _sphere_under_sky(albedo=0.7, E=2.0):
+    s = scenes.SceneSpec(name="sky")
+    m = s.add_material(1, [albedo] * 3)
+    s.add_sphere((0, 0, 0), 1.0, m)
+    s.camera = dict(fov_deg=30, aspect=1, focal_dist=1, aperture=0, up=(0, 1, 0), front=(0, 0, -1), pos=(0, 0, 5))
+    s.render = dict(width=8, height=8, spp_x=2, spp_y=2, max_depth=1, sampler=0, estimator=2, rr_depth=0,
+                    light_samples=4, seed=1, bg=(E, E, E))
+    return s
+
+
+def test_naive_depth_limit_returns_background():
+    """maxdepth_naive returns bg_color at depth 0 and on misses: a convex
+    Lambertian sphere under a uniform background gives a*E at every depth
+    (cosine sampling: f cos / pdf = a), E where the ray misses."""
+    spec = _sphere_under_sky()
+    o = O.OracleScene(spec)
+    for depth in (1, 3):
+        img, _, smp, _ = o.render(o.cfg(max_depth=depth), want_samples=True)
+        v = smp[..., 0]
+        on = np.isclose(v, 0.7 * 2.0, rtol=1e-5)
+        off = v == np.float32(2.0)
+        assert np.all(on | off) and on.any() and off.any()
+
+
+def test_roulette_depth_limit_returns_zero():
+    spec = _sphere_under_sky()
+    o = O.OracleScene(spec)
+    _, _, smp1, _ = o.render(o.cfg(estimator=3, max_depth=1), want_samples=True)
+    v = smp1[..., 0]
+    assert np.all((v == 0) | (v == np.float32(2.0)))        # sphere paths end at the limit with 0
+    # one bounce more: the sphere vertex survives roulette (q = clamp(a/pi, 0.5, 1) = 0.5) with
+    # weight 1/q, so its samples are 0 or 2 a E, half of each; misses are E as before
+    cfg = o.cfg(estimator=3, max_depth=2, spp_x=8, spp_y=8)
+    _, _, smp2, _ = o.render(cfg, want_samples=True)
+    _, _, ref, _ = o.render(o.cfg(estimator=2, max_depth=2, spp_x=8, spp_y=8), want_samples=True)
+    r, n = smp2[..., 0], ref[..., 0]
+    miss = n == np.float32(2.0)
+    assert np.array_equal(r[miss], n[miss])
+    hit = ~miss
+    assert np.all((r[hit] == 0) | np.isclose(r[hit], 2 * 1.4, rtol=1e-5))
+    frac = np.mean(r[hit] > 0)
+    assert abs(frac - 0.5) < 4 * np.sqrt(0.25 / hit.sum())
+
+
+def _sphere_lit_room():
+    s = scenes.SceneSpec(name="sphere_light")
+    w = s.add_material(1, [0.6, 0.6, 0.6])
+    r = s.add_material(1, [0.7, 0.2, 0.1])
+    lm = s.add_material(2, [1.0, 0.9, 0.7, 6.0])
+    s.add_rect(1, 0.0, -3, 3, -3, 3, w)
+    s.add_rect(2, -2.0, -3, 3, 0, 4, r)
+    s.add_sphere((0.0, 0.6, 0.0), 0.6, w)
+    s.add_sphere((1.2, 2.0, 0.5), 0.35, lm)
+    s.camera = dict(fov_deg=45, aspect=1, focal_dist=1, aperture=0, up=(0, 1, 0), front=(0, -0.3, -1),
+                    pos=(0, 1.5, 4.5))
+    s.render = dict(width=12, height=12, spp_x=16, spp_y=16, max_depth=4, sampler=0, estimator=0, rr_depth=0,
+                    light_samples=4, seed=3, bg=(0, 0, 0))
+    return s
+
+
+@pytest.mark.parametrize("estimator", [2, 3])
+def test_mixture_estimators_agree_with_reference_on_a_sphere_light(estimator):
+    """Where the reference estimator is unbiased (area lights, no environment,
+    no background: the naive depth limit adds 0) the mixture integrators
+    converge to the same image."""
+    spec = _sphere_lit_room()
+    o = O.OracleScene(spec)
+    a, va, _, _ = o.render(o.cfg(estimator=0, spp_x=32, spp_y=32), want_var=True)
+    b, vb, _, _ = o.render(o.cfg(estimator=estimator, spp_x=64, spp_y=64, seed=11), want_var=True)
+    ma, mb = a.mean((0, 1)), b.mean((0, 1))
+    se = np.sqrt(va.mean((0, 1)) / (32 * 32 * 144) + vb.mean((0, 1)) / (64 * 64 * 144))
+    assert np.all(np.abs(ma - mb) < 5 * se + 0.01 * ma), (ma, mb, se)
