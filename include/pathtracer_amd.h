@@ -116,9 +116,12 @@ enum pt_estimator {
   PT_EST_NAIVE = 2,         /* maxdepth_naive.hpp:9-75: no NEE; each bounce samples
                                a uniform mixture of light_samples shape pdfs and
                                the BSDF; misses and the depth limit return bg     */
-  PT_EST_ROULETTE = 3       /* roulette_naive.hpp:9-78: the same mixture with
+  PT_EST_ROULETTE = 3,      /* roulette_naive.hpp:9-78: the same mixture with
                                Russian roulette q = clamp(max(f), 0.5, 1) at every
                                vertex; the depth limit returns 0                  */
+  PT_EST_RESERVOIR = 4      /* maxdepth_reservoir_di.hpp:9-131: direct light
+                               resampled from light_samples shadowed candidates;
+                               a vertex whose continuation hits an emitter adds 0 */
 };
 
 typedef struct pt_render_cfg {

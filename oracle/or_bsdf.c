@@ -107,6 +107,13 @@ static float cosine_pdf(v3 n, v3 wi) {
   onb_t b = onb_make(n);
   return (float)(fmaxf(vdot(vnorm(wi), b.w), 0.f) / M_PI);
 }
+/* CosinePDF(n) (pdf.hpp:19-37): get(uc, u) and at(dir), for the integrators
+ * that fall back to it when a material has no pdf of its own. */
+v3 or_cosine_dir(v3 n, float u0, float u1) {
+  onb_t b = onb_make(n);
+  return vnorm(onb_to_world(&b, cosine_sample_hemisphere(u0, u1)));
+}
+float or_cosine_pdf(v3 n, v3 wi) { return cosine_pdf(n, wi); }
 
 /* ---------------- Conductor (conductor.hpp:4-145, mfPDF.hpp:6-62) ---------- */
 static float mf_pdf_at(trd_t d, const onb_t *b, v3 wo_l, v3 wi_world) {

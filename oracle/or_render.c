@@ -29,7 +29,7 @@
 #include "or_scene.h"
 
 enum { SLOT_LENS_X = 0, SLOT_LENS_Y = 1, SLOT_TIME = 2, SLOT_AUX = 4, SLOT_RR = 5,
-       SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_NEE_LIGHT = 40 };
+       SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_RIS = 16, SLOT_NEE_LIGHT = 40 };
 static inline uint32_t slot(uint32_t d, uint32_t purpose) { return d * 64u + purpose; }
 
 typedef struct {
@@ -292,10 +292,113 @@ static v3 trace_path_naive(const rctx *c, uint32_t i, uint32_t j, uint32_t k, in
   return L;
 }
 
+/* One path sample of getRayColor in integrator/maxdepth_reservoir_di.hpp:9-131,
+ * unrolled:
+ *   - max_depth == 0 and misses return bg_color;
+ *   - after a successful scatter the continuation direction comes from the
+ *     material's pdf, or from CosinePDF(n) when it has none (specular
+ *     lobes), with weight attenuation * cos / pdf (:39-50, 126-129);
+ *   - direct light is resampled from light_samples candidates (light,
+ *     jittered point): contribution f Le A G V with the visibility inside,
+ *     reservoir weight |contribution| (vector length), streaming selection
+ *     rand < w / W, estimate sample * W / light_samples (:52-100);
+ *   - when the continuation ray hits an emitter the vertex returns 0: its
+ *     direct light and everything after it are dropped (:113-115);
+ *   - the environment light is a candidate like the others, with area
+ *     +infinity and its texture read at the candidate's offset (so such a
+ *     sample is not finite, and the film's NaN rule drops it, as the
+ *     reference's pixel sum does). */
+static v3 trace_path_reservoir(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t *cnt) {
+  const or_scene *s = c->s;
+  const pt_render_cfg *cfg = c->cfg;
+  uint32_t g = 1u + k + c->spp * (j + cfg->height * i);
+  uint32_t pixel = j * cfg->width + i;
+  v3 ro, rd;
+  camera_ray(c, i, j, k, g, &ro, &rd);
+  v3 L = vs(0.f), beta = vs(1.f);
+  const v3 bg = V3(cfg->bg[0], cfg->bg[1], cfg->bg[2]);
+  const uint32_t D = cfg->max_depth;
+  const int N = s->n_lights;
+  const uint32_t ns = c->ns < 4u ? c->ns : 4u;
+  v3 pending = vs(0.f);  /* the previous vertex's direct light, times its throughput */
+  if (cnt) cnt[0]++;
+  for (uint32_t d = 0;; ++d) {
+    or_hit h;
+    if (cnt) cnt[1]++;
+    int hit = or_scene_intersect(s, ro, rd, 1e-4f, 1e5f, 0, &h, cnt ? cnt + 4 : NULL);
+    if (d > 0 && hit && or_is_emitter(&s->mats[h.mat])) break;  /* :113-115 */
+    L = vadd(L, pending);
+    pending = vs(0.f);
+    if (d >= D || !hit) {
+      L = vadd(L, vmul(beta, bg));
+      break;
+    }
+    const pt_material_desc *m = &s->mats[h.mat];
+    float uc = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d);
+    float u0 = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d + 1);
+    float u1 = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d + 2);
+    float aux = or_rng(g, slot(d, SLOT_AUX), cfg->seed);
+    or_srec sr;
+    if (!or_scatter(m, rd, &h, uc, u0, u1, aux, &sr)) {
+      L = vadd(L, vmul(beta, or_emit(m, rd, &h)));
+      break;
+    }
+    v3 wo = vneg(vnorm(rd));
+    v3 wi;
+    float pdf;
+    if (sr.flags & OR_SPECULAR) { /* CosinePDF(n): get(uc, u) and at(dir) */
+      wi = or_cosine_dir(h.n, u0, u1);
+      pdf = or_cosine_pdf(h.n, wi);
+    } else {
+      wi = vnorm(sr.wi);
+      pdf = or_bsdf_pdf(m, wo, wi, &h);
+    }
+    float cs = fabsf(vdot(h.n, wi));
+    /* resampled direct light */
+    v3 sample = vs(0.f);
+    float W = 0.f;
+    if (N > 0) {
+      for (uint32_t si = 0; si < ns; ++si) {
+        uint32_t ii = si / c->ly, jj = si % c->ly;
+        float a0 = (float)ii / (float)c->lx, a1 = (float)(ii + 1) / (float)c->lx;
+        float b0 = (float)jj / (float)c->ly, b1 = (float)(jj + 1) / (float)c->ly;
+        float ox = a0 + (a1 - a0) * or_rng(g, slot(d, SLOT_NEE_OFF + 2 * si), cfg->seed);
+        float oy = b0 + (b1 - b0) * or_rng(g, slot(d, SLOT_NEE_OFF + 2 * si + 1), cfg->seed);
+        int li = (int)(or_rng(g, slot(d, SLOT_NEE_LIGHT + si), cfg->seed) * (float)N) % N;
+        const or_light *Lg = &s->lights[li];
+        v3 nl;
+        v3 lp = or_light_point(s, Lg, ox, oy, &nl);
+        v3 ld = vsub(lp, h.p);
+        v3 lwi = vnorm(ld);
+        float len = vlen(ld);
+        if (cnt) cnt[2]++;
+        int occ = or_scene_intersect(s, h.p, lwi, 0.001f, len - 0.001f, 1, NULL, NULL);
+        float noi = fmaxf(vdot(h.n, lwi), 0.f);
+        float nol = fmaxf(vdot(nl, vneg(lwi)), 0.f);
+        float G = nol * noi / vdot(ld, ld);
+        float V = occ ? 0.f : 1.f;
+        v3 f = or_bsdf_f(m, wo, lwi, &h);
+        v3 le = Lg->type == OR_LIGHT_ENV ? or_env_lookup_uv(s, ox, oy) : Lg->le;
+        float area = Lg->type == OR_LIGHT_ENV ? INFINITY : Lg->area;
+        v3 cand = vscale(vmul(f, le), area * G * V);
+        float w = vlen(cand);
+        W += w;
+        if (or_rng(g, slot(d, SLOT_RIS + si), cfg->seed) < w / W) sample = cand;
+      }
+    }
+    pending = vmul(beta, vscale(sample, W / (float)(ns ? ns : 1)));
+    beta = vscale(vmul(beta, sr.att), cs / pdf);
+    ro = h.p;
+    rd = wi;
+  }
+  return L;
+}
+
 /* One path sample.  Returns the radiance. */
 static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t *cnt) {
   if (c->cfg->estimator == PT_EST_NAIVE || c->cfg->estimator == PT_EST_ROULETTE)
     return trace_path_naive(c, i, j, k, c->cfg->estimator == PT_EST_ROULETTE, cnt);
+  if (c->cfg->estimator == PT_EST_RESERVOIR) return trace_path_reservoir(c, i, j, k, cnt);
   const or_scene *s = c->s;
   const pt_render_cfg *cfg = c->cfg;
   uint32_t g = 1u + k + c->spp * (j + cfg->height * i);

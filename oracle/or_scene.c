@@ -613,6 +613,26 @@ v3 or_env_lookup(const or_scene *s, v3 dir) {
   return vscale(c, s->env_scale);
 }
 
+/* ImageTexture::getTexelColor at texture coordinates (u, v) (texture.cpp:
+ * 111-142, bilinear with the clamp to w-2/h-2), times the environment scale:
+ * the lookup EnvironmentLight::texture->getTexelColor(offset) makes. */
+v3 or_env_lookup_uv(const or_scene *s, float u, float v) {
+  int W = s->env_w, H = s->env_h;
+  float fx = u * (float)W, fy = v * (float)H;
+  unsigned x = (unsigned)fx, y = (unsigned)fy;
+  float x1 = fx - (float)x, y1 = fy - (float)y;
+  if (x > (unsigned)(W - 2)) x = (unsigned)(W - 2);
+  if (y > (unsigned)(H - 2)) y = (unsigned)(H - 2);
+  float x2 = 1.0f - x1, y2 = 1.0f - y1;
+#define TX(xx, yy) V3(s->env[3 * ((size_t)(yy) * W + (xx))], s->env[3 * ((size_t)(yy) * W + (xx)) + 1], s->env[3 * ((size_t)(yy) * W + (xx)) + 2])
+  v3 c1 = TX(x, y), c2 = TX(x + 1, y), c3 = TX(x, y + 1), c4 = TX(x + 1, y + 1);
+#undef TX
+  v3 c12 = vadd(vscale(c1, x2), vscale(c2, x1));
+  v3 c34 = vadd(vscale(c3, x2), vscale(c4, x1));
+  v3 c = vadd(vscale(c12, y2), vscale(c34, y1));
+  return vscale(c, s->env_scale);
+}
+
 /* uniformSampleSphere (utils/random.hpp:7-15) */
 v3 or_uniform_sphere(float u1, float u2) {
   float z = 1.0f - 2.0f * u1;

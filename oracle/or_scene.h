@@ -66,6 +66,7 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
                        or_hit *h, uint64_t *cnt);
 int or_prim_light(const or_scene *s, int prim);
 v3 or_env_lookup(const or_scene *s, v3 dir);
+v3 or_env_lookup_uv(const or_scene *s, float u, float v);
 v3 or_uniform_sphere(float u1, float u2);
 v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf);
 float or_env_pdf_dir(const or_scene *s, v3 dir);

@@ -64,6 +64,8 @@ typedef struct {
 int or_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, float uc, float u0,
                float u1, float aux, or_srec *s);
 v3 or_bsdf_f(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h);
+v3 or_cosine_dir(v3 n, float u0, float u1);
+float or_cosine_pdf(v3 n, v3 wi);
 float or_bsdf_pdf(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h);
 v3 or_emit(const pt_material_desc *m, v3 ray_d, const or_hit *h);
 int or_is_emitter(const pt_material_desc *m);

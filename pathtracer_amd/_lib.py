@@ -22,7 +22,7 @@ MAT_THINDIELECTRIC, MAT_PHONG, MAT_DISNEY, MAT_HAIR, MAT_KAJIYA_KAY = 5, 6, 7, 8
 SAMPLER_HALTON_OWEN, SAMPLER_HALTON_PERM, SAMPLER_HALTON_NONE = 0, 1, 2
 SAMPLER_SOBOL, SAMPLER_STRATIFIED = 3, 4
 # pt_estimator
-EST_REFERENCE_NEE, EST_NEE_MIS, EST_NAIVE, EST_ROULETTE = 0, 1, 2, 3
+EST_REFERENCE_NEE, EST_NEE_MIS, EST_NAIVE, EST_ROULETTE, EST_RESERVOIR = 0, 1, 2, 3, 4
 # render flags
 RENDER_COUNTERS, RENDER_NO_SORT, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16

@@ -103,6 +103,11 @@ PT_D float cosine_pdf(f3 n, f3 wi) {
   Frame b = make_frame(n);
   return fmaxf(dot3(norm3(wi), b.w), 0.f) / PT_PI;
 }
+// CosinePDF(n)::get (pdf.hpp:19-37), for integrators that fall back to it
+PT_D f3 cosine_dir(f3 n, float u0, float u1) {
+  Frame b = make_frame(n);
+  return norm3(to_world(b, cosine_hemi(u0, u1)));
+}
 PT_D bool lambert_scatter(const pt_material_desc &m, const ShadePoint &h, float u0, float u1, Scatter &s) {
   Frame b = make_frame(h.n);
   s.flags = BX_DIFFUSE | BX_REFL;

@@ -524,6 +524,22 @@ PT_D f3 env_lookup(const DevScene &s, f3 dir) {
   f3 c34 = add3(scl3(c3, x2), scl3(c4, x1));
   return scl3(add3(scl3(c12, y2), scl3(c34, y1)), s.env_scale);
 }
+// ImageTexture::getTexelColor at texture coordinates (u, v) (texture.cpp:
+// 111-142), times the environment scale: EnvironmentLight::texture->
+// getTexelColor(offset) of the reservoir estimator.
+PT_D f3 env_lookup_uv(const DevScene &s, float u, float v) {
+  float fx = u * (float)s.env_w, fy = v * (float)s.env_h;
+  uint32_t x = (uint32_t)fx, y = (uint32_t)fy;
+  float x1 = fx - (float)x, y1 = fy - (float)y;
+  x = min(x, (uint32_t)(s.env_w - 2));
+  y = min(y, (uint32_t)(s.env_h - 2));
+  float x2 = 1.f - x1, y2 = 1.f - y1;
+  f3 c1 = env_texel(s, x, y), c2 = env_texel(s, x + 1, y);
+  f3 c3 = env_texel(s, x, y + 1), c4 = env_texel(s, x + 1, y + 1);
+  f3 c12 = add3(scl3(c1, x2), scl3(c2, x1));
+  f3 c34 = add3(scl3(c3, x2), scl3(c4, x1));
+  return scl3(add3(scl3(c12, y2), scl3(c34, y1)), s.env_scale);
+}
 // uniformSampleSphere (utils/random.hpp:7-15)
 PT_D f3 uniform_sphere(float u1, float u2) {
   float z = 1.f - 2.f * u1;

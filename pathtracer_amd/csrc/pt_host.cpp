@@ -118,7 +118,7 @@ struct pt_scene {
   std::vector<DevLight> lights;
   pt_scene_info info{};
   // wavefront
-  DevBuf w_buf, w_counts, w_stats, d_off;
+  DevBuf w_buf, w_counts, w_stats, d_off, w_res;
   Wave wave{};
   uint32_t wave_cap = 0;
   DevBuf film, samples;
@@ -866,7 +866,7 @@ extern "C" pt_status pt_scene_get_info(const pt_scene *s, pt_scene_info *info) {
 // ------------------------------------------------------------------ wavefront buffers
 static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
   if (s->wave_cap >= cap && s->w_buf.p) return PT_OK;
-  const size_t per_state = 17;   // 4-byte words per entry of one live-path buffer
+  const size_t per_state = 20;   // 4-byte words per entry of one live-path buffer
   const size_t per_slot = 3;     // radiance per slot
   const size_t per_shadow = 11;  // 4-byte words per shadow-queue entry
   const size_t n_arrays = 2 * per_state + per_slot + per_shadow;
@@ -900,6 +900,7 @@ static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
     ps.px = (float *)take(lcap); ps.py = (float *)take(lcap); ps.pz = (float *)take(lcap);
     ps.flags = (uint32_t *)take(lcap);
     ps.slot = (uint32_t *)take(lcap);
+    ps.pbx = (float *)take(lcap); ps.pby = (float *)take(lcap); ps.pbz = (float *)take(lcap);
   }
   w.lx = (float *)take(cap); w.ly = (float *)take(cap); w.lz = (float *)take(cap);
   w.sox = (float *)take(scap); w.soy = (float *)take(scap); w.soz = (float *)take(scap);
@@ -913,7 +914,22 @@ static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
   w.cap = cap;
   w.part_cap = (uint32_t)part_cap;
   w.shadow_part_cap = (uint32_t)shadow_part_cap;
+  w.rc = nullptr;
+  w.rvis = nullptr;
+  s->w_res.release();
   s->wave_cap = cap;
+  return PT_OK;
+}
+
+// The reservoir estimator's candidate buffers (4 candidates per slot).
+static pt_status ensure_reservoir(pt_scene *s) {
+  Wave &w = s->wave;
+  if (w.rc) return PT_OK;
+  const size_t cap = w.cap;
+  const size_t rc_bytes = ((12 * cap * 4 + 255) / 256) * 256;
+  HIP_TRY(s->w_res.alloc(rc_bytes + 4 * cap));
+  w.rc = s->w_res.as<float>();
+  w.rvis = reinterpret_cast<uint8_t *>(s->w_res.as<char>() + rc_bytes);
   return PT_OK;
 }
 
@@ -955,7 +971,7 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   if (!s->has_cam) return fail(PT_ERR_INVALID, "pt_render: camera is not initialized");  // renderManager.hpp:58-62
   if (cfg->width == 0 || cfg->height == 0 || cfg->spp_x == 0 || cfg->spp_y == 0)
     return fail(PT_ERR_INVALID, "pt_render: width, height, spp_x, spp_y must be > 0");
-  if (cfg->estimator > PT_EST_ROULETTE) return fail(PT_ERR_INVALID, "pt_render: unknown estimator");
+  if (cfg->estimator > PT_EST_RESERVOIR) return fail(PT_ERR_INVALID, "pt_render: unknown estimator");
   if (cfg->sampler > PT_SAMPLER_STRATIFIED) return fail(PT_ERR_INVALID, "pt_render: unknown sampler");
   if (cfg->estimator == PT_EST_REFERENCE_NEE && cfg->light_samples > 4)
     return fail(PT_ERR_UNSUPPORTED, "pt_render: at most 4 light samples per vertex");
@@ -979,6 +995,10 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   uint32_t cap = (uint32_t)(chunk_pix * sppc);
   st = ensure_wave(s, cap);
   if (st != PT_OK) return st;
+  if (cfg->estimator == PT_EST_RESERVOIR) {
+    st = ensure_reservoir(s);
+    if (st != PT_OK) return st;
+  }
 
   // film
   if (s->film_w != cfg->width || s->film_h != cfg->height || !(cfg->flags & PT_RENDER_ACCUMULATE)) {
@@ -1034,7 +1054,11 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   // traversal: one resident round (8 waves per SIMD = 8 blocks per CU), each wave
   // refilling its lanes from its share of the rays
   const uint32_t trav_cap = std::max<uint32_t>(PT_PARTS, s->n_cu * PT_TRAV_BLOCKS_PER_CU / PT_PARTS * PT_PARTS);
-  const uint32_t nverts = cfg->estimator == PT_EST_NEE_MIS ? cfg->max_depth + 1 : cfg->max_depth;
+  // NEE+MIS and the reservoir estimator trace one vertex more: the hit that
+  // weights (MIS) or drops (reservoir) the last vertex's direct light
+  const uint32_t nverts = (cfg->estimator == PT_EST_NEE_MIS || cfg->estimator == PT_EST_RESERVOIR)
+                              ? cfg->max_depth + 1
+                              : cfg->max_depth;
 
   struct Ev { hipEvent_t a, b; int kind; };
   std::vector<Ev> evs;
@@ -1073,8 +1097,8 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
       launch_shade(s->stream, s->ds, rp, s->wave, qi, d, n, grid_cap);
       ev_end(e);
       e = ev_begin(3);
-      if (cfg->estimator <= PT_EST_NEE_MIS)  // the mixture estimators trace no shadow rays
-        launch_shadow(s->stream, s->ds, s->wave, 4u * n, trav_cap, count);
+      if (cfg->estimator != PT_EST_NAIVE && cfg->estimator != PT_EST_ROULETTE)  // those trace no shadow rays
+        launch_shadow(s->stream, s->ds, s->wave, 4u * n, trav_cap, count, cfg->estimator == PT_EST_RESERVOIR);
       ev_end(e);
       qi ^= 1;
     }

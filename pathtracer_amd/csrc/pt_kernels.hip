@@ -133,7 +133,9 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
 }
 
 // ------------------------------------------------------------------ shadow
-template <bool COUNT>
+// VIS: the entries are reservoir candidates (slot field = 4 slot + candidate),
+// whose visibility is recorded instead of adding a contribution.
+template <bool COUNT, bool VIS>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_shadow(DevScene s, Wave w) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_SHADOW, false)];
   // blocks b, b + P, b + 2P, ... share partition p = b % P (the grid is a multiple of P)
@@ -159,9 +161,13 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
         ++rays;
         if (!occ) {
           uint32_t slot = w.sslot[i];
-          atomicAdd(&w.lx[slot], w.scr[i]);
-          atomicAdd(&w.ly[slot], w.scg[i]);
-          atomicAdd(&w.lz[slot], w.scb[i]);
+          if (VIS) {
+            w.rvis[slot] = 1u;
+          } else {
+            atomicAdd(&w.lx[slot], w.scr[i]);
+            atomicAdd(&w.ly[slot], w.scg[i]);
+            atomicAdd(&w.lz[slot], w.scb[i]);
+          }
         }
       },
       nn, np);
@@ -218,7 +224,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
     i += base;
     bool cont = false;
     uint32_t slot = 0;
-    f3 p = s3(0.f), wi = s3(0.f), beta = s3(0.f), pprev = s3(0.f);
+    f3 p = s3(0.f), wi = s3(0.f), beta = s3(0.f), pprev = s3(0.f), pbeta = s3(0.f);
     float ppdf = 0.f;
     uint32_t fl = 0u;
     if (active) {
@@ -239,7 +245,97 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
       f3 L = s3(0.f);  // radiance this vertex adds (emission); NEE goes through shadow rays
       bool prev_spec = fl & 1u;
       bool done = false;
-      if constexpr (EST >= PT_EST_NAIVE) {
+      if constexpr (EST == PT_EST_RESERVOIR) {
+        // getRayColor of maxdepth_reservoir_di.hpp:9-131
+        const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
+        const uint32_t ns = min(rp.light_samples, 4u);
+        (void)prev_spec;
+        // the previous vertex's resampled direct light, now that its candidates' visibility is known
+        f3 pend = s3(0.f);
+        if (fl & 2u) {
+          const uint32_t dp = d - 1u;
+          f3 sample = s3(0.f);
+          float W = 0.f;
+          for (uint32_t si = 0; si < ns; ++si) {
+            const size_t c0 = (size_t)(3u * si) * w.cap + slot;
+            f3 c = mk3(w.rc[c0], w.rc[c0 + w.cap], w.rc[c0 + 2 * (size_t)w.cap]);
+            f3 cand = scl3(c, w.rvis[4u * slot + si] ? 1.f : 0.f);
+            float wgt = len3(cand);
+            W += wgt;
+            if (pt_rng(g, rslot(dp, SLOT_RIS + si), rp.seed) < wgt / W) sample = cand;
+          }
+          pend = mul3(mk3(in.pbx[i], in.pby[i], in.pbz[i]), scl3(sample, W / (float)ns));
+        }
+        SurfHit h;
+        const pt_material_desc *mat = nullptr;
+        bool drop = false;
+        if (prim >= 0) {
+          h = surface(s, prim, o, dir, t);
+          mat = &s.mats[h.mat];
+          drop = d > 0u && mat->kind == PT_MAT_EMITTER;  // :113-115: the vertex returns 0
+        }
+        fl = 0u;
+        if (!drop) {
+          L = add3(L, pend);
+          if (prim < 0 || d >= D) {
+            L = add3(L, mul3(beta, bg));
+          } else {
+            p = h.p;
+            float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
+            float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
+            float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
+            float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
+            f3 wo = neg3(norm3(dir));
+            Scatter sc;
+            if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
+              L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+            } else {
+              float pdf;
+              if (sc.flags & BX_SPECULAR) {  // no pdf of its own: CosinePDF(n) (:39-41)
+                wi = cosine_dir(h.sp.n, u0, u1);
+                pdf = cosine_pdf(h.sp.n, wi);
+              } else {
+                wi = norm3(sc.wi);
+                pdf = bsdf_pdf<MS>(*mat, wo, wi, h.sp);
+              }
+              const float cs = fabsf(dot3(h.sp.n, wi));
+              if (N > 0 && ns > 0u) {
+                for (uint32_t si = 0; si < ns; ++si) {
+                  uint32_t ii = si / rp.ly, jj = si % rp.ly;
+                  float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
+                  float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
+                  float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
+                  float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
+                  int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
+                  const DevLight &Lg = s.lights[li];
+                  f3 nl;
+                  f3 lp = light_point(Lg, ox, oy, nl);
+                  f3 ld = sub3(lp, p);
+                  f3 lwi = norm3(ld);
+                  float len = len3(ld);
+                  float noi = fmaxf(dot3(h.sp.n, lwi), 0.f);
+                  float nol = fmaxf(dot3(nl, neg3(lwi)), 0.f);
+                  float G = nol * noi / dot3(ld, ld);
+                  f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+                  f3 le = Lg.type == LIGHT_ENV ? env_lookup_uv(s, ox, oy) : Lg.le;
+                  float area = Lg.type == LIGHT_ENV ? INFINITY : Lg.area;
+                  f3 c = scl3(mul3(f, le), area * G);
+                  const size_t c0 = (size_t)(3u * si) * w.cap + slot;
+                  w.rc[c0] = c.x; w.rc[c0 + w.cap] = c.y; w.rc[c0 + 2 * (size_t)w.cap] = c.z;
+                  w.rvis[4u * slot + si] = 0u;
+                  // only the visibility of a non-zero candidate can matter
+                  if (c.x != 0.f || c.y != 0.f || c.z != 0.f || !finite3(c))
+                    emit_shadow(w, p, lwi, len - 0.001f, c, 4u * slot + si);
+                }
+                pbeta = beta;
+                fl = 2u;
+              }
+              beta = scl3(mul3(beta, sc.att), cs / pdf);
+              cont = d + 1u <= D;  // vertex D only decides whether this vertex counts
+            }
+          }
+        }
+      } else if constexpr (EST >= PT_EST_NAIVE) {
         // maxdepth_naive.hpp:9-75 / roulette_naive.hpp:9-78: no NEE; misses and
         // the naive depth limit return bg_color (the environment is not consulted)
         const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
@@ -484,6 +580,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
       out.ppdf[j] = ppdf;
       out.px[j] = pprev.x; out.py[j] = pprev.y; out.pz[j] = pprev.z;
       out.slot[j] = slot;
+      if (EST == PT_EST_RESERVOIR) {
+        out.pbx[j] = pbeta.x; out.pby[j] = pbeta.y; out.pbz[j] = pbeta.z;
+      }
     }
   }
 }
@@ -648,15 +747,24 @@ void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, con
     case PT_EST_NEE_MIS: launch_shade_est<PT_EST_NEE_MIS>(st, s, rp, w, qi, d, g); break;
     case PT_EST_NAIVE: launch_shade_est<PT_EST_NAIVE>(st, s, rp, w, qi, d, g); break;
     case PT_EST_ROULETTE: launch_shade_est<PT_EST_ROULETTE>(st, s, rp, w, qi, d, g); break;
+    case PT_EST_RESERVOIR: launch_shade_est<PT_EST_RESERVOIR>(st, s, rp, w, qi, d, g); break;
     default: launch_shade_est<PT_EST_REFERENCE_NEE>(st, s, rp, w, qi, d, g); break;
   }
 }
-void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count) {
+void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count,
+                   bool visibility) {
   uint32_t g = part_grid(max_n, grid_cap);
-  if (count)
-    hipLaunchKernelGGL(k_shadow<true>, dim3(g), dim3(PT_BLOCK), 0, st, s, w);
-  else
-    hipLaunchKernelGGL(k_shadow<false>, dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+  if (visibility) {
+    if (count)
+      hipLaunchKernelGGL((k_shadow<true, true>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+    else
+      hipLaunchKernelGGL((k_shadow<false, true>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+  } else {
+    if (count)
+      hipLaunchKernelGGL((k_shadow<true, false>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+    else
+      hipLaunchKernelGGL((k_shadow<false, false>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+  }
 }
 void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
                  unsigned long long *stats) {

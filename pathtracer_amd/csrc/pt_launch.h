@@ -11,7 +11,8 @@ void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uin
                    bool count);
 void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
                   uint32_t max_n, uint32_t grid_cap);
-void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count);
+void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count,
+                   bool visibility);
 void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
                  unsigned long long *stats);
 void launch_sample(hipStream_t st, const DevScene &s, uint32_t sampler, uint32_t seed, uint32_t sx, uint32_t sy,

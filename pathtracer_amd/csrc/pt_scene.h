@@ -96,8 +96,9 @@ struct PathState {
   float *bx, *by, *bz;  // throughput
   float *ppdf;          // pdf of the BSDF sample that led here (MIS)
   float *px, *py, *pz;  // previous vertex (MIS)
-  uint32_t *flags;      // bit0: previous vertex specular (or camera)
+  uint32_t *flags;      // bit0: previous vertex specular (or camera); bit1: reservoir pending
   uint32_t *slot;       // the path's slot: its sample and its radiance accumulator
+  float *pbx, *pby, *pbz;  // throughput of the vertex whose resampled direct light is pending
 };
 
 // Wavefront buffers for `cap` path slots.  Live paths move between the two
@@ -123,6 +124,10 @@ struct Wave {
   // shadow queue: partition p holds entries [p * shadow_part_cap, p * shadow_part_cap + shadow(p))
   float *sox, *soy, *soz, *sdx, *sdy, *sdz, *stmax, *scr, *scg, *scb;
   uint32_t *sslot;
+  // reservoir direct light (allocated for that estimator only): candidate i of
+  // slot q at rc[(3 i + c) cap + q], its visibility at rvis[4 q + i]
+  float *rc;
+  uint8_t *rvis;
   uint32_t cap, part_cap, shadow_part_cap;
   PT_HD uint32_t *live(uint32_t b, uint32_t p) const { return counts + (b * PT_PARTS + p) * PT_COUNTER_STRIDE; }
   PT_HD uint32_t *shadow(uint32_t p) const { return counts + (2 * PT_PARTS + p) * PT_COUNTER_STRIDE; }

@@ -13,6 +13,13 @@
   (bg_color for naive, 0 for roulette) and agreement with the reference
   estimator on a scene lit by a sphere light (pins the cone pdf
   1/(2 pi (1 - cos theta_max)) and its sampler).
+* The reservoir integrator (maxdepth_reservoir_di.hpp): over the floor, its
+  estimate is the closed form of what it computes -- the selection keeps
+  candidate i with probability w_i / W and scales it by W / M, so the
+  expectation is E_x[|c(x)| c(x)] for one candidate c = f Le A G (the
+  vector length |c| = sqrt(3) c for grey values), times the probability
+  (1 - F) that the continuation ray misses the light, which would make the
+  vertex return 0.
 """
 import numpy as np
 import pytest
@@ -164,3 +171,23 @@ def test_mixture_estimators_agree_with_reference_on_a_sphere_light(estimator):
     ma, mb = a.mean((0, 1)), b.mean((0, 1))
     se = np.sqrt(va.mean((0, 1)) / (32 * 32 * 144) + vb.mean((0, 1)) / (64 * 64 * 144))
     assert np.all(np.abs(ma - mb) < 5 * se + 0.01 * ma), (ma, mb, se)
+
+
+def test_reservoir_closed_form():
+    A, B, h, albedo, Le = 1.0, 0.6, 1.5, 0.6, 5.0
+    spec, _ = floor_under_light(A, B, h, albedo, Le)
+    o = O.OracleScene(spec)
+    n = 192
+    img, var, _, _ = o.render(o.cfg(estimator=4, max_depth=1, spp_x=n, spp_y=n), want_var=True)
+    # E_x over the light of c(x)^2, c = (a/pi) Le A h^2 / d^4 (cos_i = cos_l = h/d)
+    g, wts = np.polynomial.legendre.leggauss(64)
+    u = g * A / 2
+    w = g * B / 2
+    U, Wg = np.meshgrid(u, w, indexing="ij")
+    WW = np.outer(wts, wts) / 4.0
+    c = albedo / np.pi * Le * (A * B) * h * h / (U * U + h * h + Wg * Wg) ** 2
+    e_c2 = float(np.sum(WW * c * c))
+    F = 4 * _corner_form_factor(A / 2, B / 2, h)
+    expect = (1 - F) * np.sqrt(3.0) * e_c2
+    se = np.sqrt(var[0, 0, 0] / (n * n))
+    assert abs(img[0, 0, 0] - expect) < max(4 * se, 3e-3 * expect), (img[0, 0, 0], expect, se)
