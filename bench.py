@@ -113,6 +113,106 @@ def quality(scene, estimator, res=128, spp_side=8, ref_side=64):
             "resolution": f"{res}x{res}", "ms_render": round(t_img * 1e3, 1), "ms_ref": round(t_ref * 1e3, 1)}
 
 
+def kernel_roofline(cst, ms, launches, read_traffic_for=None):
+    """Per-kernel algorithmic bytes (from the frame's counters) over HIP-event
+    time; the roofline object is for the kernel with the most device time."""
+    kernels = {
+        "k_extend": ("closest-hit traversal", cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE +
+                     cst["prim_tests"] * B_PRIM, ms["ms_extend"]),
+        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_NODE +
+                     cst["shadow_prim_tests"] * B_PRIM, ms["ms_shadow"]),
+        "k_shade": ("materials, lights, NEE, queue compaction", cst["segments"] * B_SHADE_SEG +
+                    cst["shadow_rays"] * B_SHADE_SHADOW, ms["ms_shade"]),
+    }
+    per_kernel = {k: {"what": w, "gb_per_frame": round(b / 1e9, 3), "ms_per_frame": round(t, 2),
+                      "achieved_gbs": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else 0.0}
+                  for k, (w, b, t) in kernels.items()}
+    kname = max(kernels, key=lambda k: kernels[k][2])
+    _, kbytes, kms = kernels[kname]
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": (read_traffic(kname, read_traffic_for)
+                                                             if read_traffic_for is not None else None),
+                "kernel": kname, "launches_per_frame": launches, "bytes_per_launch": int(kbytes / max(launches, 1)),
+                "ms_per_launch": round(kms / max(launches, 1), 4)}
+    return roofline, per_kernel
+
+
+def main_batch(args):
+    """64 independent renders (seed 4): displaced geodesic spheres of 1e4-5e5
+    triangles (log-uniform), 1-4 quad lights, a material from {Lambertian,
+    conductor, rough dielectric, Disney}, 128^2-1024^2 pixels, 4-256 spp
+    (log-uniform), depth 8-16.  One step renders all 64 in one wavefront
+    (captured chunk graph replayed per chunk); the same 64 renders launched
+    one after another with pt_render are timed beside it."""
+    from pathtracer_amd import _lib as L, render as R, scenes
+    t0 = time.perf_counter()
+    specs = scenes.batch64(seed=4, n=8 if args.small else 64)
+    if args.small:
+        for sp in specs:
+            sp.render.update(width=64, height=64)
+    log(f"{len(specs)} scenes generated in {time.perf_counter() - t0:.1f} s, "
+        f"{sum(sp.n_triangles for sp in specs)} triangles")
+    t0 = time.perf_counter()
+    scs = [R.Scene(sp) for sp in specs]
+    log(f"BVHs built and uploaded in {time.perf_counter() - t0:.1f} s")
+    cfgs = [sc.cfg(estimator=args.estimator) for sc in scs]
+    total = sum(c.width * c.height * c.spp_x * c.spp_y for c in cfgs)
+    b = R.Batch()
+    for sc, c in zip(scs, cfgs):
+        b.add(sc, c)
+    tuning = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuning = b.autotune(spp_div=8)
+        log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuning}")
+    # counting + per-stage timing pass (eager launches, untimed)
+    cst = b.render(flags=L.RENDER_COUNTERS | L.RENDER_TIMING | L.BATCH_NO_GRAPH)
+    for i in range(args.warmup):
+        b.render()
+        log(f"warmup {i} done")
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        st = b.render()
+        log(f"step {i}: {st['ms_total']:.1f} ms")
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    dt = time.perf_counter() - t1
+    value = total * args.steps / dt / 1e6
+    # the same renders one after another (plain pt_render, default launch parameters)
+    for sc, c in zip(scs, cfgs):
+        sc.render_device(c)
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    t2 = time.perf_counter()
+    for sc, c in zip(scs, cfgs):
+        sc.render_device(c)
+    L.check(L.lib.pt_device_synchronize(), "sync")
+    sdt = time.perf_counter() - t2
+    serial = {"value": round(total / sdt / 1e6, 2), "ms_per_step": round(sdt * 1e3, 2),
+              "what": "the same renders launched one after another with pt_render"}
+    log(f"serial: {serial}")
+    ms = {k: cst[k] for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate", "ms_film")}
+    roofline, per_kernel = kernel_roofline(cst, ms, cst["extend_launches"])
+    res = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch64: 64 seeded renders (displaced geodesic spheres 1e4-5e5 tris, 1-4 quad "
+                               "lights, 128^2-1024^2, 4-256 spp, depth 8-16) in one wavefront",
+                   "renders": len(specs), "path_samples_per_step": total,
+                   "triangles": sum(sp.n_triangles for sp in specs),
+                   "estimator": ["reference_nee", "nee_mis"][args.estimator], "tuning": tuning},
+        "serial": serial, "speedup_vs_serial": round(sdt / (dt / args.steps), 3),
+        "mrays_per_s": round((cst["segments"] + cst["shadow_rays"]) * args.steps / dt / 1e6, 1),
+        "kernel_ms_per_step_eager": {k: round(v, 2) for k, v in ms.items()},
+        "roofline": roofline, "kernels": per_kernel, "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(specs[0], cfgs[0], target_s=10.0)
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +223,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--small", action="store_true", help="256x256 quick check (not a bench line)")
     ap.add_argument("--save", default="", help="write the film as PFM/PNG with this path prefix")
+    ap.add_argument("--workload", default="glass_hdri", choices=("glass_hdri", "batch64"),
+                    help="glass_hdri: the single triangle-heavy frame (default); batch64: 64 uneven renders "
+                         "in one wavefront against 64 serial renders")
+    ap.add_argument("--no-tune", action="store_true", help="batch64: skip the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -130,6 +234,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batch64":
+        return main_batch(args)
     from pathtracer_amd import _lib as L, render as R, scenes
 
     t0 = time.perf_counter()
@@ -181,24 +287,7 @@ def main():
     ms = {k: sum(s[k] for s in stats) / args.steps for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate",
                                                               "ms_film")}
     launches = stats[0]["extend_launches"]
-    kernels = {
-        "k_extend": ("closest-hit traversal", cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE +
-                     cst["prim_tests"] * B_PRIM, ms["ms_extend"]),
-        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_NODE +
-                     cst["shadow_prim_tests"] * B_PRIM, ms["ms_shadow"]),
-        "k_shade": ("materials, lights, NEE, queue compaction", cst["segments"] * B_SHADE_SEG +
-                    cst["shadow_rays"] * B_SHADE_SHADOW, ms["ms_shade"]),
-    }
-    per_kernel = {k: {"what": w, "gb_per_frame": round(b / 1e9, 3), "ms_per_frame": round(t, 2),
-                      "achieved_gbs": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else 0.0}
-                  for k, (w, b, t) in kernels.items()}
-    kname = max(kernels, key=lambda k: kernels[k][2])
-    _, kbytes, kms = kernels[kname]
-    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(kname, args.estimator),
-                "kernel": kname, "launches_per_frame": launches, "bytes_per_launch": int(kbytes / max(launches, 1)),
-                "ms_per_launch": round(kms / max(launches, 1), 4)}
+    roofline, per_kernel = kernel_roofline(cst, ms, launches, read_traffic_for=args.estimator)
 
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,

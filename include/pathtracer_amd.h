@@ -235,6 +235,72 @@ pt_status pt_film_read(pt_scene *scene, float *out_rgb);
 pt_status pt_film_read_samples(pt_scene *scene, float *out_rgb, size_t n);
 
 /* ------------------------------------------------------------------ */
+/* Launch tuning and the autotuner (no counterpart in the reference,    */
+/* whose OpenMP loop has no launch parameters, renderManager.hpp:99).   */
+/* ------------------------------------------------------------------ */
+typedef struct pt_tuning {
+  uint32_t wavefront;            /* path slots in flight (0 = 8M); a render's
+                                    pt_render_cfg.wavefront overrides it          */
+  uint32_t trav_blocks_per_cu;   /* traversal grid, blocks of 256 per CU (0 = 8) */
+  uint32_t shade_blocks_per_cu;  /* shading grid cap, blocks per CU (0 = 32)     */
+  uint32_t leaf_size;            /* BVH leaf size of the last commit (autotuner
+                                    output; pt_scene_set_tuning ignores it)     */
+} pt_tuning;
+pt_status pt_scene_set_tuning(pt_scene *scene, const pt_tuning *tuning);
+
+enum {
+  PT_TUNE_WAVEFRONT = 1u << 0,   /* try wavefront sizes 1M..16M                  */
+  PT_TUNE_GRIDS = 1u << 1,       /* try traversal / shading grid sizes           */
+  PT_TUNE_LEAF = 1u << 2         /* rebuild the BVH with leaf sizes 1, 2, 4, 8   */
+};
+typedef struct pt_autotune_cfg {
+  uint32_t flags;      /* PT_TUNE_* (0 = all)                                     */
+  uint32_t spp_div;    /* time candidates on 1/spp_div of the samples (0 = 8)     */
+  uint32_t repeats;    /* timed renders per candidate, the fastest kept (0 = 1)  */
+} pt_autotune_cfg;
+/* Times short renders of cfg's frame for each candidate and keeps the
+ * fastest: the result is stored with pt_scene_set_tuning, and with
+ * PT_TUNE_LEAF the scene stays committed with the best leaf size.          */
+pt_status pt_autotune(pt_scene *scene, const pt_render_cfg *cfg,
+                      const pt_autotune_cfg *tc, pt_tuning *out);
+
+/* ------------------------------------------------------------------ */
+/* Batched rendering (no counterpart in the reference: it renders one   */
+/* SceneInfo per call, renderManager.hpp:51-158).  Many renders of      */
+/* uneven resolution, spp and scene size share one wavefront: their     */
+/* path samples are concatenated and cut into chunks, and one captured  */
+/* HIP graph (the launches of a chunk) is replayed per chunk.           */
+/* ------------------------------------------------------------------ */
+typedef struct pt_batch pt_batch;
+pt_status pt_batch_create(pt_batch **out);
+void pt_batch_destroy(pt_batch *batch);
+/* Adds a render of a committed scene (the scene must outlive the batch;
+ * one scene may appear in several renders).  All renders of a batch use
+ * one estimator.  PT_RENDER_KEEP_SAMPLES is not supported.                 */
+pt_status pt_batch_add(pt_batch *batch, pt_scene *scene, const pt_render_cfg *cfg,
+                       uint32_t *out_index);
+/* Changes the configuration of render `index` (e.g. its seed).            */
+pt_status pt_batch_set_cfg(pt_batch *batch, uint32_t index, const pt_render_cfg *cfg);
+
+enum {
+  PT_BATCH_NO_GRAPH = 1u << 8    /* launch eagerly instead of replaying a graph */
+};
+typedef struct pt_batch_cfg {
+  uint32_t wavefront;   /* path slots in flight (0 = the batch tuning, else 8M) */
+  uint32_t flags;       /* PT_RENDER_COUNTERS, PT_RENDER_TIMING (eager only),
+                           PT_BATCH_NO_GRAPH                                    */
+} pt_batch_cfg;
+/* Renders every render of the batch into its own film.  stats sums over
+ * the renders.                                                            */
+pt_status pt_batch_render(pt_batch *batch, const pt_batch_cfg *cfg, pt_stats *stats);
+/* Film of render `index` (width*height*3 per-pixel means, as pt_film_read). */
+pt_status pt_batch_film_read(pt_batch *batch, uint32_t index, float *out_rgb);
+pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
+/* Picks the wavefront size (and grid sizes) for the batch as pt_autotune
+ * does for one render; leaf sizes are not tuned here.                     */
+pt_status pt_batch_autotune(pt_batch *batch, const pt_autotune_cfg *tc, pt_tuning *out);
+
+/* ------------------------------------------------------------------ */
 /* Parity / test entry points (batched, executed on the device).        */
 /* ------------------------------------------------------------------ */
 

@@ -26,6 +26,8 @@ EST_REFERENCE_NEE, EST_NEE_MIS, EST_NAIVE, EST_ROULETTE, EST_RESERVOIR = 0, 1, 2
 # render flags
 RENDER_COUNTERS, RENDER_NO_SORT, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
+BATCH_NO_GRAPH = 256
+TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF = 1, 2, 4
 PRIM_FLIP_FACE, PRIM_FORCE_INSIDE, PRIM_NOT_LIGHT = 1, 2, 4
 NPARAMS = 30
 
@@ -78,11 +80,30 @@ class SceneInfo(ctypes.Structure):
                 ("build_ms", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("wavefront", ctypes.c_uint32), ("trav_blocks_per_cu", ctypes.c_uint32),
+                ("shade_blocks_per_cu", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class AutotuneCfg(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint32), ("spp_div", ctypes.c_uint32), ("repeats", ctypes.c_uint32)]
+
+
+class BatchCfg(ctypes.Structure):
+    _fields_ = [("wavefront", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
 # every symbol include/pathtracer_amd.h declares
 EXPORTED = [
     "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_mesh", "pt_scene_add_mesh_normals",
     "pt_scene_add_sphere", "pt_scene_add_rect", "pt_scene_set_env", "pt_scene_set_camera",
     "pt_scene_commit", "pt_scene_get_info", "pt_render", "pt_film_read", "pt_film_read_samples",
+    "pt_scene_set_tuning", "pt_autotune",
+    "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_cfg", "pt_batch_render",
+    "pt_batch_film_read", "pt_batch_set_tuning", "pt_batch_autotune",
     "pt_sample", "pt_intersect", "pt_bsdf_eval", "pt_bsdf_sample", "pt_light_sample",
     "pt_device_synchronize", "pt_last_error", "pt_version",
 ]
@@ -113,6 +134,16 @@ def _load():
         "pt_render": (i32, [_P, ctypes.POINTER(RenderCfg), ctypes.POINTER(Stats)]),
         "pt_film_read": (i32, [_P, _P]),
         "pt_film_read_samples": (i32, [_P, _P, sz]),
+        "pt_scene_set_tuning": (i32, [_P, ctypes.POINTER(Tuning)]),
+        "pt_autotune": (i32, [_P, ctypes.POINTER(RenderCfg), ctypes.POINTER(AutotuneCfg), ctypes.POINTER(Tuning)]),
+        "pt_batch_create": (i32, [ctypes.POINTER(_P)]),
+        "pt_batch_destroy": (None, [_P]),
+        "pt_batch_add": (i32, [_P, _P, ctypes.POINTER(RenderCfg), ctypes.POINTER(u32)]),
+        "pt_batch_set_cfg": (i32, [_P, u32, ctypes.POINTER(RenderCfg)]),
+        "pt_batch_render": (i32, [_P, ctypes.POINTER(BatchCfg), ctypes.POINTER(Stats)]),
+        "pt_batch_film_read": (i32, [_P, u32, _P]),
+        "pt_batch_set_tuning": (i32, [_P, ctypes.POINTER(Tuning)]),
+        "pt_batch_autotune": (i32, [_P, ctypes.POINTER(AutotuneCfg), ctypes.POINTER(Tuning)]),
         "pt_sample": (i32, [u32, u32, u32, _P, _P, _P, sz, _P]),
         "pt_intersect": (i32, [_P, _P, sz, ctypes.c_int, _P, _P]),
         "pt_bsdf_eval": (i32, [ctypes.POINTER(MaterialDesc), _P, _P, _P, _P, sz, _P, _P]),

@@ -285,12 +285,16 @@ PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tm
 // next rays of the range once REFILL of them are idle, so lanes do not idle
 // while the wave's longest ray finishes.  The per-ray results equal traverse<>'s:
 // the same node order, the same culling and the same primitive tests.
-// fetch(i, o, d, tmin, tmax) loads ray i; finish(i, hit, t, prim) stores it.
-// begin, end must be the same in every lane of the wave.
+// fetch(i, o, d, tmin, tmax, scene) loads ray i (and, batched, points scene
+// at the ray's instance); finish(i, hit, t, prim) stores it.  begin, end must
+// be the same in every lane of the wave.
 // ---------------------------------------------------------------------
-template <bool ANY, bool COUNT, int K, int REFILL, class Fetch, class Finish>
+template <bool ANY, bool COUNT, int K, int REFILL, bool BATCH, class Fetch, class Finish>
 PT_D void trace_range(const DevScene &s, int *lds, uint32_t begin, uint32_t end, Fetch fetch, Finish finish,
                       uint32_t &n_nodes, uint32_t &n_prims) {
+  // BATCH: fetch names each ray's scene, whose tree the lane then walks
+  const float4 *lane_nodes = s.nodes, *lane_prims = s.prims;
+  int lane_root = s.root;
   constexpr bool WT = !ANY;
   int pc[PT_STACK4];
   float ptv[WT ? PT_STACK4 : 1];
@@ -345,12 +349,18 @@ PT_D void trace_range(const DevScene &s, int *lds, uint32_t begin, uint32_t end,
       if (ray < 0 && mine < end) {
         ray = (int)mine;
         float tmax;
-        fetch(mine, o, d, tmin, tmax);
+        const DevScene *sc = &s;
+        fetch(mine, o, d, tmin, tmax, sc);
+        if (BATCH) {
+          lane_nodes = sc->nodes;
+          lane_prims = sc->prims;
+          lane_root = sc->root;
+        }
         ix = rcp_fast(d.x); iy = rcp_fast(d.y); iz = rcp_fast(d.z);
         oix = o.x * ix; oiy = o.y * iy; oiz = o.z * iz;
         tb = tmax;
         best = -1;
-        cur = s.root;
+        cur = BATCH ? lane_root : s.root;
         sp = 0;
         lo = 0;
       }
@@ -361,7 +371,7 @@ PT_D void trace_range(const DevScene &s, int *lds, uint32_t begin, uint32_t end,
     bool need_pop = true;
     bool done = false;
     if (cur >= 0) {
-      const float4 *nd = s.nodes + 8 * (size_t)cur;
+      const float4 *nd = (BATCH ? lane_nodes : s.nodes) + 8 * (size_t)cur;
       const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
       const int4 ch = *reinterpret_cast<const int4 *>(nd + 6);
       if (COUNT) ++n_nodes;
@@ -404,7 +414,7 @@ PT_D void trace_range(const DevScene &s, int *lds, uint32_t begin, uint32_t end,
         float t;
         uint32_t id;
         if (COUNT) ++n_prims;
-        if (hit_prim(s.prims, q, o, d, tmin, tb, t, id)) {
+        if (hit_prim(BATCH ? lane_prims : s.prims, q, o, d, tmin, tb, t, id)) {
           tb = t;
           best = (int)id;
           if (ANY) {

@@ -86,6 +86,13 @@ static hipError_t upload(DevBuf &b, const void *src, size_t n) {
   return hipSuccess;
 }
 
+// Wavefront state of one render stream (a scene's own renders, or a batch).
+struct WaveStore {
+  DevBuf buf, counts, stats, res;
+  Wave wave{};
+  uint32_t cap = 0;
+};
+
 // ------------------------------------------------------------------ host-side scene records
 struct HTri { f3 p[3]; uint32_t mesh; int smooth; };  // smooth: index into pt_scene::normals3 or -1
 struct HMesh { uint32_t first, count, mat, flags, seq; };
@@ -118,9 +125,10 @@ struct pt_scene {
   std::vector<DevLight> lights;
   pt_scene_info info{};
   // wavefront
-  DevBuf w_buf, w_counts, w_stats, d_off, w_res;
-  Wave wave{};
-  uint32_t wave_cap = 0;
+  WaveStore ws;
+  DevBuf d_off;
+  pt_tuning tuning{};
+  pt_build_cfg build_cfg{};
   DevBuf film, samples;
   uint32_t film_w = 0, film_h = 0;
   uint64_t film_spp = 0;
@@ -546,8 +554,7 @@ static const int kHostPrimes[25] = {2,  3,  5,  7,  11, 13, 17, 19, 23, 29, 31, 
                                     43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97};
 
 // DigitPermutation (sampler/digitPerm.hpp:12-56) for all 25 bases.
-static pt_status ensure_perm(pt_scene *s, uint32_t seed) {
-  if (s->perm_valid && s->perm_seed == seed) return PT_OK;
+static pt_status build_perm(uint32_t seed, DevBuf &d_tab, DevBuf &d_off) {
   std::vector<uint16_t> tab;
   std::vector<uint32_t> off(25);
   for (int bi = 0; bi < 25; ++bi) {
@@ -567,8 +574,14 @@ static pt_status ensure_perm(pt_scene *s, uint32_t seed) {
         tab.push_back((uint16_t)permutation_element((uint32_t)dv, (uint32_t)base, (uint32_t)dseed));
     }
   }
-  HIP_TRY(upload(s->d_perm, tab.data(), tab.size() * sizeof(uint16_t)));
-  HIP_TRY(upload(s->d_perm_off, off.data(), off.size() * sizeof(uint32_t)));
+  HIP_TRY(upload(d_tab, tab.data(), tab.size() * sizeof(uint16_t)));
+  HIP_TRY(upload(d_off, off.data(), off.size() * sizeof(uint32_t)));
+  return PT_OK;
+}
+static pt_status ensure_perm(pt_scene *s, uint32_t seed) {
+  if (s->perm_valid && s->perm_seed == seed) return PT_OK;
+  pt_status st = build_perm(seed, s->d_perm, s->d_perm_off);
+  if (st != PT_OK) return st;
   s->perm_seed = seed;
   s->perm_valid = true;
   s->ds.perm = s->d_perm.as<uint16_t>();
@@ -648,6 +661,8 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   // -- primitive references and per-primitive info
   Builder B;
   B.leaf_size = (cfg && cfg->leaf_size) ? std::min<uint32_t>(cfg->leaf_size, 16u) : 4u;
+  s->build_cfg = cfg ? *cfg : pt_build_cfg{};
+  s->tuning.leaf_size = B.leaf_size;
   B.bins = (cfg && cfg->sah_bins) ? std::min<uint32_t>(std::max<uint32_t>(cfg->sah_bins, 2u), 64u) : 16u;
   B.refs.resize(nprim);
   std::vector<uint2> info(nprim);
@@ -864,8 +879,8 @@ extern "C" pt_status pt_scene_get_info(const pt_scene *s, pt_scene_info *info) {
 }
 
 // ------------------------------------------------------------------ wavefront buffers
-static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
-  if (s->wave_cap >= cap && s->w_buf.p) return PT_OK;
+static pt_status ensure_wave(WaveStore &ws, uint32_t cap) {
+  if (ws.cap >= cap && ws.buf.p) return PT_OK;
   const size_t per_state = 20;   // 4-byte words per entry of one live-path buffer
   const size_t per_slot = 3;     // radiance per slot
   const size_t per_shadow = 11;  // 4-byte words per shadow-queue entry
@@ -880,16 +895,16 @@ static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
   const size_t lcap = (size_t)(part_cap * PT_PARTS);
   const size_t scap = (size_t)(shadow_part_cap * PT_PARTS);
   size_t bytes = (2 * per_state * lcap + per_slot * (size_t)cap + per_shadow * scap) * 4 + n_arrays * 256;
-  HIP_TRY(s->w_buf.alloc(bytes));
-  HIP_TRY(s->w_counts.alloc(PT_COUNTER_WORDS * sizeof(uint32_t)));
-  HIP_TRY(s->w_stats.alloc(CNT_N * sizeof(unsigned long long)));
-  char *p = s->w_buf.as<char>();
+  HIP_TRY(ws.buf.alloc(bytes));
+  HIP_TRY(ws.counts.alloc(PT_COUNTER_WORDS * sizeof(uint32_t)));
+  HIP_TRY(ws.stats.alloc(CNT_N * sizeof(unsigned long long)));
+  char *p = ws.buf.as<char>();
   auto take = [&](size_t n) {
     char *r = p;
     p += ((n * 4 + 255) / 256) * 256;
     return r;
   };
-  Wave &w = s->wave;
+  Wave &w = ws.wave;
   for (int b = 0; b < 2; ++b) {
     PathState &ps = w.st[b];
     ps.ox = (float *)take(lcap); ps.oy = (float *)take(lcap); ps.oz = (float *)take(lcap);
@@ -908,28 +923,28 @@ static pt_status ensure_wave(pt_scene *s, uint32_t cap) {
   w.stmax = (float *)take(scap);
   w.scr = (float *)take(scap); w.scg = (float *)take(scap); w.scb = (float *)take(scap);
   w.sslot = (uint32_t *)take(scap);
-  if ((size_t)(p - s->w_buf.as<char>()) > s->w_buf.bytes) return fail(PT_ERR_INVALID, "internal: wave layout overflow");
-  w.counts = s->w_counts.as<uint32_t>();
-  w.stats = s->w_stats.as<unsigned long long>();
+  if ((size_t)(p - ws.buf.as<char>()) > ws.buf.bytes) return fail(PT_ERR_INVALID, "internal: wave layout overflow");
+  w.counts = ws.counts.as<uint32_t>();
+  w.stats = ws.stats.as<unsigned long long>();
   w.cap = cap;
   w.part_cap = (uint32_t)part_cap;
   w.shadow_part_cap = (uint32_t)shadow_part_cap;
   w.rc = nullptr;
   w.rvis = nullptr;
-  s->w_res.release();
-  s->wave_cap = cap;
+  ws.res.release();
+  ws.cap = cap;
   return PT_OK;
 }
 
 // The reservoir estimator's candidate buffers (4 candidates per slot).
-static pt_status ensure_reservoir(pt_scene *s) {
-  Wave &w = s->wave;
+static pt_status ensure_reservoir(WaveStore &ws) {
+  Wave &w = ws.wave;
   if (w.rc) return PT_OK;
   const size_t cap = w.cap;
   const size_t rc_bytes = ((12 * cap * 4 + 255) / 256) * 256;
-  HIP_TRY(s->w_res.alloc(rc_bytes + 4 * cap));
-  w.rc = s->w_res.as<float>();
-  w.rvis = reinterpret_cast<uint8_t *>(s->w_res.as<char>() + rc_bytes);
+  HIP_TRY(ws.res.alloc(rc_bytes + 4 * cap));
+  w.rc = ws.res.as<float>();
+  w.rvis = reinterpret_cast<uint8_t *>(ws.res.as<char>() + rc_bytes);
   return PT_OK;
 }
 
@@ -965,38 +980,206 @@ static void grid_factor(uint32_t n, uint32_t &a, uint32_t &b) {
 }
 
 // ------------------------------------------------------------------ render
-extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *stats) {
-  if (!s || !cfg) return fail(PT_ERR_INVALID, "pt_render: NULL argument");
-  if (!s->committed) return fail(PT_ERR_INVALID, "pt_render: scene not committed");
-  if (!s->has_cam) return fail(PT_ERR_INVALID, "pt_render: camera is not initialized");  // renderManager.hpp:58-62
+// Checks a render configuration (the preconditions of SceneInfo::render,
+// renderManager.hpp:51-70, and this renderer's limits).  spp: samples per
+// pixel of the frame; sppc: samples this call renders per pixel.
+static pt_status check_cfg(const char *who, const pt_scene *s, const pt_render_cfg *cfg, uint32_t &spp,
+                           uint32_t &sppc) {
+  const std::string w(who);
+  if (!s || !cfg) return fail(PT_ERR_INVALID, w + ": NULL argument");
+  if (!s->committed) return fail(PT_ERR_INVALID, w + ": scene not committed");
+  if (!s->has_cam) return fail(PT_ERR_INVALID, w + ": camera is not initialized");  // renderManager.hpp:58-62
   if (cfg->width == 0 || cfg->height == 0 || cfg->spp_x == 0 || cfg->spp_y == 0)
-    return fail(PT_ERR_INVALID, "pt_render: width, height, spp_x, spp_y must be > 0");
-  if (cfg->estimator > PT_EST_RESERVOIR) return fail(PT_ERR_INVALID, "pt_render: unknown estimator");
-  if (cfg->sampler > PT_SAMPLER_STRATIFIED) return fail(PT_ERR_INVALID, "pt_render: unknown sampler");
+    return fail(PT_ERR_INVALID, w + ": width, height, spp_x, spp_y must be > 0");
+  if (cfg->estimator > PT_EST_RESERVOIR) return fail(PT_ERR_INVALID, w + ": unknown estimator");
+  if (cfg->sampler > PT_SAMPLER_STRATIFIED) return fail(PT_ERR_INVALID, w + ": unknown sampler");
   if (cfg->estimator == PT_EST_REFERENCE_NEE && cfg->light_samples > 4)
-    return fail(PT_ERR_UNSUPPORTED, "pt_render: at most 4 light samples per vertex");
-  const uint32_t spp = cfg->spp_x * cfg->spp_y;
-  if (cfg->spp_begin >= spp) return fail(PT_ERR_INVALID, "pt_render: spp_begin out of range");
-  const uint32_t sppc = cfg->spp_count ? std::min(cfg->spp_count, spp - cfg->spp_begin) : spp - cfg->spp_begin;
+    return fail(PT_ERR_UNSUPPORTED, w + ": at most 4 light samples per vertex");
+  spp = cfg->spp_x * cfg->spp_y;
+  if (cfg->spp_begin >= spp) return fail(PT_ERR_INVALID, w + ": spp_begin out of range");
+  sppc = cfg->spp_count ? std::min(cfg->spp_count, spp - cfg->spp_begin) : spp - cfg->spp_begin;
   const uint64_t npix = (uint64_t)cfg->width * cfg->height;
-  if (npix * spp >= (1ull << 32)) return fail(PT_ERR_INVALID, "pt_render: width*height*spp must be < 2^32");
+  if (npix * spp >= (1ull << 32)) return fail(PT_ERR_INVALID, w + ": width*height*spp must be < 2^32");
+  return PT_OK;
+}
+
+static RenderParams make_params(const pt_render_cfg *cfg, uint32_t spp, uint32_t sppc) {
+  RenderParams rp{};
+  rp.width = cfg->width;
+  rp.height = cfg->height;
+  rp.spp_x = cfg->spp_x;
+  rp.spp_y = cfg->spp_y;
+  rp.spp = spp;
+  rp.spp_begin = cfg->spp_begin;
+  rp.spp_count = sppc;
+  rp.max_depth = cfg->max_depth;
+  rp.sampler = cfg->sampler;
+  rp.estimator = cfg->estimator;
+  rp.rr_depth = cfg->rr_depth;
+  rp.light_samples = cfg->light_samples;
+  grid_factor(std::max(1u, cfg->light_samples), rp.lx, rp.ly);
+  rp.seed = cfg->seed;
+  rp.bg[0] = cfg->bg[0]; rp.bg[1] = cfg->bg[1]; rp.bg[2] = cfg->bg[2];
+  return rp;
+}
+
+// Per-frame jittered pixel offsets (FrameBuffer::setOffsets,
+// framebuffer.cpp:131-145): off[k] and off[spp + k] for sample k.
+static void pixel_offsets(const pt_render_cfg *cfg, uint32_t spp, float *off) {
+  for (uint32_t i = 0; i < cfg->spp_x; ++i)
+    for (uint32_t j = 0; j < cfg->spp_y; ++j) {
+      uint32_t k = i * cfg->spp_y + j;
+      float a0 = (float)i / (float)cfg->spp_x, a1 = (float)(i + 1) / (float)cfg->spp_x;
+      float b0 = (float)j / (float)cfg->spp_y, b1 = (float)(j + 1) / (float)cfg->spp_y;
+      off[k] = (float)(a0 + (a1 - a0) * (double)pt_rng(0xffffffffu - k, 0u, cfg->seed ^ 0x5bd1e995u));
+      off[spp + k] = (float)(b0 + (b1 - b0) * (double)pt_rng(0xffffffffu - k, 1u, cfg->seed ^ 0x5bd1e995u));
+    }
+}
+
+// Path vertices traced per path.  NEE+MIS and the reservoir estimator trace
+// one vertex more: the hit that weights (MIS) or drops (reservoir) the last
+// vertex's direct light.
+static uint32_t n_vertices(const pt_render_cfg *cfg) {
+  return (cfg->estimator == PT_EST_NEE_MIS || cfg->estimator == PT_EST_RESERVOIR) ? cfg->max_depth + 1
+                                                                                  : cfg->max_depth;
+}
+
+// Launch grids.  Shading: at most shade_per_cu blocks of 256 per CU (4
+// resident, each looping).  Traversal: one resident round (8 waves per SIMD
+// = 8 blocks per CU), each wave refilling its lanes from its share of the
+// rays.  Both are multiples of the partition count.
+struct Grids {
+  uint32_t shade, trav;
+};
+static Grids make_grids(uint32_t n_cu, const pt_tuning &t) {
+  const uint32_t tb = t.trav_blocks_per_cu ? t.trav_blocks_per_cu : 8u;
+  const uint32_t sb = t.shade_blocks_per_cu ? t.shade_blocks_per_cu : 32u;
+  Grids g;
+  g.shade = std::max<uint32_t>(PT_PARTS, n_cu * sb / PT_PARTS * PT_PARTS);
+  g.trav = std::max<uint32_t>(PT_PARTS, n_cu * tb / PT_PARTS * PT_PARTS);
+  return g;
+}
+
+// HIP-event timing of the stages (generate, extend, shade, shadow, film).
+struct StageEvents {
+  struct Ev { hipEvent_t a, b; int kind; };
+  bool on = false;
+  hipStream_t st = nullptr;
+  std::vector<Ev> evs;
+  int begin(int kind) {
+    if (!on) return -1;
+    Ev e;
+    (void)hipEventCreate(&e.a);
+    (void)hipEventCreate(&e.b);
+    e.kind = kind;
+    (void)hipEventRecord(e.a, st);
+    evs.push_back(e);
+    return (int)evs.size() - 1;
+  }
+  void end(int i) {
+    if (i >= 0) (void)hipEventRecord(evs[i].b, st);
+  }
+  void add_to(pt_stats *stats) const {
+    for (const Ev &ev : evs) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, ev.a, ev.b);
+      double *dst[5] = {&stats->ms_generate, &stats->ms_extend, &stats->ms_shade, &stats->ms_shadow, &stats->ms_film};
+      *dst[ev.kind] += t;
+    }
+  }
+  ~StageEvents() {
+    for (Ev &ev : evs) {
+      (void)hipEventDestroy(ev.a);
+      (void)hipEventDestroy(ev.b);
+    }
+  }
+};
+
+// The launches of one chunk: camera rays, nverts x (extend, shade, shadow),
+// film.  b == nullptr renders rp's chunk of one scene; otherwise the batch's
+// current chunk, and rp only carries the estimator and the largest chunk.
+struct ChunkPlan {
+  uint32_t max_paths;  // grid sizing: largest chunk
+  uint32_t max_pix;
+  uint32_t nverts;
+  uint32_t estimator;
+  uint32_t mat_kinds;
+  bool count;
+  Grids grids;
+};
+static void launch_chunk(hipStream_t st, const DevScene &s, const DevCamera &cam, const RenderParams &rp,
+                         const Wave &w, const float *offx, const float *offy, float *film, float *samples,
+                         const BatchView *b, const ChunkPlan &pl, StageEvents *ev) {
+  const uint32_t n = pl.max_paths;
+  int e = ev ? ev->begin(0) : -1;
+  launch_generate(st, s, cam, rp, w, offx, offy, b);
+  if (ev) ev->end(e);
+  int qi = 0;
+  for (uint32_t d = 0; d < pl.nverts; ++d) {
+    e = ev ? ev->begin(1) : -1;
+    launch_extend(st, s, w, qi, n, pl.grids.trav, pl.count, b);
+    if (ev) ev->end(e);
+    e = ev ? ev->begin(2) : -1;
+    launch_shade(st, s, rp, w, qi, d, n, pl.grids.shade, b, pl.mat_kinds);
+    if (ev) ev->end(e);
+    e = ev ? ev->begin(3) : -1;
+    if (pl.estimator != PT_EST_NAIVE && pl.estimator != PT_EST_ROULETTE)  // those trace no shadow rays
+      launch_shadow(st, s, w, 4u * n, pl.grids.trav, pl.count, pl.estimator == PT_EST_RESERVOIR, b);
+    if (ev) ev->end(e);
+    qi ^= 1;
+  }
+  e = ev ? ev->begin(4) : -1;
+  launch_film(st, rp, w, film, samples, w.stats, b, pl.max_pix);
+  if (ev) ev->end(e);
+  if (b) launch_next_chunk(st, *b);
+}
+
+static void fill_stats(pt_stats *stats, const unsigned long long *c, uint64_t paths, uint64_t launches, float ms) {
+  std::memset(stats, 0, sizeof(*stats));
+  stats->ms_total = ms;
+  stats->paths = paths;
+  stats->segments = c[CNT_SEGMENTS];
+  stats->shadow_rays = c[CNT_SHADOW_RAYS];
+  stats->node_visits = c[CNT_NODES];
+  stats->prim_tests = c[CNT_PRIMS];
+  stats->shadow_node_visits = c[CNT_SNODES];
+  stats->shadow_prim_tests = c[CNT_SPRIMS];
+  stats->nan_samples = c[CNT_NAN];
+  stats->extend_launches = launches;
+}
+
+// 8M path slots in flight by default (~2.7 GB of wavefront state): enough
+// rays in the late, sparse bounces to keep 256 CUs busy.
+static const uint32_t kDefaultWavefront = 1u << 23;
+
+extern "C" pt_status pt_scene_set_tuning(pt_scene *s, const pt_tuning *t) {
+  if (!s || !t) return fail(PT_ERR_INVALID, "pt_scene_set_tuning: NULL argument");
+  if (t->trav_blocks_per_cu > 64 || t->shade_blocks_per_cu > 256)
+    return fail(PT_ERR_INVALID, "pt_scene_set_tuning: grid sizes out of range");
+  s->tuning = *t;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *stats) {
+  uint32_t spp = 0, sppc = 0;
+  pt_status st = check_cfg("pt_render", s, cfg, spp, sppc);
+  if (st != PT_OK) return st;
+  const uint64_t npix = (uint64_t)cfg->width * cfg->height;
   const uint64_t total = npix * sppc;
-  pt_status st = ensure_stream(s);
+  st = ensure_stream(s);
   if (st != PT_OK) return st;
   if (cfg->sampler == PT_SAMPLER_HALTON_PERM) {
     st = ensure_perm(s, cfg->seed);
     if (st != PT_OK) return st;
   }
-  // 8M path slots in flight by default (~2.7 GB of wavefront state): enough rays
-  // in the late, sparse bounces to keep 256 CUs busy.
-  uint32_t want = cfg->wavefront ? cfg->wavefront : (1u << 23);
+  uint32_t want = cfg->wavefront ? cfg->wavefront : (s->tuning.wavefront ? s->tuning.wavefront : kDefaultWavefront);
   uint64_t chunk_pix = std::max<uint64_t>(1, want / sppc);
   chunk_pix = std::min(chunk_pix, npix);
   uint32_t cap = (uint32_t)(chunk_pix * sppc);
-  st = ensure_wave(s, cap);
+  st = ensure_wave(s->ws, cap);
   if (st != PT_OK) return st;
   if (cfg->estimator == PT_EST_RESERVOIR) {
-    st = ensure_reservoir(s);
+    st = ensure_reservoir(s->ws);
     if (st != PT_OK) return st;
   }
 
@@ -1014,67 +1197,24 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
     HIP_TRY(s->samples.alloc(s->samples_n * sizeof(float)));
     samples = s->samples.as<float>();
   }
-  // per-frame jittered pixel offsets (FrameBuffer::setOffsets, framebuffer.cpp:131-145)
   std::vector<float> off(2 * spp);
-  for (uint32_t i = 0; i < cfg->spp_x; ++i)
-    for (uint32_t j = 0; j < cfg->spp_y; ++j) {
-      uint32_t k = i * cfg->spp_y + j;
-      float a0 = (float)i / (float)cfg->spp_x, a1 = (float)(i + 1) / (float)cfg->spp_x;
-      float b0 = (float)j / (float)cfg->spp_y, b1 = (float)(j + 1) / (float)cfg->spp_y;
-      off[k] = (float)(a0 + (a1 - a0) * (double)pt_rng(0xffffffffu - k, 0u, cfg->seed ^ 0x5bd1e995u));
-      off[spp + k] = (float)(b0 + (b1 - b0) * (double)pt_rng(0xffffffffu - k, 1u, cfg->seed ^ 0x5bd1e995u));
-    }
+  pixel_offsets(cfg, spp, off.data());
   HIP_TRY(s->d_off.alloc(off.size() * sizeof(float)));
   HIP_TRY(hipMemcpyAsync(s->d_off.p, off.data(), off.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
-  HIP_TRY(hipMemsetAsync(s->w_stats.p, 0, CNT_N * sizeof(unsigned long long), s->stream));
+  HIP_TRY(hipMemsetAsync(s->ws.stats.p, 0, CNT_N * sizeof(unsigned long long), s->stream));
 
-  RenderParams rp{};
-  rp.width = cfg->width;
-  rp.height = cfg->height;
-  rp.spp_x = cfg->spp_x;
-  rp.spp_y = cfg->spp_y;
-  rp.spp = spp;
-  rp.spp_begin = cfg->spp_begin;
-  rp.spp_count = sppc;
-  rp.max_depth = cfg->max_depth;
-  rp.sampler = cfg->sampler;
-  rp.estimator = cfg->estimator;
-  rp.rr_depth = cfg->rr_depth;
-  rp.light_samples = cfg->light_samples;
-  grid_factor(std::max(1u, cfg->light_samples), rp.lx, rp.ly);
-  rp.seed = cfg->seed;
-  rp.bg[0] = cfg->bg[0]; rp.bg[1] = cfg->bg[1]; rp.bg[2] = cfg->bg[2];
-  DevCamera cam = make_camera(s->cam);
-  const bool count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
-  const bool timing = (cfg->flags & PT_RENDER_TIMING) != 0;
-  const uint32_t grid_cap = s->n_cu * 32;  // shading: 4 blocks of 256 threads per CU resident, x8
-#ifndef PT_TRAV_BLOCKS_PER_CU
-#define PT_TRAV_BLOCKS_PER_CU 8
-#endif
-  // traversal: one resident round (8 waves per SIMD = 8 blocks per CU), each wave
-  // refilling its lanes from its share of the rays
-  const uint32_t trav_cap = std::max<uint32_t>(PT_PARTS, s->n_cu * PT_TRAV_BLOCKS_PER_CU / PT_PARTS * PT_PARTS);
-  // NEE+MIS and the reservoir estimator trace one vertex more: the hit that
-  // weights (MIS) or drops (reservoir) the last vertex's direct light
-  const uint32_t nverts = (cfg->estimator == PT_EST_NEE_MIS || cfg->estimator == PT_EST_RESERVOIR)
-                              ? cfg->max_depth + 1
-                              : cfg->max_depth;
+  RenderParams rp = make_params(cfg, spp, sppc);
+  const DevCamera cam = make_camera(s->cam);
+  ChunkPlan pl{};
+  pl.nverts = n_vertices(cfg);
+  pl.estimator = cfg->estimator;
+  pl.mat_kinds = s->ds.mat_kinds;
+  pl.count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
+  pl.grids = make_grids(s->n_cu, s->tuning);
+  StageEvents ev;
+  ev.on = (cfg->flags & PT_RENDER_TIMING) != 0;
+  ev.st = s->stream;
 
-  struct Ev { hipEvent_t a, b; int kind; };
-  std::vector<Ev> evs;
-  auto ev_begin = [&](int kind) -> int {
-    if (!timing) return -1;
-    Ev e;
-    (void)hipEventCreate(&e.a);
-    (void)hipEventCreate(&e.b);
-    e.kind = kind;
-    (void)hipEventRecord(e.a, s->stream);
-    evs.push_back(e);
-    return (int)evs.size() - 1;
-  };
-  auto ev_end = [&](int i) {
-    if (i >= 0) (void)hipEventRecord(evs[i].b, s->stream);
-  };
   hipEvent_t tstart, tstop;
   HIP_TRY(hipEventCreate(&tstart));
   HIP_TRY(hipEventCreate(&tstop));
@@ -1084,58 +1224,22 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
     uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - first);
     rp.chunk_first = first;
     rp.chunk_paths = n;
-    int e = ev_begin(0);
-    launch_generate(s->stream, s->ds, cam, rp, s->wave, s->d_off.as<float>(), s->d_off.as<float>() + spp);
-    ev_end(e);
-    int qi = 0;
-    for (uint32_t d = 0; d < nverts; ++d) {
-      e = ev_begin(1);
-      launch_extend(s->stream, s->ds, s->wave, qi, n, trav_cap, count);
-      ev_end(e);
-      ++launches;
-      e = ev_begin(2);
-      launch_shade(s->stream, s->ds, rp, s->wave, qi, d, n, grid_cap);
-      ev_end(e);
-      e = ev_begin(3);
-      if (cfg->estimator != PT_EST_NAIVE && cfg->estimator != PT_EST_ROULETTE)  // those trace no shadow rays
-        launch_shadow(s->stream, s->ds, s->wave, 4u * n, trav_cap, count, cfg->estimator == PT_EST_RESERVOIR);
-      ev_end(e);
-      qi ^= 1;
-    }
-    e = ev_begin(4);
-    launch_film(s->stream, rp, s->wave, s->film.as<float>(), samples, s->wave.stats);
-    ev_end(e);
+    pl.max_paths = n;
+    launch_chunk(s->stream, s->ds, cam, rp, s->ws.wave, s->d_off.as<float>(), s->d_off.as<float>() + spp,
+                 s->film.as<float>(), samples, nullptr, pl, &ev);
+    launches += pl.nverts;
   }
   HIP_TRY(hipEventRecord(tstop, s->stream));
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->film_spp += sppc;
   if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, tstart, tstop);
-    stats->ms_total = ms;
     unsigned long long c[CNT_N];
-    HIP_TRY(hipMemcpy(c, s->w_stats.p, sizeof(c), hipMemcpyDeviceToHost));
-    stats->paths = total;
-    stats->segments = c[CNT_SEGMENTS];
-    stats->shadow_rays = c[CNT_SHADOW_RAYS];
-    stats->node_visits = c[CNT_NODES];
-    stats->prim_tests = c[CNT_PRIMS];
-    stats->shadow_node_visits = c[CNT_SNODES];
-    stats->shadow_prim_tests = c[CNT_SPRIMS];
-    stats->nan_samples = c[CNT_NAN];
-    stats->extend_launches = launches;
-    for (auto &ev : evs) {
-      float t = 0.f;
-      (void)hipEventElapsedTime(&t, ev.a, ev.b);
-      double *dst[5] = {&stats->ms_generate, &stats->ms_extend, &stats->ms_shade, &stats->ms_shadow, &stats->ms_film};
-      *dst[ev.kind] += t;
-    }
-  }
-  for (auto &ev : evs) {
-    (void)hipEventDestroy(ev.a);
-    (void)hipEventDestroy(ev.b);
+    HIP_TRY(hipMemcpy(c, s->ws.stats.p, sizeof(c), hipMemcpyDeviceToHost));
+    fill_stats(stats, c, total, launches, ms);
+    ev.add_to(stats);
   }
   (void)hipEventDestroy(tstart);
   (void)hipEventDestroy(tstop);
@@ -1156,6 +1260,480 @@ extern "C" pt_status pt_film_read_samples(pt_scene *s, float *out, size_t n) {
   if (!s || !out) return fail(PT_ERR_INVALID, "pt_film_read_samples: NULL argument");
   if (!s->samples.p || n != s->samples_n) return fail(PT_ERR_INVALID, "pt_film_read_samples: size mismatch");
   HIP_TRY(hipMemcpy(out, s->samples.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ batches
+// One render of a batch: its scene, configuration and film.
+struct BatchInst {
+  pt_scene *scene = nullptr;
+  pt_render_cfg cfg{};
+  DevBuf film, perm, perm_off;
+  uint32_t perm_seed = 0xffffffffu;
+  uint32_t film_w = 0, film_h = 0;
+  uint64_t film_spp = 0;
+};
+
+struct pt_batch {
+  std::vector<std::unique_ptr<BatchInst>> insts;
+  hipStream_t stream = nullptr;
+  int device = -1;
+  uint32_t n_cu = 256;
+  pt_tuning tuning{};
+  WaveStore ws;
+  // per-render tables and the chunk schedule (device copies of the host vectors)
+  DevBuf d_scenes, d_params, d_cams, d_offs, d_off_first, d_films, d_segs, d_chunks, d_cur, d_slots;
+  // the captured launches of one chunk, and what they were captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned char> graph_key;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_key.clear();
+  }
+};
+
+extern "C" pt_status pt_batch_create(pt_batch **out) {
+  if (!out) return fail(PT_ERR_INVALID, "pt_batch_create: out is NULL");
+  try {
+    *out = new pt_batch();
+  } catch (...) {
+    return fail(PT_ERR_NOMEM, "pt_batch_create: out of host memory");
+  }
+  return PT_OK;
+}
+
+extern "C" void pt_batch_destroy(pt_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->drop_graph();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+static pt_status batch_check(const char *who, pt_scene *scene, const pt_render_cfg *cfg) {
+  uint32_t spp, sppc;
+  pt_status st = check_cfg(who, scene, cfg, spp, sppc);
+  if (st != PT_OK) return st;
+  if (cfg->flags & PT_RENDER_KEEP_SAMPLES)
+    return fail(PT_ERR_UNSUPPORTED, std::string(who) + ": batches do not keep per-sample radiance");
+  return PT_OK;
+}
+
+extern "C" pt_status pt_batch_add(pt_batch *b, pt_scene *scene, const pt_render_cfg *cfg, uint32_t *out_index) {
+  if (!b) return fail(PT_ERR_INVALID, "pt_batch_add: NULL batch");
+  pt_status st = batch_check("pt_batch_add", scene, cfg);
+  if (st != PT_OK) return st;
+  if (b->insts.size() >= (1u << 20)) return fail(PT_ERR_INVALID, "pt_batch_add: too many renders");
+  auto in = std::make_unique<BatchInst>();
+  in->scene = scene;
+  in->cfg = *cfg;
+  b->insts.push_back(std::move(in));
+  if (out_index) *out_index = (uint32_t)(b->insts.size() - 1);
+  return PT_OK;
+}
+
+extern "C" pt_status pt_batch_set_cfg(pt_batch *b, uint32_t index, const pt_render_cfg *cfg) {
+  if (!b) return fail(PT_ERR_INVALID, "pt_batch_set_cfg: NULL batch");
+  if (index >= b->insts.size()) return fail(PT_ERR_INVALID, "pt_batch_set_cfg: index out of range");
+  pt_status st = batch_check("pt_batch_set_cfg", b->insts[index]->scene, cfg);
+  if (st != PT_OK) return st;
+  b->insts[index]->cfg = *cfg;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_batch_set_tuning(pt_batch *b, const pt_tuning *t) {
+  if (!b || !t) return fail(PT_ERR_INVALID, "pt_batch_set_tuning: NULL argument");
+  if (t->trav_blocks_per_cu > 64 || t->shade_blocks_per_cu > 256)
+    return fail(PT_ERR_INVALID, "pt_batch_set_tuning: grid sizes out of range");
+  b->tuning = *t;
+  return PT_OK;
+}
+
+// Cuts the concatenated path space of the renders (each pixel-major, whole
+// pixels) into chunks of at most cap slots, and each chunk into one segment
+// per render it touches.
+static void plan_chunks(const std::vector<uint32_t> &npix, const std::vector<uint32_t> &sppc, uint32_t cap,
+                        std::vector<ChunkSeg> &segs, std::vector<ChunkInfo> &chunks) {
+  segs.clear();
+  chunks.clear();
+  ChunkInfo cur{0u, 0u, 0u, 0u};
+  for (uint32_t i = 0; i < (uint32_t)npix.size(); ++i) {
+    uint32_t done = 0;
+    while (done < npix[i]) {
+      uint32_t room = (cap - cur.paths) / sppc[i];
+      if (room == 0) {  // the chunk cannot take a whole pixel of this render
+        chunks.push_back(cur);
+        cur = ChunkInfo{(uint32_t)segs.size(), 0u, 0u, 0u};
+        continue;
+      }
+      uint32_t m = std::min(room, npix[i] - done);
+      ChunkSeg sg{i, cur.paths, cur.npix, done, m};
+      segs.push_back(sg);
+      cur.n_segs += 1;
+      cur.paths += m * sppc[i];
+      cur.npix += m;
+      done += m;
+    }
+  }
+  if (cur.n_segs) chunks.push_back(cur);
+}
+
+template <class T>
+static hipError_t upload_vec(DevBuf &d, const std::vector<T> &v) {
+  return upload(d, v.data(), std::max<size_t>(1, v.size()) * sizeof(T));
+}
+
+static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *stats, uint32_t spp_div) {
+  const size_t ni = b->insts.size();
+  if (ni == 0) return fail(PT_ERR_INVALID, "pt_batch_render: the batch is empty");
+  const uint32_t flags = bc ? bc->flags : 0u;
+  const uint32_t est = b->insts[0]->cfg.estimator;
+  std::vector<uint32_t> spp(ni), sppc(ni), npix(ni);
+  uint32_t nverts = 0, mat_kinds = 0, max_sppc = 0;
+  uint64_t total = 0;
+  for (size_t i = 0; i < ni; ++i) {
+    BatchInst &in = *b->insts[i];
+    pt_status st = batch_check("pt_batch_render", in.scene, &in.cfg);
+    if (st != PT_OK) return st;
+    if (in.cfg.estimator != est)
+      return fail(PT_ERR_UNSUPPORTED, "pt_batch_render: all renders of a batch must use one estimator");
+    st = ensure_stream(in.scene);
+    if (st != PT_OK) return st;
+    if (b->device < 0) b->device = in.scene->device;
+    if (in.scene->device != b->device)
+      return fail(PT_ERR_UNSUPPORTED, "pt_batch_render: the scenes of a batch must be on one device");
+    spp[i] = in.cfg.spp_x * in.cfg.spp_y;
+    const uint32_t left = spp[i] - in.cfg.spp_begin;
+    sppc[i] = in.cfg.spp_count ? std::min(in.cfg.spp_count, left) : left;
+    if (spp_div > 1) sppc[i] = std::max(1u, sppc[i] / spp_div);
+    npix[i] = in.cfg.width * in.cfg.height;
+    nverts = std::max(nverts, n_vertices(&in.cfg));
+    mat_kinds |= in.scene->ds.mat_kinds;
+    max_sppc = std::max(max_sppc, sppc[i]);
+    total += (uint64_t)npix[i] * sppc[i];
+  }
+  if (total >= (1ull << 40)) return fail(PT_ERR_INVALID, "pt_batch_render: too many path samples");
+  if (!b->stream) {
+    HIP_TRY(hipSetDevice(b->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, b->device));
+    b->n_cu = (uint32_t)std::max(1, prop.multiProcessorCount);
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  }
+  // The scenes' uploads went through their own streams.
+  for (size_t i = 0; i < ni; ++i) HIP_TRY(hipStreamSynchronize(b->insts[i]->scene->stream));
+
+  uint32_t want = (bc && bc->wavefront) ? bc->wavefront : (b->tuning.wavefront ? b->tuning.wavefront : kDefaultWavefront);
+  uint32_t cap = (uint32_t)std::min<uint64_t>(std::max(want, max_sppc), total);
+  std::vector<ChunkSeg> segs;
+  std::vector<ChunkInfo> chunks;
+  plan_chunks(npix, sppc, cap, segs, chunks);
+  uint32_t max_paths = 0, max_pix = 0;
+  for (const ChunkInfo &c : chunks) {
+    max_paths = std::max(max_paths, c.paths);
+    max_pix = std::max(max_pix, c.npix);
+  }
+  pt_status st = ensure_wave(b->ws, max_paths);
+  if (st != PT_OK) return st;
+  if (est == PT_EST_RESERVOIR) {
+    st = ensure_reservoir(b->ws);
+    if (st != PT_OK) return st;
+  }
+
+  // per-render tables
+  std::vector<DevScene> scenes(ni);
+  std::vector<RenderParams> params(ni);
+  std::vector<DevCamera> cams(ni);
+  std::vector<float> offs;
+  std::vector<uint32_t> off_first(ni);
+  std::vector<float *> films(ni);
+  for (size_t i = 0; i < ni; ++i) {
+    BatchInst &in = *b->insts[i];
+    scenes[i] = in.scene->ds;
+    if (in.cfg.sampler == PT_SAMPLER_HALTON_PERM) {  // this render's own digit permutations
+      if (in.perm_seed != in.cfg.seed || !in.perm.p) {
+        st = build_perm(in.cfg.seed, in.perm, in.perm_off);
+        if (st != PT_OK) return st;
+        in.perm_seed = in.cfg.seed;
+      }
+      scenes[i].perm = in.perm.as<uint16_t>();
+      scenes[i].perm_off = in.perm_off.as<uint32_t>();
+      scenes[i].perm_seed = in.cfg.seed;
+    }
+    params[i] = make_params(&in.cfg, spp[i], sppc[i]);
+    cams[i] = make_camera(in.scene->cam);
+    off_first[i] = (uint32_t)offs.size();
+    offs.resize(offs.size() + 2 * spp[i]);
+    pixel_offsets(&in.cfg, spp[i], offs.data() + off_first[i]);
+    const size_t fbytes = (size_t)npix[i] * 3 * sizeof(float);
+    if (in.film_w != in.cfg.width || in.film_h != in.cfg.height || !(in.cfg.flags & PT_RENDER_ACCUMULATE) ||
+        !in.film.p) {
+      HIP_TRY(in.film.alloc(fbytes));
+      HIP_TRY(hipMemsetAsync(in.film.p, 0, fbytes, b->stream));
+      in.film_w = in.cfg.width;
+      in.film_h = in.cfg.height;
+      in.film_spp = 0;
+    }
+    films[i] = in.film.as<float>();
+  }
+  // Table uploads are synchronous: the previous render of this batch has
+  // finished (pt_batch_render synchronises before returning).
+  HIP_TRY(upload_vec(b->d_scenes, scenes));
+  HIP_TRY(upload_vec(b->d_params, params));
+  HIP_TRY(upload_vec(b->d_cams, cams));
+  HIP_TRY(upload_vec(b->d_offs, offs));
+  HIP_TRY(upload_vec(b->d_off_first, off_first));
+  HIP_TRY(upload_vec(b->d_films, films));
+  HIP_TRY(upload_vec(b->d_segs, segs));
+  HIP_TRY(upload_vec(b->d_chunks, chunks));
+  HIP_TRY(b->d_cur.alloc(256));
+  HIP_TRY(b->d_slots.alloc((size_t)3 * b->ws.cap * sizeof(uint32_t)));
+  BatchView bv{};
+  bv.scenes = b->d_scenes.as<DevScene>();
+  bv.params = b->d_params.as<RenderParams>();
+  bv.cams = b->d_cams.as<DevCamera>();
+  bv.offs = b->d_offs.as<float>();
+  bv.off_first = b->d_off_first.as<uint32_t>();
+  bv.films = b->d_films.as<float *>();
+  bv.segs = b->d_segs.as<ChunkSeg>();
+  bv.chunks = b->d_chunks.as<ChunkInfo>();
+  bv.cur = b->d_cur.as<uint32_t>();
+  bv.slot_inst = b->d_slots.as<uint32_t>();
+  bv.slot_pix = bv.slot_inst + b->ws.cap;
+  bv.slot_k = bv.slot_pix + b->ws.cap;
+
+  ChunkPlan pl{};
+  pl.max_paths = max_paths;
+  pl.max_pix = max_pix;
+  pl.nverts = nverts;
+  pl.estimator = est;
+  pl.mat_kinds = mat_kinds;
+  pl.count = (flags & PT_RENDER_COUNTERS) != 0;
+  pl.grids = make_grids(b->n_cu, b->tuning);
+  RenderParams rp0{};  // the kernels read the renders' own parameters
+  rp0.estimator = est;
+  rp0.chunk_paths = max_paths;
+  const DevScene ds0 = scenes[0];
+  const DevCamera cam0{};
+  const bool use_graph = !(flags & PT_BATCH_NO_GRAPH) && !(flags & PT_RENDER_TIMING);
+
+  HIP_TRY(hipMemsetAsync(b->ws.stats.p, 0, CNT_N * sizeof(unsigned long long), b->stream));
+  if (use_graph) {
+    // The graph's kernel arguments are fixed device addresses and sizes:
+    // re-capture when any of them changed.
+    std::vector<unsigned char> key(sizeof(pl) + sizeof(bv) + sizeof(Wave) + sizeof(DevScene));
+    std::memcpy(key.data(), &pl, sizeof(pl));
+    std::memcpy(key.data() + sizeof(pl), &bv, sizeof(bv));
+    std::memcpy(key.data() + sizeof(pl) + sizeof(bv), &b->ws.wave, sizeof(Wave));
+    std::memcpy(key.data() + sizeof(pl) + sizeof(bv) + sizeof(Wave), &ds0, sizeof(DevScene));
+    if (key != b->graph_key || !b->exec) {
+      b->drop_graph();
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      launch_chunk(b->stream, ds0, cam0, rp0, b->ws.wave, nullptr, nullptr, nullptr, nullptr, &bv, pl, nullptr);
+      hipGraph_t g = nullptr;
+      hipError_t e1 = hipGetLastError();
+      hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      if (e1 != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(PT_ERR_DEVICE, std::string("pt_batch_render: graph capture failed: ") +
+                                       hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+      }
+      b->graph = g;
+      HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_key = key;
+    }
+  }
+  StageEvents ev;
+  ev.on = (flags & PT_RENDER_TIMING) != 0;
+  ev.st = b->stream;
+  hipEvent_t tstart, tstop;
+  HIP_TRY(hipEventCreate(&tstart));
+  HIP_TRY(hipEventCreate(&tstop));
+  HIP_TRY(hipEventRecord(tstart, b->stream));
+  launch_set_chunk(b->stream, bv, 0u);
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    if (use_graph)
+      HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    else
+      launch_chunk(b->stream, ds0, cam0, rp0, b->ws.wave, nullptr, nullptr, nullptr, nullptr, &bv, pl, &ev);
+  }
+  HIP_TRY(hipEventRecord(tstop, b->stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < ni; ++i) b->insts[i]->film_spp += sppc[i];
+  if (stats) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, tstart, tstop);
+    unsigned long long c[CNT_N];
+    HIP_TRY(hipMemcpy(c, b->ws.stats.p, sizeof(c), hipMemcpyDeviceToHost));
+    fill_stats(stats, c, total, (uint64_t)chunks.size() * nverts, ms);
+    ev.add_to(stats);
+  }
+  (void)hipEventDestroy(tstart);
+  (void)hipEventDestroy(tstop);
+  return PT_OK;
+}
+
+extern "C" pt_status pt_batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *stats) {
+  if (!b) return fail(PT_ERR_INVALID, "pt_batch_render: NULL batch");
+  return batch_render(b, bc, stats, 1u);
+}
+
+extern "C" pt_status pt_batch_film_read(pt_batch *b, uint32_t index, float *out) {
+  if (!b || !out) return fail(PT_ERR_INVALID, "pt_batch_film_read: NULL argument");
+  if (index >= b->insts.size()) return fail(PT_ERR_INVALID, "pt_batch_film_read: index out of range");
+  BatchInst &in = *b->insts[index];
+  if (!in.film.p || in.film_spp == 0) return fail(PT_ERR_INVALID, "pt_batch_film_read: nothing rendered");
+  size_t n = (size_t)in.film_w * in.film_h * 3;
+  HIP_TRY(hipMemcpy(out, in.film.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  float inv = 1.f / (float)in.film_spp;
+  for (size_t i = 0; i < n; ++i) out[i] *= inv;
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Coordinate search: each tuned parameter in turn, the others held at their
+// best so far; a candidate's time is the fastest of `repeats` renders of a
+// 1/spp_div sample of the work.
+static const uint32_t kTuneWavefront[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24};
+static const uint32_t kTuneTrav[] = {4u, 6u, 8u};
+static const uint32_t kTuneShade[] = {8u, 16u, 32u, 64u};
+static const uint32_t kTuneLeaf[] = {1u, 2u, 4u, 8u};
+
+template <class F>
+static pt_status tune_param(uint32_t *field, const uint32_t *cands, size_t nc, uint32_t repeats, F &&time_one,
+                            double &best_ms) {
+  uint32_t best = *field;
+  for (size_t i = 0; i < nc; ++i) {
+    *field = cands[i];
+    double ms = 1e300;
+    for (uint32_t r = 0; r < repeats; ++r) {
+      double t = 0;
+      pt_status st = time_one(t);
+      if (st != PT_OK) return st;
+      ms = std::min(ms, t);
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cands[i];
+    }
+  }
+  *field = best;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_autotune(pt_scene *s, const pt_render_cfg *cfg, const pt_autotune_cfg *tc, pt_tuning *out) {
+  uint32_t spp = 0, sppc = 0;
+  pt_status st = check_cfg("pt_autotune", s, cfg, spp, sppc);
+  if (st != PT_OK) return st;
+  const uint32_t flags = (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS | PT_TUNE_LEAF);
+  const uint32_t div = (tc && tc->spp_div) ? tc->spp_div : 8u;
+  const uint32_t repeats = (tc && tc->repeats) ? tc->repeats : 1u;
+  pt_render_cfg c = *cfg;
+  c.spp_count = std::max(1u, sppc / div);
+  c.wavefront = 0;
+  c.flags &= ~(uint32_t)(PT_RENDER_ACCUMULATE | PT_RENDER_KEEP_SAMPLES | PT_RENDER_TIMING | PT_RENDER_COUNTERS);
+  pt_tuning &t = s->tuning;
+  if (!t.wavefront) t.wavefront = kDefaultWavefront;
+  if (!t.trav_blocks_per_cu) t.trav_blocks_per_cu = 8u;
+  if (!t.shade_blocks_per_cu) t.shade_blocks_per_cu = 32u;
+  auto time_one = [&](double &ms) -> pt_status {
+    pt_stats sst;
+    pt_status r = pt_render(s, &c, &sst);
+    ms = sst.ms_total;
+    return r;
+  };
+  double best = 1e300;
+  {  // warm-up and the time of the starting point
+    double ms = 0;
+    st = time_one(ms);
+    if (st != PT_OK) return st;
+    st = time_one(ms);
+    if (st != PT_OK) return st;
+    best = ms;
+  }
+  if (flags & PT_TUNE_LEAF) {
+    pt_build_cfg bcfg = s->build_cfg;
+    uint32_t leaf = t.leaf_size ? t.leaf_size : 4u, best_leaf = leaf;
+    for (uint32_t cand : kTuneLeaf) {
+      if (cand == leaf) continue;
+      bcfg.leaf_size = cand;
+      st = pt_scene_commit(s, &bcfg);
+      if (st != PT_OK) return st;
+      double ms = 1e300;
+      for (uint32_t r = 0; r < repeats; ++r) {
+        double x = 0;
+        st = time_one(x);
+        if (st != PT_OK) return st;
+        ms = std::min(ms, x);
+      }
+      if (ms < best) {
+        best = ms;
+        best_leaf = cand;
+      }
+    }
+    bcfg.leaf_size = best_leaf;
+    st = pt_scene_commit(s, &bcfg);
+    if (st != PT_OK) return st;
+  }
+  if (flags & PT_TUNE_WAVEFRONT) {
+    st = tune_param(&t.wavefront, kTuneWavefront, sizeof(kTuneWavefront) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+  }
+  if (flags & PT_TUNE_GRIDS) {
+    st = tune_param(&t.trav_blocks_per_cu, kTuneTrav, sizeof(kTuneTrav) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+    st = tune_param(&t.shade_blocks_per_cu, kTuneShade, sizeof(kTuneShade) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+  }
+  // the films of the tuning renders are not the caller's
+  s->film_spp = 0;
+  if (out) *out = t;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_batch_autotune(pt_batch *b, const pt_autotune_cfg *tc, pt_tuning *out) {
+  if (!b) return fail(PT_ERR_INVALID, "pt_batch_autotune: NULL batch");
+  const uint32_t flags = (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS);
+  const uint32_t div = (tc && tc->spp_div) ? tc->spp_div : 8u;
+  const uint32_t repeats = (tc && tc->repeats) ? tc->repeats : 1u;
+  pt_tuning &t = b->tuning;
+  if (!t.wavefront) t.wavefront = kDefaultWavefront;
+  if (!t.trav_blocks_per_cu) t.trav_blocks_per_cu = 8u;
+  if (!t.shade_blocks_per_cu) t.shade_blocks_per_cu = 32u;
+  pt_batch_cfg bc{};
+  auto time_one = [&](double &ms) -> pt_status {
+    pt_stats sst;
+    pt_status r = batch_render(b, &bc, &sst, div);
+    ms = sst.ms_total;
+    return r;
+  };
+  double best = 1e300;
+  {
+    double ms = 0;
+    pt_status st = time_one(ms);
+    if (st != PT_OK) return st;
+    st = time_one(ms);
+    if (st != PT_OK) return st;
+    best = ms;
+  }
+  pt_status st = PT_OK;
+  if (flags & PT_TUNE_WAVEFRONT) {
+    st = tune_param(&t.wavefront, kTuneWavefront, sizeof(kTuneWavefront) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+  }
+  if (flags & PT_TUNE_GRIDS) {
+    st = tune_param(&t.trav_blocks_per_cu, kTuneTrav, sizeof(kTuneTrav) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+    st = tune_param(&t.shade_blocks_per_cu, kTuneShade, sizeof(kTuneShade) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+  }
+  for (auto &in : b->insts) in->film_spp = 0;
+  if (out) *out = t;
   return PT_OK;
 }
 

@@ -28,9 +28,31 @@ __device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + th
 // an even mix of the chunk's pixels while each wave starts coherent.  Thread
 // t writes entry t % m of partition t / m (m = R * ceil(n / (R P))), so the
 // writes are unit-stride.  Also zeroes the radiance and sets the counters.
-__global__ void __launch_bounds__(256) k_generate(DevScene s, DevCamera cam, RenderParams rp, Wave w,
-                                                  const float *offx, const float *offy) {
-  const uint32_t t = gtid(), n = rp.chunk_paths;
+// Batched: the chunk comes from the batch tables; the slot's segment gives
+// its instance, pixel and sample.
+PT_D uint32_t find_seg(const BatchView &b, const ChunkInfo &ci, uint32_t slot) {
+  uint32_t lo = ci.seg_begin, hi = ci.seg_begin + ci.n_segs - 1u;
+  while (lo < hi) {  // last segment with path_first <= slot
+    uint32_t mid = (lo + hi + 1u) / 2u;
+    if (b.segs[mid].path_first <= slot) lo = mid;
+    else hi = mid - 1u;
+  }
+  return lo;
+}
+PT_D uint32_t find_seg_pix(const BatchView &b, const ChunkInfo &ci, uint32_t pl) {
+  uint32_t lo = ci.seg_begin, hi = ci.seg_begin + ci.n_segs - 1u;
+  while (lo < hi) {  // last segment with pix_offset <= pl
+    uint32_t mid = (lo + hi + 1u) / 2u;
+    if (b.segs[mid].pix_offset <= pl) lo = mid;
+    else hi = mid - 1u;
+  }
+  return lo;
+}
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_generate(DevScene s0, DevCamera cam0, RenderParams rp0, Wave w,
+                                                  const float *offx0, const float *offy0, BatchView b) {
+  const uint32_t t = gtid();
+  const uint32_t n = BATCH ? b.chunks[*b.cur].paths : rp0.chunk_paths;
   const uint32_t per = (n + PT_PART_RUN * PT_PARTS - 1) / (PT_PART_RUN * PT_PARTS) * PT_PART_RUN;
   if (t < n) {
     w.lx[t] = 0.f; w.ly[t] = 0.f; w.lz[t] = 0.f;
@@ -46,9 +68,26 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s, DevCamera cam, Ren
   const uint32_t slot = ((l / PT_PART_RUN) * PT_PARTS + part) * PT_PART_RUN + l % PT_PART_RUN;
   if (slot >= n) return;
   const uint32_t e = part * w.part_cap + l;
-  uint64_t q = rp.chunk_first + slot;
-  uint32_t pixel = (uint32_t)(q / rp.spp_count);
-  uint32_t k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
+  uint32_t pixel, k, inst = 0u;
+  if (BATCH) {
+    const ChunkInfo ci = b.chunks[*b.cur];
+    const ChunkSeg sg = b.segs[find_seg(b, ci, slot)];
+    inst = sg.inst;
+    const uint32_t local = slot - sg.path_first, sppc = b.params[inst].spp_count;
+    pixel = sg.pix_first + local / sppc;
+    k = b.params[inst].spp_begin + local % sppc;
+    b.slot_inst[slot] = inst;
+    b.slot_pix[slot] = pixel;
+    b.slot_k[slot] = k;
+  } else {
+    uint64_t q = rp0.chunk_first + slot;
+    pixel = (uint32_t)(q / rp0.spp_count);
+    k = rp0.spp_begin + (uint32_t)(q % rp0.spp_count);
+  }
+  const RenderParams &rp = BATCH ? b.params[inst] : rp0;
+  const DevCamera &cam = BATCH ? b.cams[inst] : cam0;
+  const float *offx = BATCH ? b.offs + b.off_first[inst] : offx0;
+  const float *offy = BATCH ? offx + rp.spp : offy0;
   uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
   uint32_t g = 1u + k + rp.spp * (pj + rp.height * pi);
   float u = div_rn((float)pi + offx[k], (float)rp.width);
@@ -92,8 +131,8 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s, DevCamera cam, Ren
 #ifndef PT_REFILL_SHADOW
 #define PT_REFILL_SHADOW 8
 #endif
-template <bool COUNT>
-__global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_extend(DevScene s, Wave w, int qi) {
+template <bool COUNT, bool BATCH>
+__global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_extend(DevScene s, Wave w, int qi, BatchView b) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_EXT, true)];
   const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
@@ -109,10 +148,11 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   const uint32_t wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
   const uint32_t per = (n + nw - 1) / nw;
   const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
-  trace_range<false, COUNT, PT_K_EXT, PT_REFILL_EXT>(
+  trace_range<false, COUNT, PT_K_EXT, PT_REFILL_EXT, BATCH>(
       s, lds, begin, end,
-      [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax) {
+      [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
+        if (BATCH) sc = &b.scenes[b.slot_inst[ps.slot[i]]];
         o = mk3(ps.ox[i], ps.oy[i], ps.oz[i]);
         d = mk3(ps.dx[i], ps.dy[i], ps.dz[i]);
         tmin = 1e-4f;
@@ -135,8 +175,8 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
 // ------------------------------------------------------------------ shadow
 // VIS: the entries are reservoir candidates (slot field = 4 slot + candidate),
 // whose visibility is recorded instead of adding a contribution.
-template <bool COUNT, bool VIS>
-__global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_shadow(DevScene s, Wave w) {
+template <bool COUNT, bool VIS, bool BATCH>
+__global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_shadow(DevScene s, Wave w, BatchView b) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_SHADOW, false)];
   // blocks b, b + P, b + 2P, ... share partition p = b % P (the grid is a multiple of P)
   const uint32_t p = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
@@ -147,10 +187,11 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   const uint32_t wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
   const uint32_t per = (n + nw - 1) / nw;
   const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
-  trace_range<true, COUNT, PT_K_SHADOW, PT_REFILL_SHADOW>(
+  trace_range<true, COUNT, PT_K_SHADOW, PT_REFILL_SHADOW, BATCH>(
       s, lds, begin, end,
-      [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax) {
+      [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
+        if (BATCH) sc = &b.scenes[b.slot_inst[VIS ? w.sslot[i] / 4u : w.sslot[i]]];
         o = mk3(w.sox[i], w.soy[i], w.soz[i]);
         d = mk3(w.sdx[i], w.sdy[i], w.sdz[i]);
         tmin = 0.001f;
@@ -209,15 +250,13 @@ PT_D float power_w(float a, float b) { return (a * a) / (a * a + b * b); }
 #ifndef PT_SHADE_WAVES
 #define PT_SHADE_WAVES 4
 #endif
-template <int EST, uint32_t MS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHADE_WAVES))) k_shade(DevScene s, RenderParams rp, Wave w, int qi, uint32_t d) {
+template <int EST, uint32_t MS, bool BATCH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHADE_WAVES))) k_shade(DevScene s0, RenderParams rp0, Wave w, int qi, uint32_t d, BatchView b) {
   const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
   const PathState &in = w.st[qi];
   const PathState &out = w.st[qi ^ 1];
   uint32_t *cout = w.live(qi ^ 1, part);
-  const uint32_t D = rp.max_depth;
-  const int N = s.n_lights;
   for (uint32_t i0 = (blockIdx.x / PT_PARTS) * blockDim.x; i0 < n; i0 += nb * blockDim.x) {
     uint32_t i = i0 + threadIdx.x;
     bool active = i < n;
@@ -229,9 +268,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
     uint32_t fl = 0u;
     if (active) {
       slot = in.slot[i];
-      uint64_t q = rp.chunk_first + slot;
-      uint32_t pixel = (uint32_t)(q / rp.spp_count);
-      uint32_t k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
+      // the path's scene and render parameters (batched: its instance's)
+      const uint32_t inst = BATCH ? b.slot_inst[slot] : 0u;
+      const DevScene &s = BATCH ? b.scenes[inst] : s0;
+      const RenderParams &rp = BATCH ? b.params[inst] : rp0;
+      const uint32_t D = rp.max_depth;
+      const int N = s.n_lights;
+      uint32_t pixel, k;
+      if (BATCH) {
+        pixel = b.slot_pix[slot];
+        k = b.slot_k[slot];
+      } else {
+        uint64_t q = rp.chunk_first + slot;
+        pixel = (uint32_t)(q / rp.spp_count);
+        k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
+      }
       uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
       uint32_t g = 1u + k + rp.spp * (pj + rp.height * pi);
       f3 o = mk3(in.ox[i], in.oy[i], in.oz[i]);
@@ -588,31 +639,58 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
 }
 
 // ------------------------------------------------------------------ film
-__global__ void __launch_bounds__(256) k_film(RenderParams rp, Wave w, float *film, float *samples,
-                                              unsigned long long *stats) {
+template <bool BATCH>
+__global__ void __launch_bounds__(256) k_film(RenderParams rp0, Wave w, float *film0, float *samples,
+                                              unsigned long long *stats, BatchView b) {
   uint32_t pl = gtid();
-  uint32_t npix = rp.chunk_paths / rp.spp_count;
-  if (pl >= npix) return;
-  uint64_t pixel = rp.chunk_first / rp.spp_count + pl;
+  uint32_t npix, sppc, slot0;
+  uint64_t pixel;
+  float *film;
+  if (BATCH) {
+    const ChunkInfo ci = b.chunks[*b.cur];
+    npix = ci.npix;
+    if (pl >= npix) return;
+    const ChunkSeg sg = b.segs[find_seg_pix(b, ci, pl)];
+    sppc = b.params[sg.inst].spp_count;
+    pixel = sg.pix_first + (pl - sg.pix_offset);
+    slot0 = sg.path_first + (pl - sg.pix_offset) * sppc;
+    film = b.films[sg.inst];
+    samples = nullptr;
+  } else {
+    sppc = rp0.spp_count;
+    npix = rp0.chunk_paths / sppc;
+    if (pl >= npix) return;
+    pixel = rp0.chunk_first / sppc + pl;
+    slot0 = pl * sppc;
+    film = film0;
+  }
   float sr = 0.f, sg = 0.f, sb = 0.f;
   uint32_t nan = 0;
-  for (uint32_t k = 0; k < rp.spp_count; ++k) {
-    uint32_t slot = pl * rp.spp_count + k;
-    float r = w.lx[slot], g = w.ly[slot], b = w.lz[slot];
-    if (!(isfinite(r) && isfinite(g) && isfinite(b))) {
-      r = g = b = 0.f;
+  for (uint32_t k = 0; k < sppc; ++k) {
+    uint32_t slot = slot0 + k;
+    float r = w.lx[slot], g = w.ly[slot], bb = w.lz[slot];
+    if (!(isfinite(r) && isfinite(g) && isfinite(bb))) {
+      r = g = bb = 0.f;
       ++nan;
     }
     if (samples) {
-      size_t o = 3 * ((size_t)pixel * rp.spp_count + k);
-      samples[o] = r; samples[o + 1] = g; samples[o + 2] = b;
+      size_t o = 3 * ((size_t)pixel * sppc + k);
+      samples[o] = r; samples[o + 1] = g; samples[o + 2] = bb;
     }
-    sr += r; sg += g; sb += b;
+    sr += r; sg += g; sb += bb;
   }
   film[3 * pixel] += sr;
   film[3 * pixel + 1] += sg;
   film[3 * pixel + 2] += sb;
   if (nan) atomicAdd(&stats[CNT_NAN], (unsigned long long)nan);
+}
+
+// Advances the batch to its next chunk (the last launch of a chunk).
+__global__ void k_next_chunk(BatchView b) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *b.cur += 1u;
+}
+__global__ void k_set_chunk(BatchView b, uint32_t c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *b.cur = c;
 }
 
 // ------------------------------------------------------------------ parity kernels
@@ -717,59 +795,98 @@ static inline uint32_t part_grid(size_t max_n, uint32_t grid_cap) {
   return std::max<uint32_t>(PT_PARTS, g / PT_PARTS * PT_PARTS);
 }
 
+// The render launchers take a BatchView: null renders the one scene of the
+// arguments, non-null renders the batch's current chunk (max_n / the chunk
+// fields of rp then only size the grids).
 void launch_generate(hipStream_t st, const DevScene &s, const DevCamera &c, const RenderParams &rp, const Wave &w,
-                     const float *offx, const float *offy) {
+                     const float *offx, const float *offy, const BatchView *b) {
   const size_t per = (rp.chunk_paths + PT_PART_RUN * PT_PARTS - 1) / (PT_PART_RUN * PT_PARTS) * PT_PART_RUN;
   const size_t threads = std::max<size_t>(per * PT_PARTS, rp.chunk_paths);
-  hipLaunchKernelGGL(k_generate, dim3(blocks_for(threads)), dim3(256), 0, st, s, c, rp, w, offx, offy);
+  if (b)
+    hipLaunchKernelGGL(k_generate<true>, dim3(blocks_for(threads)), dim3(256), 0, st, s, c, rp, w, offx, offy, *b);
+  else
+    hipLaunchKernelGGL(k_generate<false>, dim3(blocks_for(threads)), dim3(256), 0, st, s, c, rp, w, offx, offy,
+                       BatchView{});
 }
 void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uint32_t max_n, uint32_t grid_cap,
-                   bool count) {
+                   bool count, const BatchView *b) {
   uint32_t g = part_grid(max_n, grid_cap);
-  if (count)
-    hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi);
-  else
-    hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi);
-}
-template <int EST>
-static void launch_shade_est(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi,
-                             uint32_t d, uint32_t g) {
-  // the basic material set when the scene uses nothing else (fewer registers)
-  if ((s.mat_kinds & ~kMatBasic) == 0u)
-    hipLaunchKernelGGL((k_shade<EST, kMatBasic>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
-  else
-    hipLaunchKernelGGL((k_shade<EST, kMatAll>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d);
-}
-void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
-                  uint32_t max_n, uint32_t grid_cap) {
-  uint32_t g = part_grid(max_n, grid_cap);
-  switch (rp.estimator) {
-    case PT_EST_NEE_MIS: launch_shade_est<PT_EST_NEE_MIS>(st, s, rp, w, qi, d, g); break;
-    case PT_EST_NAIVE: launch_shade_est<PT_EST_NAIVE>(st, s, rp, w, qi, d, g); break;
-    case PT_EST_ROULETTE: launch_shade_est<PT_EST_ROULETTE>(st, s, rp, w, qi, d, g); break;
-    case PT_EST_RESERVOIR: launch_shade_est<PT_EST_RESERVOIR>(st, s, rp, w, qi, d, g); break;
-    default: launch_shade_est<PT_EST_REFERENCE_NEE>(st, s, rp, w, qi, d, g); break;
-  }
-}
-void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count,
-                   bool visibility) {
-  uint32_t g = part_grid(max_n, grid_cap);
-  if (visibility) {
+  const BatchView bv = b ? *b : BatchView{};
+  if (b) {
     if (count)
-      hipLaunchKernelGGL((k_shadow<true, true>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+      hipLaunchKernelGGL((k_extend<true, true>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi, bv);
     else
-      hipLaunchKernelGGL((k_shadow<false, true>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+      hipLaunchKernelGGL((k_extend<false, true>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi, bv);
   } else {
     if (count)
-      hipLaunchKernelGGL((k_shadow<true, false>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+      hipLaunchKernelGGL((k_extend<true, false>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi, bv);
     else
-      hipLaunchKernelGGL((k_shadow<false, false>), dim3(g), dim3(PT_BLOCK), 0, st, s, w);
+      hipLaunchKernelGGL((k_extend<false, false>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, qi, bv);
   }
 }
+template <int EST, bool BATCH>
+static void launch_shade_est(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi,
+                             uint32_t d, uint32_t g, const BatchView &b, uint32_t mat_kinds) {
+  // the basic material set when the scene uses nothing else (fewer registers)
+  if ((mat_kinds & ~kMatBasic) == 0u)
+    hipLaunchKernelGGL((k_shade<EST, kMatBasic, BATCH>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d, b);
+  else
+    hipLaunchKernelGGL((k_shade<EST, kMatAll, BATCH>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d, b);
+}
+template <bool BATCH>
+static void launch_shade_b(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi,
+                           uint32_t d, uint32_t g, const BatchView &b, uint32_t mat_kinds) {
+  switch (rp.estimator) {
+    case PT_EST_NEE_MIS: launch_shade_est<PT_EST_NEE_MIS, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
+    case PT_EST_NAIVE: launch_shade_est<PT_EST_NAIVE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
+    case PT_EST_ROULETTE: launch_shade_est<PT_EST_ROULETTE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
+    case PT_EST_RESERVOIR: launch_shade_est<PT_EST_RESERVOIR, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
+    default: launch_shade_est<PT_EST_REFERENCE_NEE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
+  }
+}
+void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
+                  uint32_t max_n, uint32_t grid_cap, const BatchView *b, uint32_t mat_kinds) {
+  uint32_t g = part_grid(max_n, grid_cap);
+  if (b)
+    launch_shade_b<true>(st, s, rp, w, qi, d, g, *b, mat_kinds);
+  else
+    launch_shade_b<false>(st, s, rp, w, qi, d, g, BatchView{}, mat_kinds);
+}
+void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count,
+                   bool visibility, const BatchView *b) {
+  uint32_t g = part_grid(max_n, grid_cap);
+  const BatchView bv = b ? *b : BatchView{};
+#define PT_SHADOW_LAUNCH(C, V, B) hipLaunchKernelGGL((k_shadow<C, V, B>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, bv)
+  if (b) {
+    if (visibility) {
+      if (count) PT_SHADOW_LAUNCH(true, true, true); else PT_SHADOW_LAUNCH(false, true, true);
+    } else {
+      if (count) PT_SHADOW_LAUNCH(true, false, true); else PT_SHADOW_LAUNCH(false, false, true);
+    }
+  } else {
+    if (visibility) {
+      if (count) PT_SHADOW_LAUNCH(true, true, false); else PT_SHADOW_LAUNCH(false, true, false);
+    } else {
+      if (count) PT_SHADOW_LAUNCH(true, false, false); else PT_SHADOW_LAUNCH(false, false, false);
+    }
+  }
+#undef PT_SHADOW_LAUNCH
+}
 void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
-                 unsigned long long *stats) {
-  uint32_t npix = rp.chunk_paths / rp.spp_count;
-  hipLaunchKernelGGL(k_film, dim3(blocks_for(npix)), dim3(256), 0, st, rp, w, film, samples, stats);
+                 unsigned long long *stats, const BatchView *b, uint32_t max_pix) {
+  if (b) {
+    hipLaunchKernelGGL(k_film<true>, dim3(blocks_for(max_pix)), dim3(256), 0, st, rp, w, film, samples, stats, *b);
+  } else {
+    uint32_t npix = rp.chunk_paths / rp.spp_count;
+    hipLaunchKernelGGL(k_film<false>, dim3(blocks_for(npix)), dim3(256), 0, st, rp, w, film, samples, stats,
+                       BatchView{});
+  }
+}
+void launch_next_chunk(hipStream_t st, const BatchView &b) {
+  hipLaunchKernelGGL(k_next_chunk, dim3(1), dim3(64), 0, st, b);
+}
+void launch_set_chunk(hipStream_t st, const BatchView &b, uint32_t c) {
+  hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(64), 0, st, b, c);
 }
 void launch_sample(hipStream_t st, const DevScene &s, uint32_t sampler, uint32_t seed, uint32_t sx, uint32_t sy,
                    const uint32_t *dim, const uint64_t *index, const uint32_t *pixel, size_t n, float *out) {

@@ -6,15 +6,17 @@
 namespace pt {
 
 void launch_generate(hipStream_t st, const DevScene &s, const DevCamera &c, const RenderParams &rp, const Wave &w,
-                     const float *offx, const float *offy);
+                     const float *offx, const float *offy, const BatchView *b);
 void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uint32_t max_n, uint32_t grid_cap,
-                   bool count);
+                   bool count, const BatchView *b);
 void launch_shade(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi, uint32_t d,
-                  uint32_t max_n, uint32_t grid_cap);
+                  uint32_t max_n, uint32_t grid_cap, const BatchView *b, uint32_t mat_kinds);
 void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t max_n, uint32_t grid_cap, bool count,
-                   bool visibility);
+                   bool visibility, const BatchView *b);
 void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
-                 unsigned long long *stats);
+                 unsigned long long *stats, const BatchView *b, uint32_t max_pix);
+void launch_next_chunk(hipStream_t st, const BatchView &b);
+void launch_set_chunk(hipStream_t st, const BatchView &b, uint32_t c);
 void launch_sample(hipStream_t st, const DevScene &s, uint32_t sampler, uint32_t seed, uint32_t sx, uint32_t sy,
                    const uint32_t *dim, const uint64_t *index, const uint32_t *pixel, size_t n, float *out);
 void launch_intersect(hipStream_t st, const DevScene &s, const float *rays, size_t n, int any, int32_t *prim,

@@ -87,6 +87,38 @@ struct RenderParams {
   uint32_t chunk_paths;
 };
 
+// Batched rendering: many scenes rendered in one wavefront.  The global path
+// space is the concatenation of the instances' pixel-major path ranges; it is
+// cut into chunks of whole pixels, and a chunk into segments, one per
+// instance it touches.  All of it lives in device memory, so one captured
+// sequence of launches serves every chunk (the chunk index is a device
+// counter that k_next_chunk advances).
+struct ChunkSeg {
+  uint32_t inst;        // instance
+  uint32_t path_first;  // first chunk slot of the segment
+  uint32_t pix_offset;  // first chunk pixel of the segment
+  uint32_t pix_first;   // the instance's pixel index of that pixel
+  uint32_t npix;
+};
+struct ChunkInfo {
+  uint32_t seg_begin, n_segs;  // into BatchView::segs
+  uint32_t paths, npix;        // slots and pixels of the chunk
+};
+struct BatchView {
+  const DevScene *scenes;      // per instance
+  const RenderParams *params;  // per instance (the chunk fields are unused)
+  const DevCamera *cams;
+  const float *offs;           // per instance 2 * spp pixel offsets, at off_first[inst]
+  const uint32_t *off_first;
+  float *const *films;         // per instance width * height * 3
+  const ChunkSeg *segs;
+  const ChunkInfo *chunks;
+  uint32_t *cur;               // index of the chunk being rendered
+  uint32_t *slot_inst;         // per slot: instance
+  uint32_t *slot_pix;          // per slot: the instance's pixel index
+  uint32_t *slot_k;            // per slot: sample index within the pixel
+};
+
 // One compacted buffer of live paths: entry i is the i-th live path of the
 // current vertex, so every kernel reads and writes it with unit stride.
 struct PathState {

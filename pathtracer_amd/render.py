@@ -92,6 +92,17 @@ class Scene:
         L.check(L.lib.pt_film_read_samples(self.h, L.ptr(out), n), "pt_film_read_samples")
         return out.reshape(cfg.height * cfg.width, sppc, 3)
 
+    def autotune(self, cfg: L.RenderCfg, flags: int = 0, spp_div: int = 8, repeats: int = 1) -> dict:
+        """pt_autotune: picks wavefront / grid sizes (and the BVH leaf size) for cfg's frame."""
+        tc = L.AutotuneCfg(flags, spp_div, repeats)
+        t = L.Tuning()
+        L.check(L.lib.pt_autotune(self.h, ctypes.byref(cfg), ctypes.byref(tc), ctypes.byref(t)), "pt_autotune")
+        return t.as_dict()
+
+    def set_tuning(self, wavefront=0, trav_blocks_per_cu=0, shade_blocks_per_cu=0):
+        t = L.Tuning(wavefront, trav_blocks_per_cu, shade_blocks_per_cu, 0)
+        L.check(L.lib.pt_scene_set_tuning(self.h, ctypes.byref(t)), "pt_scene_set_tuning")
+
     def intersect(self, rays: np.ndarray, any_hit: bool = False):
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
@@ -146,3 +157,61 @@ def bsdf_sample(kind: int, params, wo, nrm, uc, u2, aux=None, inside=None):
     L.check(L.lib.pt_bsdf_sample(ctypes.byref(m), L.ptr(wo), L.ptr(nrm), L.ptr(ins), L.ptr(uc), L.ptr(u2), L.ptr(ax),
                                  n, L.ptr(wi), L.ptr(f), L.ptr(pdf), L.ptr(flags), L.ptr(ok)), "pt_bsdf_sample")
     return wi, f, pdf, flags, ok
+
+
+class Batch:
+    """Many renders in one wavefront (pt_batch_*): each entry is a committed
+    Scene and its RenderCfg; every render gets its own film."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        L.check(L.lib.pt_batch_create(ctypes.byref(h)), "pt_batch_create")
+        self.h = h
+        self.cfgs = []
+        self.scenes = []  # the scenes must outlive the batch
+
+    def add(self, scene: Scene, cfg: L.RenderCfg) -> int:
+        idx = ctypes.c_uint32()
+        L.check(L.lib.pt_batch_add(self.h, scene.h, ctypes.byref(cfg), ctypes.byref(idx)), "pt_batch_add")
+        self.cfgs.append(cfg)
+        self.scenes.append(scene)
+        return idx.value
+
+    def set_cfg(self, index: int, cfg: L.RenderCfg):
+        L.check(L.lib.pt_batch_set_cfg(self.h, index, ctypes.byref(cfg)), "pt_batch_set_cfg")
+        self.cfgs[index] = cfg
+
+    def render(self, wavefront: int = 0, flags: int = 0) -> dict:
+        bc = L.BatchCfg(wavefront, flags)
+        st = L.Stats()
+        L.check(L.lib.pt_batch_render(self.h, ctypes.byref(bc), ctypes.byref(st)), "pt_batch_render")
+        return st.as_dict()
+
+    def film(self, index: int) -> np.ndarray:
+        if not 0 <= index < len(self.cfgs):  # the library reports the error
+            L.check(L.lib.pt_batch_film_read(self.h, index, L.ptr(np.empty(3, np.float32))), "pt_batch_film_read")
+        c = self.cfgs[index]
+        img = np.empty((c.height, c.width, 3), np.float32)
+        L.check(L.lib.pt_batch_film_read(self.h, index, L.ptr(img)), "pt_batch_film_read")
+        return img
+
+    def autotune(self, flags: int = 0, spp_div: int = 8, repeats: int = 1) -> dict:
+        tc = L.AutotuneCfg(flags, spp_div, repeats)
+        t = L.Tuning()
+        L.check(L.lib.pt_batch_autotune(self.h, ctypes.byref(tc), ctypes.byref(t)), "pt_batch_autotune")
+        return t.as_dict()
+
+    def set_tuning(self, wavefront=0, trav_blocks_per_cu=0, shade_blocks_per_cu=0):
+        t = L.Tuning(wavefront, trav_blocks_per_cu, shade_blocks_per_cu, 0)
+        L.check(L.lib.pt_batch_set_tuning(self.h, ctypes.byref(t)), "pt_batch_set_tuning")
+
+    def close(self):
+        if self.h:
+            L.lib.pt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -309,3 +309,92 @@ def furnace(albedo=0.5, env_value=1.0, max_depth=64, width=8, height=8, spp_x=8,
                     sampler=L.SAMPLER_HALTON_OWEN, estimator=L.EST_NEE_MIS, rr_depth=0, light_samples=4, seed=3,
                     bg=(0, 0, 0))
     return s
+
+
+def geodesic_sphere(freq: int):
+    """Unit geodesic sphere: every icosahedron face cut into freq^2 triangles
+    (20*freq^2 in all), vertices projected to the sphere and shared."""
+    v0, f0 = icosphere(0)
+    ii, jj = np.meshgrid(np.arange(freq + 1), np.arange(freq + 1), indexing="ij")
+    keep = ii + jj <= freq
+    ii, jj = ii[keep], jj[keep]
+    lid = -np.ones((freq + 1, freq + 1), np.int64)
+    lid[ii, jj] = np.arange(ii.size)
+    # local triangles of one face: up (i,j)(i+1,j)(i,j+1), down (i+1,j)(i+1,j+1)(i,j+1)
+    tri = []
+    for i in range(freq):
+        for j in range(freq - i):
+            tri.append((lid[i, j], lid[i + 1, j], lid[i, j + 1]))
+            if i + j + 1 < freq:
+                tri.append((lid[i + 1, j], lid[i + 1, j + 1], lid[i, j + 1]))
+    tri = np.asarray(tri, np.int64)
+    a, b, c = v0[f0[:, 0]], v0[f0[:, 1]], v0[f0[:, 2]]
+    pts = (a[:, None, :] + (b - a)[:, None, :] * (ii[None, :, None] / freq) +
+           (c - a)[:, None, :] * (jj[None, :, None] / freq)).reshape(-1, 3)
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    faces = (tri[None, :, :] + (np.arange(20) * ii.size)[:, None, None]).reshape(-1, 3)
+    # points on shared edges are computed once per face: merge them
+    key = np.round(pts * 1e6).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    return pts[first], inv.reshape(-1)[faces]
+
+
+BATCH_RES = (128, 256, 512, 1024)
+
+
+def batch_instance(rng: np.random.Generator, index: int, tri_range=(1e4, 5e5), res_choices=BATCH_RES,
+                   spp_range=(4, 256), depth_range=(8, 16), estimator=L.EST_NEE_MIS) -> SceneSpec:
+    """One render of the batched workload: a noise-displaced geodesic sphere of
+    N triangles (N log-uniform in tri_range) with a material drawn from
+    {Lambertian, conductor, rough dielectric, Disney}, on a Lambertian floor,
+    lit by 1-4 quad area lights of radiance U[2, 20]; resolution drawn from
+    res_choices (square), spp log-uniform in spp_range, max depth uniform in
+    depth_range."""
+    s = SceneSpec(name=f"batch_{index}")
+    n_tri = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+    freq = max(1, int(round((n_tri / 20.0) ** 0.5)))
+    kind = int(rng.integers(4))
+    if kind == 0:
+        mat = s.add_material(L.MAT_LAMBERTIAN, list(rng.uniform(0.2, 0.9, 3)))
+    elif kind == 1:
+        a = float(rng.uniform(0.05, 0.5))
+        mat = s.add_material(L.MAT_CONDUCTOR, [*GOLD_ETA, *GOLD_K, a, a] if rng.uniform() < 0.5
+                             else [*COPPER_ETA, *COPPER_K, a, a])
+    elif kind == 2:
+        a = float(rng.uniform(0.0, 0.3))
+        mat = s.add_material(L.MAT_MFDIELECTRIC, [float(rng.uniform(1.3, 1.8)), a, a])
+    else:
+        base = rng.uniform(0, 1, 3)
+        metallic, roughness, specular, spec_tint, sheen, clearcoat, gloss, subsurface, aniso, spec_trans = \
+            rng.uniform(0, 1, 10)
+        mat = s.add_material(L.MAT_DISNEY, [*base, spec_trans, metallic, subsurface, specular, roughness,
+                                            spec_tint, aniso, sheen, 0.5, clearcoat, gloss, 1.5])
+    v, f = geodesic_sphere(freq)
+    r = 1.0 + 0.08 * value_noise(v, int(rng.integers(1 << 30)))
+    s.add_mesh((v * r[:, None]).astype(np.float32), f.astype(np.int32), mat)
+    floor = s.add_material(L.MAT_LAMBERTIAN, [0.6, 0.6, 0.6])
+    s.add_mesh(*quad_tris((-8, -1.1, -8), (-8, -1.1, 8), (8, -1.1, 8), (8, -1.1, -8)), floor)
+    for _ in range(int(rng.integers(1, 5))):
+        le = float(rng.uniform(2.0, 20.0))
+        lm = s.add_material(L.MAT_EMITTER, [1.0, 1.0, 1.0, le])
+        cx, cz = rng.uniform(-2.5, 2.5, 2)
+        h = float(rng.uniform(0.5, 1.5)) / 2
+        s.add_rect(1, float(rng.uniform(2.5, 4.0)), cx - h, cx + h, cz - h, cz + h, lm)
+    res = int(rng.choice(res_choices))
+    spp = int(round(np.exp(rng.uniform(np.log(spp_range[0]), np.log(spp_range[1])))))
+    sx = max(d for d in range(1, int(spp ** 0.5) + 1) if spp % d == 0)
+    depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+    az = float(rng.uniform(0, 2 * np.pi))
+    pos = (4.2 * np.sin(az), 0.8, 4.2 * np.cos(az))
+    front = (-np.sin(az), -0.15, -np.cos(az))
+    s.camera = dict(fov_deg=40.0, aspect=1.0, focal_dist=1.0, aperture=0.0, up=(0, 1, 0), front=front, pos=pos)
+    s.render = dict(width=res, height=res, spp_x=sx, spp_y=spp // sx, max_depth=depth,
+                    sampler=L.SAMPLER_HALTON_OWEN, estimator=estimator, rr_depth=3, light_samples=4,
+                    seed=1000 + index, bg=(0, 0, 0))
+    return s
+
+
+def batch64(seed=4, n=64, **kw):
+    """The batched workload: n independent renders drawn from one seed."""
+    rng = np.random.default_rng(seed)
+    return [batch_instance(rng, i, **kw) for i in range(n)]
