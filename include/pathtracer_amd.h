@@ -301,6 +301,79 @@ pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
 pt_status pt_batch_autotune(pt_batch *batch, const pt_autotune_cfg *tc, pt_tuning *out);
 
 /* ------------------------------------------------------------------ */
+/* Learned path guiding (no counterpart in the reference: its           */
+/* integrators sample the BSDF and the lights only,                     */
+/* integrator/maxdepth_nee.hpp:9-256).  A small MLP maps a surface      */
+/* point (position in the scene box, normal) to a mixture of 8 von      */
+/* Mises-Fisher lobes over incident directions and the probability c of */
+/* sampling it instead of the BSDF.  The network has BF16 weights and   */
+/* FP32 accumulation, runs on the matrix cores, and is trained online   */
+/* with Adam on the radiance the renderer's own paths carry.  Guided    */
+/* vertices weight every direction by the one-sample MIS pdf            */
+/* c p_guide + (1 - c) p_bsdf, and NEE uses that pdf in its power       */
+/* heuristic, so the estimator stays unbiased for any network state.    */
+/* ------------------------------------------------------------------ */
+typedef struct pt_guide pt_guide;
+#define PT_GUIDE_INPUTS 6      /* position in [0,1]^3, (normal + 1) / 2        */
+#define PT_GUIDE_OUTPUTS 41    /* raw outputs (see pathtracer_amd/csrc/pt_guide.h) */
+#define PT_GUIDE_NPARAMS 27497 /* FP32 parameters: per layer W[k][n] then b[n]   */
+#define PT_GUIDE_MIX 48        /* floats of a processed mixture                */
+
+/* A guide for a committed scene (its bounding box normalises positions),
+ * with seeded initial weights.                                            */
+pt_status pt_guide_create(pt_scene *scene, uint32_t seed, pt_guide **out);
+void pt_guide_destroy(pt_guide *guide);
+pt_status pt_guide_get_params(pt_guide *guide, float *params);
+/* Sets the FP32 parameters (BF16 copies follow) and resets Adam.          */
+pt_status pt_guide_set_params(pt_guide *guide, const float *params);
+/* The network on n points x (n*6): raw outputs (n*41) and/or processed
+ * mixtures (n*48: per lobe (kappa mu, log-weight), the 8 weights, c).      */
+pt_status pt_guide_forward(pt_guide *guide, const float *x, size_t n, float *raw,
+                           float *mix);
+/* On n processed mixtures: guide pdf at dirs (n*3) and/or samples from
+ * uniforms u (n*3) with their pdf.                                        */
+pt_status pt_guide_eval(const float *mix, const float *dirs, const float *u, size_t n,
+                        float *pdf, float *sample_dir, float *sample_pdf);
+/* Training samples: n*11 floats (x[6], direction[3], p_bsdf, weight F/q).
+ * Mean loss -w log(c p_guide + (1 - c) p_bsdf) and its gradient
+ * (PT_GUIDE_NPARAMS floats) over the n samples.                           */
+pt_status pt_guide_gradient(pt_guide *guide, const float *samples, size_t n, float *grad,
+                            float *loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the n samples.            */
+pt_status pt_guide_train_step(pt_guide *guide, const float *samples, size_t n, float lr,
+                              float *loss);
+
+enum {
+  PT_GUIDE_NO_TRAIN = 1u << 0    /* render with the current network only     */
+};
+typedef struct pt_guide_cfg {
+  uint32_t train_spp;      /* samples per pixel rendered while training
+                              (0 = a quarter of the frame's)                */
+  uint32_t first_pass_spp; /* training passes: 1, 2, 4, ... spp (0 = 1)     */
+  uint32_t batch;          /* training batch (0 = 16384)                    */
+  uint32_t max_steps;      /* Adam steps per pass at most (0 = 64)          */
+  uint32_t record_stride;  /* one path in this many records (0 = 1)         */
+  uint32_t record_verts;   /* vertices recorded per path (0 = 4)            */
+  uint32_t kinds;          /* material kinds guided, bit per pt_mat_kind
+                              (0 = Lambertian, rough conductor, Disney)     */
+  float lr;                /* Adam learning rate (0 = 1e-3)                 */
+  uint32_t flags;          /* PT_GUIDE_*                                    */
+} pt_guide_cfg;
+typedef struct pt_guide_stats {
+  uint32_t passes, steps;
+  uint64_t samples;        /* training samples collected                    */
+  float last_loss;
+  double ms_train;         /* Adam steps (device time)                      */
+} pt_guide_stats;
+/* Renders cfg's frame (estimator PT_EST_NEE_MIS) with the guide: training
+ * passes of growing spp, each followed by Adam steps on its samples, then
+ * the rest of the frame with the trained network.  Every pass is unbiased,
+ * so all count equally in the film.                                        */
+pt_status pt_render_guided(pt_scene *scene, const pt_render_cfg *cfg, pt_guide *guide,
+                           const pt_guide_cfg *gcfg, pt_stats *stats,
+                           pt_guide_stats *gstats);
+
+/* ------------------------------------------------------------------ */
 /* Parity / test entry points (batched, executed on the device).        */
 /* ------------------------------------------------------------------ */
 

@@ -28,6 +28,12 @@
 #include "oracle.h"
 #include "or_scene.h"
 
+/* Materials whose BSDF only reflects (their f is not zero below the surface in
+ * the reference: Lambertian::f, material.hpp:128-133). */
+static int reflect_only(uint32_t kind) {
+  return kind == PT_MAT_LAMBERTIAN || kind == PT_MAT_CONDUCTOR || kind == PT_MAT_PHONG;
+}
+
 enum { SLOT_LENS_X = 0, SLOT_LENS_Y = 1, SLOT_TIME = 2, SLOT_AUX = 4, SLOT_RR = 5,
        SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_RIS = 16, SLOT_NEE_LIGHT = 40 };
 static inline uint32_t slot(uint32_t d, uint32_t purpose) { return d * 64u + purpose; }
@@ -504,6 +510,9 @@ static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t
             le = Lg->le;
           }
         }
+        // a reflection-only BSDF takes no light from behind its surface (the
+        // NEE integrator's max(0, n.wi), maxdepth_nee.hpp:137, 192)
+        if (reflect_only(m->kind) && vdot(h.n, lwi) <= 0.f) pl = 0.f;
         if (vis && pl > 0.f) {
           pl /= (float)N;
           v3 f = or_bsdf_f(m, wo, lwi, &h);

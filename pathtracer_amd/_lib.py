@@ -28,6 +28,8 @@ RENDER_COUNTERS, RENDER_NO_SORT, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
 BATCH_NO_GRAPH = 256
 TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF = 1, 2, 4
+GUIDE_NO_TRAIN = 1
+GUIDE_INPUTS, GUIDE_OUTPUTS, GUIDE_NPARAMS, GUIDE_MIX = 6, 41, 27497, 48
 PRIM_FLIP_FACE, PRIM_FORCE_INSIDE, PRIM_NOT_LIGHT = 1, 2, 4
 NPARAMS = 30
 
@@ -96,6 +98,21 @@ class BatchCfg(ctypes.Structure):
     _fields_ = [("wavefront", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
+class GuideCfg(ctypes.Structure):
+    _fields_ = [("train_spp", ctypes.c_uint32), ("first_pass_spp", ctypes.c_uint32), ("batch", ctypes.c_uint32),
+                ("max_steps", ctypes.c_uint32), ("record_stride", ctypes.c_uint32),
+                ("record_verts", ctypes.c_uint32), ("kinds", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("flags", ctypes.c_uint32)]
+
+
+class GuideStats(ctypes.Structure):
+    _fields_ = [("passes", ctypes.c_uint32), ("steps", ctypes.c_uint32), ("samples", ctypes.c_uint64),
+                ("last_loss", ctypes.c_float), ("ms_train", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # every symbol include/pathtracer_amd.h declares
 EXPORTED = [
     "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_mesh", "pt_scene_add_mesh_normals",
@@ -104,6 +121,8 @@ EXPORTED = [
     "pt_scene_set_tuning", "pt_autotune",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_cfg", "pt_batch_render",
     "pt_batch_film_read", "pt_batch_set_tuning", "pt_batch_autotune",
+    "pt_guide_create", "pt_guide_destroy", "pt_guide_get_params", "pt_guide_set_params", "pt_guide_forward",
+    "pt_guide_eval", "pt_guide_gradient", "pt_guide_train_step", "pt_render_guided",
     "pt_sample", "pt_intersect", "pt_bsdf_eval", "pt_bsdf_sample", "pt_light_sample",
     "pt_device_synchronize", "pt_last_error", "pt_version",
 ]
@@ -144,6 +163,16 @@ def _load():
         "pt_batch_film_read": (i32, [_P, u32, _P]),
         "pt_batch_set_tuning": (i32, [_P, ctypes.POINTER(Tuning)]),
         "pt_batch_autotune": (i32, [_P, ctypes.POINTER(AutotuneCfg), ctypes.POINTER(Tuning)]),
+        "pt_guide_create": (i32, [_P, u32, ctypes.POINTER(_P)]),
+        "pt_guide_destroy": (None, [_P]),
+        "pt_guide_get_params": (i32, [_P, _P]),
+        "pt_guide_set_params": (i32, [_P, _P]),
+        "pt_guide_forward": (i32, [_P, _P, sz, _P, _P]),
+        "pt_guide_eval": (i32, [_P, _P, _P, sz, _P, _P, _P]),
+        "pt_guide_gradient": (i32, [_P, _P, sz, _P, ctypes.POINTER(ctypes.c_float)]),
+        "pt_guide_train_step": (i32, [_P, _P, sz, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+        "pt_render_guided": (i32, [_P, ctypes.POINTER(RenderCfg), _P, ctypes.POINTER(GuideCfg),
+                                   ctypes.POINTER(Stats), ctypes.POINTER(GuideStats)]),
         "pt_sample": (i32, [u32, u32, u32, _P, _P, _P, sz, _P]),
         "pt_intersect": (i32, [_P, _P, sz, ctypes.c_int, _P, _P]),
         "pt_bsdf_eval": (i32, [ctypes.POINTER(MaterialDesc), _P, _P, _P, _P, sz, _P, _P]),

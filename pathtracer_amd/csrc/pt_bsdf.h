@@ -391,6 +391,13 @@ namespace pt {
 // a kernel instantiated for a scene's kinds carries no register pressure from
 // the materials that scene does not use.  Kinds outside MS scatter nothing.
 constexpr uint32_t kMatAll = 0xffffffffu;
+// BSDFs that only reflect: their f (Lambertian's in particular,
+// material.hpp:128-133) is not zero behind the surface, so estimators that
+// use |cos| must exclude those directions.
+PT_HD bool bsdf_reflect_only(uint32_t kind) {
+  return kind == PT_MAT_LAMBERTIAN || kind == PT_MAT_CONDUCTOR || kind == PT_MAT_PHONG;
+}
+
 constexpr uint32_t kMatBasic = (1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_EMITTER) | (1u << PT_MAT_CONDUCTOR) |
                                (1u << PT_MAT_MFDIELECTRIC) | (1u << PT_MAT_THINDIELECTRIC) |
                                (1u << PT_MAT_PHONG) | (1u << PT_MAT_KAJIYA_KAY);

@@ -23,67 +23,18 @@
 #include <vector>
 
 #include "../../include/pathtracer_amd.h"
+#include "pt_guide_host.h"
 #include "pt_launch.h"
 
 using namespace pt;
 
+#include "pt_common.h"
+
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
-static pt_status fail(pt_status st, const std::string &msg) {
+pt_status pt_fail(pt_status st, const std::string &msg) {
   g_err = msg;
   return st;
-}
-#define HIP_TRY(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t _e = (expr);                                                                    \
-    if (_e != hipSuccess)                                                                      \
-      return fail(PT_ERR_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(_e));      \
-  } while (0)
-
-// ------------------------------------------------------------------ device buffers
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;  // owns a device allocation: never copied
-  DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
-    o.p = nullptr;
-    o.bytes = 0;
-  }
-  DevBuf &operator=(DevBuf &&o) noexcept {
-    if (this != &o) {
-      release();
-      p = o.p;
-      bytes = o.bytes;
-      o.p = nullptr;
-      o.bytes = 0;
-    }
-    return *this;
-  }
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  hipError_t alloc(size_t n) {
-    if (n <= bytes && p) return hipSuccess;
-    release();
-    if (n == 0) n = 256;
-    hipError_t e = hipMalloc(&p, n);
-    if (e == hipSuccess) bytes = n;
-    return e;
-  }
-  template <class T>
-  T *as() const { return reinterpret_cast<T *>(p); }
-};
-
-static hipError_t upload(DevBuf &b, const void *src, size_t n) {
-  hipError_t e = b.alloc(n);
-  if (e != hipSuccess) return e;
-  if (n) return hipMemcpy(b.p, src, n, hipMemcpyHostToDevice);
-  return hipSuccess;
 }
 
 // Wavefront state of one render stream (a scene's own renders, or a batch).
@@ -129,6 +80,10 @@ struct pt_scene {
   DevBuf d_off;
   pt_tuning tuning{};
   pt_build_cfg build_cfg{};
+  float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};  // all primitives
+  // guided renders: processed mixtures per live entry, training records
+  DevBuf g_mix, g_rec;
+  uint32_t guide_tag = 0;  // tags of record sets, never reused
   DevBuf film, samples;
   uint32_t film_w = 0, film_h = 0;
   uint64_t film_spp = 0;
@@ -717,6 +672,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
 
   // -- BVH
   B.nodes.resize(4 * std::max<size_t>(1, 2 * nprim));
+  Builder::bounds(B.refs.data(), nprim, s->box_lo, s->box_hi);
   int root = B.build(0, nprim, 0, true);
   uint32_t nn = B.node_count.load();
   if (root < 0) {  // the whole scene is one leaf: wrap it in a root with both children on it
@@ -1106,6 +1062,9 @@ struct ChunkPlan {
   uint32_t mat_kinds;
   bool count;
   Grids grids;
+  // guided renders: the network and the material kinds it guides
+  const GuideView *gv = nullptr;
+  uint32_t gkinds = 0, infer_grid = 0;
 };
 static void launch_chunk(hipStream_t st, const DevScene &s, const DevCamera &cam, const RenderParams &rp,
                          const Wave &w, const float *offx, const float *offy, float *film, float *samples,
@@ -1120,6 +1079,7 @@ static void launch_chunk(hipStream_t st, const DevScene &s, const DevCamera &cam
     launch_extend(st, s, w, qi, n, pl.grids.trav, pl.count, b);
     if (ev) ev->end(e);
     e = ev ? ev->begin(2) : -1;
+    if (pl.gv) launch_guide_infer(st, s, w, qi, *pl.gv, pl.gkinds, pl.infer_grid);
     launch_shade(st, s, rp, w, qi, d, n, pl.grids.shade, b, pl.mat_kinds);
     if (ev) ev->end(e);
     e = ev ? ev->begin(3) : -1;
@@ -1160,7 +1120,23 @@ extern "C" pt_status pt_scene_set_tuning(pt_scene *s, const pt_tuning *t) {
   return PT_OK;
 }
 
+// One pass of a guided render: the guide, the kinds it guides, and whether
+// the pass writes training records (one path in `stride`, `verts` vertices)
+// and collects their samples.
+struct GuidePass {
+  pt_guide *g;
+  uint32_t kinds;
+  bool record;
+  uint32_t stride, verts;
+};
+
+static pt_status render_frame(pt_scene *s, const pt_render_cfg *cfg, pt_stats *stats, const GuidePass *gp);
+
 extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *stats) {
+  return render_frame(s, cfg, stats, nullptr);
+}
+
+static pt_status render_frame(pt_scene *s, const pt_render_cfg *cfg, pt_stats *stats, const GuidePass *gp) {
   uint32_t spp = 0, sppc = 0;
   pt_status st = check_cfg("pt_render", s, cfg, spp, sppc);
   if (st != PT_OK) return st;
@@ -1211,6 +1187,33 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   pl.mat_kinds = s->ds.mat_kinds;
   pl.count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
   pl.grids = make_grids(s->n_cu, s->tuning);
+  Wave wave = s->ws.wave;
+  GuideView gview{};
+  uint32_t n_rec = 0;
+  if (gp) {
+    const size_t lcap = (size_t)wave.part_cap * PT_PARTS;
+    HIP_TRY(s->g_mix.alloc(lcap * PT_G_MIX * sizeof(float)));
+    wave.gmix = s->g_mix.as<float>();
+    wave.grec = nullptr;
+    wave.grec_stride = std::max(1u, gp->stride);
+    wave.grec_verts = gp->verts;
+    if (gp->record) {
+      n_rec = (cap + wave.grec_stride - 1) / wave.grec_stride;
+      const size_t bytes = (size_t)n_rec * gp->verts * PT_G_REC * sizeof(float);
+      if (s->g_rec.bytes < bytes) {
+        HIP_TRY(s->g_rec.alloc(bytes));
+        HIP_TRY(hipMemset(s->g_rec.p, 0, s->g_rec.bytes));  // no stale tags
+      }
+      wave.grec = s->g_rec.as<float>();
+    }
+    gview = gp->g->view();
+    pl.gv = &gview;
+    pl.gkinds = gp->kinds;
+    pl.estimator = PT_EST_GUIDED;
+    rp.estimator = PT_EST_GUIDED;
+    // the network kernel holds 66 KB of LDS: two resident blocks per CU
+    pl.infer_grid = std::max<uint32_t>(PT_PARTS, s->n_cu * 2u / PT_PARTS * PT_PARTS);
+  }
   StageEvents ev;
   ev.on = (cfg->flags & PT_RENDER_TIMING) != 0;
   ev.st = s->stream;
@@ -1225,8 +1228,13 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
     rp.chunk_first = first;
     rp.chunk_paths = n;
     pl.max_paths = n;
-    launch_chunk(s->stream, s->ds, cam, rp, s->ws.wave, s->d_off.as<float>(), s->d_off.as<float>() + spp,
+    if (gp) wave.gpass = ++s->guide_tag;
+    launch_chunk(s->stream, s->ds, cam, rp, wave, s->d_off.as<float>(), s->d_off.as<float>() + spp,
                  s->film.as<float>(), samples, nullptr, pl, &ev);
+    if (gp && gp->record)
+      launch_guide_collect(s->stream, wave, (n + wave.grec_stride - 1) / wave.grec_stride, wave.grec_verts,
+                           wave.grec_stride, wave.gpass, gp->g->tset.as<float>(), gp->g->tcap,
+                           gp->g->tcount.as<uint32_t>());
     launches += pl.nverts;
   }
   HIP_TRY(hipEventRecord(tstop, s->stream));
@@ -1245,6 +1253,136 @@ extern "C" pt_status pt_render(pt_scene *s, const pt_render_cfg *cfg, pt_stats *
   (void)hipEventDestroy(tstop);
   return PT_OK;
 }
+
+// ------------------------------------------------------------------ guided render
+pt_status ptx_scene_box(pt_scene *s, float lo[3], float hi[3], int *device) {
+  if (!s->committed) return fail(PT_ERR_INVALID, "pt_guide_create: scene not committed");
+  pt_status st = ensure_stream(s);
+  if (st != PT_OK) return st;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = s->box_lo[a];
+    hi[a] = s->box_hi[a];
+  }
+  *device = s->device;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_render_guided(pt_scene *s, const pt_render_cfg *cfg, pt_guide *g, const pt_guide_cfg *gc,
+                                      pt_stats *stats, pt_guide_stats *gs) {
+  uint32_t spp = 0, sppc = 0;
+  pt_status st = check_cfg("pt_render_guided", s, cfg, spp, sppc);
+  if (st != PT_OK) return st;
+  if (!g) return fail(PT_ERR_INVALID, "pt_render_guided: NULL guide");
+  if (cfg->estimator != PT_EST_NEE_MIS)
+    return fail(PT_ERR_UNSUPPORTED, "pt_render_guided: guiding extends the PT_EST_NEE_MIS estimator");
+  if (cfg->flags & PT_RENDER_KEEP_SAMPLES)
+    return fail(PT_ERR_UNSUPPORTED, "pt_render_guided: per-sample radiance is not kept");
+  st = ensure_stream(s);
+  if (st != PT_OK) return st;
+  if (g->device != s->device) return fail(PT_ERR_UNSUPPORTED, "pt_render_guided: guide and scene devices differ");
+  const pt_guide_cfg c0{};
+  const pt_guide_cfg &c = gc ? *gc : c0;
+  const bool train = !(c.flags & PT_GUIDE_NO_TRAIN);
+  const uint32_t train_spp = train ? std::min(sppc, c.train_spp ? c.train_spp : std::max(1u, sppc / 4u)) : 0u;
+  const uint32_t batch = c.batch ? std::min(c.batch, 1u << 20) : 16384u;
+  const uint32_t max_steps = c.max_steps ? c.max_steps : 64u;
+  const float lr = c.lr > 0.f ? c.lr : 1e-3f;
+  GuidePass gp{};
+  gp.g = g;
+  gp.kinds = c.kinds ? c.kinds
+                     : ((1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_CONDUCTOR) | (1u << PT_MAT_DISNEY));
+  gp.stride = c.record_stride ? c.record_stride : 1u;
+  gp.verts = std::min(c.record_verts ? c.record_verts : 4u, 64u);
+  if (train) {
+    g->tcap = 1u << 21;
+    HIP_TRY(g->tset.alloc((size_t)g->tcap * PT_G_SAMPLE * sizeof(float)));
+    HIP_TRY(g->tcount.alloc(256));
+  }
+  pt_stats acc{};
+  pt_guide_stats gacc{};
+  double ms_train = 0.0;
+  hipEvent_t ta, tb;
+  HIP_TRY(hipEventCreate(&ta));
+  HIP_TRY(hipEventCreate(&tb));
+  auto add = [&](const pt_stats &p) {
+    acc.paths += p.paths;
+    acc.segments += p.segments;
+    acc.shadow_rays += p.shadow_rays;
+    acc.node_visits += p.node_visits;
+    acc.prim_tests += p.prim_tests;
+    acc.shadow_node_visits += p.shadow_node_visits;
+    acc.shadow_prim_tests += p.shadow_prim_tests;
+    acc.nan_samples += p.nan_samples;
+    acc.extend_launches += p.extend_launches;
+    acc.ms_total += p.ms_total;
+    acc.ms_generate += p.ms_generate;
+    acc.ms_extend += p.ms_extend;
+    acc.ms_shade += p.ms_shade;
+    acc.ms_shadow += p.ms_shadow;
+    acc.ms_film += p.ms_film;
+  };
+  uint32_t begin = cfg->spp_begin, done = 0, pass_spp = c.first_pass_spp ? c.first_pass_spp : 1u;
+  bool first = true;
+  pt_status result = PT_OK;
+  while (done < sppc && result == PT_OK) {
+    const bool training = done < train_spp;
+    const uint32_t cnt = training ? std::min(pass_spp, train_spp - done) : sppc - done;
+    pt_render_cfg pc = *cfg;
+    pc.spp_begin = begin;
+    pc.spp_count = cnt;
+    if (!first) pc.flags |= PT_RENDER_ACCUMULATE;
+    gp.record = training;
+    if (training) {
+      if (hipMemsetAsync(g->tcount.p, 0, sizeof(uint32_t), s->stream) != hipSuccess) {
+        result = fail(PT_ERR_DEVICE, "pt_render_guided: memset failed");
+        break;
+      }
+    }
+    pt_stats ps{};
+    result = render_frame(s, &pc, &ps, &gp);
+    if (result != PT_OK) break;
+    add(ps);
+    gacc.passes += 1;
+    if (training) {
+      uint32_t n = 0;
+      if (hipMemcpy(&n, g->tcount.p, sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+        result = fail(PT_ERR_DEVICE, "pt_render_guided: reading the sample count failed");
+        break;
+      }
+      n = std::min(n, g->tcap);
+      gacc.samples += n;
+      if (n > 0) {
+        const uint32_t bsz = std::min(batch, n);
+        const uint32_t steps = std::min(max_steps, (n + bsz - 1) / bsz);
+        (void)hipEventRecord(ta, s->stream);
+        for (uint32_t k = 0; k < steps && result == PT_OK; ++k) {
+          float *lp = (k + 1 == steps) ? &gacc.last_loss : nullptr;
+          result = ptx_guide_step(g, s->stream, g->tset.as<float>(), n, bsz, 0x9E3779B9u * (s->guide_tag + 1u) + k + 1u,
+                                  lr, lp);
+          gacc.steps += 1;
+        }
+        (void)hipEventRecord(tb, s->stream);
+        (void)hipEventSynchronize(tb);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ta, tb);
+        ms_train += ms;
+      }
+      pass_spp *= 2u;
+    }
+    begin += cnt;
+    done += cnt;
+    first = false;
+  }
+  (void)hipEventDestroy(ta);
+  (void)hipEventDestroy(tb);
+  if (result != PT_OK) return result;
+  acc.ms_total += ms_train;
+  gacc.ms_train = ms_train;
+  if (stats) *stats = acc;
+  if (gs) *gs = gacc;
+  return PT_OK;
+}
+
 
 extern "C" pt_status pt_film_read(pt_scene *s, float *out) {
   if (!s || !out) return fail(PT_ERR_INVALID, "pt_film_read: NULL argument");

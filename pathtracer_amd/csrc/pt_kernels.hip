@@ -15,6 +15,7 @@
 // The estimator is getRayColor of integrator/maxdepth_nee.hpp:9-256
 // unrolled into a loop (EST 0), or NEE+MIS (EST 1).
 #include "pt_device.h"
+#include "pt_guide.h"
 #include "pt_launch.h"
 
 namespace pt {
@@ -296,6 +297,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
       f3 L = s3(0.f);  // radiance this vertex adds (emission); NEE goes through shadow rays
       bool prev_spec = fl & 1u;
       bool done = false;
+      if constexpr (EST == PT_EST_GUIDED) {
+        // the path reached vertex d: complete the training record of vertex d - 1
+        // (everything up to and including its direct light is in the slot)
+        if (w.grec && d >= 1u && d - 1u < w.grec_verts && slot % w.grec_stride == 0u) {
+          float *rec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + (d - 1u)) * PT_G_REC;
+          rec[13] = w.lx[slot];
+          rec[14] = w.ly[slot];
+          rec[15] = w.lz[slot];
+          rec[17] = __uint_as_float(w.gpass);
+        }
+      }
       if constexpr (EST == PT_EST_RESERVOIR) {
         // getRayColor of maxdepth_reservoir_di.hpp:9-131
         const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
@@ -500,7 +512,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
           float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
           f3 wo = neg3(norm3(dir));
           Scatter sc;
-          if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
+          // guided: the learned mixture of this point (c = 0: not guided); the
+          // technique (guide with probability c, else the BSDF) is drawn first
+          const float *mix = nullptr;
+          float gc = 0.f;
+          bool use_guide = false;
+          if constexpr (EST == PT_EST_GUIDED) {
+            mix = w.gmix + (size_t)i * PT_G_MIX;
+            gc = mix[40];
+            use_guide = gc > 0.f && pt_rng(g, rslot(d, SLOT_GUIDE), rp.seed) < gc;
+          }
+          bool scat;
+          if (use_guide) {
+            sc.wi = g_sample(mix, pt_rng(g, rslot(d, SLOT_GUIDE + 1u), rp.seed),
+                             pt_rng(g, rslot(d, SLOT_GUIDE + 2u), rp.seed),
+                             pt_rng(g, rslot(d, SLOT_GUIDE + 3u), rp.seed));
+            sc.flags = 0u;
+            sc.att = s3(0.f);
+            sc.pdf = 0.f;
+            scat = true;
+          } else {
+            scat = bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc);
+          }
+          if (!scat) {
             L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
             done = true;
           } else if (sc.flags & BX_SPECULAR) {
@@ -586,23 +620,47 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
                   tmax = len - 0.001f;
                 }
               }
+              // a reflection-only BSDF takes no light from behind its surface (the NEE
+              // integrator's max(0, n.wi), maxdepth_nee.hpp:137, 192)
+              if (bsdf_reflect_only(mat->kind) && dot3(h.sp.n, lwi) <= 0.f) pl = 0.f;
               if (pl > 0.f) {
                 pl /= (float)N;
                 f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
                 float pb = bsdf_pdf<MS>(*mat, wo, lwi, h.sp);
+                if (EST == PT_EST_GUIDED && gc > 0.f) pb = gc * g_pdf(mix, lwi) + (1.f - gc) * pb;
                 float cl = fabsf(dot3(h.sp.n, lwi));
                 f3 c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
                 if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)) emit_shadow(w, p, lwi, tmax, c, slot);
               }
             }
-            beta = scl3(mul3(beta, sc.att), cs / sc.pdf);
+            float pm = sc.pdf;
+            if (EST == PT_EST_GUIDED && gc > 0.f) {
+              // one-sample MIS over the two techniques (a guided direction behind a
+              // reflection-only surface carries nothing)
+              const bool below = bsdf_reflect_only(mat->kind) && dot3(h.sp.n, wn) <= 0.f;
+              const f3 f = below ? s3(0.f) : bsdf_f<MS>(*mat, wo, wn, h.sp);
+              const float pbs = bsdf_pdf<MS>(*mat, wo, wn, h.sp);
+              pm = gc * g_pdf(mix, wn) + (1.f - gc) * pbs;
+              if (w.grec && d < w.grec_verts && slot % w.grec_stride == 0u) {
+                float *rec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + d) * PT_G_REC;
+                for (int v = 0; v < 6; ++v) rec[v] = mix[41 + v];
+                rec[6] = wn.x; rec[7] = wn.y; rec[8] = wn.z;
+                rec[9] = pbs;
+                rec[10] = beta.x; rec[11] = beta.y; rec[12] = beta.z;
+                rec[16] = __uint_as_float(w.gpass);
+              }
+              beta = pm > 0.f ? scl3(mul3(beta, f), cs / pm) : s3(0.f);
+            } else {
+              beta = scl3(mul3(beta, sc.att), cs / sc.pdf);
+            }
             wi = wn;
             fl = 0u;
             if (EST == 0) {
               cont = d + 1u < D;  // the emitter check of the next hit happens at the next vertex
             } else {
               cont = d + 1u <= D;
-              ppdf = sc.pdf;
+              if (EST == PT_EST_GUIDED) cont = cont && (beta.x != 0.f || beta.y != 0.f || beta.z != 0.f);
+              ppdf = pm;
               pprev = p;
             }
           }
@@ -841,6 +899,9 @@ static void launch_shade_b(hipStream_t st, const DevScene &s, const RenderParams
     case PT_EST_NAIVE: launch_shade_est<PT_EST_NAIVE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
     case PT_EST_ROULETTE: launch_shade_est<PT_EST_ROULETTE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
     case PT_EST_RESERVOIR: launch_shade_est<PT_EST_RESERVOIR, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
+    case PT_EST_GUIDED:
+      if constexpr (!BATCH) launch_shade_est<PT_EST_GUIDED, false>(st, s, rp, w, qi, d, g, b, mat_kinds);
+      break;
     default: launch_shade_est<PT_EST_REFERENCE_NEE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
   }
 }

@@ -149,6 +149,6 @@ PT_HD uint32_t pt_pixel_seed(uint32_t pixel, uint32_t seed) {
 // Random-stream slots: slot = vertex*64 + purpose.
 enum {
   SLOT_LENS_X = 0, SLOT_LENS_Y = 1, SLOT_TIME = 2, SLOT_AUX = 4, SLOT_RR = 5,
-  SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_RIS = 16, SLOT_NEE_LIGHT = 40
+  SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_RIS = 16, SLOT_NEE_LIGHT = 40, SLOT_GUIDE = 44
 };
 PT_HD uint32_t rslot(uint32_t d, uint32_t purpose) { return d * 64u + purpose; }

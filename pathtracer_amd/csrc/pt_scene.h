@@ -26,6 +26,9 @@
 namespace pt {
 
 enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
+// NEE+MIS with the learned guide as a second directional technique (only
+// through pt_render_guided, so not in the public estimator list).
+enum { PT_EST_GUIDED = 5 };
 enum { LIGHT_QUAD = 1, LIGHT_SPHERE = 2, LIGHT_ENV = 3 };
 
 // A child reference: >= 0 internal node index; < 0 leaf ~(start<<4 | count-1);
@@ -160,6 +163,13 @@ struct Wave {
   // slot q at rc[(3 i + c) cap + q], its visibility at rvis[4 q + i]
   float *rc;
   uint8_t *rvis;
+  // path guiding (allocated for guided renders only): the processed mixture
+  // of live entry e at gmix[48 e]; training records of vertex v of slot
+  // r * grec_stride at grec[20 (r * grec_verts + v)]; the pass number tags
+  // the records this pass wrote
+  float *gmix;
+  float *grec;
+  uint32_t grec_stride, grec_verts, gpass;
   uint32_t cap, part_cap, shadow_part_cap;
   PT_HD uint32_t *live(uint32_t b, uint32_t p) const { return counts + (b * PT_PARTS + p) * PT_COUNTER_STRIDE; }
   PT_HD uint32_t *shadow(uint32_t p) const { return counts + (2 * PT_PARTS + p) * PT_COUNTER_STRIDE; }

@@ -215,3 +215,85 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Guide:
+    """The learned path-guiding network of a committed Scene (pt_guide_*)."""
+
+    def __init__(self, scene: Scene, seed: int = 1):
+        h = ctypes.c_void_p()
+        L.check(L.lib.pt_guide_create(scene.h, seed, ctypes.byref(h)), "pt_guide_create")
+        self.h = h
+        self.scene = scene
+
+    def params(self) -> np.ndarray:
+        p = np.empty(L.GUIDE_NPARAMS, np.float32)
+        L.check(L.lib.pt_guide_get_params(self.h, L.ptr(p)), "pt_guide_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (L.GUIDE_NPARAMS,)
+        L.check(L.lib.pt_guide_set_params(self.h, L.ptr(p)), "pt_guide_set_params")
+
+    def forward(self, x):
+        """Raw outputs (n, 41) and processed mixtures (n, 48) of points x (n, 6)."""
+        x = np.ascontiguousarray(x, np.float32)
+        n = x.shape[0]
+        raw = np.empty((n, L.GUIDE_OUTPUTS), np.float32)
+        mix = np.empty((n, L.GUIDE_MIX), np.float32)
+        L.check(L.lib.pt_guide_forward(self.h, L.ptr(x), n, L.ptr(raw), L.ptr(mix)), "pt_guide_forward")
+        return raw, mix
+
+    def gradient(self, samples):
+        samples = np.ascontiguousarray(samples, np.float32)
+        g = np.empty(L.GUIDE_NPARAMS, np.float32)
+        loss = ctypes.c_float()
+        L.check(L.lib.pt_guide_gradient(self.h, L.ptr(samples), samples.shape[0], L.ptr(g), ctypes.byref(loss)),
+                "pt_guide_gradient")
+        return loss.value, g
+
+    def train_step(self, samples, lr=1e-3) -> float:
+        samples = np.ascontiguousarray(samples, np.float32)
+        loss = ctypes.c_float()
+        L.check(L.lib.pt_guide_train_step(self.h, L.ptr(samples), samples.shape[0], lr, ctypes.byref(loss)),
+                "pt_guide_train_step")
+        return loss.value
+
+    def render(self, cfg: L.RenderCfg, gcfg: L.GuideCfg | None = None):
+        """pt_render_guided: (image, stats dict, guide stats dict)."""
+        gcfg = gcfg or L.GuideCfg()
+        st, gs = L.Stats(), L.GuideStats()
+        L.check(L.lib.pt_render_guided(self.scene.h, ctypes.byref(cfg), self.h, ctypes.byref(gcfg),
+                                       ctypes.byref(st), ctypes.byref(gs)), "pt_render_guided")
+        img = np.empty((cfg.height, cfg.width, 3), np.float32)
+        L.check(L.lib.pt_film_read(self.scene.h, L.ptr(img)), "pt_film_read")
+        return img, st.as_dict(), gs.as_dict()
+
+    def close(self):
+        if self.h:
+            L.lib.pt_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def guide_eval(mix, dirs=None, u=None):
+    """Guide pdf at dirs and/or samples (dir, pdf) from uniforms u, on the device."""
+    mix = np.ascontiguousarray(mix, np.float32)
+    n = mix.shape[0]
+    pdf = sdir = spdf = None
+    d = None if dirs is None else np.ascontiguousarray(dirs, np.float32)
+    uu = None if u is None else np.ascontiguousarray(u, np.float32)
+    if d is not None:
+        pdf = np.empty(n, np.float32)
+    if uu is not None:
+        sdir = np.empty((n, 3), np.float32)
+        spdf = np.empty(n, np.float32)
+    L.check(L.lib.pt_guide_eval(L.ptr(mix), L.ptr(d), L.ptr(uu), n, L.ptr(pdf), L.ptr(sdir), L.ptr(spdf)),
+            "pt_guide_eval")
+    return pdf, sdir, spdf

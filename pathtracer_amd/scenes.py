@@ -398,3 +398,44 @@ def batch64(seed=4, n=64, **kw):
     """The batched workload: n independent renders drawn from one seed."""
     rng = np.random.default_rng(seed)
     return [batch_instance(rng, i, **kw) for i in range(n)]
+
+
+def guiding_room(width=1024, height=1024, spp_x=16, spp_y=16, max_depth=16, rr_depth=3, seed=5,
+                 statue_freq=25, slit=0.05, estimator=L.EST_NEE_MIS) -> SceneSpec:
+    """Indirect-dominated interior: a closed 4x3x3 room (x in [0,4], y in
+    [0,3], z in [0,3]) of Lambertian walls (albedo 0.7), a Disney floor
+    (roughness 0.3, metallic 0), a 12.5k-triangle noise-displaced glossy
+    conductor statue, and one quad light of radiance 200 outside the left
+    wall, seen only through a vertical slit `slit` wide in that wall."""
+    s = SceneSpec(name="guiding_room")
+    wall = s.add_material(L.MAT_LAMBERTIAN, [0.7, 0.7, 0.7])
+    floor = s.add_material(L.MAT_DISNEY, [0.6, 0.55, 0.5, 0.0, 0.0, 0.0, 0.5, 0.3, 0.0, 0.0, 0.0, 0.5, 0.0, 1.0,
+                                          1.5])
+    statue = s.add_material(L.MAT_CONDUCTOR, [*GOLD_ETA, *GOLD_K, 0.15, 0.15])
+    light = s.add_material(L.MAT_EMITTER, [1.0, 1.0, 1.0, 200.0])
+    X, Y, Z = 4.0, 3.0, 3.0
+    s.add_rect(1, 0.0, 0.0, X, 0.0, Z, floor)          # floor
+    s.add_rect(1, Y, 0.0, X, 0.0, Z, wall)             # ceiling
+    s.add_rect(2, 0.0, 0.0, X, 0.0, Y, wall)           # back  (z = 0)
+    s.add_rect(2, Z, 0.0, X, 0.0, Y, wall)             # front (z = 3)
+    s.add_rect(0, X, 0.0, Y, 0.0, Z, wall)             # right (x = 4)
+    z0, z1 = Z / 2 - slit / 2, Z / 2 + slit / 2
+    y0, y1 = 0.6, 2.4                                   # the slit's extent
+    s.add_rect(0, 0.0, 0.0, Y, 0.0, z0, wall)          # left wall (x = 0) around the slit
+    s.add_rect(0, 0.0, 0.0, Y, z1, Z, wall)
+    s.add_rect(0, 0.0, 0.0, y0, z0, z1, wall)
+    s.add_rect(0, 0.0, y1, Y, z0, z1, wall)
+    s.add_rect(0, -0.3, 0.9, 2.1, 0.9, 2.1, light)     # outside, emitting towards +x
+    v, f = geodesic_sphere(statue_freq)
+    r = 0.55 * (1.0 + 0.08 * value_noise(v, seed))
+    s.add_mesh((v * r[:, None] + np.array([2.5, 0.58, 1.2])).astype(np.float32), f.astype(np.int32), statue)
+    pos = np.array([3.7, 1.7, 2.85])
+    look = np.array([1.2, 0.9, 0.9])
+    front = look - pos
+    front = tuple(front / np.linalg.norm(front))
+    s.camera = dict(fov_deg=60.0, aspect=width / height, focal_dist=1.0, aperture=0.0, up=(0, 1, 0),
+                    front=front, pos=tuple(pos))
+    s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=max_depth,
+                    sampler=L.SAMPLER_HALTON_OWEN, estimator=estimator, rr_depth=rr_depth, light_samples=4,
+                    seed=23, bg=(0, 0, 0))
+    return s
