@@ -335,10 +335,17 @@ pt_status pt_guide_forward(pt_guide *guide, const float *x, size_t n, float *raw
 pt_status pt_guide_eval(const float *mix, const float *dirs, const float *u, size_t n,
                         float *pdf, float *sample_dir, float *sample_pdf);
 /* Training samples: n*11 floats (x[6], direction[3], p_bsdf, weight F/q).
- * Mean loss -w log(c p_guide + (1 - c) p_bsdf) and its gradient
+ * Mean loss -w (log p_guide + log(c p_guide + (1 - c) p_bsdf)) and its gradient
  * (PT_GUIDE_NPARAMS floats) over the n samples.                           */
 pt_status pt_guide_gradient(pt_guide *guide, const float *samples, size_t n, float *grad,
                             float *loss);
+/* The range of the selection probability c = c_min + (c_max - c_min)
+ * sigmoid(s) (default [0.05, 0.5]): 1 / (1 - c_max) bounds how much a guided
+ * vertex can raise a path's throughput over BSDF sampling.                 */
+pt_status pt_guide_set_select_range(pt_guide *guide, float c_min, float c_max);
+/* The training samples the last training pass of pt_render_guided
+ * collected (n*11 floats, the layout above): copies min(n, cap) of them.  */
+pt_status pt_guide_samples(pt_guide *guide, float *out, size_t cap, size_t *n);
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the n samples.            */
 pt_status pt_guide_train_step(pt_guide *guide, const float *samples, size_t n, float lr,
                               float *loss);
@@ -348,16 +355,17 @@ enum {
 };
 typedef struct pt_guide_cfg {
   uint32_t train_spp;      /* samples per pixel rendered while training
-                              (0 = a quarter of the frame's)                */
+                              (0 = an eighth of the frame's)                */
   uint32_t first_pass_spp; /* training passes: 1, 2, 4, ... spp (0 = 1)     */
   uint32_t batch;          /* training batch (0 = 16384)                    */
-  uint32_t max_steps;      /* Adam steps per pass at most (0 = 64)          */
+  uint32_t max_steps;      /* Adam steps after each training pass (0 = 64)  */
   uint32_t record_stride;  /* one path in this many records (0 = 1)         */
   uint32_t record_verts;   /* vertices recorded per path (0 = 4)            */
   uint32_t kinds;          /* material kinds guided, bit per pt_mat_kind
                               (0 = Lambertian, rough conductor, Disney)     */
   float lr;                /* Adam learning rate (0 = 1e-3)                 */
   uint32_t flags;          /* PT_GUIDE_*                                    */
+  uint32_t guide_verts;    /* vertices 0 .. guide_verts-1 are guided (0 = 2) */
 } pt_guide_cfg;
 typedef struct pt_guide_stats {
   uint32_t passes, steps;

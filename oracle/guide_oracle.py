@@ -12,12 +12,13 @@ Model (the same as pathtracer_amd/csrc/pt_guide.h):
   x (6 values in [0,1]) -> one-blob encoding, 32 bins per value, feature
   (v, b) = bump(b - (32 x_v - 0.5)), bump(t) = max(0, 1 - t^2/9)^2
   -> 192 -> 64 -> 64 -> 64 -> 64 (ReLU after each) -> 41 raw outputs:
-  o[0..7] lobe logits, o[8] selection logit, o[9..16] log-concentrations
-  (clamped to [-5, 8]), o[17..40] mean directions (x of lobes 0..7, y, z).
-  pi = softmax, kappa = exp(clamp(r)), mu = normalize, c = 0.05 + 0.9 sigmoid(s).
+  o[0..7] lobe logits, o[8] selection logit, o[9..16] concentrations r,
+  o[17..40] mean directions (x of lobes 0..7, y, z).
+  pi = softmax, kappa = exp(1.5 + 6.5 tanh(r / 6.5)), mu = normalize,
+  c = C_LO + C_SPAN sigmoid(s) (the guide's default range [0.05, 0.5]).
   p_g(w) = sum_i pi_i kappa_i / (2 pi (1 - exp(-2 kappa_i))) exp(kappa_i (mu_i.w - 1)).
 Loss of a sample (x, w, p_bsdf, weight F/q), averaged over the batch:
-  -weight * log(c p_g(w) + (1 - c) p_bsdf).
+  -weight * (log p_g(w) + log(c p_g(w) + (1 - c) p_bsdf)).
 
 emulate=True rounds as the device does: weights, layer inputs and output
 gradients to BF16 (round to nearest even), products accumulated in FP32,
@@ -31,6 +32,7 @@ IN, BINS, H, OUT, LOBES = 6, 32, 64, 41, 8
 K = [IN * BINS, H, H, H, H]
 N = [H, H, H, H, OUT]
 NPARAMS = sum(k * n + n for k, n in zip(K, N))
+C_LO, C_SPAN = 0.05, 0.45
 
 
 def w_offset(l):
@@ -95,11 +97,11 @@ def mixture(raw):
     a = a - a.max(1, keepdims=True)
     e = np.exp(a)
     pi = e / e.sum(1, keepdims=True)
-    kappa = np.exp(np.clip(r, -5.0, 8.0))
+    kappa = np.exp(1.5 + 6.5 * np.tanh(r / 6.5))
     ln = np.maximum(np.linalg.norm(m, axis=-1), 1e-12)
     mu = m / ln[..., None]
     sg = 1.0 / (1.0 + np.exp(-s))
-    c = 0.05 + 0.9 * sg
+    c = C_LO + C_SPAN * sg
     return pi, kappa, mu, c
 
 
@@ -174,26 +176,33 @@ def loss_and_grad(params, samples, emulate=True):
     raw, xs, zs = forward(params, x, emulate)
     raw64 = raw.astype(np.float64)
     pi, kappa, mu, c = mixture(raw64)
-    v = vmf(kappa, mu, w)
-    pg = (pi * v).sum(1)
+    d = np.einsum("nij,nj->ni", mu, w)
+    with np.errstate(divide="ignore"):
+        lj = np.log(pi) + log_norm(kappa) + kappa * (d - 1)          # log(pi_i v_i)
+    lmax = lj.max(1, keepdims=True)
+    log_pg = (lmax + np.log(np.exp(lj - lmax).sum(1, keepdims=True)))[:, 0]
+    pg = np.exp(log_pg)
     pm = c * pg + (1 - c) * pb
-    live = (wt > 0) & (pm > 0)
-    loss = np.where(live, -wt * np.log(np.where(live, pm, 1.0)), 0.0).sum() / n
-    dpm = np.where(live, -wt / np.where(live, pm, 1.0) / n, 0.0)
+    with np.errstate(divide="ignore"):
+        log_pm = np.where(pb > 0, np.log(np.where(pm > 0, pm, 1.0)), np.log(c) + log_pg)
+    rho = np.exp(np.log(c) + log_pg - log_pm)
+    live = (wt > 0) & np.isfinite(log_pg) & np.isfinite(log_pm)
+    loss = np.where(live, -wt * (log_pg + log_pm), 0.0).sum() / n
+    coef = np.where(live, -wt * (1 + rho) / n, 0.0)               # times d log p_g
+    resp = np.exp(lj - log_pg[:, None])
     dout = np.zeros((n, OUT))
-    a = dpm[:, None] * c[:, None] * pi
-    dout[:, 0:8] = a * (v - pg[:, None])
+    dout[:, 0:8] = coef[:, None] * (resp - pi)
     s = raw64[:, 8]
     sg = 1 / (1 + np.exp(-s))
-    dout[:, 8] = dpm * (pg - pb) * 0.9 * sg * (1 - sg)
+    dsel = np.where(live, -wt / n * (rho / c - (1 - rho) / (1 - c)), 0.0)
+    dout[:, 8] = dsel * C_SPAN * sg * (1 - sg)
     e2 = np.exp(-2 * kappa)
-    d = np.einsum("nij,nj->ni", mu, w)
-    dv_dk = v * (1 / kappa - 2 * e2 / (1 - e2) + d - 1)
+    dlv_dk = 1 / kappa - 2 * e2 / (1 - e2) + d - 1
     r = raw64[:, 9:17]
-    dout[:, 9:17] = np.where((r >= -5) & (r <= 8), a * dv_dk * kappa, 0.0)
+    dout[:, 9:17] = coef[:, None] * resp * dlv_dk * kappa * (1 - np.tanh(r / 6.5) ** 2)
     m = np.stack([raw64[:, 17:25], raw64[:, 25:33], raw64[:, 33:41]], -1)
     ln = np.maximum(np.linalg.norm(m, axis=-1), 1e-12)
-    gm = a * v * kappa / ln
+    gm = coef[:, None] * resp * kappa / ln
     for k in range(3):
         dout[:, 17 + 8 * k:25 + 8 * k] = gm * (w[:, None, k] - d * mu[:, :, k])
     dt = np.float32 if emulate else np.float64

@@ -102,7 +102,7 @@ class GuideCfg(ctypes.Structure):
     _fields_ = [("train_spp", ctypes.c_uint32), ("first_pass_spp", ctypes.c_uint32), ("batch", ctypes.c_uint32),
                 ("max_steps", ctypes.c_uint32), ("record_stride", ctypes.c_uint32),
                 ("record_verts", ctypes.c_uint32), ("kinds", ctypes.c_uint32), ("lr", ctypes.c_float),
-                ("flags", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32), ("guide_verts", ctypes.c_uint32)]
 
 
 class GuideStats(ctypes.Structure):
@@ -122,7 +122,8 @@ EXPORTED = [
     "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_cfg", "pt_batch_render",
     "pt_batch_film_read", "pt_batch_set_tuning", "pt_batch_autotune",
     "pt_guide_create", "pt_guide_destroy", "pt_guide_get_params", "pt_guide_set_params", "pt_guide_forward",
-    "pt_guide_eval", "pt_guide_gradient", "pt_guide_train_step", "pt_render_guided",
+    "pt_guide_eval", "pt_guide_gradient", "pt_guide_train_step", "pt_guide_samples", "pt_render_guided",
+    "pt_guide_set_select_range",
     "pt_sample", "pt_intersect", "pt_bsdf_eval", "pt_bsdf_sample", "pt_light_sample",
     "pt_device_synchronize", "pt_last_error", "pt_version",
 ]
@@ -171,6 +172,8 @@ def _load():
         "pt_guide_eval": (i32, [_P, _P, _P, sz, _P, _P, _P]),
         "pt_guide_gradient": (i32, [_P, _P, sz, _P, ctypes.POINTER(ctypes.c_float)]),
         "pt_guide_train_step": (i32, [_P, _P, sz, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+        "pt_guide_samples": (i32, [_P, _P, sz, ctypes.POINTER(sz)]),
+        "pt_guide_set_select_range": (i32, [_P, ctypes.c_float, ctypes.c_float]),
         "pt_render_guided": (i32, [_P, ctypes.POINTER(RenderCfg), _P, ctypes.POINTER(GuideCfg),
                                    ctypes.POINTER(Stats), ctypes.POINTER(GuideStats)]),
         "pt_sample": (i32, [u32, u32, u32, _P, _P, _P, sz, _P]),

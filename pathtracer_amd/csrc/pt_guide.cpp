@@ -30,7 +30,7 @@ static void init_params(uint32_t seed, std::vector<float> &p) {
   }
   float *b4 = &p[g_b_offset(4)];
   for (int i = 0; i < PT_G_LOBES; ++i) {
-    b4[9 + i] = std::log(2.f);
+    b4[9 + i] = 6.5f * std::atanh((std::log(2.f) - 1.5f) / 6.5f);  // kappa = 2
     b4[17 + i] = (i & 1) ? 1.f : -1.f;
     b4[25 + i] = (i & 2) ? 1.f : -1.f;
     b4[33 + i] = (i & 4) ? 1.f : -1.f;
@@ -239,5 +239,28 @@ extern "C" pt_status pt_guide_train_step(pt_guide *g, const float *samples, size
   s = ptx_guide_step(g, g->stream, g->io_a.as<float>(), (uint32_t)n, (uint32_t)n, 0u, lr, loss);
   if (s != PT_OK) return s;
   HIP_TRY(hipStreamSynchronize(g->stream));
+  return PT_OK;
+}
+
+extern "C" pt_status pt_guide_samples(pt_guide *g, float *out, size_t cap, size_t *n) {
+  if (!g || !n || (cap && !out)) return fail(PT_ERR_INVALID, "pt_guide_samples: NULL argument");
+  *n = 0;
+  if (!g->tcount.p) return PT_OK;
+  uint32_t cnt = 0;
+  HIP_TRY(hipMemcpy(&cnt, g->tcount.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  const size_t m = std::min<size_t>(std::min<size_t>(cnt, g->tcap), cap);
+  std::vector<float> v(m * PT_G_SAMPLE);
+  if (m) HIP_TRY(hipMemcpy(v.data(), g->tset.p, v.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < m; ++i) std::memcpy(&out[i * 11], &v[i * PT_G_SAMPLE], 11 * sizeof(float));
+  *n = std::min<size_t>(cnt, g->tcap);
+  return PT_OK;
+}
+
+extern "C" pt_status pt_guide_set_select_range(pt_guide *g, float c_min, float c_max) {
+  if (!g) return fail(PT_ERR_INVALID, "pt_guide_set_select_range: NULL guide");
+  if (!(c_min > 0.f) || !(c_max < 1.f) || !(c_min <= c_max))
+    return fail(PT_ERR_INVALID, "pt_guide_set_select_range: need 0 < c_min <= c_max < 1");
+  g->c_lo = c_min;
+  g->c_span = c_max - c_min;
   return PT_OK;
 }

@@ -13,8 +13,10 @@
 //   L4      dense 64->41 + bias (raw outputs)
 // Raw output o (41 values):
 //   o[0..7]   lobe logits a_i        -> pi = softmax(a)
-//   o[8]      selection logit s      -> c = 0.05 + 0.9 sigmoid(s)
-//   o[9..16]  log-concentration r_i  -> kappa_i = exp(clamp(r_i, -5, 8))
+//   o[8]      selection logit s      -> c = c_lo + c_span sigmoid(s), in a range
+//                                       set per guide (default [0.05, 0.5])
+//   o[9..16]  concentration r_i      -> kappa_i = exp(1.5 + 6.5 tanh(r_i / 6.5)),
+//                                       smoothly within [e^-5, e^8]
 //   o[17..40] mean directions, x of lobes 0..7, then y, then z
 //             -> mu_i = normalize(o[17+i], o[25+i], o[33+i])
 // Guide pdf: p_g(w) = sum_i pi_i kappa_i / (2 pi (1 - e^{-2 kappa_i}))
@@ -87,12 +89,20 @@ PT_HD uint32_t g_bwd_frag_offset(int l) { return (uint32_t)((l - 1) * 4 * 2) * P
 //   [32 + i]          pi_i
 //   [40]              c (0: this point is not guided)
 #define PT_G_MIX 48
-// Training record of one guided vertex (20 floats): [0..5] the point's
+// Training record of one guided vertex (24 floats): [0..5] the point's
 // inputs, [6..8] the sampled direction, [9] p_bsdf of it, [10..12] the
 // throughput arriving at the vertex, [13..15] the slot's radiance when the
 // path reached the next vertex, [16] / [17] the pass that wrote the vertex
-// fields / the radiance (as uint32 bits).
-#define PT_G_REC 20
+// fields / the radiance (as uint32 bits), [18..20] the share of the next
+// vertex's emission that MIS gave to light sampling (the training target is
+// the radiance arriving along the direction, unweighted).
+#define PT_G_REC 24
+// kappa from the raw concentration output, and d kappa / d r.
+PT_HD float g_kappa(float r) { return expf(1.5f + 6.5f * tanhf(r * (1.f / 6.5f))); }
+PT_HD float g_dkappa(float r, float kappa) {
+  const float t = tanhf(r * (1.f / 6.5f));
+  return kappa * (1.f - t * t);
+}
 
 // Training scratch, transposed (bf16, row stride = the batch padded to 64):
 // layer inputs X_0 (192 rows) and X_1..X_4 (64 rows each) from row
@@ -107,6 +117,7 @@ struct GuideView {
   const uint16_t *bwd;   // backward fragments
   const float *bias;     // 5 x 64
   float lo[3], inv_ext[3];  // position normalisation: (p - lo) * inv_ext
+  float c_lo, c_span;       // selection probability range
 };
 
 // Materials the guide samples for: kinds in the mask whose lobes are all

@@ -115,7 +115,7 @@ struct LobeQuad {
   float inv_len[4], cos_mw[4];
   float c, s_sig;
 };
-PT_D void g_process(const f32x4 (&o)[4], uint32_t g, LobeQuad &r) {
+PT_D void g_process(const f32x4 (&o)[4], uint32_t g, float c_lo, float c_span, LobeQuad &r) {
   // y components of lobes 4g + j sit in tile 2 of group g + 2
   float my[4];
 #pragma unroll
@@ -134,7 +134,7 @@ PT_D void g_process(const f32x4 (&o)[4], uint32_t g, LobeQuad &r) {
   const float s_sel = __shfl(o[0][0], (int)((threadIdx.x & 15u) | 32u));  // row 8: group 2, j = 0
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    r.kappa[j] = expf(fminf(fmaxf(o[1][j], -5.f), 8.f));
+    r.kappa[j] = g_kappa(o[1][j]);
     float l = sqrtf(o[2][j] * o[2][j] + my[j] * my[j] + o[3][j] * o[3][j]);
     l = fmaxf(l, 1e-12f);
     r.inv_len[j] = 1.f / l;
@@ -146,7 +146,7 @@ PT_D void g_process(const f32x4 (&o)[4], uint32_t g, LobeQuad &r) {
   }
   const float sg = 1.f / (1.f + expf(-s_sel));
   r.s_sig = sg;
-  r.c = 0.05f + 0.9f * sg;
+  r.c = c_lo + c_span * sg;
   (void)g;
 }
 
@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) k_guide_infer(DevScene s, Wave w, int qi,
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       LobeQuad r;
-      g_process(acc[t], g, r);
+      g_process(acc[t], g, gv.c_lo, gv.c_span, r);
       const uint32_t pl = 64u * wv + 16u * t + q;  // point of the block
       const uint32_t i = i0 + pl;
       if (i < n) {
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(256) k_guide_points(GuideView gv, const float 
       }
     }
     LobeQuad r;
-    g_process(acc[t], g, r);
+    g_process(acc[t], g, gv.c_lo, gv.c_span, r);
     if (mixout && p < n) {
       float *mx = mixout + p * PT_G_MIX;
       if (g < 2u) {
@@ -306,6 +306,7 @@ __global__ void k_guide_mix(const float *mix, const float *dirs, const float *u,
 
 // ------------------------------------------------------------------ training
 // Sample record: x (6), direction (3), p_bsdf, weight F/q, pad (12 floats).
+// Loss per sample: -w (log p_g + log(c p_g + (1 - c) p_bsdf)), averaged.
 #ifndef PT_G_SAMPLE
 #define PT_G_SAMPLE 12
 #endif
@@ -401,29 +402,42 @@ __global__ void __launch_bounds__(256) k_guide_train_a(GuideView gv, const float
 #pragma unroll
       for (uint32_t nt = 0; nt < 4; ++nt) acc[nt] = mfma(wl[(nt * 2 + s) * 64 + lane], in[s], acc[nt]);
   }
-  // -- loss -w log(c p_g + (1 - c) p_bsdf) and its gradient at the raw outputs
+  // -- loss -w (log p_g + log(c p_g + (1 - c) p_bsdf)) and its gradient at the raw outputs.
+  // The first term keeps the lobes learning the radiance while c is small, the
+  // second trains c (and the lobes) for the mixture.  Gradients go through the
+  // lobe responsibilities r_i = pi_i v_i / p_g (log-sum-exp), so no 1/p_g appears:
+  //   d(-log p_g - log p_mix) = -(1 + rho) d log p_g - (p_g - p_bsdf) / p_mix dc,
+  //   rho = c p_g / p_mix in [0, 1].
   LobeQuad r;
-  g_process(acc, g, r);
-  float pg = 0.f, dv_dk[4], dot_mw[4];
+  g_process(acc, g, gv.c_lo, gv.c_span, r);
+  float lj[4], dot_mw[4], dlv_dk[4];
+  float lmax = -INFINITY;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     dot_mw[j] = r.mux[j] * wx + r.muy[j] * wy + r.muz[j] * wz;
-    // vMF density and its derivative in kappa
     const float k = r.kappa[j];
     const float e2 = expf(-2.f * k);
-    r.v[j] = expf(g_log_norm(k) + k * (dot_mw[j] - 1.f));
-    dv_dk[j] = r.v[j] * (1.f / k - 2.f * e2 / (1.f - e2) + dot_mw[j] - 1.f);
-    pg += r.pi[j] * r.v[j];
+    lj[j] = logf(r.pi[j]) + g_log_norm(k) + k * (dot_mw[j] - 1.f);  // log(pi_i v_i)
+    dlv_dk[j] = 1.f / k - 2.f * e2 / (1.f - e2) + dot_mw[j] - 1.f;  // d log v_i / d kappa
+    lmax = fmaxf(lmax, lj[j]);
   }
-  pg = (g < 2u ? pg : 0.f);
-  pg += __shfl_xor(pg, 16);  // groups 0 and 1 hold the 8 lobes
-  pg = __shfl(pg, (int)q);   // every lane of the point (group 2 needs it for c)
+  if (g >= 2u) lmax = -INFINITY;
+  lmax = fmaxf(lmax, __shfl_xor(lmax, 16));  // groups 0 and 1 hold the 8 lobes
+  float esum = 0.f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) esum += (g < 2u) ? expf(lj[j] - lmax) : 0.f;
+  esum += __shfl_xor(esum, 16);
+  float log_pg = lmax + logf(esum);
+  log_pg = __shfl(log_pg, (int)q);  // every lane of the point (group 2 needs it for c)
   const float c = __shfl(r.c, (int)(q | 32u));
   const float sg = __shfl(r.s_sig, (int)(q | 32u));
+  const float pg = expf(log_pg);
   const float pm = c * pg + (1.f - c) * pb;
-  const bool live = b < batch && wt > 0.f && pm > 0.f;
-  const float dl_dpm = live ? -wt / pm * inv_batch : 0.f;
-  float loss = (live && g == 0u) ? -wt * logf(pm) * inv_batch : 0.f;
+  const float log_pm = pb > 0.f ? logf(pm) : logf(c) + log_pg;
+  const float rho = expf(logf(c) + log_pg - log_pm);
+  const bool live = b < batch && wt > 0.f && isfinite(log_pg) && isfinite(log_pm);
+  const float coef = live ? -wt * (1.f + rho) * inv_batch : 0.f;  // times d log p_g
+  float loss = (live && g == 0u) ? -wt * (log_pg + log_pm) * inv_batch : 0.f;
   f32x4 dout[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) dout[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -431,20 +445,15 @@ __global__ void __launch_bounds__(256) k_guide_train_a(GuideView gv, const float
   if (g < 2u) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const float a = dl_dpm * c * r.pi[j];
-      dout[0][j] = a * (r.v[j] - pg);  // logits (softmax)
-      dout[1][j] = a * dv_dk[j] * r.kappa[j];  // log-concentration
-      const float gm = a * r.v[j] * r.kappa[j] * r.inv_len[j];
+      const float resp = expf(lj[j] - log_pg);
+      dout[0][j] = coef * (resp - r.pi[j]);                                      // logits
+      dout[1][j] = coef * resp * dlv_dk[j] * g_dkappa(acc[1][j], r.kappa[j]);    // concentration
+      const float gm = coef * resp * r.kappa[j] * r.inv_len[j];
       dout[2][j] = gm * (wx - dot_mw[j] * r.mux[j]);
       dmuy[j] = gm * (wy - dot_mw[j] * r.muy[j]);
       dout[3][j] = gm * (wz - dot_mw[j] * r.muz[j]);
     }
   }
-  // the clamp of the log-concentration passes no gradient
-  // (recomputed here from the raw output rows 16 + 4 g + j)
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-    if (acc[1][j] < -5.f || acc[1][j] > 8.f) dout[1][j] = 0.f;
   // y components go back to the lanes of group g + 2 (tile 2, rows 40 + 4 g' + j)
   float back[4];
 #pragma unroll
@@ -457,7 +466,9 @@ __global__ void __launch_bounds__(256) k_guide_train_a(GuideView gv, const float
 #pragma unroll
     for (int j = 0; j < 4; ++j) dout[1][j] = 0.f;  // rows 24..31
     if (g == 2u) {
-      dout[0][0] = dl_dpm * (pg - pb) * 0.9f * sg * (1.f - sg);  // selection logit (row 8)
+      // (p_g - p_bsdf) / p_mix = rho / c - (1 - rho) / (1 - c)
+      const float dsel = live ? -wt * inv_batch * (rho / c - (1.f - rho) / (1.f - c)) : 0.f;
+      dout[0][0] = dsel * gv.c_span * sg * (1.f - sg);  // selection logit (row 8)
       dout[0][1] = dout[0][2] = dout[0][3] = 0.f;
     } else {
       dout[0] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -641,7 +652,8 @@ __global__ void k_guide_collect(Wave w, uint32_t n_rec, uint32_t verts, uint32_t
   if (__float_as_uint(rec[16]) != pass || __float_as_uint(rec[17]) != pass) return;
   const uint32_t slot = (r / verts) * stride;
   const f3 lf = mk3(w.lx[slot], w.ly[slot], w.lz[slot]);
-  const f3 la = mk3(rec[13], rec[14], rec[15]);
+  // the next vertex's emission at full weight: the radiance along the direction
+  const f3 la = mk3(rec[13] - rec[18], rec[14] - rec[19], rec[15] - rec[20]);
   const f3 bt = mk3(rec[10], rec[11], rec[12]);
   float s = 0.f;
   s += bt.x > 0.f ? (lf.x - la.x) / bt.x : 0.f;

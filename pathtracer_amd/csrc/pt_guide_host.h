@@ -8,6 +8,7 @@ struct pt_guide {
   int device = -1;
   hipStream_t stream = nullptr;  // the guide's own entry points
   float lo[3] = {0.f, 0.f, 0.f}, inv_ext[3] = {1.f, 1.f, 1.f};
+  float c_lo = 0.05f, c_span = 0.45f;  // selection probability in [c_lo, c_lo + c_span]
   // parameters: FP32 master, Adam moments, the kernels' BF16 / padded copies
   DevBuf params, mom, vel, fwd, bwd, bias;
   uint32_t adam_t = 0;
@@ -29,6 +30,8 @@ struct pt_guide {
       v.lo[a] = lo[a];
       v.inv_ext[a] = inv_ext[a];
     }
+    v.c_lo = c_lo;
+    v.c_span = c_span;
     return v;
   }
   pt::GuideParamBufs pbufs() const {

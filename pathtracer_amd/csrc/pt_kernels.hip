@@ -297,15 +297,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
       f3 L = s3(0.f);  // radiance this vertex adds (emission); NEE goes through shadow rays
       bool prev_spec = fl & 1u;
       bool done = false;
+      // guided: the training record of vertex d - 1, completed here (the slot
+      // holds everything up to and including that vertex's direct light)
+      float *prec = nullptr;
       if constexpr (EST == PT_EST_GUIDED) {
-        // the path reached vertex d: complete the training record of vertex d - 1
-        // (everything up to and including its direct light is in the slot)
         if (w.grec && d >= 1u && d - 1u < w.grec_verts && slot % w.grec_stride == 0u) {
-          float *rec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + (d - 1u)) * PT_G_REC;
-          rec[13] = w.lx[slot];
-          rec[14] = w.ly[slot];
-          rec[15] = w.lz[slot];
-          rec[17] = __uint_as_float(w.gpass);
+          prec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + (d - 1u)) * PT_G_REC;
+          prec[13] = w.lx[slot];
+          prec[14] = w.ly[slot];
+          prec[15] = w.lz[slot];
+          prec[17] = __uint_as_float(w.gpass);
+          prec[18] = prec[19] = prec[20] = 0.f;
         }
       }
       if constexpr (EST == PT_EST_RESERVOIR) {
@@ -398,7 +400,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
             }
           }
         }
-      } else if constexpr (EST >= PT_EST_NAIVE) {
+      } else if constexpr (EST == PT_EST_NAIVE || EST == PT_EST_ROULETTE) {
         // maxdepth_naive.hpp:9-75 / roulette_naive.hpp:9-78: no NEE; misses and
         // the naive depth limit return bg_color (the environment is not consulted)
         const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
@@ -473,7 +475,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
             if (d < D) L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
           } else if (s.has_env) {
             float pl = env_pdf(s, dir) / (float)N;
-            L = add3(L, scl3(mul3(beta, env_lookup(s, dir)), power_w(ppdf, pl)));
+            const f3 le = mul3(beta, env_lookup(s, dir));
+            const float wm = power_w(ppdf, pl);
+            L = add3(L, scl3(le, wm));
+            if (EST == PT_EST_GUIDED && prec) {  // the target is the unweighted radiance
+              prec[18] = le.x * (1.f - wm); prec[19] = le.y * (1.f - wm); prec[20] = le.z * (1.f - wm);
+            }
           } else {
             L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
           }
@@ -498,7 +505,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
               f3 nl = (Lg.type == LIGHT_QUAD) ? Lg.n : h.ng;
               f3 le = (dot3(nl, neg3(norm3(dir))) > 0.f) ? Lg.le : s3(0.f);
               float pl = light_pdf_sa(s, Lg, pprev, h.p) / (float)N;
-              L = add3(L, scl3(mul3(beta, le), power_w(ppdf, pl)));
+              const float wm = power_w(ppdf, pl);
+              L = add3(L, scl3(mul3(beta, le), wm));
+              if (EST == PT_EST_GUIDED && prec) {  // the target is the unweighted radiance
+                const f3 r = scl3(mul3(beta, le), 1.f - wm);
+                prec[18] = r.x; prec[19] = r.y; prec[20] = r.z;
+              }
             }
             done = true;
           }
@@ -519,7 +531,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
           bool use_guide = false;
           if constexpr (EST == PT_EST_GUIDED) {
             mix = w.gmix + (size_t)i * PT_G_MIX;
-            gc = mix[40];
+            gc = d < w.gverts ? mix[40] : 0.f;
             use_guide = gc > 0.f && pt_rng(g, rslot(d, SLOT_GUIDE), rp.seed) < gc;
           }
           bool scat;

@@ -260,6 +260,16 @@ class Guide:
                 "pt_guide_train_step")
         return loss.value
 
+    def set_select_range(self, c_min: float, c_max: float):
+        L.check(L.lib.pt_guide_set_select_range(self.h, c_min, c_max), "pt_guide_set_select_range")
+
+    def samples(self, cap: int = 1 << 21) -> np.ndarray:
+        """The training samples of the last training pass (n, 11)."""
+        out = np.empty((cap, 11), np.float32)
+        n = ctypes.c_size_t()
+        L.check(L.lib.pt_guide_samples(self.h, L.ptr(out), cap, ctypes.byref(n)), "pt_guide_samples")
+        return out[:min(n.value, cap)].copy()
+
     def render(self, cfg: L.RenderCfg, gcfg: L.GuideCfg | None = None):
         """pt_render_guided: (image, stats dict, guide stats dict)."""
         gcfg = gcfg or L.GuideCfg()

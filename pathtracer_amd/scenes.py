@@ -406,7 +406,8 @@ def guiding_room(width=1024, height=1024, spp_x=16, spp_y=16, max_depth=16, rr_d
     [0,3], z in [0,3]) of Lambertian walls (albedo 0.7), a Disney floor
     (roughness 0.3, metallic 0), a 12.5k-triangle noise-displaced glossy
     conductor statue, and one quad light of radiance 200 outside the left
-    wall, seen only through a vertical slit `slit` wide in that wall."""
+    wall (0.4 x 0.4, 1.2 behind it), seen only through a vertical slit `slit`
+    wide in that wall."""
     s = SceneSpec(name="guiding_room")
     wall = s.add_material(L.MAT_LAMBERTIAN, [0.7, 0.7, 0.7])
     floor = s.add_material(L.MAT_DISNEY, [0.6, 0.55, 0.5, 0.0, 0.0, 0.0, 0.5, 0.3, 0.0, 0.0, 0.0, 0.5, 0.0, 1.0,
@@ -425,7 +426,11 @@ def guiding_room(width=1024, height=1024, spp_x=16, spp_y=16, max_depth=16, rr_d
     s.add_rect(0, 0.0, 0.0, Y, z1, Z, wall)
     s.add_rect(0, 0.0, 0.0, y0, z0, z1, wall)
     s.add_rect(0, 0.0, y1, Y, z0, z1, wall)
-    s.add_rect(0, -0.3, 0.9, 2.1, 0.9, 2.1, light)     # outside, emitting towards +x
+    # outside, 1.2 behind the wall, emitting towards +x: the beam through the slit
+    # lights a band of the right wall (z in ~[0.75, 2.25]) and parts of the floor
+    # and ceiling; the rest of the room, and most of what the camera sees, is lit
+    # only after two or more bounces
+    s.add_rect(0, -1.2, 1.3, 1.7, 1.3, 1.7, light)
     v, f = geodesic_sphere(statue_freq)
     r = 0.55 * (1.0 + 0.08 * value_noise(v, seed))
     s.add_mesh((v * r[:, None] + np.array([2.5, 0.58, 1.2])).astype(np.float32), f.astype(np.int32), statue)

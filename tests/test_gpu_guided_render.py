@@ -1,0 +1,89 @@
+"""Guided rendering (pt_render_guided) on the slit-lit room.
+
+1. With a kinds mask that guides nothing, the guided estimator is the NEE+MIS
+   estimator path for path (it draws no extra random numbers), so the two
+   films agree to float-atomic order.
+2. Unbiased: guided and plain NEE+MIS films agree in 6x6-pixel block means to
+   within their estimated noise (z-test over the blocks).
+3. What the trained network gains: at equal samples per pixel, the guided
+   film's relMSE against a high-spp plain reference is below the plain
+   film's and below that of the same method with an untrained network.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+W = H = 36
+
+
+@pytest.fixture(scope="module")
+def room(gpu):
+    from pathtracer_amd import render as R, scenes
+    return R.Scene(scenes.guiding_room(width=W, height=H, spp_x=16, spp_y=16))
+
+
+def _plain(room, spp_side, seed):
+    img, st = room.render(room.cfg(spp_x=spp_side, spp_y=spp_side, seed=seed))
+    return img.astype(np.float64), st
+
+
+def _guided(room, spp_side, seed, gseed=1, **gk):
+    from pathtracer_amd import render as R, _lib as L
+    g = R.Guide(room, seed=gseed)
+    gc = L.GuideCfg()
+    for k, v in gk.items():
+        setattr(gc, k, v)
+    img, st, gs = g.render(room.cfg(spp_x=spp_side, spp_y=spp_side, seed=seed), gc)
+    g.close()
+    return img.astype(np.float64), st, gs
+
+
+def test_unguided_kinds_equal_plain(room):
+    from pathtracer_amd import _lib as L
+    plain, _ = _plain(room, 4, 3)
+    img, _, gs = _guided(room, 4, 3, kinds=1 << L.MAT_EMITTER, flags=L.GUIDE_NO_TRAIN)
+    assert gs["steps"] == 0
+    np.testing.assert_allclose(img, plain, rtol=1e-5, atol=1e-6)
+
+
+def _blocks(img, b=6):
+    h, w, _ = img.shape
+    return img.reshape(h // b, b, w // b, b, 3).mean((1, 3))
+
+
+def test_guided_is_unbiased(room):
+    """Four independent films of each method: per 6x6 block, the difference of
+    the method means over its standard error (from the replicate spread)."""
+    P = np.stack([_blocks(_plain(room, 24, 11 + k)[0]) for k in range(4)])
+    runs = [_guided(room, 24, 21 + k, gseed=1 + k, train_spp=96) for k in range(4)]
+    assert all(r[2]["steps"] > 0 and r[2]["samples"] > 0 and np.isfinite(r[2]["last_loss"]) for r in runs)
+    Gm = np.stack([_blocks(r[0]) for r in runs])
+    var = P.var(0, ddof=1) / 4 + Gm.var(0, ddof=1) / 4
+    z = (Gm.mean(0) - P.mean(0)) / np.sqrt(var.mean(-1, keepdims=True) + 1e-12)
+    assert np.mean(z ** 2) < 2.5, np.mean(z ** 2)
+    # the whole-image means
+    pm, gm = P.mean((1, 2, 3)), Gm.mean((1, 2, 3))
+    zi = (gm.mean() - pm.mean()) / np.sqrt(pm.var(ddof=1) / 4 + gm.var(ddof=1) / 4)
+    assert abs(zi) < 4, zi
+
+
+def _relmse(img, ref):
+    """relMSE of the films scaled to unit mean (the room is dim)."""
+    mu = ref.mean()
+    return float(np.mean(((img - ref) / mu) ** 2 / ((ref / mu) ** 2 + 1e-2)))
+
+
+def test_trained_guide_reduces_error(gpu):
+    """At 256x256 and 256 spp (32 of them while training): guided below the
+    plain tracer and below the same method with an untrained network."""
+    from pathtracer_amd import render as R, scenes, _lib as L
+    room = R.Scene(scenes.guiding_room(width=256, height=256, spp_x=16, spp_y=16))
+    ref = sum(_plain(room, 64, 900 + k)[0] for k in range(8)) / 8   # 32768 spp
+    plain = np.mean([_relmse(_plain(room, 16, s)[0], ref) for s in (21, 22)])
+    guided = np.mean([_relmse(_guided(room, 16, s, gseed=s)[0], ref) for s in (31, 32)])
+    untrained = np.mean([_relmse(_guided(room, 16, s, gseed=s, flags=L.GUIDE_NO_TRAIN)[0], ref)
+                         for s in (41, 42)])
+    print(f"relMSE at 256 spp: plain {plain:.4g}, guided {guided:.4g}, untrained guide {untrained:.4g}")
+    assert guided < 0.92 * plain
+    assert guided < 0.92 * untrained
