@@ -214,8 +214,55 @@ def main_batch(args):
 
 
 def relmse(img, ref):
+    """relMSE of the films scaled to the reference's unit mean (the room is dim)."""
     import numpy as np
-    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)))
+    mu = float(np.mean(ref))
+    x, r = img.astype(np.float64) / mu, ref.astype(np.float64) / mu
+    return float(np.mean((x - r) ** 2 / (r ** 2 + 1e-2)))
+
+
+def guided_quality(scenes, R, L, q):
+    """relMSE against a 32768-spp plain reference at q x q for the plain and the
+    guided method at 16, 64, 256 spp, and the guided time to reach the plain
+    method's 256-spp relMSE (log-log interpolation)."""
+    import numpy as np
+    qs = R.Scene(scenes.guiding_room(width=q, height=q))
+    ref = np.zeros((q, q, 3))
+    for k in range(8):
+        img, _ = qs.render(qs.cfg(spp_x=64, spp_y=64, seed=900 + k))
+        ref += img / 8
+    curve = {"plain": [], "guided": []}
+    for side in (4, 8, 16):
+        pe, pt, ge, gt = [], [], [], []
+        for k in range(3):  # three films of each method, averaged
+            c = qs.cfg(spp_x=side, spp_y=side, seed=11 + k)
+            t = time.perf_counter()
+            img, _ = qs.render(c)
+            pt.append(time.perf_counter() - t)
+            pe.append(relmse(img, ref))
+            g = R.Guide(qs, seed=5 + k)
+            t = time.perf_counter()
+            img, _, _ = g.render(c, L.GuideCfg())
+            gt.append(time.perf_counter() - t)
+            ge.append(relmse(img, ref))
+            g.close()
+        curve["plain"].append({"spp": side * side, "ms": round(float(np.mean(pt)) * 1e3, 2),
+                               "relmse": float(np.mean(pe))})
+        curve["guided"].append({"spp": side * side, "ms": round(float(np.mean(gt)) * 1e3, 2),
+                                "relmse": float(np.mean(ge))})
+    log(f"quality: {curve}")
+    # time for the guided method to reach the plain 256-spp relMSE (log-log interpolation)
+    target = curve["plain"][-1]["relmse"]
+    gm = [(np.log(e["ms"]), np.log(e["relmse"])) for e in curve["guided"]]
+    t_need = None
+    for (ta, ea), (tb, eb) in zip(gm, gm[1:]):
+        if (ea - np.log(target)) * (eb - np.log(target)) <= 0 and ea != eb:
+            t_need = float(np.exp(ta + (np.log(target) - ea) * (tb - ta) / (eb - ea)))
+            break
+    if t_need is None and gm[-1][1] < np.log(target):  # beyond the curve's end: extrapolate at slope -1
+        t_need = float(np.exp(gm[-1][0] + (np.log(target) - gm[-1][1]) * -1.0))
+    ratio = (t_need / curve["plain"][-1]["ms"]) if t_need else None
+    return curve, target, t_need, ratio
 
 
 def main_guided(args):
@@ -223,12 +270,14 @@ def main_guided(args):
     displaced conductor statue, a quad light of radiance 200 seen through a
     0.05-wide slit), 1024x1024, 256 spp, depth 16.  One step renders the frame
     with the learned guide, trained online from a fresh network inside the
-    step (training passes of 1, 2, 4, ... spp up to 64 spp, Adam after each,
-    then the remaining 192 spp with the trained network).  The plain NEE+MIS
+    step (training passes of 1, 2, 4, ... spp up to 32 spp, 64 Adam steps after
+    each, then the remaining 224 spp with the trained network; the first two
+    vertices of a path are guided).  The plain NEE+MIS
     tracer renders the same frame beside it.  Quality: relMSE against a
-    16384-spp plain reference at 256x256 for both methods at 16, 64 and 256
+    32768-spp plain reference at 256x256 for both methods at 16, 64 and 256
     spp, and the time the guided method needs to reach the plain method's
-    relMSE at 256 spp."""
+    relMSE at 256 spp.  relMSE is taken on films scaled to the reference's
+    unit mean."""
     import numpy as np
     from pathtracer_amd import _lib as L, render as R, scenes
     t0 = time.perf_counter()
@@ -238,8 +287,7 @@ def main_guided(args):
     log(f"scene built in {time.perf_counter() - t0:.1f} s: {spec.n_triangles} triangles")
     cfg = scene.cfg()
     total = cfg.width * cfg.height * cfg.spp_x * cfg.spp_y
-    gcfg = L.GuideCfg()
-    gcfg.train_spp = 64
+    gcfg = L.GuideCfg()  # the defaults: train on an eighth of the samples
 
     def guided_frame(c, seed):
         g = R.Guide(scene, seed=seed)
@@ -272,40 +320,11 @@ def main_guided(args):
     plain_value = total * args.steps / dt_p / 1e6
     log(f"guided {value:.1f} Mpath/s, plain {plain_value:.1f} Mpath/s")
 
-    # quality at 256x256 against a 16384-spp plain reference
+    # quality at 256x256 against a 32768-spp plain reference
     q = 256
-    qs = R.Scene(scenes.guiding_room(width=q, height=q))
-    ref = np.zeros((q, q, 3))
-    for k in range(4):
-        img, _ = qs.render(qs.cfg(spp_x=64, spp_y=64, seed=900 + k))
-        ref += img / 4
-    curve = {"plain": [], "guided": []}
-    for side in (4, 8, 16):
-        c = qs.cfg(spp_x=side, spp_y=side, seed=11)
-        t = time.perf_counter()
-        img, _ = qs.render(c)
-        curve["plain"].append({"spp": side * side, "ms": round((time.perf_counter() - t) * 1e3, 2),
-                               "relmse": relmse(img, ref)})
-        g = R.Guide(qs, seed=5)
-        gq = L.GuideCfg()
-        gq.train_spp = max(1, side * side // 4)
-        t = time.perf_counter()
-        img, _, _ = g.render(c, gq)
-        curve["guided"].append({"spp": side * side, "ms": round((time.perf_counter() - t) * 1e3, 2),
-                                "relmse": relmse(img, ref)})
-        g.close()
-    log(f"quality: {curve}")
-    # time for the guided method to reach the plain 256-spp relMSE (log-log interpolation)
-    target = curve["plain"][-1]["relmse"]
-    gm = [(np.log(e["ms"]), np.log(e["relmse"])) for e in curve["guided"]]
-    t_need = None
-    for (ta, ea), (tb, eb) in zip(gm, gm[1:]):
-        if (ea - np.log(target)) * (eb - np.log(target)) <= 0 and ea != eb:
-            t_need = float(np.exp(ta + (np.log(target) - ea) * (tb - ta) / (eb - ea)))
-            break
-    if t_need is None and gm[-1][1] < np.log(target):  # beyond the curve's end: extrapolate at slope -1
-        t_need = float(np.exp(gm[-1][0] + (np.log(target) - gm[-1][1]) * -1.0))
-    ratio = (t_need / curve["plain"][-1]["ms"]) if t_need else None
+    curve, target, t_need, ratio = None, None, None, None
+    if not args.no_quality:
+        curve, target, t_need, ratio = guided_quality(scenes, R, L, q)
     cst = gst[0][1]
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
@@ -315,12 +334,13 @@ def main_guided(args):
         "config": {"workload": "guided_room: slit-lit 4x3x3 room, Disney floor, 12.5k-tri conductor statue; "
                                "learned guide trained online inside the step",
                    "width": cfg.width, "height": cfg.height, "spp": cfg.spp_x * cfg.spp_y, "max_depth": cfg.max_depth,
-                   "train_spp": gcfg.train_spp, "triangles": spec.n_triangles},
+                   "train_spp": cfg.spp_x * cfg.spp_y // 8, "guided_vertices": 2,
+                   "triangles": spec.n_triangles},
         "plain": {"value": round(plain_value, 2), "ms_per_step": round(dt_p / args.steps * 1e3, 2)},
         "guide": {k: gst[0][2][k] for k in ("passes", "steps", "samples", "last_loss", "ms_train")},
-        "quality": {"resolution": f"{q}x{q}", "ref_spp": 16384, "curve": curve,
-                    "plain_relmse_256spp": target, "guided_ms_to_plain_relmse": t_need,
-                    "time_ratio_guided_vs_plain": ratio},
+        "quality": None if curve is None else {
+            "resolution": f"{q}x{q}", "ref_spp": 32768, "curve": curve, "plain_relmse_256spp": target,
+            "guided_ms_to_plain_relmse": t_need, "time_ratio_guided_vs_plain": ratio},
         "segments": cst["segments"], "shadow_rays": cst["shadow_rays"],
         "cpu_baseline": None,
     }
