@@ -164,6 +164,7 @@ pt_status ptx_guide_reserve(pt_guide *g, uint32_t batch) {
   HIP_TRY(g->gpart.alloc((size_t)g->splits * PT_G_NPARAMS * sizeof(float)));
   HIP_TRY(g->dbpart.alloc((size_t)(ld / 16u) * PT_G_LAYERS * 64 * sizeof(float)));
   HIP_TRY(g->losspart.alloc((size_t)(ld / 16u) * sizeof(float)));
+  HIP_TRY(g->bsum.alloc(PT_G_BIAS_VALUES * sizeof(float)));
   g->ld = ld;
   return PT_OK;
 }

@@ -588,8 +588,19 @@ PT_HD uint32_t g_slot_of(uint32_t r32, uint32_t &g, uint32_t &j) {  // inverse o
   return 0u;
 }
 
+// Bias gradients: one wave per bias, summing its per-tile partials.
+__global__ void __launch_bounds__(64) k_guide_bias_sum(const float *dbpart, uint32_t nwaves, float *bsum) {
+  const uint32_t lb = blockIdx.x;  // layer * 64 + row
+  const uint32_t l = lb / 64u, row = lb % 64u;
+  float acc = 0.f;
+  for (uint32_t wv = threadIdx.x; wv < nwaves; wv += 64u) acc += dbpart[((size_t)wv * PT_G_LAYERS + l) * 64u + row];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (threadIdx.x == 0) bsum[lb] = acc;
+}
+
 // mode 0: pack the master weights; mode 1: Adam step (gradient = sum of the
-// partials) and pack; mode 2: the gradient alone, into mom.
+// partials; biases from k_guide_bias_sum) and pack; mode 2: the gradient
+// alone, into mom.
 __global__ void k_guide_update(float *params, float *mom, float *vel, const float *gpart, uint32_t splits,
                                const float *dbpart, uint32_t nwaves, float lr, float b1, float b2, float eps,
                                float c1, float c2, int mode, uint16_t *fwd, uint16_t *bwd, float *bias) {
@@ -604,7 +615,8 @@ __global__ void k_guide_update(float *params, float *mom, float *vel, const floa
   if (mode >= 1) {
     float gsum = 0.f;
     if (is_bias) {
-      for (uint32_t wv = 0; wv < nwaves; ++wv) gsum += dbpart[((size_t)wv * PT_G_LAYERS + l) * 64u + row];
+      gsum = dbpart[l * 64 + row];  // the reduced sums
+      (void)nwaves;
     } else {
       for (uint32_t s = 0; s < splits; ++s) gsum += gpart[(size_t)s * PT_G_NPARAMS + p];
     }
@@ -647,22 +659,24 @@ __global__ void k_guide_update(float *params, float *mom, float *vel, const floa
 __global__ void k_guide_collect(Wave w, uint32_t n_rec, uint32_t verts, uint32_t stride, uint32_t pass,
                                 float *samples, uint32_t cap, uint32_t *count) {
   const uint32_t r = g_tid();
-  if (r >= n_rec * verts) return;
-  const float *rec = w.grec + (size_t)r * PT_G_REC;
-  if (__float_as_uint(rec[16]) != pass || __float_as_uint(rec[17]) != pass) return;
-  const uint32_t slot = (r / verts) * stride;
-  const f3 lf = mk3(w.lx[slot], w.ly[slot], w.lz[slot]);
+  bool keep = false;
+  float s = 0.f;
+  const float *rec = w.grec + (size_t)min(r, n_rec * verts - 1u) * PT_G_REC;
+  if (r < n_rec * verts && __float_as_uint(rec[16]) == pass && __float_as_uint(rec[17]) == pass) {
+    const uint32_t slot = (r / verts) * stride;
+    const f3 lf = mk3(w.lx[slot], w.ly[slot], w.lz[slot]);
   // the next vertex's emission at full weight: the radiance along the direction
   const f3 la = mk3(rec[13] - rec[18], rec[14] - rec[19], rec[15] - rec[20]);
-  const f3 bt = mk3(rec[10], rec[11], rec[12]);
-  float s = 0.f;
-  s += bt.x > 0.f ? (lf.x - la.x) / bt.x : 0.f;
-  s += bt.y > 0.f ? (lf.y - la.y) / bt.y : 0.f;
-  s += bt.z > 0.f ? (lf.z - la.z) / bt.z : 0.f;
-  s *= (1.f / 3.f);
-  if (!(s > 0.f) || !isfinite(s) || !(rec[9] >= 0.f)) return;
-  const uint32_t idx = atomicAdd(count, 1u);
-  if (idx >= cap) return;
+    const f3 bt = mk3(rec[10], rec[11], rec[12]);
+    s += bt.x > 0.f ? (lf.x - la.x) / bt.x : 0.f;
+    s += bt.y > 0.f ? (lf.y - la.y) / bt.y : 0.f;
+    s += bt.z > 0.f ? (lf.z - la.z) / bt.z : 0.f;
+    s *= (1.f / 3.f);
+    keep = s > 0.f && isfinite(s) && rec[9] >= 0.f;
+  }
+  // converged here: one atomic per wave
+  const uint32_t idx = wave_append(count, keep);
+  if (!keep || idx >= cap) return;
   float *o = samples + (size_t)idx * PT_G_SAMPLE;
   for (int v = 0; v < 10; ++v) o[v] = rec[v];
   o[10] = s;
@@ -696,8 +710,10 @@ void launch_guide_train(hipStream_t st, const GuideView &gv, const float *sample
 }
 void launch_guide_update(hipStream_t st, const GuideParamBufs &pb, const GuideTrainBufs *tb, float lr, float b1,
                          float b2, float eps, float c1, float c2, int mode) {
+  if (tb) hipLaunchKernelGGL(k_guide_bias_sum, dim3(PT_G_LAYERS * 64), dim3(64), 0, st, tb->dbpart, tb->ld / 16u,
+                             tb->bsum);
   hipLaunchKernelGGL(k_guide_update, dim3(blocks_of(PT_G_NPARAMS, 256)), dim3(256), 0, st, pb.params, pb.mom,
-                     pb.vel, tb ? tb->gpart : nullptr, tb ? tb->splits : 0u, tb ? tb->dbpart : nullptr,
+                     pb.vel, tb ? tb->gpart : nullptr, tb ? tb->splits : 0u, tb ? tb->bsum : nullptr,
                      tb ? tb->ld / 16u : 0u, lr, b1, b2, eps, c1, c2, mode, pb.fwd, pb.bwd, pb.bias);
 }
 void launch_guide_collect(hipStream_t st, const Wave &w, uint32_t n_rec, uint32_t verts, uint32_t stride,

@@ -13,7 +13,7 @@ struct pt_guide {
   DevBuf params, mom, vel, fwd, bwd, bias;
   uint32_t adam_t = 0;
   // training scratch (grown to the largest batch)
-  DevBuf xt, dzt, gpart, dbpart, losspart;
+  DevBuf xt, dzt, gpart, dbpart, losspart, bsum;
   uint32_t ld = 0, splits = 8;
   // samples collected by the guided renderer, and their count
   DevBuf tset, tcount;
@@ -53,6 +53,7 @@ struct pt_guide {
     t.gpart = gpart.as<float>();
     t.dbpart = dbpart.as<float>();
     t.losspart = losspart.as<float>();
+    t.bsum = bsum.as<float>();
     return t;
   }
 };

@@ -21,6 +21,7 @@ struct GuideTrainBufs {
   uint32_t ld, splits;
   uint16_t *xt, *dzt;
   float *gpart, *dbpart, *losspart;
+  float *bsum;  // reduced bias gradients (5 x 64)
 };
 #define PT_G_SAMPLE 12
 
