@@ -119,9 +119,14 @@ enum pt_estimator {
   PT_EST_ROULETTE = 3,      /* roulette_naive.hpp:9-78: the same mixture with
                                Russian roulette q = clamp(max(f), 0.5, 1) at every
                                vertex; the depth limit returns 0                  */
-  PT_EST_RESERVOIR = 4      /* maxdepth_reservoir_di.hpp:9-131: direct light
+  PT_EST_RESERVOIR = 4,     /* maxdepth_reservoir_di.hpp:9-131: direct light
                                resampled from light_samples shadowed candidates;
                                a vertex whose continuation hits an emitter adds 0 */
+  PT_EST_VOLUME = 5         /* maxdepth_volume.hpp:72-118: single scattering in
+                               the scene's homogeneous medium (pt_scene_set_medium):
+                               a free-flight distance, light_samples shadowed light
+                               samples with HG phase and transmittance; a surface
+                               reached first adds only its emission               */
 };
 
 typedef struct pt_render_cfg {
@@ -204,6 +209,13 @@ pt_status pt_scene_add_rect(pt_scene *scene, int axis, float k, float d0min,
 pt_status pt_scene_set_env(pt_scene *scene, const float *rgb, uint32_t w,
                            uint32_t h, float scale);
 pt_status pt_scene_set_camera(pt_scene *scene, const pt_camera_desc *cam);
+/* HomogeneousMedium(sigma_a, sigma_s, sigma_scale, Le, Le_scale, g) as
+ * SceneInfo::global_medium (medium/homogeneous.hpp:31-68,
+ * renderManager.hpp:48): the scene-filling medium PT_EST_VOLUME renders.
+ * sigma_a == NULL removes it.  Le is kept but, as in the reference's
+ * integrator, not rendered.                                                */
+pt_status pt_scene_set_medium(pt_scene *scene, const float sigma_a[3], const float sigma_s[3],
+                              float sigma_scale, const float le[3], float le_scale, float g);
 
 typedef struct pt_build_cfg {
   uint32_t leaf_size;   /* max primitives per BVH leaf (0 = 4)                */

@@ -702,8 +702,28 @@ class EnvironmentLight {
 // render/renderManager.hpp:35-158.  The render settings the reference takes
 // from config.hpp (MAX_RAY_DEPTH 10, 2x2 light samples, GAMMA 2.0, the
 // maxdepth_nee integrator, the Halton sampler) are fields here.
+// Medium / HomogeneousMedium(sigma_a, sigma_s, sigma_scale, Le, Le_scale, g)
+// (medium/homogeneous.hpp:31-68): the scene-filling medium the volume
+// estimator (PT_EST_VOLUME, the reference's USE_MAXDEPTH_VOLUME) renders.
+class Medium {
+ public:
+  virtual ~Medium() = default;
+  gl::vec3 sigma_a, sigma_s, Le;
+  float g = 0.f;
+};
+class HomogeneousMedium : public Medium {
+ public:
+  HomogeneousMedium(gl::vec3 sa, gl::vec3 ss, float sigma_scale, gl::vec3 le, float le_scale, float g_) {
+    sigma_a = sa * sigma_scale;
+    sigma_s = ss * sigma_scale;
+    Le = le * le_scale;
+    g = g_;
+  }
+};
+
 struct SceneInfo {
   std::shared_ptr<PerspectiveCamera> camera = nullptr;
+  std::shared_ptr<Medium> global_medium = nullptr;  // renderManager.hpp:48
   ObjectList objects;
   std::shared_ptr<EnvironmentLight> environment_light = nullptr;
   gl::vec3 bg_color = gl::vec3(0.7f, 0.8f, 1.0f);
@@ -761,6 +781,12 @@ struct SceneInfo {
                      "pt_scene_set_env");
       pt_camera_desc cd = camera->desc();
       ptamd::check(pt_scene_set_camera(s, &cd), "pt_scene_set_camera");
+      if (global_medium) {
+        const float sa[3] = {global_medium->sigma_a[0], global_medium->sigma_a[1], global_medium->sigma_a[2]};
+        const float ss[3] = {global_medium->sigma_s[0], global_medium->sigma_s[1], global_medium->sigma_s[2]};
+        const float le[3] = {global_medium->Le[0], global_medium->Le[1], global_medium->Le[2]};
+        ptamd::check(pt_scene_set_medium(s, sa, ss, 1.f, le, 1.f, global_medium->g), "pt_scene_set_medium");
+      }
       ptamd::check(pt_scene_commit(s, nullptr), "pt_scene_commit");
     } catch (...) {
       pt_scene_destroy(s);

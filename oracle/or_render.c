@@ -401,7 +401,90 @@ static v3 trace_path_reservoir(const rctx *c, uint32_t i, uint32_t j, uint32_t k
 }
 
 /* One path sample.  Returns the radiance. */
+/* HenyeyGreenstein (medium/phaseMath.hpp:7-15): (1 - g^2) / (4 pi (1 + g^2 +
+ * 2 g cos)^1.5), the PBRT (wo, wi) sign convention; INV4PI is 1 / (4 M_PI)
+ * rounded to float. */
+static float hg_phase(float cos_theta, float g) {
+  float upper = 1.f - g * g;
+  float lower = 1.f + g * g + 2.f * g * cos_theta;
+  lower *= sqrtf(fmaxf(0.f, lower));
+  float inv4pi = (float)(1.0 / (4.0 * M_PI));
+  return upper * inv4pi / lower;
+}
+
+/* One path sample of getRayColor in integrator/maxdepth_volume.hpp:72-118,
+ * single scattering in the scene's global HomogeneousMedium, with
+ * estimate_l_scatter1 (:10-70):
+ *   - no medium: 0;
+ *   - the camera ray's closest hit (HitRecord::t = FLT_MAX on a miss);
+ *   - a free-flight distance t = -log(1 - u) / max(sigma_t), u the sample's
+ *     first dimension (halton_sampler.get1D());
+ *   - t before the hit: (sigma_s / sigma_t) * L_s1 / max(sigma_t), where L_s1
+ *     is the mean over LIGHT_SAMPLE_NUM light samples: a uniformly chosen
+ *     light, a point p' on it, the shadow ray from p' to the scatter point
+ *     (0.001, dist - 0.001), Le = L_emit towards the scatter point (one-sided),
+ *     |n' . dir| * HG(-d . -dir) * exp(-sigma_t dist) / pdf / dist^2 with pdf =
+ *     1 / (area N); samples closer than gl::epsilon = 1e-3 are skipped;
+ *   - otherwise the surface's emission (Material::emit), else 0.
+ * The reference divides by max(sigma_t) once more than the free-flight pdf
+ * asks for; that is kept. Light choice and points use the counter stream of
+ * vertex 0 (D-1). */
+static v3 trace_path_volume(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t *cnt) {
+  const or_scene *s = c->s;
+  const pt_render_cfg *cfg = c->cfg;
+  uint32_t g = 1u + k + c->spp * (j + cfg->height * i);
+  uint32_t pixel = j * cfg->width + i;
+  v3 ro, rd;
+  camera_ray(c, i, j, k, g, &ro, &rd);
+  if (cnt) cnt[0]++;
+  if (!s->has_medium) return vs(0.f);
+  or_hit h;
+  if (cnt) cnt[1]++;
+  int hit = or_scene_intersect(s, ro, rd, 1e-4f, 1e5f, 0, &h, cnt ? cnt + 4 : NULL);
+  float th = hit ? h.t : FLT_MAX;
+  v3 st = vadd(s->med_sa, s->med_ss);
+  float smax = vmaxc(st);
+  float u = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 0);
+  float tf = -logf(1.f - u) / smax;
+  if (tf < th) {
+    v3 ps = vadd(ro, vscale(rd, tf));
+    v3 wo = vneg(vnorm(rd));
+    const int N = s->n_lights;
+    const uint32_t ns = cfg->light_samples < 4u ? cfg->light_samples : 4u;
+    v3 alb = V3(s->med_ss.x / st.x, s->med_ss.y / st.y, s->med_ss.z / st.z);
+    v3 acc = vs(0.f);
+    for (uint32_t si = 0; N > 0 && si < ns; ++si) {
+      int li = (int)(or_rng(g, slot(0, SLOT_NEE_LIGHT + si), cfg->seed) * (float)N) % N;
+      const or_light *Lg = &s->lights[li];
+      v3 nl;
+      v3 lp = or_light_point(s, Lg, or_rng(g, slot(0, SLOT_NEE_OFF + 2 * si), cfg->seed),
+                             or_rng(g, slot(0, SLOT_NEE_OFF + 2 * si + 1), cfg->seed), &nl);
+      v3 dv = vsub(ps, lp);
+      v3 dl = vnorm(dv);
+      float dist = vlen(dv);
+      if (cnt) cnt[2]++;
+      int vis = !or_scene_intersect(s, lp, dl, 0.001f, dist - 0.001f, 1, NULL, NULL);
+      if (!vis || dist < 1e-3f) continue;
+      v3 le = (vdot(nl, dl) > 0.f) ? Lg->le : vs(0.f);
+      float pdf = 1.f / (Lg->area * (float)N);
+      float cosl = fabsf(vdot(nl, dl));
+      float ph = hg_phase(vdot(wo, vneg(dl)), s->med_g);
+      v3 T = V3(expf(-st.x * dist), expf(-st.y * dist), expf(-st.z * dist));
+      v3 cc = vdivs(vdivs(vmul(vscale(le, cosl * ph), T), pdf), dist * dist);
+      acc = vadd(acc, cc);
+    }
+    if (ns > 0) acc = vdivs(acc, (float)ns);
+    return vdivs(vmul(alb, acc), smax);
+  }
+  if (hit) {
+    const pt_material_desc *m = &s->mats[h.mat];
+    if (m->kind == PT_MAT_EMITTER) return or_emit(m, rd, &h);
+  }
+  return vs(0.f);
+}
+
 static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t *cnt) {
+  if (c->cfg->estimator == PT_EST_VOLUME) return trace_path_volume(c, i, j, k, cnt);
   if (c->cfg->estimator == PT_EST_NAIVE || c->cfg->estimator == PT_EST_ROULETTE)
     return trace_path_naive(c, i, j, k, c->cfg->estimator == PT_EST_ROULETTE, cnt);
   if (c->cfg->estimator == PT_EST_RESERVOIR) return trace_path_reservoir(c, i, j, k, cnt);

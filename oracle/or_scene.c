@@ -130,6 +130,19 @@ int or_scene_set_env(or_scene *s, const float *rgb, uint32_t w, uint32_t h, floa
   return 0;
 }
 
+int or_scene_set_medium(or_scene *s, const float *sigma_a, const float *sigma_s, float sigma_scale, float g) {
+  if (!sigma_a) {
+    s->has_medium = 0;
+    return 0;
+  }
+  /* HomogeneousMedium::HomogeneousMedium (homogeneous.hpp:41-47) */
+  s->med_sa = V3(sigma_a[0] * sigma_scale, sigma_a[1] * sigma_scale, sigma_a[2] * sigma_scale);
+  s->med_ss = V3(sigma_s[0] * sigma_scale, sigma_s[1] * sigma_scale, sigma_s[2] * sigma_scale);
+  s->med_g = g;
+  s->has_medium = 1;
+  return 0;
+}
+
 int or_scene_set_camera(or_scene *s, const pt_camera_desc *c) {
   s->cam = *c;
   s->has_cam = 1;

@@ -58,6 +58,10 @@ struct or_scene {
   float env_integral;
   pt_camera_desc cam;
   int has_cam;
+  /* SceneInfo::global_medium: HomogeneousMedium (homogeneous.hpp:31-68) */
+  v3 med_sa, med_ss;
+  float med_g;
+  int has_medium;
   v3 cam_h, cam_v, cam_horiz, cam_vert, cam_bl_rel;
   int committed;
 };

@@ -92,6 +92,8 @@ int or_scene_add_rect(or_scene *s, int axis, float k, float d0min, float d0max, 
                       float d1max, uint32_t mat, uint32_t flags);
 int or_scene_set_env(or_scene *s, const float *rgb, uint32_t w, uint32_t h, float scale);
 int or_scene_set_camera(or_scene *s, const pt_camera_desc *c);
+/* HomogeneousMedium(sigma_a, sigma_s, sigma_scale, Le, Le_scale, g); sigma_a NULL removes it. */
+int or_scene_set_medium(or_scene *s, const float *sigma_a, const float *sigma_s, float sigma_scale, float g);
 int or_scene_commit(or_scene *s);
 int or_scene_num_lights(const or_scene *s);
 int or_surface_batch(const or_scene *s, const float *rays, size_t n, float *nrm, int32_t *inside);

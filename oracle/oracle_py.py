@@ -48,6 +48,7 @@ def _load():
         "or_scene_add_rect": (ctypes.c_int, [_P, ctypes.c_int, f32, f32, f32, f32, f32, u32, u32]),
         "or_scene_set_env": (ctypes.c_int, [_P, _P, u32, u32, f32]),
         "or_scene_set_camera": (ctypes.c_int, [_P, _P]),
+        "or_scene_set_medium": (ctypes.c_int, [_P, _P, _P, f32, f32]),
         "or_scene_commit": (ctypes.c_int, [_P]),
         "or_scene_num_lights": (ctypes.c_int, [_P]),
         "or_intersect_batch": (ctypes.c_int, [_P, _P, sz, ctypes.c_int, _P, _P]),
@@ -118,6 +119,10 @@ class OracleScene:
             lib.or_scene_set_env(self.h, ptr(rgb), rgb.shape[1], rgb.shape[0], scale)
         self.cam = camera_desc(**spec.camera)
         lib.or_scene_set_camera(self.h, sref(self.cam))
+        if getattr(spec, "medium", None) is not None:
+            sa, ss, scale, _le, _les, gph = spec.medium
+            lib.or_scene_set_medium(self.h, ptr(np.asarray(sa, np.float32)), ptr(np.asarray(ss, np.float32)),
+                                    scale, gph)
         lib.or_scene_commit(self.h)
 
     def __del__(self):

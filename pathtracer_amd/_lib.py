@@ -22,7 +22,7 @@ MAT_THINDIELECTRIC, MAT_PHONG, MAT_DISNEY, MAT_HAIR, MAT_KAJIYA_KAY = 5, 6, 7, 8
 SAMPLER_HALTON_OWEN, SAMPLER_HALTON_PERM, SAMPLER_HALTON_NONE = 0, 1, 2
 SAMPLER_SOBOL, SAMPLER_STRATIFIED = 3, 4
 # pt_estimator
-EST_REFERENCE_NEE, EST_NEE_MIS, EST_NAIVE, EST_ROULETTE, EST_RESERVOIR = 0, 1, 2, 3, 4
+EST_REFERENCE_NEE, EST_NEE_MIS, EST_NAIVE, EST_ROULETTE, EST_RESERVOIR, EST_VOLUME = 0, 1, 2, 3, 4, 5
 # render flags
 RENDER_COUNTERS, RENDER_NO_SORT, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
@@ -116,7 +116,7 @@ class GuideStats(ctypes.Structure):
 # every symbol include/pathtracer_amd.h declares
 EXPORTED = [
     "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_mesh", "pt_scene_add_mesh_normals",
-    "pt_scene_add_sphere", "pt_scene_add_rect", "pt_scene_set_env", "pt_scene_set_camera",
+    "pt_scene_add_sphere", "pt_scene_add_rect", "pt_scene_set_env", "pt_scene_set_camera", "pt_scene_set_medium",
     "pt_scene_commit", "pt_scene_get_info", "pt_render", "pt_film_read", "pt_film_read_samples",
     "pt_scene_set_tuning", "pt_autotune",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_cfg", "pt_batch_render",
@@ -149,6 +149,7 @@ def _load():
         "pt_scene_add_rect": (i32, [_P, ctypes.c_int] + [ctypes.c_float] * 5 + [u32, u32]),
         "pt_scene_set_env": (i32, [_P, _P, u32, u32, ctypes.c_float]),
         "pt_scene_set_camera": (i32, [_P, ctypes.POINTER(CameraDesc)]),
+        "pt_scene_set_medium": (i32, [_P, _P, _P, ctypes.c_float, _P, ctypes.c_float, ctypes.c_float]),
         "pt_scene_commit": (i32, [_P, ctypes.POINTER(BuildCfg)]),
         "pt_scene_get_info": (i32, [_P, ctypes.POINTER(SceneInfo)]),
         "pt_render": (i32, [_P, ctypes.POINTER(RenderCfg), ctypes.POINTER(Stats)]),

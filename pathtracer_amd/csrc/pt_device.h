@@ -513,6 +513,14 @@ PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
 // bilinear ImageTexture lookup with index clamp to size-2 taken after the
 // fractions (texture.cpp:111-142).
 // ---------------------------------------------------------------------
+// HenyeyGreenstein (medium/phaseMath.hpp:7-15), PBRT's (wo, wi) sign convention.
+PT_D float hg_phase(float cos_theta, float g) {
+  const float upper = 1.f - g * g;
+  float lower = 1.f + g * g + 2.f * g * cos_theta;
+  lower *= sqrtf(fmaxf(0.f, lower));
+  return upper * 0.0795774715f / lower;  // 1 / (4 pi) rounded to float
+}
+
 PT_D f3 env_texel(const DevScene &s, uint32_t x, uint32_t y) {
   const float *c = s.env + 3 * ((size_t)y * s.env_w + x);
   return mk3(c[0], c[1], c[2]);

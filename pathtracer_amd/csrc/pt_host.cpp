@@ -223,6 +223,35 @@ extern "C" pt_status pt_scene_set_env(pt_scene *s, const float *rgb, uint32_t w,
   return PT_OK;
 }
 
+extern "C" pt_status pt_scene_set_medium(pt_scene *s, const float sigma_a[3], const float sigma_s[3],
+                                         float sigma_scale, const float le[3], float le_scale, float g) {
+  if (!s) return fail(PT_ERR_INVALID, "pt_scene_set_medium: NULL scene");
+  if (!sigma_a) {
+    s->ds.has_medium = 0;
+    return PT_OK;
+  }
+  if (!sigma_s) return fail(PT_ERR_INVALID, "pt_scene_set_medium: sigma_s is NULL");
+  float sa[3], ss[3];
+  for (int c = 0; c < 3; ++c) {
+    sa[c] = sigma_a[c] * sigma_scale;  // HomogeneousMedium::HomogeneousMedium (homogeneous.hpp:41-47)
+    ss[c] = sigma_s[c] * sigma_scale;
+    if (!(sa[c] >= 0.f) || !(ss[c] >= 0.f) || !std::isfinite(sa[c] + ss[c]))
+      return fail(PT_ERR_INVALID, "pt_scene_set_medium: coefficients must be finite and >= 0");
+  }
+  if (!(sa[0] + ss[0] > 0.f || sa[1] + ss[1] > 0.f || sa[2] + ss[2] > 0.f))
+    return fail(PT_ERR_INVALID, "pt_scene_set_medium: sigma_a + sigma_s must not be zero");
+  if (!(g > -1.f && g < 1.f)) return fail(PT_ERR_INVALID, "pt_scene_set_medium: g must be in (-1, 1)");
+  for (int c = 0; c < 3; ++c) {
+    s->ds.med_sa[c] = sa[c];
+    s->ds.med_ss[c] = ss[c];
+  }
+  (void)le;
+  (void)le_scale;
+  s->ds.med_g = g;
+  s->ds.has_medium = 1;
+  return PT_OK;
+}
+
 extern "C" pt_status pt_scene_set_camera(pt_scene *s, const pt_camera_desc *c) {
   if (!s || !c) return fail(PT_ERR_INVALID, "pt_scene_set_camera: NULL argument");
   if (!(c->fov > 0.f) || !(c->aspect > 0.f) || !(c->focal_dist > 0.f))
@@ -947,7 +976,9 @@ static pt_status check_cfg(const char *who, const pt_scene *s, const pt_render_c
   if (!s->has_cam) return fail(PT_ERR_INVALID, w + ": camera is not initialized");  // renderManager.hpp:58-62
   if (cfg->width == 0 || cfg->height == 0 || cfg->spp_x == 0 || cfg->spp_y == 0)
     return fail(PT_ERR_INVALID, w + ": width, height, spp_x, spp_y must be > 0");
-  if (cfg->estimator > PT_EST_RESERVOIR) return fail(PT_ERR_INVALID, w + ": unknown estimator");
+  if (cfg->estimator > PT_EST_VOLUME) return fail(PT_ERR_INVALID, w + ": unknown estimator");
+  if (cfg->estimator == PT_EST_VOLUME && s->ds.has_env)
+    return fail(PT_ERR_UNSUPPORTED, w + ": the volume estimator samples area lights only");
   if (cfg->sampler > PT_SAMPLER_STRATIFIED) return fail(PT_ERR_INVALID, w + ": unknown sampler");
   if (cfg->estimator == PT_EST_REFERENCE_NEE && cfg->light_samples > 4)
     return fail(PT_ERR_UNSUPPORTED, w + ": at most 4 light samples per vertex");
@@ -996,6 +1027,7 @@ static void pixel_offsets(const pt_render_cfg *cfg, uint32_t spp, float *off) {
 // one vertex more: the hit that weights (MIS) or drops (reservoir) the last
 // vertex's direct light.
 static uint32_t n_vertices(const pt_render_cfg *cfg) {
+  if (cfg->estimator == PT_EST_VOLUME) return 1u;  // single scattering: the camera ray only
   return (cfg->estimator == PT_EST_NEE_MIS || cfg->estimator == PT_EST_RESERVOIR) ? cfg->max_depth + 1
                                                                                   : cfg->max_depth;
 }

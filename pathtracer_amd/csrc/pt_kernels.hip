@@ -400,6 +400,52 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHA
             }
           }
         }
+      } else if constexpr (EST == PT_EST_VOLUME) {
+        // getRayColor of maxdepth_volume.hpp:72-118 with estimate_l_scatter1 (:10-70):
+        // single scattering in the global medium; the path ends here
+        if (s.has_medium) {
+          const f3 sa = mk3(s.med_sa[0], s.med_sa[1], s.med_sa[2]);
+          const f3 ss = mk3(s.med_ss[0], s.med_ss[1], s.med_ss[2]);
+          const f3 st = add3(sa, ss);
+          const float smax = fmaxf(st.x, fmaxf(st.y, st.z));
+          const float u = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 0u);
+          const float tf = -logf(1.f - u) / smax;
+          const float th = prim >= 0 ? t : 3.402823466e+38f;  // HitRecord::t = FLT_MAX on a miss
+          if (tf < th) {
+            const f3 ps = add3(o, scl3(dir, tf));
+            const f3 wo = neg3(norm3(dir));
+            const uint32_t ns = min(rp.light_samples, 4u);
+            // albedo * L_s1 / sigma_t: the reference divides by max(sigma_t) once more than
+            // the free-flight pdf asks for
+            const f3 alb = mk3(ss.x / st.x, ss.y / st.y, ss.z / st.z);
+            for (uint32_t si = 0; N > 0 && si < ns; ++si) {
+              const int li = (int)(pt_rng(g, rslot(0u, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
+              const DevLight &Lg = s.lights[li];
+              f3 nl;
+              const f3 lp = light_point(Lg, pt_rng(g, rslot(0u, SLOT_NEE_OFF + 2u * si), rp.seed),
+                                        pt_rng(g, rslot(0u, SLOT_NEE_OFF + 2u * si + 1u), rp.seed), nl);
+              const f3 dv = sub3(ps, lp);
+              const f3 dl = norm3(dv);
+              const float dist = len3(dv);
+              if (dist < 1e-3f) continue;  // gl::epsilon
+              const f3 le = (dot3(nl, dl) > 0.f) ? Lg.le : s3(0.f);
+              const float pdf = 1.f / (Lg.area * (float)N);
+              const float cosl = fabsf(dot3(nl, dl));
+              const float ph = hg_phase(dot3(wo, neg3(dl)), s.med_g);
+              const f3 T = mk3(expf(-st.x * dist), expf(-st.y * dist), expf(-st.z * dist));
+              f3 c = divs3(divs3(mul3(scl3(le, cosl * ph), T), pdf), dist * dist);
+              c = divs3(mul3(alb, divs3(c, (float)ns)), smax);
+              if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c))
+                emit_shadow(w, lp, dl, dist - 0.001f, c, slot);
+            }
+          } else if (prim >= 0) {
+            const SurfHit h = surface(s, prim, o, dir, t);
+            const pt_material_desc &m = s.mats[h.mat];
+            if (m.kind == PT_MAT_EMITTER) L = add3(L, bsdf_emit(m, dir, h.sp));
+          }
+        }
+        cont = false;
+        (void)prev_spec;
       } else if constexpr (EST == PT_EST_NAIVE || EST == PT_EST_ROULETTE) {
         // maxdepth_naive.hpp:9-75 / roulette_naive.hpp:9-78: no NEE; misses and
         // the naive depth limit return bg_color (the environment is not consulted)
@@ -911,6 +957,7 @@ static void launch_shade_b(hipStream_t st, const DevScene &s, const RenderParams
     case PT_EST_NAIVE: launch_shade_est<PT_EST_NAIVE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
     case PT_EST_ROULETTE: launch_shade_est<PT_EST_ROULETTE, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
     case PT_EST_RESERVOIR: launch_shade_est<PT_EST_RESERVOIR, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
+    case PT_EST_VOLUME: launch_shade_est<PT_EST_VOLUME, BATCH>(st, s, rp, w, qi, d, g, b, mat_kinds); break;
     case PT_EST_GUIDED:
       if constexpr (!BATCH) launch_shade_est<PT_EST_GUIDED, false>(st, s, rp, w, qi, d, g, b, mat_kinds);
       break;

@@ -28,7 +28,7 @@ namespace pt {
 enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
 // NEE+MIS with the learned guide as a second directional technique (only
 // through pt_render_guided, so not in the public estimator list).
-enum { PT_EST_GUIDED = 5 };
+enum { PT_EST_GUIDED = 6 };
 enum { LIGHT_QUAD = 1, LIGHT_SPHERE = 2, LIGHT_ENV = 3 };
 
 // A child reference: >= 0 internal node index; < 0 leaf ~(start<<4 | count-1);
@@ -69,6 +69,9 @@ struct DevScene {
   const float *env_func, *env_cdf, *env_mcdf;
   float env_integral;
   int has_env;
+  // the global homogeneous medium (PT_EST_VOLUME)
+  float med_sa[3], med_ss[3], med_g;
+  int has_medium;
   // Halton permutation tables
   const uint16_t *perm;
   const uint32_t *perm_off;

@@ -47,6 +47,11 @@ class Scene:
             L.check(L.lib.pt_scene_set_env(h, L.ptr(rgb), rgb.shape[1], rgb.shape[0], scale), "pt_scene_set_env")
         cam = camera_desc(**spec.camera)
         L.check(L.lib.pt_scene_set_camera(h, ctypes.byref(cam)), "pt_scene_set_camera")
+        if spec.medium is not None:
+            sa, ss, scale, le, les, gph = spec.medium
+            f3 = lambda v: np.ascontiguousarray(v, np.float32)
+            L.check(L.lib.pt_scene_set_medium(h, L.ptr(f3(sa)), L.ptr(f3(ss)), scale, L.ptr(f3(le)), les, gph),
+                    "pt_scene_set_medium")
         b = L.BuildCfg()
         b.leaf_size, b.sah_bins = leaf_size, sah_bins
         L.check(L.lib.pt_scene_commit(h, ctypes.byref(b)), "pt_scene_commit")

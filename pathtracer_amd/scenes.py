@@ -38,6 +38,8 @@ class SceneSpec:
     render: dict = dataclasses.field(default_factory=dict)
     order: list = dataclasses.field(default_factory=list)        # insertion order of ("mesh"|"sphere"|"rect", index)
     mesh_normals: dict = dataclasses.field(default_factory=dict)  # mesh index -> (normals f32 (k,3), nidx i32 (m,3))
+    # global HomogeneousMedium: (sigma_a(3), sigma_s(3), sigma_scale, Le(3), Le_scale, g) or None
+    medium: tuple | None = None
     name: str = ""
 
     def add_material(self, kind, params) -> int:
@@ -444,3 +446,33 @@ def guiding_room(width=1024, height=1024, spp_x=16, spp_y=16, max_depth=16, rr_d
                     sampler=L.SAMPLER_HALTON_OWEN, estimator=estimator, rr_depth=rr_depth, light_samples=4,
                     seed=23, bg=(0, 0, 0))
     return s
+
+
+def _medium_scene(name, sigma_a, sigma_s, g, second_light, width, height, spp_x, spp_y):
+    s = SceneSpec(name=name)
+    m = s.add_material(L.MAT_EMITTER, [0.4, 2.32, 3.2, 1.0])
+    s.add_sphere((0.0, 0.0, 0.0), 1.0, m)
+    if second_light:
+        m2 = s.add_material(L.MAT_EMITTER, [24.0, 10.0, 24.0, 1.0])
+        s.add_sphere((-3.0, 0.0, -1.5), 1.0, m2)
+    s.medium = (sigma_a, sigma_s, 1.0, (0.0, 0.0, 0.0), 1.0, g)
+    s.camera = dict(fov_deg=45.0, aspect=width / height, focal_dist=10.0, aperture=1000.0, up=(0, 1, 0),
+                    front=(0, 0, 1), pos=(0, 0, -3))
+    s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=1,
+                    sampler=L.SAMPLER_HALTON_OWEN, estimator=L.EST_VOLUME, rr_depth=0, light_samples=4, seed=3,
+                    bg=(0, 0, 0))
+    return s
+
+
+def absorption_only_medium(width=128, height=128, spp_x=4, spp_y=4) -> SceneSpec:
+    """absorption_only_medium (scenes/scenes_medium.hpp:6-31): an emissive
+    sphere in a purely absorbing medium (sigma_a 1.5)."""
+    return _medium_scene("absorption_only_medium", (1.5, 1.5, 1.5), (0.0, 0.0, 0.0), 0.0, False,
+                         width, height, spp_x, spp_y)
+
+
+def single_scatter_medium(width=128, height=128, spp_x=4, spp_y=4, g=0.0) -> SceneSpec:
+    """single_scatter_medium (scenes/scenes_medium.hpp:33-61): two emissive
+    spheres in a scattering medium (sigma_a 0.1, sigma_s 0.7, HG g)."""
+    return _medium_scene("single_scatter_medium", (0.1, 0.1, 0.1), (0.7, 0.7, 0.7), g, True,
+                         width, height, spp_x, spp_y)

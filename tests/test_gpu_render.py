@@ -126,3 +126,16 @@ def test_cornell_image_statistics(gpu):
     z = (gimg - oimg) / se
     assert np.abs(z).mean() < 1.0
     assert (np.abs(z) > 5).mean() < 0.01
+
+
+@pytest.mark.parametrize("which", ["absorption", "scatter", "scatter_hg"])
+def test_volume_per_sample(gpu, which):
+    """maxdepth_volume (single scattering in a homogeneous medium) per sample."""
+    from pathtracer_amd import scenes
+    if which == "absorption":
+        spec = scenes.absorption_only_medium(width=40, height=40, spp_x=2, spp_y=2)
+    else:
+        spec = scenes.single_scatter_medium(width=40, height=40, spp_x=2, spp_y=2, g=0.6 if which == "scatter_hg" else 0.0)
+    _, gs, _, os_, st = _per_sample(spec)
+    assert _sample_agreement(gs, os_) > 0.995
+    assert st["nan_samples"] == 0

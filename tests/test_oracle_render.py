@@ -191,3 +191,101 @@ def test_reservoir_closed_form():
     expect = (1 - F) * np.sqrt(3.0) * e_c2
     se = np.sqrt(var[0, 0, 0] / (n * n))
     assert abs(img[0, 0, 0] - expect) < max(4 * se, 3e-3 * expect), (img[0, 0, 0], expect, se)
+
+
+# ---------------------------------------------------------------- maxdepth_volume
+
+def test_volume_absorption_only_matches_transmittance():
+    """absorption_only_medium (scenes_medium.hpp:6-31): with sigma_s = 0 a sample
+    returns the sphere's emission when the free flight passes the hit, else 0,
+    so its expectation is Le exp(-sigma_t t_hit) for its own camera ray."""
+    from pathtracer_amd import scenes
+    spec = scenes.absorption_only_medium(width=48, height=48, spp_x=8, spp_y=8)
+    o = O.OracleScene(spec)
+    cfg = o.cfg()
+    img, _, smp, _ = o.render(cfg, want_samples=True)
+    spp = 64
+    pix = np.repeat(np.arange(48 * 48, dtype=np.uint32), spp)
+    kk = np.tile(np.arange(spp, dtype=np.uint32), 48 * 48)
+    rays = o.camera_rays(cfg, pix, kk)
+    prim, t = o.intersect(rays)
+    le = np.array([0.4, 2.32, 3.2])
+    expect = np.where((prim >= 0)[:, None], le * np.exp(-1.5 * t.astype(np.float64))[:, None], 0.0)
+    expect = expect.reshape(48 * 48, spp, 3).mean(1)
+    hit = (prim.reshape(48 * 48, spp) >= 0).all(1)
+    got = smp.reshape(48 * 48, spp, 3).mean(1)
+    # each sample is Le or 0: binomial noise of the mean over all hit pixels
+    p = (expect[hit] / le).mean(0)
+    n = hit.sum() * spp
+    assert np.all(np.abs((got[hit] / le).mean(0) - p) < 4 * np.sqrt(p * (1 - p) / n))
+    miss = (prim.reshape(48 * 48, spp) < 0).all(1)
+    assert miss.sum() > 100 and np.all(got[miss] == 0)
+
+
+def _sphere_light_quadrature(p, wo, c, r, le, sig_t, g, occluders, n_th=96, n_ph=192):
+    """The single-scattering source term at p from one sphere light, by
+    quadrature over the sphere: sum over the surface of Le (one-sided, the
+    outward normal towards p) |n . dir| HG(wo . wi) exp(-sig_t d) / d^2 dA,
+    with the segment tested against the occluder spheres analytically."""
+    th = (np.arange(n_th) + 0.5) / n_th * np.pi
+    ph = (np.arange(n_ph) + 0.5) / n_ph * 2 * np.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    nrm = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    dA = (r * r * np.sin(T) * (np.pi / n_th) * (2 * np.pi / n_ph)).reshape(-1)
+    x = c + r * nrm
+    dv = p - x
+    d = np.linalg.norm(dv, axis=1)
+    dirv = dv / d[:, None]
+    front = (nrm * dirv).sum(1) > 0
+    wi = -dirv
+    cosg = (wo[None, :] * wi).sum(1)
+    hg = (1 - g * g) / (4 * np.pi * (1 + g * g + 2 * g * cosg) ** 1.5)
+    vis = np.ones(len(d), bool)
+    for (oc, orad) in occluders:
+        # segment x -> p against sphere (oc, orad)
+        m = x - oc
+        b = (m * dirv).sum(1)
+        cc = (m * m).sum(1) - orad * orad
+        disc = b * b - cc
+        sq = np.sqrt(np.maximum(disc, 0))
+        t1 = -b - sq
+        t2 = -b + sq
+        hitseg = (disc > 0) & (((t1 > 1e-3) & (t1 < d - 1e-3)) | ((t2 > 1e-3) & (t2 < d - 1e-3)))
+        vis &= ~hitseg
+    w = np.where(front & vis, (nrm * dirv).sum(1) * hg * np.exp(-sig_t * d) / (d * d) * dA, 0.0)
+    return le[None, :] * w.sum()
+
+
+@pytest.mark.parametrize("g", [0.0, 0.6])
+def test_volume_single_scatter_matches_quadrature(g):
+    """A thin camera cone (one pixel, fov 0.02 deg) through single_scatter_medium,
+    missing both spheres: the estimator's expectation along the ray is
+    (sigma_s / sigma_t) * integral of exp(-sigma_t t) L_s1(r(t)) dt, where L_s1
+    sums the sphere lights' quadrature; the reference's estimator divides by
+    sigma_t once more than the free-flight pdf calls for, so this is its
+    expectation (not the physical sigma_s times the integral)."""
+    from pathtracer_amd import scenes
+    spec = scenes.single_scatter_medium(width=1, height=1, spp_x=256, spp_y=256, g=g)
+    d0 = np.array([0.35, 0.55, 1.0])
+    d0 /= np.linalg.norm(d0)
+    spec.camera = dict(fov_deg=0.02, aspect=1.0, focal_dist=1.0, aperture=0.0, up=(0, 1, 0), front=tuple(d0),
+                       pos=(0, 0, -3))
+    o = O.OracleScene(spec)
+    img, var, _, _ = o.render(o.cfg(), want_var=True)
+    sig_t, sig_s = 0.8, 0.7
+    lights = [((0.0, 0.0, 0.0), 1.0, np.array([0.4, 2.32, 3.2])), ((-3.0, 0.0, -1.5), 1.0, np.array([24.0, 10.0, 24.0]))]
+    cam = np.array([0.0, 0.0, -3.0])
+    ts = (np.arange(300) + 0.5) / 300 * 24.0   # exp(-0.8 * 24) is negligible
+    dt = 24.0 / 300
+    total = np.zeros(3)
+    for t in ts:
+        p = cam + t * d0
+        ls = np.zeros(3)
+        for li, (c, r, le) in enumerate(lights):
+            occ = [(np.array(oc), orad) for j, (oc, orad, _) in enumerate(lights) if j != li]
+            ls += _sphere_light_quadrature(p, -d0, np.array(c), r, le, sig_t, g, occ)[0]
+        total += np.exp(-sig_t * t) * ls * dt
+    expect = sig_s / sig_t * total
+    n = 256 * 256
+    err = np.abs(img[0, 0] - expect)
+    assert np.all(err < 5 * np.sqrt(var[0, 0] / n) + 5e-3 * expect), (img[0, 0], expect, np.sqrt(var[0, 0] / n))
