@@ -203,6 +203,17 @@ pt_status pt_scene_add_sphere(pt_scene *scene, const float center[3],
 pt_status pt_scene_add_rect(pt_scene *scene, int axis, float k, float d0min,
                             float d0max, float d1min, float d1max,
                             uint32_t material, uint32_t flags);
+/* Curve(cps, width0, width1, type, material)  (primitives/curve.hpp:13-140):
+ * a cubic Bezier (cps: 4 control points, 12 floats) hit as 8 cylinder
+ * segments, radius the mean of the widths interpolated at each segment's
+ * ends.  The hit sets the hair tangent (HitRecord::hair_tangent) that
+ * HairMarschner::scatter frames on.  Every type intersects as a cylinder, as
+ * in the reference.  Curves are never lights.  Their primitive ids follow the
+ * rects', 8 per curve in insertion order.                                   */
+enum { PT_CURVE_FLAT = 0, PT_CURVE_CYLINDER = 1, PT_CURVE_RIBBON = 2 };
+pt_status pt_scene_add_curve(pt_scene *scene, const float cps[12], float width0,
+                             float width1, uint32_t type, uint32_t material,
+                             uint32_t flags);
 /* EnvironmentLight(hdri, scale)  (light/envLight.hpp:4-92).  rgb holds
  * w*h texels, texel (x,y) at rgb[(y*w+x)*3], y=0 being the direction -Y
  * (the reference stores texels[x][y] after a vertical flip, texture.cpp:39-42). */
@@ -407,7 +418,8 @@ pt_status pt_sample(uint32_t sampler, uint32_t seed, uint32_t spp,
 
 /* BVHNode::intersect (primitives/bvh.hpp:27-41) over the committed scene.
  * rays: n*8 floats (ox,oy,oz,dx,dy,dz,tmin,tmax).  prim: primitive id
- * (triangles first in insertion order, then spheres, then rects) or -1.   */
+ * (triangles first in insertion order, then spheres, then rects, then 8
+ * segments per curve) or -1.                                               */
 pt_status pt_intersect(pt_scene *scene, const float *rays, size_t n,
                        int any_hit, int32_t *prim, float *t);
 

@@ -118,6 +118,10 @@ struct SceneSink {
   virtual void sphere(const float c[3], float r, uint32_t mat, uint32_t flags) = 0;
   virtual void rect(int axis, float k, float d0min, float d0max, float d1min, float d1max, uint32_t mat,
                     uint32_t flags) = 0;
+  virtual void curve(const float cps[12], float w0, float w1, uint32_t type, uint32_t mat, uint32_t flags) {
+    (void)cps, (void)w0, (void)w1, (void)type, (void)mat, (void)flags;
+    throw std::runtime_error("this scene sink does not take curves");
+  }
 };
 
 inline void check(pt_status st, const char *what) {
@@ -148,6 +152,9 @@ struct AbiSink : SceneSink {
   void rect(int axis, float k, float d0min, float d0max, float d1min, float d1max, uint32_t mat,
             uint32_t flags) override {
     check(pt_scene_add_rect(s, axis, k, d0min, d0max, d1min, d1max, mat, flags), "pt_scene_add_rect");
+  }
+  void curve(const float cps[12], float w0, float w1, uint32_t type, uint32_t mat, uint32_t flags) override {
+    check(pt_scene_add_curve(s, cps, w0, w1, type, mat, flags), "pt_scene_add_curve");
   }
 };
 
@@ -393,6 +400,33 @@ class Sphere : public Hittable {
   }
   gl::vec3 center;
   float radius;
+  std::shared_ptr<Material> material;
+};
+
+// Curve (primitives/curve.hpp:6-140): a cubic Bezier hit as 8 cylinder
+// segments; every type intersects as a cylinder, as in the reference.
+enum class CurveType { Flat, Cylinder, Ribbon };
+class Curve : public Hittable {
+ public:
+  Curve(const std::array<gl::vec3, 4> &cps, float width0, float width1, CurveType type = CurveType::Flat,
+        std::shared_ptr<Material> material = nullptr)
+      : cps(cps), width0(width0), width1(width1), type(type), material(std::move(material)) {}
+  std::shared_ptr<Material> get_material() const override { return material; }
+  void lower(ptamd::Lowering &L, const ptamd::Xform &X, const ptamd::Chain &C, bool light_ok) const override {
+    (void)light_ok;  // curves are never lights ("for now, we don't treat them as emitters", curve.hpp:133)
+    if (!(X.scale > 0.f)) throw std::runtime_error("Scale must be positive");
+    float p[12];
+    for (int i = 0; i < 4; ++i) {
+      gl::vec3 q = X.point(cps[i]);
+      p[3 * i] = q[0];
+      p[3 * i + 1] = q[1];
+      p[3 * i + 2] = q[2];
+    }
+    L.sink.curve(p, width0 * X.scale, width1 * X.scale, (uint32_t)type, L.mat(material), C.flags());
+  }
+  std::array<gl::vec3, 4> cps;
+  float width0, width1;
+  CurveType type;
   std::shared_ptr<Material> material;
 };
 

@@ -584,7 +584,11 @@ int or_hair_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, float 
                     float u1, or_srec *s) {
   hair_t H = hair_params(m);
   v3 wo = vneg(vnorm(ray_d));
-  onb_t b = onb_make(h->n);
+  /* HairMarschner::scatter (hairMarsch.cpp:4-28): the sample and pdf_val frame
+   * on hair_tangent, the normal when it is unset (near_zero); attenuation is
+   * f(), which frames on the normal (hairMarsch.hpp:44-54) */
+  const int has_t = h->tangent.x != 0.f || h->tangent.y != 0.f || h->tangent.z != 0.f;
+  onb_t b = onb_make(has_t ? h->tangent : h->n);
   v3 wo_l = onb_to_local(&b, wo);
   float so = wo_l.x, co = safe_sqrt(1 - so * so);
   float phio = atan2f(wo_l.z, wo_l.y);
@@ -611,7 +615,12 @@ int or_hair_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, float 
   float phii = dphi + phio;
   v3 wi_l = V3(si, ci * cosf(phii), ci * sinf(phii));
   v3 wi = vnorm(onb_to_world(&b, wi_l));
-  s->att = hair_eval(&H, onb_to_local(&b, wi), onb_to_local(&b, wo));
+  if (has_t) {
+    onb_t bn = onb_make(h->n);
+    s->att = hair_eval(&H, onb_to_local(&bn, wi), onb_to_local(&bn, wo));
+  } else {
+    s->att = hair_eval(&H, onb_to_local(&b, wi), onb_to_local(&b, wo));
+  }
   s->pdf = hair_pdf_local(&H, wo_l, onb_to_local(&b, wi));
   s->wi = wi;
   s->flags = OR_GLOSSY | OR_REFL;

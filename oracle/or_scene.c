@@ -38,6 +38,7 @@ void or_scene_destroy(or_scene *s) {
   free(s->meshes);
   free(s->spheres);
   free(s->rects);
+  free(s->curves);
   free(s->lights);
   free(s->env);
   free(s->env_func);
@@ -117,6 +118,34 @@ int or_scene_add_rect(or_scene *s, int axis, float k, float d0min, float d0max, 
   r->seq = s->seq++;
   r->light = -1;
   return s->n_rects++;
+}
+
+/* gl::evalBezier / evalBezierDerivative (primitives/bezier.hpp:12-35) */
+static v3 bezier_at(const v3 *c, float u) {
+  float u1 = 1.0f - u;
+  float b0 = u1 * u1 * u1, b1 = 3.0f * u * u1 * u1, b2 = 3.0f * u * u * u1, b3 = u * u * u;
+  return vadd(vadd(vadd(vscale(c[0], b0), vscale(c[1], b1)), vscale(c[2], b2)), vscale(c[3], b3));
+}
+static v3 bezier_deriv(const v3 *c, float u) {
+  v3 d0 = vscale(vsub(c[1], c[0]), 3.0f), d1 = vscale(vsub(c[2], c[1]), 3.0f), d2 = vscale(vsub(c[3], c[2]), 3.0f);
+  float u1 = 1.0f - u;
+  return vadd(vadd(vscale(d0, u1 * u1), vscale(d1, 2.0f * u * u1)), vscale(d2, u * u));
+}
+
+int or_scene_add_curve(or_scene *s, const float *cps, float w0, float w1, uint32_t mat, uint32_t flags) {
+  GROW(s->curves, s->n_curves, s->cap_curves, or_curve);
+  or_curve *c = &s->curves[s->n_curves];
+  for (int i = 0; i < 4; ++i) c->cps[i] = V3(cps[3 * i], cps[3 * i + 1], cps[3 * i + 2]);
+  c->w0 = w0;
+  c->w1 = w1;
+  for (int i = 0; i <= 8; ++i) {  /* Curve::intersect (curve.hpp:60-66) */
+    float u = (float)i / 8.0f;
+    c->P[i] = bezier_at(c->cps, u);
+    c->dP[i] = bezier_deriv(c->cps, u);
+  }
+  c->mat = mat;
+  c->flags = flags & (PT_PRIM_FLIP_FACE | PT_PRIM_FORCE_INSIDE);
+  return s->n_curves++;
 }
 
 int or_scene_set_env(or_scene *s, const float *rgb, uint32_t w, uint32_t h, float scale) {
@@ -448,6 +477,52 @@ static int rect_hit(const or_rect *r, v3 o, v3 d, float tmin, float tmax, float 
 /* Closest hit over the scene: every object in turn with a shrinking bound
  * (the reference's BVHNode::intersect, bvh.hpp:27-41, visits objects in a
  * different order; closest-hit results agree except for exact ties). */
+/* Curve::intersect (primitives/curve.hpp:47-130): 8 cylinder segments, the
+ * closest root whose projection lies on its segment; solveQuadratic
+ * (utils/utility.hpp:112-124).  Returns the segment and its hit. */
+static int curve_hit(const or_curve *c, v3 o, v3 d, float tmin, float tmax, float *tout, int *seg_out,
+                     v3 *n_out, v3 *tan_out, v3 *p_out) {
+  float best = tmax;
+  int found = 0;
+  for (int i = 0; i < 8; ++i) {
+    v3 pa = c->P[i], pb = c->P[i + 1];
+    float u0 = (float)i / 8.0f, u1 = (float)(i + 1) / 8.0f;
+    float r0 = c->w0 + (c->w1 - c->w0) * u0, r1 = c->w0 + (c->w1 - c->w0) * u1;
+    float radius = 0.5f * (r0 + r1);
+    v3 axis = vnorm(vsub(pb, pa));
+    float len = vlen(vsub(pb, pa));
+    v3 dp = vsub(d, vscale(axis, vdot(d, axis)));
+    v3 de = vsub(o, pa);
+    v3 op = vsub(de, vscale(axis, vdot(de, axis)));
+    float A = vdot(dp, dp), B = 2.0f * vdot(dp, op), C = vdot(op, op) - radius * radius;
+    float disc = B * B - 4.f * A * C;
+    if (disc < 0.f) continue;
+    float sq = sqrtf(disc);
+    float q = B < 0.f ? -0.5f * (B - sq) : -0.5f * (B + sq);
+    float roots[2] = {q / A, C / q};
+    if (roots[0] > roots[1]) { float x = roots[0]; roots[0] = roots[1]; roots[1] = x; }
+    int nr = roots[0] == roots[1] ? 1 : 2;
+    for (int r = 0; r < nr; ++r) {
+      float t = roots[r];
+      if (t < tmin || t > best) continue;
+      v3 hp = vadd(o, vscale(d, t));
+      float proj = vdot(vsub(hp, pa), axis);
+      if (proj < 0.f || proj > len) continue;
+      best = t;
+      found = 1;
+      *seg_out = i;
+      *p_out = hp;
+      float uu = proj / len;
+      v3 tg = vnorm(vadd(c->dP[i], vscale(vsub(c->dP[i + 1], c->dP[i]), uu)));
+      v3 v = vsub(hp, vadd(pa, vscale(axis, proj)));
+      *tan_out = tg;
+      *n_out = vnorm(vsub(v, vscale(tg, vdot(v, tg))));
+    }
+  }
+  if (found) *tout = best;
+  return found;
+}
+
 int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, int any,
                        or_hit *h, uint64_t *cnt) {
   float best = tmax;
@@ -478,10 +553,32 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
       if (any) return 1;
     }
   }
+  int cseg = 0;
+  v3 cn = vs(0.f), ctan = vs(0.f), cp = vs(0.f);
+  for (int i = 0; i < s->n_curves; ++i) {
+    float t;
+    int sg;
+    v3 nn, tt, pp;
+    if (cnt) cnt[1] += 8;
+    if (curve_hit(&s->curves[i], o, d, tmin, best, &t, &sg, &nn, &tt, &pp)) {
+      best = t; kind = 3; idx = i; cseg = sg; cn = nn; ctan = tt; cp = pp;
+      if (any) return 1;
+    }
+  }
   if (kind < 0) return 0;
   if (!h) return 1;
   h->t = best;
   h->ng_light = vs(0.f);
+  h->tangent = vs(0.f);
+  if (kind == 3) {
+    const or_curve *c = &s->curves[idx];
+    h->p = cp;
+    set_normal(h, d, cn, c->flags);
+    h->tangent = ctan;
+    h->mat = c->mat;
+    h->prim = s->n_tris + s->n_spheres + s->n_rects + 8 * idx + cseg;
+    return 1;
+  }
   if (kind == 0) {
     const or_tri *tr = &s->tris[idx];
     v3 e1 = vsub(tr->p[1], tr->p[0]), e2 = vsub(tr->p[2], tr->p[0]);
@@ -531,6 +628,7 @@ int or_prim_light(const or_scene *s, int prim) {
   prim -= s->n_tris;
   if (prim < s->n_spheres) return s->spheres[prim].light;
   prim -= s->n_spheres;
+  if (prim >= s->n_rects) return -1;  /* curves are never lights */
   return s->rects[prim].light;
 }
 
@@ -591,6 +689,15 @@ int or_brute_intersect_batch(const or_scene *s, const float *rays, size_t n, int
       if (rect_hit(&s->rects[k], o, d, r[6], best, &tt)) {
         best = tt;
         id = s->n_tris + s->n_spheres + k;
+      }
+    }
+    for (int k = 0; k < s->n_curves; ++k) {
+      float tt;
+      int sg;
+      v3 a, b, c;
+      if (curve_hit(&s->curves[k], o, d, r[6], best, &tt, &sg, &a, &b, &c)) {
+        best = tt;
+        id = s->n_tris + s->n_spheres + s->n_rects + 8 * k + sg;
       }
     }
     prim[i] = id;

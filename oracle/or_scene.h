@@ -27,6 +27,14 @@ typedef struct {
   uint32_t mat, flags, seq;
   int light;
 } or_rect;
+/* Curve (primitives/curve.hpp:13-140) with its 9 Bezier points and
+ * derivatives precomputed as Curve::intersect computes them. */
+typedef struct {
+  v3 cps[4];
+  float w0, w1;
+  v3 P[9], dP[9];
+  uint32_t mat, flags;
+} or_curve;
 enum { OR_LIGHT_QUAD = 1, OR_LIGHT_SPHERE = 2, OR_LIGHT_ENV = 3 };
 typedef struct {
   int type;
@@ -48,6 +56,8 @@ struct or_scene {
   int n_spheres, cap_spheres;
   or_rect *rects;
   int n_rects, cap_rects;
+  or_curve *curves;
+  int n_curves, cap_curves;
   or_light *lights;
   int n_lights;
   uint32_t seq;

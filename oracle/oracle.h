@@ -57,6 +57,7 @@ typedef struct {
   float t;
   v3 p, n;      /* n opposes the incoming ray (HitRecord::set_normal) */
   v3 ng_light;  /* un-flipped geometric normal (for one-sided emitters) */
+  v3 tangent;   /* HitRecord::hair_tangent (curves), else 0             */
   int inside;   /* HitRecord::is_inside = front face                   */
   int prim;
   uint32_t mat;
@@ -91,6 +92,7 @@ int or_scene_add_sphere(or_scene *s, const float *c, float r, uint32_t mat, uint
 int or_scene_add_rect(or_scene *s, int axis, float k, float d0min, float d0max, float d1min,
                       float d1max, uint32_t mat, uint32_t flags);
 int or_scene_set_env(or_scene *s, const float *rgb, uint32_t w, uint32_t h, float scale);
+int or_scene_add_curve(or_scene *s, const float *cps, float w0, float w1, uint32_t mat, uint32_t flags);
 int or_scene_set_camera(or_scene *s, const pt_camera_desc *c);
 /* HomogeneousMedium(sigma_a, sigma_s, sigma_scale, Le, Le_scale, g); sigma_a NULL removes it. */
 int or_scene_set_medium(or_scene *s, const float *sigma_a, const float *sigma_s, float sigma_scale, float g);

@@ -49,6 +49,7 @@ def _load():
         "or_scene_set_env": (ctypes.c_int, [_P, _P, u32, u32, f32]),
         "or_scene_set_camera": (ctypes.c_int, [_P, _P]),
         "or_scene_set_medium": (ctypes.c_int, [_P, _P, _P, f32, f32]),
+        "or_scene_add_curve": (ctypes.c_int, [_P, _P, f32, f32, u32, u32]),
         "or_scene_commit": (ctypes.c_int, [_P]),
         "or_scene_num_lights": (ctypes.c_int, [_P]),
         "or_intersect_batch": (ctypes.c_int, [_P, _P, sz, ctypes.c_int, _P, _P]),
@@ -110,6 +111,9 @@ class OracleScene:
                 c, r, mat, flags = spec.spheres[i]
                 ca = np.asarray(c, np.float32)
                 lib.or_scene_add_sphere(self.h, ptr(ca), r, mat, flags)
+            elif kind == "curve":
+                cps, w0, w1, _ctype, mat, flags = spec.curves[i]
+                lib.or_scene_add_curve(self.h, ptr(np.ascontiguousarray(cps, np.float32)), w0, w1, mat, flags)
             else:
                 axis, k, a0, a1, b0, b1, mat, flags = spec.rects[i]
                 lib.or_scene_add_rect(self.h, axis, k, a0, a1, b0, b1, mat, flags)

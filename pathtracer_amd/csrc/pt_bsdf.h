@@ -14,6 +14,7 @@ enum { BX_REFL = 1, BX_TRANS = 2, BX_DIFFUSE = 4, BX_GLOSSY = 8, BX_SPECULAR = 1
 struct ShadePoint {
   f3 n;        // normal opposing the incoming ray (HitRecord::set_normal)
   bool inside; // HitRecord::is_inside (= front face, materialBase.hpp:50)
+  f3 t = {0.f, 0.f, 0.f};  // HitRecord::hair_tangent: curves set it, others leave it 0
 };
 
 struct Scatter {

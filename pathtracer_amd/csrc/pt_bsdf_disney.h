@@ -512,7 +512,11 @@ PT_D float hair_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h
 }
 PT_D bool hair_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float uc, float u0, float u1, Scatter &s) {
   Hair H = hair_params(m);
-  Frame b = make_frame(h.n);
+  // HairMarschner::scatter (hairMarsch.cpp:4-28): the sample and its pdf_val
+  // frame on hair_tangent (the normal when unset); the attenuation is f(),
+  // which frames on the normal (hairMarsch.hpp:44-54)
+  const bool has_t = h.t.x != 0.f || h.t.y != 0.f || h.t.z != 0.f;
+  Frame b = make_frame(has_t ? h.t : h.n);
   f3 wo_l = to_local(b, wo);
   float so = wo_l.x, co = safe_sqrtf(1 - so * so);
   float phio = atan2f(wo_l.z, wo_l.y);
@@ -534,7 +538,12 @@ PT_D bool hair_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float uc
                                : 2.f * PT_PI * uc;
   float phii = dphi + phio;
   f3 wi = norm3(to_world(b, mk3(si, ci * cosf(phii), ci * sinf(phii))));
-  s.att = hair_eval(H, to_local(b, wi), to_local(b, wo));
+  if (has_t) {
+    Frame bn = make_frame(h.n);
+    s.att = hair_eval(H, to_local(bn, wi), to_local(bn, norm3(wo)));
+  } else {
+    s.att = hair_eval(H, to_local(b, wi), to_local(b, wo));
+  }
   s.pdf = hair_pdf_local(H, wo_l, to_local(b, wi));
   s.wi = wi;
   s.flags = BX_GLOSSY | BX_REFL;

@@ -109,6 +109,43 @@ PT_D bool hit_rect(float4 A, float4 B, f3 o, f3 d, float tmin, float tmax, float
   return true;
 }
 
+// One cylinder segment of a Curve (primitives/curve.hpp:47-118): the ray
+// against the infinite cylinder about the segment's axis (solveQuadratic,
+// utility.hpp:112-124), a root kept when it lies within [tmin, tmax] and its
+// projection on the axis within the segment.  Raw direction, as the reference.
+PT_D bool hit_curve(float4 A, float4 B, float4 C, f3 o, f3 d, float tmin, float tmax, float &t) {
+  const f3 pa = mk3(A.x, A.y, A.z), ax = mk3(B.x, B.y, B.z);
+  const float r = B.w, len = C.x;
+  const f3 dp = sub3(d, scl3(ax, dot3(d, ax)));
+  const f3 de = sub3(o, pa);
+  const f3 op = sub3(de, scl3(ax, dot3(de, ax)));
+  const float qa = dot3(dp, dp), qb = 2.f * dot3(dp, op), qc = dot3(op, op) - r * r;
+  const float disc = qb * qb - 4.f * qa * qc;
+  if (disc < 0.f) return false;
+  const float sq = sqrt_rn(disc);
+  const float q = qb < 0.f ? -0.5f * (qb - sq) : -0.5f * (qb + sq);
+  float r0 = div_rn(q, qa), r1 = div_rn(qc, q);
+  if (r0 > r1) {
+    const float x = r0;
+    r0 = r1;
+    r1 = x;
+  }
+  const int nr = (r0 == r1) ? 1 : 2;
+  float best = tmax;
+  bool found = false;
+  for (int k = 0; k < nr; ++k) {
+    const float tt = k == 0 ? r0 : r1;
+    if (tt < tmin || tt > best) continue;
+    const f3 hp = add3(o, scl3(d, tt));
+    const float proj = dot3(sub3(hp, pa), ax);
+    if (proj < 0.f || proj > len) continue;
+    best = tt;
+    found = true;
+  }
+  if (found) t = best;
+  return found;
+}
+
 PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, float tmax, float &t, uint32_t &id) {
   float4 A = prims[3 * p];
   float4 B = prims[3 * p + 1];
@@ -120,6 +157,10 @@ PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, floa
     return hit_tri(A, B, C, o, d, tmin, tmax, t);
   }
   if (type == PRIM_SPHERE) return hit_sphere(A, B, o, d, tmin, tmax, t);
+  if (type == PRIM_CURVE) {
+    float4 C = prims[3 * p + 2];
+    return hit_curve(A, B, C, o, d, tmin, tmax, t);
+  }
   return hit_rect(A, B, o, d, tmin, tmax, t);
 }
 #pragma clang fp contract(fast)
@@ -484,9 +525,23 @@ PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
   float4 g = s.prim_geom[prim];
   uint32_t type = (info.x >> 24) & 3u, fl = info.x >> 28;
   h.mat = info.x & 0xffffffu;
-  h.light = (type == PRIM_TRI) ? -1 : (int)info.y;
+  h.light = (type == PRIM_TRI || type == PRIM_CURVE) ? -1 : (int)info.y;
   f3 n;
-  if (type == PRIM_TRI) {
+  f3 tan = s3(0.f);
+  if (type == PRIM_CURVE) {
+    // Curve::intersect's hit record (curve.hpp:101-129): the tangent from the
+    // Bezier derivatives at the segment's ends, the normal from the axis to the
+    // hit point, perpendicular to the tangent
+    const float4 *cg = s.curve_geom + 4 * (size_t)info.y;
+    const f3 pa = mk3(cg[0].x, cg[0].y, cg[0].z), ax = mk3(cg[1].x, cg[1].y, cg[1].z);
+    const f3 d0 = mk3(cg[2].x, cg[2].y, cg[2].z), d1 = mk3(cg[3].x, cg[3].y, cg[3].z);
+    h.p = add3(o, scl3(d, t));
+    const float proj = dot3(sub3(h.p, pa), ax);
+    const float uu = div_rn(proj, cg[0].w);
+    tan = norm3(add3(d0, scl3(sub3(d1, d0), uu)));
+    const f3 v = sub3(h.p, add3(pa, scl3(ax, proj)));
+    n = norm3(sub3(v, scl3(tan, dot3(v, tan))));
+  } else if (type == PRIM_TRI) {
     h.p = add3(o, scl3(d, t));
     n = mk3(g.x, g.y, g.z);
     h.ng = n;
@@ -505,6 +560,7 @@ PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
   if (fl & PT_PRIM_FORCE_INSIDE) inside = true;
   if (fl & PT_PRIM_FLIP_FACE) inside = !inside;
   h.sp.inside = inside;
+  h.sp.t = tan;
   return h;
 }
 

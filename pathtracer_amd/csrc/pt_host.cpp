@@ -49,6 +49,7 @@ struct HTri { f3 p[3]; uint32_t mesh; int smooth; };  // smooth: index into pt_s
 struct HMesh { uint32_t first, count, mat, flags, seq; };
 struct HSphere { f3 c; float r; uint32_t mat, flags, seq; int light; };
 struct HRect { int axis; float k, d0min, d0max, d1min, d1max; uint32_t mat, flags, seq; int light; };
+struct HCurve { f3 cps[4]; float w0, w1; uint32_t mat, flags; };
 
 struct pt_scene {
   std::vector<pt_material_desc> mats;
@@ -57,6 +58,7 @@ struct pt_scene {
   std::vector<HMesh> meshes;
   std::vector<HSphere> spheres;
   std::vector<HRect> rects;
+  std::vector<HCurve> curves;
   uint32_t seq = 0;
   std::vector<float> env;
   uint32_t env_w = 0, env_h = 0;
@@ -68,7 +70,8 @@ struct pt_scene {
   // device
   hipStream_t stream = nullptr;
   int device = -1;
-  DevBuf d_nodes, d_prims, d_info, d_geom, d_smooth, d_mats, d_lights, d_env, d_env_func, d_env_cdf, d_env_mcdf;
+  DevBuf d_nodes, d_prims, d_info, d_geom, d_smooth, d_curve, d_mats, d_lights, d_env, d_env_func, d_env_cdf,
+      d_env_mcdf;
   DevBuf d_perm, d_perm_off;
   uint32_t perm_seed = 0xffffffffu;
   bool perm_valid = false;
@@ -200,6 +203,64 @@ extern "C" pt_status pt_scene_add_sphere(pt_scene *s, const float c[3], float r,
   s->spheres.push_back(HSphere{mk3(c[0], c[1], c[2]), r, mat, flags, s->seq++, -1});
   s->committed = false;
   return PT_OK;
+}
+
+extern "C" pt_status pt_scene_add_curve(pt_scene *s, const float cps[12], float width0, float width1,
+                                        uint32_t type, uint32_t mat, uint32_t flags) {
+  if (!s || !cps) return fail(PT_ERR_INVALID, "pt_scene_add_curve: NULL argument");
+  if (type > PT_CURVE_RIBBON) return fail(PT_ERR_INVALID, "pt_scene_add_curve: unknown curve type");
+  if (mat >= s->mats.size()) return fail(PT_ERR_INVALID, "pt_scene_add_curve: material id out of range");
+  if (!(width0 >= 0.f) || !(width1 >= 0.f) || !std::isfinite(width0 + width1))
+    return fail(PT_ERR_INVALID, "pt_scene_add_curve: widths must be finite and >= 0");
+  HCurve c;
+  for (int i = 0; i < 4; ++i) {
+    c.cps[i] = mk3(cps[3 * i], cps[3 * i + 1], cps[3 * i + 2]);
+    if (!std::isfinite(cps[3 * i] + cps[3 * i + 1] + cps[3 * i + 2]))
+      return fail(PT_ERR_INVALID, "pt_scene_add_curve: non-finite control point");
+  }
+  c.w0 = width0;
+  c.w1 = width1;
+  c.mat = mat;
+  c.flags = flags & (PT_PRIM_FLIP_FACE | PT_PRIM_FORCE_INSIDE);
+  s->curves.push_back(c);
+  s->committed = false;
+  return PT_OK;
+}
+
+// evalBezier / evalBezierDerivative (primitives/bezier.hpp:12-35), the
+// reference's operation order.
+static f3 bezier_at(const f3 *c, float u) {
+  float u1 = 1.0f - u;
+  float b0 = u1 * u1 * u1, b1 = 3.0f * u * u1 * u1, b2 = 3.0f * u * u * u1, b3 = u * u * u;
+  return add3(add3(add3(scl3(c[0], b0), scl3(c[1], b1)), scl3(c[2], b2)), scl3(c[3], b3));
+}
+static f3 bezier_deriv(const f3 *c, float u) {
+  f3 d0 = scl3(sub3(c[1], c[0]), 3.0f), d1 = scl3(sub3(c[2], c[1]), 3.0f), d2 = scl3(sub3(c[3], c[2]), 3.0f);
+  float u1 = 1.0f - u;
+  float b0 = u1 * u1, b1 = 2.0f * u * u1, b2 = u * u;
+  return add3(add3(scl3(d0, b0), scl3(d1, b1)), scl3(d2, b2));
+}
+// The 8 cylinder segments of a curve (curve.hpp:54-83): ends, axis, length,
+// radius (the mean of the widths at the ends) and the end derivatives.
+struct CurveSeg { f3 pa, axis, dpa, dpb; float len, r; };
+static void curve_segments(const HCurve &c, CurveSeg seg[8]) {
+  f3 P[9], dP[9];
+  for (int i = 0; i <= 8; ++i) {
+    float u = (float)i / 8.0f;
+    P[i] = bezier_at(c.cps, u);
+    dP[i] = bezier_deriv(c.cps, u);
+  }
+  for (int i = 0; i < 8; ++i) {
+    float u0 = (float)i / 8.0f, u1 = (float)(i + 1) / 8.0f;
+    float r0 = c.w0 + (c.w1 - c.w0) * u0, r1 = c.w0 + (c.w1 - c.w0) * u1;
+    seg[i].pa = P[i];
+    f3 dv = sub3(P[i + 1], P[i]);
+    seg[i].axis = norm3(dv);
+    seg[i].len = len3(dv);
+    seg[i].r = 0.5f * (r0 + r1);
+    seg[i].dpa = dP[i];
+    seg[i].dpb = dP[i + 1];
+  }
 }
 
 extern "C" pt_status pt_scene_add_rect(pt_scene *s, int axis, float k, float d0min, float d0max, float d1min,
@@ -582,7 +643,7 @@ static float texel_max(const pt_scene *s, uint32_t x, uint32_t y) {
 
 extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   if (!s) return fail(PT_ERR_INVALID, "pt_scene_commit: NULL scene");
-  size_t nprim = s->tris.size() + s->spheres.size() + s->rects.size();
+  size_t nprim = s->tris.size() + s->spheres.size() + s->rects.size() + 8 * s->curves.size();
   if (nprim == 0) return fail(PT_ERR_INVALID, "pt_scene_commit: the scene has no objects");  // renderManager.hpp:64-68
   if (nprim >= (1u << 27)) return fail(PT_ERR_INVALID, "pt_scene_commit: too many primitives");
   pt_status st = ensure_stream(s);
@@ -698,6 +759,31 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     info[id] = make_uint2(rc.mat | (PRIM_RECT << 24) | ((rc.flags & 3u) << 28), (uint32_t)rc.light);
     geom[id] = make_float4((float)a, rc.k, 0.f, 0.f);
   }
+  // curves: 8 segments each, after the rects; their shading data per segment
+  const uint32_t nrect = (uint32_t)s->rects.size();
+  std::vector<float4> cgeom(4 * std::max<size_t>(1, 8 * s->curves.size()));
+  std::vector<CurveSeg> csegs(8 * s->curves.size());
+  for (uint32_t ci = 0; ci < (uint32_t)s->curves.size(); ++ci) {
+    const HCurve &c = s->curves[ci];
+    curve_segments(c, &csegs[8 * (size_t)ci]);
+    for (uint32_t k = 0; k < 8; ++k) {
+      const uint32_t si = 8 * ci + k, id = ntri + nsph + nrect + si;
+      const CurveSeg &sg = csegs[si];
+      f3 pb = add3(sg.pa, scl3(sg.axis, sg.len));
+      BRef &r = B.refs[id];
+      for (int a = 0; a < 3; ++a) {
+        r.lo[a] = std::min(comp3(sg.pa, a), comp3(pb, a)) - sg.r;
+        r.hi[a] = std::max(comp3(sg.pa, a), comp3(pb, a)) + sg.r;
+      }
+      r.prim = id;
+      info[id] = make_uint2(c.mat | (PRIM_CURVE << 24) | ((c.flags & 3u) << 28), si);
+      geom[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+      cgeom[4 * (size_t)si + 0] = make_float4(sg.pa.x, sg.pa.y, sg.pa.z, sg.len);
+      cgeom[4 * (size_t)si + 1] = make_float4(sg.axis.x, sg.axis.y, sg.axis.z, sg.r);
+      cgeom[4 * (size_t)si + 2] = make_float4(sg.dpa.x, sg.dpa.y, sg.dpa.z, 0.f);
+      cgeom[4 * (size_t)si + 3] = make_float4(sg.dpb.x, sg.dpb.y, sg.dpb.z, 0.f);
+    }
+  }
 
   // -- BVH
   B.nodes.resize(4 * std::max<size_t>(1, 2 * nprim));
@@ -742,12 +828,18 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
       P[0] = make_float4(sp.c.x, sp.c.y, sp.c.z, __builtin_bit_cast(float, tag));
       P[1] = make_float4(sp.r, 0.f, 0.f, 0.f);
       P[2] = make_float4(0.f, 0.f, 0.f, 0.f);
-    } else {
+    } else if (id < ntri + nsph + nrect) {
       const HRect &rc = s->rects[id - ntri - nsph];
       uint32_t tag = id | ((uint32_t)PRIM_RECT << 30);
       P[0] = make_float4(rc.k, rc.d0min, rc.d0max, __builtin_bit_cast(float, tag));
       P[1] = make_float4(rc.d1min, rc.d1max, (float)rc.axis, 0.f);
       P[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      const CurveSeg &sg = csegs[id - ntri - nsph - nrect];
+      uint32_t tag = id | ((uint32_t)PRIM_CURVE << 30);
+      P[0] = make_float4(sg.pa.x, sg.pa.y, sg.pa.z, __builtin_bit_cast(float, tag));
+      P[1] = make_float4(sg.axis.x, sg.axis.y, sg.axis.z, sg.r);
+      P[2] = make_float4(sg.len, 0.f, 0.f, 0.f);
     }
   }
 
@@ -800,6 +892,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   HIP_TRY(upload(s->d_prims, prims.data(), prims.size() * sizeof(float4)));
   HIP_TRY(upload(s->d_info, info.data(), info.size() * sizeof(uint2)));
   HIP_TRY(upload(s->d_geom, geom.data(), geom.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_curve, cgeom.data(), cgeom.size() * sizeof(float4)));
   if (smooth.empty())
     HIP_TRY(s->d_smooth.alloc(sizeof(float4)));
   else
@@ -821,6 +914,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.prim_info = s->d_info.as<uint2>();
   ds.prim_geom = s->d_geom.as<float4>();
   ds.tri_smooth = s->d_smooth.as<float4>();
+  ds.curve_geom = s->d_curve.as<float4>();
   ds.mats = s->d_mats.as<pt_material_desc>();
   ds.lights = s->d_lights.as<DevLight>();
   ds.n_lights = (int)s->lights.size();

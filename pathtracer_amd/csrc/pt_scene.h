@@ -25,7 +25,7 @@
 
 namespace pt {
 
-enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
+enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2, PRIM_CURVE = 3 };
 // NEE+MIS with the learned guide as a second directional technique (only
 // through pt_render_guided, so not in the public estimator list).
 enum { PT_EST_GUIDED = 6 };
@@ -57,6 +57,7 @@ struct DevScene {
   const uint2 *prim_info;
   const float4 *prim_geom;
   const float4 *tri_smooth;  // per smooth triangle: (p0,n0.x) (e1,n0.y) (e2,n0.z) (n1,n2.x) (n2.y,n2.z,-,-)
+  const float4 *curve_geom;  // per curve segment: (pa, len) (axis, radius) (dP_a, -) (dP_b, -)
   const pt_material_desc *mats;
   const DevLight *lights;
   int n_lights;

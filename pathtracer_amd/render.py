@@ -38,6 +38,10 @@ class Scene:
                 c, r, mat, flags = spec.spheres[i]
                 ca = np.asarray(c, np.float32)
                 L.check(L.lib.pt_scene_add_sphere(h, L.ptr(ca), r, mat, flags), "pt_scene_add_sphere")
+            elif kind == "curve":
+                cps, w0, w1, ctype, mat, flags = spec.curves[i]
+                cps = np.ascontiguousarray(cps, np.float32)
+                L.check(L.lib.pt_scene_add_curve(h, L.ptr(cps), w0, w1, ctype, mat, flags), "pt_scene_add_curve")
             else:
                 axis, k, a0, a1, b0, b1, mat, flags = spec.rects[i]
                 L.check(L.lib.pt_scene_add_rect(h, axis, k, a0, a1, b0, b1, mat, flags), "pt_scene_add_rect")

@@ -38,6 +38,7 @@ class SceneSpec:
     render: dict = dataclasses.field(default_factory=dict)
     order: list = dataclasses.field(default_factory=list)        # insertion order of ("mesh"|"sphere"|"rect", index)
     mesh_normals: dict = dataclasses.field(default_factory=dict)  # mesh index -> (normals f32 (k,3), nidx i32 (m,3))
+    curves: list = dataclasses.field(default_factory=list)      # (cps f32 (4,3), width0, width1, type, mat, flags)
     # global HomogeneousMedium: (sigma_a(3), sigma_s(3), sigma_scale, Le(3), Le_scale, g) or None
     medium: tuple | None = None
     name: str = ""
@@ -60,6 +61,12 @@ class SceneSpec:
     def add_rect(self, axis, k, d0min, d0max, d1min, d1max, mat, flags=0):
         self.rects.append((axis, k, d0min, d0max, d1min, d1max, mat, flags))
         self.order.append(("rect", len(self.rects) - 1))
+
+    def add_curve(self, cps, width0, width1, mat, ctype=1, flags=0):
+        """Curve(cps, width0, width1, type, material) (curve.hpp:13-140)."""
+        self.curves.append((np.asarray(cps, np.float32).reshape(4, 3), float(width0), float(width1), ctype, mat,
+                            flags))
+        self.order.append(("curve", len(self.curves) - 1))
 
     @property
     def n_triangles(self):
@@ -476,3 +483,34 @@ def single_scatter_medium(width=128, height=128, spp_x=4, spp_y=4, g=0.0) -> Sce
     spheres in a scattering medium (sigma_a 0.1, sigma_s 0.7, HG g)."""
     return _medium_scene("single_scatter_medium", (0.1, 0.1, 0.1), (0.7, 0.7, 0.7), g, True,
                          width, height, spp_x, spp_y)
+
+
+def hair_curves(width=96, height=96, spp_x=4, spp_y=4, n_curves=60, seed=9, estimator=L.EST_NEE_MIS,
+                max_depth=8) -> SceneSpec:
+    """A tuft of Marschner-hair Bezier curves (the curve primitive with its
+    hair tangent) over a Lambertian floor under a quad light."""
+    rng = np.random.default_rng(seed)
+    s = SceneSpec(name="hair_curves")
+    floor = s.add_material(L.MAT_LAMBERTIAN, [0.6, 0.6, 0.6])
+    light = s.add_material(L.MAT_EMITTER, [1.0, 1.0, 1.0, 12.0])
+    mats = []
+    for mel in (0.3, 0.8, 1.3):
+        mats.append(s.add_material(L.MAT_HAIR, [mel * EUMELANIN_SIGMA_A[0], mel * EUMELANIN_SIGMA_A[1],
+                                                mel * EUMELANIN_SIGMA_A[2], 0.0, 1.55, 0.3, 0.3,
+                                                math.radians(2.0)]))
+    s.add_rect(1, 0.0, -3, 3, -3, 3, floor)
+    s.add_rect(1, 3.0, -0.75, 0.75, -0.75, 0.75, light)
+    for i in range(n_curves):
+        x0, z0 = rng.uniform(-0.8, 0.8, 2)
+        lean = rng.normal(size=2) * 0.3
+        cps = np.array([[x0, 0.0, z0],
+                        [x0 + lean[0] * 0.3, 0.5, z0 + lean[1] * 0.3],
+                        [x0 + lean[0] * 0.8, 1.0, z0 + lean[1] * 0.8],
+                        [x0 + lean[0] * 1.5, 1.4, z0 + lean[1] * 1.5]])
+        s.add_curve(cps, 0.03, 0.008, mats[i % 3])
+    s.camera = dict(fov_deg=40.0, aspect=width / height, focal_dist=1.0, aperture=0.0, up=(0, 1, 0),
+                    front=(0, -0.25, -1), pos=(0, 1.2, 4.0))
+    s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=max_depth,
+                    sampler=L.SAMPLER_HALTON_OWEN, estimator=estimator, rr_depth=3, light_samples=4, seed=5,
+                    bg=(0.05, 0.05, 0.06))
+    return s

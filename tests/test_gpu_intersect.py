@@ -76,3 +76,22 @@ def test_displaced_mesh_closest_hit(gpu):
     ga, _ = g.intersect(rays, any_hit=True)
     ca, _ = o.intersect(rays, any_hit=True)
     assert np.array_equal(ga >= 0, ca >= 0)
+
+
+def test_curves_closest_and_any_hit(gpu):
+    """Bezier curves (8 cylinder segments each) among rects: ids, t, any hit."""
+    from pathtracer_amd import render as R, scenes
+    spec = scenes.hair_curves(n_curves=80)
+    g = R.Scene(spec)
+    o = O.OracleScene(spec)
+    rng = np.random.default_rng(12)
+    rays = _rays_towards(rng, 400_000, [-1.2, 0.0, -1.2], [1.2, 1.6, 1.2], ([-2, 0.2, -2], [2, 2.5, 4]))
+    gp, gt = g.intersect(rays)
+    cp, ct = o.intersect(rays)
+    ties = _compare(gp, gt, cp, ct)
+    assert ties < 20
+    n_curve_hits = int((gp >= len(spec.rects)).sum())  # curve segments follow the two rects
+    assert n_curve_hits > 10_000
+    ga, _ = g.intersect(rays, any_hit=True)
+    ca, _ = o.intersect(rays, any_hit=True)
+    assert np.array_equal(ga >= 0, ca >= 0)

@@ -139,3 +139,14 @@ def test_volume_per_sample(gpu, which):
     _, gs, _, os_, st = _per_sample(spec)
     assert _sample_agreement(gs, os_) > 0.995
     assert st["nan_samples"] == 0
+
+
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_hair_curves_per_sample(gpu, estimator):
+    """Marschner hair on Bezier curves: the curve's tangent frames the hair
+    sample (HairMarschner::scatter), the normal frames f and the pdf."""
+    from pathtracer_amd import scenes
+    spec = scenes.hair_curves(width=40, height=40, spp_x=2, spp_y=2, estimator=estimator)
+    _, gs, _, os_, st = _per_sample(spec, estimator=estimator, rr_depth=0)
+    assert _sample_agreement(gs, os_) > 0.99
+    assert st["nan_samples"] == 0

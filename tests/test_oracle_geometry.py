@@ -239,3 +239,94 @@ def test_smooth_icosphere_approaches_the_sphere_normal():
     err_f = np.linalg.norm(nf - true_n, axis=1)   # the ray is the outward one
     assert err_s.max() < 0.02
     assert err_s.mean() < 0.25 * err_f.mean()
+
+
+# ---------------------------------------------------------------- curves
+
+def _curve_scene(cps, w0, w1):
+    from pathtracer_amd import scenes, _lib as L
+    s = scenes.SceneSpec(name="curve")
+    m = s.add_material(L.MAT_LAMBERTIAN, [0.5, 0.5, 0.5])
+    s.add_curve(cps, w0, w1, m)
+    s.camera = dict(fov_deg=40.0, aspect=1.0, focal_dist=1.0, aperture=0.0, up=(0, 1, 0), front=(0, 0, -1),
+                    pos=(0, 0, 5))
+    s.render = dict(width=8, height=8, spp_x=1, spp_y=1, max_depth=2, sampler=0, estimator=1, rr_depth=0,
+                    light_samples=1, seed=1, bg=(0, 0, 0))
+    return s
+
+
+def test_straight_curve_is_a_cylinder():
+    """Collinear, evenly spaced control points make the Bezier a straight line
+    (uniform in u), and equal widths a cylinder of that radius: rays
+    perpendicular to the axis at offset b hit at t = D - sqrt(r^2 - b^2), with
+    the radial normal and the axis as tangent; rays past the ends or beyond the
+    radius miss (curve.hpp:47-130)."""
+    cps = np.array([[0, -1, 0], [0, -1 / 3, 0], [0, 1 / 3, 0], [0, 1, 0]], np.float64)
+    r = 0.2
+    o = O.OracleScene(_curve_scene(cps, r, r))
+    rng = np.random.default_rng(3)
+    n = 2000
+    b = rng.uniform(-0.99 * r, 0.99 * r, n)
+    y = rng.uniform(-0.95, 0.95, n)
+    D = 3.0
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0] = b
+    rays[:, 1] = y
+    rays[:, 2] = D
+    rays[:, 5] = -1.0
+    rays[:, 6] = 1e-4
+    rays[:, 7] = 1e5
+    prim, t = o.intersect(rays)
+    assert np.all(prim >= 0)
+    np.testing.assert_allclose(t, D - np.sqrt(r * r - b * b), rtol=0, atol=2e-5)
+    # segment ids follow y: 8 segments over [-1, 1]
+    np.testing.assert_array_equal(prim, np.clip(((y + 1) / 2 * 8).astype(int), 0, 7))
+    nrm, inside = o.surface(rays)
+    hitp = np.stack([b, y, D - t], 1)
+    radial = hitp * [1, 0, 1]
+    radial /= np.linalg.norm(radial, axis=1, keepdims=True)
+    np.testing.assert_allclose(nrm, radial, atol=1e-4)   # faces the ray (+z side)
+    assert np.all(inside == 1)
+    # misses: past the radius, beyond the ends
+    rays[:, 0] = np.where(b >= 0, r * 1.01, -r * 1.01)
+    prim, _ = o.intersect(rays)
+    assert np.all(prim < 0)
+    rays[:, 0] = 0.0
+    rays[:, 1] = np.where(y >= 0, 1.05, -1.05)
+    prim, _ = o.intersect(rays)
+    assert np.all(prim < 0)
+
+
+def test_curved_curve_hits_lie_on_the_tube():
+    """A bent Bezier with tapering width: every hit point lies within the
+    segment's radius of the Bezier polyline (the 8-segment approximation), on
+    the surface (distance = radius to its segment's axis), and brute force
+    agrees with the BVH-free scene test."""
+    cps = np.array([[-1, -1, 0], [-0.2, 0.8, 0.3], [0.4, -0.6, -0.2], [1.0, 1.0, 0.1]], np.float64)
+    w0, w1 = 0.15, 0.05
+    o = O.OracleScene(_curve_scene(cps, w0, w1))
+    rng = np.random.default_rng(4)
+    n = 4000
+    org = np.stack([rng.uniform(-1.5, 1.5, n), rng.uniform(-1.5, 1.5, n), np.full(n, 4.0)], 1)
+    tgt = np.stack([rng.uniform(-1.2, 1.2, n), rng.uniform(-1.2, 1.2, n), rng.uniform(-0.3, 0.3, n)], 1)
+    d = tgt - org
+    rays = np.concatenate([org, d, np.full((n, 1), 1e-4), np.full((n, 1), 1e5)], 1).astype(np.float32)
+    prim, t = o.intersect(rays)
+    bp, bt = o.intersect(rays, brute=True)
+    np.testing.assert_array_equal(prim, bp)
+    np.testing.assert_allclose(t[prim >= 0], bt[prim >= 0], rtol=1e-6)
+    hit = prim >= 0
+    assert hit.sum() > 200
+    u = np.linspace(0, 1, 9)
+    B = ((1 - u) ** 3)[:, None] * cps[0] + (3 * u * (1 - u) ** 2)[:, None] * cps[1] + \
+        (3 * u * u * (1 - u))[:, None] * cps[2] + (u ** 3)[:, None] * cps[3]
+    p = rays[hit, :3].astype(np.float64) + t[hit, None] * rays[hit, 3:6]
+    seg = prim[hit]
+    pa, pb = B[seg], B[seg + 1]
+    ax = (pb - pa) / np.linalg.norm(pb - pa, axis=1, keepdims=True)
+    v = p - pa
+    dist = np.linalg.norm(v - (v * ax).sum(1, keepdims=True) * ax, axis=1)
+    rad = 0.5 * ((w0 + (w1 - w0) * seg / 8) + (w0 + (w1 - w0) * (seg + 1) / 8))
+    np.testing.assert_allclose(dist, rad, atol=5e-5)
+    proj = (v * ax).sum(1)
+    assert np.all(proj >= -1e-5) and np.all(proj <= np.linalg.norm(pb - pa, axis=1) + 1e-5)
