@@ -138,6 +138,22 @@ def kernel_roofline(cst, ms, launches, read_traffic_for=None):
     return roofline, per_kernel
 
 
+def build_flags(args):
+    from pathtracer_amd import _lib as L
+    return L.BUILD_DEVICE_LBVH if args.build == "lbvh" else 0
+
+
+def bvh_info(args, scs, seconds):
+    """The BVH build of the scenes: kind, build time (host wall clock of the
+    build alone, device kernels waited for), whole commit, nodes, depth."""
+    return {"build": {"sah": "host_sah", "lbvh": "device_lbvh"}[args.build],
+            "bvh_ms": round(sum(sc.info.bvh_ms for sc in scs), 2),
+            "commit_ms": round(sum(sc.info.build_ms for sc in scs), 2),
+            "scene_s": round(seconds, 2),
+            "nodes": sum(sc.info.bvh_nodes for sc in scs),
+            "max_depth": max(sc.info.bvh_depth for sc in scs)}
+
+
 def main_batch(args):
     """64 independent renders (seed 4): displaced geodesic spheres of 1e4-5e5
     triangles (log-uniform), 1-4 quad lights, a material from {Lambertian,
@@ -154,8 +170,9 @@ def main_batch(args):
     log(f"{len(specs)} scenes generated in {time.perf_counter() - t0:.1f} s, "
         f"{sum(sp.n_triangles for sp in specs)} triangles")
     t0 = time.perf_counter()
-    scs = [R.Scene(sp) for sp in specs]
-    log(f"BVHs built and uploaded in {time.perf_counter() - t0:.1f} s")
+    scs = [R.Scene(sp, build_flags=build_flags(args)) for sp in specs]
+    build = bvh_info(args, scs, time.perf_counter() - t0)
+    log(f"BVHs built and uploaded: {build}")
     cfgs = [sc.cfg(estimator=args.estimator) for sc in scs]
     total = sum(c.width * c.height * c.spp_x * c.spp_y for c in cfgs)
     b = R.Batch()
@@ -205,7 +222,7 @@ def main_batch(args):
         "serial": serial, "speedup_vs_serial": round(sdt / (dt / args.steps), 3),
         "mrays_per_s": round((cst["segments"] + cst["shadow_rays"]) * args.steps / dt / 1e6, 1),
         "kernel_ms_per_step_eager": {k: round(v, 2) for k, v in ms.items()},
-        "roofline": roofline, "kernels": per_kernel, "cpu_baseline": None,
+        "roofline": roofline, "kernels": per_kernel, "bvh": build, "cpu_baseline": None,
     }
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -365,6 +382,8 @@ def main():
                          "in one wavefront against 64 serial renders; guided_room: the slit-lit room with the "
                          "learned guide against the plain tracer")
     ap.add_argument("--no-tune", action="store_true", help="batch64: skip the autotuner")
+    ap.add_argument("--build", default="sah", choices=("sah", "lbvh"),
+                    help="BVH build: sah (host binned SAH, default) or lbvh (on the device)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -385,9 +404,9 @@ def main():
         spec = scenes.glass_hdri()
     log(f"scene generated in {time.perf_counter() - t0:.1f} s: {spec.n_triangles} triangles")
     t0 = time.perf_counter()
-    scene = R.Scene(spec)
-    log(f"BVH built and uploaded in {time.perf_counter() - t0:.1f} s: {scene.info.bvh_nodes} nodes, "
-        f"depth {scene.info.bvh_depth}")
+    scene = R.Scene(spec, build_flags=build_flags(args))
+    build = bvh_info(args, [scene], time.perf_counter() - t0)
+    log(f"BVH built and uploaded: {build}")
     cfg = scene.cfg(estimator=args.estimator)
     total = cfg.width * cfg.height * cfg.spp_x * cfg.spp_y
 
@@ -446,6 +465,7 @@ def main():
         "roofline": roofline,
         "kernels": per_kernel,
         "other_estimator": other_res,
+        "bvh": build,
         "quality": None,
         "cpu_baseline": None,
     }

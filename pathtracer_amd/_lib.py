@@ -28,6 +28,7 @@ RENDER_COUNTERS, RENDER_NO_SORT, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
 BATCH_NO_GRAPH = 256
 TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF = 1, 2, 4
+BUILD_DEVICE_LBVH = 1
 GUIDE_NO_TRAIN = 1
 GUIDE_INPUTS, GUIDE_OUTPUTS, GUIDE_NPARAMS, GUIDE_MIX = 6, 41, 27497, 48
 PRIM_FLIP_FACE, PRIM_FORCE_INSIDE, PRIM_NOT_LIGHT = 1, 2, 4
@@ -79,7 +80,7 @@ class SceneInfo(ctypes.Structure):
                 ("n_spheres", ctypes.c_uint32), ("n_rects", ctypes.c_uint32),
                 ("n_lights", ctypes.c_uint32), ("bvh_nodes", ctypes.c_uint32),
                 ("bvh_depth", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64),
-                ("build_ms", ctypes.c_double)]
+                ("build_ms", ctypes.c_double), ("bvh_ms", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):

@@ -25,6 +25,7 @@
 #include "../../include/pathtracer_amd.h"
 #include "pt_guide_host.h"
 #include "pt_launch.h"
+#include "pt_lbvh.h"
 
 using namespace pt;
 
@@ -73,6 +74,7 @@ struct pt_scene {
   DevBuf d_nodes, d_prims, d_info, d_geom, d_smooth, d_curve, d_mats, d_lights, d_env, d_env_func, d_env_cdf,
       d_env_mcdf;
   DevBuf d_perm, d_perm_off;
+  DevBuf lb_in, lb_scratch, lb_nodes;  // device BVH build: input records, scratch, nodes before trimming
   uint32_t perm_seed = 0xffffffffu;
   bool perm_valid = false;
   DevScene ds{};
@@ -704,6 +706,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   }
 
   // -- primitive references and per-primitive info
+  const bool device_build = cfg && (cfg->flags & PT_BUILD_DEVICE_LBVH);
   Builder B;
   B.leaf_size = (cfg && cfg->leaf_size) ? std::min<uint32_t>(cfg->leaf_size, 16u) : 4u;
   s->build_cfg = cfg ? *cfg : pt_build_cfg{};
@@ -785,36 +788,9 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     }
   }
 
-  // -- BVH
-  B.nodes.resize(4 * std::max<size_t>(1, 2 * nprim));
-  Builder::bounds(B.refs.data(), nprim, s->box_lo, s->box_hi);
-  int root = B.build(0, nprim, 0, true);
-  uint32_t nn = B.node_count.load();
-  if (root < 0) {  // the whole scene is one leaf: wrap it in a root with both children on it
-    uint32_t ni = nn++;
-    B.write_node(ni, 0, nprim, nprim, root, root);
-    // write_node bounded [0,nprim) and [nprim,nprim); give child 1 the same box
-    B.nodes[4 * (size_t)ni + 1] = B.nodes[4 * (size_t)ni + 0];
-    float4 z = B.nodes[4 * (size_t)ni + 2];
-    B.nodes[4 * (size_t)ni + 2] = make_float4(z.x, z.y, z.x, z.y);
-    // the root must be node 0: swap it into place
-    if (ni != 0) return fail(PT_ERR_INVALID, "internal: leaf root not at node 0");
-    root = (int)ni;
-  }
-  B.nodes.resize(4 * (size_t)nn);
-
-  // -- collapse the binary tree into 4-wide nodes
-  std::vector<float4> nodes4;
-  uint32_t depth4 = 0;
-  collapse_bvh4(B.nodes, nodes4, depth4);
-  if (3 * depth4 + 3 > PT_STACK4)
-    return fail(PT_ERR_UNSUPPORTED, "pt_scene_commit: BVH too deep for the traversal stack");
-
-  // -- primitives in leaf order
-  std::vector<float4> prims(3 * nprim);
-  for (size_t k = 0; k < nprim; ++k) {
-    uint32_t id = B.refs[k].prim;
-    float4 *P = &prims[3 * k];
+  // -- primitive records (3 float4 each): triangle p0, e1, e2; sphere centre
+  // and radius; rect k, extents, axis; curve segment start, axis, radius, length
+  auto prim_record = [&](uint32_t id, float4 *P) {
     if (id < ntri) {
       const HTri &t = s->tris[id];
       f3 e1 = sub3(t.p[1], t.p[0]), e2 = sub3(t.p[2], t.p[0]);  // meshBVH.cpp:93-94
@@ -841,6 +817,69 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
       P[1] = make_float4(sg.axis.x, sg.axis.y, sg.axis.z, sg.r);
       P[2] = make_float4(sg.len, 0.f, 0.f, 0.f);
     }
+  };
+
+  std::vector<float4> nodes4;
+  uint32_t depth4 = 0, n_nodes4 = 0;
+  int root = 0;
+  double bvh_ms = 0.0;
+  if (device_build) {
+    // -- BVH on the device: records in id order in, leaf order out
+    std::vector<float4> prims(3 * nprim);
+    for (size_t k = 0; k < nprim; ++k) prim_record((uint32_t)k, &prims[3 * k]);
+    HIP_TRY(upload(s->lb_in, prims.data(), prims.size() * sizeof(float4)));
+    HIP_TRY(s->lb_scratch.alloc(lbvh_scratch_bytes((uint32_t)nprim)));
+    HIP_TRY(s->lb_nodes.alloc(std::max<size_t>(1, nprim - 1) * 8 * sizeof(float4)));
+    HIP_TRY(s->d_prims.alloc(prims.size() * sizeof(float4)));
+    LbvhScratch sc{};
+    lbvh_carve(s->lb_scratch.p, (uint32_t)nprim, sc);
+    LbvhResult res{};
+    auto b0 = std::chrono::steady_clock::now();
+    HIP_TRY(lbvh_build(s->stream, s->lb_in.as<float4>(), (uint32_t)nprim, B.leaf_size, sc,
+                       s->d_prims.as<float4>(), s->lb_nodes.as<float4>(), res));
+    bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b0).count();
+    depth4 = res.depth4;
+    n_nodes4 = res.nodes4;
+    root = res.root;
+    for (int a = 0; a < 3; ++a) {
+      s->box_lo[a] = res.lo[a];
+      s->box_hi[a] = res.hi[a];
+    }
+    if (3 * depth4 + 3 > PT_STACK4)
+      return fail(PT_ERR_UNSUPPORTED, "pt_scene_commit: BVH too deep for the traversal stack");
+    HIP_TRY(s->d_nodes.alloc((size_t)n_nodes4 * 8 * sizeof(float4)));
+    HIP_TRY(hipMemcpyAsync(s->d_nodes.p, s->lb_nodes.p, (size_t)n_nodes4 * 8 * sizeof(float4),
+                           hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  } else {
+    // -- BVH on the host: binned SAH, then collapsed into 4-wide nodes
+    auto b0 = std::chrono::steady_clock::now();
+    B.nodes.resize(4 * std::max<size_t>(1, 2 * nprim));
+    Builder::bounds(B.refs.data(), nprim, s->box_lo, s->box_hi);
+    root = B.build(0, nprim, 0, true);
+    uint32_t nn = B.node_count.load();
+    if (root < 0) {  // the whole scene is one leaf: wrap it in a root with both children on it
+      uint32_t ni = nn++;
+      B.write_node(ni, 0, nprim, nprim, root, root);
+      // write_node bounded [0,nprim) and [nprim,nprim); give child 1 the same box
+      B.nodes[4 * (size_t)ni + 1] = B.nodes[4 * (size_t)ni + 0];
+      float4 z = B.nodes[4 * (size_t)ni + 2];
+      B.nodes[4 * (size_t)ni + 2] = make_float4(z.x, z.y, z.x, z.y);
+      // the root must be node 0: swap it into place
+      if (ni != 0) return fail(PT_ERR_INVALID, "internal: leaf root not at node 0");
+      root = (int)ni;
+    }
+    B.nodes.resize(4 * (size_t)nn);
+    collapse_bvh4(B.nodes, nodes4, depth4);
+    bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b0).count();
+    n_nodes4 = (uint32_t)(nodes4.size() / 8);
+    if (3 * depth4 + 3 > PT_STACK4)
+      return fail(PT_ERR_UNSUPPORTED, "pt_scene_commit: BVH too deep for the traversal stack");
+    // primitives in leaf order
+    std::vector<float4> prims(3 * nprim);
+    for (size_t k = 0; k < nprim; ++k) prim_record(B.refs[k].prim, &prims[3 * k]);
+    HIP_TRY(upload(s->d_nodes, nodes4.data(), nodes4.size() * sizeof(float4)));
+    HIP_TRY(upload(s->d_prims, prims.data(), prims.size() * sizeof(float4)));
   }
 
   // -- environment tables (2-D piecewise-constant; same construction as the oracle)
@@ -888,8 +927,6 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   }
 
   // -- upload
-  HIP_TRY(upload(s->d_nodes, nodes4.data(), nodes4.size() * sizeof(float4)));
-  HIP_TRY(upload(s->d_prims, prims.data(), prims.size() * sizeof(float4)));
   HIP_TRY(upload(s->d_info, info.data(), info.size() * sizeof(uint2)));
   HIP_TRY(upload(s->d_geom, geom.data(), geom.size() * sizeof(float4)));
   HIP_TRY(upload(s->d_curve, cgeom.data(), cgeom.size() * sizeof(float4)));
@@ -941,8 +978,9 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   in.n_spheres = nsph;
   in.n_rects = (uint32_t)s->rects.size();
   in.n_lights = (uint32_t)s->lights.size();
-  in.bvh_nodes = (uint32_t)(nodes4.size() / 8);
+  in.bvh_nodes = n_nodes4;
   in.bvh_depth = depth4;
+  in.bvh_ms = bvh_ms;
   in.device_bytes = s->d_nodes.bytes + s->d_prims.bytes + s->d_info.bytes + s->d_geom.bytes + s->d_smooth.bytes +
                     s->d_env.bytes +
                     s->d_env_func.bytes + s->d_env_cdf.bytes;

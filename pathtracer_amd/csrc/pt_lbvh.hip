@@ -1,0 +1,403 @@
+// pt_lbvh.hip -- on-device BVH build (linear BVH, Karras 2012) for scenes
+// whose instances change often: primitive bounds -> 30-bit Morton codes of
+// the centroids -> radix sort -> the binary radix tree of the sorted codes
+// (one thread per internal node) -> bottom-up box refit -> collapse into the
+// renderer's 4-wide nodes, level by level, with subtrees of at most leaf_size
+// primitives as leaves.  The primitives are reordered into leaf order on the
+// device.  The result has the same layout as the host SAH build (pt_scene.h),
+// so traversal is unchanged.
+#include <hipcub/hipcub.hpp>
+
+#include "pt_device.h"
+#include "pt_lbvh.h"
+
+namespace pt {
+
+__device__ __forceinline__ uint32_t l_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// Order-preserving float <-> uint for atomic min / max.
+PT_D uint32_t f2o(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+PT_D float o2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+
+// Bounds of primitive record i (the prims layout of pt_scene.h), before padding.
+PT_D void prim_bounds(const float4 *P, f3 &lo, f3 &hi) {
+  const float4 A = P[0], B = P[1], C = P[2];
+  const uint32_t type = __float_as_uint(A.w) >> 30;
+  if (type == PRIM_TRI) {
+    const f3 p0 = mk3(A.x, A.y, A.z), p1 = add3(p0, mk3(B.x, B.y, B.z)), p2 = add3(p0, mk3(C.x, C.y, C.z));
+    lo = mk3(fminf(p0.x, fminf(p1.x, p2.x)), fminf(p0.y, fminf(p1.y, p2.y)), fminf(p0.z, fminf(p1.z, p2.z)));
+    hi = mk3(fmaxf(p0.x, fmaxf(p1.x, p2.x)), fmaxf(p0.y, fmaxf(p1.y, p2.y)), fmaxf(p0.z, fmaxf(p1.z, p2.z)));
+  } else if (type == PRIM_SPHERE) {
+    const f3 c = mk3(A.x, A.y, A.z);
+    lo = sub3(c, s3(B.x));
+    hi = add3(c, s3(B.x));
+  } else if (type == PRIM_RECT) {
+    const int ax = (int)B.z, b0 = ax == 0 ? 1 : 0, b1 = ax == 2 ? 1 : 2;
+    float l[3], h[3];
+    l[ax] = h[ax] = A.x;
+    l[b0] = A.y; h[b0] = A.z;
+    l[b1] = B.x; h[b1] = B.y;
+    lo = mk3(l[0], l[1], l[2]);
+    hi = mk3(h[0], h[1], h[2]);
+  } else {  // curve segment
+    const f3 pa = mk3(A.x, A.y, A.z), pb = add3(pa, scl3(mk3(B.x, B.y, B.z), C.x));
+    lo = sub3(mk3(fminf(pa.x, pb.x), fminf(pa.y, pb.y), fminf(pa.z, pb.z)), s3(B.w));
+    hi = add3(mk3(fmaxf(pa.x, pb.x), fmaxf(pa.y, pb.y), fmaxf(pa.z, pb.z)), s3(B.w));
+  }
+}
+
+// The same relative padding as the host build: boxes stay conservative under
+// the slab test's approximate reciprocal.
+PT_D void pad_box(f3 &lo, f3 &hi) {
+  const float px = 1e-6f * (fabsf(lo.x) + fabsf(hi.x)) + 1e-12f;
+  const float py = 1e-6f * (fabsf(lo.y) + fabsf(hi.y)) + 1e-12f;
+  const float pz = 1e-6f * (fabsf(lo.z) + fabsf(hi.z)) + 1e-12f;
+  lo = mk3(lo.x - px, lo.y - py, lo.z - pz);
+  hi = mk3(hi.x + px, hi.y + py, hi.z + pz);
+}
+
+__global__ void k_lb_bounds(const float4 *prims, uint32_t n, float4 *box, uint32_t *cb) {
+  const uint32_t i = l_tid();
+  if (i >= n) return;
+  f3 lo, hi;
+  prim_bounds(prims + 3 * (size_t)i, lo, hi);
+  box[2 * (size_t)i] = make_float4(lo.x, lo.y, lo.z, 0.f);
+  box[2 * (size_t)i + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
+  const f3 c = scl3(add3(lo, hi), 0.5f);
+  // centroid bounds, then the scene bounds: cb = [cmin xyz, cmax xyz, smin xyz, smax xyz]
+  atomicMin(&cb[0], f2o(c.x)); atomicMin(&cb[1], f2o(c.y)); atomicMin(&cb[2], f2o(c.z));
+  atomicMax(&cb[3], f2o(c.x)); atomicMax(&cb[4], f2o(c.y)); atomicMax(&cb[5], f2o(c.z));
+  atomicMin(&cb[6], f2o(lo.x)); atomicMin(&cb[7], f2o(lo.y)); atomicMin(&cb[8], f2o(lo.z));
+  atomicMax(&cb[9], f2o(hi.x)); atomicMax(&cb[10], f2o(hi.y)); atomicMax(&cb[11], f2o(hi.z));
+}
+
+__global__ void k_lb_init_bounds(uint32_t *cb) {
+  const uint32_t i = threadIdx.x;
+  if (i < 12) cb[i] = ((i % 6) < 3) ? 0xffffffffu : 0u;
+}
+
+PT_D uint32_t spread10(uint32_t v) {  // 10 bits -> every third bit
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void k_lb_morton(const float4 *box, uint32_t n, const uint32_t *cb, uint32_t *code, uint32_t *idx) {
+  const uint32_t i = l_tid();
+  if (i >= n) return;
+  const float4 lo = box[2 * (size_t)i], hi = box[2 * (size_t)i + 1];
+  const f3 c = mk3(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z));
+  const f3 mn = mk3(o2f(cb[0]), o2f(cb[1]), o2f(cb[2])), mx = mk3(o2f(cb[3]), o2f(cb[4]), o2f(cb[5]));
+  auto q = [](float v, float a, float b) {
+    const float e = b - a;
+    const float t = e > 0.f ? (v - a) / e : 0.5f;
+    return (uint32_t)fminf(fmaxf(t * 1024.f, 0.f), 1023.f);
+  };
+  code[i] = (spread10(q(c.x, mn.x, mx.x)) << 2) | (spread10(q(c.y, mn.y, mx.y)) << 1) | spread10(q(c.z, mn.z, mx.z));
+  idx[i] = i;
+}
+
+// delta of Karras 2012: common prefix length of sorted keys i and j, with the
+// index as a tie-break for equal keys; -1 outside the array.
+PT_D int delta(const uint32_t *k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = k[i], b = k[j];
+  if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+  return __clz(a ^ b);
+}
+
+// Internal node i of n - 1: its range and children.  Child codes: >= 0
+// internal node, < 0 leaf ~k (sorted primitive k).
+__global__ void k_lb_karras(const uint32_t *key, int n, int2 *child, int2 *range, int *parent_int,
+                            int *parent_leaf) {
+  const int i = (int)l_tid();
+  if (i >= n - 1) return;
+  const int d = (delta(key, n, i, i + 1) - delta(key, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(key, n, i, i - d);
+  int lmax = 2;
+  while (delta(key, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (delta(key, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(key, n, i, j);
+  int s = 0;
+  for (int t = (l + 1) / 2;; t = (t + 1) / 2) {
+    if (delta(key, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t == 1) break;
+  }
+  const int g = i + s * d + (d < 0 ? -1 : 0);
+  const int lo = min(i, j), hi = max(i, j);
+  const int c0 = (lo == g) ? ~g : g;
+  const int c1 = (hi == g + 1) ? ~(g + 1) : g + 1;
+  child[i] = make_int2(c0, c1);
+  range[i] = make_int2(lo, hi);
+  if (c0 < 0) parent_leaf[~c0] = i; else parent_int[c0] = i;
+  if (c1 < 0) parent_leaf[~c1] = i; else parent_int[c1] = i;
+}
+
+// Boxes of the internal nodes, bottom-up: the second child to arrive at a
+// node computes its box and goes on.
+PT_D void child_box(int c, const float4 *leafbox, const float4 *nodebox, f3 &lo, f3 &hi) {
+  const float4 *b = c < 0 ? leafbox + 2 * (size_t)(~c) : nodebox + 2 * (size_t)c;
+  const float4 l = b[0], h = b[1];
+  lo = mk3(l.x, l.y, l.z);
+  hi = mk3(h.x, h.y, h.z);
+}
+__global__ void k_lb_refit(int n, const int2 *child, const int *parent_int, const int *parent_leaf,
+                           const float4 *leafbox, float4 *nodebox, uint32_t *flags) {
+  const int k = (int)l_tid();
+  if (k >= n) return;
+  int p = parent_leaf[k];
+  // a radix tree over 30-bit codes with index tie-breaks is at most 62 deep
+  for (int level = 0; level < 64 && p >= 0 && p < n - 1; ++level) {
+    __threadfence();
+    if (atomicAdd(&flags[p], 1u) == 0u) return;  // the other child is not done yet
+    __threadfence();
+    const int2 c = child[p];
+    f3 l0, h0, l1, h1;
+    child_box(c.x, leafbox, nodebox, l0, h0);
+    child_box(c.y, leafbox, nodebox, l1, h1);
+    nodebox[2 * (size_t)p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+    nodebox[2 * (size_t)p + 1] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    p = p == 0 ? -1 : parent_int[p];
+  }
+}
+
+// Leaf order: primitive records and their boxes by sorted index.
+__global__ void k_lb_reorder(const float4 *prims, const float4 *box, const uint32_t *idx, uint32_t n,
+                             float4 *prims_out, float4 *leafbox) {
+  const uint32_t k = l_tid();
+  if (k >= n) return;
+  const uint32_t i = idx[k];
+  for (int q = 0; q < 3; ++q) prims_out[3 * (size_t)k + q] = prims[3 * (size_t)i + q];
+  leafbox[2 * (size_t)k] = box[2 * (size_t)i];
+  leafbox[2 * (size_t)k + 1] = box[2 * (size_t)i + 1];
+}
+
+// One level of the collapse into 4-wide nodes.  Work item w: binary internal
+// node q[w].x becomes 4-wide node q[w].y; its children, with internal children
+// of more than leaf_size primitives replaced by their own two children, become
+// the node's up to 4 slots.
+__global__ void k_lb_collapse(const int2 *qin, uint32_t nq, const int2 *child, const int2 *range,
+                              const float4 *leafbox, const float4 *nodebox, uint32_t leaf_size, float4 *nodes4,
+                              uint32_t *n4, int2 *qout, uint32_t *nqout) {
+  const uint32_t w = l_tid();
+  if (w >= nq) return;
+  const int2 it = qin[w];
+  int items[4];
+  int ni = 0;
+  const int2 c = child[it.x];
+  const int cc[2] = {c.x, c.y};
+  auto size_of = [&](int x) { return x < 0 ? 1 : range[x].y - range[x].x + 1; };
+  for (int k = 0; k < 2; ++k) {
+    const int x = cc[k];
+    if (x >= 0 && size_of(x) > (int)leaf_size) {
+      items[ni++] = child[x].x;
+      items[ni++] = child[x].y;
+    } else {
+      items[ni++] = x;
+    }
+  }
+  float lo[3][4], hi[3][4];
+  int codes[4];
+  for (int k = 0; k < 4; ++k) {
+    if (k >= ni) {  // empty slot: an inverted box, as the host build writes
+      codes[k] = kEmptyChild;
+      for (int a = 0; a < 3; ++a) {
+        lo[a][k] = 1.f;
+        hi[a][k] = -1.f;
+      }
+      continue;
+    }
+    const int x = items[k];
+    f3 l, h;
+    child_box(x, leafbox, nodebox, l, h);
+    pad_box(l, h);
+    lo[0][k] = l.x; lo[1][k] = l.y; lo[2][k] = l.z;
+    hi[0][k] = h.x; hi[1][k] = h.y; hi[2][k] = h.z;
+    const int sz = size_of(x);
+    if (x < 0 || sz <= (int)leaf_size) {
+      const int first = x < 0 ? ~x : range[x].x;
+      codes[k] = leaf_code((uint32_t)first, (uint32_t)sz);
+    } else {
+      const uint32_t j = atomicAdd(n4, 1u);
+      codes[k] = (int)j;
+      qout[atomicAdd(nqout, 1u)] = make_int2(x, (int)j);
+    }
+  }
+  float4 *nd = nodes4 + 8 * (size_t)it.y;
+  nd[0] = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+  nd[1] = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+  nd[2] = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+  nd[3] = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+  nd[4] = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+  nd[5] = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+  nd[6] = make_float4(__int_as_float(codes[0]), __int_as_float(codes[1]), __int_as_float(codes[2]),
+                      __int_as_float(codes[3]));
+  nd[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_lb_root(int2 *q, uint32_t *nq, uint32_t *n4) {
+  if (threadIdx.x == 0) {
+    q[0] = make_int2(0, 0);
+    *nq = 1u;
+    *n4 = 1u;
+  }
+}
+
+// A single primitive: node 0 with the one leaf.
+__global__ void k_lb_single(const float4 *leafbox, float4 *nodes4) {
+  if (threadIdx.x != 0) return;
+  f3 l, h;
+  child_box(~0, leafbox, nullptr, l, h);
+  pad_box(l, h);
+  nodes4[0] = make_float4(l.x, 1.f, 1.f, 1.f);
+  nodes4[1] = make_float4(h.x, -1.f, -1.f, -1.f);
+  nodes4[2] = make_float4(l.y, 1.f, 1.f, 1.f);
+  nodes4[3] = make_float4(h.y, -1.f, -1.f, -1.f);
+  nodes4[4] = make_float4(l.z, 1.f, 1.f, 1.f);
+  nodes4[5] = make_float4(h.z, -1.f, -1.f, -1.f);
+  nodes4[6] = make_float4(__int_as_float(leaf_code(0u, 1u)), __int_as_float(kEmptyChild),
+                          __int_as_float(kEmptyChild), __int_as_float(kEmptyChild));
+  nodes4[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+static uint32_t lb_blocks(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+static size_t sort_bytes(uint32_t n) {
+  size_t tb = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 30);
+  return tb;
+}
+
+// Scratch layout: every array starts on a 256-byte boundary.
+template <class F>
+static size_t lbvh_layout(uint32_t n, F place) {
+  const size_t m = n > 1 ? n - 1 : 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  place(0, take(2 * (size_t)n * sizeof(float4)));
+  place(1, take(2 * (size_t)n * sizeof(float4)));
+  place(2, take(2 * m * sizeof(float4)));
+  for (int k = 0; k < 4; ++k) place(3 + k, take((size_t)n * sizeof(uint32_t)));
+  place(7, take(m * sizeof(int2)));
+  place(8, take(m * sizeof(int2)));
+  place(9, take(m * sizeof(int)));
+  place(10, take((size_t)n * sizeof(int)));
+  place(11, take(m * sizeof(uint32_t)));
+  place(12, take(m * sizeof(int2)));
+  place(13, take(m * sizeof(int2)));
+  place(14, take(12 * sizeof(uint32_t)));
+  place(15, take(4 * sizeof(uint32_t)));
+  place(16, take(sort_bytes(n)));
+  return off;
+}
+
+size_t lbvh_scratch_bytes(uint32_t n) {
+  return lbvh_layout(n, [](int, size_t) {});
+}
+
+void lbvh_carve(void *base, uint32_t n, LbvhScratch &sc) {
+  char *b = static_cast<char *>(base);
+  lbvh_layout(n, [&](int k, size_t o) {
+    void *p = b + o;
+    switch (k) {
+      case 0: sc.box = (float4 *)p; break;
+      case 1: sc.leafbox = (float4 *)p; break;
+      case 2: sc.nodebox = (float4 *)p; break;
+      case 3: sc.key = (uint32_t *)p; break;
+      case 4: sc.key2 = (uint32_t *)p; break;
+      case 5: sc.idx = (uint32_t *)p; break;
+      case 6: sc.idx2 = (uint32_t *)p; break;
+      case 7: sc.child = (int2 *)p; break;
+      case 8: sc.range = (int2 *)p; break;
+      case 9: sc.parent_int = (int *)p; break;
+      case 10: sc.parent_leaf = (int *)p; break;
+      case 11: sc.flags = (uint32_t *)p; break;
+      case 12: sc.q0 = (int2 *)p; break;
+      case 13: sc.q1 = (int2 *)p; break;
+      case 14: sc.cb = (uint32_t *)p; break;
+      case 15: sc.counts = (uint32_t *)p; break;
+      default: sc.sort_tmp = p; break;
+    }
+  });
+  sc.sort_bytes = sort_bytes(n);
+}
+
+static float ordered_to_float(uint32_t u) {
+  return __builtin_bit_cast(float, (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t leaf_size, LbvhScratch &sc,
+                      float4 *prims_out, float4 *nodes4, LbvhResult &res) {
+  hipError_t e;
+  hipLaunchKernelGGL(k_lb_init_bounds, dim3(1), dim3(64), 0, st, sc.cb);
+  hipLaunchKernelGGL(k_lb_bounds, dim3(lb_blocks(n)), dim3(256), 0, st, prims, n, sc.box, sc.cb);
+  hipLaunchKernelGGL(k_lb_morton, dim3(lb_blocks(n)), dim3(256), 0, st, sc.box, n, sc.cb, sc.key, sc.idx);
+  size_t tb = sc.sort_bytes;
+  e = hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tb, sc.key, sc.key2, sc.idx, sc.idx2, (int)n, 0, 30, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lb_reorder, dim3(lb_blocks(n)), dim3(256), 0, st, prims, sc.box, sc.idx2, n, prims_out,
+                     sc.leafbox);
+  res.root = 0;
+  if (n < 2) {
+    hipLaunchKernelGGL(k_lb_single, dim3(1), dim3(64), 0, st, sc.leafbox, nodes4);
+    res.nodes4 = 1;
+    res.depth4 = 1;
+  } else {
+    e = hipMemsetAsync(sc.flags, 0, (size_t)(n - 1) * sizeof(uint32_t), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lb_karras, dim3(lb_blocks(n - 1)), dim3(256), 0, st, sc.key2, (int)n, sc.child,
+                       sc.range, sc.parent_int, sc.parent_leaf);
+    hipLaunchKernelGGL(k_lb_refit, dim3(lb_blocks(n)), dim3(256), 0, st, (int)n, sc.child, sc.parent_int,
+                       sc.parent_leaf, sc.leafbox, sc.nodebox, sc.flags);
+    hipLaunchKernelGGL(k_lb_root, dim3(1), dim3(64), 0, st, sc.q0, sc.counts, sc.counts + 2);
+    // one launch per level of 4-wide nodes; the host reads each level's size
+    uint32_t nq = 1, depth = 0;
+    int2 *qa = sc.q0, *qb = sc.q1;
+    while (nq > 0) {
+      e = hipMemsetAsync(sc.counts + 1, 0, sizeof(uint32_t), st);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_lb_collapse, dim3(lb_blocks(nq)), dim3(256), 0, st, qa, nq, sc.child, sc.range,
+                         sc.leafbox, sc.nodebox, leaf_size, nodes4, sc.counts + 2, qb, sc.counts + 1);
+      e = hipMemcpyAsync(&nq, sc.counts + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+      if (e != hipSuccess) return e;
+      e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return e;
+      ++depth;
+      int2 *t = qa;
+      qa = qb;
+      qb = t;
+    }
+    res.depth4 = depth;
+  }
+  uint32_t tail[16];
+  e = hipMemcpyAsync(tail, sc.cb, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  if (n >= 2) {
+    e = hipMemcpyAsync(tail + 12, sc.counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+  }
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  if (n >= 2) res.nodes4 = tail[14];
+  for (int a = 0; a < 3; ++a) {
+    res.lo[a] = ordered_to_float(tail[6 + a]);
+    res.hi[a] = ordered_to_float(tail[9 + a]);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pt

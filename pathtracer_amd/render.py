@@ -13,7 +13,7 @@ from .scenes import SceneSpec, camera_desc, render_cfg
 class Scene:
     """A committed device scene."""
 
-    def __init__(self, spec: SceneSpec, leaf_size: int = 4, sah_bins: int = 16):
+    def __init__(self, spec: SceneSpec, leaf_size: int = 4, sah_bins: int = 16, build_flags: int = 0):
         self.spec = spec
         h = ctypes.c_void_p()
         L.check(L.lib.pt_scene_create(ctypes.byref(h)), "pt_scene_create")
@@ -57,7 +57,7 @@ class Scene:
             L.check(L.lib.pt_scene_set_medium(h, L.ptr(f3(sa)), L.ptr(f3(ss)), scale, L.ptr(f3(le)), les, gph),
                     "pt_scene_set_medium")
         b = L.BuildCfg()
-        b.leaf_size, b.sah_bins = leaf_size, sah_bins
+        b.leaf_size, b.sah_bins, b.flags = leaf_size, sah_bins, build_flags
         L.check(L.lib.pt_scene_commit(h, ctypes.byref(b)), "pt_scene_commit")
         info = L.SceneInfo()
         L.check(L.lib.pt_scene_get_info(h, ctypes.byref(info)), "pt_scene_get_info")
