@@ -143,15 +143,23 @@ __global__ void k_lb_karras(const uint32_t *key, int n, int2 *child, int2 *range
 }
 
 // Boxes of the internal nodes, bottom-up: the second child to arrive at a
-// node computes its box and goes on.
-PT_D void child_box(int c, const float4 *leafbox, const float4 *nodebox, f3 &lo, f3 &hi) {
+// node computes its box and goes on.  The same pass takes the leaf decisions
+// that minimise the surface-area cost of each subtree (traversal and
+// primitive test both cost 1, as in the host build): a subtree of at most
+// leaf_size primitives becomes one leaf when that costs no more than its best
+// split.  lo.w of a node's box holds the subtree's cost, hi.w 1 for a leaf.
+PT_D void child_box(int c, const float4 *leafbox, const float4 *nodebox, float4 &lo, float4 &hi) {
   const float4 *b = c < 0 ? leafbox + 2 * (size_t)(~c) : nodebox + 2 * (size_t)c;
-  const float4 l = b[0], h = b[1];
-  lo = mk3(l.x, l.y, l.z);
-  hi = mk3(h.x, h.y, h.z);
+  lo = b[0];
+  hi = b[1];
 }
-__global__ void k_lb_refit(int n, const int2 *child, const int *parent_int, const int *parent_leaf,
-                           const float4 *leafbox, float4 *nodebox, uint32_t *flags) {
+PT_D float half_area(const float4 &lo, const float4 &hi) {
+  const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  return dx * dy + dy * dz + dz * dx;
+}
+__global__ void k_lb_refit(int n, const int2 *child, const int2 *range, const int *parent_int,
+                           const int *parent_leaf, const float4 *leafbox, float4 *nodebox, uint32_t *flags,
+                           uint32_t leaf_size) {
   const int k = (int)l_tid();
   if (k >= n) return;
   int p = parent_leaf[k];
@@ -161,11 +169,19 @@ __global__ void k_lb_refit(int n, const int2 *child, const int *parent_int, cons
     if (atomicAdd(&flags[p], 1u) == 0u) return;  // the other child is not done yet
     __threadfence();
     const int2 c = child[p];
-    f3 l0, h0, l1, h1;
+    float4 l0, h0, l1, h1;
     child_box(c.x, leafbox, nodebox, l0, h0);
     child_box(c.y, leafbox, nodebox, l1, h1);
-    nodebox[2 * (size_t)p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
-    nodebox[2 * (size_t)p + 1] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+    const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    const float c0 = c.x < 0 ? 1.f : l0.w, c1 = c.y < 0 ? 1.f : l1.w;
+    const float ap = half_area(lo, hi);
+    const float split = 1.f + (ap > 0.f ? (half_area(l0, h0) * c0 + half_area(l1, h1) * c1) / ap : c0 + c1);
+    const int2 r = range[p];
+    const int size = r.y - r.x + 1;
+    const bool leaf = size <= (int)leaf_size && (float)size <= split;
+    nodebox[2 * (size_t)p] = make_float4(lo.x, lo.y, lo.z, leaf ? (float)size : split);
+    nodebox[2 * (size_t)p + 1] = make_float4(hi.x, hi.y, hi.z, leaf ? 1.f : 0.f);
     p = p == 0 ? -1 : parent_int[p];
   }
 }
@@ -182,28 +198,37 @@ __global__ void k_lb_reorder(const float4 *prims, const float4 *box, const uint3
 }
 
 // One level of the collapse into 4-wide nodes.  Work item w: binary internal
-// node q[w].x becomes 4-wide node q[w].y; its children, with internal children
-// of more than leaf_size primitives replaced by their own two children, become
-// the node's up to 4 slots.
+// node q[w].x becomes 4-wide node q[w].y.  Its slots start as the node's two
+// children; while fewer than four, the largest-area child that is not a leaf
+// is replaced by its own two children (the host collapse's rule).  Children
+// that stay internal become the next level's work items.
 __global__ void k_lb_collapse(const int2 *qin, uint32_t nq, const int2 *child, const int2 *range,
-                              const float4 *leafbox, const float4 *nodebox, uint32_t leaf_size, float4 *nodes4,
-                              uint32_t *n4, int2 *qout, uint32_t *nqout) {
+                              const float4 *leafbox, const float4 *nodebox, float4 *nodes4, uint32_t *n4,
+                              int2 *qout, uint32_t *nqout) {
   const uint32_t w = l_tid();
   if (w >= nq) return;
   const int2 it = qin[w];
+  auto is_leaf = [&](int x) { return x < 0 || nodebox[2 * (size_t)x + 1].w != 0.f; };
   int items[4];
-  int ni = 0;
   const int2 c = child[it.x];
-  const int cc[2] = {c.x, c.y};
-  auto size_of = [&](int x) { return x < 0 ? 1 : range[x].y - range[x].x + 1; };
-  for (int k = 0; k < 2; ++k) {
-    const int x = cc[k];
-    if (x >= 0 && size_of(x) > (int)leaf_size) {
-      items[ni++] = child[x].x;
-      items[ni++] = child[x].y;
-    } else {
-      items[ni++] = x;
+  items[0] = c.x;
+  items[1] = c.y;
+  int ni = 2;
+  while (ni < 4) {
+    int best = -1;
+    float ba = -1.f;
+    for (int k = 0; k < ni; ++k) {
+      if (is_leaf(items[k])) continue;
+      const float a = half_area(nodebox[2 * (size_t)items[k]], nodebox[2 * (size_t)items[k] + 1]);
+      if (a > ba) {
+        ba = a;
+        best = k;
+      }
     }
+    if (best < 0) break;
+    const int2 g = child[items[best]];
+    items[best] = g.x;
+    items[ni++] = g.y;
   }
   float lo[3][4], hi[3][4];
   int codes[4];
@@ -217,14 +242,15 @@ __global__ void k_lb_collapse(const int2 *qin, uint32_t nq, const int2 *child, c
       continue;
     }
     const int x = items[k];
-    f3 l, h;
-    child_box(x, leafbox, nodebox, l, h);
+    float4 l4, h4;
+    child_box(x, leafbox, nodebox, l4, h4);
+    f3 l = mk3(l4.x, l4.y, l4.z), h = mk3(h4.x, h4.y, h4.z);
     pad_box(l, h);
     lo[0][k] = l.x; lo[1][k] = l.y; lo[2][k] = l.z;
     hi[0][k] = h.x; hi[1][k] = h.y; hi[2][k] = h.z;
-    const int sz = size_of(x);
-    if (x < 0 || sz <= (int)leaf_size) {
+    if (is_leaf(x)) {
       const int first = x < 0 ? ~x : range[x].x;
+      const int sz = x < 0 ? 1 : range[x].y - range[x].x + 1;
       codes[k] = leaf_code((uint32_t)first, (uint32_t)sz);
     } else {
       const uint32_t j = atomicAdd(n4, 1u);
@@ -255,8 +281,8 @@ __global__ void k_lb_root(int2 *q, uint32_t *nq, uint32_t *n4) {
 // A single primitive: node 0 with the one leaf.
 __global__ void k_lb_single(const float4 *leafbox, float4 *nodes4) {
   if (threadIdx.x != 0) return;
-  f3 l, h;
-  child_box(~0, leafbox, nullptr, l, h);
+  const float4 l4 = leafbox[0], h4 = leafbox[1];
+  f3 l = mk3(l4.x, l4.y, l4.z), h = mk3(h4.x, h4.y, h4.z);
   pad_box(l, h);
   nodes4[0] = make_float4(l.x, 1.f, 1.f, 1.f);
   nodes4[1] = make_float4(h.x, -1.f, -1.f, -1.f);
@@ -361,8 +387,8 @@ hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_lb_karras, dim3(lb_blocks(n - 1)), dim3(256), 0, st, sc.key2, (int)n, sc.child,
                        sc.range, sc.parent_int, sc.parent_leaf);
-    hipLaunchKernelGGL(k_lb_refit, dim3(lb_blocks(n)), dim3(256), 0, st, (int)n, sc.child, sc.parent_int,
-                       sc.parent_leaf, sc.leafbox, sc.nodebox, sc.flags);
+    hipLaunchKernelGGL(k_lb_refit, dim3(lb_blocks(n)), dim3(256), 0, st, (int)n, sc.child, sc.range,
+                       sc.parent_int, sc.parent_leaf, sc.leafbox, sc.nodebox, sc.flags, leaf_size);
     hipLaunchKernelGGL(k_lb_root, dim3(1), dim3(64), 0, st, sc.q0, sc.counts, sc.counts + 2);
     // one launch per level of 4-wide nodes; the host reads each level's size
     uint32_t nq = 1, depth = 0;
@@ -371,7 +397,7 @@ hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t 
       e = hipMemsetAsync(sc.counts + 1, 0, sizeof(uint32_t), st);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_lb_collapse, dim3(lb_blocks(nq)), dim3(256), 0, st, qa, nq, sc.child, sc.range,
-                         sc.leafbox, sc.nodebox, leaf_size, nodes4, sc.counts + 2, qb, sc.counts + 1);
+                         sc.leafbox, sc.nodebox, nodes4, sc.counts + 2, qb, sc.counts + 1);
       e = hipMemcpyAsync(&nq, sc.counts + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
       if (e != hipSuccess) return e;
       e = hipStreamSynchronize(st);
