@@ -10,7 +10,8 @@ namespace pt {
 // lbvh_scratch_bytes(n) bytes.
 struct LbvhScratch {
   float4 *box, *leafbox, *nodebox;   // per-primitive, sorted-primitive and internal-node boxes (2 float4 each)
-  uint32_t *key, *key2, *idx, *idx2; // Morton codes and primitive indices, before and after the sort
+  uint64_t *key, *key2;              // Morton codes, before and after the sort
+  uint32_t *idx, *idx2;              // primitive indices, before and after the sort
   int2 *child, *range;               // binary tree: children and primitive range of each internal node
   int *parent_int, *parent_leaf;     // parent of each internal node / sorted primitive
   uint32_t *flags;                   // refit arrival counters, one per internal node

@@ -1,5 +1,5 @@
 // pt_lbvh.hip -- on-device BVH build (linear BVH, Karras 2012) for scenes
-// whose instances change often: primitive bounds -> 30-bit Morton codes of
+// whose instances change often: primitive bounds -> 63-bit Morton codes of
 // the centroids -> radix sort -> the binary radix tree of the sorted codes
 // (one thread per internal node) -> bottom-up box refit -> collapse into the
 // renderer's 4-wide nodes, level by level, with subtrees of at most leaf_size
@@ -79,16 +79,19 @@ __global__ void k_lb_init_bounds(uint32_t *cb) {
   if (i < 12) cb[i] = ((i % 6) < 3) ? 0xffffffffu : 0u;
 }
 
-PT_D uint32_t spread10(uint32_t v) {  // 10 bits -> every third bit
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
+PT_D uint64_t spread21(uint64_t v) {  // 21 bits -> every third bit
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
   return v;
 }
 
-__global__ void k_lb_morton(const float4 *box, uint32_t n, const uint32_t *cb, uint32_t *code, uint32_t *idx) {
+// 63-bit Morton code of the centroid in the centroid bounds (21 bits per
+// axis: a small object next to a large one still gets distinct codes).
+__global__ void k_lb_morton(const float4 *box, uint32_t n, const uint32_t *cb, uint64_t *code, uint32_t *idx) {
   const uint32_t i = l_tid();
   if (i >= n) return;
   const float4 lo = box[2 * (size_t)i], hi = box[2 * (size_t)i + 1];
@@ -97,24 +100,24 @@ __global__ void k_lb_morton(const float4 *box, uint32_t n, const uint32_t *cb, u
   auto q = [](float v, float a, float b) {
     const float e = b - a;
     const float t = e > 0.f ? (v - a) / e : 0.5f;
-    return (uint32_t)fminf(fmaxf(t * 1024.f, 0.f), 1023.f);
+    return (uint64_t)fminf(fmaxf(t * 2097152.f, 0.f), 2097151.f);
   };
-  code[i] = (spread10(q(c.x, mn.x, mx.x)) << 2) | (spread10(q(c.y, mn.y, mx.y)) << 1) | spread10(q(c.z, mn.z, mx.z));
+  code[i] = (spread21(q(c.x, mn.x, mx.x)) << 2) | (spread21(q(c.y, mn.y, mx.y)) << 1) | spread21(q(c.z, mn.z, mx.z));
   idx[i] = i;
 }
 
 // delta of Karras 2012: common prefix length of sorted keys i and j, with the
 // index as a tie-break for equal keys; -1 outside the array.
-PT_D int delta(const uint32_t *k, int n, int i, int j) {
+PT_D int delta(const uint64_t *k, int n, int i, int j) {
   if (j < 0 || j >= n) return -1;
-  const uint32_t a = k[i], b = k[j];
-  if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-  return __clz(a ^ b);
+  const uint64_t a = k[i], b = k[j];
+  if (a == b) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
+  return __clzll(a ^ b);
 }
 
 // Internal node i of n - 1: its range and children.  Child codes: >= 0
 // internal node, < 0 leaf ~k (sorted primitive k).
-__global__ void k_lb_karras(const uint32_t *key, int n, int2 *child, int2 *range, int *parent_int,
+__global__ void k_lb_karras(const uint64_t *key, int n, int2 *child, int2 *range, int *parent_int,
                             int *parent_leaf) {
   const int i = (int)l_tid();
   if (i >= n - 1) return;
@@ -163,8 +166,9 @@ __global__ void k_lb_refit(int n, const int2 *child, const int2 *range, const in
   const int k = (int)l_tid();
   if (k >= n) return;
   int p = parent_leaf[k];
-  // a radix tree over 30-bit codes with index tie-breaks is at most 62 deep
-  for (int level = 0; level < 64 && p >= 0 && p < n - 1; ++level) {
+  // a radix tree over 63-bit codes with index tie-breaks over n < 2^27
+  // primitives is at most 90 deep
+  for (int level = 0; level < 96 && p >= 0 && p < n - 1; ++level) {
     __threadfence();
     if (atomicAdd(&flags[p], 1u) == 0u) return;  // the other child is not done yet
     __threadfence();
@@ -299,8 +303,8 @@ static uint32_t lb_blocks(size_t n) { return (uint32_t)((n + 255) / 256); }
 
 static size_t sort_bytes(uint32_t n) {
   size_t tb = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                           (uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 30);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                           (uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 63);
   return tb;
 }
 
@@ -317,7 +321,10 @@ static size_t lbvh_layout(uint32_t n, F place) {
   place(0, take(2 * (size_t)n * sizeof(float4)));
   place(1, take(2 * (size_t)n * sizeof(float4)));
   place(2, take(2 * m * sizeof(float4)));
-  for (int k = 0; k < 4; ++k) place(3 + k, take((size_t)n * sizeof(uint32_t)));
+  place(3, take((size_t)n * sizeof(uint64_t)));
+  place(4, take((size_t)n * sizeof(uint64_t)));
+  place(5, take((size_t)n * sizeof(uint32_t)));
+  place(6, take((size_t)n * sizeof(uint32_t)));
   place(7, take(m * sizeof(int2)));
   place(8, take(m * sizeof(int2)));
   place(9, take(m * sizeof(int)));
@@ -343,8 +350,8 @@ void lbvh_carve(void *base, uint32_t n, LbvhScratch &sc) {
       case 0: sc.box = (float4 *)p; break;
       case 1: sc.leafbox = (float4 *)p; break;
       case 2: sc.nodebox = (float4 *)p; break;
-      case 3: sc.key = (uint32_t *)p; break;
-      case 4: sc.key2 = (uint32_t *)p; break;
+      case 3: sc.key = (uint64_t *)p; break;
+      case 4: sc.key2 = (uint64_t *)p; break;
       case 5: sc.idx = (uint32_t *)p; break;
       case 6: sc.idx2 = (uint32_t *)p; break;
       case 7: sc.child = (int2 *)p; break;
@@ -373,7 +380,7 @@ hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t 
   hipLaunchKernelGGL(k_lb_bounds, dim3(lb_blocks(n)), dim3(256), 0, st, prims, n, sc.box, sc.cb);
   hipLaunchKernelGGL(k_lb_morton, dim3(lb_blocks(n)), dim3(256), 0, st, sc.box, n, sc.cb, sc.key, sc.idx);
   size_t tb = sc.sort_bytes;
-  e = hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tb, sc.key, sc.key2, sc.idx, sc.idx2, (int)n, 0, 30, st);
+  e = hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tb, sc.key, sc.key2, sc.idx, sc.idx2, (int)n, 0, 63, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lb_reorder, dim3(lb_blocks(n)), dim3(256), 0, st, prims, sc.box, sc.idx2, n, prims_out,
                      sc.leafbox);
