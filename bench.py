@@ -282,6 +282,67 @@ def guided_quality(scenes, R, L, q):
     return curve, target, t_need, ratio
 
 
+def main_churn(args):
+    """Geometry that moves every frame: the glass_hdri frame (1.31M triangles,
+    1024x1024) at 16 spp, whose icosphere turns about the vertical axis
+    between frames.  One step = pt_scene_update_mesh + pt_scene_commit (BVH
+    rebuilt) + the render.  Both builds are timed on the same frames; `value`
+    is the on-device build's path samples per second, rebuild included."""
+    import numpy as np
+    from pathtracer_amd import _lib as L, render as R, scenes
+    spec = scenes.glass_hdri(spp_x=4, spp_y=4)
+    if args.small:
+        spec = scenes.glass_hdri(width=256, height=256, spp_x=2, spp_y=2, subdiv=6)
+    pos0 = spec.meshes[0][0]
+    n = args.warmup + args.steps
+    frames = []
+    for f in range(n):
+        a = 0.05 * (f + 1)
+        rot = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]], np.float32)
+        frames.append(np.ascontiguousarray(pos0 @ rot.T, np.float32))
+    out = {}
+    for build in ("lbvh", "sah"):
+        flags = L.BUILD_DEVICE_LBVH if build == "lbvh" else 0
+        sc = R.Scene(spec, build_flags=flags)
+        cfg = sc.cfg(estimator=args.estimator)
+        total = cfg.width * cfg.height * cfg.spp_x * cfg.spp_y
+        parts = {"update_ms": 0.0, "commit_ms": 0.0, "bvh_ms": 0.0, "render_ms": 0.0}
+        t_all = 0.0
+        for f in range(n):
+            L.check(L.lib.pt_device_synchronize(), "sync")
+            t0 = time.perf_counter()
+            sc.update_mesh(0, frames[f])
+            t1 = time.perf_counter()
+            sc.commit(build_flags=flags)
+            t2 = time.perf_counter()
+            sc.render_device(cfg)
+            L.check(L.lib.pt_device_synchronize(), "sync")
+            t3 = time.perf_counter()
+            if f >= args.warmup:
+                parts["update_ms"] += (t1 - t0) * 1e3
+                parts["commit_ms"] += (t2 - t1) * 1e3
+                parts["bvh_ms"] += sc.info.bvh_ms
+                parts["render_ms"] += (t3 - t2) * 1e3
+                t_all += t3 - t0
+        out[build] = {"value": round(total * args.steps / t_all / 1e6, 2),
+                      "ms_per_step": round(t_all / args.steps * 1e3, 2),
+                      **{k: round(v / args.steps, 2) for k, v in parts.items()}}
+        log(f"{build}: {out[build]}")
+        sc.close()
+    res = {
+        "metric": METRIC, "value": out["lbvh"]["value"], "unit": "Mpath-samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": out["lbvh"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "churn: glass_hdri with the 1.31M-tri icosphere turned every frame, 1024x1024, "
+                               "16 spp; update + commit (BVH rebuild) + render per step",
+                   "width": cfg.width, "height": cfg.height, "spp": cfg.spp_x * cfg.spp_y,
+                   "triangles": spec.n_triangles, "estimator": ["reference_nee", "nee_mis"][args.estimator]},
+        "device_lbvh": out["lbvh"], "host_sah": out["sah"],
+        "speedup_vs_host_build": round(out["sah"]["ms_per_step"] / out["lbvh"]["ms_per_step"], 3),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main_guided(args):
     """The slit-lit room (4x3x3 Lambertian room, Disney floor, 12.5k-triangle
     displaced conductor statue, a quad light of radiance 200 seen through a
@@ -377,10 +438,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--small", action="store_true", help="256x256 quick check (not a bench line)")
     ap.add_argument("--save", default="", help="write the film as PFM/PNG with this path prefix")
-    ap.add_argument("--workload", default="glass_hdri", choices=("glass_hdri", "batch64", "guided_room"),
+    ap.add_argument("--workload", default="glass_hdri", choices=("glass_hdri", "batch64", "guided_room", "churn"),
                     help="glass_hdri: the single triangle-heavy frame (default); batch64: 64 uneven renders "
                          "in one wavefront against 64 serial renders; guided_room: the slit-lit room with the "
-                         "learned guide against the plain tracer")
+                         "learned guide against the plain tracer; churn: glass_hdri with the mesh moved and the "
+                         "BVH rebuilt every frame, device build against host build")
     ap.add_argument("--no-tune", action="store_true", help="batch64: skip the autotuner")
     ap.add_argument("--build", default="sah", choices=("sah", "lbvh"),
                     help="BVH build: sah (host binned SAH, default) or lbvh (on the device)")
@@ -395,6 +457,8 @@ def main():
         return main_batch(args)
     if args.workload == "guided_room":
         return main_guided(args)
+    if args.workload == "churn":
+        return main_churn(args)
     from pathtracer_amd import _lib as L, render as R, scenes
 
     t0 = time.perf_counter()

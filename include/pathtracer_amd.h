@@ -195,6 +195,13 @@ pt_status pt_scene_add_mesh_normals(pt_scene *scene, const float *pos, uint32_t 
                                     const float *nrm, uint32_t nnormals,
                                     const int32_t *nidx, uint32_t material,
                                     uint32_t flags);
+/* Replaces the vertex positions of mesh `mesh` (meshes are numbered in the
+ * order they were added, from 0) with pos (nverts*3 floats, the vertex count
+ * it was added with; its index buffer and shading normals are kept).  Takes
+ * effect at the next pt_scene_commit.  No reference counterpart: the
+ * reference builds its scene once per render (render/renderManager.hpp:80-81);
+ * this is for geometry that moves between frames, with PT_BUILD_DEVICE_LBVH. */
+pt_status pt_scene_update_mesh(pt_scene *scene, uint32_t mesh, const float *pos, uint32_t nverts);
 /* Sphere(center, radius, material)  (base/primitive.hpp:81-203)           */
 pt_status pt_scene_add_sphere(pt_scene *scene, const float center[3],
                               float radius, uint32_t material, uint32_t flags);
@@ -253,8 +260,8 @@ typedef struct pt_scene_info {
   uint32_t bvh_nodes, bvh_depth;
   uint64_t device_bytes;
   double build_ms;  /* the whole commit                                       */
-  double bvh_ms;    /* the BVH build alone (with PT_BUILD_DEVICE_LBVH: device
-                       kernels, waited for)                                   */
+  double bvh_ms;    /* the BVH build alone (with PT_BUILD_DEVICE_LBVH: the
+                       records' upload and the device kernels, waited for)    */
 } pt_scene_info;
 pt_status pt_scene_get_info(const pt_scene *scene, pt_scene_info *info);
 

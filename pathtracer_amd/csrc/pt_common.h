@@ -57,6 +57,30 @@ struct DevBuf {
   T *as() const { return reinterpret_cast<T *>(p); }
 };
 
+// Page-locked host memory (hipHostMalloc), for staging large uploads.
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    hipError_t e = hipHostMalloc(&p, n == 0 ? 256 : n, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
 static inline hipError_t upload(DevBuf &b, const void *src, size_t n) {
   hipError_t e = b.alloc(n);
   if (e != hipSuccess) return e;

@@ -47,7 +47,11 @@ struct WaveStore {
 
 // ------------------------------------------------------------------ host-side scene records
 struct HTri { f3 p[3]; uint32_t mesh; int smooth; };  // smooth: index into pt_scene::normals3 or -1
-struct HMesh { uint32_t first, count, mat, flags, seq; };
+struct HMesh {
+  uint32_t first, count, mat, flags, seq;
+  uint32_t nverts = 0;
+  std::vector<int32_t> idx;  // the index buffer, for position updates
+};
 struct HSphere { f3 c; float r; uint32_t mat, flags, seq; int light; };
 struct HRect { int axis; float k, d0min, d0max, d1min, d1max; uint32_t mat, flags, seq; int light; };
 struct HCurve { f3 cps[4]; float w0, w1; uint32_t mat, flags; };
@@ -64,6 +68,8 @@ struct pt_scene {
   std::vector<float> env;
   uint32_t env_w = 0, env_h = 0;
   float env_scale = 1.f;
+  bool env_dirty = true;      // the environment's device tables need building
+  float env_integral = 0.f;   // of the environment distribution's function
   pt_camera_desc cam{};
   bool has_cam = false;
   bool committed = false;
@@ -75,6 +81,7 @@ struct pt_scene {
       d_env_mcdf;
   DevBuf d_perm, d_perm_off;
   DevBuf lb_in, lb_scratch, lb_nodes;  // device BVH build: input records, scratch, nodes before trimming
+  PinnedBuf lb_host;                   // the input records, staged
   uint32_t perm_seed = 0xffffffffu;
   bool perm_valid = false;
   DevScene ds{};
@@ -148,6 +155,26 @@ extern "C" pt_status pt_scene_add_material(pt_scene *s, const pt_material_desc *
   return PT_OK;
 }
 
+// f(i) for i in [0, n) on up to 16 host threads, in contiguous chunks.
+template <class F>
+static void parallel_for(size_t n, F f) {
+  const size_t kMin = 1u << 16;
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  nt = (unsigned)std::min<size_t>(nt, (n + kMin - 1) / kMin);
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      const size_t b = t * per, e = std::min(n, b + per);
+      for (size_t i = b; i < e; ++i) f(i);
+    });
+  for (auto &x : th) x.join();
+}
+
 static pt_status add_mesh(pt_scene *s, const char *fn, const float *pos, uint32_t nv, const int32_t *idx, uint32_t nt,
                           const float *nrm, uint32_t nn, const int32_t *nidx, uint32_t mat, uint32_t flags) {
   const std::string f(fn);
@@ -181,7 +208,27 @@ static pt_status add_mesh(pt_scene *s, const char *fn, const float *pos, uint32_
     s->tris.push_back(tr);
   }
   m.seq = s->seq++;
-  s->meshes.push_back(m);
+  m.nverts = nv;
+  m.idx.assign(idx, idx + 3 * (size_t)nt);
+  s->meshes.push_back(std::move(m));
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_update_mesh(pt_scene *s, uint32_t mesh, const float *pos, uint32_t nv) {
+  if (!s || (!pos && nv)) return fail(PT_ERR_INVALID, "pt_scene_update_mesh: NULL argument");
+  if (mesh >= s->meshes.size()) return fail(PT_ERR_INVALID, "pt_scene_update_mesh: mesh index out of range");
+  const HMesh &m = s->meshes[mesh];
+  if (nv != m.nverts) return fail(PT_ERR_INVALID, "pt_scene_update_mesh: vertex count differs from the mesh's");
+  for (size_t k = 0; k < 3 * (size_t)nv; ++k)
+    if (!std::isfinite(pos[k])) return fail(PT_ERR_INVALID, "pt_scene_update_mesh: non-finite position");
+  parallel_for(m.count, [&](size_t t) {
+    HTri &tr = s->tris[m.first + t];
+    for (int c = 0; c < 3; ++c) {
+      const int32_t v = m.idx[3 * t + c];
+      tr.p[c] = mk3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
+    }
+  });
   s->committed = false;
   return PT_OK;
 }
@@ -282,6 +329,7 @@ extern "C" pt_status pt_scene_set_env(pt_scene *s, const float *rgb, uint32_t w,
   s->env_w = w;
   s->env_h = h;
   s->env_scale = scale;
+  s->env_dirty = true;
   s->committed = false;
   return PT_OK;
 }
@@ -655,49 +703,52 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   const uint32_t ntri = (uint32_t)s->tris.size(), nsph = (uint32_t)s->spheres.size();
   // -- lights (insertion order; environment last)
   s->lights.clear();
-  for (uint32_t q = 0; q < s->seq; ++q) {
-    for (auto &r : s->rects) {
-      if (r.seq != q) continue;
-      r.light = -1;
-      const pt_material_desc &md = s->mats[r.mat];
-      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f) || (r.flags & PT_PRIM_NOT_LIGHT)) continue;
-      f3 v[4];
-      float k = r.k;
-      if (r.axis == 0) {
-        v[0] = mk3(k, r.d0min, r.d1min); v[1] = mk3(k, r.d0max, r.d1min); v[3] = mk3(k, r.d0min, r.d1max);
-      } else if (r.axis == 1) {
-        v[0] = mk3(r.d0min, k, r.d1min); v[1] = mk3(r.d0max, k, r.d1min); v[3] = mk3(r.d0min, k, r.d1max);
-      } else {
-        v[0] = mk3(r.d0min, r.d1min, k); v[1] = mk3(r.d0max, r.d1min, k); v[3] = mk3(r.d0min, r.d1max, k);
-      }
-      DevLight L{};
-      L.type = LIGHT_QUAD;
-      L.v0 = v[0];
-      L.e1 = sub3(v[1], v[0]);
-      L.e2 = sub3(v[3], v[0]);
-      L.n = norm3(cross3(L.e1, L.e2));
-      L.area = len3(cross3(L.e1, L.e2));
-      L.le = scl3(mk3(md.p[0], md.p[1], md.p[2]), md.p[3]);
-      L.axis = r.axis;
-      L.k = r.k;
-      L.d0min = r.d0min; L.d0max = r.d0max; L.d1min = r.d1min; L.d1max = r.d1max;
-      r.light = (int)s->lights.size();
-      s->lights.push_back(L);
+  auto rect_light = [&](HRect &r) {
+    r.light = -1;
+    const pt_material_desc &md = s->mats[r.mat];
+    if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f) || (r.flags & PT_PRIM_NOT_LIGHT)) return;
+    f3 v[4];
+    float k = r.k;
+    if (r.axis == 0) {
+      v[0] = mk3(k, r.d0min, r.d1min); v[1] = mk3(k, r.d0max, r.d1min); v[3] = mk3(k, r.d0min, r.d1max);
+    } else if (r.axis == 1) {
+      v[0] = mk3(r.d0min, k, r.d1min); v[1] = mk3(r.d0max, k, r.d1min); v[3] = mk3(r.d0min, k, r.d1max);
+    } else {
+      v[0] = mk3(r.d0min, r.d1min, k); v[1] = mk3(r.d0max, r.d1min, k); v[3] = mk3(r.d0min, r.d1max, k);
     }
-    for (auto &sp : s->spheres) {
-      if (sp.seq != q) continue;
-      sp.light = -1;
-      const pt_material_desc &md = s->mats[sp.mat];
-      if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f) || (sp.flags & PT_PRIM_NOT_LIGHT)) continue;
-      DevLight L{};
-      L.type = LIGHT_SPHERE;
-      L.c = sp.c;
-      L.r = sp.r;
-      L.area = (float)(4 * M_PI * sp.r * sp.r);
-      L.le = scl3(mk3(md.p[0], md.p[1], md.p[2]), md.p[3]);
-      sp.light = (int)s->lights.size();
-      s->lights.push_back(L);
-    }
+    DevLight L{};
+    L.type = LIGHT_QUAD;
+    L.v0 = v[0];
+    L.e1 = sub3(v[1], v[0]);
+    L.e2 = sub3(v[3], v[0]);
+    L.n = norm3(cross3(L.e1, L.e2));
+    L.area = len3(cross3(L.e1, L.e2));
+    L.le = scl3(mk3(md.p[0], md.p[1], md.p[2]), md.p[3]);
+    L.axis = r.axis;
+    L.k = r.k;
+    L.d0min = r.d0min; L.d0max = r.d0max; L.d1min = r.d1min; L.d1max = r.d1max;
+    r.light = (int)s->lights.size();
+    s->lights.push_back(L);
+  };
+  auto sphere_light = [&](HSphere &sp) {
+    sp.light = -1;
+    const pt_material_desc &md = s->mats[sp.mat];
+    if (md.kind != PT_MAT_EMITTER || !(md.p[3] > 0.f) || (sp.flags & PT_PRIM_NOT_LIGHT)) return;
+    DevLight L{};
+    L.type = LIGHT_SPHERE;
+    L.c = sp.c;
+    L.r = sp.r;
+    L.area = (float)(4 * M_PI * sp.r * sp.r);
+    L.le = scl3(mk3(md.p[0], md.p[1], md.p[2]), md.p[3]);
+    sp.light = (int)s->lights.size();
+    s->lights.push_back(L);
+  };
+  // rects and spheres each hold increasing sequence numbers: merge them
+  for (size_t ri = 0, si = 0; ri < s->rects.size() || si < s->spheres.size();) {
+    if (si >= s->spheres.size() || (ri < s->rects.size() && s->rects[ri].seq < s->spheres[si].seq))
+      rect_light(s->rects[ri++]);
+    else
+      sphere_light(s->spheres[si++]);
   }
   if (!s->env.empty()) {
     DevLight L{};
@@ -712,35 +763,41 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   s->build_cfg = cfg ? *cfg : pt_build_cfg{};
   s->tuning.leaf_size = B.leaf_size;
   B.bins = (cfg && cfg->sah_bins) ? std::min<uint32_t>(std::max<uint32_t>(cfg->sah_bins, 2u), 64u) : 16u;
-  B.refs.resize(nprim);
+  B.refs.resize(nprim);  // the device build reads the primitive records instead
   std::vector<uint2> info(nprim);
   std::vector<float4> geom(nprim);
-  std::vector<float4> smooth;  // 5 float4 per triangle with shading normals
-  for (uint32_t i = 0; i < ntri; ++i) {
+  // 5 float4 per triangle with shading normals, in triangle order
+  std::vector<uint32_t> smooth_at(ntri);
+  uint32_t n_smooth = 0;
+  for (uint32_t i = 0; i < ntri; ++i) smooth_at[i] = s->tris[i].smooth >= 0 ? n_smooth++ : 0xffffffffu;
+  std::vector<float4> smooth(5 * (size_t)n_smooth);
+  parallel_for(ntri, [&](size_t i) {
     const HTri &t = s->tris[i];
-    BRef &r = B.refs[i];
-    for (int a = 0; a < 3; ++a) {
-      r.lo[a] = std::min(comp3(t.p[0], a), std::min(comp3(t.p[1], a), comp3(t.p[2], a)));
-      r.hi[a] = std::max(comp3(t.p[0], a), std::max(comp3(t.p[1], a), comp3(t.p[2], a)));
+    if (!device_build) {
+      BRef &r = B.refs[i];
+      for (int a = 0; a < 3; ++a) {
+        r.lo[a] = std::min(comp3(t.p[0], a), std::min(comp3(t.p[1], a), comp3(t.p[2], a)));
+        r.hi[a] = std::max(comp3(t.p[0], a), std::max(comp3(t.p[1], a), comp3(t.p[2], a)));
+      }
+      r.prim = (uint32_t)i;
     }
-    r.prim = i;
     const HMesh &m = s->meshes[t.mesh];
     f3 e1 = sub3(t.p[1], t.p[0]), e2 = sub3(t.p[2], t.p[0]);
-    uint32_t sm = 0xffffffffu;
-    if (t.smooth >= 0) {
+    const uint32_t sm = smooth_at[i];
+    if (sm != 0xffffffffu) {
       // p0, e1, e2 for the barycentrics of the hit, then the three corner normals
-      sm = (uint32_t)(smooth.size() / 5);
       const f3 *n = &s->normals3[3 * (size_t)t.smooth];
-      smooth.push_back(make_float4(t.p[0].x, t.p[0].y, t.p[0].z, n[0].x));
-      smooth.push_back(make_float4(e1.x, e1.y, e1.z, n[0].y));
-      smooth.push_back(make_float4(e2.x, e2.y, e2.z, n[0].z));
-      smooth.push_back(make_float4(n[1].x, n[1].y, n[1].z, n[2].x));
-      smooth.push_back(make_float4(n[2].y, n[2].z, 0.f, 0.f));
+      float4 *q = &smooth[5 * (size_t)sm];
+      q[0] = make_float4(t.p[0].x, t.p[0].y, t.p[0].z, n[0].x);
+      q[1] = make_float4(e1.x, e1.y, e1.z, n[0].y);
+      q[2] = make_float4(e2.x, e2.y, e2.z, n[0].z);
+      q[3] = make_float4(n[1].x, n[1].y, n[1].z, n[2].x);
+      q[4] = make_float4(n[2].y, n[2].z, 0.f, 0.f);
     }
     info[i] = make_uint2(m.mat | (PRIM_TRI << 24) | ((m.flags & 3u) << 28), sm);
     f3 ng = norm3(cross3(e1, e2));
     geom[i] = make_float4(ng.x, ng.y, ng.z, 0.f);
-  }
+  });
   for (uint32_t i = 0; i < nsph; ++i) {
     const HSphere &sp = s->spheres[i];
     BRef &r = B.refs[ntri + i];
@@ -825,12 +882,15 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   double bvh_ms = 0.0;
   if (device_build) {
     // -- BVH on the device: records in id order in, leaf order out
-    std::vector<float4> prims(3 * nprim);
-    for (size_t k = 0; k < nprim; ++k) prim_record((uint32_t)k, &prims[3 * k]);
-    HIP_TRY(upload(s->lb_in, prims.data(), prims.size() * sizeof(float4)));
+    const size_t rec_bytes = 3 * nprim * sizeof(float4);
+    HIP_TRY(s->lb_host.alloc(rec_bytes));
+    float4 *prims = s->lb_host.as<float4>();
+    parallel_for(nprim, [&](size_t k) { prim_record((uint32_t)k, &prims[3 * k]); });
+    HIP_TRY(s->lb_in.alloc(rec_bytes));
+    HIP_TRY(hipMemcpyAsync(s->lb_in.p, prims, rec_bytes, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(s->lb_scratch.alloc(lbvh_scratch_bytes((uint32_t)nprim)));
     HIP_TRY(s->lb_nodes.alloc(std::max<size_t>(1, nprim - 1) * 8 * sizeof(float4)));
-    HIP_TRY(s->d_prims.alloc(prims.size() * sizeof(float4)));
+    HIP_TRY(s->d_prims.alloc(rec_bytes));
     LbvhScratch sc{};
     lbvh_carve(s->lb_scratch.p, (uint32_t)nprim, sc);
     LbvhResult res{};
@@ -877,15 +937,16 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
       return fail(PT_ERR_UNSUPPORTED, "pt_scene_commit: BVH too deep for the traversal stack");
     // primitives in leaf order
     std::vector<float4> prims(3 * nprim);
-    for (size_t k = 0; k < nprim; ++k) prim_record(B.refs[k].prim, &prims[3 * k]);
+    parallel_for(nprim, [&](size_t k) { prim_record(B.refs[k].prim, &prims[3 * k]); });
     HIP_TRY(upload(s->d_nodes, nodes4.data(), nodes4.size() * sizeof(float4)));
     HIP_TRY(upload(s->d_prims, prims.data(), prims.size() * sizeof(float4)));
   }
 
   // -- environment tables (2-D piecewise-constant; same construction as the oracle)
+  // (built again only after pt_scene_set_env)
   std::vector<float> efunc, ecdf, emarg, emcdf;
-  float eint = 0.f;
-  if (!s->env.empty()) {
+  const bool env_build = !s->env.empty() && s->env_dirty;
+  if (env_build) {
     uint32_t W = s->env_w, H = s->env_h;
     efunc.resize((size_t)W * H);
     ecdf.resize((size_t)(W + 1) * H);
@@ -922,7 +983,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
       acc += (double)emarg[y] / H;
       emcdf[y + 1] = (float)acc;
     }
-    eint = (float)acc;
+    s->env_integral = (float)acc;
     for (uint32_t y = 1; y <= H; ++y) emcdf[y] = acc > 0.0 ? (float)((double)emcdf[y] / acc) : (float)y / H;
   }
 
@@ -939,11 +1000,12 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     HIP_TRY(s->d_lights.alloc(sizeof(DevLight)));
   else
     HIP_TRY(upload(s->d_lights, s->lights.data(), s->lights.size() * sizeof(DevLight)));
-  if (!s->env.empty()) {
+  if (env_build) {
     HIP_TRY(upload(s->d_env, s->env.data(), s->env.size() * sizeof(float)));
     HIP_TRY(upload(s->d_env_func, efunc.data(), efunc.size() * sizeof(float)));
     HIP_TRY(upload(s->d_env_cdf, ecdf.data(), ecdf.size() * sizeof(float)));
     HIP_TRY(upload(s->d_env_mcdf, emcdf.data(), emcdf.size() * sizeof(float)));
+    s->env_dirty = false;
   }
   DevScene &ds = s->ds;
   ds.nodes = s->d_nodes.as<float4>();
@@ -966,9 +1028,8 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.env_func = ds.has_env ? s->d_env_func.as<float>() : nullptr;
   ds.env_cdf = ds.has_env ? s->d_env_cdf.as<float>() : nullptr;
   ds.env_mcdf = ds.has_env ? s->d_env_mcdf.as<float>() : nullptr;
-  ds.env_integral = eint;
-  s->perm_valid = false;
-  st = ensure_perm(s, 0u);
+  ds.env_integral = s->env_integral;
+  st = ensure_perm(s, s->perm_valid ? s->perm_seed : 0u);  // the tables do not depend on the geometry
   if (st != PT_OK) return st;
 
   auto t1 = std::chrono::steady_clock::now();

@@ -15,9 +15,9 @@ struct LbvhScratch {
   int2 *child, *range;               // binary tree: children and primitive range of each internal node
   int *parent_int, *parent_leaf;     // parent of each internal node / sorted primitive
   uint32_t *flags;                   // refit arrival counters, one per internal node
-  int2 *q0, *q1;                     // collapse work lists: (binary node, 4-wide node)
+  int2 *q0, *q1;                     // collapse work lists: (binary node, 4-wide node); q0 first holds the refit frontier
   uint32_t *cb;                      // centroid and scene bounds as ordered uints (12)
-  uint32_t *counts;                  // [0] unused, [1] next work-list length, [2] 4-wide node count
+  uint32_t *counts;                  // [0] refit frontier length, [1] next work-list length, [2] 4-wide node count
   void *sort_tmp;
   size_t sort_bytes;
 };

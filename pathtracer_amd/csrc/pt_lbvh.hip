@@ -59,19 +59,42 @@ PT_D void pad_box(f3 &lo, f3 &hi) {
   hi = mk3(hi.x + px, hi.y + py, hi.z + pz);
 }
 
-__global__ void k_lb_bounds(const float4 *prims, uint32_t n, float4 *box, uint32_t *cb) {
+// Per-primitive boxes, and the centroid and scene bounds: a block reduces its
+// 256 primitives (wave shuffles, then LDS) before one atomic per bound, so
+// the twelve counters see one atomic per block, not per primitive.
+// cb = [centroid min xyz, centroid max xyz, scene min xyz, scene max xyz].
+__global__ __launch_bounds__(256) void k_lb_bounds(const float4 *prims, uint32_t n, float4 *box, uint32_t *cb) {
   const uint32_t i = l_tid();
-  if (i >= n) return;
-  f3 lo, hi;
-  prim_bounds(prims + 3 * (size_t)i, lo, hi);
-  box[2 * (size_t)i] = make_float4(lo.x, lo.y, lo.z, 0.f);
-  box[2 * (size_t)i + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
-  const f3 c = scl3(add3(lo, hi), 0.5f);
-  // centroid bounds, then the scene bounds: cb = [cmin xyz, cmax xyz, smin xyz, smax xyz]
-  atomicMin(&cb[0], f2o(c.x)); atomicMin(&cb[1], f2o(c.y)); atomicMin(&cb[2], f2o(c.z));
-  atomicMax(&cb[3], f2o(c.x)); atomicMax(&cb[4], f2o(c.y)); atomicMax(&cb[5], f2o(c.z));
-  atomicMin(&cb[6], f2o(lo.x)); atomicMin(&cb[7], f2o(lo.y)); atomicMin(&cb[8], f2o(lo.z));
-  atomicMax(&cb[9], f2o(hi.x)); atomicMax(&cb[10], f2o(hi.y)); atomicMax(&cb[11], f2o(hi.z));
+  uint32_t v[12];
+  for (int k = 0; k < 12; ++k) v[k] = ((k % 6) < 3) ? 0xffffffffu : 0u;  // identities of min / max
+  if (i < n) {
+    f3 lo, hi;
+    prim_bounds(prims + 3 * (size_t)i, lo, hi);
+    box[2 * (size_t)i] = make_float4(lo.x, lo.y, lo.z, 0.f);
+    box[2 * (size_t)i + 1] = make_float4(hi.x, hi.y, hi.z, 0.f);
+    const f3 c = scl3(add3(lo, hi), 0.5f);
+    v[0] = v[3] = f2o(c.x);
+    v[1] = v[4] = f2o(c.y);
+    v[2] = v[5] = f2o(c.z);
+    v[6] = f2o(lo.x); v[7] = f2o(lo.y); v[8] = f2o(lo.z);
+    v[9] = f2o(hi.x); v[10] = f2o(hi.y); v[11] = f2o(hi.z);
+  }
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int k = 0; k < 12; ++k) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)v[k], off);
+      v[k] = ((k % 6) < 3) ? min(v[k], o) : max(v[k], o);
+    }
+  __shared__ uint32_t red[4][12];
+  const uint32_t wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0)
+    for (int k = 0; k < 12; ++k) red[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int k = threadIdx.x;
+    uint32_t r = red[0][k];
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) r = ((k % 6) < 3) ? min(r, red[w][k]) : max(r, red[w][k]);
+    if ((k % 6) < 3) atomicMin(&cb[k], r); else atomicMax(&cb[k], r);
+  }
 }
 
 __global__ void k_lb_init_bounds(uint32_t *cb) {
@@ -160,32 +183,71 @@ PT_D float half_area(const float4 &lo, const float4 &hi) {
   const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
   return dx * dy + dy * dz + dz * dx;
 }
-__global__ void k_lb_refit(int n, const int2 *child, const int2 *range, const int *parent_int,
-                           const int *parent_leaf, const float4 *leafbox, float4 *nodebox, uint32_t *flags,
-                           uint32_t leaf_size) {
+PT_D void refit_node(int p, const int2 *child, const int2 *range, const float4 *leafbox, float4 *nodebox,
+                     uint32_t leaf_size) {
+  const int2 c = child[p];
+  float4 l0, h0, l1, h1;
+  child_box(c.x, leafbox, nodebox, l0, h0);
+  child_box(c.y, leafbox, nodebox, l1, h1);
+  const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+  const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+  const float c0 = c.x < 0 ? 1.f : l0.w, c1 = c.y < 0 ? 1.f : l1.w;
+  const float ap = half_area(lo, hi);
+  const float split = 1.f + (ap > 0.f ? (half_area(l0, h0) * c0 + half_area(l1, h1) * c1) / ap : c0 + c1);
+  const int2 r = range[p];
+  const int size = r.y - r.x + 1;
+  const bool leaf = size <= (int)leaf_size && (float)size <= split;
+  nodebox[2 * (size_t)p] = make_float4(lo.x, lo.y, lo.z, leaf ? (float)size : split);
+  nodebox[2 * (size_t)p + 1] = make_float4(hi.x, hi.y, hi.z, leaf ? 1.f : 0.f);
+}
+
+// Phase 1: the nodes whose primitives all belong to one block's 256 leaves,
+// synchronised inside the block (workgroup-scope fences, no cache flushes).
+// A thread whose next node reaches beyond its block's leaves puts its
+// finished subtree on the frontier list instead.
+__global__ __launch_bounds__(256) void k_lb_refit_local(int n, const int2 *child, const int2 *range,
+                                                        const int *parent_int, const int *parent_leaf,
+                                                        const float4 *leafbox, float4 *nodebox, uint32_t *flags,
+                                                        uint32_t leaf_size, int *frontier, uint32_t *n_frontier) {
   const int k = (int)l_tid();
   if (k >= n) return;
+  const int b0 = (int)(blockIdx.x * blockDim.x), b1 = min(n, b0 + (int)blockDim.x) - 1;
+  int done = ~k;
   int p = parent_leaf[k];
   // a radix tree over 63-bit codes with index tie-breaks over n < 2^27
   // primitives is at most 90 deep
   for (int level = 0; level < 96 && p >= 0 && p < n - 1; ++level) {
-    __threadfence();
-    if (atomicAdd(&flags[p], 1u) == 0u) return;  // the other child is not done yet
-    __threadfence();
-    const int2 c = child[p];
-    float4 l0, h0, l1, h1;
-    child_box(c.x, leafbox, nodebox, l0, h0);
-    child_box(c.y, leafbox, nodebox, l1, h1);
-    const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
-    const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
-    const float c0 = c.x < 0 ? 1.f : l0.w, c1 = c.y < 0 ? 1.f : l1.w;
-    const float ap = half_area(lo, hi);
-    const float split = 1.f + (ap > 0.f ? (half_area(l0, h0) * c0 + half_area(l1, h1) * c1) / ap : c0 + c1);
     const int2 r = range[p];
-    const int size = r.y - r.x + 1;
-    const bool leaf = size <= (int)leaf_size && (float)size <= split;
-    nodebox[2 * (size_t)p] = make_float4(lo.x, lo.y, lo.z, leaf ? (float)size : split);
-    nodebox[2 * (size_t)p + 1] = make_float4(hi.x, hi.y, hi.z, leaf ? 1.f : 0.f);
+    if (r.x < b0 || r.y > b1) {
+      frontier[atomicAdd(n_frontier, 1u)] = done;
+      return;
+    }
+    __threadfence_block();
+    if (atomicAdd(&flags[p], 1u) == 0u) return;  // the other child is not done yet
+    __threadfence_block();
+    refit_node(p, child, range, leafbox, nodebox, leaf_size);
+    done = p;
+    p = p == 0 ? -1 : parent_int[p];
+  }
+}
+
+// Phase 2: from the frontier up, across blocks: the second child to arrive
+// at a node computes it and goes on (device-scope fences around an atomic
+// counter per node).
+__global__ __launch_bounds__(256) void k_lb_refit_global(int n, const int *frontier, uint32_t n_frontier,
+                                                         const int2 *child, const int2 *range,
+                                                         const int *parent_int, const int *parent_leaf,
+                                                         const float4 *leafbox, float4 *nodebox, uint32_t *flags,
+                                                         uint32_t leaf_size) {
+  const uint32_t i = l_tid();
+  if (i >= n_frontier) return;
+  const int f = frontier[i];
+  int p = f < 0 ? parent_leaf[~f] : (f == 0 ? -1 : parent_int[f]);
+  for (int level = 0; level < 96 && p >= 0 && p < n - 1; ++level) {
+    __threadfence();
+    if (atomicAdd(&flags[p], 1u) == 0u) return;
+    __threadfence();
+    refit_node(p, child, range, leafbox, nodebox, leaf_size);
     p = p == 0 ? -1 : parent_int[p];
   }
 }
@@ -394,8 +456,21 @@ hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_lb_karras, dim3(lb_blocks(n - 1)), dim3(256), 0, st, sc.key2, (int)n, sc.child,
                        sc.range, sc.parent_int, sc.parent_leaf);
-    hipLaunchKernelGGL(k_lb_refit, dim3(lb_blocks(n)), dim3(256), 0, st, (int)n, sc.child, sc.range,
-                       sc.parent_int, sc.parent_leaf, sc.leafbox, sc.nodebox, sc.flags, leaf_size);
+    e = hipMemsetAsync(sc.counts, 0, 4 * sizeof(uint32_t), st);
+    if (e != hipSuccess) return e;
+    // the frontier list borrows the first collapse work list (both n - 1 long)
+    int *frontier = reinterpret_cast<int *>(sc.q0);
+    hipLaunchKernelGGL(k_lb_refit_local, dim3(lb_blocks(n)), dim3(256), 0, st, (int)n, sc.child, sc.range,
+                       sc.parent_int, sc.parent_leaf, sc.leafbox, sc.nodebox, sc.flags, leaf_size, frontier,
+                       sc.counts);
+    uint32_t nf = 0;
+    e = hipMemcpyAsync(&nf, sc.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (nf > 0)
+      hipLaunchKernelGGL(k_lb_refit_global, dim3(lb_blocks(nf)), dim3(256), 0, st, (int)n, frontier, nf, sc.child,
+                         sc.range, sc.parent_int, sc.parent_leaf, sc.leafbox, sc.nodebox, sc.flags, leaf_size);
     hipLaunchKernelGGL(k_lb_root, dim3(1), dim3(64), 0, st, sc.q0, sc.counts, sc.counts + 2);
     // one launch per level of 4-wide nodes; the host reads each level's size
     uint32_t nq = 1, depth = 0;

@@ -56,12 +56,22 @@ class Scene:
             f3 = lambda v: np.ascontiguousarray(v, np.float32)
             L.check(L.lib.pt_scene_set_medium(h, L.ptr(f3(sa)), L.ptr(f3(ss)), scale, L.ptr(f3(le)), les, gph),
                     "pt_scene_set_medium")
+        self.commit(leaf_size, sah_bins, build_flags)
+
+    def commit(self, leaf_size: int = 4, sah_bins: int = 16, build_flags: int = 0):
+        """pt_scene_commit: builds the BVH and uploads the scene."""
         b = L.BuildCfg()
         b.leaf_size, b.sah_bins, b.flags = leaf_size, sah_bins, build_flags
-        L.check(L.lib.pt_scene_commit(h, ctypes.byref(b)), "pt_scene_commit")
+        L.check(L.lib.pt_scene_commit(self.h, ctypes.byref(b)), "pt_scene_commit")
         info = L.SceneInfo()
-        L.check(L.lib.pt_scene_get_info(h, ctypes.byref(info)), "pt_scene_get_info")
+        L.check(L.lib.pt_scene_get_info(self.h, ctypes.byref(info)), "pt_scene_get_info")
         self.info = info
+
+    def update_mesh(self, mesh: int, pos: np.ndarray):
+        """pt_scene_update_mesh: new vertex positions for mesh `mesh` (add order);
+        takes effect at the next commit()."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        L.check(L.lib.pt_scene_update_mesh(self.h, mesh, L.ptr(pos), pos.shape[0]), "pt_scene_update_mesh")
 
     def close(self):
         if self.h:

@@ -52,3 +52,25 @@ def test_errors_are_reported_not_thrown():
     assert L.lib.pt_scene_commit(h, None) == 1                            # empty scene
     assert b"no objects" in L.lib.pt_last_error()
     L.lib.pt_scene_destroy(h)
+
+
+def test_update_mesh_checks_its_arguments():
+    import ctypes
+    import numpy as np
+    h = ctypes.c_void_p()
+    assert L.lib.pt_scene_create(ctypes.byref(h)) == 0
+    mid = ctypes.c_uint32()
+    assert L.lib.pt_scene_add_material(h, ctypes.byref(L.material(1, [0.5, 0.5, 0.5])), ctypes.byref(mid)) == 0
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    idx = np.array([[0, 1, 2], [1, 3, 2]], np.int32)
+    assert L.lib.pt_scene_add_mesh(h, L.ptr(pos), 4, L.ptr(idx), 2, 0, 0) == 0
+    assert L.lib.pt_scene_update_mesh(h, 0, L.ptr(pos + 1), 4) == 0
+    assert L.lib.pt_scene_update_mesh(h, 1, L.ptr(pos), 4) == 1
+    assert b"mesh index out of range" in L.lib.pt_last_error()
+    assert L.lib.pt_scene_update_mesh(h, 0, L.ptr(pos[:3].copy()), 3) == 1
+    assert b"vertex count" in L.lib.pt_last_error()
+    bad = pos.copy()
+    bad[2, 1] = np.nan
+    assert L.lib.pt_scene_update_mesh(h, 0, L.ptr(bad), 4) == 1
+    assert b"non-finite" in L.lib.pt_last_error()
+    L.lib.pt_scene_destroy(h)

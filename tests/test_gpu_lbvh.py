@@ -6,8 +6,9 @@ differ only on ties, whose winner depends on traversal order), any-hit must
 agree, and renders must equal those over the host SAH tree sample for sample.
 The cases cover the build's special shapes: one primitive, a handful, all
 centroids coincident (equal Morton codes, so the tree is split by index),
-every primitive type, every leaf size, and a second commit of a grown scene
-on the same handle (scratch reuse).
+every primitive type, every leaf size, a second commit of a grown scene on
+the same handle (scratch reuse), and frames of a moving, deforming mesh
+(pt_scene_update_mesh).
 """
 import ctypes
 
@@ -141,3 +142,26 @@ def test_render_equals_sah_tree(gpu):
     same = np.all(np.abs(sa - sb) <= 1e-5 * np.maximum(np.abs(sa), np.abs(sb)) + 1e-7, axis=-1)
     assert same.mean() > 0.999, same.mean()
     np.testing.assert_allclose(ib, ia, rtol=2e-3, atol=2e-3)
+
+
+def test_animated_mesh_frames(gpu):
+    """Three frames of a moving, deforming mesh: pt_scene_update_mesh, a device
+    re-build, then closest hits against the oracle of that frame's geometry."""
+    from pathtracer_amd import _lib as L
+    spec = _displaced_spec()
+    pos0, idx, mat, flags = spec.meshes[0]
+    g = _scene(spec)
+    rng = np.random.default_rng(21)
+    for f in range(3):
+        a = 0.4 * (f + 1)
+        rot = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+        pos = (pos0 * (1 + 0.05 * np.sin(5 * pos0[:, 1:2] + f))) @ rot.T + [0.1 * f, 0, 0]
+        pos = pos.astype(np.float32)
+        g.update_mesh(0, pos)
+        g.commit(build_flags=L.BUILD_DEVICE_LBVH)
+        spec.meshes[0] = (pos, idx, mat, flags)
+        rays = _rays_towards(rng, 300_000, [-1.2, -1.2, -1.2], [2.5, 1.2, 1.2], ([-4, -1, 3], [4, 4, 6]))
+        gp, gt = g.intersect(rays)
+        cp, ct = O.OracleScene(spec).intersect(rays)
+        assert _compare(gp, gt, cp, ct) <= 60
+        assert (gp >= 0).mean() > 0.5
