@@ -30,6 +30,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mpath-samples/s on 1 MI355X; relMSE vs 4096-spp reference at equal time"
+WORKLOADS = {
+    "glass_hdri": "glass_hdri: 1.31M-tri displaced rough-glass icosphere + copper plane + 1024x512 sky/sun HDRI, "
+                  "thin-lens DoF",
+    "cornell": "cornell: 20-tri Cornell box, 1280-tri gold icosphere, Lambertian sphere, 0.5x0.5 quad light; "
+               "256x256, 16 spp Halton, depth 8",
+    "disney_hair": "disney_hair: 8x8 Disney spheres with seeded parameters + 8 Marschner hair spheres, 4x4 quad "
+                   "light; 1024x1024, 128 spp stratified, depth 12",
+}
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (layout constants of pathtracer_amd/csrc/pt_scene.h)
 B_EXT_RAY = 24 + 8            # ray (o, d) read + hit (t, prim) write, unit stride
@@ -72,9 +80,16 @@ def cpu_baseline(spec, cfg, target_s=15.0):
     t1 = time.perf_counter()
     o.render_range(cfg, start, n_final, threads)
     t = time.perf_counter() - t1
+    # one thread on a sample of about 5 s
+    n1 = max(256, min(n_final, int(n_final / max(t, 1e-3) / max(threads, 1) * 5.0)))
+    t1 = time.perf_counter()
+    o.render_range(cfg, start, n1, 1)
+    t_one = time.perf_counter() - t1
     return {"value": n_final / t / 1e6, "unit": "Mpath-samples/s", "cores": threads, "kind": "port",
             "sample": f"{n_final} consecutive path samples (pixel-major) from the middle of the same frame, "
-                      f"{t:.1f} s on {threads} OpenMP threads; oracle BVH/scene build {build_s:.1f} s untimed"}
+                      f"{t:.1f} s on {threads} OpenMP threads; oracle BVH/scene build {build_s:.1f} s untimed",
+            "single_thread": {"value": n1 / t_one / 1e6, "cores": 1,
+                              "sample": f"the first {n1} of those path samples, {t_one:.1f} s"}}
 
 
 def read_traffic(kernel, estimator):
@@ -438,12 +453,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--small", action="store_true", help="256x256 quick check (not a bench line)")
     ap.add_argument("--save", default="", help="write the film as PFM/PNG with this path prefix")
-    ap.add_argument("--workload", default="glass_hdri", choices=("glass_hdri", "batch64", "guided_room", "churn"),
-                    help="glass_hdri: the single triangle-heavy frame (default); batch64: 64 uneven renders "
+    ap.add_argument("--workload", default="glass_hdri",
+                    choices=("glass_hdri", "cornell", "disney_hair", "batch64", "guided_room", "churn"),
+                    help="glass_hdri: the single triangle-heavy frame (default); cornell: the Cornell box; "
+                         "disney_hair: 64 Disney + 8 hair spheres (material divergence); batch64: 64 uneven renders "
                          "in one wavefront against 64 serial renders; guided_room: the slit-lit room with the "
                          "learned guide against the plain tracer; churn: glass_hdri with the mesh moved and the "
                          "BVH rebuilt every frame, device build against host build")
     ap.add_argument("--no-tune", action="store_true", help="batch64: skip the autotuner")
+    ap.add_argument("--sort", action="store_true", help="order hits by material kind before shading "
+                                                         "(PT_RENDER_SORT_HITS)")
     ap.add_argument("--build", default="sah", choices=("sah", "lbvh"),
                     help="BVH build: sah (host binned SAH, default) or lbvh (on the device)")
     args = ap.parse_args()
@@ -462,10 +481,9 @@ def main():
     from pathtracer_amd import _lib as L, render as R, scenes
 
     t0 = time.perf_counter()
-    if args.small:
-        spec = scenes.glass_hdri(width=256, height=256, spp_x=4, spp_y=4)
-    else:
-        spec = scenes.glass_hdri()
+    small = dict(width=256, height=256, spp_x=4, spp_y=4) if args.small else {}
+    spec = {"glass_hdri": scenes.glass_hdri, "cornell": scenes.cornell,
+            "disney_hair": scenes.disney_hair_grid}[args.workload](**small)
     log(f"scene generated in {time.perf_counter() - t0:.1f} s: {spec.n_triangles} triangles")
     t0 = time.perf_counter()
     scene = R.Scene(spec, build_flags=build_flags(args))
@@ -475,11 +493,12 @@ def main():
     total = cfg.width * cfg.height * cfg.spp_x * cfg.spp_y
 
     # counting frame (untimed): traversal counters for the roofline
-    ccfg = scene.cfg(estimator=args.estimator, flags=L.RENDER_COUNTERS)
+    sort_flag = L.RENDER_SORT_HITS if args.sort else 0
+    ccfg = scene.cfg(estimator=args.estimator, flags=L.RENDER_COUNTERS | sort_flag)
     cst = scene.render_device(ccfg)
     log(f"counting frame: {cst['segments']} extension rays, {cst['shadow_rays']} shadow rays")
 
-    tcfg = scene.cfg(estimator=args.estimator, flags=L.RENDER_TIMING)
+    tcfg = scene.cfg(estimator=args.estimator, flags=L.RENDER_TIMING | sort_flag)
     for i in range(args.warmup):
         scene.render_device(tcfg)
         log(f"warmup {i} done")
@@ -495,7 +514,7 @@ def main():
 
     # the other estimator on the same frame (one warm-up, one timed frame)
     other = 1 - args.estimator
-    ocfg = scene.cfg(estimator=other, flags=L.RENDER_TIMING)
+    ocfg = scene.cfg(estimator=other, flags=L.RENDER_TIMING | sort_flag)
     scene.render_device(ocfg)
     L.check(L.lib.pt_device_synchronize(), "sync")
     t2 = time.perf_counter()
@@ -510,16 +529,18 @@ def main():
     ms = {k: sum(s[k] for s in stats) / args.steps for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate",
                                                               "ms_film")}
     launches = stats[0]["extend_launches"]
-    roofline, per_kernel = kernel_roofline(cst, ms, launches, read_traffic_for=args.estimator)
+    roofline, per_kernel = kernel_roofline(cst, ms, launches,
+                                           read_traffic_for=args.estimator if args.workload == "glass_hdri" else None)
 
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "glass_hdri: 1.31M-tri displaced rough-glass icosphere + copper plane + 1024x512 "
-                               "sky/sun HDRI, thin-lens DoF" if not args.small else "glass_hdri small",
+        "config": {"workload": WORKLOADS[args.workload] + (" (small)" if args.small else ""),
                    "width": cfg.width, "height": cfg.height, "spp": cfg.spp_x * cfg.spp_y,
-                   "path_samples_per_step": total, "max_depth": cfg.max_depth, "sampler": "sobol",
+                   "path_samples_per_step": total, "max_depth": cfg.max_depth,
+                   "sampler": ["halton_owen", "halton_perm", "halton", "sobol", "stratified"][cfg.sampler],
+                   "material_sort": bool(args.sort),
                    "estimator": ["reference_nee", "nee_mis"][args.estimator], "rr_depth": cfg.rr_depth,
                    "triangles": spec.n_triangles},
         "mrays_per_s": round((cst["segments"] + cst["shadow_rays"]) * args.steps / dt / 1e6, 1),
@@ -533,7 +554,7 @@ def main():
         "quality": None,
         "cpu_baseline": None,
     }
-    if not args.no_quality and not args.small:
+    if not args.no_quality and not args.small and args.workload == "glass_hdri":
         res["quality"] = quality(scene, args.estimator)
         log(f"quality: {res['quality']}")
     if args.save:

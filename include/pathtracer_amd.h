@@ -147,7 +147,10 @@ typedef struct pt_render_cfg {
 
 enum {
   PT_RENDER_COUNTERS = 1u << 0,   /* collect traversal counters into pt_stats       */
-  PT_RENDER_NO_SORT = 1u << 1,    /* disable material-key sorting of hits           */
+  PT_RENDER_SORT_HITS = 1u << 1,  /* order each 256-entry tile of hits by material
+                                     kind (misses first) before shading, so a wave
+                                     shades one kind where the tile allows.  Off by
+                                     default: it did not pay on the measured scenes */
   PT_RENDER_ACCUMULATE = 1u << 2, /* add into the film instead of clearing it        */
   PT_RENDER_KEEP_SAMPLES = 1u << 3, /* keep per-sample radiance (pt_film_read_samples) */
   PT_RENDER_TIMING = 1u << 4      /* HIP-event timing of every kernel launch        */
@@ -328,7 +331,7 @@ enum {
 };
 typedef struct pt_batch_cfg {
   uint32_t wavefront;   /* path slots in flight (0 = the batch tuning, else 8M) */
-  uint32_t flags;       /* PT_RENDER_COUNTERS, PT_RENDER_TIMING (eager only),
+  uint32_t flags;       /* PT_RENDER_COUNTERS, PT_RENDER_SORT_HITS, PT_RENDER_TIMING (eager only),
                            PT_BATCH_NO_GRAPH                                    */
 } pt_batch_cfg;
 /* Renders every render of the batch into its own film.  stats sums over

@@ -24,7 +24,7 @@ SAMPLER_SOBOL, SAMPLER_STRATIFIED = 3, 4
 # pt_estimator
 EST_REFERENCE_NEE, EST_NEE_MIS, EST_NAIVE, EST_ROULETTE, EST_RESERVOIR, EST_VOLUME = 0, 1, 2, 3, 4, 5
 # render flags
-RENDER_COUNTERS, RENDER_NO_SORT, RENDER_ACCUMULATE = 1, 2, 4
+RENDER_COUNTERS, RENDER_SORT_HITS, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
 BATCH_NO_GRAPH = 256
 TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF = 1, 2, 4

@@ -1107,6 +1107,7 @@ static pt_status ensure_wave(WaveStore &ws, uint32_t cap) {
   w.cap = cap;
   w.part_cap = (uint32_t)part_cap;
   w.shadow_part_cap = (uint32_t)shadow_part_cap;
+  w.sort_keys = 0u;
   w.rc = nullptr;
   w.rvis = nullptr;
   ws.res.release();
@@ -1414,6 +1415,7 @@ static pt_status render_frame(pt_scene *s, const pt_render_cfg *cfg, pt_stats *s
   pl.count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
   pl.grids = make_grids(s->n_cu, s->tuning);
   Wave wave = s->ws.wave;
+  wave.sort_keys = (cfg->flags & PT_RENDER_SORT_HITS) ? 1u : 0u;
   GuideView gview{};
   uint32_t n_rec = 0;
   if (gp) {
@@ -1811,6 +1813,7 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
   }
   pt_status st = ensure_wave(b->ws, max_paths);
   if (st != PT_OK) return st;
+  b->ws.wave.sort_keys = (flags & PT_RENDER_SORT_HITS) ? 1u : 0u;
   if (est == PT_EST_RESERVOIR) {
     st = ensure_reservoir(b->ws);
     if (st != PT_OK) return st;

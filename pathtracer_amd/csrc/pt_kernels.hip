@@ -245,22 +245,76 @@ PT_D f3 miss_radiance(const DevScene &s, const RenderParams &rp, f3 d) {
 }
 PT_D float power_w(float a, float b) { return (a * a) / (a * a + b * b); }
 
+// ------------------------------------------------------------------ material-key order
+// Shading order of one 256-entry tile: a stable counting sort of the entries
+// by a 4-bit key (the hit's material kind; 0 = miss, 15 = no entry), so that
+// each wave shades one kind where the tile allows.  Entries stay in their
+// tile, so the permuted state loads touch the same cache lines as the
+// unpermuted ones.  Returns the tile index of the entry this thread shades.
+// Every thread of the block calls it (it synchronises the block).
+PT_D uint32_t tile_order16(uint32_t key, uint32_t (*cnt)[16], uint32_t *first, uint16_t *perm) {
+  const uint32_t lane = __lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t rank = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16u; ++k) {
+    const unsigned long long m = __ballot(key == k);
+    if (key == k) rank = (uint32_t)__popcll(m & below);
+    if (lane == 0) cnt[wv][k] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < 16u; ++k) {
+      first[k] = acc;
+      for (uint32_t v = 0; v < nw; ++v) acc += cnt[v][k];
+    }
+  }
+  __syncthreads();
+  uint32_t pos = first[key] + rank;
+  for (uint32_t v = 0; v < wv; ++v) pos += cnt[v][key];
+  perm[pos] = (uint16_t)threadIdx.x;
+  __syncthreads();
+  const uint32_t src = perm[threadIdx.x];
+  __syncthreads();  // the next tile reuses the tables
+  return src;
+}
+
 // EST 0: getRayColor (maxdepth_nee.hpp:9-256).  EST 1: NEE + MIS.
 // Occupancy target of the shading kernel (waves per SIMD): the register
 // allocator spills the rarely used material state to reach it.
 #ifndef PT_SHADE_WAVES
 #define PT_SHADE_WAVES 4
 #endif
+#ifndef PT_SHADE_WAVES_ALL
+#define PT_SHADE_WAVES_ALL 4
+#endif
 template <int EST, uint32_t MS, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PT_SHADE_WAVES))) k_shade(DevScene s0, RenderParams rp0, Wave w, int qi, uint32_t d, BatchView b) {
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_SHADE_WAVES))) k_shade(DevScene s0, RenderParams rp0, Wave w, int qi, uint32_t d, BatchView b) {
   const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
   const PathState &in = w.st[qi];
   const PathState &out = w.st[qi ^ 1];
   uint32_t *cout = w.live(qi ^ 1, part);
+  __shared__ uint32_t o_cnt[4][16], o_first[16];
+  __shared__ uint16_t o_perm[256];
   for (uint32_t i0 = (blockIdx.x / PT_PARTS) * blockDim.x; i0 < n; i0 += nb * blockDim.x) {
     uint32_t i = i0 + threadIdx.x;
     bool active = i < n;
+    if (w.sort_keys) {  // block-uniform
+      uint32_t key = 15u;
+      if (active) {
+        const int pr = in.prim[base + i];
+        key = 0u;
+        if (pr >= 0) {
+          const DevScene &s = BATCH ? b.scenes[b.slot_inst[in.slot[base + i]]] : s0;
+          key = s.mats[s.prim_info[pr].x & 0xffffffu].kind & 15u;
+        }
+      }
+      i = i0 + tile_order16(key, o_cnt, o_first, o_perm);
+      active = i < n;
+    }
     i += base;
     bool cont = false;
     uint32_t slot = 0;

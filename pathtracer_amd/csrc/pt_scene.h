@@ -176,6 +176,7 @@ struct Wave {
   uint32_t grec_stride, grec_verts, gpass;
   uint32_t gverts;  // vertices 0 .. gverts-1 are guided
   uint32_t cap, part_cap, shadow_part_cap;
+  uint32_t sort_keys;  // 1: shading orders each tile by material kind
   PT_HD uint32_t *live(uint32_t b, uint32_t p) const { return counts + (b * PT_PARTS + p) * PT_COUNTER_STRIDE; }
   PT_HD uint32_t *shadow(uint32_t p) const { return counts + (2 * PT_PARTS + p) * PT_COUNTER_STRIDE; }
 };

@@ -150,3 +150,21 @@ def test_hair_curves_per_sample(gpu, estimator):
     _, gs, _, os_, st = _per_sample(spec, estimator=estimator, rr_depth=0)
     assert _sample_agreement(gs, os_) > 0.99
     assert st["nan_samples"] == 0
+
+
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_sorted_shading_same_samples(gpu, estimator):
+    """PT_RENDER_SORT_HITS only changes which lane shades which path: every
+    sample's radiance is the same as in queue order (up to the order of the
+    float atomics of a vertex's several shadow rays)."""
+    from pathtracer_amd import render as R, scenes, _lib as L
+    for spec in (scenes.cornell(width=48, height=48, spp_x=2, spp_y=2),
+                 scenes.disney_hair_grid(width=64, height=64, spp_x=2, spp_y=2)):
+        g = R.Scene(spec)
+        cfg = g.cfg(flags=L.RENDER_KEEP_SAMPLES, estimator=estimator)
+        g.render(cfg)
+        a = g.samples(cfg)
+        cfg = g.cfg(flags=L.RENDER_KEEP_SAMPLES | L.RENDER_SORT_HITS, estimator=estimator)
+        g.render(cfg)
+        b = g.samples(cfg)
+        np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-7)
