@@ -289,13 +289,17 @@ typedef struct pt_tuning {
   uint32_t shade_blocks_per_cu;  /* shading grid cap, blocks per CU (0 = 32)     */
   uint32_t leaf_size;            /* BVH leaf size of the last commit (autotuner
                                     output; pt_scene_set_tuning ignores it)     */
+  uint32_t sort_bits;            /* bits of the material key shading orders its
+                                    tiles by: 0 = queue order, 4 = material kind
+                                    (as PT_RENDER_SORT_HITS, which forces it)   */
 } pt_tuning;
 pt_status pt_scene_set_tuning(pt_scene *scene, const pt_tuning *tuning);
 
 enum {
   PT_TUNE_WAVEFRONT = 1u << 0,   /* try wavefront sizes 1M..16M                  */
   PT_TUNE_GRIDS = 1u << 1,       /* try traversal / shading grid sizes           */
-  PT_TUNE_LEAF = 1u << 2         /* rebuild the BVH with leaf sizes 1, 2, 4, 8   */
+  PT_TUNE_LEAF = 1u << 2,        /* rebuild the BVH with leaf sizes 1, 2, 4, 8   */
+  PT_TUNE_SORT = 1u << 3         /* try sort-key bits 0 and 4                     */
 };
 typedef struct pt_autotune_cfg {
   uint32_t flags;      /* PT_TUNE_* (0 = all)                                     */

@@ -27,7 +27,7 @@ EST_REFERENCE_NEE, EST_NEE_MIS, EST_NAIVE, EST_ROULETTE, EST_RESERVOIR, EST_VOLU
 RENDER_COUNTERS, RENDER_SORT_HITS, RENDER_ACCUMULATE = 1, 2, 4
 RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
 BATCH_NO_GRAPH = 256
-TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF = 1, 2, 4
+TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF, TUNE_SORT = 1, 2, 4, 8
 BUILD_DEVICE_LBVH = 1
 GUIDE_NO_TRAIN = 1
 GUIDE_INPUTS, GUIDE_OUTPUTS, GUIDE_NPARAMS, GUIDE_MIX = 6, 41, 27497, 48
@@ -85,7 +85,8 @@ class SceneInfo(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("wavefront", ctypes.c_uint32), ("trav_blocks_per_cu", ctypes.c_uint32),
-                ("shade_blocks_per_cu", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32)]
+                ("shade_blocks_per_cu", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32),
+                ("sort_bits", ctypes.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -1342,6 +1342,8 @@ extern "C" pt_status pt_scene_set_tuning(pt_scene *s, const pt_tuning *t) {
   if (!s || !t) return fail(PT_ERR_INVALID, "pt_scene_set_tuning: NULL argument");
   if (t->trav_blocks_per_cu > 64 || t->shade_blocks_per_cu > 256)
     return fail(PT_ERR_INVALID, "pt_scene_set_tuning: grid sizes out of range");
+  if (t->sort_bits != 0 && t->sort_bits != 4)
+    return fail(PT_ERR_INVALID, "pt_scene_set_tuning: sort_bits must be 0 or 4");
   s->tuning = *t;
   return PT_OK;
 }
@@ -1415,7 +1417,7 @@ static pt_status render_frame(pt_scene *s, const pt_render_cfg *cfg, pt_stats *s
   pl.count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
   pl.grids = make_grids(s->n_cu, s->tuning);
   Wave wave = s->ws.wave;
-  wave.sort_keys = (cfg->flags & PT_RENDER_SORT_HITS) ? 1u : 0u;
+  wave.sort_keys = ((cfg->flags & PT_RENDER_SORT_HITS) || s->tuning.sort_bits) ? 1u : 0u;
   GuideView gview{};
   uint32_t n_rec = 0;
   if (gp) {
@@ -1723,6 +1725,8 @@ extern "C" pt_status pt_batch_set_tuning(pt_batch *b, const pt_tuning *t) {
   if (!b || !t) return fail(PT_ERR_INVALID, "pt_batch_set_tuning: NULL argument");
   if (t->trav_blocks_per_cu > 64 || t->shade_blocks_per_cu > 256)
     return fail(PT_ERR_INVALID, "pt_batch_set_tuning: grid sizes out of range");
+  if (t->sort_bits != 0 && t->sort_bits != 4)
+    return fail(PT_ERR_INVALID, "pt_batch_set_tuning: sort_bits must be 0 or 4");
   b->tuning = *t;
   return PT_OK;
 }
@@ -1813,7 +1817,7 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
   }
   pt_status st = ensure_wave(b->ws, max_paths);
   if (st != PT_OK) return st;
-  b->ws.wave.sort_keys = (flags & PT_RENDER_SORT_HITS) ? 1u : 0u;
+  b->ws.wave.sort_keys = ((flags & PT_RENDER_SORT_HITS) || b->tuning.sort_bits) ? 1u : 0u;
   if (est == PT_EST_RESERVOIR) {
     st = ensure_reservoir(b->ws);
     if (st != PT_OK) return st;
@@ -1978,6 +1982,7 @@ static const uint32_t kTuneWavefront[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23
 static const uint32_t kTuneTrav[] = {4u, 6u, 8u};
 static const uint32_t kTuneShade[] = {8u, 16u, 32u, 64u};
 static const uint32_t kTuneLeaf[] = {1u, 2u, 4u, 8u};
+static const uint32_t kTuneSort[] = {0u, 4u};
 
 template <class F>
 static pt_status tune_param(uint32_t *field, const uint32_t *cands, size_t nc, uint32_t repeats, F &&time_one,
@@ -2005,13 +2010,15 @@ extern "C" pt_status pt_autotune(pt_scene *s, const pt_render_cfg *cfg, const pt
   uint32_t spp = 0, sppc = 0;
   pt_status st = check_cfg("pt_autotune", s, cfg, spp, sppc);
   if (st != PT_OK) return st;
-  const uint32_t flags = (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS | PT_TUNE_LEAF);
+  const uint32_t flags =
+      (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS | PT_TUNE_LEAF | PT_TUNE_SORT);
   const uint32_t div = (tc && tc->spp_div) ? tc->spp_div : 8u;
   const uint32_t repeats = (tc && tc->repeats) ? tc->repeats : 1u;
   pt_render_cfg c = *cfg;
   c.spp_count = std::max(1u, sppc / div);
   c.wavefront = 0;
-  c.flags &= ~(uint32_t)(PT_RENDER_ACCUMULATE | PT_RENDER_KEEP_SAMPLES | PT_RENDER_TIMING | PT_RENDER_COUNTERS);
+  c.flags &= ~(uint32_t)(PT_RENDER_ACCUMULATE | PT_RENDER_KEEP_SAMPLES | PT_RENDER_TIMING | PT_RENDER_COUNTERS |
+                         PT_RENDER_SORT_HITS);
   pt_tuning &t = s->tuning;
   if (!t.wavefront) t.wavefront = kDefaultWavefront;
   if (!t.trav_blocks_per_cu) t.trav_blocks_per_cu = 8u;
@@ -2065,6 +2072,10 @@ extern "C" pt_status pt_autotune(pt_scene *s, const pt_render_cfg *cfg, const pt
     st = tune_param(&t.shade_blocks_per_cu, kTuneShade, sizeof(kTuneShade) / 4, repeats, time_one, best);
     if (st != PT_OK) return st;
   }
+  if (flags & PT_TUNE_SORT) {
+    st = tune_param(&t.sort_bits, kTuneSort, sizeof(kTuneSort) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+  }
   // the films of the tuning renders are not the caller's
   s->film_spp = 0;
   if (out) *out = t;
@@ -2073,7 +2084,7 @@ extern "C" pt_status pt_autotune(pt_scene *s, const pt_render_cfg *cfg, const pt
 
 extern "C" pt_status pt_batch_autotune(pt_batch *b, const pt_autotune_cfg *tc, pt_tuning *out) {
   if (!b) return fail(PT_ERR_INVALID, "pt_batch_autotune: NULL batch");
-  const uint32_t flags = (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS);
+  const uint32_t flags = (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS | PT_TUNE_SORT);
   const uint32_t div = (tc && tc->spp_div) ? tc->spp_div : 8u;
   const uint32_t repeats = (tc && tc->repeats) ? tc->repeats : 1u;
   pt_tuning &t = b->tuning;
@@ -2105,6 +2116,10 @@ extern "C" pt_status pt_batch_autotune(pt_batch *b, const pt_autotune_cfg *tc, p
     st = tune_param(&t.trav_blocks_per_cu, kTuneTrav, sizeof(kTuneTrav) / 4, repeats, time_one, best);
     if (st != PT_OK) return st;
     st = tune_param(&t.shade_blocks_per_cu, kTuneShade, sizeof(kTuneShade) / 4, repeats, time_one, best);
+    if (st != PT_OK) return st;
+  }
+  if (flags & PT_TUNE_SORT) {
+    st = tune_param(&t.sort_bits, kTuneSort, sizeof(kTuneSort) / 4, repeats, time_one, best);
     if (st != PT_OK) return st;
   }
   for (auto &in : b->insts) in->film_spp = 0;

@@ -32,12 +32,12 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
     src = tmp_path / "sizes.c"
     src.write_text('#include <stdio.h>\n#include "pathtracer_amd.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu\\n",'
                    'sizeof(pt_material_desc), sizeof(pt_camera_desc), sizeof(pt_render_cfg), sizeof(pt_stats),'
-                   'sizeof(pt_build_cfg), sizeof(pt_scene_info)); return 0;}\n')
+                   'sizeof(pt_build_cfg), sizeof(pt_scene_info)); printf("%zu\\n", sizeof(pt_tuning)); return 0;}\n')
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.dirname(HEADER), "-o", str(exe), str(src)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert got == [ctypes.sizeof(c) for c in (L.MaterialDesc, L.CameraDesc, L.RenderCfg, L.Stats, L.BuildCfg,
-                                              L.SceneInfo)]
+                                              L.SceneInfo, L.Tuning)]
     assert got[0] == 128
 
 
