@@ -113,7 +113,8 @@ def read_traffic(kernel, estimator):
 def quality(scene, estimator, res=128, spp_side=8, ref_side=64):
     """relMSE of a 64-spp frame against a 4096-spp frame (another seed) of the
     same camera at res x res: mean over pixels and channels of
-    (x - r)^2 / (r^2 + 1e-2)."""
+    (x - r)^2 / (r^2 + 1e-2); relmse_trim leaves out the 0.1% of pixels with
+    the largest error."""
     import numpy as np
     t0 = time.perf_counter()
     ref, _ = scene.render(scene.cfg(estimator=estimator, width=res, height=res, spp_x=ref_side, spp_y=ref_side,
@@ -123,8 +124,10 @@ def quality(scene, estimator, res=128, spp_side=8, ref_side=64):
     img, _ = scene.render(scene.cfg(estimator=estimator, width=res, height=res, spp_x=spp_side, spp_y=spp_side,
                                     seed=7))
     t_img = time.perf_counter() - t0
-    rel = float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)))
-    return {"relmse": rel, "spp": spp_side * spp_side, "ref_spp": ref_side * ref_side,
+    e = ((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)).mean(-1).ravel()
+    k = int(0.999 * e.size)
+    return {"relmse": float(e.mean()), "relmse_trim": float(np.partition(e, k)[:k].mean()),
+            "spp": spp_side * spp_side, "ref_spp": ref_side * ref_side,
             "resolution": f"{res}x{res}", "ms_render": round(t_img * 1e3, 1), "ms_ref": round(t_ref * 1e3, 1)}
 
 
@@ -245,18 +248,38 @@ def main_batch(args):
     print(json.dumps(res), flush=True)
 
 
-def relmse(img, ref):
-    """relMSE of the films scaled to the reference's unit mean (the room is dim)."""
+def relmse(img, ref, keep=1.0):
+    """relMSE of the films scaled to the reference's unit mean (the room is
+    dim); keep < 1 averages only the pixels below that quantile of the error
+    (keep = 0.999: the usual 0.1% outlier rejection)."""
     import numpy as np
     mu = float(np.mean(ref))
     x, r = img.astype(np.float64) / mu, ref.astype(np.float64) / mu
-    return float(np.mean((x - r) ** 2 / (r ** 2 + 1e-2)))
+    e = ((x - r) ** 2 / (r ** 2 + 1e-2)).mean(-1).ravel()
+    if keep < 1.0:
+        k = int(keep * e.size)
+        e = np.partition(e, k)[:k]
+    return float(e.mean())
+
+
+def time_to_reach(curve, target):
+    """Time at which a log-log interpolated (ms, error) curve reaches target
+    (extrapolated at slope -1 beyond its end)."""
+    import numpy as np
+    gm = [(np.log(e[0]), np.log(e[1])) for e in curve]
+    for (ta, ea), (tb, eb) in zip(gm, gm[1:]):
+        if (ea - np.log(target)) * (eb - np.log(target)) <= 0 and ea != eb:
+            return float(np.exp(ta + (np.log(target) - ea) * (tb - ta) / (eb - ea)))
+    if gm[-1][1] < np.log(target):
+        return float(np.exp(gm[-1][0] + (np.log(target) - gm[-1][1]) * -1.0))
+    return None
 
 
 def guided_quality(scenes, R, L, q):
     """relMSE against a 32768-spp plain reference at q x q for the plain and the
-    guided method at 16, 64, 256 spp, and the guided time to reach the plain
-    method's 256-spp relMSE (log-log interpolation)."""
+    guided method at 16, 64, 256 spp, in full and with the 0.1% largest pixel
+    errors rejected, and the guided time to reach the plain method's 256-spp
+    error (log-log interpolation) by each."""
     import numpy as np
     qs = R.Scene(scenes.guiding_room(width=q, height=q))
     ref = np.zeros((q, q, 3))
@@ -265,36 +288,33 @@ def guided_quality(scenes, R, L, q):
         ref += img / 8
     curve = {"plain": [], "guided": []}
     for side in (4, 8, 16):
-        pe, pt, ge, gt = [], [], [], []
+        pe, pt, ge, gt, pe_t, ge_t = [], [], [], [], [], []
         for k in range(3):  # three films of each method, averaged
             c = qs.cfg(spp_x=side, spp_y=side, seed=11 + k)
             t = time.perf_counter()
             img, _ = qs.render(c)
             pt.append(time.perf_counter() - t)
             pe.append(relmse(img, ref))
+            pe_t.append(relmse(img, ref, 0.999))
             g = R.Guide(qs, seed=5 + k)
             t = time.perf_counter()
             img, _, _ = g.render(c, L.GuideCfg())
             gt.append(time.perf_counter() - t)
             ge.append(relmse(img, ref))
+            ge_t.append(relmse(img, ref, 0.999))
             g.close()
         curve["plain"].append({"spp": side * side, "ms": round(float(np.mean(pt)) * 1e3, 2),
-                               "relmse": float(np.mean(pe))})
+                               "relmse": float(np.mean(pe)), "relmse_trim": float(np.mean(pe_t))})
         curve["guided"].append({"spp": side * side, "ms": round(float(np.mean(gt)) * 1e3, 2),
-                                "relmse": float(np.mean(ge))})
+                                "relmse": float(np.mean(ge)), "relmse_trim": float(np.mean(ge_t))})
     log(f"quality: {curve}")
-    # time for the guided method to reach the plain 256-spp relMSE (log-log interpolation)
-    target = curve["plain"][-1]["relmse"]
-    gm = [(np.log(e["ms"]), np.log(e["relmse"])) for e in curve["guided"]]
-    t_need = None
-    for (ta, ea), (tb, eb) in zip(gm, gm[1:]):
-        if (ea - np.log(target)) * (eb - np.log(target)) <= 0 and ea != eb:
-            t_need = float(np.exp(ta + (np.log(target) - ea) * (tb - ta) / (eb - ea)))
-            break
-    if t_need is None and gm[-1][1] < np.log(target):  # beyond the curve's end: extrapolate at slope -1
-        t_need = float(np.exp(gm[-1][0] + (np.log(target) - gm[-1][1]) * -1.0))
-    ratio = (t_need / curve["plain"][-1]["ms"]) if t_need else None
-    return curve, target, t_need, ratio
+    out = {}
+    for key in ("relmse", "relmse_trim"):
+        target = curve["plain"][-1][key]
+        t_need = time_to_reach([(e["ms"], e[key]) for e in curve["guided"]], target)
+        out[key] = {"plain_256spp": target, "guided_ms_to_plain": t_need,
+                    "time_ratio_guided_vs_plain": (t_need / curve["plain"][-1]["ms"]) if t_need else None}
+    return curve, out
 
 
 def main_churn(args):
@@ -415,9 +435,9 @@ def main_guided(args):
 
     # quality at 256x256 against a 32768-spp plain reference
     q = 256
-    curve, target, t_need, ratio = None, None, None, None
+    curve, reach = None, None
     if not args.no_quality:
-        curve, target, t_need, ratio = guided_quality(scenes, R, L, q)
+        curve, reach = guided_quality(scenes, R, L, q)
     cst = gst[0][1]
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
@@ -432,8 +452,7 @@ def main_guided(args):
         "plain": {"value": round(plain_value, 2), "ms_per_step": round(dt_p / args.steps * 1e3, 2)},
         "guide": {k: gst[0][2][k] for k in ("passes", "steps", "samples", "last_loss", "ms_train")},
         "quality": None if curve is None else {
-            "resolution": f"{q}x{q}", "ref_spp": 32768, "curve": curve, "plain_relmse_256spp": target,
-            "guided_ms_to_plain_relmse": t_need, "time_ratio_guided_vs_plain": ratio},
+            "resolution": f"{q}x{q}", "ref_spp": 32768, "curve": curve, "reach": reach},
         "segments": cst["segments"], "shadow_rays": cst["shadow_rays"],
         "cpu_baseline": None,
     }

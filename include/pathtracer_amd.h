@@ -241,7 +241,8 @@ pt_status pt_scene_set_medium(pt_scene *scene, const float sigma_a[3], const flo
 typedef struct pt_build_cfg {
   uint32_t leaf_size;   /* max primitives per BVH leaf (0 = 4)                */
   uint32_t sah_bins;    /* SAH bins per axis (0 = 16)                         */
-  uint32_t lds_levels;  /* top BVH levels staged in LDS (0 = none)            */
+  uint32_t lds_levels;  /* top levels of 4-wide BVH nodes traversal keeps in
+                           LDS, whole levels up to 21 nodes (0 = 3)          */
   uint32_t flags;       /* PT_BUILD_*                                         */
 } pt_build_cfg;
 
@@ -253,6 +254,9 @@ typedef struct pt_build_cfg {
  * milliseconds to build, some slower to trace than the SAH tree.  Hits are
  * the same closest hits either way.                                        */
 #define PT_BUILD_DEVICE_LBVH 1u
+/* Traversal reads every BVH node from global memory (no LDS copy of the top
+ * levels).                                                                  */
+#define PT_BUILD_NO_LDS_NODES 2u
 
 /* Builds the BVH, discovers emissive rects/spheres as lights
  * (base/lightDiscovery.hpp:5-81) and uploads everything to the device.      */

@@ -177,6 +177,12 @@ PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, floa
 // LDS words per block for a stack of K entries (codes, plus distances); at
 // least one word so that the declaration stays valid for K = 0.
 #define PT_TRAV_LDS(K, WITH_T) ((K) > 0 ? (K) * PT_BLOCK * ((WITH_T) ? 2 : 1) : 1)
+// Copies the scene's top nodes into the block's LDS (every thread calls it).
+PT_D void stage_top_nodes(const DevScene &s, float4 *top, bool batch) {
+  const uint32_t n = batch ? 0u : 8u * s.top_nodes;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) top[i] = s.nodes[i];
+  __syncthreads();
+}
 
 // ---------------------------------------------------------------------
 // BVH4 traversal.  One 128-byte node per step: the four child boxes are
@@ -331,8 +337,10 @@ PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tm
 // be the same in every lane of the wave.
 // ---------------------------------------------------------------------
 template <bool ANY, bool COUNT, int K, int REFILL, bool BATCH, class Fetch, class Finish>
-PT_D void trace_range(const DevScene &s, int *lds, uint32_t begin, uint32_t end, Fetch fetch, Finish finish,
-                      uint32_t &n_nodes, uint32_t &n_prims) {
+PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t begin, uint32_t end, Fetch fetch,
+                      Finish finish, uint32_t &n_nodes, uint32_t &n_prims) {
+  // nodes [0, ntop) are read from the block's LDS copy `top` (single scenes)
+  const uint32_t ntop = BATCH ? 0u : s.top_nodes;
   // BATCH: fetch names each ray's scene, whose tree the lane then walks
   const float4 *lane_nodes = s.nodes, *lane_prims = s.prims;
   int lane_root = s.root;
@@ -412,9 +420,17 @@ PT_D void trace_range(const DevScene &s, int *lds, uint32_t begin, uint32_t end,
     bool need_pop = true;
     bool done = false;
     if (cur >= 0) {
-      const float4 *nd = (BATCH ? lane_nodes : s.nodes) + 8 * (size_t)cur;
-      const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-      const int4 ch = *reinterpret_cast<const int4 *>(nd + 6);
+      float4 lx, hx, ly, hy, lz, hz;
+      int4 ch;
+      if ((uint32_t)cur < ntop) {
+        const float4 *nd = top + 8 * cur;
+        lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
+        ch = *reinterpret_cast<const int4 *>(nd + 6);
+      } else {
+        const float4 *nd = (BATCH ? lane_nodes : s.nodes) + 8 * (size_t)cur;
+        lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
+        ch = *reinterpret_cast<const int4 *>(nd + 6);
+      }
       if (COUNT) ++n_nodes;
       float tn[4], tf[4];
       const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};

@@ -613,6 +613,54 @@ static void collapse_bvh4(const std::vector<float4> &n2, std::vector<float4> &n4
   collapse_node(n2, 0, n4, 1, max_depth);
 }
 
+// Renumbers 4-wide nodes breadth-first (the root stays node 0), so that the
+// top levels are a prefix of the array; level_end[l] = nodes in levels 0..l.
+static void bfs_renumber(std::vector<float4> &n4, std::vector<uint32_t> &level_end) {
+  const size_t nn = n4.size() / 8;
+  std::vector<int> q;
+  std::vector<int> newid(nn, -1);
+  q.reserve(nn);
+  q.push_back(0);
+  newid[0] = 0;
+  level_end.clear();
+  for (size_t lb = 0; lb < q.size();) {
+    const size_t le = q.size();
+    level_end.push_back((uint32_t)le);
+    for (size_t k = lb; k < le; ++k) {
+      int4 ch;
+      std::memcpy(&ch, &n4[8 * (size_t)q[k] + 6], sizeof(int4));
+      for (int c : {ch.x, ch.y, ch.z, ch.w})
+        if (c >= 0) {
+          newid[c] = (int)q.size();
+          q.push_back(c);
+        }
+    }
+    lb = le;
+  }
+  std::vector<float4> out(n4.size());
+  for (size_t k = 0; k < q.size(); ++k) {
+    const float4 *src = &n4[8 * (size_t)q[k]];
+    float4 *dst = &out[8 * k];
+    for (int r = 0; r < 8; ++r) dst[r] = src[r];
+    int4 ch;
+    std::memcpy(&ch, &dst[6], sizeof(int4));
+    int *c = &ch.x;
+    for (int j = 0; j < 4; ++j)
+      if (c[j] >= 0) c[j] = newid[c[j]];
+    std::memcpy(&dst[6], &ch, sizeof(int4));
+  }
+  n4.swap(out);
+}
+
+// Nodes staged in LDS for traversal: the largest prefix of whole levels
+// among the first `levels` that fits PT_TOP_NODES.
+static uint32_t top_node_count(const std::vector<uint32_t> &level_end, uint32_t levels) {
+  uint32_t top = 0;
+  for (size_t l = 0; l < level_end.size() && l < levels; ++l)
+    if (level_end[l] <= PT_TOP_NODES) top = level_end[l];
+  return top;
+}
+
 // ------------------------------------------------------------------ murmur / digit permutations
 // gl::pbrt::murmurHash64A (utils/hash.hpp:11-56), for DigitPermutation tables.
 static uint64_t murmur64a(const unsigned char *key, size_t len, uint64_t seed) {
@@ -877,6 +925,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   };
 
   std::vector<float4> nodes4;
+  std::vector<uint32_t> level_end;  // breadth-first node numbering: nodes in levels 0..l
   uint32_t depth4 = 0, n_nodes4 = 0;
   int root = 0;
   double bvh_ms = 0.0;
@@ -901,6 +950,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     depth4 = res.depth4;
     n_nodes4 = res.nodes4;
     root = res.root;
+    level_end.assign(res.level_end, res.level_end + std::min<uint32_t>(res.depth4, 4u));
     for (int a = 0; a < 3; ++a) {
       s->box_lo[a] = res.lo[a];
       s->box_hi[a] = res.hi[a];
@@ -931,6 +981,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     }
     B.nodes.resize(4 * (size_t)nn);
     collapse_bvh4(B.nodes, nodes4, depth4);
+    bfs_renumber(nodes4, level_end);
     bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b0).count();
     n_nodes4 = (uint32_t)(nodes4.size() / 8);
     if (3 * depth4 + 3 > PT_STACK4)
@@ -1020,6 +1071,9 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.mat_kinds = 0u;
   for (const pt_material_desc &m : s->mats) ds.mat_kinds |= 1u << m.kind;
   ds.root = root;
+  ds.top_nodes = (cfg && (cfg->flags & PT_BUILD_NO_LDS_NODES))
+                     ? 0u
+                     : top_node_count(level_end, (cfg && cfg->lds_levels) ? cfg->lds_levels : 3u);
   ds.has_env = !s->env.empty();
   ds.env = ds.has_env ? s->d_env.as<float>() : nullptr;
   ds.env_w = (int)s->env_w;

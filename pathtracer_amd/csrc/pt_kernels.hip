@@ -135,6 +135,8 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s0, DevCamera cam0, R
 template <bool COUNT, bool BATCH>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_extend(DevScene s, Wave w, int qi, BatchView b) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_EXT, true)];
+  __shared__ float4 top[8 * PT_TOP_NODES];
+  stage_top_nodes(s, top, BATCH);
   const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
   const PathState &ps = w.st[qi];
@@ -150,7 +152,7 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   const uint32_t per = (n + nw - 1) / nw;
   const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
   trace_range<false, COUNT, PT_K_EXT, PT_REFILL_EXT, BATCH>(
-      s, lds, begin, end,
+      s, lds, top, begin, end,
       [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
         if (BATCH) sc = &b.scenes[b.slot_inst[ps.slot[i]]];
@@ -179,6 +181,8 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
 template <bool COUNT, bool VIS, bool BATCH>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_shadow(DevScene s, Wave w, BatchView b) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_SHADOW, false)];
+  __shared__ float4 top[8 * PT_TOP_NODES];
+  stage_top_nodes(s, top, BATCH);
   // blocks b, b + P, b + 2P, ... share partition p = b % P (the grid is a multiple of P)
   const uint32_t p = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = min(*w.shadow(p), w.shadow_part_cap);
@@ -189,7 +193,7 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   const uint32_t per = (n + nw - 1) / nw;
   const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
   trace_range<true, COUNT, PT_K_SHADOW, PT_REFILL_SHADOW, BATCH>(
-      s, lds, begin, end,
+      s, lds, top, begin, end,
       [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
         if (BATCH) sc = &b.scenes[b.slot_inst[VIS ? w.sslot[i] / 4u : w.sslot[i]]];

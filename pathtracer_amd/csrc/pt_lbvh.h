@@ -26,6 +26,7 @@ struct LbvhResult {
   int root;          // 0, or a leaf code when every primitive fits one leaf
   uint32_t nodes4;   // 4-wide nodes written
   uint32_t depth4;   // levels of 4-wide nodes
+  uint32_t level_end[4];  // nodes in levels 0..l (nodes are numbered level by level)
   float lo[3], hi[3];  // scene bounds (unpadded)
 };
 

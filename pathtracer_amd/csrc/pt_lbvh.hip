@@ -451,6 +451,7 @@ hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t 
     hipLaunchKernelGGL(k_lb_single, dim3(1), dim3(64), 0, st, sc.leafbox, nodes4);
     res.nodes4 = 1;
     res.depth4 = 1;
+    res.level_end[0] = 1;
   } else {
     e = hipMemsetAsync(sc.flags, 0, (size_t)(n - 1) * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
@@ -473,9 +474,10 @@ hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t 
                          sc.range, sc.parent_int, sc.parent_leaf, sc.leafbox, sc.nodebox, sc.flags, leaf_size);
     hipLaunchKernelGGL(k_lb_root, dim3(1), dim3(64), 0, st, sc.q0, sc.counts, sc.counts + 2);
     // one launch per level of 4-wide nodes; the host reads each level's size
-    uint32_t nq = 1, depth = 0;
+    uint32_t nq = 1, depth = 0, numbered = 1;
     int2 *qa = sc.q0, *qb = sc.q1;
     while (nq > 0) {
+      if (depth < 4) res.level_end[depth] = numbered;  // this level's nodes are numbered
       e = hipMemsetAsync(sc.counts + 1, 0, sizeof(uint32_t), st);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_lb_collapse, dim3(lb_blocks(nq)), dim3(256), 0, st, qa, nq, sc.child, sc.range,
@@ -484,6 +486,7 @@ hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t 
       if (e != hipSuccess) return e;
       e = hipStreamSynchronize(st);
       if (e != hipSuccess) return e;
+      numbered += nq;
       ++depth;
       int2 *t = qa;
       qa = qb;

@@ -35,6 +35,10 @@ enum { LIGHT_QUAD = 1, LIGHT_SPHERE = 2, LIGHT_ENV = 3 };
 // kEmptyChild marks an unused slot of a 4-wide node.
 constexpr int kEmptyChild = (int)0x80000000;
 #define PT_STACK4 96
+// Top BVH nodes a traversal block keeps in LDS: nodes are numbered
+// breadth-first, so these are the root, its 4 children and their 16 (three
+// full levels of 4-wide nodes) at most.
+#define PT_TOP_NODES 21
 PT_HD int leaf_code(uint32_t start, uint32_t count) { return ~(int)((start << 4) | (count - 1)); }
 PT_HD uint32_t leaf_start(int c) { return ((uint32_t)~c) >> 4; }
 PT_HD uint32_t leaf_count(int c) { return (((uint32_t)~c) & 15u) + 1u; }
@@ -63,6 +67,7 @@ struct DevScene {
   int n_lights;
   uint32_t mat_kinds;  // bit k set when some material has kind k
   int root;  // child code of the root (node 0 or a single leaf)
+  uint32_t top_nodes;  // nodes [0, top_nodes) (breadth-first) are staged in LDS by traversal
   // environment (texels: (y*w + x)*3)
   const float *env;
   int env_w, env_h;

