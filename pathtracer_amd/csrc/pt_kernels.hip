@@ -859,6 +859,51 @@ __global__ void __launch_bounds__(256) k_film(RenderParams rp0, Wave w, float *f
   if (nan) atomicAdd(&stats[CNT_NAN], (unsigned long long)nan);
 }
 
+// Per-pixel sums of a single render with T lanes per pixel (T a power of two,
+// at most 64 and at most the pixel's samples): lane j sums the pixel's
+// samples j, j + T, j + 2T, ... (so consecutive lanes read consecutive
+// slots), then the T partial sums are added pairwise by an xor-shuffle tree.
+// The order is fixed by (samples per pixel, T), so films are deterministic.
+// One lane per pixel strided through its pixel's samples and fetched 2.2 GB
+// per 8M-slot chunk for 100 MB of radiance.
+__global__ void __launch_bounds__(256) k_film_lanes(RenderParams rp, Wave w, float *film, float *samples,
+                                                    unsigned long long *stats, uint32_t log2_t) {
+  const uint32_t t = gtid(), T = 1u << log2_t;
+  const uint32_t sppc = rp.spp_count, npix = rp.chunk_paths / sppc;
+  const uint32_t pl = t >> log2_t, j = t & (T - 1u);
+  const bool act = pl < npix;  // the T lanes of a pixel agree
+  const uint64_t pixel = rp.chunk_first / sppc + pl;
+  float sr = 0.f, sg = 0.f, sb = 0.f;
+  uint32_t nan = 0;
+  if (act) {
+    const uint32_t slot0 = pl * sppc;
+    for (uint32_t k = j; k < sppc; k += T) {
+      const uint32_t slot = slot0 + k;
+      float r = w.lx[slot], g = w.ly[slot], bb = w.lz[slot];
+      if (!(isfinite(r) && isfinite(g) && isfinite(bb))) {
+        r = g = bb = 0.f;
+        ++nan;
+      }
+      if (samples) {
+        const size_t o = 3 * ((size_t)pixel * sppc + k);
+        samples[o] = r; samples[o + 1] = g; samples[o + 2] = bb;
+      }
+      sr += r; sg += g; sb += bb;
+    }
+  }
+  for (uint32_t off = T >> 1; off >= 1u; off >>= 1) {  // every lane of the wave takes part
+    sr += __shfl_xor(sr, (int)off);
+    sg += __shfl_xor(sg, (int)off);
+    sb += __shfl_xor(sb, (int)off);
+  }
+  if (act && j == 0u) {
+    film[3 * pixel] += sr;
+    film[3 * pixel + 1] += sg;
+    film[3 * pixel + 2] += sb;
+  }
+  if (nan) atomicAdd(&stats[CNT_NAN], (unsigned long long)nan);
+}
+
 // Advances the batch to its next chunk (the last launch of a chunk).
 __global__ void k_next_chunk(BatchView b) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *b.cur += 1u;
@@ -1055,9 +1100,11 @@ void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *f
   if (b) {
     hipLaunchKernelGGL(k_film<true>, dim3(blocks_for(max_pix)), dim3(256), 0, st, rp, w, film, samples, stats, *b);
   } else {
-    uint32_t npix = rp.chunk_paths / rp.spp_count;
-    hipLaunchKernelGGL(k_film<false>, dim3(blocks_for(npix)), dim3(256), 0, st, rp, w, film, samples, stats,
-                       BatchView{});
+    const uint32_t npix = rp.chunk_paths / rp.spp_count;
+    uint32_t lg = 0;
+    while (lg < 6u && (2u << lg) <= rp.spp_count) ++lg;  // T = 2^lg <= min(spp, 64)
+    hipLaunchKernelGGL(k_film_lanes, dim3(blocks_for((size_t)npix << lg)), dim3(256), 0, st, rp, w, film, samples,
+                       stats, lg);
   }
 }
 void launch_next_chunk(hipStream_t st, const BatchView &b) {

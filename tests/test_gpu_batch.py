@@ -3,9 +3,10 @@ depth and scene size in one wavefront.
 
 A path sample's radiance depends only on its render's scene, configuration
 and (pixel, sample) index, never on where in the wavefront it runs, and the
-film sums a pixel's samples in a fixed order.  So every film of a batch must
-equal the film of the same render done alone with pt_render, up to the order
-of the float atomics that add shadow-ray contributions (rtol 1e-5).  This
+film sums a pixel's samples in a fixed order (in sample order in a batch, by
+a fixed lane tree in a single render).  So every film of a batch must equal
+the film of the same render done alone with pt_render, up to that order and
+the order of the float atomics that add shadow-ray contributions (rtol 1e-5).  This
 holds for eager launches and for every replay of the captured chunk graph.
 """
 import numpy as np
