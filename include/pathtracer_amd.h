@@ -477,6 +477,12 @@ pt_status pt_light_sample(pt_scene *scene, uint32_t light, const float *u2,
                           const float *ref, size_t n, float *p, float *nrm,
                           float *pdf, float *le);
 
+/* The environment's importance sampling as the NEE+MIS estimator uses it
+ * (2-D piecewise-constant distribution; the reference samples its
+ * environment uniformly, texture/texture.hpp:120-131): u2 n*2 -> unit
+ * direction dir (n*3) and its solid-angle pdf (n).                          */
+pt_status pt_env_sample(pt_scene *scene, const float *u2, size_t n, float *dir, float *pdf);
+
 pt_status pt_device_synchronize(void);
 const char *pt_last_error(void);
 const char *pt_version(void);

@@ -127,7 +127,7 @@ EXPORTED = [
     "pt_guide_create", "pt_guide_destroy", "pt_guide_get_params", "pt_guide_set_params", "pt_guide_forward",
     "pt_guide_eval", "pt_guide_gradient", "pt_guide_train_step", "pt_guide_samples", "pt_render_guided",
     "pt_guide_set_select_range",
-    "pt_sample", "pt_intersect", "pt_bsdf_eval", "pt_bsdf_sample", "pt_light_sample",
+    "pt_sample", "pt_intersect", "pt_bsdf_eval", "pt_bsdf_sample", "pt_light_sample", "pt_env_sample",
     "pt_device_synchronize", "pt_last_error", "pt_version",
 ]
 
@@ -188,6 +188,7 @@ def _load():
         "pt_bsdf_sample": (i32, [ctypes.POINTER(MaterialDesc), _P, _P, _P, _P, _P, _P, sz,
                                  _P, _P, _P, _P, _P]),
         "pt_light_sample": (i32, [_P, u32, _P, _P, sz, _P, _P, _P, _P]),
+        "pt_env_sample": (i32, [_P, _P, sz, _P, _P]),
         "pt_device_synchronize": (i32, []),
         "pt_last_error": (ctypes.c_char_p, []),
         "pt_version": (ctypes.c_char_p, []),

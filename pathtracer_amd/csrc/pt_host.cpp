@@ -78,7 +78,7 @@ struct pt_scene {
   hipStream_t stream = nullptr;
   int device = -1;
   DevBuf d_nodes, d_prims, d_info, d_geom, d_smooth, d_curve, d_mats, d_lights, d_env, d_env_func, d_env_cdf,
-      d_env_mcdf;
+      d_env_mcdf, d_env_guide;
   DevBuf d_perm, d_perm_off;
   DevBuf lb_in, lb_scratch, lb_nodes;  // device BVH build: input records, scratch, nodes before trimming
   PinnedBuf lb_host;                   // the input records, staged
@@ -734,6 +734,22 @@ static pt_status ensure_perm(pt_scene *s, uint32_t seed) {
 }
 
 // ------------------------------------------------------------------ commit
+// Search ranges for inverting the CDF cdf[0..n] with G buckets: for every
+// u in [b/G, (b+1)/G) the answer max{i < n : cdf[i] <= u} lies in
+// [out[b].x, out[b].y].  The bucket bounds are widened by 1e-6 relative, far
+// more than the rounding of the device's bucket index u * G.
+static void cdf_buckets(const float *cdf, uint32_t n, uint32_t G, uint2 *out) {
+  auto answer = [&](double v) {  // cdf is non-decreasing and cdf[0] = 0 <= v
+    const float *e = std::upper_bound(cdf, cdf + n, v, [](double a, float c) { return a < (double)c; });
+    return (uint32_t)std::max<ptrdiff_t>(0, (e - cdf) - 1);
+  };
+  for (uint32_t b = 0; b < G; ++b) {
+    const double lb = (double)b / G * (1.0 - 1e-6);
+    const double ub = b + 1 == G ? 1.0 : (double)(b + 1) / G * (1.0 + 1e-6);
+    out[b] = make_uint2(answer(lb), answer(ub));
+  }
+}
+
 static float texel_max(const pt_scene *s, uint32_t x, uint32_t y) {
   const float *c = &s->env[3 * ((size_t)y * s->env_w + x)];
   return std::max(c[0], std::max(c[1], c[2]));
@@ -996,6 +1012,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   // -- environment tables (2-D piecewise-constant; same construction as the oracle)
   // (built again only after pt_scene_set_env)
   std::vector<float> efunc, ecdf, emarg, emcdf;
+  std::vector<uint2> eguide;
   const bool env_build = !s->env.empty() && s->env_dirty;
   if (env_build) {
     uint32_t W = s->env_w, H = s->env_h;
@@ -1036,6 +1053,10 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     }
     s->env_integral = (float)acc;
     for (uint32_t y = 1; y <= H; ++y) emcdf[y] = acc > 0.0 ? (float)((double)emcdf[y] / acc) : (float)y / H;
+    // search ranges: W buckets per row, then H for the marginal
+    eguide.resize((size_t)H * W + H);
+    for (uint32_t y = 0; y < H; ++y) cdf_buckets(&ecdf[(size_t)y * (W + 1)], W, W, &eguide[(size_t)y * W]);
+    cdf_buckets(emcdf.data(), H, H, &eguide[(size_t)H * W]);
   }
 
   // -- upload
@@ -1056,6 +1077,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
     HIP_TRY(upload(s->d_env_func, efunc.data(), efunc.size() * sizeof(float)));
     HIP_TRY(upload(s->d_env_cdf, ecdf.data(), ecdf.size() * sizeof(float)));
     HIP_TRY(upload(s->d_env_mcdf, emcdf.data(), emcdf.size() * sizeof(float)));
+    HIP_TRY(upload(s->d_env_guide, eguide.data(), eguide.size() * sizeof(uint2)));
     s->env_dirty = false;
   }
   DevScene &ds = s->ds;
@@ -1082,6 +1104,8 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.env_func = ds.has_env ? s->d_env_func.as<float>() : nullptr;
   ds.env_cdf = ds.has_env ? s->d_env_cdf.as<float>() : nullptr;
   ds.env_mcdf = ds.has_env ? s->d_env_mcdf.as<float>() : nullptr;
+  ds.env_guide = ds.has_env ? s->d_env_guide.as<uint2>() : nullptr;
+  ds.env_mguide = ds.has_env ? s->d_env_guide.as<uint2>() + (size_t)s->env_w * s->env_h : nullptr;
   ds.env_integral = s->env_integral;
   st = ensure_perm(s, s->perm_valid ? s->perm_seed : 0u);  // the tables do not depend on the geometry
   if (st != PT_OK) return st;
@@ -2336,6 +2360,24 @@ extern "C" pt_status pt_light_sample(pt_scene *s, uint32_t light, const float *u
   HIP_TRY(hipMemcpy(nrm, on, n * 12, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(pdf, od, n * 4, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(le, ol, n * 12, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+extern "C" pt_status pt_env_sample(pt_scene *s, const float *u2, size_t n, float *dir, float *pdf) {
+  if (!s || !u2 || !dir || !pdf) return fail(PT_ERR_INVALID, "pt_env_sample: NULL argument");
+  if (!s->committed) return fail(PT_ERR_INVALID, "pt_env_sample: scene not committed");
+  if (!s->ds.has_env) return fail(PT_ERR_INVALID, "pt_env_sample: the scene has no environment");
+  if (n == 0) return PT_OK;
+  Scratch sc;
+  auto *a = (const float *)sc.put(u2, n * 8);
+  auto *od = (float *)sc.put(nullptr, n * 12);
+  auto *op = (float *)sc.put(nullptr, n * 4);
+  NEED(a); NEED(od); NEED(op);
+  launch_env_sample(s->stream, s->ds, a, n, od, op);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(dir, od, n * 12, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(pdf, op, n * 4, hipMemcpyDeviceToHost));
   return PT_OK;
 }
 

@@ -1003,6 +1003,15 @@ __global__ void k_light_sample(DevScene s, uint32_t light, const float *u2, cons
   le[3 * i] = e.x; le[3 * i + 1] = e.y; le[3 * i + 2] = e.z;
 }
 
+__global__ void k_env_sample(DevScene s, const float *u2, size_t n, float *dir, float *pdf) {
+  size_t i = gtid();
+  if (i >= n) return;
+  float pd;
+  const f3 d = env_sample(s, u2[2 * i], u2[2 * i + 1], pd);
+  dir[3 * i] = d.x; dir[3 * i + 1] = d.y; dir[3 * i + 2] = d.z;
+  pdf[i] = pd;
+}
+
 // ------------------------------------------------------------------ launchers
 static inline uint32_t blocks_for(size_t n, uint32_t bs = 256) { return (uint32_t)((n + bs - 1) / bs); }
 // Grid of a partitioned kernel: enough blocks for max_n entries spread over the
@@ -1135,6 +1144,9 @@ void launch_bsdf_sample(hipStream_t st, const pt_material_desc &m, const float *
 void launch_light_sample(hipStream_t st, const DevScene &s, uint32_t light, const float *u2, const float *ref,
                          size_t n, float *p, float *nrm, float *pdf, float *le) {
   hipLaunchKernelGGL(k_light_sample, dim3(blocks_for(n)), dim3(256), 0, st, s, light, u2, ref, n, p, nrm, pdf, le);
+}
+void launch_env_sample(hipStream_t st, const DevScene &s, const float *u2, size_t n, float *dir, float *pdf) {
+  hipLaunchKernelGGL(k_env_sample, dim3(blocks_for(n)), dim3(256), 0, st, s, u2, n, dir, pdf);
 }
 
 }  // namespace pt

@@ -28,5 +28,6 @@ void launch_bsdf_sample(hipStream_t st, const pt_material_desc &m, const float *
                         float *f, float *pdf, uint32_t *flags, uint8_t *ok);
 void launch_light_sample(hipStream_t st, const DevScene &s, uint32_t light, const float *u2, const float *ref,
                          size_t n, float *p, float *nrm, float *pdf, float *le);
+void launch_env_sample(hipStream_t st, const DevScene &s, const float *u2, size_t n, float *dir, float *pdf);
 
 }  // namespace pt

@@ -73,6 +73,10 @@ struct DevScene {
   int env_w, env_h;
   float env_scale;
   const float *env_func, *env_cdf, *env_mcdf;
+  // search ranges of the CDF inversions: bucket b of a table of G buckets
+  // holds [lo, hi], the intervals the answer for any u in [b/G, (b+1)/G)
+  // lies in; env_guide has W buckets per row, env_mguide H for the marginal
+  const uint2 *env_guide, *env_mguide;
   float env_integral;
   int has_env;
   // the global homogeneous medium (PT_EST_VOLUME)

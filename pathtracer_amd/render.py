@@ -130,6 +130,15 @@ class Scene:
         L.check(L.lib.pt_intersect(self.h, L.ptr(rays), n, int(any_hit), L.ptr(prim), L.ptr(t)), "pt_intersect")
         return prim, t
 
+    def env_sample(self, u2: np.ndarray):
+        """pt_env_sample: importance-sampled environment directions and pdfs."""
+        u2 = np.ascontiguousarray(u2, np.float32)
+        n = u2.shape[0]
+        d = np.empty((n, 3), np.float32)
+        pdf = np.empty(n, np.float32)
+        L.check(L.lib.pt_env_sample(self.h, L.ptr(u2), n, L.ptr(d), L.ptr(pdf)), "pt_env_sample")
+        return d, pdf
+
     def light_sample(self, light: int, u2: np.ndarray, ref: np.ndarray):
         u2 = np.ascontiguousarray(u2, np.float32)
         ref = np.ascontiguousarray(ref, np.float32)
