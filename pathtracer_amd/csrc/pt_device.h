@@ -325,13 +325,15 @@ PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tm
 }
 
 // ---------------------------------------------------------------------
-// Persistent traversal with per-lane ray refill (the "while-while with
-// dynamic fetch" of Aila & Laine, HPG 2009, on 64-wide waves).  A wave owns
-// the contiguous ray range [begin, end) and runs one traversal step (one
-// node or one leaf) per iteration; lanes whose rays are finished take the
-// next rays of the range once REFILL of them are idle, so lanes do not idle
-// while the wave's longest ray finishes.  The per-ray results equal traverse<>'s:
-// the same node order, the same culling and the same primitive tests.
+// Persistent traversal with per-lane ray refill and speculative while-while
+// loops (Aila & Laine, HPG 2009, on 64-wide waves).  A wave owns the
+// contiguous ray range [begin, end); each iteration runs a node phase and a
+// leaf phase (below), and lanes whose rays are finished take the next rays
+// of the range once REFILL of them are idle, so lanes do not idle while the
+// wave's longest ray finishes.  The per-ray results equal traverse<>'s (the
+// closest hit, or whether any hit exists) up to the winner of exact ties:
+// a postponed leaf is tested after some of the nodes traverse<> would visit
+// after it, which changes the order of tests and culls fewer subtrees.
 // fetch(i, o, d, tmin, tmax, scene) loads ray i (and, batched, points scene
 // at the ray's instance); finish(i, hit, t, prim) stores it.  begin, end must
 // be the same in every lane of the wave.
@@ -387,6 +389,71 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
     c = pc[sp];
     t = WT ? ptv[sp] : 0.f;
   };
+  // nearest stack entry whose box starts no farther than the closest hit
+  auto pop_next = [&](int &c) -> bool {
+    while (sp > 0) {
+      int cc;
+      float te;
+      pop(cc, te);
+      if (ANY || te <= tb) {
+        c = cc;
+        return true;
+      }
+    }
+    return false;
+  };
+  bool have_cur = false, have_leaf = false;
+  int leaf = 0;
+  // one inner-node step: test the 4 children, go to the nearest hit one and
+  // push the others far to near, else pop
+  auto node_step = [&]() {
+    float4 lx, hx, ly, hy, lz, hz;
+    int4 ch;
+    if ((uint32_t)cur < ntop) {
+      const float4 *nd = top + 8 * cur;
+      lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
+      ch = *reinterpret_cast<const int4 *>(nd + 6);
+    } else {
+      const float4 *nd = (BATCH ? lane_nodes : s.nodes) + 8 * (size_t)cur;
+      lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
+      ch = *reinterpret_cast<const int4 *>(nd + 6);
+    }
+    if (COUNT) ++n_nodes;
+    float tn[4], tf[4];
+    const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float a0 = fmaf(lxs[k], ix, -oix), a1 = fmaf(hxs[k], ix, -oix);
+      float b0 = fmaf(lys[k], iy, -oiy), b1 = fmaf(hys[k], iy, -oiy);
+      float c0 = fmaf(lzs[k], iz, -oiz), c1 = fmaf(hzs[k], iz, -oiz);
+      tn[k] = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
+      tf[k] = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tb));
+    }
+    const int cs[4] = {ch.x, ch.y, ch.z, ch.w};
+    int c0 = cs[0], c1 = cs[1], c2 = cs[2], c3 = cs[3];
+    const bool h0 = tn[0] <= tf[0] && cs[0] != kEmptyChild;
+    const bool h1 = tn[1] <= tf[1] && cs[1] != kEmptyChild;
+    const bool h2 = tn[2] <= tf[2] && cs[2] != kEmptyChild;
+    const bool h3 = tn[3] <= tf[3] && cs[3] != kEmptyChild;
+    float t0 = h0 ? tn[0] : INFINITY, t1 = h1 ? tn[1] : INFINITY;
+    float t2 = h2 ? tn[2] : INFINITY, t3 = h3 ? tn[3] : INFINITY;
+    const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+    if (nh > 0) {
+      PT_CSWAP(t0, c0, t1, c1);
+      PT_CSWAP(t2, c2, t3, c3);
+      PT_CSWAP(t0, c0, t2, c2);
+      PT_CSWAP(t1, c1, t3, c3);
+      PT_CSWAP(t1, c1, t2, c2);
+      if (nh >= 4) push(c3, t3);
+      if (nh >= 3) push(c2, t2);
+      if (nh >= 2) push(c1, t1);
+      cur = c0;
+    } else {
+      have_cur = pop_next(cur);
+    }
+  };
   while (true) {
     // refill the lanes that have no ray once at least REFILL of them are idle
     // (or all are): waiting for a group keeps the new rays of a wave adjacent
@@ -410,6 +477,8 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
         tb = tmax;
         best = -1;
         cur = BATCH ? lane_root : s.root;
+        have_cur = true;
+        have_leaf = false;
         sp = 0;
         lo = 0;
       }
@@ -417,85 +486,84 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
     }
     if (__ballot(ray >= 0) == 0ull) break;
     if (ray < 0) continue;
-    bool need_pop = true;
-    bool done = false;
-    if (cur >= 0) {
-      float4 lx, hx, ly, hy, lz, hz;
-      int4 ch;
-      if ((uint32_t)cur < ntop) {
-        const float4 *nd = top + 8 * cur;
-        lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
-        ch = *reinterpret_cast<const int4 *>(nd + 6);
+    if constexpr (ANY) {
+      // Any hit: one node or one leaf per iteration.  One hit ends the ray,
+      // so a leaf is tested as soon as it is reached (the speculative loops
+      // below cost the shadow rays 8% on the 1.3M-triangle frame, plain
+      // while-while loops 26%).
+      if (cur >= 0) {
+        node_step();
       } else {
-        const float4 *nd = (BATCH ? lane_nodes : s.nodes) + 8 * (size_t)cur;
-        lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
-        ch = *reinterpret_cast<const int4 *>(nd + 6);
-      }
-      if (COUNT) ++n_nodes;
-      float tn[4], tf[4];
-      const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float a0 = fmaf(lxs[k], ix, -oix), a1 = fmaf(hxs[k], ix, -oix);
-        float b0 = fmaf(lys[k], iy, -oiy), b1 = fmaf(hys[k], iy, -oiy);
-        float c0 = fmaf(lzs[k], iz, -oiz), c1 = fmaf(hzs[k], iz, -oiz);
-        tn[k] = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
-        tf[k] = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tb));
-      }
-      const int cs[4] = {ch.x, ch.y, ch.z, ch.w};
-      int c0 = cs[0], c1 = cs[1], c2 = cs[2], c3 = cs[3];
-      const bool h0 = tn[0] <= tf[0] && cs[0] != kEmptyChild;
-      const bool h1 = tn[1] <= tf[1] && cs[1] != kEmptyChild;
-      const bool h2 = tn[2] <= tf[2] && cs[2] != kEmptyChild;
-      const bool h3 = tn[3] <= tf[3] && cs[3] != kEmptyChild;
-      float t0 = h0 ? tn[0] : INFINITY, t1 = h1 ? tn[1] : INFINITY;
-      float t2 = h2 ? tn[2] : INFINITY, t3 = h3 ? tn[3] : INFINITY;
-      const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-      if (nh > 0) {
-        PT_CSWAP(t0, c0, t1, c1);
-        PT_CSWAP(t2, c2, t3, c3);
-        PT_CSWAP(t0, c0, t2, c2);
-        PT_CSWAP(t1, c1, t3, c3);
-        PT_CSWAP(t1, c1, t2, c2);
-        if (nh >= 4) push(c3, t3);
-        if (nh >= 3) push(c2, t2);
-        if (nh >= 2) push(c1, t1);
-        cur = c0;
-        need_pop = false;
-      }
-    } else {
-      const uint32_t st = leaf_start(cur), cn = leaf_count(cur);
-      for (uint32_t q = st; q < st + cn; ++q) {
-        float t;
-        uint32_t id;
-        if (COUNT) ++n_prims;
-        if (hit_prim(BATCH ? lane_prims : s.prims, q, o, d, tmin, tb, t, id)) {
-          tb = t;
-          best = (int)id;
-          if (ANY) {
-            done = true;
+        bool hit = false;
+        const uint32_t st = leaf_start(cur), cn = leaf_count(cur);
+        for (uint32_t q = st; q < st + cn; ++q) {
+          float t;
+          uint32_t id;
+          if (COUNT) ++n_prims;
+          if (hit_prim(BATCH ? lane_prims : s.prims, q, o, d, tmin, tb, t, id)) {
+            tb = t;
+            best = (int)id;
+            hit = true;
             break;
           }
         }
+        have_cur = !hit && pop_next(cur);
       }
+      if (!have_cur) {
+        finish((uint32_t)ray, best >= 0, tb, best);
+        ray = -1;
+      }
+      continue;
     }
-    if (need_pop && !done) {
-      bool found = false;
-      while (sp > 0) {
-        int c;
-        float te;
-        pop(c, te);
-        if (ANY || te <= tb) {
-          cur = c;
-          found = true;
-          break;
+    // Node phase (speculative while-while, Aila & Laine 2009): lanes step
+    // through inner nodes; the first leaf a lane reaches is postponed and
+    // the lane goes on with its stack, until every lane holds a leaf or has
+    // no inner node left, so that the node steps and the leaf tests each run
+    // with most lanes of the wave at once (closest hit: 46.8 -> 44.2 ms on
+    // the 1.3M-triangle frame).
+    while (true) {
+      const bool step = have_cur && cur >= 0;
+      if (__ballot(step) == 0ull) break;
+      if (step) node_step();
+      // postpone the first leaf reached
+      if (have_cur && cur < 0 && !have_leaf) {
+        leaf = cur;
+        have_leaf = true;
+        have_cur = pop_next(cur);
+      }
+      // stop once no lane is still looking for a leaf among inner nodes
+      if (__ballot(have_cur && cur >= 0 && !have_leaf) == 0ull) break;
+    }
+    // Leaf phase: the postponed leaf, then leaves the stack yields next.
+    bool done = false;
+    while (true) {
+      if (__ballot(have_leaf) == 0ull) break;
+      if (have_leaf) {
+        const uint32_t st = leaf_start(leaf), cn = leaf_count(leaf);
+        for (uint32_t q = st; q < st + cn; ++q) {
+          float t;
+          uint32_t id;
+          if (COUNT) ++n_prims;
+          if (hit_prim(BATCH ? lane_prims : s.prims, q, o, d, tmin, tb, t, id)) {
+            tb = t;
+            best = (int)id;
+            if (ANY) {
+              done = true;
+              break;
+            }
+          }
+        }
+        if (done) {
+          have_leaf = have_cur = false;
+        } else if (have_cur && cur < 0) {
+          leaf = cur;
+          have_cur = pop_next(cur);
+        } else {
+          have_leaf = false;
         }
       }
-      if (!found) done = true;
     }
-    if (done) {
+    if (!have_cur && !have_leaf) {
       finish((uint32_t)ray, best >= 0, tb, best);
       ray = -1;
     }
