@@ -287,7 +287,7 @@ pt_status pt_film_read_samples(pt_scene *scene, float *out_rgb, size_t n);
 /* whose OpenMP loop has no launch parameters, renderManager.hpp:99).   */
 /* ------------------------------------------------------------------ */
 typedef struct pt_tuning {
-  uint32_t wavefront;            /* path slots in flight (0 = 8M); a render's
+  uint32_t wavefront;            /* path slots in flight (0 = 64M); a render's
                                     pt_render_cfg.wavefront overrides it          */
   uint32_t trav_blocks_per_cu;   /* traversal grid, blocks of 256 per CU (0 = 8) */
   uint32_t shade_blocks_per_cu;  /* shading grid cap, blocks per CU (0 = 32)     */
@@ -300,7 +300,7 @@ typedef struct pt_tuning {
 pt_status pt_scene_set_tuning(pt_scene *scene, const pt_tuning *tuning);
 
 enum {
-  PT_TUNE_WAVEFRONT = 1u << 0,   /* try wavefront sizes 1M..16M                  */
+  PT_TUNE_WAVEFRONT = 1u << 0,   /* try wavefront sizes 4M..64M                  */
   PT_TUNE_GRIDS = 1u << 1,       /* try traversal / shading grid sizes           */
   PT_TUNE_LEAF = 1u << 2,        /* rebuild the BVH with leaf sizes 1, 2, 4, 8   */
   PT_TUNE_SORT = 1u << 3         /* try sort-key bits 0 and 4                     */
@@ -338,7 +338,7 @@ enum {
   PT_BATCH_NO_GRAPH = 1u << 8    /* launch eagerly instead of replaying a graph */
 };
 typedef struct pt_batch_cfg {
-  uint32_t wavefront;   /* path slots in flight (0 = the batch tuning, else 8M) */
+  uint32_t wavefront;   /* path slots in flight (0 = the batch tuning, else 64M) */
   uint32_t flags;       /* PT_RENDER_COUNTERS, PT_RENDER_SORT_HITS, PT_RENDER_TIMING (eager only),
                            PT_BATCH_NO_GRAPH                                    */
 } pt_batch_cfg;

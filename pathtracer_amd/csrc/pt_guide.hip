@@ -262,13 +262,13 @@ __global__ void __launch_bounds__(256) k_guide_points(GuideView gv, const float 
     if (raw && p < n) {
       for (uint32_t o = 0; o < PT_G_OUT; ++o) {
         const uint32_t row = g_out_row(o);
-        if ((row & 15u) / 4u == g) raw[p * PT_G_OUT + o] = acc[t][row / 16u][row & 3u];
+        if ((row & 15u) / 4u == g) raw[(size_t)p * PT_G_OUT + o] = acc[t][row / 16u][row & 3u];
       }
     }
     LobeQuad r;
     g_process(acc[t], g, gv.c_lo, gv.c_span, r);
     if (mixout && p < n) {
-      float *mx = mixout + p * PT_G_MIX;
+      float *mx = mixout + (size_t)p * PT_G_MIX;
       if (g < 2u) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {

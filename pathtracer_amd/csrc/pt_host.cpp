@@ -1412,9 +1412,12 @@ static void fill_stats(pt_stats *stats, const unsigned long long *c, uint64_t pa
   stats->extend_launches = launches;
 }
 
-// 8M path slots in flight by default (~2.7 GB of wavefront state): enough
-// rays in the late, sparse bounces to keep 256 CUs busy.
-static const uint32_t kDefaultWavefront = 1u << 23;
+// Path slots in flight by default: 64M (about 22 GB of HBM).  A frame whose
+// paths fit takes one chunk; on the 1.3M-triangle 1024^2 x 64-spp frame
+// 8M / 16M / 32M / 64M slots took 122.7 / 105.4 / 95.4 / 92.2 ms: the late
+// vertices of a chunk have few live paths, and a larger chunk keeps those
+// launches full.
+static const uint32_t kDefaultWavefront = 1u << 26;
 
 extern "C" pt_status pt_scene_set_tuning(pt_scene *s, const pt_tuning *t) {
   if (!s || !t) return fail(PT_ERR_INVALID, "pt_scene_set_tuning: NULL argument");
@@ -2056,7 +2059,7 @@ extern "C" pt_status pt_batch_film_read(pt_batch *b, uint32_t index, float *out)
 // Coordinate search: each tuned parameter in turn, the others held at their
 // best so far; a candidate's time is the fastest of `repeats` renders of a
 // 1/spp_div sample of the work.
-static const uint32_t kTuneWavefront[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24};
+static const uint32_t kTuneWavefront[] = {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
 static const uint32_t kTuneTrav[] = {4u, 6u, 8u};
 static const uint32_t kTuneShade[] = {8u, 16u, 32u, 64u};
 static const uint32_t kTuneLeaf[] = {1u, 2u, 4u, 8u};

@@ -136,7 +136,7 @@ def test_autotune(gpu):
     cfg = s.cfg()
     before, _ = s.render(cfg)
     t = s.autotune(cfg, flags=L.TUNE_WAVEFRONT | L.TUNE_GRIDS, spp_div=2)
-    assert t["wavefront"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
+    assert t["wavefront"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
     assert t["trav_blocks_per_cu"] in (4, 6, 8)
     assert t["shade_blocks_per_cu"] in (8, 16, 32, 64)
     after, _ = s.render(cfg)
@@ -151,5 +151,5 @@ def test_autotune(gpu):
     for r, c in _mixed_renders(1):
         b.add(r, c)
     bt = b.autotune(spp_div=2)
-    assert bt["wavefront"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
+    assert bt["wavefront"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
     b.close()
