@@ -24,6 +24,7 @@
 
 #include "../../include/pathtracer_amd.h"
 #include "pt_guide_host.h"
+#include "pt_bsdf.h"
 #include "pt_launch.h"
 #include "pt_lbvh.h"
 
@@ -1740,16 +1741,19 @@ struct pt_batch {
   WaveStore ws;
   // per-render tables and the chunk schedule (device copies of the host vectors)
   DevBuf d_scenes, d_params, d_cams, d_offs, d_off_first, d_films, d_segs, d_chunks, d_cur, d_slots;
-  // the captured launches of one chunk, and what they were captured for
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<unsigned char> graph_key;
+  // the captured launches of one chunk, per shading material set (0: the
+  // basic set, 1: every kind), and what they were captured for
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<unsigned char> graph_key[2];
   void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-    graph_key.clear();
+    for (int k = 0; k < 2; ++k) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      if (graph[k]) (void)hipGraphDestroy(graph[k]);
+      exec[k] = nullptr;
+      graph[k] = nullptr;
+      graph_key[k].clear();
+    }
   }
 };
 
@@ -1813,14 +1817,15 @@ extern "C" pt_status pt_batch_set_tuning(pt_batch *b, const pt_tuning *t) {
 }
 
 // Cuts the concatenated path space of the renders (each pixel-major, whole
-// pixels) into chunks of at most cap slots, and each chunk into one segment
-// per render it touches.
-static void plan_chunks(const std::vector<uint32_t> &npix, const std::vector<uint32_t> &sppc, uint32_t cap,
-                        std::vector<ChunkSeg> &segs, std::vector<ChunkInfo> &chunks) {
+// pixels, taken in `order`) into chunks of at most cap slots, and each chunk
+// into one segment per render it touches.
+static void plan_chunks(const std::vector<uint32_t> &npix, const std::vector<uint32_t> &sppc,
+                        const std::vector<uint32_t> &order, uint32_t cap, std::vector<ChunkSeg> &segs,
+                        std::vector<ChunkInfo> &chunks) {
   segs.clear();
   chunks.clear();
   ChunkInfo cur{0u, 0u, 0u, 0u};
-  for (uint32_t i = 0; i < (uint32_t)npix.size(); ++i) {
+  for (uint32_t i : order) {
     uint32_t done = 0;
     while (done < npix[i]) {
       uint32_t room = (cap - cur.paths) / sppc[i];
@@ -1888,9 +1893,26 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
 
   uint32_t want = (bc && bc->wavefront) ? bc->wavefront : (b->tuning.wavefront ? b->tuning.wavefront : kDefaultWavefront);
   uint32_t cap = (uint32_t)std::min<uint64_t>(std::max(want, max_sppc), total);
+  // Renders whose scenes use only the basic material kinds come first, so
+  // that most chunks hold one class and shade with that class's kernel (the
+  // basic set needs fewer registers); a chunk with any other kind shades
+  // with every kind.
+  std::vector<uint32_t> order;
+  uint32_t kinds_basic = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (uint32_t i = 0; i < (uint32_t)ni; ++i) {
+      const uint32_t mk = b->insts[i]->scene->ds.mat_kinds;
+      const bool basic = (mk & ~kMatBasic) == 0u;
+      if (basic == (pass == 0)) order.push_back(i);
+      if (basic) kinds_basic |= mk;
+    }
   std::vector<ChunkSeg> segs;
   std::vector<ChunkInfo> chunks;
-  plan_chunks(npix, sppc, cap, segs, chunks);
+  plan_chunks(npix, sppc, order, cap, segs, chunks);
+  std::vector<uint8_t> chunk_class(chunks.size(), 0);
+  for (size_t c = 0; c < chunks.size(); ++c)
+    for (uint32_t k = 0; k < chunks[c].n_segs; ++k)
+      if (b->insts[segs[chunks[c].seg_begin + k].inst]->scene->ds.mat_kinds & ~kMatBasic) chunk_class[c] = 1;
   uint32_t max_paths = 0, max_pix = 0;
   for (const ChunkInfo &c : chunks) {
     max_paths = std::max(max_paths, c.paths);
@@ -1966,14 +1988,17 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
   bv.slot_pix = bv.slot_inst + b->ws.cap;
   bv.slot_k = bv.slot_pix + b->ws.cap;
 
-  ChunkPlan pl{};
-  pl.max_paths = max_paths;
-  pl.max_pix = max_pix;
-  pl.nverts = nverts;
-  pl.estimator = est;
-  pl.mat_kinds = mat_kinds;
-  pl.count = (flags & PT_RENDER_COUNTERS) != 0;
-  pl.grids = make_grids(b->n_cu, b->tuning);
+  ChunkPlan plans[2]{};
+  for (int k = 0; k < 2; ++k) {
+    ChunkPlan &pl = plans[k];
+    pl.max_paths = max_paths;
+    pl.max_pix = max_pix;
+    pl.nverts = nverts;
+    pl.estimator = est;
+    pl.mat_kinds = k == 0 ? kinds_basic : mat_kinds;
+    pl.count = (flags & PT_RENDER_COUNTERS) != 0;
+    pl.grids = make_grids(b->n_cu, b->tuning);
+  }
   RenderParams rp0{};  // the kernels read the renders' own parameters
   rp0.estimator = est;
   rp0.chunk_paths = max_paths;
@@ -1982,7 +2007,11 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
   const bool use_graph = !(flags & PT_BATCH_NO_GRAPH) && !(flags & PT_RENDER_TIMING);
 
   HIP_TRY(hipMemsetAsync(b->ws.stats.p, 0, CNT_N * sizeof(unsigned long long), b->stream));
-  if (use_graph) {
+  bool used[2] = {false, false};
+  for (uint8_t c : chunk_class) used[c] = true;
+  for (int k = 0; k < 2 && use_graph; ++k) {
+    if (!used[k]) continue;
+    const ChunkPlan &pl = plans[k];
     // The graph's kernel arguments are fixed device addresses and sizes:
     // re-capture when any of them changed.
     std::vector<unsigned char> key(sizeof(pl) + sizeof(bv) + sizeof(Wave) + sizeof(DevScene));
@@ -1990,8 +2019,12 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
     std::memcpy(key.data() + sizeof(pl), &bv, sizeof(bv));
     std::memcpy(key.data() + sizeof(pl) + sizeof(bv), &b->ws.wave, sizeof(Wave));
     std::memcpy(key.data() + sizeof(pl) + sizeof(bv) + sizeof(Wave), &ds0, sizeof(DevScene));
-    if (key != b->graph_key || !b->exec) {
-      b->drop_graph();
+    if (key != b->graph_key[k] || !b->exec[k]) {
+      if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+      if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+      b->exec[k] = nullptr;
+      b->graph[k] = nullptr;
+      b->graph_key[k].clear();
       HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
       launch_chunk(b->stream, ds0, cam0, rp0, b->ws.wave, nullptr, nullptr, nullptr, nullptr, &bv, pl, nullptr);
       hipGraph_t g = nullptr;
@@ -2002,9 +2035,9 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
         return fail(PT_ERR_DEVICE, std::string("pt_batch_render: graph capture failed: ") +
                                        hipGetErrorString(e1 != hipSuccess ? e1 : e2));
       }
-      b->graph = g;
-      HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-      b->graph_key = key;
+      b->graph[k] = g;
+      HIP_TRY(hipGraphInstantiate(&b->exec[k], b->graph[k], nullptr, nullptr, 0));
+      b->graph_key[k] = key;
     }
   }
   StageEvents ev;
@@ -2016,10 +2049,11 @@ static pt_status batch_render(pt_batch *b, const pt_batch_cfg *bc, pt_stats *sta
   HIP_TRY(hipEventRecord(tstart, b->stream));
   launch_set_chunk(b->stream, bv, 0u);
   for (size_t c = 0; c < chunks.size(); ++c) {
+    const int k = chunk_class[c];
     if (use_graph)
-      HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+      HIP_TRY(hipGraphLaunch(b->exec[k], b->stream));
     else
-      launch_chunk(b->stream, ds0, cam0, rp0, b->ws.wave, nullptr, nullptr, nullptr, nullptr, &bv, pl, &ev);
+      launch_chunk(b->stream, ds0, cam0, rp0, b->ws.wave, nullptr, nullptr, nullptr, nullptr, &bv, plans[k], &ev);
   }
   HIP_TRY(hipEventRecord(tstop, b->stream));
   HIP_TRY(hipGetLastError());

@@ -813,49 +813,45 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
 }
 
 // ------------------------------------------------------------------ film
-template <bool BATCH>
-__global__ void __launch_bounds__(256) k_film(RenderParams rp0, Wave w, float *film0, float *samples,
-                                              unsigned long long *stats, BatchView b) {
-  uint32_t pl = gtid();
-  uint32_t npix, sppc, slot0;
-  uint64_t pixel;
-  float *film;
-  if (BATCH) {
-    const ChunkInfo ci = b.chunks[*b.cur];
-    npix = ci.npix;
-    if (pl >= npix) return;
-    const ChunkSeg sg = b.segs[find_seg_pix(b, ci, pl)];
-    sppc = b.params[sg.inst].spp_count;
-    pixel = sg.pix_first + (pl - sg.pix_offset);
-    slot0 = sg.path_first + (pl - sg.pix_offset) * sppc;
-    film = b.films[sg.inst];
-    samples = nullptr;
-  } else {
-    sppc = rp0.spp_count;
-    npix = rp0.chunk_paths / sppc;
-    if (pl >= npix) return;
-    pixel = rp0.chunk_first / sppc + pl;
-    slot0 = pl * sppc;
-    film = film0;
-  }
+// Per-pixel sums of a batch chunk, PT_FILM_BATCH_T lanes per pixel (the
+// renders of a chunk differ in samples per pixel, so T is fixed): lane j sums
+// the pixel's samples j, j + T, ... and an xor-shuffle tree adds the T
+// partial sums, a fixed order.
+#define PT_FILM_BATCH_T 8u
+__global__ void __launch_bounds__(256) k_film_batch(Wave w, unsigned long long *stats, BatchView b) {
+  const uint32_t t = gtid(), j = t % PT_FILM_BATCH_T, pl = t / PT_FILM_BATCH_T;
+  const ChunkInfo ci = b.chunks[*b.cur];
+  const bool act = pl < ci.npix;  // the T lanes of a pixel agree
   float sr = 0.f, sg = 0.f, sb = 0.f;
   uint32_t nan = 0;
-  for (uint32_t k = 0; k < sppc; ++k) {
-    uint32_t slot = slot0 + k;
-    float r = w.lx[slot], g = w.ly[slot], bb = w.lz[slot];
-    if (!(isfinite(r) && isfinite(g) && isfinite(bb))) {
-      r = g = bb = 0.f;
-      ++nan;
+  uint64_t pixel = 0;
+  float *film = nullptr;
+  if (act) {
+    const ChunkSeg sg0 = b.segs[find_seg_pix(b, ci, pl)];
+    const uint32_t sppc = b.params[sg0.inst].spp_count;
+    pixel = sg0.pix_first + (pl - sg0.pix_offset);
+    const uint32_t slot0 = sg0.path_first + (pl - sg0.pix_offset) * sppc;
+    film = b.films[sg0.inst];
+    for (uint32_t k = j; k < sppc; k += PT_FILM_BATCH_T) {
+      const uint32_t slot = slot0 + k;
+      float r = w.lx[slot], g = w.ly[slot], bb = w.lz[slot];
+      if (!(isfinite(r) && isfinite(g) && isfinite(bb))) {
+        r = g = bb = 0.f;
+        ++nan;
+      }
+      sr += r; sg += g; sb += bb;
     }
-    if (samples) {
-      size_t o = 3 * ((size_t)pixel * sppc + k);
-      samples[o] = r; samples[o + 1] = g; samples[o + 2] = bb;
-    }
-    sr += r; sg += g; sb += bb;
   }
-  film[3 * pixel] += sr;
-  film[3 * pixel + 1] += sg;
-  film[3 * pixel + 2] += sb;
+  for (uint32_t off = PT_FILM_BATCH_T >> 1; off >= 1u; off >>= 1) {  // every lane of the wave takes part
+    sr += __shfl_xor(sr, (int)off);
+    sg += __shfl_xor(sg, (int)off);
+    sb += __shfl_xor(sb, (int)off);
+  }
+  if (act && j == 0u) {
+    film[3 * pixel] += sr;
+    film[3 * pixel + 1] += sg;
+    film[3 * pixel + 2] += sb;
+  }
   if (nan) atomicAdd(&stats[CNT_NAN], (unsigned long long)nan);
 }
 
@@ -1107,7 +1103,8 @@ void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t ma
 void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
                  unsigned long long *stats, const BatchView *b, uint32_t max_pix) {
   if (b) {
-    hipLaunchKernelGGL(k_film<true>, dim3(blocks_for(max_pix)), dim3(256), 0, st, rp, w, film, samples, stats, *b);
+    hipLaunchKernelGGL(k_film_batch, dim3(blocks_for((size_t)max_pix * PT_FILM_BATCH_T)), dim3(256), 0, st, w, stats,
+                       *b);
   } else {
     const uint32_t npix = rp.chunk_paths / rp.spp_count;
     uint32_t lg = 0;
