@@ -174,6 +174,32 @@ PT_D float conductor_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoi
   return mf_pdf_at(d, b, wo_l, wi);
 }
 
+// conductor_f and conductor_pdf in one pass: the frame, the local directions,
+// the half vector, D and Lambda(wo) are computed once; each output keeps its
+// own early-outs and the same operations in the same order, so both equal
+// the separate functions' bit for bit.
+PT_D void conductor_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
+  f = s3(0.f);
+  pdf = 0.f;
+  TRD d{m.p[6], m.p[7]};
+  if (trd_smooth(d)) return;
+  Frame b = make_frame(h.n);
+  f3 wo_l = to_local(b, norm3(wo)), wi_l = to_local(b, norm3(wi));
+  if (wi_l.z <= 0.f) return;  // both outputs are 0
+  f3 ml = norm3(add3(wo_l, wi_l));
+  const float D = trd_D(d, ml), lo = trd_lambda(d, wo_l);
+  if (wo_l.z > 0.f) {
+    const float G = 1.f / (1.f + lo + trd_lambda(d, wi_l));
+    f3 F = fresnel_complex(fabsf(dot3(wo_l, ml)), mp3(m, 0), mp3(m, 3));
+    float den = 4.f * abs_cost(wo_l) * abs_cost(wi_l);
+    f = divs3(scl3(scl3(F, D), G), den);
+  }
+  if (dot3(h.n, wi) <= 0.f) return;
+  const float den = 4.f * fabsf(dot3(wo_l, ml));
+  if (den < 1e-6f) return;
+  pdf = 1.f / (1.f + lo) / abs_cost(wo_l) * D * fabsf(dot3(wo_l, ml)) / den;
+}
+
 // ---- MFDielectric (dielectric.hpp:5-167, MFDielectricPDF mfDielectricPDF.hpp:7-147) ----
 PT_D f3 mfdiel_f(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
   float eta = m.p[0];
@@ -212,6 +238,41 @@ PT_D float mfdiel_pdf_local(float eta, TRD d, f3 wo, f3 wi) {
   float den = sqrf(dot3(wi, wm) + dot3(wo, wm) / etap);
   float dwm = fabsf(dot3(wi, wm)) / den;
   return trd_Dw(d, wo, wm) * dwm * T / (R + T);
+}
+// mfdiel_f and mfdiel_pdf in one pass (shared frame, local directions,
+// half vector, Fresnel term, D and Lambda(wo)); both outputs equal the
+// separate functions' bit for bit.
+PT_D void mfdiel_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
+  f = s3(0.f);
+  pdf = 0.f;
+  float eta = m.p[0];
+  TRD d{m.p[1], m.p[2]};
+  if (eta == 1.f || trd_smooth(d)) return;
+  Frame b = make_frame(h.n);
+  f3 wo_l = to_local(b, norm3(wo)), wi_l = to_local(b, norm3(wi));
+  float co = wo_l.z, ci = wi_l.z;
+  bool refl = ci * co > 0.f;
+  float etap = 1.f;
+  if (!refl) etap = co > 0.f ? eta : (1.f / eta);
+  f3 wm = add3(scl3(wi_l, etap), wo_l);
+  if (ci == 0.f || co == 0.f || len3(wm) == 0.f) return;
+  wm = face_fwd(norm3(wm), mk3(0.f, 0.f, 1.f));
+  if (dot3(wm, wi_l) * ci < 0.f || dot3(wm, wo_l) * co < 0.f) return;
+  const float F = fresnel_diel(dot3(wo_l, wm), eta), T = 1.f - F;
+  const float D = trd_D(d, wm), lo = trd_lambda(d, wo_l);
+  const float G = 1.f / (1.f + lo + trd_lambda(d, wi_l));
+  const float Dw = 1.f / (1.f + lo) / abs_cost(wo_l) * D * fabsf(dot3(wo_l, wm));
+  if (refl) {
+    f = s3(D * G * F / fabsf(4.f * ci * co));
+    if (F + T != 0.f) pdf = Dw / (4.f * fabsf(dot3(wo_l, wm))) * F / (F + T);
+    return;
+  }
+  const float den = sqrf(dot3(wi_l, wm) + dot3(wo_l, wm) / etap) * ci * co;
+  f = s3(D * (1.f - F) * G * fabsf(dot3(wi_l, wm) * dot3(wo_l, wm) / den));
+  if (F + T != 0.f) {
+    const float den_p = sqrf(dot3(wi_l, wm) + dot3(wo_l, wm) / etap);
+    pdf = Dw * (fabsf(dot3(wi_l, wm)) / den_p) * T / (F + T);
+  }
 }
 PT_D bool mfdiel_scatter(const pt_material_desc &m, f3 wo, const ShadePoint &h, float uc, float u0,
                          float u1, Scatter &s) {
@@ -256,8 +317,7 @@ PT_D bool mfdiel_scatter(const pt_material_desc &m, f3 wo, const ShadePoint &h, 
   bool is_refl = dot3(to_local(b, wi), to_local(b, wo)) > 0.f;
   s.wi = wi;
   s.flags = BX_GLOSSY | (is_refl ? BX_REFL : BX_TRANS);
-  s.att = mfdiel_f(m, wo, wi, h);
-  s.pdf = mfdiel_pdf_local(eta, d, wo_l, to_local(b, norm3(wi)));
+  mfdiel_f_pdf(m, wo, wi, h, s.att, s.pdf);
   return true;
 }
 PT_D float mfdiel_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
@@ -445,6 +505,20 @@ PT_D float bsdf_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h
   if (PT_IN(MS, PT_MAT_HAIR) && k == PT_MAT_HAIR) return hair_pdf(m, wo, wi, h);
   if (PT_IN(MS, PT_MAT_KAJIYA_KAY) && k == PT_MAT_KAJIYA_KAY) return cosine_pdf(h.n, wi);
   return 0.f;
+}
+// bsdf_f and bsdf_pdf of one direction, fused where the kinds share terms
+// (the results equal the two calls').
+template <uint32_t MS = kMatAll>
+PT_D void bsdf_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
+  const uint32_t k = m.kind;
+  if (PT_IN(MS, PT_MAT_CONDUCTOR) && k == PT_MAT_CONDUCTOR) {
+    conductor_f_pdf(m, wo, wi, h, f, pdf);
+  } else if (PT_IN(MS, PT_MAT_MFDIELECTRIC) && k == PT_MAT_MFDIELECTRIC) {
+    mfdiel_f_pdf(m, wo, wi, h, f, pdf);
+  } else {
+    f = bsdf_f<MS>(m, wo, wi, h);
+    pdf = bsdf_pdf<MS>(m, wo, wi, h);
+  }
 }
 // DiffuseEmitter::emit (material.hpp:487-493)
 PT_D f3 bsdf_emit(const pt_material_desc &m, f3 ray_d, const ShadePoint &h) {

@@ -741,8 +741,9 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
               if (bsdf_reflect_only(mat->kind) && dot3(h.sp.n, lwi) <= 0.f) pl = 0.f;
               if (pl > 0.f) {
                 pl /= (float)N;
-                f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
-                float pb = bsdf_pdf<MS>(*mat, wo, lwi, h.sp);
+                f3 f;
+                float pb;
+                bsdf_f_pdf<MS>(*mat, wo, lwi, h.sp, f, pb);
                 if (EST == PT_EST_GUIDED && gc > 0.f) pb = gc * g_pdf(mix, lwi) + (1.f - gc) * pb;
                 float cl = fabsf(dot3(h.sp.n, lwi));
                 f3 c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
@@ -946,9 +947,11 @@ __global__ void k_bsdf_eval(pt_material_desc m, const float *wo, const float *wi
   h.inside = inside ? inside[i] != 0 : true;
   f3 o = mk3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
   f3 w = mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
-  f3 r = bsdf_f(m, o, w, h);
+  f3 r;
+  float p;
+  bsdf_f_pdf(m, o, w, h, r, p);  // the renderer's evaluation of a light direction
   f[3 * i] = r.x; f[3 * i + 1] = r.y; f[3 * i + 2] = r.z;
-  pdf[i] = bsdf_pdf(m, o, w, h);
+  pdf[i] = p;
 }
 
 __global__ void k_bsdf_sample(pt_material_desc m, const float *wo, const float *nrm, const uint8_t *inside,
