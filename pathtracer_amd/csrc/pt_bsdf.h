@@ -515,6 +515,8 @@ PT_D void bsdf_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &
     conductor_f_pdf(m, wo, wi, h, f, pdf);
   } else if (PT_IN(MS, PT_MAT_MFDIELECTRIC) && k == PT_MAT_MFDIELECTRIC) {
     mfdiel_f_pdf(m, wo, wi, h, f, pdf);
+  } else if (PT_IN(MS, PT_MAT_DISNEY) && k == PT_MAT_DISNEY) {
+    disney_f_pdf(m, wo, wi, h, f, pdf);
   } else {
     f = bsdf_f<MS>(m, wo, wi, h);
     pdf = bsdf_pdf<MS>(m, wo, wi, h);

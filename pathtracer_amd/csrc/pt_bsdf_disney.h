@@ -273,6 +273,124 @@ PT_D float disney_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint 
   p += (d.cw / sum) * dcc_pdf(d, wo, wi, h);
   return p;
 }
+// Glass lobe f and pdf from the normalised local directions (dglass_f and
+// dglass_pdf's operations, shared terms once).
+PT_D void dglass_f_pdf_local(const Disney &d, f3 wo_l, f3 wi_l, f3 &f, float &pdf) {
+  f = s3(0.f);
+  pdf = 0.f;
+  const float eta = d.eta;
+  const TRD t = d.dist;
+  if (eta == 1.f || trd_smooth(t)) return;
+  float co = wo_l.z, ci = wi_l.z;
+  bool refl = ci * co > 0.f;
+  float etap = 1.f;
+  if (!refl) etap = co > 0.f ? eta : (1.f / eta);
+  f3 wm = add3(scl3(wi_l, etap), wo_l);
+  if (ci == 0.f || co == 0.f || len3(wm) == 0.f) return;
+  wm = face_fwd(norm3(wm), mk3(0.f, 0.f, 1.f));
+  if (dot3(wm, wi_l) * ci < 0.f || dot3(wm, wo_l) * co < 0.f) return;
+  const float F = fresnel_diel(dot3(wo_l, wm), eta), T = 1.f - F;
+  const float D = trd_D(t, wm), lo = trd_lambda(t, wo_l);
+  const float G = 1.f / (1.f + lo + trd_lambda(t, wi_l));
+  const float Dw = 1.f / (1.f + lo) / abs_cost(wo_l) * D * fabsf(dot3(wo_l, wm));
+  if (refl) {
+    f = scl3(d.base, D * G * F / fabsf(4.f * ci * co));
+    if (F + T != 0.f) pdf = Dw / (4.f * fabsf(dot3(wo_l, wm))) * F / (F + T);
+    return;
+  }
+  const float den = sqrf(dot3(wi_l, wm) + dot3(wo_l, wm) / etap) * ci * co;
+  f3 sc = mk3(safe_sqrtf(d.base.x), safe_sqrtf(d.base.y), safe_sqrtf(d.base.z));
+  f = scl3(sc, D * (1.f - F) * G * fabsf(dot3(wi_l, wm) * dot3(wo_l, wm) / den));
+  if (F + T != 0.f) {
+    const float den_p = sqrf(dot3(wi_l, wm) + dot3(wo_l, wm) / etap);
+    pdf = Dw * (fabsf(dot3(wi_l, wm)) / den_p) * T / (F + T);
+  }
+}
+
+// disney_f and disney_pdf in one pass: the parameters, the shading frame, the
+// local directions (as given and normalised, as the lobes use them), the half
+// vectors and the shared microfacet terms once.  Every lobe keeps its own
+// early-outs and operation order, so f and pdf equal the separate functions'
+// bit for bit.
+PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
+  const Disney d = disney_params(m);
+  const Frame b = make_frame(h.n);
+  const f3 wo_n = to_local(b, norm3(wo)), wi_n = to_local(b, norm3(wi));
+  f3 gf;
+  float gp;
+  dglass_f_pdf_local(d, wo_n, wi_n, gf, gp);
+  if (h.inside) {
+    f = gf;
+    pdf = gp;
+    return;
+  }
+  const f3 wo_r = to_local(b, wo), wi_r = to_local(b, wi);  // ddiffuse_f, dmetal_f_c0, dcc_f, dsheen_f
+  const f3 wh = norm3(add3(wo_r, wi_r));
+  const float whwi = dot3(wh, wi_r);
+  // diffuse (ddiffuse_f)
+  f3 fd;
+  {
+    float aci = fabsf(wi_r.z), aco = fabsf(wo_r.z);
+    float fd90 = 0.5f + 2.f * d.roughness * powf(fabsf(whwi), 2.f);
+    f3 bd = scl3(scl3(d.base, 1.f / PT_PI), hacked_schlick(aci, fd90) * hacked_schlick(aco, fd90));
+    float fss90 = d.roughness * powf(fabsf(whwi), 2.f);
+    f3 bs = scl3(scl3(d.base, 1.25f), 1.f / PT_PI);
+    bs = scl3(bs, hacked_schlick(aci, fss90) * hacked_schlick(aco, fss90) * (1.f / (aci + aco) - 0.5f) + 0.5f);
+    fd = add3(scl3(bd, 1.f - d.subsurface), scl3(bs, d.subsurface));
+  }
+  // metal (dmetal_f_variant -> dmetal_f_c0)
+  f3 fm = s3(0.f);
+  if (!(wi_r.z <= 0.f || wo_r.z <= 0.f || trd_smooth(d.dist))) {
+    f3 ks = add3(s3(0.5f), scl3(d.base, 0.5f));
+    f3 c0 = add3(scl3(scl3(ks, 0.f * (1.f - 0.5f)), 0.5f), scl3(d.base, 0.5f));
+    f3 F = fresnel_schlick(fabsf(whwi), c0);
+    float D = trd_D(d.dist, wh), G = trd_G(d.dist, wo_r, wi_r);
+    fm = divs3(scl3(scl3(F, D), G), fabsf(4.f * wi_r.z * wo_r.z));
+  }
+  // clearcoat (dcc_f)
+  f3 fc = s3(0.f);
+  if (!(wi_r.z <= 0.f || wo_r.z <= 0.f)) {
+    float r0 = sqrf((1.5f - 1) / (1.5f + 1));
+    f3 Fc = fresnel_schlick(fabsf(whwi), s3(r0));
+    float Dc = cc_D(d.cc_alpha, wh), Gc = 1.f / (1.f + cc_lambda(wo_r) + cc_lambda(wi_r));
+    fc = divs3(scl3(scl3(Fc, Dc), Gc), fabsf(4.f * wi_r.z * wo_r.z));
+  }
+  // sheen (dsheen_f)
+  f3 fs;
+  {
+    float a = fabsf(whwi);
+    float lum = rgb_luminance(d.base);
+    f3 ct = lum > 0.f ? divs3(d.base, lum) : s3(1.f);
+    f3 cs = add3(s3(1.f), scl3(sub3(ct, s3(1.f)), d.sheen_tint));
+    fs = scl3(cs, powf(1 - a, 5.f));
+  }
+  f3 r = scl3(fd, d.dw);
+  r = add3(r, scl3(fm, d.mw));
+  r = add3(r, scl3(gf, d.gw));
+  r = add3(r, scl3(fc, d.cw));
+  r = add3(r, scl3(fs, d.sw));
+  f = r;
+  // pdf: the lobes' pdfs at the normalised directions
+  float pm = 0.f, pc = 0.f;
+  if (wi_n.z > 0.f) {
+    const f3 hn = norm3(add3(wo_n, wi_n));
+    if (!trd_smooth(d.dist)) {  // dmetal_pdf -> mf_pdf_at
+      float den = 4.f * fabsf(dot3(wo_n, hn));
+      if (!(den < 1e-6f)) pm = trd_Dw(d.dist, wo_n, hn) / den;
+    }
+    {  // dcc_pdf -> cc_pdf_at
+      float den = 4.f * fabsf(dot3(wi_n, hn));
+      if (!(den < 1e-6f)) pc = cc_D(d.cc_alpha, hn) * fabsf(hn.z) / den;
+    }
+  }
+  float sum = d.dw + d.mw + d.gw + d.cw;
+  float p = (d.dw / sum) * cosine_pdf(h.n, wi);
+  p += (d.mw / sum) * pm;
+  p += (d.gw / sum) * gp;
+  p += (d.cw / sum) * pc;
+  pdf = p;
+}
+
 // gl::sampleDiscrete (utility.hpp:155-190)
 template <int N>
 PT_D int sample_discrete(const float (&w)[N], float u, float *pmf, float *u_remap) {
@@ -322,8 +440,7 @@ PT_D bool disney_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float 
   if (s.flags & BX_SPECULAR) return true;
   if (s.pdf <= 1e-6f) return false;
   f3 wiw = norm3(s.wi);
-  s.att = disney_f(m, wo, wiw, h);
-  s.pdf = disney_pdf(m, wo, wiw, h);
+  disney_f_pdf(m, wo, wiw, h, s.att, s.pdf);
   return s.pdf > 1e-7f;
 }
 
