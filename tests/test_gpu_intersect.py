@@ -124,3 +124,27 @@ def test_env_sample_matches_oracle(gpu):
     np.testing.assert_allclose(gd, od, atol=2e-5)
     assert np.all(np.isfinite(gp))
     assert (gp > 0).mean() > 0.999   # pdf 0 only at the pole (u1 = 0: sin(theta) = 0)
+
+
+def test_ten_million_rays(gpu):
+    """Oracle level 2 at its full size: 10^7 seeded rays on the 20,480-triangle
+    displaced mesh with a sphere and a rect, closest-hit ids identical up to
+    exact ties and t within 1e-5 relative."""
+    from pathtracer_amd import render as R, scenes
+    spec = scenes.SceneSpec()
+    m = spec.add_material(1, [0.5, 0.5, 0.5])
+    pos, idx = scenes.displaced_icosphere(5, 1.0, 0.08, 2)
+    spec.add_mesh(pos, idx, m)
+    spec.add_sphere((2.0, 0.0, 0.0), 0.5, m)
+    spec.add_rect(1, -1.2, -3, 3, -3, 3, m)
+    spec.camera = dict(fov_deg=40, aspect=1, focal_dist=1, aperture=0, up=(0, 1, 0), front=(0, 0, -1), pos=(0, 0, 5))
+    g = R.Scene(spec)
+    o = O.OracleScene(spec)
+    rng = np.random.default_rng(17)
+    total_ties = 0
+    for k in range(5):   # 5 x 2M rays
+        rays = _rays_towards(rng, 2_000_000, [-1.2, -1.2, -1.2], [2.5, 1.2, 1.2], ([-4, -1, 3], [4, 4, 6]))
+        gp, gt = g.intersect(rays)
+        cp, ct = o.intersect(rays)
+        total_ties += _compare(gp, gt, cp, ct)
+    assert total_ties < 1000, total_ties
