@@ -9,7 +9,7 @@
 //                                    the other live-path buffer
 //                  k_shadow          any-hit of the shadow queue, adds the
 //                                    unoccluded contributions
-//   k_film                           per-pixel sums (fixed order)
+//   k_film_lanes / k_film_batch      per-pixel sums (fixed order)
 // Live paths are stored densely, so extension and shading read and write
 // them with unit stride; only radiance is addressed by slot.
 // The estimator is getRayColor of integrator/maxdepth_nee.hpp:9-256
@@ -290,6 +290,9 @@ PT_D uint32_t tile_order16(uint32_t key, uint32_t (*cnt)[16], uint32_t *first, u
 #ifndef PT_SHADE_WAVES
 #define PT_SHADE_WAVES 4
 #endif
+// The same for the all-kinds variant (Disney and hair): 3 waves/SIMD (150
+// VGPRs, no spills) shaded the 64 Disney + 8 hair spheres in 116.5 ms against
+// 109.7 ms at 4 waves (128 VGPRs, 48 bytes per lane spilled).
 #ifndef PT_SHADE_WAVES_ALL
 #define PT_SHADE_WAVES_ALL 4
 #endif
