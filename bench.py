@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 B_EXT_RAY = 24 + 8            # ray (o, d) read + hit (t, prim) write, unit stride
 B_SHADOW_RAY = 44             # shadow-queue entry (o, d, tmax, contribution, slot)
 B_NODE = 112                  # BVH4 node: four child boxes (SoA) + four child codes
+B_QNODE = 64                  # any-hit BVH4 node: four quantised child boxes, origin and scales, child codes
 B_PRIM = 48                   # triangle record (v0, e1, e2)
 # shading, per extension segment: the live-path record (ray 24, hit 8,
 # throughput 12, flags 4, MIS pdf + vertex 16, slot 4 = 68) plus primitive
@@ -137,7 +138,7 @@ def kernel_roofline(cst, ms, launches, read_traffic_for=None):
     kernels = {
         "k_extend": ("closest-hit traversal", cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE +
                      cst["prim_tests"] * B_PRIM, ms["ms_extend"]),
-        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_NODE +
+        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_QNODE +
                      cst["shadow_prim_tests"] * B_PRIM, ms["ms_shadow"]),
         "k_shade": ("materials, lights, NEE, queue compaction", cst["segments"] * B_SHADE_SEG +
                     cst["shadow_rays"] * B_SHADE_SHADOW, ms["ms_shade"]),

@@ -257,6 +257,16 @@ typedef struct pt_build_cfg {
 /* Traversal reads every BVH node from global memory (no LDS copy of the top
  * levels).                                                                  */
 #define PT_BUILD_NO_LDS_NODES 2u
+/* Node format of any-hit (shadow-ray) traversal.  Every commit builds the
+ * 128-byte nodes (four full-precision child boxes) that closest-hit
+ * traversal reads, and a 64-byte copy whose child boxes are 8-bit offsets
+ * from a per-node origin with per-axis power-of-two scales, rounded outwards
+ * (conservative: the same hits).  Any hit reads the 64-byte copy when the
+ * 128-byte nodes exceed 4 MiB (about 32k nodes), where it is bound by the
+ * node bytes it fetches; these flags force one format.  A batch's shadow rays
+ * read the 64-byte nodes when any of its scenes selects them.               */
+#define PT_BUILD_ANY_HIT_QNODES 4u
+#define PT_BUILD_ANY_HIT_FULL_NODES 8u
 
 /* Builds the BVH, discovers emissive rects/spheres as lights
  * (base/lightDiscovery.hpp:5-81) and uploads everything to the device.      */

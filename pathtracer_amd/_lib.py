@@ -29,6 +29,7 @@ RENDER_KEEP_SAMPLES, RENDER_TIMING = 8, 16
 BATCH_NO_GRAPH = 256
 TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF, TUNE_SORT = 1, 2, 4, 8
 BUILD_DEVICE_LBVH, BUILD_NO_LDS_NODES = 1, 2
+BUILD_ANY_HIT_QNODES, BUILD_ANY_HIT_FULL_NODES = 4, 8
 GUIDE_NO_TRAIN = 1
 GUIDE_INPUTS, GUIDE_OUTPUTS, GUIDE_NPARAMS, GUIDE_MIX = 6, 41, 27497, 48
 PRIM_FLIP_FACE, PRIM_FORCE_INSIDE, PRIM_NOT_LIGHT = 1, 2, 4

@@ -177,11 +177,35 @@ PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, floa
 // LDS words per block for a stack of K entries (codes, plus distances); at
 // least one word so that the declaration stays valid for K = 0.
 #define PT_TRAV_LDS(K, WITH_T) ((K) > 0 ? (K) * PT_BLOCK * ((WITH_T) ? 2 : 1) : 1)
-// Copies the scene's top nodes into the block's LDS (every thread calls it).
+// Copies the scene's top nodes into the block's LDS (every thread calls it):
+// the quantised nodes for any-hit traversal (QN), else the 128-byte nodes.
+template <bool QN>
 PT_D void stage_top_nodes(const DevScene &s, float4 *top, bool batch) {
-  const uint32_t n = batch ? 0u : 8u * s.top_nodes;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) top[i] = s.nodes[i];
+  const uint32_t n = batch ? 0u : (QN ? (uint32_t)PT_QNODE_F4 : 8u) * s.top_nodes;
+  const float4 *src = QN ? s.qnodes : s.nodes;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) top[i] = src[i];
   __syncthreads();
+}
+// float4 per node in the array traversal reads (QN: any hit)
+template <bool QN>
+constexpr int node_f4() { return QN ? PT_QNODE_F4 : 8; }
+// The four child boxes (SoA) and codes of node nd: a quantised node (QN,
+// decode_node) or a 128-byte one.
+template <bool QN>
+PT_D void load_node(const float4 *nd, float (&lx)[4], float (&hx)[4], float (&ly)[4], float (&hy)[4],
+                    float (&lz)[4], float (&hz)[4], int4 &ch) {
+  if constexpr (QN) {
+    decode_node(nd, lx, hx, ly, hy, lz, hz, ch);
+  } else {
+    const float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3], e = nd[4], f = nd[5];
+    ch = *reinterpret_cast<const int4 *>(nd + 6);
+    lx[0] = a.x; lx[1] = a.y; lx[2] = a.z; lx[3] = a.w;
+    hx[0] = b.x; hx[1] = b.y; hx[2] = b.z; hx[3] = b.w;
+    ly[0] = c.x; ly[1] = c.y; ly[2] = c.z; ly[3] = c.w;
+    hy[0] = d.x; hy[1] = d.y; hy[2] = d.z; hy[3] = d.w;
+    lz[0] = e.x; lz[1] = e.y; lz[2] = e.z; lz[3] = e.w;
+    hz[0] = f.x; hz[1] = f.y; hz[2] = f.z; hz[3] = f.w;
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -189,7 +213,9 @@ PT_D void stage_top_nodes(const DevScene &s, float4 *top, bool batch) {
 // tested together, the nearest hit child is visited next and the others are
 // pushed far-to-near with their entry distance, so a pop whose box starts
 // beyond the current closest hit is skipped.  ANY: stop at the first
-// primitive hit in [tmin, tmax] (shadow rays); no ordering needed.
+// primitive hit in [tmin, tmax] (shadow rays); no ordering needed.  QN: read
+// the 64-byte quantised nodes (conservative boxes, same hits) instead of the
+// 128-byte ones.
 // `lds` is the calling block's PT_TRAV_LDS(K, !ANY) words of LDS.
 // ---------------------------------------------------------------------
 #define PT_CSWAP(ta, ca, tb_, cb)          \
@@ -197,7 +223,7 @@ PT_D void stage_top_nodes(const DevScene &s, float4 *top, bool batch) {
     float _t = ta; ta = tb_; tb_ = _t;    \
     int _c = ca; ca = cb; cb = _c;        \
   }
-template <bool ANY, bool COUNT, int K>
+template <bool ANY, bool QN, bool COUNT, int K>
 PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tmax, float &t_out, int &prim_out,
                    uint32_t &n_nodes, uint32_t &n_prims) {
   const float ix = rcp_fast(d.x), iy = rcp_fast(d.y), iz = rcp_fast(d.z);
@@ -244,14 +270,11 @@ PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tm
   int cur = s.root;
   while (true) {
     if (cur >= 0) {
-      const float4 *nd = s.nodes + 8 * (size_t)cur;
-      const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
-      const int4 ch = *reinterpret_cast<const int4 *>(nd + 6);
+      float lxs[4], hxs[4], lys[4], hys[4], lzs[4], hzs[4];
+      int4 ch;
+      load_node<QN>((QN ? s.qnodes : s.nodes) + node_f4<QN>() * (size_t)cur, lxs, hxs, lys, hys, lzs, hzs, ch);
       if (COUNT) ++n_nodes;
       float tn[4], tf[4];
-      const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
-      const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
-      const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         float a0 = fmaf(lxs[k], ix, -oix), a1 = fmaf(hxs[k], ix, -oix);
@@ -338,13 +361,16 @@ PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tm
 // at the ray's instance); finish(i, hit, t, prim) stores it.  begin, end must
 // be the same in every lane of the wave.
 // ---------------------------------------------------------------------
-template <bool ANY, bool COUNT, int K, int REFILL, bool BATCH, class Fetch, class Finish>
+template <bool ANY, bool QN, bool COUNT, int K, int REFILL, bool BATCH, class Fetch, class Finish>
 PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t begin, uint32_t end, Fetch fetch,
                       Finish finish, uint32_t &n_nodes, uint32_t &n_prims) {
   // nodes [0, ntop) are read from the block's LDS copy `top` (single scenes)
   const uint32_t ntop = BATCH ? 0u : s.top_nodes;
   // BATCH: fetch names each ray's scene, whose tree the lane then walks
-  const float4 *lane_nodes = s.nodes, *lane_prims = s.prims;
+  // QN: walk the quantised nodes (any hit over a large tree)
+  const float4 *const snodes = QN ? s.qnodes : s.nodes;
+  constexpr int NF4 = node_f4<QN>();
+  const float4 *lane_nodes = snodes, *lane_prims = s.prims;
   int lane_root = s.root;
   constexpr bool WT = !ANY;
   int pc[PT_STACK4];
@@ -407,22 +433,14 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
   // one inner-node step: test the 4 children, go to the nearest hit one and
   // push the others far to near, else pop
   auto node_step = [&]() {
-    float4 lx, hx, ly, hy, lz, hz;
+    float lxs[4], hxs[4], lys[4], hys[4], lzs[4], hzs[4];
     int4 ch;
-    if ((uint32_t)cur < ntop) {
-      const float4 *nd = top + 8 * cur;
-      lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
-      ch = *reinterpret_cast<const int4 *>(nd + 6);
-    } else {
-      const float4 *nd = (BATCH ? lane_nodes : s.nodes) + 8 * (size_t)cur;
-      lx = nd[0]; hx = nd[1]; ly = nd[2]; hy = nd[3]; lz = nd[4]; hz = nd[5];
-      ch = *reinterpret_cast<const int4 *>(nd + 6);
-    }
+    if ((uint32_t)cur < ntop)
+      load_node<QN>(top + NF4 * cur, lxs, hxs, lys, hys, lzs, hzs, ch);
+    else
+      load_node<QN>((BATCH ? lane_nodes : snodes) + NF4 * (size_t)cur, lxs, hxs, lys, hys, lzs, hzs, ch);
     if (COUNT) ++n_nodes;
     float tn[4], tf[4];
-    const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       float a0 = fmaf(lxs[k], ix, -oix), a1 = fmaf(hxs[k], ix, -oix);
@@ -468,7 +486,7 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
         const DevScene *sc = &s;
         fetch(mine, o, d, tmin, tmax, sc);
         if (BATCH) {
-          lane_nodes = sc->nodes;
+          lane_nodes = QN ? sc->qnodes : sc->nodes;
           lane_prims = sc->prims;
           lane_root = sc->root;
         }

@@ -136,7 +136,7 @@ template <bool COUNT, bool BATCH>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_extend(DevScene s, Wave w, int qi, BatchView b) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_EXT, true)];
   __shared__ float4 top[8 * PT_TOP_NODES];
-  stage_top_nodes(s, top, BATCH);
+  stage_top_nodes<false>(s, top, BATCH);
   const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
   const PathState &ps = w.st[qi];
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   const uint32_t wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
   const uint32_t per = (n + nw - 1) / nw;
   const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
-  trace_range<false, COUNT, PT_K_EXT, PT_REFILL_EXT, BATCH>(
+  trace_range<false, false, COUNT, PT_K_EXT, PT_REFILL_EXT, BATCH>(
       s, lds, top, begin, end,
       [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
@@ -178,11 +178,11 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
 // ------------------------------------------------------------------ shadow
 // VIS: the entries are reservoir candidates (slot field = 4 slot + candidate),
 // whose visibility is recorded instead of adding a contribution.
-template <bool COUNT, bool VIS, bool BATCH>
+template <bool COUNT, bool VIS, bool BATCH, bool QN>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_shadow(DevScene s, Wave w, BatchView b) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_SHADOW, false)];
-  __shared__ float4 top[8 * PT_TOP_NODES];
-  stage_top_nodes(s, top, BATCH);
+  __shared__ float4 top[node_f4<QN>() * PT_TOP_NODES];
+  stage_top_nodes<QN>(s, top, BATCH);
   // blocks b, b + P, b + 2P, ... share partition p = b % P (the grid is a multiple of P)
   const uint32_t p = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = min(*w.shadow(p), w.shadow_part_cap);
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   const uint32_t wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
   const uint32_t per = (n + nw - 1) / nw;
   const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
-  trace_range<true, COUNT, PT_K_SHADOW, PT_REFILL_SHADOW, BATCH>(
+  trace_range<true, QN, COUNT, PT_K_SHADOW, PT_REFILL_SHADOW, BATCH>(
       s, lds, top, begin, end,
       [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
@@ -933,10 +933,12 @@ __global__ void __launch_bounds__(PT_BLOCK) k_intersect(DevScene s, const float 
   int pr;
   uint32_t a = 0, b = 0;
   bool hit;
-  if (any)
-    hit = traverse<true, false, PT_K_SHADOW>(s, lds, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
+  if (any && s.qany)
+    hit = traverse<true, true, false, PT_K_SHADOW>(s, lds, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
+  else if (any)
+    hit = traverse<true, false, false, PT_K_SHADOW>(s, lds, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
   else
-    hit = traverse<false, false, PT_K_EXT>(s, lds, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
+    hit = traverse<false, false, false, PT_K_EXT>(s, lds, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6], r[7], t, pr, a, b);
   prim[i] = hit ? (any ? 1 : pr) : -1;
   tout[i] = hit ? (any ? 0.f : t) : __int_as_float(0x7f800000);
 }
@@ -1090,7 +1092,16 @@ void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t ma
                    bool visibility, const BatchView *b) {
   uint32_t g = part_grid(max_n, grid_cap);
   const BatchView bv = b ? *b : BatchView{};
-#define PT_SHADOW_LAUNCH(C, V, B) hipLaunchKernelGGL((k_shadow<C, V, B>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, bv)
+  // any hit reads the quantised nodes when the scene (batched: the first
+  // scene, which carries the batch's choice) selects them
+  const bool qn = s.qany != 0u;
+#define PT_SHADOW_LAUNCH(C, V, B)                                                                     \
+  do {                                                                                                \
+    if (qn)                                                                                           \
+      hipLaunchKernelGGL((k_shadow<C, V, B, true>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, bv);       \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_shadow<C, V, B, false>), dim3(g), dim3(PT_BLOCK), 0, st, s, w, bv);      \
+  } while (0)
   if (b) {
     if (visibility) {
       if (count) PT_SHADOW_LAUNCH(true, true, true); else PT_SHADOW_LAUNCH(false, true, true);

@@ -39,4 +39,8 @@ void lbvh_carve(void *base, uint32_t n, LbvhScratch &sc);
 hipError_t lbvh_build(hipStream_t st, const float4 *prims, uint32_t n, uint32_t leaf_size, LbvhScratch &sc,
                       float4 *prims_out, float4 *nodes4, LbvhResult &res);
 
+// The n 128-byte nodes (8 float4 each) as quantised any-hit traversal nodes
+// (PT_QNODE_F4 float4 each, quantize_node).
+hipError_t lbvh_quantize(hipStream_t st, const float4 *nodes8, uint32_t n, float4 *nodes_q);
+
 }  // namespace pt

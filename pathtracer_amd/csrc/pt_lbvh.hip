@@ -363,6 +363,17 @@ __global__ void k_lb_single(const float4 *leafbox, float4 *nodes4) {
 
 static uint32_t lb_blocks(size_t n) { return (uint32_t)((n + 255) / 256); }
 
+__global__ void k_lb_quantize(const float4 *in, uint32_t n, float4 *out) {
+  const uint32_t i = l_tid();
+  if (i >= n) return;
+  quantize_node(in + 8 * (size_t)i, out + PT_QNODE_F4 * (size_t)i);
+}
+
+hipError_t lbvh_quantize(hipStream_t st, const float4 *nodes8, uint32_t n, float4 *nodes_q) {
+  hipLaunchKernelGGL(k_lb_quantize, dim3(lb_blocks(n)), dim3(256), 0, st, nodes8, n, nodes_q);
+  return hipGetLastError();
+}
+
 static size_t sort_bytes(uint32_t n) {
   size_t tb = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,

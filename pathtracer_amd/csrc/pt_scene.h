@@ -2,8 +2,12 @@
 // host orchestration (pt_host.cpp) and the kernels (pt_kernels.hip).
 //
 // HBM layout (all SoA, 256-byte aligned allocations):
-//   nodes      BVH2 nodes, 64 B each: the boxes of both children plus their
-//              indices, so one fetch decides both children (4 x float4).
+//   nodes      4-wide BVH nodes, 128 B each (8 x float4): lo.x, hi.x, lo.y,
+//              hi.y, lo.z, hi.z of the four children (SoA), their four child
+//              codes, one unused float4.  Closest-hit traversal reads these.
+//   qnodes     the same nodes at 64 B each (4 x float4, quantize_node): the
+//              child boxes as 8-bit offsets, conservative.  Any-hit traversal
+//              reads these.
 //   prims      primitives in BVH leaf order, 48 B each (3 x float4):
 //                triangle  (v0.xyz, id|type) (e1.xyz, -) (e2.xyz, -)
 //                sphere    (c.xyz,  id|type) (r, -, -, -) (-)
@@ -40,6 +44,112 @@ constexpr int kEmptyChild = (int)0x80000000;
 // full levels of 4-wide nodes) at most.
 #define PT_TOP_NODES 21
 PT_HD int leaf_code(uint32_t start, uint32_t count) { return ~(int)((start << 4) | (count - 1)); }
+// ---- 4-wide node, 64 bytes (4 x float4) -----------------------------------
+//   [0] origin x, y, z; w: the exponent bytes e_a + 127 of the per-axis
+//       scales s_a = 2^e_a (bits 0-7 x, 8-15 y, 16-23 z)
+//   [1] lo_x, hi_x, lo_y, hi_y   as uint32: byte k = child k's bound
+//   [2] lo_z, hi_z, 0, 0
+//   [3] the four child codes (int)
+// Child k's box on axis a is [fma(lo_a[k], s_a, o_a), fma(hi_a[k], s_a, o_a)],
+// which quantize_node makes contain the full-precision box: the same fma
+// decides at build time and in traversal, and the slab distances are monotone
+// in the bounds, so a ray's box test passes wherever it passes on the full
+// node.  Any-hit traversal reads these nodes: it is bound by the node bytes
+// it fetches.  Closest hit keeps the 128-byte nodes, where the decode's
+// arithmetic cost more than the halved fetch saved.
+#define PT_QNODE_F4 4
+PT_HD float pt_bits_float(uint32_t u) {
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+PT_HD uint32_t pt_float_bits(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return u;
+}
+// The full-precision node `in` (8 x float4: lo.x, hi.x, lo.y, hi.y, lo.z,
+// hi.z of the four children, then their codes) as a quantised node `out`.
+PT_HD void quantize_node(const float4 *in, float4 *out) {
+  const float lo[3][4] = {{in[0].x, in[0].y, in[0].z, in[0].w}, {in[2].x, in[2].y, in[2].z, in[2].w},
+                          {in[4].x, in[4].y, in[4].z, in[4].w}};
+  const float hi[3][4] = {{in[1].x, in[1].y, in[1].z, in[1].w}, {in[3].x, in[3].y, in[3].z, in[3].w},
+                          {in[5].x, in[5].y, in[5].z, in[5].w}};
+  const int codes[4] = {(int)pt_float_bits(in[6].x), (int)pt_float_bits(in[6].y), (int)pt_float_bits(in[6].z),
+                        (int)pt_float_bits(in[6].w)};
+  float org[3];
+  uint32_t ql[3], qh[3], eb = 0;
+  for (int a = 0; a < 3; ++a) {
+    float o = 0.f, m = 0.f;
+    bool any = false;
+    for (int k = 0; k < 4; ++k) {
+      if (codes[k] == kEmptyChild) continue;
+      o = any ? fminf(o, lo[a][k]) : lo[a][k];
+      m = any ? fmaxf(m, hi[a][k]) : hi[a][k];
+      any = true;
+    }
+    const float E = m - o;
+    int e = -126;
+    if (E > 0.f) {
+      int x;
+      (void)frexpf(E, &x);  // 2^(x-1) <= E < 2^x
+      e = x - 9;
+      if (e < -126) e = -126;
+      while (e < 127 && ldexpf(255.f, e) < E) ++e;
+    }
+    for (;; ++e) {  // widen the scale until every child's box fits
+      const float s = ldexpf(1.f, e);
+      uint32_t L = 0, H = 0;
+      bool ok = true;
+      for (int k = 0; k < 4 && ok; ++k) {
+        uint32_t l = 255u, h = 0u;  // empty slot: an inverted box
+        if (codes[k] != kEmptyChild) {
+          float fl = floorf((lo[a][k] - o) / s), fh = ceilf((hi[a][k] - o) / s);
+          fl = fminf(fmaxf(fl, 0.f), 255.f);
+          fh = fminf(fmaxf(fh, 0.f), 255.f);
+          while (fl > 0.f && fmaf(fl, s, o) > lo[a][k]) fl -= 1.f;
+          while (fh < 255.f && fmaf(fh, s, o) < hi[a][k]) fh += 1.f;
+          ok = fmaf(fl, s, o) <= lo[a][k] && fmaf(fh, s, o) >= hi[a][k];
+          l = (uint32_t)fl;
+          h = (uint32_t)fh;
+        }
+        L |= l << (8 * k);
+        H |= h << (8 * k);
+      }
+      if (ok || e >= 127) {
+        ql[a] = L;
+        qh[a] = H;
+        break;
+      }
+    }
+    org[a] = o;
+    eb |= (uint32_t)(e + 127) << (8 * a);
+  }
+  out[0] = make_float4(org[0], org[1], org[2], pt_bits_float(eb));
+  out[1] = make_float4(pt_bits_float(ql[0]), pt_bits_float(qh[0]), pt_bits_float(ql[1]), pt_bits_float(qh[1]));
+  out[2] = make_float4(pt_bits_float(ql[2]), pt_bits_float(qh[2]), 0.f, 0.f);
+  out[3] = in[6];
+}
+// Decodes node nd: the child boxes (SoA over the four children) and codes.
+PT_HD void decode_node(const float4 *nd, float (&lx)[4], float (&hx)[4], float (&ly)[4], float (&hy)[4],
+                       float (&lz)[4], float (&hz)[4], int4 &ch) {
+  const float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+  const uint32_t eb = pt_float_bits(a.w);
+  const float sx = pt_bits_float((eb & 255u) << 23), sy = pt_bits_float(((eb >> 8) & 255u) << 23),
+              sz = pt_bits_float(((eb >> 16) & 255u) << 23);
+  const uint32_t qlx = pt_float_bits(b.x), qhx = pt_float_bits(b.y), qly = pt_float_bits(b.z),
+                 qhy = pt_float_bits(b.w), qlz = pt_float_bits(c.x), qhz = pt_float_bits(c.y);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    lx[k] = fmaf((float)((qlx >> (8 * k)) & 255u), sx, a.x);
+    hx[k] = fmaf((float)((qhx >> (8 * k)) & 255u), sx, a.x);
+    ly[k] = fmaf((float)((qly >> (8 * k)) & 255u), sy, a.y);
+    hy[k] = fmaf((float)((qhy >> (8 * k)) & 255u), sy, a.y);
+    lz[k] = fmaf((float)((qlz >> (8 * k)) & 255u), sz, a.z);
+    hz[k] = fmaf((float)((qhz >> (8 * k)) & 255u), sz, a.z);
+  }
+  ch = make_int4((int)pt_float_bits(d.x), (int)pt_float_bits(d.y), (int)pt_float_bits(d.z), (int)pt_float_bits(d.w));
+}
 PT_HD uint32_t leaf_start(int c) { return ((uint32_t)~c) >> 4; }
 PT_HD uint32_t leaf_count(int c) { return (((uint32_t)~c) & 15u) + 1u; }
 
@@ -56,7 +166,9 @@ struct DevLight {
 };
 
 struct DevScene {
-  const float4 *nodes;
+  const float4 *nodes;   // 128-byte nodes (closest hit)
+  const float4 *qnodes;  // 64-byte quantised nodes (any hit when qany)
+  uint32_t qany;         // any-hit traversal reads qnodes
   const float4 *prims;
   const uint2 *prim_info;
   const float4 *prim_geom;
