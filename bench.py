@@ -132,13 +132,20 @@ def quality(scene, estimator, res=128, spp_side=8, ref_side=64):
             "resolution": f"{res}x{res}", "ms_render": round(t_img * 1e3, 1), "ms_ref": round(t_ref * 1e3, 1)}
 
 
-def kernel_roofline(cst, ms, launches, read_traffic_for=None):
+def any_hit_qnodes(scs):
+    """Whether any-hit traversal reads the 64-byte quantised nodes: the
+    library picks them for a tree whose 128-byte nodes exceed 4 MiB (a batch:
+    when any of its scenes does)."""
+    return any(sc.info.bvh_nodes * 128 > (4 << 20) for sc in scs)
+
+
+def kernel_roofline(cst, ms, launches, read_traffic_for=None, qnodes=True):
     """Per-kernel algorithmic bytes (from the frame's counters) over HIP-event
     time; the roofline object is for the kernel with the most device time."""
     kernels = {
         "k_extend": ("closest-hit traversal", cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE +
                      cst["prim_tests"] * B_PRIM, ms["ms_extend"]),
-        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * B_QNODE +
+        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * (B_QNODE if qnodes else B_NODE) +
                      cst["shadow_prim_tests"] * B_PRIM, ms["ms_shadow"]),
         "k_shade": ("materials, lights, NEE, queue compaction", cst["segments"] * B_SHADE_SEG +
                     cst["shadow_rays"] * B_SHADE_SHADOW, ms["ms_shade"]),
@@ -228,7 +235,7 @@ def main_batch(args):
               "what": "the same renders launched one after another with pt_render"}
     log(f"serial: {serial}")
     ms = {k: cst[k] for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate", "ms_film")}
-    roofline, per_kernel = kernel_roofline(cst, ms, cst["extend_launches"])
+    roofline, per_kernel = kernel_roofline(cst, ms, cst["extend_launches"], qnodes=any_hit_qnodes(scs))
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
@@ -550,7 +557,8 @@ def main():
                                                               "ms_film")}
     launches = stats[0]["extend_launches"]
     roofline, per_kernel = kernel_roofline(cst, ms, launches,
-                                           read_traffic_for=args.estimator if args.workload == "glass_hdri" else None)
+                                           read_traffic_for=args.estimator if args.workload == "glass_hdri" else None,
+                                           qnodes=any_hit_qnodes([scene]))
 
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
