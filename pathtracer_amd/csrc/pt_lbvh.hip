@@ -370,6 +370,7 @@ __global__ void k_lb_quantize(const float4 *in, uint32_t n, float4 *out) {
 }
 
 hipError_t lbvh_quantize(hipStream_t st, const float4 *nodes8, uint32_t n, float4 *nodes_q) {
+  if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_lb_quantize, dim3(lb_blocks(n)), dim3(256), 0, st, nodes8, n, nodes_q);
   return hipGetLastError();
 }
