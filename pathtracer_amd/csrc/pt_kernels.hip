@@ -286,9 +286,11 @@ PT_D uint32_t tile_order16(uint32_t key, uint32_t (*cnt)[16], uint32_t *first, u
 
 // EST 0: getRayColor (maxdepth_nee.hpp:9-256).  EST 1: NEE + MIS.
 // Occupancy target of the shading kernel (waves per SIMD): the register
-// allocator spills the rarely used material state to reach it.
+// allocator spills the rarely used material state to reach it.  On the
+// 1.3M-triangle glass frame, shading took 26.3 / 26.2 / 24.8 / 24.4 ms at
+// 3 / 4 / 5 / 6 waves (whole frame 83.9 / 83.2-83.8 / 83.0 / 82.6 ms).
 #ifndef PT_SHADE_WAVES
-#define PT_SHADE_WAVES 4
+#define PT_SHADE_WAVES 6
 #endif
 // The same for the all-kinds variant (Disney and hair): 3 waves/SIMD (150
 // VGPRs, no spills) shaded the 64 Disney + 8 hair spheres in 116.5 ms against
