@@ -15,8 +15,9 @@ Prints one JSON line.  `value` is path samples per second over K timed frames
 of device time, from algorithmic bytes (HIP-event-timed launches, counters of
 an identical counting frame).  `quality` is the relMSE of a 64-spp render
 against a 4096-spp render of the same camera at 128x128.  `cpu_baseline`
-times the CPU oracle (the C restatement of the reference's estimators,
-OpenMP) on the host cores over a bounded slice of the same frame.
+times the CPU oracle (the C restatement of the reference's path, OpenMP) on
+the host cores over a bounded slice of the same frame: the reference's own
+estimator as `value`, NEE+MIS beside it.
 """
 from __future__ import annotations
 
@@ -39,18 +40,23 @@ WORKLOADS = {
                    "light; 1024x1024, 128 spp stratified, depth 12",
 }
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes (layout constants of pathtracer_amd/csrc/pt_scene.h)
-B_EXT_RAY = 24 + 8            # ray (o, d) read + hit (t, prim) write, unit stride
-B_SHADOW_RAY = 44             # shadow-queue entry (o, d, tmax, contribution, slot)
-B_NODE = 112                  # BVH4 node: four child boxes (SoA) + four child codes
-B_QNODE = 64                  # any-hit BVH4 node: four quantised child boxes, origin and scales, child codes
-B_PRIM = 48                   # triangle record (v0, e1, e2)
-# shading, per extension segment: the live-path record (ray 24, hit 8,
-# throughput 12, flags 4, MIS pdf + vertex 16, slot 4 = 68) plus primitive
-# info + geometry 24 and material 32 read; the survivor's record (68) written
-B_SHADE_SEG = 68 + 24 + 32 + 68
-# shading, per shadow ray written: the queue entry
-B_SHADE_SHADOW = B_SHADOW_RAY
+# Algorithmic bytes per unit of work, the hot path's cost model: per extension
+# segment a ray record (32) read and a hit record (16) written by traversal,
+# two path-state records (40 each, read and written) and a material record
+# (32) in shading; per shadow ray a 48-byte record written by shading and read
+# by any-hit traversal, plus 4 bytes of visibility; 32 bytes per BVH node visit
+# and 36 per triangle test (three float3); 12 bytes of film per path sample.
+B_RAY, B_HIT, B_STATE, B_MAT = 32, 16, 40, 32
+B_SHADOW, B_VIS = 48, 4
+B_NODE, B_TRI, B_FILM = 32, 36, 12
+# What this build's layout fetches per unit (node and primitive bytes mostly
+# served by LDS, L2 and the Infinity Cache): reported as the cache-inclusive
+# rate, never as the roofline's `achieved`.
+L_EXT_RAY = 24 + 8            # ray (o, d) read + hit (t, prim) written
+L_SHADOW_RAY = 44             # shadow-queue entry (o, d, tmax, contribution, slot)
+L_NODE = 112                  # BVH4 node: four child boxes (SoA) + four child codes
+L_QNODE = 64                  # quantised BVH4 node
+L_PRIM = 48                   # triangle record (v0, e1, e2)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_glass_hdri.txt")
 
 
@@ -58,17 +64,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(spec, cfg, target_s=15.0):
-    from oracle import oracle_py as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    o = O.OracleScene(spec)
-    build_s = time.perf_counter() - t0
-    total = cfg.width * cfg.height * cfg.spp_x * cfg.spp_y
-    # spread the slice over the frame: a band of rows in the middle of the image
-    start = (total // 2) // (cfg.spp_x * cfg.spp_y) * (cfg.spp_x * cfg.spp_y)
-    n = 4096
-    t = 0.0
+OR_REFERENCE_TRAVERSALS = 1 << 24   # the oracle repeats the reference's second continuation traversal
+
+
+def _time_oracle_leg(o, cfg, start, total, threads, target_s):
+    """Path samples per second of the oracle over consecutive samples from
+    `start`, grown until the timed slice takes about target_s."""
+    n, t = 4096, 0.0
     while True:
         t1 = time.perf_counter()
         o.render_range(cfg, start, n, threads)
@@ -76,21 +78,43 @@ def cpu_baseline(spec, cfg, target_s=15.0):
         if t > 1.0 or n >= 1 << 24:
             break
         n *= 4
-    n_final = max(n, int(n * target_s / max(t, 1e-3)))
-    n_final = min(n_final, total - start)
+    n = min(max(n, int(n * target_s / max(t, 1e-3))), total - start)
     t1 = time.perf_counter()
-    o.render_range(cfg, start, n_final, threads)
-    t = time.perf_counter() - t1
-    # one thread on a sample of about 5 s
-    n1 = max(256, min(n_final, int(n_final / max(t, 1e-3) / max(threads, 1) * 5.0)))
+    o.render_range(cfg, start, n, threads)
+    return n, time.perf_counter() - t1
+
+
+def cpu_baseline(spec, cfg, target_s=8.0):
+    """The CPU oracle (the C restatement of the reference's path, OpenMP) on
+    the host cores, over a bounded slice from the middle of the same frame.
+    `value` is the reference's own estimator (maxdepth_nee.hpp:9-256: 4 light
+    samples per non-specular vertex, each continuation ray traversed twice as
+    the reference does); `nee_mis` times the estimator the GPU runs."""
+    from oracle import oracle_py as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    o = O.OracleScene(spec)
+    build_s = time.perf_counter() - t0
+    spp = cfg.spp_x * cfg.spp_y
+    total = cfg.width * cfg.height * spp
+    start = (total // 2) // spp * spp
+    ref_cfg = o.cfg(estimator=0, flags=OR_REFERENCE_TRAVERSALS)
+    n, t = _time_oracle_leg(o, ref_cfg, start, total, threads, target_s)
+    # one thread on a sample of about 4 s
+    n1 = max(256, min(n, int(n / max(t, 1e-3) / max(threads, 1) * 4.0)))
     t1 = time.perf_counter()
-    o.render_range(cfg, start, n1, 1)
+    o.render_range(ref_cfg, start, n1, 1)
     t_one = time.perf_counter() - t1
-    return {"value": n_final / t / 1e6, "unit": "Mpath-samples/s", "cores": threads, "kind": "port",
-            "sample": f"{n_final} consecutive path samples (pixel-major) from the middle of the same frame, "
-                      f"{t:.1f} s on {threads} OpenMP threads; oracle BVH/scene build {build_s:.1f} s untimed",
+    m, tm = _time_oracle_leg(o, o.cfg(estimator=1), start, total, threads, target_s)
+    return {"value": n / t / 1e6, "unit": "Mpath-samples/s", "cores": threads, "kind": "port",
+            "estimator": "reference_nee",
+            "sample": f"{n} consecutive path samples (pixel-major) from the middle of the same frame under the "
+                      f"reference's estimator, {t:.1f} s on {threads} OpenMP threads; oracle BVH/scene build "
+                      f"{build_s:.1f} s untimed",
             "single_thread": {"value": n1 / t_one / 1e6, "cores": 1,
-                              "sample": f"the first {n1} of those path samples, {t_one:.1f} s"}}
+                              "sample": f"the first {n1} of those path samples, {t_one:.1f} s"},
+            "nee_mis": {"value": m / tm / 1e6, "cores": threads,
+                        "sample": f"{m} path samples from the same place under NEE+MIS, {tm:.1f} s"}}
 
 
 def read_traffic(kernel, estimator):
@@ -139,28 +163,43 @@ def any_hit_qnodes(scs):
     return any(sc.info.bvh_nodes * 128 > (4 << 20) for sc in scs)
 
 
-def kernel_roofline(cst, ms, launches, read_traffic_for=None, qnodes=True):
-    """Per-kernel algorithmic bytes (from the frame's counters) over HIP-event
-    time; the roofline object is for the kernel with the most device time."""
+def kernel_roofline(cst, ms, launches, read_traffic_for=None, qnodes=True, paths=0):
+    """Per-kernel algorithmic bytes (the cost model above, times the frame's
+    counters) over HIP-event time; the roofline object is for the kernel with
+    the most device time.  `cache_inclusive_gbs` is the same kernel's rate in
+    the bytes its layout fetches (most of them from caches)."""
+    seg, shr = cst["segments"], cst["shadow_rays"]
     kernels = {
-        "k_extend": ("closest-hit traversal", cst["segments"] * B_EXT_RAY + cst["node_visits"] * B_NODE +
-                     cst["prim_tests"] * B_PRIM, ms["ms_extend"]),
-        "k_shadow": ("any-hit traversal", cst["shadow_rays"] * B_SHADOW_RAY + cst["shadow_node_visits"] * (B_QNODE if qnodes else B_NODE) +
-                     cst["shadow_prim_tests"] * B_PRIM, ms["ms_shadow"]),
-        "k_shade": ("materials, lights, NEE, queue compaction", cst["segments"] * B_SHADE_SEG +
-                    cst["shadow_rays"] * B_SHADE_SHADOW, ms["ms_shade"]),
+        "k_extend": ("closest-hit traversal",
+                     seg * (B_RAY + B_HIT) + cst["node_visits"] * B_NODE + cst["prim_tests"] * B_TRI,
+                     seg * L_EXT_RAY + cst["node_visits"] * L_NODE + cst["prim_tests"] * L_PRIM, ms["ms_extend"]),
+        "k_shadow": ("any-hit traversal",
+                     shr * (B_SHADOW + B_VIS) + cst["shadow_node_visits"] * B_NODE + cst["shadow_prim_tests"] * B_TRI,
+                     shr * L_SHADOW_RAY + cst["shadow_node_visits"] * (L_QNODE if qnodes else L_NODE) +
+                     cst["shadow_prim_tests"] * L_PRIM, ms["ms_shadow"]),
+        "k_shade": ("materials, lights, NEE, queue compaction", seg * (2 * B_STATE + B_MAT) + shr * B_SHADOW,
+                    seg * (68 + 24 + 32 + 68) + shr * L_SHADOW_RAY, ms["ms_shade"]),
+        "k_film": ("per-pixel sums", paths * B_FILM, paths * 12, ms["ms_film"]),
     }
+    gbs = lambda b, t: round(b / (t * 1e-3) / 1e9, 1) if t > 0 else 0.0
     per_kernel = {k: {"what": w, "gb_per_frame": round(b / 1e9, 3), "ms_per_frame": round(t, 2),
-                      "achieved_gbs": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else 0.0}
-                  for k, (w, b, t) in kernels.items()}
-    kname = max(kernels, key=lambda k: kernels[k][2])
-    _, kbytes, kms = kernels[kname]
+                      "achieved_gbs": gbs(b, t), "frac": round(gbs(b, t) / HBM_PEAK_GBS, 4),
+                      "cache_inclusive_gbs": gbs(lb, t)}
+                  for k, (w, b, lb, t) in kernels.items()}
+    kname = max(kernels, key=lambda k: kernels[k][3])
+    _, kbytes, lbytes, kms = kernels[kname]
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": (read_traffic(kname, read_traffic_for)
                                                              if read_traffic_for is not None else None),
                 "kernel": kname, "launches_per_frame": launches, "bytes_per_launch": int(kbytes / max(launches, 1)),
-                "ms_per_launch": round(kms / max(launches, 1), 4)}
+                "ms_per_launch": round(kms / max(launches, 1), 4),
+                "cache_inclusive_gbs": round(lbytes / (kms * 1e-3) / 1e9, 1) if kms > 0 else 0.0}
+    path_bytes = sum(v[1] for v in kernels.values())
+    path_ms = sum(v[3] for v in kernels.values())
+    roofline["path"] = {"bytes_per_frame": int(path_bytes), "bytes_per_path": round(path_bytes / max(paths, 1), 1),
+                        "kernel_ms": round(path_ms, 2), "achieved": gbs(path_bytes, path_ms),
+                        "frac": round(gbs(path_bytes, path_ms) / HBM_PEAK_GBS, 4)}
     return roofline, per_kernel
 
 
@@ -235,7 +274,8 @@ def main_batch(args):
               "what": "the same renders launched one after another with pt_render"}
     log(f"serial: {serial}")
     ms = {k: cst[k] for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate", "ms_film")}
-    roofline, per_kernel = kernel_roofline(cst, ms, cst["extend_launches"], qnodes=any_hit_qnodes(scs))
+    roofline, per_kernel = kernel_roofline(cst, ms, cst["extend_launches"], qnodes=any_hit_qnodes(scs),
+                                           paths=cst["paths"])
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
@@ -252,7 +292,7 @@ def main_batch(args):
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline(specs[0], cfgs[0], target_s=10.0)
+        res["cpu_baseline"] = cpu_baseline(specs[0], cfgs[0])
     print(json.dumps(res), flush=True)
 
 
@@ -466,7 +506,7 @@ def main_guided(args):
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline(spec, scene.cfg(), target_s=10.0)
+        res["cpu_baseline"] = cpu_baseline(spec, scene.cfg())
     print(json.dumps(res), flush=True)
 
 
@@ -478,6 +518,7 @@ def main():
     ap.add_argument("--estimator", type=int, default=1, help="0: reference NEE, 1: NEE+MIS (default)")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-other", action="store_true", help="skip the other estimator's frame (profiling runs)")
     ap.add_argument("--small", action="store_true", help="256x256 quick check (not a bench line)")
     ap.add_argument("--save", default="", help="write the film as PFM/PNG with this path prefix")
     ap.add_argument("--workload", default="glass_hdri",
@@ -540,17 +581,19 @@ def main():
     value = total * args.steps / dt / 1e6
 
     # the other estimator on the same frame (one warm-up, one timed frame)
-    other = 1 - args.estimator
-    ocfg = scene.cfg(estimator=other, flags=L.RENDER_TIMING | sort_flag)
-    scene.render_device(ocfg)
-    L.check(L.lib.pt_device_synchronize(), "sync")
-    t2 = time.perf_counter()
-    scene.render_device(ocfg)
-    L.check(L.lib.pt_device_synchronize(), "sync")
-    odt = time.perf_counter() - t2
-    other_res = {"estimator": ["reference_nee", "nee_mis"][other], "value": round(total / odt / 1e6, 2),
-                 "ms_per_step": round(odt * 1e3, 2)}
-    log(f"other estimator: {other_res}")
+    other_res = None
+    if not args.no_other:
+        other = 1 - args.estimator
+        ocfg = scene.cfg(estimator=other, flags=L.RENDER_TIMING | sort_flag)
+        scene.render_device(ocfg)
+        L.check(L.lib.pt_device_synchronize(), "sync")
+        t2 = time.perf_counter()
+        scene.render_device(ocfg)
+        L.check(L.lib.pt_device_synchronize(), "sync")
+        odt = time.perf_counter() - t2
+        other_res = {"estimator": ["reference_nee", "nee_mis"][other], "value": round(total / odt / 1e6, 2),
+                     "ms_per_step": round(odt * 1e3, 2)}
+        log(f"other estimator: {other_res}")
 
     # roofline of the dominant kernel (HIP-event times of the timed frames)
     ms = {k: sum(s[k] for s in stats) / args.steps for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_generate",
@@ -558,7 +601,7 @@ def main():
     launches = stats[0]["extend_launches"]
     roofline, per_kernel = kernel_roofline(cst, ms, launches,
                                            read_traffic_for=args.estimator if args.workload == "glass_hdri" else None,
-                                           qnodes=any_hit_qnodes([scene]))
+                                           qnodes=any_hit_qnodes([scene]), paths=total)
 
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,

@@ -279,7 +279,11 @@ typedef struct pt_scene_info {
   double build_ms;  /* the whole commit                                       */
   double bvh_ms;    /* the BVH build alone (with PT_BUILD_DEVICE_LBVH: the
                        records' upload and the device kernels, waited for)    */
+  uint32_t flags;   /* PT_INFO_*                                              */
 } pt_scene_info;
+/* pt_scene_info.flags: any-hit traversal reads the 64-byte quantised nodes
+ * (chosen for trees above 4 MiB, or forced by PT_BUILD_ANY_HIT_QNODES).     */
+#define PT_INFO_ANY_HIT_QNODES 1u
 pt_status pt_scene_get_info(const pt_scene *scene, pt_scene_info *info);
 
 /* SceneInfo::render (renderManager.hpp:51-158): renders into the scene's

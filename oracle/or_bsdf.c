@@ -372,19 +372,31 @@ static float phong_pdf(const pt_material_desc *m, v3 wo_w, v3 wi_w, const or_hit
 }
 
 /* ---------------- Kajiya-Kay (no reference code: README.md:74 lists it,
- * TODO.md:20 is unchecked).  Defined here as the classic Kajiya & Kay (1989)
- * hair model about the shading normal used as fibre tangent t (the reference
- * hair falls back to the normal as tangent, hairMarsch.cpp:13-16):
- *   f = kd/pi * sin(t,wi) + ks * (n+2)/(2pi) * max(0, cos(t,wi)cos(t,wo) +
- *       sin(t,wi) sin(t,wo))^n, sampled by cosine hemisphere about wo's side.
- * Parity pinned only by non-negativity, reciprocity and sample/pdf
- * consistency. */
+ * TODO.md:20 is unchecked).  The Kajiya & Kay (1989) fibre model about the
+ * fibre tangent t: the hit's hair tangent on curves (HitRecord::hair_tangent,
+ * curve.hpp:101-129); on a surface the first axis u of OrthoBasis(n), a
+ * direction in the tangent plane (a hair card):
+ *   f = kd/pi sin(t,wi)
+ *     + ks (p+2)/(2pi) max(0, sin(t,wi) sin(t,wo) - cos(t,wi) cos(t,wo))^p
+ * The diffuse term is Lambert integrated over the lit half of a thin
+ * cylinder.  The specular term is cos^p of the angle between wo and the cone
+ * of mirror directions: light reflected by a cylinder keeps its component
+ * along the axis, so the cone is t.wo = -t.wi.  Sampled by the cosine lobe
+ * about n.  Pinned by tests/test_oracle_bsdf.py: the closed-form values on
+ * the cone, at normal incidence and retro-reflection, the tangent choice, the
+ * pdf's normalisation and sample/pdf consistency. */
+static v3 kk_tangent(const or_hit *h) {
+  if (!(fabsf(h->tangent.x) < 1e-6f && fabsf(h->tangent.y) < 1e-6f && fabsf(h->tangent.z) < 1e-6f))
+    return vnorm(h->tangent);
+  onb_t b = onb_make(h->n);
+  return b.u;
+}
 static v3 kk_f(const pt_material_desc *m, v3 wo_w, v3 wi_w, const or_hit *h) {
-  v3 t = vnorm(h->n), wo = vnorm(wo_w), wi = vnorm(wi_w);
+  v3 t = kk_tangent(h), wo = vnorm(wo_w), wi = vnorm(wi_w);
   float ci = vdot(t, wi), co = vdot(t, wo);
   float si = sqrtf(fmaxf(0.f, 1.f - ci * ci)), so = sqrtf(fmaxf(0.f, 1.f - co * co));
   float ex = m->p[6];
-  float spec = powf(fmaxf(0.f, ci * co + si * so), ex) * (float)((ex + 2.f) / (2.f * M_PI));
+  float spec = powf(fmaxf(0.f, si * so - ci * co), ex) * (float)((ex + 2.f) / (2.f * M_PI));
   v3 kd = vscale(p3(m, 0), (float)(si / M_PI));
   return vadd(kd, vscale(p3(m, 3), spec));
 }

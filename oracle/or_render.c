@@ -558,12 +558,14 @@ static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t
       v3 wi = vnorm(sr.wi);
       float cs = fabsf(vdot(h.n, wi));
       float pdf = sr.pdf;
+      const v3 vp = h.p; /* this vertex (h is overwritten by the continuation's hit below) */
       if (!mis) {
         L = vadd(L, vmul(beta, nee_reference(c, &h, wo, m, g, d, cnt)));
         if (cnt) cnt[1]++;
         hit = or_scene_intersect(s, h.p, wi, 1e-4f, 1e5f, 0, &h, cnt ? cnt + 4 : NULL);
-        have = 1;
         if (hit && or_is_emitter(&s->mats[h.mat])) break; /* maxdepth_nee.hpp:246-247 */
+        /* the recursion traverses the same ray again (maxdepth_nee.hpp:252-255) */
+        have = !(cfg->flags & OR_RENDER_REFERENCE_TRAVERSALS);
       } else if (N > 0) {
         int li = (int)(or_rng(g, slot(d, SLOT_MIS_LIGHT), cfg->seed) * (float)N) % N;
         const or_light *Lg = &s->lights[li];
@@ -607,9 +609,9 @@ static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t
       }
       beta = vscale(vmul(beta, sr.att), cs / pdf);
       prev_pdf = pdf;
-      prev_p = h.p;
+      prev_p = vp;
       prev_spec = 0;
-      ro = h.p;
+      ro = vp;
       rd = wi;
     }
     if (cfg->rr_depth && d >= cfg->rr_depth) {

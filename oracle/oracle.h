@@ -24,6 +24,13 @@
 extern "C" {
 #endif
 
+/* pt_render_cfg.flags bit read by the oracle only: the reference estimator
+ * traverses each continuation ray twice, once for its emitter check and once
+ * more in the recursion (maxdepth_nee.hpp:238-255).  The result is the same;
+ * with this bit the oracle repeats the second traversal too, so that its
+ * timing counts the reference's traversals (the CPU baseline).            */
+#define OR_RENDER_REFERENCE_TRAVERSALS (1u << 24)
+
 /* ---- samplers (or_sampler.c) ---- */
 uint64_t or_mix_bits(uint64_t v);
 uint32_t or_permutation_element(uint32_t i, uint32_t l, uint32_t p);

@@ -30,6 +30,7 @@ BATCH_NO_GRAPH = 256
 TUNE_WAVEFRONT, TUNE_GRIDS, TUNE_LEAF, TUNE_SORT = 1, 2, 4, 8
 BUILD_DEVICE_LBVH, BUILD_NO_LDS_NODES = 1, 2
 BUILD_ANY_HIT_QNODES, BUILD_ANY_HIT_FULL_NODES = 4, 8
+INFO_ANY_HIT_QNODES = 1
 GUIDE_NO_TRAIN = 1
 GUIDE_INPUTS, GUIDE_OUTPUTS, GUIDE_NPARAMS, GUIDE_MIX = 6, 41, 27497, 48
 PRIM_FLIP_FACE, PRIM_FORCE_INSIDE, PRIM_NOT_LIGHT = 1, 2, 4
@@ -81,7 +82,7 @@ class SceneInfo(ctypes.Structure):
                 ("n_spheres", ctypes.c_uint32), ("n_rects", ctypes.c_uint32),
                 ("n_lights", ctypes.c_uint32), ("bvh_nodes", ctypes.c_uint32),
                 ("bvh_depth", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64),
-                ("build_ms", ctypes.c_double), ("bvh_ms", ctypes.c_double)]
+                ("build_ms", ctypes.c_double), ("bvh_ms", ctypes.c_double), ("flags", ctypes.c_uint32)]
 
 
 class Tuning(ctypes.Structure):

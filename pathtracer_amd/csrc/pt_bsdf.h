@@ -420,15 +420,22 @@ PT_D float phong_pdf(const pt_material_desc &m, f3 wo_w, f3 wi_w, const ShadePoi
 }
 
 // ---- Kajiya-Kay (no reference code; README.md:74 lists it, TODO.md:20 is
-// open).  Classic Kajiya & Kay (1989) about the shading normal used as the
-// fibre tangent (the reference's hair falls back to the normal as tangent,
-// hairMarsch.cpp:13-16), sampled by the cosine lobe. ----
+// open).  Kajiya & Kay (1989) about the fibre tangent t: the hair tangent of
+// a curve hit, else the first axis of OrthoBasis(n) (a direction in the
+// surface, as for a hair card):
+//   f = kd/pi sin(t,wi) + ks (p+2)/(2pi) max(0, sin(t,wi) sin(t,wo) - cos(t,wi) cos(t,wo))^p,
+// whose specular term peaks on the mirror cone t.wo = -t.wi.  Sampled by the
+// cosine lobe about n. ----
+PT_D f3 kk_tangent(const ShadePoint &h) {
+  if (!near_zero3(h.t)) return norm3(h.t);
+  return make_frame(h.n).u;
+}
 PT_D f3 kk_f(const pt_material_desc &m, f3 wo_w, f3 wi_w, const ShadePoint &h) {
-  f3 t = norm3(h.n), wo = norm3(wo_w), wi = norm3(wi_w);
+  f3 t = kk_tangent(h), wo = norm3(wo_w), wi = norm3(wi_w);
   float ci = dot3(t, wi), co = dot3(t, wo);
   float si = sqrtf(fmaxf(0.f, 1.f - ci * ci)), so = sqrtf(fmaxf(0.f, 1.f - co * co));
   float ex = m.p[6];
-  float spec = powf(fmaxf(0.f, ci * co + si * so), ex) * ((ex + 2.f) / (2.f * PT_PI));
+  float spec = powf(fmaxf(0.f, si * so - ci * co), ex) * ((ex + 2.f) / (2.f * PT_PI));
   return add3(scl3(mp3(m, 0), si / PT_PI), scl3(mp3(m, 3), spec));
 }
 PT_D bool kk_scatter(const pt_material_desc &m, f3 wo, const ShadePoint &h, float u0, float u1, Scatter &s) {

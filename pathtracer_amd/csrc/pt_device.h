@@ -163,7 +163,6 @@ PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, floa
   }
   return hit_rect(A, B, o, d, tmin, tmax, t);
 }
-#pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------
 // Traversal stack.  Entries [0, lo) live in a private array; with K > 0 the
@@ -619,7 +618,6 @@ PT_D f3 smooth_normal(const float4 *r, f3 o, f3 d, f3 ng) {
   f3 ns = norm3_rn(add3(add3(scl3(n0, 1.f - u - v), scl3(n1, u)), scl3(n2, v)));
   return dot3(ns, ng) < 0.f ? neg3(ns) : ns;
 }
-#pragma clang fp contract(fast)
 
 PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
   SurfHit h;

@@ -1143,6 +1143,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
                     s->d_env.bytes +
                     s->d_env_func.bytes + s->d_env_cdf.bytes;
   in.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  in.flags = ds.qany ? PT_INFO_ANY_HIT_QNODES : 0u;
   s->committed = true;
   return PT_OK;
 }

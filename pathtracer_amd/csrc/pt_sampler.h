@@ -189,4 +189,3 @@ PT_HD float stratified(uint32_t pixel_seed, uint32_t k, uint32_t spp_x, uint32_t
 
 }  // namespace pt
 
-#pragma clang fp contract(fast)

@@ -486,18 +486,23 @@ def single_scatter_medium(width=128, height=128, spp_x=4, spp_y=4, g=0.0) -> Sce
 
 
 def hair_curves(width=96, height=96, spp_x=4, spp_y=4, n_curves=60, seed=9, estimator=L.EST_NEE_MIS,
-                max_depth=8) -> SceneSpec:
-    """A tuft of Marschner-hair Bezier curves (the curve primitive with its
-    hair tangent) over a Lambertian floor under a quad light."""
+                max_depth=8, kajiya_kay=False) -> SceneSpec:
+    """A tuft of hair Bezier curves (the curve primitive with its hair
+    tangent) over a Lambertian floor under a quad light: Marschner hair of
+    three melanin levels, or Kajiya-Kay fibres of three colours."""
     rng = np.random.default_rng(seed)
     s = SceneSpec(name="hair_curves")
     floor = s.add_material(L.MAT_LAMBERTIAN, [0.6, 0.6, 0.6])
     light = s.add_material(L.MAT_EMITTER, [1.0, 1.0, 1.0, 12.0])
     mats = []
     for mel in (0.3, 0.8, 1.3):
-        mats.append(s.add_material(L.MAT_HAIR, [mel * EUMELANIN_SIGMA_A[0], mel * EUMELANIN_SIGMA_A[1],
-                                                mel * EUMELANIN_SIGMA_A[2], 0.0, 1.55, 0.3, 0.3,
-                                                math.radians(2.0)]))
+        if kajiya_kay:
+            tint = np.exp(-mel * np.array(EUMELANIN_SIGMA_A))
+            mats.append(s.add_material(L.MAT_KAJIYA_KAY, [*(0.6 * tint), *(0.3 * tint), 40.0]))
+        else:
+            mats.append(s.add_material(L.MAT_HAIR, [mel * EUMELANIN_SIGMA_A[0], mel * EUMELANIN_SIGMA_A[1],
+                                                    mel * EUMELANIN_SIGMA_A[2], 0.0, 1.55, 0.3, 0.3,
+                                                    math.radians(2.0)]))
     s.add_rect(1, 0.0, -3, 3, -3, 3, floor)
     s.add_rect(1, 3.0, -0.75, 0.75, -0.75, 0.75, light)
     for i in range(n_curves):

@@ -1,12 +1,17 @@
 """GPU materials against the oracle: Material::f / scatter_pdf within 1e-4
 relative on seeded (wo, wi, normal, parameter) tuples, and Material::scatter
 with explicit samples (same lobe, same flags, directions and values within
-1e-4).  The oracle's materials are pinned in tests/test_oracle_bsdf.py."""
+1e-4 except where the oracle's own result moves by more than that when its
+inputs move by 1e-6).  Seeds are fixed, so every run draws the same tuples.
+The oracle's materials are pinned in tests/test_oracle_bsdf.py and
+tests/test_oracle_disney_hair.py."""
+import zlib
+
 import numpy as np
 import pytest
 
 from oracle import oracle_py as O
-from tests.helpers import hemi_dirs, random_dirs
+from tests.helpers import explained_by_conditioning as _explained, hemi_dirs, random_dirs
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +39,12 @@ MATERIALS = {
 }
 
 TOL = 1e-4
+MAX_ILL = 1e-3  # at most this fraction of tuples may be ill-conditioned
+
+
+def _seed(name, k=0):
+    """Stable seed per material (Python's str hash is salted per process)."""
+    return zlib.crc32(name.encode()) + k
 
 
 def _close(a, b, rtol=TOL, atol=1e-6):
@@ -44,9 +55,10 @@ def _close(a, b, rtol=TOL, atol=1e-6):
 
 @pytest.mark.parametrize("name", sorted(MATERIALS))
 def test_eval_matches_oracle(gpu, name):
+    """f and pdf within 1e-4 relative on every one of 200k tuples."""
     from pathtracer_amd import render as R
     kind, params = MATERIALS[name]
-    rng = np.random.default_rng(abs(hash(name)) % 2 ** 31)
+    rng = np.random.default_rng(_seed(name))
     n = 200_000
     nrm = random_dirs(rng, n)
     wo = hemi_dirs(rng, n, nrm)          # wo on the normal's side (set_normal)
@@ -56,15 +68,19 @@ def test_eval_matches_oracle(gpu, name):
     cf, cp = O.bsdf_eval(kind, params, wo, wi, nrm, inside)
     ok_f = np.all(_close(gf, cf), axis=1)
     ok_p = _close(gp, cp)
-    assert ok_f.mean() == 1.0, f"f: {np.sum(~ok_f)} tuples differ"
-    assert ok_p.mean() == 1.0, f"pdf: {np.sum(~ok_p)} tuples differ"
+    assert ok_f.all(), f"f: {np.sum(~ok_f)} tuples differ"
+    assert ok_p.all(), f"pdf: {np.sum(~ok_p)} tuples differ"
 
 
 @pytest.mark.parametrize("name", sorted(MATERIALS))
 def test_sample_matches_oracle(gpu, name):
+    """Material::scatter with the same samples: the same lobe and outcome on
+    every tuple; the direction, and the value and pdf at that direction, within
+    1e-4 relative except where the oracle is ill-conditioned (at most 1e-3 of
+    the tuples, each within the bar plus its own perturbation spread)."""
     from pathtracer_amd import render as R
     kind, params = MATERIALS[name]
-    rng = np.random.default_rng(1 + abs(hash(name)) % 2 ** 31)
+    rng = np.random.default_rng(_seed(name, 1))
     n = 200_000
     nrm = random_dirs(rng, n)
     wo = hemi_dirs(rng, n, nrm)
@@ -74,37 +90,20 @@ def test_sample_matches_oracle(gpu, name):
     inside = rng.integers(0, 2, size=n).astype(np.uint8)
     g = R.bsdf_sample(kind, params, wo, nrm, uc, u2, aux, inside)
     c = O.bsdf_sample(kind, params, wo, nrm, uc, u2, aux, inside)
+    prng = np.random.default_rng(_seed(name, 2))
     same = (g[4] == c[4]) & (g[3] == c[3])
-    # a branch decision at a floating-point boundary may flip for a handful of tuples
-    assert same.mean() > 1 - 1e-4, f"{np.sum(~same)} tuples took another lobe"
-    m = same & (c[4] == 1)
-    wi_ok = np.all(_close(g[0][m], c[0][m], atol=1e-5), axis=1)
-    # The rough dielectric's BIASED_SAMPLING fallbacks (mfDielectricPDF.hpp:47-49, 70-72;
-    # also the Disney glass lobe) switch between refraction and reflection on the sign of
-    # wi.z; for directions within rounding of the horizon the two sides may pick
-    # different branches.
-    frac = 5e-4 if kind in (4, 7) else 1e-4
-    assert wi_ok.mean() > 1 - frac, f"wi: {np.sum(~wi_ok)} of {m.sum()} differ"
-    spec = (g[3] & 16) != 0
-    if kind == 6:
-        # Phong returns its lobe colour, not f (material.hpp:350)
-        assert np.mean(np.all(_close(g[1][m], c[1][m]), axis=1)) > 1 - frac, "attenuation"
-        assert np.mean(_close(g[2][m], c[2][m])) > 1 - frac, "pdf"
-        return
-    # specular lobes: path weight and selection probability compared directly
-    ms = m & spec
-    assert np.mean(np.all(_close(g[1][ms], c[1][ms]), axis=1)) > 1 - frac if ms.any() else True
-    # glossy/diffuse lobes: the value and pdf returned with the GPU's sampled direction are
-    # the oracle's f and scatter_pdf at that direction (narrow lobes turn a 1e-6 direction
-    # difference into more than 1e-4 of f, so f is compared at the same direction)
-    mg = (g[4] == 1) & ~spec
-    if not mg.any():
-        return
-    # Sampled directions crowd into the peak of a narrow lobe, where the microfacet
-    # distribution turns one rounding step of the half vector into ~1e-4 of f, so the
-    # narrow-lobe materials get the same tuple allowance as the direction check.
-    of, op = O.bsdf_eval(kind, params, wo[mg], g[0][mg], nrm[mg], inside[mg])
-    assert np.mean(np.all(_close(g[1][mg], of), axis=1)) > 1 - frac, "attenuation vs f(wo, wi)"
-    assert np.mean(_close(g[2][mg], op)) > 1 - frac, "pdf vs scatter_pdf(wo, wi)"
-    # and no tuple is off by more than a rounding-amplified amount
-    assert np.mean(np.all(_close(g[1][mg], of, rtol=1e-2), axis=1)) > 1 - 1e-4, "attenuation (1e-2)"
+    assert same.all(), f"{np.sum(~same)} tuples took another lobe"
+    m = c[4] == 1
+    # the sampled direction, the returned value (f, or the path weight of a
+    # specular lobe, or Phong's lobe colour, material.hpp:350) and the returned
+    # pdf against the oracle's scatter on the same inputs
+    ok = np.all(_close(g[0], c[0], atol=1e-5), axis=1) & np.all(_close(g[1], c[1]), axis=1) & _close(g[2], c[2])
+    idx = np.nonzero(m & ~ok)[0]
+    print(f"{name}: {idx.size} of {m.sum()} sampled tuples outside 1e-4")
+    if idx.size:
+        sample_fn = lambda wo_, nrm_, uc_, u2_, aux_, in_: O.bsdf_sample(kind, params, wo_, nrm_, uc_, u2_, aux_,
+                                                                         in_)[:3]
+        ok = _explained(sample_fn, [wo, nrm, uc, u2, aux, inside], ["dir", "dir", "u", "u", "u", "flag"], idx,
+                        g[:3], [TOL, TOL, TOL], [1e-5, 1e-6, 1e-6], prng)
+        assert ok.all(), f"{np.sum(~ok)} tuples differ beyond the oracle's own conditioning"
+        assert idx.size <= MAX_ILL * n

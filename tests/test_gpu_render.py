@@ -2,7 +2,9 @@
 
 1. Per sample: GPU and oracle draw the same sample streams, so each path
    sample's radiance must agree (1e-3 relative) except for the rare path whose
-   branch decision flips at a floating-point boundary.
+   branch decision flips at a floating-point boundary: at least 99.9% of the
+   samples agree and the others carry at most 5e-4 of the film's radiance
+   (tests/helpers.py check_samples).
 2. Closed form: white furnace (a*E unbiased, 2*a*E reference estimator).
 3. Images: GPU mean vs oracle mean, per-pixel z-test and relMSE.
 """
@@ -10,8 +12,17 @@ import numpy as np
 import pytest
 
 from oracle import oracle_py as O
+from tests.helpers import check_samples
 
 pytestmark = pytest.mark.gpu
+
+
+def _counts(spec, **cfg_kw):
+    """Stats of a counting render (PT_RENDER_COUNTERS) of the same frame."""
+    from pathtracer_amd import render as R, _lib as L
+    g = R.Scene(spec)
+    _, st = g.render(g.cfg(flags=L.RENDER_COUNTERS, **cfg_kw))
+    return st
 
 
 def _per_sample(spec, **cfg_kw):
@@ -26,32 +37,30 @@ def _per_sample(spec, **cfg_kw):
     return gimg, gs, oimg, osmp, st
 
 
-def _sample_agreement(gs, os_, rtol=1e-3, atol=1e-4):
-    close = np.all(np.abs(gs - os_) <= rtol * np.maximum(np.abs(gs), np.abs(os_)) + atol, axis=-1)
-    return close.mean()
-
-
 @pytest.mark.parametrize("estimator", [0, 1])
 def test_cornell_per_sample(gpu, estimator):
     from pathtracer_amd import scenes
     spec = scenes.cornell(width=48, height=48, spp_x=2, spp_y=2)
     gimg, gs, oimg, os_, st = _per_sample(spec, estimator=estimator, rr_depth=0)
-    frac = _sample_agreement(gs, os_)
-    assert frac > 0.995, f"only {frac:.4f} of path samples agree"
+    check_samples(gs, os_, f"cornell est {estimator}")
     assert st["nan_samples"] == 0
 
 
-@pytest.mark.parametrize("estimator", [2, 3, 4])
-def test_other_integrators_per_sample(gpu, estimator):
-    """maxdepth_naive / roulette_naive / maxdepth_reservoir_di: the Cornell box
-    (quad light) and a room lit by a sphere light."""
+@pytest.mark.parametrize("estimator", [0, 1, 2, 3, 4])
+def test_integrators_per_sample(gpu, estimator):
+    """The reference NEE, NEE+MIS, maxdepth_naive, roulette_naive and
+    maxdepth_reservoir_di estimators on the Cornell box (quad light) and on a
+    room lit by a sphere light (SphereLight, light.hpp:215-306)."""
     from pathtracer_amd import scenes
     from tests.test_oracle_render import _sphere_lit_room
     for spec in (scenes.cornell(width=32, height=32, spp_x=2, spp_y=2), _sphere_lit_room()):
         _, gs, _, os_, st = _per_sample(spec, estimator=estimator, rr_depth=0, spp_x=2, spp_y=2)
-        assert _sample_agreement(gs, os_) > 0.995, spec.name
+        check_samples(gs, os_, f"{spec.name} est {estimator}")
+        st = _counts(spec, estimator=estimator, rr_depth=0, spp_x=2, spp_y=2)
         if estimator in (2, 3):
             assert st["shadow_rays"] == 0
+        else:
+            assert st["shadow_rays"] > 0
 
 
 def test_cornell_smooth_normals_per_sample(gpu):
@@ -59,7 +68,7 @@ def test_cornell_smooth_normals_per_sample(gpu):
     from pathtracer_amd import scenes
     spec = scenes.cornell(width=40, height=40, spp_x=2, spp_y=2, smooth=True)
     _, gs, _, os_, st = _per_sample(spec, rr_depth=0)
-    assert _sample_agreement(gs, os_) > 0.995
+    check_samples(gs, os_, "cornell smooth")
     assert st["nan_samples"] == 0
 
 
@@ -68,15 +77,33 @@ def test_cornell_per_sample_samplers(gpu, sampler):
     from pathtracer_amd import scenes
     spec = scenes.cornell(width=32, height=32, spp_x=2, spp_y=2)
     _, gs, _, os_, _ = _per_sample(spec, sampler=sampler, rr_depth=3)
-    assert _sample_agreement(gs, os_) > 0.995
+    check_samples(gs, os_, f"cornell sampler {sampler}")
+
+
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_motion_blur_and_lens_per_sample(gpu, estimator):
+    """A camera moving from pos to pos1 while the shutter is open (time uniform
+    in [0, shutter)), with a thin lens: every sample's time and lens point are
+    drawn from the same stream on both sides."""
+    from pathtracer_amd import scenes
+    spec = scenes.cornell(width=40, height=40, spp_x=2, spp_y=2)
+    spec.camera = dict(spec.camera, pos1=(0.3, 1.2, 3.6), shutter=0.8, focal_dist=3.0, aperture=3.0 / 0.08)
+    _, gs, _, os_, st = _per_sample(spec, estimator=estimator, rr_depth=3)
+    check_samples(gs, os_, f"motion blur est {estimator}")
+    static = scenes.cornell(width=40, height=40, spp_x=2, spp_y=2)
+    static.camera = dict(spec.camera, shutter=0.0)
+    _, gs0, _, _, _ = _per_sample(static, estimator=estimator, rr_depth=3)
+    assert np.mean(np.any(gs0 != gs, axis=-1)) > 0.1      # the shutter changes the paths
 
 
 def test_glass_hdri_per_sample(gpu):
+    """A small stand-in of the glass/HDRI workload; the full one is in
+    tests/test_gpu_workloads.py."""
     from pathtracer_amd import scenes
     spec = scenes.glass_hdri(width=32, height=32, spp_x=2, spp_y=2, subdiv=4, env_w=128, env_h=64, max_depth=8)
     for est in (0, 1):
         _, gs, _, os_, _ = _per_sample(spec, estimator=est, rr_depth=0)
-        assert _sample_agreement(gs, os_) > 0.99
+        check_samples(gs, os_, f"glass small est {est}")
 
 
 def test_disney_hair_grid_per_sample(gpu):
@@ -84,7 +111,7 @@ def test_disney_hair_grid_per_sample(gpu):
     spec = scenes.disney_hair_grid(width=48, height=48, spp_x=2, spp_y=2, max_depth=6)
     for est in (0, 1):
         _, gs, _, os_, st = _per_sample(spec, estimator=est, rr_depth=0)
-        assert _sample_agreement(gs, os_) > 0.99
+        check_samples(gs, os_, f"disney grid est {est}")
         assert st["nan_samples"] == 0
 
 
@@ -137,18 +164,28 @@ def test_volume_per_sample(gpu, which):
     else:
         spec = scenes.single_scatter_medium(width=40, height=40, spp_x=2, spp_y=2, g=0.6 if which == "scatter_hg" else 0.0)
     _, gs, _, os_, st = _per_sample(spec)
-    assert _sample_agreement(gs, os_) > 0.995
+    check_samples(gs, os_, f"volume {which}")
     assert st["nan_samples"] == 0
 
 
 @pytest.mark.parametrize("estimator", [0, 1])
-def test_hair_curves_per_sample(gpu, estimator):
-    """Marschner hair on Bezier curves: the curve's tangent frames the hair
-    sample (HairMarschner::scatter), the normal frames f and the pdf."""
+@pytest.mark.parametrize("kk", [False, True])
+def test_hair_curves_per_sample(gpu, estimator, kk):
+    """Hair on Bezier curves.  Marschner: the curve's tangent frames the hair
+    sample (HairMarschner::scatter), the normal frames f and the pdf.
+    Kajiya-Kay: the curve's tangent is the fibre axis of f."""
     from pathtracer_amd import scenes
-    spec = scenes.hair_curves(width=40, height=40, spp_x=2, spp_y=2, estimator=estimator)
+    spec = scenes.hair_curves(width=40, height=40, spp_x=2, spp_y=2, estimator=estimator, kajiya_kay=kk)
     _, gs, _, os_, st = _per_sample(spec, estimator=estimator, rr_depth=0)
-    assert _sample_agreement(gs, os_) > 0.99
+    moved = None
+    if kk:
+        # exponent-40 cones on fibres 0.008-0.03 wide: a 1e-6 camera move
+        # changes about 10% of the oracle's own samples here
+        sp2 = scenes.hair_curves(width=40, height=40, spp_x=2, spp_y=2, estimator=estimator, kajiya_kay=kk)
+        sp2.camera = dict(sp2.camera, pos=tuple(np.asarray(sp2.camera["pos"]) + 1e-6))
+        o2 = O.OracleScene(sp2)
+        _, _, moved, _ = o2.render(o2.cfg(estimator=estimator, rr_depth=0), want_samples=True)
+    check_samples(gs, os_, f"{'kajiya-kay' if kk else 'marschner'} curves est {estimator}", self_moved=moved)
     assert st["nan_samples"] == 0
 
 

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_py as O
-from tests.helpers import TETRA_OBJ, dump_to_spec, parse_dump
+from tests.helpers import check_samples, TETRA_OBJ, dump_to_spec, parse_dump
 
 pytestmark = pytest.mark.gpu
 
@@ -43,5 +43,4 @@ def test_scene_api_render_matches_binding_and_oracle(gpu, tmp_path):
     gs = g.samples(cfg)
     o = O.OracleScene(spec)
     _, _, os_, _ = o.render(o.cfg(), want_samples=True)
-    close = np.all(np.abs(gs - os_) <= 1e-3 * np.maximum(np.abs(gs), np.abs(os_)) + 1e-4, axis=-1)
-    assert close.mean() > 0.99
+    check_samples(gs, os_, "scene API")

@@ -247,3 +247,95 @@ def test_phong_lobe_normalised():
     # pure specular Phong (diffuse 0) at normal incidence: lobe around the normal
     _, pdf = O.bsdf_eval(PHONG, [0, 0, 0, 0.5, 0.5, 0.5, 0, 0, 0, 20.0], wo, d, nrm)
     assert abs(np.sum(pdf * dw) - 1.0) < 5e-3
+
+
+# ---------------------------------------------------------------- ThinDielectric
+def _fresnel(ci, eta):
+    """Textbook unpolarised Fresnel reflectance of a dielectric interface (float64)."""
+    st2 = (1.0 - ci * ci) / eta ** 2
+    if st2 >= 1.0:
+        return 1.0
+    ct = np.sqrt(1.0 - st2)
+    r_par = (eta * ci - ct) / (eta * ci + ct)
+    r_perp = (ci - eta * ct) / (ci + eta * ct)
+    return 0.5 * (r_par ** 2 + r_perp ** 2)
+
+
+@pytest.mark.parametrize("cos_o", [1.0, 0.8, 0.5, 0.2, 0.05])
+def test_thin_dielectric_closed_form(cos_o):
+    """ThinDielectric (thinDielectric.hpp:26-33): a thin slab reflects
+    R' = R + T^2 R / (1 - R^2), which sums to 2R / (1 + R), and passes the rest
+    straight through (wi = -wo).  The branch is chosen by uc against R', its
+    pdf is R' or 1 - R', and the path weight is 1 either way."""
+    R = _fresnel(cos_o, 1.5)
+    Rt = 2 * R / (1 + R)
+    s = np.sqrt(1 - cos_o ** 2)
+    wo = (s, 0.0, cos_o)
+    wi, f, pdf, fl, ok = smooth_scatter(THIN, [1.5, 0.0], wo, Rt * (1 - 1e-4))
+    assert ok[0] and fl[0] == 16 | 1
+    assert abs(pdf[0] - Rt) < 1e-5 * Rt
+    assert np.allclose(wi[0], [-s, 0, cos_o], atol=1e-6) and np.allclose(f[0], 1.0)
+    wi, f, pdf, fl, ok = smooth_scatter(THIN, [1.5, 0.0], wo, min(Rt * (1 + 1e-4), 0.9999))
+    assert ok[0] and fl[0] == 16 | 2
+    assert abs(pdf[0] - (1 - Rt)) < 1e-5
+    assert np.allclose(wi[0], [-s, 0, -cos_o], atol=1e-6) and np.allclose(f[0], 1.0)
+
+
+def test_thin_dielectric_index_matched_passes_through():
+    """eta = 1: no reflection at all (R = 0), every sample passes straight through."""
+    for uc in (0.0, 0.5, 0.999):
+        wi, f, pdf, fl, ok = smooth_scatter(THIN, [1.0, 0.0], (0.3, 0.2, 0.9), uc)
+        assert ok[0] and fl[0] == 16 | 2 and abs(pdf[0] - 1.0) < 1e-6
+        assert np.allclose(wi[0], -unit(np.array([0.3, 0.2, 0.9])), atol=1e-6)
+
+
+# ---------------------------------------------------------------- Kajiya-Kay
+KK_PARAMS = [0.4, 0.3, 0.2, 0.3, 0.25, 0.2, 30.0]
+
+
+def _kk_expect(sin_i, spec_cos):
+    """Kajiya & Kay (1989): diffuse kd/pi sin(t, wi); specular ks (p+2)/(2pi) cos^p
+    of the angle to the mirror cone, or 0 where that cosine is negative."""
+    kd, ks, p = np.array(KK_PARAMS[0:3]), np.array(KK_PARAMS[3:6]), KK_PARAMS[6]
+    return kd / np.pi * sin_i + ks * (p + 2) / (2 * np.pi) * max(spec_cos, 0.0) ** p
+
+
+def test_kajiya_kay_closed_forms():
+    """On a surface the fibre tangent is OrthoBasis(n).u: (-1,0,0) for n = +z and
+    (0,-1,0) for n = +x (orthoBasis.hpp:15-21).  Values at normal incidence, on
+    the mirror cone t.wo = -t.wi (the specular peak), at retro-reflection
+    (cos of the cone angle = cos 2a) and along the tangent (no diffuse term)."""
+    a = 0.6
+    sa, ca = np.sin(a), np.cos(a)
+    z = (0.0, 0.0, 1.0)
+    cases = [  # (normal, wo, wi, expected f)
+        (z, z, z, _kk_expect(1.0, 1.0)),
+        (z, (-sa, 0, ca), (sa, 0, ca), _kk_expect(ca, 1.0)),                 # mirror cone
+        (z, (sa, 0, ca), (sa, 0, ca), _kk_expect(ca, np.cos(2 * a))),        # retro-reflection
+        (z, (np.sin(0.9), 0, np.cos(0.9)), (np.sin(0.9), 0, np.cos(0.9)), _kk_expect(np.cos(0.9), np.cos(1.8))),
+        (z, (0, 0.6, 0.8), (0, -0.6, 0.8), _kk_expect(1.0, 1.0)),            # plane normal to the fibre
+        ((1.0, 0, 0), (1.0, 0, 0), (1.0, 0, 0), _kk_expect(1.0, 1.0)),
+        ((1.0, 0, 0), (1.0, 0, 0), (0.0, 1.0, 0), _kk_expect(0.0, 0.0)),     # wi along the tangent
+    ]
+    for nrm, wo, wi, want in cases:
+        f, pdf = O.bsdf_eval(KK, KK_PARAMS, np.array([wo], np.float32), np.array([wi], np.float32),
+                             np.array([nrm], np.float32))
+        assert np.allclose(f[0], want, rtol=1e-5, atol=1e-7), (nrm, wo, wi, f[0], want)
+        assert abs(pdf[0] - max(np.dot(wi, nrm), 0.0) / np.pi) < 1e-6
+
+
+def test_kajiya_kay_sampling():
+    """Cosine-lobe sampling: returned f and pdf are f and scatter_pdf at the
+    sampled direction, and E[cos^2 / pdf] is the hemisphere integral 2pi/3."""
+    n = 200_000
+    rng = np.random.default_rng(13)
+    nrm = np.repeat(np.array([[0, 0, 1]], np.float32), n, 0)
+    wo = np.repeat(unit(np.array([[0.3, -0.2, 0.9]])), n, 0)
+    wi, f, pdf, fl, ok = O.bsdf_sample(KK, KK_PARAMS, wo, nrm, rng.random(n, np.float32),
+                                       rng.random((n, 2), np.float32))
+    assert np.all(ok == 1) and np.all(fl == 8 | 1)
+    est = np.mean(wi[:, 2] ** 2 / pdf)
+    assert abs(est - 2 * np.pi / 3) < 0.01 * 2 * np.pi / 3
+    f2, p2 = O.bsdf_eval(KK, KK_PARAMS, wo[:2000], wi[:2000], nrm[:2000])
+    assert np.allclose(p2, pdf[:2000], rtol=1e-5) and np.allclose(f2, f[:2000], rtol=1e-5)
+    assert np.all(f >= 0)

@@ -170,3 +170,75 @@ def test_disney_sample_pdf_consistency():
     g = np.maximum(wi[:, 2], 0) ** 2
     est = np.where(m, g / np.maximum(pdf, 1e-30), 0).mean()
     assert 0.95 * 2 * np.pi / 3 < est < 1.01 * 2 * np.pi / 3
+
+
+# ---------------------------------------------------------------- hair: closed-form identities
+def _fresnel_dielectric(ci, eta):
+    """Textbook unpolarised Fresnel reflectance of a dielectric interface (float64)."""
+    if ci < 0:
+        eta, ci = 1.0 / eta, -ci
+    st2 = (1.0 - ci * ci) / eta ** 2
+    if st2 >= 1.0:
+        return 1.0
+    ct = np.sqrt(1.0 - st2)
+    r_par = (eta * ci - ct) / (eta * ci + ct)
+    r_perp = (ci - eta * ct) / (ci + eta * ct)
+    return 0.5 * (r_par ** 2 + r_perp ** 2)
+
+
+def _fibre_sphere_grid(n_s=1024, n_phi=1024):
+    """Midpoint grid over the sphere in the fibre's coordinates for the normal
+    n = (0,0,1): the fibre axis is OrthoBasis(n).u = (-1,0,0), sin(theta) is
+    the component along it and phi the angle about it, so dw = dsin(theta) dphi."""
+    s = (np.arange(n_s) + 0.5) / n_s * 2 - 1
+    ph = (np.arange(n_phi) + 0.5) / n_phi * 2 * np.pi - np.pi
+    S, P = np.meshgrid(s, ph, indexing="ij")
+    c = np.sqrt(1 - S ** 2)
+    d = np.stack([-S, c * np.cos(P), c * np.sin(P)], -1).reshape(-1, 3).astype(np.float32)
+    return d, S.reshape(-1), P.reshape(-1), (2.0 / n_s) * (2 * np.pi / n_phi)
+
+
+def _fibre_dir(sin_t, phi):
+    c = np.sqrt(1 - sin_t ** 2)
+    return np.array([-sin_t, c * np.cos(phi), c * np.sin(phi)], np.float32)
+
+
+@pytest.mark.parametrize("beta_m", [0.2, 0.4, 0.6])
+def test_hair_white_furnace(beta_m):
+    """sigma_a = 0: the lobes' energies A_p sum to 1 (the p = pMax term is the
+    geometric remainder), Mp is normalised in theta_i for any theta_o (so the
+    unrotated theta_o of evalMarschner keeps the identity, hairMarsch.cpp:67-92)
+    and Np in phi, hence the integral of f |cos| over the sphere is 1.  The
+    reference's fastExp / I0 approximations leave it within 1.5% below."""
+    d, _, _, dw = _fibre_sphere_grid()
+    n = d.shape[0]
+    nrm = np.repeat(np.array([[0, 0, 1]], np.float32), n, 0)
+    for h in (0.0, 0.5, -0.7):
+        for sin_o, phi_o in ((0.0, 1.5707964), (0.6, 0.0), (0.3, 1.03)):
+            wo = np.repeat(_fibre_dir(sin_o, phi_o)[None], n, 0)
+            f, _ = O.bsdf_eval(HAIR, [0, 0, 0, h, 1.55, beta_m, 0.3, np.radians(2.0)], wo, d, nrm)
+            e = (f * np.abs(d[:, 2:3])).sum(0) * dw
+            assert np.all(e > 0.985) and np.all(e < 1.005), (h, sin_o, e)
+
+
+def test_hair_r_lobe_energy_and_direction():
+    """A strongly absorbing fibre (sigma_a = 50) leaves the R lobe alone: its
+    energy is A0 = Fresnel(cos theta_o cos gamma_o, eta) with cos gamma_o =
+    sqrt(1 - h^2) (hairMarsch.hpp:96-103), its peak lies on the specular cone
+    sin theta_i = -sin theta_o (Mp's exp(-sin_i sin_o / v), :81-86), and in
+    azimuth at Phi(0) = -2 gamma_o (:89-92) with the reference's gamma_o =
+    asin(sin theta_o) (hairMarsch.cpp:38), not pbrt's asin(h)."""
+    d, S, P, dw = _fibre_sphere_grid()
+    n = d.shape[0]
+    nrm = np.repeat(np.array([[0, 0, 1]], np.float32), n, 0)
+    for h in (0.0, 0.5, -0.7):
+        for sin_o, phi_o in ((0.3, 0.4), (-0.5, 1.2)):
+            wo = np.repeat(_fibre_dir(sin_o, phi_o)[None], n, 0)
+            f, _ = O.bsdf_eval(HAIR, [50.0, 50.0, 50.0, h, 1.55, 0.2, 0.3, np.radians(2.0)], wo, d, nrm)
+            g = f[:, 0].astype(np.float64) * np.abs(d[:, 2])
+            a0 = _fresnel_dielectric(np.sqrt(1 - sin_o ** 2) * np.sqrt(1 - h * h), 1.55)
+            assert abs(g.sum() * dw / a0 - 1.0) < 0.01, (h, sin_o)
+            k = int(np.argmax(g))
+            assert abs(S[k] + sin_o) < 0.012
+            dphi = (P[k] - phi_o + np.pi) % (2 * np.pi) - np.pi
+            assert abs(dphi + 2 * np.arcsin(sin_o)) < 0.01

@@ -175,6 +175,39 @@ def _smooth_scene(pos, idx, nrm, nidx, flags=0):
     return s
 
 
+def test_motion_blur_camera_moments():
+    """A shutter moves the camera from pos (time 0) to pos1 (time 1), the time
+    uniform in [0, shutter): with a pinhole every origin lies on the segment,
+    its position along it has mean shutter/2 and variance shutter^2/12 (the
+    first two moments of a uniform law), and the direction through a pixel
+    does not depend on the time (the image plane moves with the camera)."""
+    spec = scenes.cornell(width=16, height=16, spp_x=4, spp_y=4)
+    p0, p1, shutter = np.array([0.0, 1.0, 3.9]), np.array([0.6, 1.3, 3.5]), 0.75
+    spec.camera = dict(spec.camera, pos1=tuple(p1), shutter=shutter)
+    o = O.OracleScene(spec)
+    cfg = o.cfg()
+    n = 256 * 16
+    pix = np.repeat(np.arange(256, dtype=np.uint32), 16)
+    k = np.tile(np.arange(16, dtype=np.uint32), 256)
+    rays = o.camera_rays(cfg, pix, k)
+    delta = p1 - p0
+    s = (rays[:, :3] - p0) @ delta / (delta @ delta)
+    off = rays[:, :3] - (p0 + s[:, None] * delta)
+    assert np.abs(off).max() < 1e-5                        # on the segment
+    assert s.min() >= -1e-6 and s.max() < shutter + 1e-6
+    se_mean = shutter / np.sqrt(12 * n)
+    assert abs(s.mean() - shutter / 2) < 4 * se_mean
+    assert abs(s.var() - shutter ** 2 / 12) < 0.05 * shutter ** 2 / 12
+    static = scenes.cornell(width=16, height=16, spp_x=4, spp_y=4)
+    os_ = O.OracleScene(static)
+    rays0 = os_.camera_rays(os_.cfg(), pix, k)
+    assert np.allclose(rays[:, 3:6], rays0[:, 3:6], atol=2e-6)
+    # zero shutter is the static camera
+    spec.camera = dict(spec.camera, shutter=0.0)
+    oz = O.OracleScene(spec)
+    assert np.array_equal(oz.camera_rays(oz.cfg(), pix, k), rays0)
+
+
 def test_smooth_normal_interpolation_corners():
     """MeshBVHNode::hitTriangle (meshBVH.cpp:116-130): the shading normal is
     normalize((1-u-v) n0 + u n1 + v n2) with u along p1-p0 and v along p2-p0,

@@ -289,3 +289,16 @@ def test_volume_single_scatter_matches_quadrature(g):
     n = 256 * 256
     err = np.abs(img[0, 0] - expect)
     assert np.all(err < 5 * np.sqrt(var[0, 0] / n) + 5e-3 * expect), (img[0, 0], expect, np.sqrt(var[0, 0] / n))
+
+
+def test_reference_traversal_count_changes_no_sample():
+    """OR_RENDER_REFERENCE_TRAVERSALS repeats the reference's second traversal
+    of each continuation ray (maxdepth_nee.hpp:238-255): every sample is the
+    same, and the closest-hit traversals per non-specular vertex double."""
+    spec = scenes.cornell(width=24, height=24, spp_x=2, spp_y=2)
+    o = O.OracleScene(spec)
+    _, _, a, ca = o.render(o.cfg(estimator=0), want_samples=True)
+    _, _, b, cb = o.render(o.cfg(estimator=0, flags=1 << 24), want_samples=True)
+    assert np.array_equal(a, b)
+    assert ca[2] == cb[2]                      # shadow rays unchanged
+    assert 1.4 * ca[1] < cb[1] < 2.0 * ca[1]   # camera rays are traversed once, continuations twice
