@@ -296,6 +296,24 @@ pt_status pt_film_read(pt_scene *scene, float *out_rgb);
  * (n = width*height*spp*3 floats), ordered pixel-major then sample.        */
 pt_status pt_film_read_samples(pt_scene *scene, float *out_rgb, size_t n);
 
+/* pt_render without any device allocation.  No counterpart in the
+ * reference (whose renderer allocates per frame, renderManager.hpp:80-95):
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * scene must be committed (the commit allocates its device data).  The film
+ * is the caller's device buffer of width*height*3 floats; it receives the
+ * per-pixel means, row-major with j = 0 first (what pt_film_read returns
+ * after pt_render).  The wavefront, pixel offsets and counters live in one
+ * static 64 MiB device array per GPU, held until the process ends: at most
+ * 131,072 path slots in flight (cfg->wavefront 0 means that maximum; larger
+ * values are refused) and 65,536 samples per pixel.  Not supported (refused
+ * with PT_ERR_UNSUPPORTED): PT_RENDER_KEEP_SAMPLES, PT_RENDER_TIMING,
+ * PT_RENDER_ACCUMULATE, and the permuted-digit Halton sampler for a seed
+ * whose tables the scene has not built.  The scene's device must be
+ * current.  Calls on one GPU share the static array, so calls that overlap
+ * on one GPU (from several threads or streams) race.                       */
+pt_status pt_render_noalloc(pt_scene *scene, const pt_render_cfg *cfg, float *film, pt_stats *stats);
+
 /* ------------------------------------------------------------------ */
 /* Launch tuning and the autotuner (no counterpart in the reference,    */
 /* whose OpenMP loop has no launch parameters, renderManager.hpp:99).   */

@@ -122,7 +122,7 @@ EXPORTED = [
     "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_mesh", "pt_scene_add_mesh_normals",
     "pt_scene_add_sphere", "pt_scene_add_rect", "pt_scene_add_curve", "pt_scene_set_env", "pt_scene_set_camera",
     "pt_scene_set_medium", "pt_scene_update_mesh",
-    "pt_scene_commit", "pt_scene_get_info", "pt_render", "pt_film_read", "pt_film_read_samples",
+    "pt_scene_commit", "pt_scene_get_info", "pt_render", "pt_render_noalloc", "pt_film_read", "pt_film_read_samples",
     "pt_scene_set_tuning", "pt_autotune",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_cfg", "pt_batch_render",
     "pt_batch_film_read", "pt_batch_set_tuning", "pt_batch_autotune",
@@ -160,6 +160,7 @@ def _load():
         "pt_scene_commit": (i32, [_P, ctypes.POINTER(BuildCfg)]),
         "pt_scene_get_info": (i32, [_P, ctypes.POINTER(SceneInfo)]),
         "pt_render": (i32, [_P, ctypes.POINTER(RenderCfg), ctypes.POINTER(Stats)]),
+        "pt_render_noalloc": (i32, [_P, ctypes.POINTER(RenderCfg), _P, ctypes.POINTER(Stats)]),
         "pt_film_read": (i32, [_P, _P]),
         "pt_film_read_samples": (i32, [_P, _P, sz]),
         "pt_scene_set_tuning": (i32, [_P, ctypes.POINTER(Tuning)]),

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -1155,32 +1156,40 @@ extern "C" pt_status pt_scene_get_info(const pt_scene *s, pt_scene_info *info) {
 }
 
 // ------------------------------------------------------------------ wavefront buffers
-static pt_status ensure_wave(WaveStore &ws, uint32_t cap) {
-  if (ws.cap >= cap && ws.buf.p) return PT_OK;
+// Layout of the wavefront of `cap` path slots: two live-path buffers, the
+// radiance per slot and the shadow queue, each array 256-byte aligned.
+// Partition p of a live-path buffer starts with at most R * ceil(n / (R P))
+// paths (runs of R slots dealt round-robin) and never grows; its shadow rays,
+// at most 4 per path, stay in partition p.
+struct WaveLayout {
+  uint64_t part_cap, shadow_part_cap;
+  size_t lcap, scap, bytes;
+};
+static bool wave_layout(uint32_t cap, WaveLayout &l) {
   const size_t per_state = 20;   // 4-byte words per entry of one live-path buffer
   const size_t per_slot = 3;     // radiance per slot
   const size_t per_shadow = 11;  // 4-byte words per shadow-queue entry
   const size_t n_arrays = 2 * per_state + per_slot + per_shadow;
-  // Partition p of a live-path buffer starts with at most R * ceil(n / (R P))
-  // paths (runs of R slots dealt round-robin) and never grows; its shadow
-  // rays, at most 4 per path, stay in partition p.
   const uint64_t run = PT_PART_RUN;
-  const uint64_t part_cap = ((uint64_t)cap + run * PT_PARTS - 1) / (run * PT_PARTS) * run;
-  const uint64_t shadow_part_cap = 4 * part_cap;
-  if (shadow_part_cap * PT_PARTS >= (1ull << 32)) return fail(PT_ERR_INVALID, "pt_render: wavefront too large");
-  const size_t lcap = (size_t)(part_cap * PT_PARTS);
-  const size_t scap = (size_t)(shadow_part_cap * PT_PARTS);
-  size_t bytes = (2 * per_state * lcap + per_slot * (size_t)cap + per_shadow * scap) * 4 + n_arrays * 256;
-  HIP_TRY(ws.buf.alloc(bytes));
-  HIP_TRY(ws.counts.alloc(PT_COUNTER_WORDS * sizeof(uint32_t)));
-  HIP_TRY(ws.stats.alloc(CNT_N * sizeof(unsigned long long)));
-  char *p = ws.buf.as<char>();
+  l.part_cap = ((uint64_t)cap + run * PT_PARTS - 1) / (run * PT_PARTS) * run;
+  l.shadow_part_cap = 4 * l.part_cap;
+  if (l.shadow_part_cap * PT_PARTS >= (1ull << 32)) return false;
+  l.lcap = (size_t)(l.part_cap * PT_PARTS);
+  l.scap = (size_t)(l.shadow_part_cap * PT_PARTS);
+  l.bytes = (2 * per_state * l.lcap + per_slot * (size_t)cap + per_shadow * l.scap) * 4 + n_arrays * 256;
+  return true;
+}
+// Points the arrays of w into base (at least l.bytes), counters and stats
+// into their own buffers.
+static void carve_wave(char *base, const WaveLayout &l, uint32_t cap, uint32_t *counts, unsigned long long *stats,
+                       Wave &w) {
+  char *p = base;
   auto take = [&](size_t n) {
     char *r = p;
     p += ((n * 4 + 255) / 256) * 256;
     return r;
   };
-  Wave &w = ws.wave;
+  const size_t lcap = l.lcap, scap = l.scap;
   for (int b = 0; b < 2; ++b) {
     PathState &ps = w.st[b];
     ps.ox = (float *)take(lcap); ps.oy = (float *)take(lcap); ps.oz = (float *)take(lcap);
@@ -1199,15 +1208,24 @@ static pt_status ensure_wave(WaveStore &ws, uint32_t cap) {
   w.stmax = (float *)take(scap);
   w.scr = (float *)take(scap); w.scg = (float *)take(scap); w.scb = (float *)take(scap);
   w.sslot = (uint32_t *)take(scap);
-  if ((size_t)(p - ws.buf.as<char>()) > ws.buf.bytes) return fail(PT_ERR_INVALID, "internal: wave layout overflow");
-  w.counts = ws.counts.as<uint32_t>();
-  w.stats = ws.stats.as<unsigned long long>();
+  w.counts = counts;
+  w.stats = stats;
   w.cap = cap;
-  w.part_cap = (uint32_t)part_cap;
-  w.shadow_part_cap = (uint32_t)shadow_part_cap;
+  w.part_cap = (uint32_t)l.part_cap;
+  w.shadow_part_cap = (uint32_t)l.shadow_part_cap;
   w.sort_keys = 0u;
   w.rc = nullptr;
   w.rvis = nullptr;
+}
+
+static pt_status ensure_wave(WaveStore &ws, uint32_t cap) {
+  if (ws.cap >= cap && ws.buf.p) return PT_OK;
+  WaveLayout l;
+  if (!wave_layout(cap, l)) return fail(PT_ERR_INVALID, "pt_render: wavefront too large");
+  HIP_TRY(ws.buf.alloc(l.bytes));
+  HIP_TRY(ws.counts.alloc(PT_COUNTER_WORDS * sizeof(uint32_t)));
+  HIP_TRY(ws.stats.alloc(CNT_N * sizeof(unsigned long long)));
+  carve_wave(ws.buf.as<char>(), l, cap, ws.counts.as<uint32_t>(), ws.stats.as<unsigned long long>(), ws.wave);
   ws.res.release();
   ws.cap = cap;
   return PT_OK;
@@ -1584,6 +1602,125 @@ static pt_status render_frame(pt_scene *s, const pt_render_cfg *cfg, pt_stats *s
   }
   (void)hipEventDestroy(tstart);
   (void)hipEventDestroy(tstop);
+  return PT_OK;
+}
+
+// ------------------------------------------------------------------ render without allocation
+// pt_render_noalloc keeps its wavefront (live paths, radiance, shadow queue,
+// reservoir candidates), pixel offsets and counters in `scratch`, a file-scope
+// device array that every GPU the library runs on holds until the process
+// ends: kScratchBytes = 64 MiB, enough for kNoAllocSlots = 131,072 path slots
+// and kNoAllocMaxSpp samples per pixel.  Its device address is looked up once
+// per device (hipGetSymbolAddress) and kept in g_scratch; host code uses that
+// address, never the array's name.  Calls on one GPU share the array: calls
+// that overlap on one GPU (different threads or streams) race.
+static constexpr size_t kScratchBytes = 64ull << 20;
+static constexpr uint32_t kNoAllocSlots = 1u << 17;
+static constexpr uint32_t kNoAllocMaxSpp = 1u << 16;
+static constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static char *g_scratch[kMaxDevices];
+
+// The device address of `scratch` on the current device (looked up on the
+// first call on that device).
+static pt_status scratch_address(int device, char **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(PT_ERR_UNSUPPORTED, "pt_render_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(PT_ERR_DEVICE, std::string("pt_render_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<char *>(p);
+  }
+  *out = g_scratch[device];
+  return PT_OK;
+}
+
+extern "C" pt_status pt_render_noalloc(pt_scene *s, const pt_render_cfg *cfg, float *film, pt_stats *stats) {
+  uint32_t spp = 0, sppc = 0;
+  pt_status st = check_cfg("pt_render_noalloc", s, cfg, spp, sppc);
+  if (st != PT_OK) return st;
+  if (!film) return fail(PT_ERR_INVALID, "pt_render_noalloc: NULL film");
+  if (cfg->flags & (PT_RENDER_KEEP_SAMPLES | PT_RENDER_TIMING | PT_RENDER_ACCUMULATE))
+    return fail(PT_ERR_UNSUPPORTED,
+                "pt_render_noalloc: PT_RENDER_KEEP_SAMPLES, PT_RENDER_TIMING and PT_RENDER_ACCUMULATE need allocations");
+  if (cfg->wavefront > kNoAllocSlots)
+    return fail(PT_ERR_INVALID, "pt_render_noalloc: wavefront above the maximum of 131072 path slots");
+  if (spp > kNoAllocMaxSpp) return fail(PT_ERR_INVALID, "pt_render_noalloc: more than 65536 samples per pixel");
+  if (cfg->sampler == PT_SAMPLER_HALTON_PERM && !(s->perm_valid && s->perm_seed == cfg->seed))
+    return fail(PT_ERR_UNSUPPORTED, "pt_render_noalloc: the digit permutations of this seed are not built "
+                                    "(a pt_render with the seed builds them)");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != s->device) return fail(PT_ERR_INVALID, "pt_render_noalloc: call with the scene's device current");
+  char *base = nullptr;
+  st = scratch_address(dev, &base);
+  if (st != PT_OK) return st;
+
+  const uint64_t npix = (uint64_t)cfg->width * cfg->height;
+  const uint64_t total = npix * sppc;
+  const uint32_t want = cfg->wavefront ? cfg->wavefront : kNoAllocSlots;
+  const uint64_t chunk_pix = std::min<uint64_t>(std::max<uint64_t>(1, want / sppc), npix);
+  const uint64_t cap64 = chunk_pix * sppc;
+  if (cap64 > kNoAllocSlots)
+    return fail(PT_ERR_INVALID, "pt_render_noalloc: one pixel's samples exceed the 131072 path slots");
+  const uint32_t cap = (uint32_t)cap64;
+
+  // scratch layout: counters, stats, pixel offsets, reservoir candidates, wavefront
+  WaveLayout l;
+  if (!wave_layout(cap, l)) return fail(PT_ERR_INVALID, "pt_render_noalloc: wavefront too large");
+  auto up = [](size_t n) { return (n + 255) / 256 * 256; };
+  const size_t o_counts = 0;
+  const size_t o_stats = o_counts + up(PT_COUNTER_WORDS * sizeof(uint32_t));
+  const size_t o_off = o_stats + up(CNT_N * sizeof(unsigned long long));
+  const size_t o_rc = o_off + up(2 * (size_t)spp * sizeof(float));
+  const size_t rc_bytes = up(12 * (size_t)cap * sizeof(float));
+  const size_t o_rvis = o_rc + rc_bytes;
+  const size_t o_wave = o_rvis + up(4 * (size_t)cap);
+  if (o_wave + l.bytes > kScratchBytes) return fail(PT_ERR_INVALID, "pt_render_noalloc: scratch too small");
+  Wave wave{};
+  carve_wave(base + o_wave, l, cap, reinterpret_cast<uint32_t *>(base + o_counts),
+             reinterpret_cast<unsigned long long *>(base + o_stats), wave);
+  if (cfg->estimator == PT_EST_RESERVOIR) {
+    wave.rc = reinterpret_cast<float *>(base + o_rc);
+    wave.rvis = reinterpret_cast<uint8_t *>(base + o_rvis);
+  }
+  wave.sort_keys = ((cfg->flags & PT_RENDER_SORT_HITS) || s->tuning.sort_bits) ? 1u : 0u;
+  float *off = reinterpret_cast<float *>(base + o_off);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipMemsetAsync(film, 0, npix * 3 * sizeof(float), s->stream));
+  HIP_TRY(hipMemsetAsync(wave.stats, 0, CNT_N * sizeof(unsigned long long), s->stream));
+  launch_pixel_offsets(s->stream, cfg->spp_x, cfg->spp_y, cfg->seed, off);
+  RenderParams rp = make_params(cfg, spp, sppc);
+  const DevCamera cam = make_camera(s->cam);
+  ChunkPlan pl{};
+  pl.nverts = n_vertices(cfg);
+  pl.estimator = cfg->estimator;
+  pl.mat_kinds = s->ds.mat_kinds;
+  pl.count = (cfg->flags & PT_RENDER_COUNTERS) != 0;
+  pl.grids = make_grids(s->n_cu, s->tuning);
+  uint64_t launches = 0;
+  for (uint64_t first = 0; first < total; first += cap) {
+    const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - first);
+    rp.chunk_first = first;
+    rp.chunk_paths = n;
+    pl.max_paths = n;
+    launch_chunk(s->stream, s->ds, cam, rp, wave, off, off + spp, film, nullptr, nullptr, pl, nullptr);
+    launches += pl.nverts;
+  }
+  // per-pixel sums to means, as pt_film_read scales them (1 / spp in float)
+  launch_film_scale(s->stream, film, npix * 3, 1.f / (float)sppc);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (stats) {
+    unsigned long long c[CNT_N];
+    HIP_TRY(hipMemcpy(c, wave.stats, sizeof(c), hipMemcpyDeviceToHost));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    fill_stats(stats, c, total, launches, (float)ms);
+  }
   return PT_OK;
 }
 

@@ -906,6 +906,24 @@ __global__ void __launch_bounds__(256) k_film_lanes(RenderParams rp, Wave w, flo
   if (nan) atomicAdd(&stats[CNT_NAN], (unsigned long long)nan);
 }
 
+// Per-frame jittered pixel offsets (FrameBuffer::setOffsets,
+// framebuffer.cpp:131-145), computed as the host computes them for pt_render:
+// sample k = i * spp_y + j lies in stratum (i, j).
+__global__ void k_pixel_offsets(uint32_t spp_x, uint32_t spp_y, uint32_t seed, float *off) {
+  const uint32_t k = gtid(), spp = spp_x * spp_y;
+  if (k >= spp) return;
+  const uint32_t i = k / spp_y, j = k % spp_y;
+  const float a0 = (float)i / (float)spp_x, a1 = (float)(i + 1) / (float)spp_x;
+  const float b0 = (float)j / (float)spp_y, b1 = (float)(j + 1) / (float)spp_y;
+  off[k] = (float)(a0 + (a1 - a0) * (double)pt_rng(0xffffffffu - k, 0u, seed ^ 0x5bd1e995u));
+  off[spp + k] = (float)(b0 + (b1 - b0) * (double)pt_rng(0xffffffffu - k, 1u, seed ^ 0x5bd1e995u));
+}
+
+__global__ void k_film_scale(float *film, size_t n, float scale) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) film[i] *= scale;
+}
+
 // Advances the batch to its next chunk (the last launch of a chunk).
 __global__ void k_next_chunk(BatchView b) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *b.cur += 1u;
@@ -1131,6 +1149,13 @@ void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *f
     hipLaunchKernelGGL(k_film_lanes, dim3(blocks_for((size_t)npix << lg)), dim3(256), 0, st, rp, w, film, samples,
                        stats, lg);
   }
+}
+void launch_pixel_offsets(hipStream_t st, uint32_t spp_x, uint32_t spp_y, uint32_t seed, float *off) {
+  hipLaunchKernelGGL(k_pixel_offsets, dim3(blocks_for((size_t)spp_x * spp_y)), dim3(256), 0, st, spp_x, spp_y, seed,
+                     off);
+}
+void launch_film_scale(hipStream_t st, float *film, size_t n, float scale) {
+  hipLaunchKernelGGL(k_film_scale, dim3(blocks_for(n)), dim3(256), 0, st, film, n, scale);
 }
 void launch_next_chunk(hipStream_t st, const BatchView &b) {
   hipLaunchKernelGGL(k_next_chunk, dim3(1), dim3(64), 0, st, b);

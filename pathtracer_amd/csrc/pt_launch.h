@@ -16,6 +16,11 @@ void launch_shadow(hipStream_t st, const DevScene &s, const Wave &w, uint32_t ma
 void launch_film(hipStream_t st, const RenderParams &rp, const Wave &w, float *film, float *samples,
                  unsigned long long *stats, const BatchView *b, uint32_t max_pix);
 void launch_next_chunk(hipStream_t st, const BatchView &b);
+// FrameBuffer::setOffsets (framebuffer.cpp:131-145) on the device: off[k] and
+// off[spp + k] of sample k, the values the host computes for pt_render.
+void launch_pixel_offsets(hipStream_t st, uint32_t spp_x, uint32_t spp_y, uint32_t seed, float *off);
+// film[i] *= scale for the n floats of a film (per-pixel sums to means).
+void launch_film_scale(hipStream_t st, float *film, size_t n, float scale);
 void launch_set_chunk(hipStream_t st, const BatchView &b, uint32_t c);
 void launch_sample(hipStream_t st, const DevScene &s, uint32_t sampler, uint32_t seed, uint32_t sx, uint32_t sy,
                    const uint32_t *dim, const uint64_t *index, const uint32_t *pixel, size_t n, float *out);
