@@ -469,6 +469,10 @@ PT_HD bool bsdf_reflect_only(uint32_t kind) {
 constexpr uint32_t kMatBasic = (1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_EMITTER) | (1u << PT_MAT_CONDUCTOR) |
                                (1u << PT_MAT_MFDIELECTRIC) | (1u << PT_MAT_THINDIELECTRIC) |
                                (1u << PT_MAT_PHONG) | (1u << PT_MAT_KAJIYA_KAY);
+// The smallest set: Lambertian walls, emitters and the microfacet conductor
+// and dielectric (the Cornell and glass/HDRI scenes).
+constexpr uint32_t kMatMicro = (1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_EMITTER) | (1u << PT_MAT_CONDUCTOR) |
+                               (1u << PT_MAT_MFDIELECTRIC);
 #define PT_IN(MS, K) (((MS) >> (K)) & 1u)
 
 template <uint32_t MS = kMatAll>

@@ -188,16 +188,22 @@ PT_D void stage_top_nodes(const DevScene &s, float4 *top, bool batch) {
 // float4 per node in the array traversal reads (QN: any hit)
 template <bool QN>
 constexpr int node_f4() { return QN ? PT_QNODE_F4 : 8; }
+// Pointers into a block's LDS, typed so that loads through them compile to
+// ds_read (a generic pointer that may address LDS or global memory compiles
+// to flat loads, which wait on both counters).
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(3))) float lds_float;
 // The four child boxes (SoA) and codes of node nd: a quantised node (QN,
-// decode_node) or a 128-byte one.
-template <bool QN>
-PT_D void load_node(const float4 *nd, float (&lx)[4], float (&hx)[4], float (&ly)[4], float (&hy)[4],
+// decode_node) or a 128-byte one.  P: const float4 * (global) or lds_v4 * (LDS).
+template <bool QN, class P>
+PT_D void load_node(P nd, float (&lx)[4], float (&hx)[4], float (&ly)[4], float (&hy)[4],
                     float (&lz)[4], float (&hz)[4], int4 &ch) {
   if constexpr (QN) {
     decode_node(nd, lx, hx, ly, hy, lz, hz, ch);
   } else {
-    const float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3], e = nd[4], f = nd[5];
-    ch = *reinterpret_cast<const int4 *>(nd + 6);
+    const float4 a = ld_f4(nd, 0), b = ld_f4(nd, 1), c = ld_f4(nd, 2), d = ld_f4(nd, 3), e = ld_f4(nd, 4),
+                 f = ld_f4(nd, 5), g = ld_f4(nd, 6);
+    ch = make_int4((int)pt_float_bits(g.x), (int)pt_float_bits(g.y), (int)pt_float_bits(g.z), (int)pt_float_bits(g.w));
     lx[0] = a.x; lx[1] = a.y; lx[2] = a.z; lx[3] = a.w;
     hx[0] = b.x; hx[1] = b.y; hx[2] = b.z; hx[3] = b.w;
     ly[0] = c.x; ly[1] = c.y; ly[2] = c.z; ly[3] = c.w;
@@ -217,10 +223,14 @@ PT_D void load_node(const float4 *nd, float (&lx)[4], float (&hx)[4], float (&ly
 // 128-byte ones.
 // `lds` is the calling block's PT_TRAV_LDS(K, !ANY) words of LDS.
 // ---------------------------------------------------------------------
-#define PT_CSWAP(ta, ca, tb_, cb)          \
-  if (tb_ < ta) {                         \
-    float _t = ta; ta = tb_; tb_ = _t;    \
-    int _c = ca; ca = cb; cb = _c;        \
+// Compare-exchange of (distance, code) pairs as selects, so that the network
+// is straight-line code (v_cndmask) rather than exec-masked branches.
+#define PT_CSWAP(ta, ca, tb_, cb)                                   \
+  {                                                                 \
+    const bool _s = tb_ < ta;                                       \
+    const float _lo = _s ? tb_ : ta, _hi = _s ? ta : tb_;           \
+    const int _cl = _s ? cb : ca, _ch = _s ? ca : cb;               \
+    ta = _lo; tb_ = _hi; ca = _cl; cb = _ch;                        \
   }
 template <bool ANY, bool QN, bool COUNT, int K>
 PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tmax, float &t_out, int &prim_out,
@@ -374,8 +384,8 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
   constexpr bool WT = !ANY;
   int pc[PT_STACK4];
   float ptv[WT ? PT_STACK4 : 1];
-  int *const lc = lds + threadIdx.x;
-  float *const lt = reinterpret_cast<float *>(lds + K * PT_BLOCK) + threadIdx.x;
+  lds_int *const lc = (lds_int *)lds + threadIdx.x;
+  lds_float *const lt = (lds_float *)(lds + K * PT_BLOCK) + threadIdx.x;
   const uint32_t lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   uint32_t next = begin;
@@ -434,10 +444,16 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
   auto node_step = [&]() {
     float lxs[4], hxs[4], lys[4], hys[4], lzs[4], hzs[4];
     int4 ch;
-    if ((uint32_t)cur < ntop)
-      load_node<QN>(top + NF4 * cur, lxs, hxs, lys, hys, lzs, hzs, ch);
-    else
+    if constexpr (QN) {
+      // one decode on a generic pointer: a decode per address space costs the
+      // any-hit kernel spills (shadow rays 23.3 -> 35.5 ms on the 1.3M-triangle frame)
+      load_node<QN>(((uint32_t)cur < ntop ? top : (BATCH ? lane_nodes : snodes)) + NF4 * (size_t)cur, lxs, hxs, lys,
+                    hys, lzs, hzs, ch);
+    } else if ((uint32_t)cur < ntop) {
+      load_node<QN>((lds_v4 *)top + NF4 * cur, lxs, hxs, lys, hys, lzs, hzs, ch);
+    } else {
       load_node<QN>((BATCH ? lane_nodes : snodes) + NF4 * (size_t)cur, lxs, hxs, lys, hys, lzs, hzs, ch);
+    }
     if (COUNT) ++n_nodes;
     float tn[4], tf[4];
 #pragma unroll

@@ -1079,8 +1079,14 @@ void launch_extend(hipStream_t st, const DevScene &s, const Wave &w, int qi, uin
 template <int EST, bool BATCH>
 static void launch_shade_est(hipStream_t st, const DevScene &s, const RenderParams &rp, const Wave &w, int qi,
                              uint32_t d, uint32_t g, const BatchView &b, uint32_t mat_kinds) {
-  // the basic material set when the scene uses nothing else (fewer registers)
-  if ((mat_kinds & ~kMatBasic) == 0u)
+  // the smallest material set that holds every kind the scene uses (fewer
+  // registers, less code); the microfacet set for the two estimators the
+  // workloads render with
+  constexpr bool micro = EST == PT_EST_NEE_MIS || EST == PT_EST_REFERENCE_NEE;
+  if (micro && (mat_kinds & ~kMatMicro) == 0u)
+    hipLaunchKernelGGL((k_shade<EST, micro ? kMatMicro : kMatBasic, BATCH>), dim3(g), dim3(256), 0, st, s, rp, w, qi,
+                       d, b);
+  else if ((mat_kinds & ~kMatBasic) == 0u)
     hipLaunchKernelGGL((k_shade<EST, kMatBasic, BATCH>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d, b);
   else
     hipLaunchKernelGGL((k_shade<EST, kMatAll, BATCH>), dim3(g), dim3(256), 0, st, s, rp, w, qi, d, b);

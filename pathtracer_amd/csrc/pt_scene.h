@@ -130,10 +130,22 @@ PT_HD void quantize_node(const float4 *in, float4 *out) {
   out[2] = make_float4(pt_bits_float(ql[2]), pt_bits_float(qh[2]), 0.f, 0.f);
   out[3] = in[6];
 }
+// float4 i of a node array in global memory, or in a block's LDS (a native
+// vector type, so that the load compiles to ds_read_b128 rather than a flat
+// load that waits on both memory counters).
+typedef float pt_v4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(3))) pt_v4 lds_v4;
+PT_HD inline float4 ld_f4(const float4 *p, int i) { return p[i]; }
+__device__ inline float4 ld_f4(lds_v4 *p, int i) {
+  const pt_v4 v = p[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 // Decodes node nd: the child boxes (SoA over the four children) and codes.
-PT_HD void decode_node(const float4 *nd, float (&lx)[4], float (&hx)[4], float (&ly)[4], float (&hy)[4],
+// P: const float4 * (global) or lds_v4 * (LDS).
+template <class P>
+PT_HD void decode_node(P nd, float (&lx)[4], float (&hx)[4], float (&ly)[4], float (&hy)[4],
                        float (&lz)[4], float (&hz)[4], int4 &ch) {
-  const float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+  const float4 a = ld_f4(nd, 0), b = ld_f4(nd, 1), c = ld_f4(nd, 2), d = ld_f4(nd, 3);
   const uint32_t eb = pt_float_bits(a.w);
   const float sx = pt_bits_float((eb & 255u) << 23), sy = pt_bits_float(((eb >> 8) & 255u) << 23),
               sz = pt_bits_float(((eb >> 16) & 255u) << 23);
