@@ -57,9 +57,14 @@ enum pt_mat_kind {
 #define PT_MAT_NPARAMS 30
 typedef struct pt_material_desc {
   uint32_t kind;              /* pt_mat_kind                                */
-  uint32_t flags;             /* reserved, 0                                */
+  uint32_t flags;             /* 0, or PT_MAT_TEXTURE(id)                    */
   float p[PT_MAT_NPARAMS];    /* kind-specific parameters, see pt_mat_kind  */
 } pt_material_desc;           /* 128 bytes                                  */
+/* pt_material_desc.flags: the material's colour -- Lambertian albedo
+ * (material.hpp:83-100) or Disney baseColor (disneyPrincipledBSDF.hpp) --
+ * is texture `id` (pt_scene_add_texture_*) at the hit's texture coordinates
+ * (HitRecord::texCoords) instead of p[0..2].  Other kinds refuse it.        */
+#define PT_MAT_TEXTURE(id) (0x1u | ((uint32_t)(id) << 8))
 
 /* Primitive flags.  The reference's wrappers change HitRecord::is_inside:
  * FlipFace toggles it (objectTransform.hpp:21); Translate/Rotate/Scale re-run
@@ -224,6 +229,25 @@ enum { PT_CURVE_FLAT = 0, PT_CURVE_CYLINDER = 1, PT_CURVE_RIBBON = 2 };
 pt_status pt_scene_add_curve(pt_scene *scene, const float cps[12], float width0,
                              float width1, uint32_t type, uint32_t material,
                              uint32_t flags);
+/* Textures (texture/texture.hpp:20-94), referenced by materials through
+ * PT_MAT_TEXTURE(id); ids count from 0 in the order of adding.
+ * ImageTexture(file, flip_y, isHDR) after loading: w*h RGB texels, texel
+ * (x, y) at rgb[(y*w + x)*3], y = 0 the image's bottom row (texture.cpp:
+ * 21-45); looked up bilinearly with the reference's clamp of the lower
+ * texel to w-2, h-2 (texture.cpp:111-142), so w, h >= 2.                   */
+pt_status pt_scene_add_texture_image(pt_scene *scene, const float *rgb, uint32_t w, uint32_t h,
+                                     uint32_t *out_id);
+/* CheckerTexture(color1, color2, scale) (texture.cpp:151-183): color1 where
+ * sin(u scale) sin(v scale) < 0, else color2.                              */
+pt_status pt_scene_add_texture_checker(pt_scene *scene, const float color1[3], const float color2[3],
+                                       float scale, uint32_t *out_id);
+/* Texture coordinates of mesh `mesh` (meshes numbered in adding order from
+ * 0): MeshData::uvs / uvIndices (mesh_io/meshData.hpp:6-13), uv: nuv*2
+ * floats, uvidx: 3 int32 per triangle.  A hit interpolates (1-u-v) uv0 +
+ * u uv1 + v uv2 with its barycentrics (meshBVH.cpp:133-138).  A mesh without
+ * them takes TriangleMesh's planar fallback (triangleMesh.cpp:38-51).      */
+pt_status pt_scene_set_mesh_uv(pt_scene *scene, uint32_t mesh, const float *uv, uint32_t nuv,
+                               const int32_t *uvidx, uint32_t ntris);
 /* EnvironmentLight(hdri, scale)  (light/envLight.hpp:4-92).  rgb holds
  * w*h texels, texel (x,y) at rgb[(y*w+x)*3], y=0 being the direction -Y
  * (the reference stores texels[x][y] after a vertical flip, texture.cpp:39-42). */

@@ -5,7 +5,8 @@
 //     zero when e == 0 -- the decode stb_image applies (stb_image.h:6907-6930).
 //   * PFM: "PF\n<w> <h>\n<scale>\n" then float RGB rows bottom to top,
 //     little-endian when scale < 0.
-//   * PNG: 8-bit RGB, one IDAT of stored (uncompressed) deflate blocks, CRC-32
+//   * PPM (P6): "P6 <w> <h> <maxval>" then 8-bit RGB rows top to bottom.
+//   * PNG (written only): 8-bit RGB, one IDAT of stored (uncompressed) deflate blocks, CRC-32
 //     and Adler-32 as RFC 1950/1951/2083 define them.
 // Environment maps come back in the layout pt_scene_set_env takes: texel
 // (x, y) at rgb[(y*w + x)*3], y = 0 being the bottom row of the image (the
@@ -164,6 +165,78 @@ inline Image read_env(const std::string &path) {
   if (ends(".hdr") || ends(".pic")) return read_hdr(path);
   if (ends(".pfm")) return read_pfm(path);
   throw std::runtime_error(path + ": environment maps are read from .hdr or .pfm files");
+}
+
+// Binary PPM (P6, maxval <= 255): rows top to bottom in the file, flipped so
+// that row 0 is the bottom one; channel values are returned as v / maxval.
+inline Image read_ppm(const std::string &path) {
+  const std::vector<unsigned char> b = read_file(path);
+  size_t p = 0;
+  auto token = [&]() {
+    for (;;) {
+      while (p < b.size() && std::isspace(b[p])) ++p;
+      if (p < b.size() && b[p] == '#') {
+        while (p < b.size() && b[p] != '\n') ++p;
+        continue;
+      }
+      break;
+    }
+    std::string s;
+    while (p < b.size() && !std::isspace(b[p])) s.push_back((char)b[p++]);
+    return s;
+  };
+  if (token() != "P6") throw std::runtime_error(path + ": only binary PPM (P6) is supported");
+  const long w = std::stol(token()), h = std::stol(token()), maxv = std::stol(token());
+  if (w <= 0 || h <= 0 || maxv <= 0 || maxv > 255) throw std::runtime_error(path + ": unsupported PPM header");
+  ++p;  // the single whitespace byte after maxval
+  const size_t n = (size_t)w * h * 3;
+  if (p + n > b.size()) throw std::runtime_error(path + ": truncated pixel data");
+  Image img;
+  img.width = (uint32_t)w;
+  img.height = (uint32_t)h;
+  img.rgb.resize(n);
+  for (long row = 0; row < h; ++row)
+    for (size_t i = 0; i < (size_t)w * 3; ++i)
+      img.rgb[(size_t)(h - 1 - row) * w * 3 + i] = (float)b[p + (size_t)row * w * 3 + i] / (float)maxv;
+  return img;
+}
+
+// ImageTexture(file, flip_y, isHDR) (texture.cpp:3-91): texels with row 0 at
+// the bottom of the image unless flip_y, which keeps the file's top row
+// first.  The reference reads through stb_image, which converts between its
+// 8-bit and float forms: an HDR file read with isHDR false becomes
+// (uint8)(x^(1/2.2) * 255 + 0.5) clamped to [0, 255], then / 255; an 8-bit
+// file read with isHDR true becomes (v / 255)^2.2 (stb_image.h's
+// stbi__hdr_to_ldr / stbi__ldr_to_hdr with their default gamma 2.2, scale 1).
+inline Image read_texture(const std::string &path, bool flip_y, bool is_hdr) {
+  auto ends = [&](const char *e) {
+    size_t n = std::strlen(e);
+    if (path.size() < n) return false;
+    for (size_t i = 0; i < n; ++i)
+      if (std::tolower((unsigned char)path[path.size() - n + i]) != e[i]) return false;
+    return true;
+  };
+  Image img;
+  if (ends(".hdr") || ends(".pic") || ends(".pfm")) {
+    img = ends(".pfm") ? read_pfm(path) : read_hdr(path);
+    if (!is_hdr)
+      for (float &x : img.rgb) {
+        float z = (float)std::pow((double)x, (double)(1.0f / 2.2f)) * 255.f + 0.5f;
+        z = z < 0.f ? 0.f : (z > 255.f ? 255.f : z);
+        x = (float)(int)z / 255.0f;
+      }
+  } else if (ends(".ppm")) {
+    img = read_ppm(path);
+    if (is_hdr)
+      for (float &x : img.rgb) x = (float)std::pow((double)x, (double)2.2f);
+  } else {
+    throw std::runtime_error(path + ": image textures are read from .hdr, .pfm or .ppm files");
+  }
+  if (flip_y)
+    for (uint32_t y = 0; y < img.height / 2; ++y)
+      std::swap_ranges(img.rgb.begin() + (size_t)y * img.width * 3, img.rgb.begin() + (size_t)(y + 1) * img.width * 3,
+                       img.rgb.begin() + (size_t)(img.height - 1 - y) * img.width * 3);
+  return img;
 }
 
 inline void write_pfm(const std::string &path, const float *rgb, uint32_t w, uint32_t h) {

@@ -29,12 +29,13 @@
 //     follows the transformed geometry.
 //   * Materials are lowered once each, in order of first use.
 //
-// Supported: Lambertian and DiffuseEmitter (solid colours), Conductor,
+// Supported: Lambertian (colour or texture), DiffuseEmitter (colour), Conductor,
 // MFDielectric, ThinDielectric, Phong, DisneyPrincipledBSDF, HairMarschner,
-// KajiyaKay; Sphere, AARectangle<Axis> (XY/XZ/YZRectangle), Box,
+// KajiyaKay; ConstantTexture, ImageTexture, CheckerTexture (on Lambertian and
+// the Disney base colour); Sphere, AARectangle<Axis> (XY/XZ/YZRectangle), Box,
 // TriangleMesh (with MeshData normals), loadOBJMesh, ObjectList, FlipFace,
 // Translate, Rotate<Axis>, Scale; PerspectiveCamera; EnvironmentLight from
-// .hdr or .pfm files.  Image textures, media and curves are not.
+// .hdr or .pfm files.  NoiseTexture and textured emitters are not.
 #pragma once
 
 #include <array>
@@ -47,6 +48,8 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <variant>
 #include <vector>
 
@@ -122,6 +125,17 @@ struct SceneSink {
     (void)cps, (void)w0, (void)w1, (void)type, (void)mat, (void)flags;
     throw std::runtime_error("this scene sink does not take curves");
   }
+  // Textures return their id; mesh_uv gives the last mesh added its texture
+  // coordinates (2 floats per uv, 3 uv indices per triangle).
+  virtual uint32_t texture_image(const std::vector<float> &rgb, uint32_t w, uint32_t h) {
+    (void)rgb, (void)w, (void)h;
+    throw std::runtime_error("this scene sink does not take textures");
+  }
+  virtual uint32_t texture_checker(const float c1[3], const float c2[3], float scale) {
+    (void)c1, (void)c2, (void)scale;
+    throw std::runtime_error("this scene sink does not take textures");
+  }
+  virtual void mesh_uv(const std::vector<float> &uv, const std::vector<int32_t> &uvidx) { (void)uv, (void)uvidx; }
 };
 
 inline void check(pt_status st, const char *what) {
@@ -130,6 +144,7 @@ inline void check(pt_status st, const char *what) {
 
 struct AbiSink : SceneSink {
   pt_scene *s;
+  uint32_t meshes = 0;
   explicit AbiSink(pt_scene *scene) : s(scene) {}
   uint32_t material(const pt_material_desc &d) override {
     uint32_t id = 0;
@@ -145,6 +160,23 @@ struct AbiSink : SceneSink {
             "pt_scene_add_mesh_normals");
     else
       check(pt_scene_add_mesh(s, pos.data(), nv, idx.data(), nt, mat, flags), "pt_scene_add_mesh");
+    ++meshes;
+  }
+  uint32_t texture_image(const std::vector<float> &rgb, uint32_t w, uint32_t h) override {
+    uint32_t id = 0;
+    check(pt_scene_add_texture_image(s, rgb.data(), w, h, &id), "pt_scene_add_texture_image");
+    return id;
+  }
+  uint32_t texture_checker(const float c1[3], const float c2[3], float scale) override {
+    uint32_t id = 0;
+    check(pt_scene_add_texture_checker(s, c1, c2, scale, &id), "pt_scene_add_texture_checker");
+    return id;
+  }
+  void mesh_uv(const std::vector<float> &uv, const std::vector<int32_t> &uvidx) override {
+    if (meshes == 0) throw std::runtime_error("mesh_uv before any mesh");
+    check(pt_scene_set_mesh_uv(s, meshes - 1, uv.data(), (uint32_t)(uv.size() / 2), uvidx.data(),
+                               (uint32_t)(uvidx.size() / 3)),
+          "pt_scene_set_mesh_uv");
   }
   void sphere(const float c[3], float r, uint32_t mat, uint32_t flags) override {
     check(pt_scene_add_sphere(s, c, r, mat, flags), "pt_scene_add_sphere");
@@ -212,28 +244,85 @@ struct Lowering;
 }  // namespace ptamd
 
 // ------------------------------------------------------------------ textures
+// texture/texture.hpp:20-94.  A texture either is one colour everywhere
+// (ConstantTexture) or lowers to a pt_scene texture that materials reference
+// by id.  LERP_MODE is BILINEAR throughout, as every caller in the
+// reference leaves it.
 class Texture2D {
  public:
   virtual ~Texture2D() = default;
-  virtual gl::vec3 constant_color() const {
-    throw std::runtime_error("only constant-colour textures are supported by the GPU materials");
-  }
+  virtual std::optional<gl::vec3> constant() const { return std::nullopt; }
+  virtual uint32_t lower(ptamd::SceneSink &sink) const = 0;
 };
 class ConstantTexture : public Texture2D {
  public:
+  ConstantTexture() = default;
   explicit ConstantTexture(const gl::vec3 &c) : c(c) {}
-  gl::vec3 constant_color() const override { return c; }
+  std::optional<gl::vec3> constant() const override { return c; }
+  uint32_t lower(ptamd::SceneSink &sink) const override {
+    const float rgb[3] = {c[0], c[1], c[2]};
+    return sink.texture_checker(rgb, rgb, 1.f);
+  }
   gl::vec3 c;
 };
 using SolidColor = ConstantTexture;
+// ImageTexture(file, flip_y, isHDR) (texture.cpp:3-91).  Files are read as
+// image_io::read_texture reads them (.hdr/.pic, .pfm, .ppm); the in-memory
+// form takes texels in the pt_scene_add_texture_image layout.
+class ImageTexture : public Texture2D {
+ public:
+  ImageTexture(const std::string &filename, bool flip_y = false, bool isHDR = false) {
+    ptamd::image_io::Image img = ptamd::image_io::read_texture(filename, flip_y, isHDR);
+    width = img.width;
+    height = img.height;
+    rgb = std::move(img.rgb);
+  }
+  ImageTexture(std::vector<float> texels, uint32_t w, uint32_t h) : rgb(std::move(texels)), width(w), height(h) {}
+  uint32_t lower(ptamd::SceneSink &sink) const override { return sink.texture_image(rgb, width, height); }
+  std::vector<float> rgb;
+  uint32_t width = 0, height = 0;
+};
+// texture.cpp:151-183: color1 where sin(u scale) sin(v scale) < 0.
+class CheckerTexture : public Texture2D {
+ public:
+  explicit CheckerTexture(float scale = 10.f) : c1(0.f), c2(1.f), scale(scale) {}
+  CheckerTexture(const gl::vec3 &color1, const gl::vec3 &color2, float scale = 10.f)
+      : c1(color1), c2(color2), scale(scale) {}
+  uint32_t lower(ptamd::SceneSink &sink) const override {
+    const float a[3] = {c1[0], c1[1], c1[2]}, b[3] = {c2[0], c2[1], c2[2]};
+    return sink.texture_checker(a, b, scale);
+  }
+  gl::vec3 c1, c2;
+  float scale;
+};
+// texture.cpp:185-193 (fractal Perlin noise): declared so that scenes that
+// name it compile; it has no GPU lowering and a scene using it is refused.
+class NoiseTexture : public Texture2D {
+ public:
+  NoiseTexture() = default;
+  NoiseTexture(float scale, int fractal = 1) : scale(scale), fractal(fractal) {}
+  uint32_t lower(ptamd::SceneSink &) const override {
+    throw std::runtime_error("NoiseTexture has no GPU lowering: use ImageTexture, CheckerTexture or a colour");
+  }
+  float scale = 1.f;
+  int fractal = 1;
+};
 using ColorVariant = std::variant<gl::vec3, std::shared_ptr<Texture2D>>;
 
 namespace ptamd {
-inline gl::vec3 color_of(const ColorVariant &v) {
-  if (std::holds_alternative<gl::vec3>(v)) return std::get<gl::vec3>(v);
+// A colour argument as (constant colour, texture): the texture is null when
+// the colour is the same everywhere.
+inline std::pair<gl::vec3, std::shared_ptr<Texture2D>> color_or_texture(const ColorVariant &v) {
+  if (std::holds_alternative<gl::vec3>(v)) return {std::get<gl::vec3>(v), nullptr};
   const auto &t = std::get<std::shared_ptr<Texture2D>>(v);
   if (!t) throw std::runtime_error("null texture");
-  return t->constant_color();
+  if (auto c = t->constant()) return {*c, nullptr};
+  return {gl::vec3(0.f), t};
+}
+inline gl::vec3 color_of(const ColorVariant &v, const char *who) {
+  auto ct = color_or_texture(v);
+  if (ct.second) throw std::runtime_error(std::string(who) + " takes constant colours only");
+  return ct.first;
 }
 }  // namespace ptamd
 
@@ -243,6 +332,8 @@ class Material {
   virtual ~Material() = default;
   virtual pt_material_desc desc() const = 0;
   virtual bool is_emitter() const { return false; }
+  // the texture that replaces the material's colour, if any
+  virtual std::shared_ptr<Texture2D> texture() const { return nullptr; }
 
  protected:
   static pt_material_desc make(uint32_t kind, std::initializer_list<float> p) {
@@ -257,14 +348,16 @@ class Material {
 // material.hpp:79-137
 class Lambertian : public Material {
  public:
-  Lambertian(const ColorVariant &a) : albedo(ptamd::color_of(a)) {}
+  Lambertian(const ColorVariant &a) { std::tie(albedo, tex) = ptamd::color_or_texture(a); }
   pt_material_desc desc() const override { return make(PT_MAT_LAMBERTIAN, {albedo[0], albedo[1], albedo[2]}); }
+  std::shared_ptr<Texture2D> texture() const override { return tex; }
   gl::vec3 albedo;
+  std::shared_ptr<Texture2D> tex;
 };
 // material.hpp:467-506
 class DiffuseEmitter : public Material {
  public:
-  DiffuseEmitter(const ColorVariant &a, float intensity = 1.0f) : color(ptamd::color_of(a)), intensity(intensity) {}
+  DiffuseEmitter(const ColorVariant &a, float intensity = 1.0f) : color(ptamd::color_of(a, "DiffuseEmitter")), intensity(intensity) {}
   pt_material_desc desc() const override {
     return make(PT_MAT_EMITTER, {color[0], color[1], color[2], intensity});
   }
@@ -318,13 +411,26 @@ class DisneyPrincipledBSDF : public Material {
                        float sheen_tint, float clearcoat, float clearcoat_gloss, float eta)
       : p{baseColor[0], baseColor[1], baseColor[2], specular_transmission, metallic, subsurface, specular,
           roughness, specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss, eta} {}
+  // beyond the reference's constructor: a textured base colour (as its
+  // Disney lobes take one, disneyDiffuse.hpp:11-19, disneyMetal.hpp:12-21)
+  DisneyPrincipledBSDF(std::shared_ptr<Texture2D> baseColor, float specular_transmission, float metallic,
+                       float subsurface, float specular, float roughness, float specular_tint, float anisotropic,
+                       float sheen, float sheen_tint, float clearcoat, float clearcoat_gloss, float eta)
+      : p{0.f, 0.f, 0.f, specular_transmission, metallic, subsurface, specular,
+          roughness, specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss, eta} {
+    gl::vec3 c;
+    std::tie(c, tex) = ptamd::color_or_texture(baseColor);
+    for (int i = 0; i < 3; ++i) p[i] = c[i];
+  }
   pt_material_desc desc() const override {
     pt_material_desc d{};
     d.kind = PT_MAT_DISNEY;
     for (int i = 0; i < 15; ++i) d.p[i] = p[i];
     return d;
   }
+  std::shared_ptr<Texture2D> texture() const override { return tex; }
   float p[15];
+  std::shared_ptr<Texture2D> tex;
 };
 // hairMarsch.hpp:7-9
 class HairMarschner : public Material {
@@ -356,12 +462,22 @@ namespace ptamd {
 struct Lowering {
   SceneSink &sink;
   std::map<const Material *, uint32_t> ids;
+  std::map<const Texture2D *, uint32_t> tex_ids;
   explicit Lowering(SceneSink &s) : sink(s) {}
+  uint32_t tex(const std::shared_ptr<Texture2D> &t) {
+    auto it = tex_ids.find(t.get());
+    if (it != tex_ids.end()) return it->second;
+    const uint32_t id = t->lower(sink);
+    tex_ids[t.get()] = id;
+    return id;
+  }
   uint32_t mat(const std::shared_ptr<Material> &m) {
     if (!m) throw std::runtime_error("object without a material");
     auto it = ids.find(m.get());
     if (it != ids.end()) return it->second;
-    uint32_t id = sink.material(m->desc());
+    pt_material_desc d = m->desc();
+    if (auto t = m->texture()) d.flags = PT_MAT_TEXTURE(tex(t));
+    uint32_t id = sink.material(d);
     ids[m.get()] = id;
     return id;
   }
@@ -464,6 +580,9 @@ class AARectangle : public Hittable {
       }
     std::vector<int32_t> idx = {0, 1, 3, 0, 3, 2};
     L.sink.mesh(pos, idx, nullptr, nullptr, mat, flags | (uint32_t)PT_PRIM_NOT_LIGHT);
+    // the rect's own texture coordinates (primitive.cpp:55-57), affine over
+    // the corners, so that interpolating them gives the rect's (u, v)
+    if (material && material->texture()) L.sink.mesh_uv({0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 1.f, 1.f}, idx);
   }
   float _d0_min, _d0_max, _d1_min, _d1_max, _k;
   std::shared_ptr<Material> material;
@@ -509,16 +628,73 @@ class TriangleMesh : public Hittable {
     const uint32_t flags = C.flags() | (uint32_t)PT_PRIM_NOT_LIGHT;
     if (!smooth) {
       L.sink.mesh(pos, idx, nullptr, nullptr, mat, flags);
+    } else {
+      std::vector<float> nrm;
+      for (const auto &n : mesh.normals) {
+        gl::vec3 w = X.dir(n);
+        nrm.insert(nrm.end(), {w[0], w[1], w[2]});
+      }
+      std::vector<int32_t> nidx;
+      for (const auto &f : mesh.normalIndices) nidx.insert(nidx.end(), {f[0], f[1], f[2]});
+      L.sink.mesh(pos, idx, &nrm, &nidx, mat, flags);
+    }
+    // texture coordinates when every face has three valid uv indices
+    // (meshBVH.cpp:133-138 interpolates them); else the planar fallback
+    bool has_uv = !mesh.uvs.empty() && mesh.uvIndices.size() == mesh.indices.size();
+    if (has_uv)
+      for (const auto &f : mesh.uvIndices)
+        for (int c = 0; c < 3; ++c)
+          if (f[c] < 0 || (size_t)f[c] >= mesh.uvs.size()) has_uv = false;
+    if (!has_uv) {
+      if ((!X.identity_rotation || X.scale != 1.f) && material && material->texture()) planar_uv(L);
       return;
     }
-    std::vector<float> nrm;
-    for (const auto &n : mesh.normals) {
-      gl::vec3 w = X.dir(n);
-      nrm.insert(nrm.end(), {w[0], w[1], w[2]});
+    std::vector<float> uv;
+    uv.reserve(mesh.uvs.size() * 2);
+    for (const auto &t : mesh.uvs) uv.insert(uv.end(), {t[0], t[1]});
+    std::vector<int32_t> uvidx;
+    uvidx.reserve(mesh.uvIndices.size() * 3);
+    for (const auto &f : mesh.uvIndices) uvidx.insert(uvidx.end(), {f[0], f[1], f[2]});
+    L.sink.mesh_uv(uv, uvidx);
+  }
+  // The planar fallback is taken in object space, over the object's box
+  // (triangleMesh.cpp:10-18, 38-51).  Without rotation or scale the
+  // library's world-space fallback is the same map; under them, the object
+  // space coordinates of each face's corners are passed as the mesh's uvs:
+  // the fallback is affine over a face whose normal picks the axes, so
+  // interpolating them reproduces it (the face normal stands in for the
+  // interpolated shading normal of a smooth mesh).
+  void planar_uv(ptamd::Lowering &L) const {
+    gl::vec3 mn(INFINITY), mx(-INFINITY);
+    for (const auto &v : mesh.positions)
+      for (int c = 0; c < 3; ++c) {
+        mn[c] = std::fmin(mn[c], v[c]);
+        mx[c] = std::fmax(mx[c], v[c]);
+      }
+    for (int c = 0; c < 3; ++c) {
+      mn[c] -= 1e-3f;  // gl::epsilon (constants.hpp:7)
+      mx[c] += 1e-3f;
     }
-    std::vector<int32_t> nidx;
-    for (const auto &f : mesh.normalIndices) nidx.insert(nidx.end(), {f[0], f[1], f[2]});
-    L.sink.mesh(pos, idx, &nrm, &nidx, mat, flags);
+    std::vector<float> uv;
+    std::vector<int32_t> uvidx;
+    uv.reserve(mesh.indices.size() * 6);
+    uvidx.reserve(mesh.indices.size() * 3);
+    for (const auto &f : mesh.indices) {
+      const gl::vec3 &p0 = mesh.positions[f[0]], &p1 = mesh.positions[f[1]], &p2 = mesh.positions[f[2]];
+      gl::vec3 e1 = p1 - p0, e2 = p2 - p0, n;
+      n[0] = std::fabs(e1[1] * e2[2] - e1[2] * e2[1]);
+      n[1] = std::fabs(e1[2] * e2[0] - e1[0] * e2[2]);
+      n[2] = std::fabs(e1[0] * e2[1] - e1[1] * e2[0]);
+      const int au = (n[0] > n[1] && n[0] > n[2]) ? 1 : 0;
+      const int av = au == 1 ? 2 : (n[1] > n[2] ? 2 : 1);
+      for (int c = 0; c < 3; ++c) {
+        const gl::vec3 &q = mesh.positions[f[c]];
+        uvidx.push_back((int32_t)(uv.size() / 2));
+        uv.push_back((q[au] - mn[au]) / (mx[au] - mn[au]));
+        uv.push_back((q[av] - mn[av]) / (mx[av] - mn[av]));
+      }
+    }
+    L.sink.mesh_uv(uv, uvidx);
   }
   MeshData mesh;
   std::shared_ptr<Material> material;

@@ -97,7 +97,7 @@ static int lambert_scatter(const pt_material_desc *m, const or_hit *h, float u0,
                            or_srec *s) {
   onb_t b = onb_make(h->n);
   s->flags = OR_DIFFUSE | OR_REFL;
-  s->att = vscale(p3(m, 0), (float)(1.0f / M_PI));
+  s->att = vscale(h->has_tex ? h->tex : p3(m, 0), (float)(1.0f / M_PI));
   v3 d = onb_to_world(&b, cosine_sample_hemisphere(u0, u1));
   s->wi = vnorm(d);
   s->pdf = (float)(fmaxf(vdot(vnorm(s->wi), b.w), 0.f) / M_PI);
@@ -448,7 +448,7 @@ int or_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, float uc, f
 
 v3 or_bsdf_f(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h) {
   switch (m->kind) {
-  case PT_MAT_LAMBERTIAN: return vscale(p3(m, 0), (float)(1.0f / M_PI));
+  case PT_MAT_LAMBERTIAN: return vscale(h->has_tex ? h->tex : p3(m, 0), (float)(1.0f / M_PI));
   case PT_MAT_CONDUCTOR: return conductor_f(m, wo, wi, h);
   case PT_MAT_MFDIELECTRIC: return mfdiel_f(m, wo, wi, h);
   case PT_MAT_PHONG: return phong_f(m, wo, wi, h);

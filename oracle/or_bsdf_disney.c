@@ -58,9 +58,9 @@ typedef struct {
   float cc_alpha;
 } disney_t;
 
-static disney_t disney_params(const pt_material_desc *m) {
+static disney_t disney_params(const pt_material_desc *m, const or_hit *h) {
   disney_t d;
-  d.base = V3(m->p[0], m->p[1], m->p[2]);
+  d.base = h->has_tex ? h->tex : V3(m->p[0], m->p[1], m->p[2]);
   d.spec_trans = m->p[3]; d.metallic = m->p[4]; d.subsurface = m->p[5]; d.specular = m->p[6];
   d.roughness = m->p[7]; d.specular_tint = m->p[8]; d.anisotropic = m->p[9]; d.sheen = m->p[10];
   d.sheen_tint = m->p[11]; d.clearcoat = m->p[12]; d.clearcoat_gloss = m->p[13]; d.eta = m->p[14];
@@ -293,7 +293,7 @@ static v3 dsheen_f(const disney_t *d, v3 wo_w, v3 wi_w, const or_hit *h) {
 
 /* ---- DisneyPrincipledBSDF (disneyPrincipledBSDF.hpp:72-99, .cpp:3-114) ---- */
 v3 or_disney_f(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h) {
-  disney_t d = disney_params(m);
+  disney_t d = disney_params(m, h);
   if (h->inside) return dglass_f(&d, wo, wi, h);
   v3 r = vs(0.f);
   r = vadd(r, vscale(ddiffuse_f(&d, wo, wi, h), d.dw));
@@ -305,7 +305,7 @@ v3 or_disney_f(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h) {
 }
 
 float or_disney_pdf(const pt_material_desc *m, v3 wo, v3 wi, const or_hit *h) {
-  disney_t d = disney_params(m);
+  disney_t d = disney_params(m, h);
   if (h->inside) return dglass_pdf(&d, wo, wi, h);
   float sum = d.dw + d.mw + d.gw + d.cw;
   float p = 0.f;
@@ -338,7 +338,7 @@ static int sample_discrete(const float *w, int n, float u, float *pmf, float *u_
 
 int or_disney_scatter(const pt_material_desc *m, v3 ray_d, const or_hit *h, float uc, float u0,
                       float u1, or_srec *s) {
-  disney_t d = disney_params(m);
+  disney_t d = disney_params(m, h);
   v3 wo = vneg(vnorm(ray_d));
   if (h->inside) return dglass_scatter(&d, wo, h, uc, u0, u1, s);
   float w[4] = {d.dw, d.mw, d.gw, d.cw};

@@ -34,7 +34,11 @@ void or_scene_destroy(or_scene *s) {
   for (int i = 0; i < s->n_meshes; ++i) {
     free(s->meshes[i].nodes);
     free(s->meshes[i].leaf_ids);
+    free(s->meshes[i].uv);
+    free(s->meshes[i].uvidx);
   }
+  for (int i = 0; i < s->n_tex; ++i) free(s->tex[i].rgb);
+  free(s->tex);
   free(s->meshes);
   free(s->spheres);
   free(s->rects);
@@ -90,7 +94,85 @@ int or_scene_add_mesh_normals(or_scene *s, const float *pos, uint32_t nv, const 
     }
     tr->mesh = s->n_meshes;
   }
+  /* TriangleMesh::TriangleMesh: the box of all positions, -/+ gl::epsilon
+   * (triangleMesh.cpp:10-18, constants.hpp:7) */
+  v3 lo = vs(INFINITY), hi = vs(-INFINITY);
+  for (uint32_t v = 0; v < nv; ++v) {
+    v3 p = V3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
+    lo = V3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
+    hi = V3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+  }
+  m->box_lo = vsub(lo, vs(1e-3f));
+  m->box_hi = vadd(hi, vs(1e-3f));
   return s->n_meshes++;
+}
+
+/* MeshData::uvs / uvIndices of mesh `mesh` (mesh_io/meshData.hpp:6-13) */
+int or_scene_set_mesh_uv(or_scene *s, uint32_t mesh, const float *uv, uint32_t nuv, const int32_t *uvidx,
+                         uint32_t nt) {
+  if ((int)mesh >= s->n_meshes) return -1;
+  or_mesh *m = &s->meshes[mesh];
+  if ((int)nt != m->n_tris) return -1;
+  for (size_t k = 0; k < 3 * (size_t)nt; ++k)
+    if (uvidx[k] < 0 || (uint32_t)uvidx[k] >= nuv) return -1;
+  free(m->uv);
+  free(m->uvidx);
+  m->uv = (float *)malloc(sizeof(float) * 2 * (nuv ? nuv : 1));
+  m->uvidx = (int32_t *)malloc(sizeof(int32_t) * 3 * (nt ? nt : 1));
+  memcpy(m->uv, uv, sizeof(float) * 2 * nuv);
+  memcpy(m->uvidx, uvidx, sizeof(int32_t) * 3 * nt);
+  return 0;
+}
+
+/* ImageTexture(file, ...) after loading (texture.cpp:3-95) */
+int or_scene_add_texture_image(or_scene *s, const float *rgb, uint32_t w, uint32_t h) {
+  if (w < 2 || h < 2) return -1;
+  GROW(s->tex, s->n_tex, s->cap_tex, or_tex);
+  or_tex *t = &s->tex[s->n_tex];
+  memset(t, 0, sizeof(*t));
+  t->kind = 0;
+  t->w = (int)w;
+  t->h = (int)h;
+  t->rgb = (float *)malloc(sizeof(float) * 3 * (size_t)w * h);
+  memcpy(t->rgb, rgb, sizeof(float) * 3 * (size_t)w * h);
+  return s->n_tex++;
+}
+
+/* CheckerTexture(color1, color2, scale) (texture.cpp:151-156) */
+int or_scene_add_texture_checker(or_scene *s, const float *c1, const float *c2, float scale) {
+  GROW(s->tex, s->n_tex, s->cap_tex, or_tex);
+  or_tex *t = &s->tex[s->n_tex];
+  memset(t, 0, sizeof(*t));
+  t->kind = 1;
+  t->c1 = V3(c1[0], c1[1], c1[2]);
+  t->c2 = V3(c2[0], c2[1], c2[2]);
+  t->scale = scale;
+  return s->n_tex++;
+}
+
+/* Texture2D::getTexelColor (texture.cpp:99-183).  ImageTexture, bilinear:
+ * x = (uint)(u*w) (negative u converts to 0, as a saturating conversion
+ * does), the fraction taken before x is clamped to w-2, likewise y.
+ * CheckerTexture: color1 where sin(u scale) sin(v scale) < 0, else color2. */
+v3 or_texture(const or_scene *s, uint32_t id, float u, float v) {
+  const or_tex *t = &s->tex[id];
+  if (t->kind == 1) {
+    float sv = sinf(u * t->scale) * sinf(v * t->scale);
+    return sv < 0.0f ? t->c1 : t->c2;
+  }
+  int W = t->w, H = t->h;
+  float fx = u * (float)W, fy = v * (float)H;
+  unsigned x = fx > 0.f ? (unsigned)fx : 0u, y = fy > 0.f ? (unsigned)fy : 0u;
+  float x1 = fx - (float)x, y1 = fy - (float)y;
+  if (x > (unsigned)(W - 2)) x = (unsigned)(W - 2);
+  if (y > (unsigned)(H - 2)) y = (unsigned)(H - 2);
+  float x2 = (float)(1.0 - x1), y2 = (float)(1.0 - y1);
+#define TX(xx, yy) V3(t->rgb[3 * ((size_t)(yy) * W + (xx))], t->rgb[3 * ((size_t)(yy) * W + (xx)) + 1], t->rgb[3 * ((size_t)(yy) * W + (xx)) + 2])
+  v3 c1 = TX(x, y), c2 = TX(x + 1, y), c3 = TX(x, y + 1), c4 = TX(x + 1, y + 1);
+#undef TX
+  v3 c12 = vadd(vscale(c1, x2), vscale(c2, x1));
+  v3 c34 = vadd(vscale(c3, x2), vscale(c4, x1));
+  return vadd(vscale(c12, y2), vscale(c34, y1));
 }
 
 int or_scene_add_sphere(or_scene *s, const float *c, float r, uint32_t mat, uint32_t flags) {
@@ -523,6 +605,14 @@ static int curve_hit(const or_curve *c, v3 o, v3 d, float tmin, float tmax, floa
   return found;
 }
 
+/* A textured material (pt_material_desc.flags bit 0) takes its colour from
+ * texture flags >> 8 at the hit's texture coordinates. */
+static void hit_texture(const or_scene *s, or_hit *h) {
+  const uint32_t fl = s->mats[h->mat].flags;
+  h->has_tex = (fl & 1u) != 0;
+  h->tex = h->has_tex ? or_texture(s, fl >> 8, h->u, h->v) : vs(0.f);
+}
+
 int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, int any,
                        or_hit *h, uint64_t *cnt) {
   float best = tmax;
@@ -570,6 +660,7 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
   h->t = best;
   h->ng_light = vs(0.f);
   h->tangent = vs(0.f);
+  h->u = h->v = 0.f;  /* Curve sets no texCoords */
   if (kind == 3) {
     const or_curve *c = &s->curves[idx];
     h->p = cp;
@@ -577,6 +668,7 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
     h->tangent = ctan;
     h->mat = c->mat;
     h->prim = s->n_tris + s->n_spheres + s->n_rects + 8 * idx + cseg;
+    hit_texture(s, h);
     return 1;
   }
   if (kind == 0) {
@@ -601,6 +693,31 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
     set_normal(h, d, ns, m->flags);
     h->mat = m->mat;
     h->prim = idx;
+    if (m->uv) {
+      /* MeshBVHNode::hitTriangle (meshBVH.cpp:133-138): the hit's
+       * barycentrics, then (1-u-v) uv0 + u uv1 + v uv2 */
+      v3 P = vcross(d, e2);
+      float inv = 1.0f / vdot(e1, P);
+      v3 T = vsub(o, tr->p[0]);
+      float bu = vdot(T, P) * inv;
+      v3 Q = vcross(T, e1);
+      float bv = vdot(d, Q) * inv;
+      const int32_t *ui = &m->uvidx[3 * (size_t)(idx - m->first_tri)];
+      const float *u0 = &m->uv[2 * (size_t)ui[0]], *u1 = &m->uv[2 * (size_t)ui[1]], *u2 = &m->uv[2 * (size_t)ui[2]];
+      float w0 = 1.f - bu - bv;
+      h->u = w0 * u0[0] + bu * u1[0] + bv * u2[0];
+      h->v = w0 * u0[1] + bu * u1[1] + bv * u2[1];
+    } else {
+      /* TriangleMesh's planar fallback (triangleMesh.cpp:38-51): the axes
+       * the normal points least along, scaled to the mesh box, clamped */
+      v3 n = V3(fabsf(h->n.x), fabsf(h->n.y), fabsf(h->n.z));
+      int au = (n.x > n.y && n.x > n.z) ? 1 : 0;
+      int av = (au == 1) ? 2 : (n.y > n.z ? 2 : 1);
+      float uu = (vget(h->p, au) - vget(m->box_lo, au)) / (vget(m->box_hi, au) - vget(m->box_lo, au));
+      float vv = (vget(h->p, av) - vget(m->box_lo, av)) / (vget(m->box_hi, av) - vget(m->box_lo, av));
+      h->u = fminf(fmaxf(uu, 0.f), 1.f);
+      h->v = fminf(fmaxf(vv, 0.f), 1.f);
+    }
   } else if (kind == 1) {
     const or_sphere *sp = &s->spheres[idx];
     v3 dn = vnorm(d);
@@ -610,6 +727,12 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
     h->ng_light = nn;
     h->mat = sp->mat;
     h->prim = s->n_tris + idx;
+    /* Sphere::intersect's texCoords (primitive.hpp:157-163) */
+    v3 q = vnorm(vsub(h->p, sp->c));
+    float phi = atan2f(q.z, q.x), theta = asinf(q.y);
+    float u = (float)(1 - (phi + M_PI) / (2 * M_PI));
+    h->v = (float)((theta + M_PI / 2) / M_PI);
+    h->u = fmodf(u + 0.75f, 1.0f);
   } else {
     const or_rect *r = &s->rects[idx];
     v3 dn = vnorm(d);
@@ -619,8 +742,40 @@ int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, in
     h->ng_light = (r->light >= 0) ? s->lights[r->light].n : nn;
     h->mat = r->mat;
     h->prim = s->n_tris + s->n_spheres + idx;
+    /* AARectangle's texCoords (primitive.cpp:55-57, 86-88, 117-119) */
+    float d0 = r->axis == 0 ? h->p.y : h->p.x, d1 = r->axis == 2 ? h->p.y : h->p.z;
+    h->u = (d0 - r->d0min) / (r->d0max - r->d0min);
+    h->v = (d1 - r->d1min) / (r->d1max - r->d1min);
   }
+  hit_texture(s, h);
   return 1;
+}
+
+/* Texture `id` at n (u, v) pairs: out n*3 (tests). */
+int or_texture_batch(const or_scene *s, uint32_t id, const float *uv, size_t n, float *out) {
+  if ((int)id >= s->n_tex) return -1;
+  for (size_t i = 0; i < n; ++i) {
+    v3 c = or_texture(s, id, uv[2 * i], uv[2 * i + 1]);
+    out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+  }
+  return 0;
+}
+
+/* HitRecord::texCoords of the closest hits of n rays (n*8 floats): uv n*2,
+ * (-1, -1) on a miss (tests). */
+int or_hit_uv_batch(const or_scene *s, const float *rays, size_t n, float *uv) {
+  for (size_t i = 0; i < n; ++i) {
+    const float *r = &rays[8 * i];
+    or_hit h;
+    memset(&h, 0, sizeof(h));
+    if (or_scene_intersect(s, V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]), r[6], r[7], 0, &h, NULL)) {
+      uv[2 * i] = h.u;
+      uv[2 * i + 1] = h.v;
+    } else {
+      uv[2 * i] = uv[2 * i + 1] = -1.f;
+    }
+  }
+  return 0;
 }
 
 int or_prim_light(const or_scene *s, int prim) {

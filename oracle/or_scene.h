@@ -19,7 +19,19 @@ typedef struct {
   int n_nodes, cap_nodes;
   int *leaf_ids;
   int n_leaf, cap_leaf;
+  v3 box_lo, box_hi;  /* TriangleMesh::meshAABB (triangleMesh.cpp:10-18)   */
+  float *uv;          /* MeshData::uvs (2 floats each), NULL: planar UVs   */
+  int32_t *uvidx;     /* MeshData::uvIndices, 3 per triangle               */
 } or_mesh;
+/* ImageTexture texels (texel (x, y) at rgb[3 (y w + x)], y = 0 the bottom
+ * row) or a CheckerTexture (texture/texture.hpp:49-94). */
+typedef struct {
+  int kind; /* 0 image, 1 checker */
+  int w, h;
+  float *rgb;
+  v3 c1, c2;
+  float scale;
+} or_tex;
 typedef struct { v3 c; float r; uint32_t mat, flags, seq; int light; } or_sphere;
 typedef struct {
   int axis;
@@ -58,6 +70,8 @@ struct or_scene {
   int n_rects, cap_rects;
   or_curve *curves;
   int n_curves, cap_curves;
+  or_tex *tex;
+  int n_tex, cap_tex;
   or_light *lights;
   int n_lights;
   uint32_t seq;

@@ -65,6 +65,9 @@ typedef struct {
   v3 p, n;      /* n opposes the incoming ray (HitRecord::set_normal) */
   v3 ng_light;  /* un-flipped geometric normal (for one-sided emitters) */
   v3 tangent;   /* HitRecord::hair_tangent (curves), else 0             */
+  float u, v;   /* HitRecord::texCoords                                 */
+  v3 tex;       /* a textured material's colour at (u, v)               */
+  int has_tex;  /* the material is textured: tex replaces its colour    */
   int inside;   /* HitRecord::is_inside = front face                   */
   int prim;
   uint32_t mat;
@@ -103,6 +106,13 @@ int or_scene_add_curve(or_scene *s, const float *cps, float w0, float w1, uint32
 int or_scene_set_camera(or_scene *s, const pt_camera_desc *c);
 /* HomogeneousMedium(sigma_a, sigma_s, sigma_scale, Le, Le_scale, g); sigma_a NULL removes it. */
 int or_scene_set_medium(or_scene *s, const float *sigma_a, const float *sigma_s, float sigma_scale, float g);
+int or_scene_add_texture_image(or_scene *s, const float *rgb, uint32_t w, uint32_t h);
+int or_scene_add_texture_checker(or_scene *s, const float *c1, const float *c2, float scale);
+int or_scene_set_mesh_uv(or_scene *s, uint32_t mesh, const float *uv, uint32_t nuv, const int32_t *uvidx,
+                         uint32_t nt);
+v3 or_texture(const or_scene *s, uint32_t id, float u, float v);
+int or_texture_batch(const or_scene *s, uint32_t id, const float *uv, size_t n, float *out);
+int or_hit_uv_batch(const or_scene *s, const float *rays, size_t n, float *uv);
 int or_scene_commit(or_scene *s);
 int or_scene_num_lights(const or_scene *s);
 int or_surface_batch(const or_scene *s, const float *rays, size_t n, float *nrm, int32_t *inside);

@@ -50,6 +50,11 @@ def _load():
         "or_scene_set_camera": (ctypes.c_int, [_P, _P]),
         "or_scene_set_medium": (ctypes.c_int, [_P, _P, _P, f32, f32]),
         "or_scene_add_curve": (ctypes.c_int, [_P, _P, f32, f32, u32, u32]),
+        "or_scene_add_texture_image": (ctypes.c_int, [_P, _P, u32, u32]),
+        "or_scene_add_texture_checker": (ctypes.c_int, [_P, _P, _P, f32]),
+        "or_scene_set_mesh_uv": (ctypes.c_int, [_P, u32, _P, u32, _P, u32]),
+        "or_texture_batch": (ctypes.c_int, [_P, u32, _P, sz, _P]),
+        "or_hit_uv_batch": (ctypes.c_int, [_P, _P, sz, _P]),
         "or_scene_commit": (ctypes.c_int, [_P]),
         "or_scene_num_lights": (ctypes.c_int, [_P]),
         "or_intersect_batch": (ctypes.c_int, [_P, _P, sz, ctypes.c_int, _P, _P]),
@@ -93,9 +98,18 @@ class OracleScene:
         self.L = L
         self.spec = spec
         self.h = lib.or_scene_create()
-        for kind, params in spec.materials:
-            m = L.material(kind, params)
+        for mi, (kind, params) in enumerate(spec.materials):
+            m = L.material(kind, params, getattr(spec, "material_flags", {}).get(mi, 0))
             lib.or_scene_add_material(self.h, sref(m))
+        for tex in getattr(spec, "textures", []):
+            if tex[0] == "image":
+                rgb = np.ascontiguousarray(tex[1], np.float32)
+                if lib.or_scene_add_texture_image(self.h, ptr(rgb), rgb.shape[1], rgb.shape[0]) < 0:
+                    raise ValueError("bad texture")
+            else:
+                c1, c2 = (np.asarray(c, np.float32) for c in tex[1:3])
+                lib.or_scene_add_texture_checker(self.h, ptr(c1), ptr(c2), tex[3])
+        n_mesh = 0
         for kind, i in spec.order:
             if kind == "mesh":
                 pos, idx, mat, flags = spec.meshes[i]
@@ -107,6 +121,11 @@ class OracleScene:
                     rc = lib.or_scene_add_mesh(self.h, ptr(pos), pos.shape[0], ptr(idx), idx.shape[0], mat, flags)
                 if rc < 0:
                     raise ValueError("bad mesh")
+                if i in getattr(spec, "mesh_uvs", {}):
+                    uv, uvidx = spec.mesh_uvs[i]
+                    if lib.or_scene_set_mesh_uv(self.h, n_mesh, ptr(uv), uv.shape[0], ptr(uvidx), uvidx.shape[0]) < 0:
+                        raise ValueError("bad mesh uvs")
+                n_mesh += 1
             elif kind == "sphere":
                 c, r, mat, flags = spec.spheres[i]
                 ca = np.asarray(c, np.float32)
@@ -179,6 +198,21 @@ class OracleScene:
         inside = np.empty(n, np.int32)
         lib.or_surface_batch(self.h, ptr(rays), n, ptr(nrm), ptr(inside))
         return nrm, inside
+
+    def texture(self, tex_id, uv):
+        """Texture tex_id at uv (n, 2): (n, 3) colours."""
+        uv = np.ascontiguousarray(uv, np.float32)
+        out = np.empty((uv.shape[0], 3), np.float32)
+        if lib.or_texture_batch(self.h, tex_id, ptr(uv), uv.shape[0], ptr(out)):
+            raise ValueError("bad texture id")
+        return out
+
+    def hit_uv(self, rays):
+        """HitRecord::texCoords of the closest hits ((-1, -1) on a miss)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        uv = np.empty((rays.shape[0], 2), np.float32)
+        lib.or_hit_uv_batch(self.h, ptr(rays), rays.shape[0], ptr(uv))
+        return uv
 
     def camera_rays(self, cfg, pixel, k):
         pixel = np.ascontiguousarray(pixel, np.uint32)

@@ -121,7 +121,8 @@ class GuideStats(ctypes.Structure):
 EXPORTED = [
     "pt_scene_create", "pt_scene_destroy", "pt_scene_add_material", "pt_scene_add_mesh", "pt_scene_add_mesh_normals",
     "pt_scene_add_sphere", "pt_scene_add_rect", "pt_scene_add_curve", "pt_scene_set_env", "pt_scene_set_camera",
-    "pt_scene_set_medium", "pt_scene_update_mesh",
+    "pt_scene_set_medium", "pt_scene_update_mesh", "pt_scene_add_texture_image", "pt_scene_add_texture_checker",
+    "pt_scene_set_mesh_uv",
     "pt_scene_commit", "pt_scene_get_info", "pt_render", "pt_render_noalloc", "pt_film_read", "pt_film_read_samples",
     "pt_scene_set_tuning", "pt_autotune",
     "pt_batch_create", "pt_batch_destroy", "pt_batch_add", "pt_batch_set_cfg", "pt_batch_render",
@@ -152,6 +153,9 @@ def _load():
         "pt_scene_add_mesh_normals": (i32, [_P, _P, u32, _P, u32, _P, u32, _P, u32, u32]),
         "pt_scene_add_sphere": (i32, [_P, _P, ctypes.c_float, u32, u32]),
         "pt_scene_update_mesh": (i32, [_P, u32, _P, u32]),
+        "pt_scene_add_texture_image": (i32, [_P, _P, u32, u32, ctypes.POINTER(u32)]),
+        "pt_scene_add_texture_checker": (i32, [_P, _P, _P, ctypes.c_float, ctypes.POINTER(u32)]),
+        "pt_scene_set_mesh_uv": (i32, [_P, u32, _P, u32, _P, u32]),
         "pt_scene_add_rect": (i32, [_P, ctypes.c_int] + [ctypes.c_float] * 5 + [u32, u32]),
         "pt_scene_add_curve": (i32, [_P, _P, ctypes.c_float, ctypes.c_float, u32, u32, u32]),
         "pt_scene_set_env": (i32, [_P, _P, u32, u32, ctypes.c_float]),
@@ -223,10 +227,15 @@ def ptr(a: np.ndarray | None):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def material(kind: int, params) -> MaterialDesc:
+def mat_texture(tex_id: int) -> int:
+    """PT_MAT_TEXTURE(id): the material's colour comes from texture tex_id."""
+    return 0x1 | (int(tex_id) << 8)
+
+
+def material(kind: int, params, flags: int = 0) -> MaterialDesc:
     m = MaterialDesc()
     m.kind = kind
-    m.flags = 0
+    m.flags = flags
     for i, v in enumerate(params):
         m.p[i] = float(v)
     return m

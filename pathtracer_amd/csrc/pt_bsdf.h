@@ -15,6 +15,10 @@ struct ShadePoint {
   f3 n;        // normal opposing the incoming ray (HitRecord::set_normal)
   bool inside; // HitRecord::is_inside (= front face, materialBase.hpp:50)
   f3 t = {0.f, 0.f, 0.f};  // HitRecord::hair_tangent: curves set it, others leave it 0
+  // a textured material's colour at the hit's texture coordinates (Lambertian
+  // albedo, Disney baseColor); has_tex false: the material's own p[0..2]
+  f3 tex = {0.f, 0.f, 0.f};
+  bool has_tex = false;
 };
 
 struct Scatter {
@@ -112,7 +116,7 @@ PT_D f3 cosine_dir(f3 n, float u0, float u1) {
 PT_D bool lambert_scatter(const pt_material_desc &m, const ShadePoint &h, float u0, float u1, Scatter &s) {
   Frame b = make_frame(h.n);
   s.flags = BX_DIFFUSE | BX_REFL;
-  s.att = scl3(mp3(m, 0), 1.f / PT_PI);
+  s.att = scl3(h.has_tex ? h.tex : mp3(m, 0), 1.f / PT_PI);
   s.wi = norm3(to_world(b, cosine_hemi(u0, u1)));
   s.pdf = fmaxf(dot3(norm3(s.wi), b.w), 0.f) / PT_PI;
   return true;
@@ -473,6 +477,9 @@ constexpr uint32_t kMatBasic = (1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_EMITTER
 // and dielectric (the Cornell and glass/HDRI scenes).
 constexpr uint32_t kMatMicro = (1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_EMITTER) | (1u << PT_MAT_CONDUCTOR) |
                                (1u << PT_MAT_MFDIELECTRIC);
+// Material-set bit of scenes with a textured material (only the all-kinds
+// set carries the texture lookups).
+constexpr uint32_t kMatTextured = 1u << 31;
 #define PT_IN(MS, K) (((MS) >> (K)) & 1u)
 
 template <uint32_t MS = kMatAll>
@@ -496,7 +503,7 @@ PT_D bool bsdf_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float uc
 template <uint32_t MS = kMatAll>
 PT_D f3 bsdf_f(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
   const uint32_t k = m.kind;
-  if (PT_IN(MS, PT_MAT_LAMBERTIAN) && k == PT_MAT_LAMBERTIAN) return scl3(mp3(m, 0), 1.f / PT_PI);
+  if (PT_IN(MS, PT_MAT_LAMBERTIAN) && k == PT_MAT_LAMBERTIAN) return scl3(h.has_tex ? h.tex : mp3(m, 0), 1.f / PT_PI);
   if (PT_IN(MS, PT_MAT_CONDUCTOR) && k == PT_MAT_CONDUCTOR) return conductor_f(m, wo, wi, h);
   if (PT_IN(MS, PT_MAT_MFDIELECTRIC) && k == PT_MAT_MFDIELECTRIC) return mfdiel_f(m, wo, wi, h);
   if (PT_IN(MS, PT_MAT_PHONG) && k == PT_MAT_PHONG) return phong_f(m, wo, wi, h);

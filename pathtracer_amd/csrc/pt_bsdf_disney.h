@@ -39,9 +39,9 @@ struct Disney {
   TRD dist;
   float cc_alpha;
 };
-PT_D Disney disney_params(const pt_material_desc &m) {
+PT_D Disney disney_params(const pt_material_desc &m, const ShadePoint &h) {
   Disney d;
-  d.base = mk3(m.p[0], m.p[1], m.p[2]);
+  d.base = h.has_tex ? h.tex : mk3(m.p[0], m.p[1], m.p[2]);
   d.spec_trans = m.p[3];
   d.metallic = m.p[4];
   d.subsurface = m.p[5];
@@ -254,7 +254,7 @@ PT_D f3 dsheen_f(const Disney &d, f3 wo_w, f3 wi_w, const ShadePoint &h) {
 }
 
 PT_D f3 disney_f(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
-  Disney d = disney_params(m);
+  Disney d = disney_params(m, h);
   if (h.inside) return dglass_f(d, wo, wi, h);
   f3 r = scl3(ddiffuse_f(d, wo, wi, h), d.dw);
   r = add3(r, scl3(dmetal_f_variant(d, wo, wi, h), d.mw));
@@ -264,7 +264,7 @@ PT_D f3 disney_f(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
   return r;
 }
 PT_D float disney_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
-  Disney d = disney_params(m);
+  Disney d = disney_params(m, h);
   if (h.inside) return dglass_pdf(d, wo, wi, h);
   float sum = d.dw + d.mw + d.gw + d.cw;
   float p = (d.dw / sum) * cosine_pdf(h.n, wi);
@@ -313,7 +313,7 @@ PT_D void dglass_f_pdf_local(const Disney &d, f3 wo_l, f3 wi_l, f3 &f, float &pd
 // early-outs and operation order, so f and pdf equal the separate functions'
 // bit for bit.
 PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
-  const Disney d = disney_params(m);
+  const Disney d = disney_params(m, h);
   const Frame b = make_frame(h.n);
   const f3 wo_n = to_local(b, norm3(wo)), wi_n = to_local(b, norm3(wi));
   f3 gf;
@@ -414,7 +414,7 @@ PT_D int sample_discrete(const float (&w)[N], float u, float *pmf, float *u_rema
 }
 PT_D bool disney_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float uc, float u0, float u1,
                          Scatter &s) {
-  Disney d = disney_params(m);
+  Disney d = disney_params(m, h);
   if (h.inside) return dglass_scatter(d, wo, h, uc, u0, u1, s);
   float w[4] = {d.dw, d.mw, d.gw, d.cw};
   float pmf, ucr;

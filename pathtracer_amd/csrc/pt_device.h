@@ -681,6 +681,109 @@ PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
 }
 
 // ---------------------------------------------------------------------
+// Textures.  ImageTexture::getTexelColor, bilinear (texture.cpp:111-142):
+// the lower texel index is (uint)(u w), the fraction taken before it is
+// clamped to w-2 (h-2): in the last cell the weights are those of the cell
+// one texel back, and u = 1 reads texel w-2; negative coordinates convert
+// to index 0 (and extrapolate), as a saturating conversion gives.  CheckerTexture (texture.cpp:165-183): colour 1 where
+// sin(u scale) sin(v scale) < 0.
+// ---------------------------------------------------------------------
+PT_D f3 tex_texel(const float *tx, uint32_t w, uint32_t x, uint32_t y) {
+  const float *c = tx + 3 * ((size_t)y * w + x);
+  return mk3(c[0], c[1], c[2]);
+}
+PT_D f3 texture_lookup(const DevScene &s, uint32_t id, float u, float v) {
+  const float4 a = s.tex_desc[2 * id];
+  if (__float_as_uint(a.x) == TEX_CHECKER) {
+    const float4 b = s.tex_desc[2 * id + 1];
+    const float sv = sinf(u * a.y) * sinf(v * a.y);
+    return sv < 0.f ? mk3(a.z, a.w, b.x) : mk3(b.y, b.z, b.w);
+  }
+  const uint32_t w = __float_as_uint(a.y), h = __float_as_uint(a.z);
+  const float *tx = s.tex + 3 * (size_t)__float_as_uint(a.w);
+  const float fx = u * (float)w, fy = v * (float)h;
+  uint32_t x = fx > 0.f ? (uint32_t)fx : 0u, y = fy > 0.f ? (uint32_t)fy : 0u;
+  const float x1 = fx - (float)x, y1 = fy - (float)y;
+  x = min(x, w - 2u);
+  y = min(y, h - 2u);
+  const float x2 = 1.0f - x1, y2 = 1.0f - y1;
+  const f3 c1 = tex_texel(tx, w, x, y), c2 = tex_texel(tx, w, x + 1, y);
+  const f3 c3 = tex_texel(tx, w, x, y + 1), c4 = tex_texel(tx, w, x + 1, y + 1);
+  const f3 c12 = add3(scl3(c1, x2), scl3(c2, x1));
+  const f3 c34 = add3(scl3(c3, x2), scl3(c4, x1));
+  return add3(scl3(c12, y2), scl3(c34, y1));
+}
+// HitRecord::texCoords of a hit on primitive prim at h.p:
+//   sphere  (1 - (phi + pi) / 2pi, (theta + pi/2) / pi) of the unit vector to
+//           the hit (phi = atan2(z, x), theta = asin(y)), u shifted by 0.75
+//           modulo 1 (primitive.hpp:157-163)
+//   rect    the hit's position within the extents (primitive.cpp:55-57)
+//   triangle (1-u-v) uv0 + u uv1 + v uv2 with the hit's barycentrics
+//           (meshBVH.cpp:133-138), or without uvs TriangleMesh's planar
+//           fallback: the two axes the normal points least along, scaled
+//           to the mesh box and clamped to [0, 1] (triangleMesh.cpp:38-51)
+//   curve   (0, 0): Curve sets none
+#pragma clang fp contract(off)
+PT_D float2 hit_uv(const DevScene &s, int prim, const SurfHit &h, f3 o, f3 d) {
+  const uint2 info = s.prim_info[prim];
+  const uint32_t type = (info.x >> 24) & 3u;
+  const float4 g = s.prim_geom[prim];
+  if (type == PRIM_SPHERE) {
+    const f3 p = norm3(sub3(h.p, mk3(g.x, g.y, g.z)));
+    const float phi = atan2f(p.z, p.x), theta = asinf(p.y);
+    const double pi = 3.14159265358979323846;
+    float u = (float)(1.0 - ((double)phi + pi) / (2.0 * pi));
+    const float v = (float)(((double)theta + pi / 2.0) / pi);
+    u = fmodf(u + 0.75f, 1.0f);
+    return make_float2(u, v);
+  }
+  const float4 pt = s.prim_tex[prim];
+  if (type == PRIM_RECT) {
+    const int ax = (int)g.x;
+    const float d0 = ax == 0 ? h.p.y : h.p.x, d1 = ax == 2 ? h.p.y : h.p.z;
+    return make_float2((d0 - pt.x) / (pt.y - pt.x), (d1 - pt.z) / (pt.w - pt.z));
+  }
+  if (type != PRIM_TRI) return make_float2(0.f, 0.f);
+  const uint32_t rec = __float_as_uint(pt.y);
+  if (rec != 0xffffffffu) {
+    const float4 *r = s.tri_uv + 4 * (size_t)rec;
+    const float4 a = r[0], b = r[1], c = r[2], e = r[3];
+    const f3 p0 = mk3(a.x, a.y, a.z), e1 = mk3(b.x, b.y, b.z), e2 = mk3(c.x, c.y, c.z);
+    const f3 P = cross3(d, e2);
+    const float inv = rcp_rn(dot3(e1, P));
+    const f3 T = sub3(o, p0);
+    const float bu = dot3(T, P) * inv;
+    const f3 Q = cross3(T, e1);
+    const float bv = dot3(d, Q) * inv;
+    const float w0 = 1.f - bu - bv;
+    return make_float2(w0 * a.w + bu * c.w + bv * e.y, w0 * b.w + bu * e.x + bv * e.z);
+  }
+  const uint32_t mesh = __float_as_uint(pt.x);
+  const float4 lo = s.mesh_box[2 * mesh], hi = s.mesh_box[2 * mesh + 1];
+  const f3 n = mk3(fabsf(h.sp.n.x), fabsf(h.sp.n.y), fabsf(h.sp.n.z));
+  const int au = (n.x > n.y && n.x > n.z) ? 1 : 0;
+  const int av = au == 1 ? 2 : (n.y > n.z ? 2 : 1);
+  const float pl[3] = {h.p.x, h.p.y, h.p.z}, ml[3] = {lo.x, lo.y, lo.z}, mh[3] = {hi.x, hi.y, hi.z};
+  const float u = (pl[au] - ml[au]) / (mh[au] - ml[au]), v = (pl[av] - ml[av]) / (mh[av] - ml[av]);
+  return make_float2(fminf(fmaxf(u, 0.f), 1.f), fminf(fmaxf(v, 0.f), 1.f));
+}
+// surface() plus, for a textured material (material sets with
+// kMatTextured), its colour at the hit's texture coordinates.
+template <uint32_t MS>
+PT_D SurfHit surface_tex(const DevScene &s, int prim, f3 o, f3 d, float t) {
+  SurfHit h = surface(s, prim, o, d, t);
+  if constexpr (PT_IN(MS, 31)) {
+    const uint32_t fl = s.mats[h.mat].flags;
+    if (fl & 1u) {
+      const float2 uv = hit_uv(s, prim, h, o, d);
+      h.sp.tex = texture_lookup(s, fl >> 8, uv.x, uv.y);
+      h.sp.has_tex = true;
+    }
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------------
 // Environment.  HDRITexture::getTexelColor (texture.hpp:105-115) and the
 // bilinear ImageTexture lookup with index clamp to size-2 taken after the
 // fractions (texture.cpp:111-142).

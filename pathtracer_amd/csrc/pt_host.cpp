@@ -53,7 +53,32 @@ struct HMesh {
   uint32_t first, count, mat, flags, seq;
   uint32_t nverts = 0;
   std::vector<int32_t> idx;  // the index buffer, for position updates
+  f3 box_lo, box_hi;          // TriangleMesh::meshAABB: vertex bounds -/+ gl::epsilon (triangleMesh.cpp:10-18)
+  std::vector<float> uv;      // MeshData::uvs (2 floats each), empty: planar fallback
+  std::vector<int32_t> uvidx; // MeshData::uvIndices, 3 per triangle
 };
+// A texture: ImageTexture texels (texel (x, y) at rgb[3 (y w + x)]) or a
+// CheckerTexture's two colours and scale.
+struct HTexture {
+  uint32_t kind;  // TEX_IMAGE, TEX_CHECKER
+  uint32_t w = 0, h = 0;
+  std::vector<float> rgb;
+  f3 c1, c2;
+  float scale = 0.f;
+};
+// TriangleMesh's bounding box, as its constructor forms it: per-axis min and
+// max over the vertices, then -/+ gl::epsilon (1e-3, constants.hpp:7).
+static void mesh_box(const float *pos, uint32_t nv, f3 &lo, f3 &hi) {
+  lo = s3(INFINITY);
+  hi = s3(-INFINITY);
+  for (uint32_t v = 0; v < nv; ++v) {
+    const f3 p = mk3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
+    lo = mk3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+    hi = mk3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+  }
+  lo = sub3(lo, s3(1e-3f));
+  hi = add3(hi, s3(1e-3f));
+}
 struct HSphere { f3 c; float r; uint32_t mat, flags, seq; int light; };
 struct HRect { int axis; float k, d0min, d0max, d1min, d1max; uint32_t mat, flags, seq; int light; };
 struct HCurve { f3 cps[4]; float w0, w1; uint32_t mat, flags; };
@@ -66,6 +91,7 @@ struct pt_scene {
   std::vector<HSphere> spheres;
   std::vector<HRect> rects;
   std::vector<HCurve> curves;
+  std::vector<HTexture> textures;
   uint32_t seq = 0;
   std::vector<float> env;
   uint32_t env_w = 0, env_h = 0;
@@ -82,6 +108,7 @@ struct pt_scene {
   DevBuf d_nodes, d_qnodes, d_prims, d_info, d_geom, d_smooth, d_curve, d_mats, d_lights, d_env, d_env_func, d_env_cdf,
       d_env_mcdf, d_env_guide;
   DevBuf d_perm, d_perm_off;
+  DevBuf d_tex, d_tex_desc, d_prim_tex, d_tri_uv, d_mesh_box;  // textures and texture coordinates
   DevBuf lb_in, lb_scratch, lb_nodes;  // device BVH build: input records, scratch, nodes before trimming
   PinnedBuf lb_host;                   // the input records, staged
   uint32_t perm_seed = 0xffffffffu;
@@ -150,6 +177,9 @@ static bool valid_kind(uint32_t k) { return k >= PT_MAT_LAMBERTIAN && k <= PT_MA
 extern "C" pt_status pt_scene_add_material(pt_scene *s, const pt_material_desc *d, uint32_t *out_id) {
   if (!s || !d) return fail(PT_ERR_INVALID, "pt_scene_add_material: NULL argument");
   if (!valid_kind(d->kind)) return fail(PT_ERR_INVALID, "pt_scene_add_material: unknown material kind");
+  if ((d->flags & ~0xffffff01u) != 0u) return fail(PT_ERR_INVALID, "pt_scene_add_material: unknown flags");
+  if ((d->flags & 1u) && d->kind != PT_MAT_LAMBERTIAN && d->kind != PT_MAT_DISNEY)
+    return fail(PT_ERR_UNSUPPORTED, "pt_scene_add_material: only Lambertian and Disney materials take a texture");
   if (s->mats.size() >= (1u << 24)) return fail(PT_ERR_INVALID, "too many materials");
   s->mats.push_back(*d);
   if (out_id) *out_id = (uint32_t)s->mats.size() - 1;
@@ -212,6 +242,7 @@ static pt_status add_mesh(pt_scene *s, const char *fn, const float *pos, uint32_
   m.seq = s->seq++;
   m.nverts = nv;
   m.idx.assign(idx, idx + 3 * (size_t)nt);
+  mesh_box(pos, nv, m.box_lo, m.box_hi);
   s->meshes.push_back(std::move(m));
   s->committed = false;
   return PT_OK;
@@ -231,6 +262,51 @@ extern "C" pt_status pt_scene_update_mesh(pt_scene *s, uint32_t mesh, const floa
       tr.p[c] = mk3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]);
     }
   });
+  mesh_box(pos, nv, s->meshes[mesh].box_lo, s->meshes[mesh].box_hi);
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_set_mesh_uv(pt_scene *s, uint32_t mesh, const float *uv, uint32_t nuv,
+                                          const int32_t *uvidx, uint32_t nt) {
+  if (!s || (!uv && nuv) || (!uvidx && nt)) return fail(PT_ERR_INVALID, "pt_scene_set_mesh_uv: NULL argument");
+  if (mesh >= s->meshes.size()) return fail(PT_ERR_INVALID, "pt_scene_set_mesh_uv: mesh index out of range");
+  HMesh &m = s->meshes[mesh];
+  if (nt != m.count) return fail(PT_ERR_INVALID, "pt_scene_set_mesh_uv: triangle count differs from the mesh's");
+  for (size_t k = 0; k < 3 * (size_t)nt; ++k)
+    if (uvidx[k] < 0 || (uint32_t)uvidx[k] >= nuv) return fail(PT_ERR_INVALID, "pt_scene_set_mesh_uv: uv index out of range");
+  m.uv.assign(uv, uv + 2 * (size_t)nuv);
+  m.uvidx.assign(uvidx, uvidx + 3 * (size_t)nt);
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_add_texture_image(pt_scene *s, const float *rgb, uint32_t w, uint32_t h,
+                                                uint32_t *out_id) {
+  if (!s || !rgb) return fail(PT_ERR_INVALID, "pt_scene_add_texture_image: NULL argument");
+  if (w < 2 || h < 2) return fail(PT_ERR_INVALID, "pt_scene_add_texture_image: width and height must be >= 2");
+  if ((uint64_t)w * h >= (1ull << 28)) return fail(PT_ERR_INVALID, "pt_scene_add_texture_image: texture too large");
+  HTexture t;
+  t.kind = TEX_IMAGE;
+  t.w = w;
+  t.h = h;
+  t.rgb.assign(rgb, rgb + 3 * (size_t)w * h);
+  s->textures.push_back(std::move(t));
+  if (out_id) *out_id = (uint32_t)s->textures.size() - 1;
+  s->committed = false;
+  return PT_OK;
+}
+
+extern "C" pt_status pt_scene_add_texture_checker(pt_scene *s, const float c1[3], const float c2[3], float scale,
+                                                  uint32_t *out_id) {
+  if (!s || !c1 || !c2) return fail(PT_ERR_INVALID, "pt_scene_add_texture_checker: NULL argument");
+  HTexture t;
+  t.kind = TEX_CHECKER;
+  t.c1 = mk3(c1[0], c1[1], c1[2]);
+  t.c2 = mk3(c2[0], c2[1], c2[2]);
+  t.scale = scale;
+  s->textures.push_back(std::move(t));
+  if (out_id) *out_id = (uint32_t)s->textures.size() - 1;
   s->committed = false;
   return PT_OK;
 }
@@ -761,6 +837,86 @@ static float texel_max(const pt_scene *s, uint32_t x, uint32_t y) {
   return std::max(c[0], std::max(c[1], c[2]));
 }
 
+// -- textures: the texel pool and texture records, and the per-primitive
+// data the texture coordinates need (only when a material is textured):
+//   tex_desc  2 float4 per texture: image (TEX_IMAGE, w, h, first texel),
+//             checker (TEX_CHECKER, scale, c1.x, c1.y) (c1.z, c2)
+//   prim_tex  float4 per primitive id: triangle (mesh, uv record or ~0u);
+//             rect (d0min, d0max, d1min, d1max)
+//   tri_uv    4 float4 per triangle of a mesh with uvs: (p0, uv0.x)
+//             (e1, uv0.y) (e2, uv1.x) (uv1.y, uv2, -)
+//   mesh_box  2 float4 per mesh: TriangleMesh::meshAABB min, max
+static pt_status commit_textures(pt_scene *s, DevScene &ds) {
+  bool any = false;
+  for (const pt_material_desc &m : s->mats)
+    if (m.flags & 1u) {
+      any = true;
+      if ((m.flags >> 8) >= s->textures.size())
+        return fail(PT_ERR_INVALID, "pt_scene_commit: a material names a texture that was not added");
+    }
+  ds.tex = nullptr;
+  ds.tex_desc = ds.prim_tex = ds.tri_uv = ds.mesh_box = nullptr;
+  if (!any) return PT_OK;
+  ds.mat_kinds |= kMatTextured;
+  auto bits = [](uint32_t u) { return __builtin_bit_cast(float, u); };
+  std::vector<float4> desc;
+  std::vector<float> texels;
+  for (const HTexture &t : s->textures) {
+    if (t.kind == TEX_IMAGE) {
+      desc.push_back(make_float4(bits(TEX_IMAGE), bits(t.w), bits(t.h), bits((uint32_t)(texels.size() / 3))));
+      desc.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+      texels.insert(texels.end(), t.rgb.begin(), t.rgb.end());
+    } else {
+      desc.push_back(make_float4(bits(TEX_CHECKER), t.scale, t.c1.x, t.c1.y));
+      desc.push_back(make_float4(t.c1.z, t.c2.x, t.c2.y, t.c2.z));
+    }
+  }
+  if (texels.empty()) texels.assign(3, 0.f);
+  const uint32_t ntri = (uint32_t)s->tris.size(), nsph = (uint32_t)s->spheres.size();
+  const size_t nprim = ntri + nsph + s->rects.size() + 8 * s->curves.size();
+  std::vector<float4> ptex(nprim, make_float4(0.f, 0.f, 0.f, 0.f)), box, tuv;
+  for (const HMesh &m : s->meshes) {
+    box.push_back(make_float4(m.box_lo.x, m.box_lo.y, m.box_lo.z, 0.f));
+    box.push_back(make_float4(m.box_hi.x, m.box_hi.y, m.box_hi.z, 0.f));
+  }
+  for (uint32_t mi = 0; mi < (uint32_t)s->meshes.size(); ++mi) {
+    const HMesh &m = s->meshes[mi];
+    for (uint32_t t = 0; t < m.count; ++t) {
+      const uint32_t id = m.first + t;
+      uint32_t rec = 0xffffffffu;
+      if (!m.uv.empty()) {
+        rec = (uint32_t)(tuv.size() / 4);
+        const HTri &tr = s->tris[id];
+        const f3 e1 = sub3(tr.p[1], tr.p[0]), e2 = sub3(tr.p[2], tr.p[0]);
+        const float *u0 = &m.uv[2 * (size_t)m.uvidx[3 * t]], *u1 = &m.uv[2 * (size_t)m.uvidx[3 * t + 1]],
+                    *u2 = &m.uv[2 * (size_t)m.uvidx[3 * t + 2]];
+        tuv.push_back(make_float4(tr.p[0].x, tr.p[0].y, tr.p[0].z, u0[0]));
+        tuv.push_back(make_float4(e1.x, e1.y, e1.z, u0[1]));
+        tuv.push_back(make_float4(e2.x, e2.y, e2.z, u1[0]));
+        tuv.push_back(make_float4(u1[1], u2[0], u2[1], 0.f));
+      }
+      ptex[id] = make_float4(bits(mi), bits(rec), 0.f, 0.f);
+    }
+  }
+  for (size_t i = 0; i < s->rects.size(); ++i) {
+    const HRect &r = s->rects[i];
+    ptex[ntri + nsph + i] = make_float4(r.d0min, r.d0max, r.d1min, r.d1max);
+  }
+  if (box.empty()) box.assign(2, make_float4(0.f, 0.f, 0.f, 0.f));
+  if (tuv.empty()) tuv.assign(4, make_float4(0.f, 0.f, 0.f, 0.f));
+  HIP_TRY(upload(s->d_tex, texels.data(), texels.size() * sizeof(float)));
+  HIP_TRY(upload(s->d_tex_desc, desc.data(), desc.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_prim_tex, ptex.data(), ptex.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_tri_uv, tuv.data(), tuv.size() * sizeof(float4)));
+  HIP_TRY(upload(s->d_mesh_box, box.data(), box.size() * sizeof(float4)));
+  ds.tex = s->d_tex.as<float>();
+  ds.tex_desc = s->d_tex_desc.as<float4>();
+  ds.prim_tex = s->d_prim_tex.as<float4>();
+  ds.tri_uv = s->d_tri_uv.as<float4>();
+  ds.mesh_box = s->d_mesh_box.as<float4>();
+  return PT_OK;
+}
+
 extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   if (!s) return fail(PT_ERR_INVALID, "pt_scene_commit: NULL scene");
   size_t nprim = s->tris.size() + s->spheres.size() + s->rects.size() + 8 * s->curves.size();
@@ -1104,6 +1260,8 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.n_lights = (int)s->lights.size();
   ds.mat_kinds = 0u;
   for (const pt_material_desc &m : s->mats) ds.mat_kinds |= 1u << m.kind;
+  st = commit_textures(s, ds);
+  if (st != PT_OK) return st;
   ds.root = root;
   ds.top_nodes = (cfg && (cfg->flags & PT_BUILD_NO_LDS_NODES))
                      ? 0u

@@ -398,7 +398,7 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
         const pt_material_desc *mat = nullptr;
         bool drop = false;
         if (prim >= 0) {
-          h = surface(s, prim, o, dir, t);
+          h = surface_tex<MS>(s, prim, o, dir, t);
           mat = &s.mats[h.mat];
           drop = d > 0u && mat->kind == PT_MAT_EMITTER;  // :113-115: the vertex returns 0
         }
@@ -502,7 +502,7 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
                 emit_shadow(w, lp, dl, dist - 0.001f, c, slot);
             }
           } else if (prim >= 0) {
-            const SurfHit h = surface(s, prim, o, dir, t);
+            const SurfHit h = surface_tex<MS>(s, prim, o, dir, t);
             const pt_material_desc &m = s.mats[h.mat];
             if (m.kind == PT_MAT_EMITTER) L = add3(L, bsdf_emit(m, dir, h.sp));
           }
@@ -518,7 +518,7 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
         if (prim < 0) {
           L = add3(L, mul3(beta, bg));
         } else {
-          SurfHit h = surface(s, prim, o, dir, t);
+          SurfHit h = surface_tex<MS>(s, prim, o, dir, t);
           const pt_material_desc *mat = &s.mats[h.mat];
           p = h.p;
           float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
@@ -598,7 +598,7 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
         SurfHit h;
         const pt_material_desc *mat = nullptr;
         if (!done) {
-          h = surface(s, prim, o, dir, t);
+          h = surface_tex<MS>(s, prim, o, dir, t);
           mat = &s.mats[h.mat];
           if (mat->kind == PT_MAT_EMITTER) {
             if (EST == 0) {

@@ -34,6 +34,7 @@ enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2, PRIM_CURVE = 3 };
 // through pt_render_guided, so not in the public estimator list).
 enum { PT_EST_GUIDED = 6 };
 enum { LIGHT_QUAD = 1, LIGHT_SPHERE = 2, LIGHT_ENV = 3 };
+enum { TEX_IMAGE = 0, TEX_CHECKER = 1 };
 
 // A child reference: >= 0 internal node index; < 0 leaf ~(start<<4 | count-1);
 // kEmptyChild marks an unused slot of a 4-wide node.
@@ -135,7 +136,7 @@ PT_HD void quantize_node(const float4 *in, float4 *out) {
 // load that waits on both memory counters).
 typedef float pt_v4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(3))) pt_v4 lds_v4;
-PT_HD inline float4 ld_f4(const float4 *p, int i) { return p[i]; }
+PT_HD float4 ld_f4(const float4 *p, int i) { return p[i]; }
 __device__ inline float4 ld_f4(lds_v4 *p, int i) {
   const pt_v4 v = p[i];
   return make_float4(v.x, v.y, v.z, v.w);
@@ -184,6 +185,11 @@ struct DevScene {
   const float4 *prims;
   const uint2 *prim_info;
   const float4 *prim_geom;
+  // textures (only when a material is textured, else null): texel pool,
+  // 2 float4 records per texture, and the data texture coordinates need
+  // (pt_host.cpp commit_textures)
+  const float *tex;
+  const float4 *tex_desc, *prim_tex, *tri_uv, *mesh_box;
   const float4 *tri_smooth;  // per smooth triangle: (p0,n0.x) (e1,n0.y) (e2,n0.z) (n1,n2.x) (n2.y,n2.z,-,-)
   const float4 *curve_geom;  // per curve segment: (pa, len) (axis, radius) (dP_a, -) (dP_b, -)
   const pt_material_desc *mats;

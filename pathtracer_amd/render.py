@@ -19,10 +19,21 @@ class Scene:
         L.check(L.lib.pt_scene_create(ctypes.byref(h)), "pt_scene_create")
         self.h = h
         self._keep = []
-        for kind, params in spec.materials:
+        for mi, (kind, params) in enumerate(spec.materials):
             mid = ctypes.c_uint32()
-            L.check(L.lib.pt_scene_add_material(h, ctypes.byref(L.material(kind, params)), ctypes.byref(mid)),
-                    "pt_scene_add_material")
+            md = L.material(kind, params, spec.material_flags.get(mi, 0))
+            L.check(L.lib.pt_scene_add_material(h, ctypes.byref(md), ctypes.byref(mid)), "pt_scene_add_material")
+        for tex in spec.textures:
+            tid = ctypes.c_uint32()
+            if tex[0] == "image":
+                rgb = np.ascontiguousarray(tex[1], np.float32)
+                L.check(L.lib.pt_scene_add_texture_image(h, L.ptr(rgb), rgb.shape[1], rgb.shape[0], ctypes.byref(tid)),
+                        "pt_scene_add_texture_image")
+            else:
+                c1, c2 = (np.asarray(c, np.float32) for c in tex[1:3])
+                L.check(L.lib.pt_scene_add_texture_checker(h, L.ptr(c1), L.ptr(c2), tex[3], ctypes.byref(tid)),
+                        "pt_scene_add_texture_checker")
+        n_mesh = 0
         for kind, i in spec.order:
             if kind == "mesh":
                 pos, idx, mat, flags = spec.meshes[i]
@@ -34,6 +45,11 @@ class Scene:
                 else:
                     L.check(L.lib.pt_scene_add_mesh(h, L.ptr(pos), pos.shape[0], L.ptr(idx), idx.shape[0], mat,
                                                     flags), "pt_scene_add_mesh")
+                if i in spec.mesh_uvs:
+                    uv, uvidx = spec.mesh_uvs[i]
+                    L.check(L.lib.pt_scene_set_mesh_uv(h, n_mesh, L.ptr(uv), uv.shape[0], L.ptr(uvidx),
+                                                       uvidx.shape[0]), "pt_scene_set_mesh_uv")
+                n_mesh += 1
             elif kind == "sphere":
                 c, r, mat, flags = spec.spheres[i]
                 ca = np.asarray(c, np.float32)

@@ -42,10 +42,31 @@ class SceneSpec:
     # global HomogeneousMedium: (sigma_a(3), sigma_s(3), sigma_scale, Le(3), Le_scale, g) or None
     medium: tuple | None = None
     name: str = ""
+    # textures: ("image", rgb f32 (h,w,3), row 0 the bottom) or ("checker", c1, c2, scale)
+    textures: list = dataclasses.field(default_factory=list)
+    material_flags: dict = dataclasses.field(default_factory=dict)  # material index -> pt_material_desc.flags
+    mesh_uvs: dict = dataclasses.field(default_factory=dict)        # mesh index -> (uv f32 (k,2), uvidx i32 (m,3))
 
-    def add_material(self, kind, params) -> int:
+    def add_material(self, kind, params, texture=None) -> int:
+        """texture: a texture id (add_texture_*) the colour parameter comes from."""
         self.materials.append((kind, list(params)))
+        if texture is not None:
+            self.material_flags[len(self.materials) - 1] = L.mat_texture(texture)
         return len(self.materials) - 1
+
+    def add_texture_image(self, rgb) -> int:
+        """ImageTexture: rgb (h, w, 3), row 0 the image's bottom row."""
+        self.textures.append(("image", np.ascontiguousarray(rgb, np.float32)))
+        return len(self.textures) - 1
+
+    def add_texture_checker(self, c1, c2, scale=10.0) -> int:
+        """CheckerTexture(color1, color2, scale)."""
+        self.textures.append(("checker", tuple(float(x) for x in c1), tuple(float(x) for x in c2), float(scale)))
+        return len(self.textures) - 1
+
+    def set_mesh_uv(self, mesh, uv, uvidx):
+        """MeshData::uvs / uvIndices of mesh `mesh` (in adding order)."""
+        self.mesh_uvs[mesh] = (np.ascontiguousarray(uv, np.float32), np.ascontiguousarray(uvidx, np.int32))
 
     def add_mesh(self, pos, idx, mat, flags=0, normals=None, nidx=None):
         self.meshes.append((np.ascontiguousarray(pos, np.float32), np.ascontiguousarray(idx, np.int32), mat, flags))
@@ -240,6 +261,42 @@ def cornell(width=256, height=256, spp_x=4, spp_y=4, max_depth=8, rr_depth=3,
                     front=(0, 0, -1), pos=(0, 1.0, 3.9))
     s.render = dict(width=width, height=height, spp_x=spp_x, spp_y=spp_y, max_depth=max_depth, sampler=sampler,
                     estimator=estimator, rr_depth=rr_depth, light_samples=4, seed=7, bg=(0, 0, 0))
+    return s
+
+
+def texture_image(w=64, h=48, seed=6) -> np.ndarray:
+    """A procedural RGB image (row 0 the bottom): stripes over value noise."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    p = np.stack([x / w * 4.0, y / h * 4.0, np.zeros_like(x)], -1).reshape(-1, 3)
+    n = value_noise(p, seed, octaves=3).reshape(h, w)
+    stripes = 0.5 + 0.5 * np.sin(x * 0.6 + 3.0 * n)
+    rgb = np.stack([0.15 + 0.7 * stripes, 0.1 + 0.5 * n, 0.2 + 0.6 * (1 - stripes) * n], -1)
+    return np.clip(rgb, 0.0, 1.0).astype(np.float32)
+
+
+def textured_room(width=64, height=64, spp_x=2, spp_y=2, max_depth=8, rr_depth=3,
+                  sampler=L.SAMPLER_HALTON_OWEN, estimator=L.EST_REFERENCE_NEE) -> SceneSpec:
+    """The Cornell box with ImageTexture and CheckerTexture materials: a
+    checker floor rect, an image-textured back wall (rect), an image-textured
+    sphere, a Disney mesh whose base colour is the image over its uvs, and a
+    textured icosphere without uvs (the planar fallback)."""
+    s = cornell(width, height, spp_x, spp_y, max_depth, rr_depth, sampler, estimator, ico_subdiv=2)
+    s.name = "textured_room"
+    img = s.add_texture_image(texture_image())
+    chk = s.add_texture_checker((0.15, 0.2, 0.1), (0.85, 0.8, 0.75), 12.0)
+    floor = s.add_material(L.MAT_LAMBERTIAN, [0, 0, 0], texture=chk)
+    wall = s.add_material(L.MAT_LAMBERTIAN, [0, 0, 0], texture=img)
+    disney = s.add_material(L.MAT_DISNEY, [0, 0, 0, 0.0, 0.4, 0.0, 0.5, 0.35, 0.0, 0.2, 0.0, 0.5, 0.3, 0.6, 1.5],
+                            texture=img)
+    s.add_rect(1, 0.001, -0.95, 0.95, -0.95, 0.95, floor)
+    s.add_rect(2, -0.999, -0.9, 0.9, 0.05, 1.9, wall)
+    s.add_sphere((0.1, 1.25, -0.5), 0.22, wall)
+    pos = np.array([[-0.9, 0.6, -0.6], [-0.5, 0.6, -0.95], [-0.5, 1.4, -0.95], [-0.9, 1.4, -0.6]], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    s.add_mesh(pos, idx, disney)
+    s.set_mesh_uv(len(s.meshes) - 1, np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32), idx)
+    v, f = icosphere(2)
+    s.add_mesh((v * 0.2 + np.array([0.55, 1.0, -0.3])).astype(np.float32), f.astype(np.int32), wall)
     return s
 
 

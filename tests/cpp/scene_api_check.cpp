@@ -4,6 +4,10 @@
 //   scene_api_check render <obj> <out.pfm>   SceneInfo::render on the GPU (FP32 film)
 //   scene_api_check hdr    <in.hdr> <out.pfm>  EnvironmentLight texels, written back as PFM
 //   scene_api_check png    <out.png>         a fixed 4x3 film through write_png
+//   scene_api_check tdump  <a.ppm> <b.pfm> <out.txt>   the textured scene's lowering
+//   scene_api_check trender <a.ppm> <b.pfm> <out.pfm>  the textured scene rendered on the GPU
+//   scene_api_check texread <in> <flip_y> <isHDR> <out.pfm>  ImageTexture texels, written back as PFM
+//   scene_api_check noise <obj> <out.txt> / temit <x>  unsupported textures: must fail
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -63,12 +67,64 @@ static SceneInfo test_scene(const std::string &obj) {
   return scene;
 }
 
+// Textures in the reference's style (compare scenes/scenes_01.hpp:229-260):
+// a checker floor, image-textured Lambertian rects and spheres, a textured
+// Disney base colour on a mesh with texture coordinates, a mesh without them
+// (planar fallback) and a constant texture that lowers to a plain colour.
+static SceneInfo texture_scene(const std::string &ppm, const std::string &pfm) {
+  SceneInfo scene;
+  auto checker = std::make_shared<Lambertian>(std::make_shared<CheckerTexture>(vec3(0.2f, 0.3f, 0.1f),
+                                                                                vec3(0.9f, 0.9f, 0.9f), 10.f));
+  auto image = std::make_shared<ImageTexture>(ppm);
+  auto wood = std::make_shared<Lambertian>(image);
+  auto disney = std::make_shared<DisneyPrincipledBSDF>(std::make_shared<ImageTexture>(pfm, true, true), 0.f, 0.3f,
+                                                       0.f, 0.5f, 0.4f, 0.f, 0.f, 0.f, 0.5f, 0.f, 0.f, 1.5f);
+  auto solid = std::make_shared<Lambertian>(std::make_shared<ConstantTexture>(vec3(0.5f, 0.6f, 0.7f)));
+  auto light = std::make_shared<DiffuseEmitter>(vec3(1.f), 12.f);
+  scene.objects.addObject(std::make_shared<XZRectangle>(0.f, -2.f, 2.f, -2.f, 2.f, checker));
+  scene.objects.addObject(std::make_shared<XYRectangle>(-1.f, -2.f, 2.f, 0.f, 2.f, wood));
+  scene.objects.addObject(std::make_shared<XZRectangle>(3.f, -0.5f, 0.5f, -0.5f, 0.5f, light));
+  scene.objects.addObject(std::make_shared<Sphere>(vec3(-0.8f, 0.4f, 0.f), 0.4f, wood));
+  scene.objects.addObject(std::make_shared<Sphere>(vec3(0.9f, 0.3f, 0.5f), 0.3f, solid));
+  MeshData quad;
+  quad.positions = {vec3(0.f, 0.05f, 0.f), vec3(0.8f, 0.05f, 0.f), vec3(0.8f, 0.85f, -0.3f), vec3(0.f, 0.85f, -0.3f)};
+  quad.indices = {{0, 1, 2}, {0, 2, 3}};
+  quad.uvs = {gl::vec2(0.f, 0.f), gl::vec2(1.f, 0.f), gl::vec2(1.f, 1.f), gl::vec2(0.f, 1.f)};
+  quad.uvIndices = {{0, 1, 2}, {0, 2, 3}};
+  scene.objects.addObject(std::make_shared<TriangleMesh>(std::move(quad), disney));
+  MeshData tet;
+  tet.positions = {vec3(0.1f, 0.f, 0.6f), vec3(0.6f, 0.f, 0.7f), vec3(0.3f, 0.f, 1.1f), vec3(0.35f, 0.5f, 0.8f)};
+  tet.indices = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
+  std::shared_ptr<Hittable> tm = std::make_shared<TriangleMesh>(std::move(tet), wood);
+  scene.objects.addObject(std::make_shared<Translate>(tm, vec3(-0.2f, 0.f, 0.f)));
+  // rotated: the rect becomes two triangles with the rect's uvs, the mesh
+  // takes object-space planar uvs
+  MeshData tet2;
+  tet2.positions = {vec3(0.f, 0.f, 0.f), vec3(0.4f, 0.f, 0.1f), vec3(0.2f, 0.f, 0.5f), vec3(0.2f, 0.45f, 0.2f)};
+  tet2.indices = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
+  std::shared_ptr<Hittable> tm2 = std::make_shared<TriangleMesh>(std::move(tet2), checker);
+  tm2 = std::make_shared<Rotate<Axis::Y>>(tm2, to_radian(35.f));
+  scene.objects.addObject(std::make_shared<Translate>(tm2, vec3(1.2f, 0.f, -0.3f)));
+  std::shared_ptr<Hittable> panel = std::make_shared<YZRectangle>(0.f, 0.f, 0.6f, -0.3f, 0.3f, wood);
+  panel = std::make_shared<Rotate<Axis::Y>>(panel, to_radian(-30.f));
+  scene.objects.addObject(std::make_shared<Translate>(panel, vec3(-1.5f, 0.1f, 0.3f)));
+  scene.camera = std::make_shared<PerspectiveCamera>(to_radian(45.f), 1.f, 1.f, 1e30f, vec3(0.f, 1.f, 0.f),
+                                                     vec3(0.f, -0.35f, -1.f), vec3(0.f, 1.4f, 3.2f));
+  scene._width = 48;
+  scene._height = 48;
+  scene.spp_x = 2;
+  scene.spp_y = 2;
+  scene.max_depth = 5;
+  scene.bg_color = vec3(0.05f);
+  return scene;
+}
+
 struct RecordingSink : ptamd::SceneSink {
   FILE *f;
   uint32_t n = 0;
   explicit RecordingSink(FILE *f) : f(f) {}
   uint32_t material(const pt_material_desc &d) override {
-    std::fprintf(f, "material %u", d.kind);
+    std::fprintf(f, "material %u %u", d.kind, d.flags);
     for (int i = 0; i < 15; ++i) std::fprintf(f, " %.9g", d.p[i]);
     std::fprintf(f, "\n");
     return n++;
@@ -91,7 +147,40 @@ struct RecordingSink : ptamd::SceneSink {
             uint32_t flags) override {
     std::fprintf(f, "rect %u %u %d %.9g %.9g %.9g %.9g %.9g\n", mat, flags, axis, k, d0min, d0max, d1min, d1max);
   }
+  uint32_t texture_image(const std::vector<float> &rgb, uint32_t w, uint32_t h) override {
+    std::fprintf(f, "texture_image %u %u", w, h);
+    for (float x : rgb) std::fprintf(f, " %.9g", x);
+    std::fprintf(f, "\n");
+    return ntex++;
+  }
+  uint32_t texture_checker(const float c1[3], const float c2[3], float scale) override {
+    std::fprintf(f, "texture_checker %.9g %.9g %.9g %.9g %.9g %.9g %.9g\n", scale, c1[0], c1[1], c1[2], c2[0], c2[1],
+                 c2[2]);
+    return ntex++;
+  }
+  void mesh_uv(const std::vector<float> &uv, const std::vector<int32_t> &uvidx) override {
+    std::fprintf(f, "mesh_uv %zu %zu", uv.size() / 2, uvidx.size() / 3);
+    for (float x : uv) std::fprintf(f, " %.9g", x);
+    for (int32_t x : uvidx) std::fprintf(f, " %d", x);
+    std::fprintf(f, "\n");
+  }
+  uint32_t ntex = 0;
 };
+
+static int dump(SceneInfo s, const char *out) {
+  FILE *f = std::fopen(out, "w");
+  if (!f) return 1;
+  RecordingSink sink(f);
+  s.lower(sink);
+  pt_camera_desc c = s.camera->desc();
+  std::fprintf(f, "camera %.9g %.9g %.9g %.9g", c.fov, c.aspect, c.focal_dist, c.aperture);
+  for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.up[i]);
+  for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.front[i]);
+  for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.pos[i]);
+  std::fprintf(f, "\n");
+  std::fclose(f);
+  return 0;
+}
 
 int main(int argc, char **argv) {
   if (argc < 3) {
@@ -100,19 +189,26 @@ int main(int argc, char **argv) {
   }
   const std::string mode = argv[1];
   try {
-    if (mode == "dump" && argc == 4) {
+    if (mode == "dump" && argc == 4) return dump(test_scene(argv[2]), argv[3]);
+    if (mode == "tdump" && argc == 5) return dump(texture_scene(argv[2], argv[3]), argv[4]);
+    if (mode == "trender" && argc == 5) {
+      SceneInfo s = texture_scene(argv[2], argv[3]);
+      s.render(argv[4], false);
+      return 0;
+    }
+    if (mode == "texread" && argc == 6) {
+      ImageTexture t(argv[2], std::strcmp(argv[3], "1") == 0, std::strcmp(argv[4], "1") == 0);
+      ptamd::image_io::write_pfm(argv[5], t.rgb.data(), t.width, t.height);
+      return 0;
+    }
+    if (mode == "noise" && argc == 4) {  // a NoiseTexture is refused at lowering
       SceneInfo s = test_scene(argv[2]);
-      FILE *f = std::fopen(argv[3], "w");
-      if (!f) return 1;
-      RecordingSink sink(f);
-      s.lower(sink);
-      pt_camera_desc c = s.camera->desc();
-      std::fprintf(f, "camera %.9g %.9g %.9g %.9g", c.fov, c.aspect, c.focal_dist, c.aperture);
-      for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.up[i]);
-      for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.front[i]);
-      for (int i = 0; i < 3; ++i) std::fprintf(f, " %.9g", c.pos[i]);
-      std::fprintf(f, "\n");
-      std::fclose(f);
+      s.objects.addObject(
+          std::make_shared<Sphere>(vec3(0.f), 0.1f, std::make_shared<Lambertian>(std::make_shared<NoiseTexture>(4.f))));
+      return dump(s, argv[3]);
+    }
+    if (mode == "temit" && argc == 2 + 1) {  // so is a textured emitter, at construction
+      DiffuseEmitter e(std::make_shared<CheckerTexture>(), 1.f);
       return 0;
     }
     if (mode == "render" && argc == 4) {

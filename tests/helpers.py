@@ -55,7 +55,18 @@ def parse_dump(path):
             t = line.split()
             kind = t[0]
             if kind == "material":
-                out.append(dict(kind="material", mat_kind=int(t[1]), p=[float(x) for x in t[2:]]))
+                out.append(dict(kind="material", mat_kind=int(t[1]), flags=int(t[2]), p=[float(x) for x in t[3:]]))
+            elif kind == "texture_image":
+                w, h = int(t[1]), int(t[2])
+                out.append(dict(kind="texture_image", rgb=np.array(t[3:], np.float32).reshape(h, w, 3)))
+            elif kind == "texture_checker":
+                v = [float(x) for x in t[1:]]
+                out.append(dict(kind="texture_checker", scale=v[0], c1=v[1:4], c2=v[4:7]))
+            elif kind == "mesh_uv":
+                nuv, nt = int(t[1]), int(t[2])
+                uv = np.array(t[3:3 + 2 * nuv], np.float32).reshape(nuv, 2)
+                uvidx = np.array(t[3 + 2 * nuv:3 + 2 * nuv + 3 * nt], np.int32).reshape(nt, 3)
+                out.append(dict(kind="mesh_uv", uv=uv, uvidx=uvidx))
             elif kind == "rect":
                 out.append(dict(kind="rect", mat=int(t[1]), flags=int(t[2]), axis=int(t[3]),
                                 v=[float(x) for x in t[4:9]]))
@@ -88,7 +99,15 @@ def dump_to_spec(calls, width=64, height=64, spp_x=2, spp_y=2, max_depth=6):
     s = scenes.SceneSpec(name="scene_api")
     for c in calls:
         if c["kind"] == "material":
-            s.add_material(c["mat_kind"], c["p"])
+            m = s.add_material(c["mat_kind"], c["p"])
+            if c["flags"]:
+                s.material_flags[m] = c["flags"]
+        elif c["kind"] == "texture_image":
+            s.add_texture_image(c["rgb"])
+        elif c["kind"] == "texture_checker":
+            s.add_texture_checker(c["c1"], c["c2"], c["scale"])
+        elif c["kind"] == "mesh_uv":
+            s.set_mesh_uv(len(s.meshes) - 1, c["uv"], c["uvidx"])
         elif c["kind"] == "rect":
             s.add_rect(c["axis"], *c["v"], c["mat"], c["flags"])
         elif c["kind"] == "sphere":

@@ -226,3 +226,122 @@ def test_hdr_reader(driver, tmp_path):
     f = np.where(e == 0, 0.0, np.ldexp(1.0, e - 136)).astype(np.float32)
     want = (rgbe[..., :3].astype(np.float32) * f[..., None])[::-1]  # row 0 = bottom of the image
     assert got.shape == want.shape and (got == want).all()
+
+
+# ---- textures (texture/texture.hpp:20-94) ----
+
+def _write_ppm(path, rgb8):
+    """P6 with the file's first row the image's top row."""
+    h, w, _ = rgb8.shape
+    with open(path, "wb") as f:
+        f.write(b"P6\n# test image\n%d %d\n255\n" % (w, h))
+        f.write(np.ascontiguousarray(rgb8, np.uint8).tobytes())
+
+
+def _write_pfm(path, rgb):
+    """Little-endian PFM, the file's first row the image's bottom row."""
+    h, w, _ = rgb.shape
+    with open(path, "wb") as f:
+        f.write(b"PF\n%d %d\n-1.0\n" % (w, h))
+        f.write(np.ascontiguousarray(rgb, "<f4").tobytes())
+
+
+@pytest.fixture(scope="module")
+def tex_files(driver):
+    exe, _, d = driver
+    rng = np.random.default_rng(11)
+    ppm8 = rng.integers(0, 256, size=(3, 4, 3), dtype=np.uint8)      # top row first
+    pfm = rng.uniform(0.0, 3.0, size=(2, 3, 3)).astype(np.float32)   # bottom row first
+    a, b = str(d / "a.ppm"), str(d / "b.pfm")
+    _write_ppm(a, ppm8)
+    _write_pfm(b, pfm)
+    out = str(d / "tdump.txt")
+    subprocess.run([exe, "tdump", a, b, out], check=True)
+    return a, b, ppm8, pfm, parse_dump(out)
+
+
+def _texread(exe, tmp_path, src, flip, hdr):
+    from pathtracer_amd.film_io import read_pfm
+    out = str(tmp_path / "t.pfm")
+    subprocess.run([exe, "texread", src, str(int(flip)), str(int(hdr)), out], check=True)
+    return read_pfm(out)
+
+
+def test_image_texture_reading(driver, tex_files, tmp_path):
+    """ImageTexture(file, flip_y, isHDR): row 0 the image's bottom row unless
+    flip_y; an 8-bit file read as HDR is (v/255)^2.2 and a float file read
+    as LDR is quantised to (uint8)(x^(1/2.2) 255 + 0.5) / 255, clamped (stb_image's
+    conversions at their default gamma and scale)."""
+    exe = driver[0]
+    a, b, ppm8, pfm, _ = tex_files
+    ldr = ppm8.astype(np.float32) / np.float32(255.0)
+    assert np.array_equal(_texread(exe, tmp_path, a, False, False), ldr[::-1])
+    assert np.array_equal(_texread(exe, tmp_path, a, True, False), ldr)
+    np.testing.assert_allclose(_texread(exe, tmp_path, a, False, True), (ldr.astype(np.float64) ** 2.2)[::-1],
+                               rtol=1e-6)
+    assert np.array_equal(_texread(exe, tmp_path, b, False, True), pfm)
+    assert np.array_equal(_texread(exe, tmp_path, b, True, True), pfm[::-1])
+    q = np.clip(np.floor(pfm.astype(np.float64) ** (1 / 2.2) * 255 + 0.5), 0, 255) / 255
+    np.testing.assert_allclose(_texread(exe, tmp_path, b, False, False), q, atol=1e-6)
+    assert q.max() == 1.0                        # values above 1 clamp to 255
+    r = subprocess.run([exe, "texread", str(tmp_path / "x.png"), "0", "0", str(tmp_path / "o.pfm")],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and ".hdr, .pfm or .ppm" in r.stderr
+
+
+def test_texture_lowering(tex_files):
+    """Textures are lowered once each before the first material that uses
+    them; a textured material carries PT_MAT_TEXTURE(id) and a zero colour;
+    a ConstantTexture lowers to the plain colour."""
+    _, _, ppm8, pfm, calls = tex_files
+    tex = [c for c in calls if c["kind"].startswith("texture")]
+    mats = [c for c in calls if c["kind"] == "material"]
+    assert [t["kind"] for t in tex] == ["texture_checker", "texture_image", "texture_image"]
+    assert tex[0]["scale"] == 10.0
+    np.testing.assert_allclose(tex[0]["c1"], [0.2, 0.3, 0.1], rtol=1e-7)
+    np.testing.assert_allclose(tex[0]["c2"], [0.9, 0.9, 0.9], rtol=1e-7)
+    assert np.array_equal(tex[1]["rgb"], (ppm8.astype(np.float32) / np.float32(255.0))[::-1])
+    assert np.array_equal(tex[2]["rgb"], pfm[::-1])          # flip_y = true, isHDR = true
+    # checker Lambertian, image Lambertian, emitter, constant-texture Lambertian, textured Disney
+    assert [(m["mat_kind"], m["flags"]) for m in mats] == [(1, 1), (1, 1 | 1 << 8), (2, 0), (1, 0), (7, 1 | 2 << 8)]
+    assert mats[0]["p"][:3] == [0.0, 0.0, 0.0]
+    np.testing.assert_allclose(mats[3]["p"][:3], [0.5, 0.6, 0.7], rtol=1e-7)
+    np.testing.assert_allclose(mats[4]["p"][3:15], [0, 0.3, 0, 0.5, 0.4, 0, 0, 0, 0.5, 0, 0, 1.5], rtol=1e-7)
+
+
+def test_mesh_uv_lowering(tex_files):
+    """MeshData uvs pass through; a rotated textured rect (two triangles)
+    takes the rect's corner uvs; a rotated mesh without uvs takes its planar
+    fallback in object space (triangleMesh.cpp:10-18, 38-51)."""
+    calls = tex_files[4]
+    seq = [c for c in calls if c["kind"] in ("mesh", "mesh_uv")]
+    kinds = [c["kind"] for c in seq]
+    assert kinds == ["mesh", "mesh_uv", "mesh", "mesh", "mesh_uv", "mesh", "mesh_uv"]
+    quad_uv = seq[1]
+    assert quad_uv["uv"].tolist() == [[0, 0], [1, 0], [1, 1], [0, 1]]
+    assert quad_uv["uvidx"].tolist() == [[0, 1, 2], [0, 2, 3]]
+    # rotated tetra: object-space positions, box padded by 1e-3
+    P = np.array([[0, 0, 0], [0.4, 0, 0.1], [0.2, 0, 0.5], [0.2, 0.45, 0.2]], np.float64)
+    F = [[0, 2, 1], [0, 1, 3], [1, 2, 3], [2, 0, 3]]
+    lo, hi = P.min(0) - 1e-3, P.max(0) + 1e-3
+    uv = seq[4]["uv"]
+    assert seq[4]["uvidx"].tolist() == [[0, 1, 2], [3, 4, 5], [6, 7, 8], [9, 10, 11]]
+    for k, f in enumerate(F):
+        n = np.abs(np.cross(P[f[1]] - P[f[0]], P[f[2]] - P[f[0]]))
+        au = 1 if (n[0] > n[1] and n[0] > n[2]) else 0
+        av = 2 if au == 1 else (2 if n[1] > n[2] else 1)
+        for c in range(3):
+            q = P[f[c]]
+            np.testing.assert_allclose(uv[3 * k + c], [(q[au] - lo[au]) / (hi[au] - lo[au]),
+                                                       (q[av] - lo[av]) / (hi[av] - lo[av])], atol=1e-6)
+    # the rotated rect's corners (d0, d1) = (0,0), (1,0), (0,1), (1,1) of its extents
+    assert seq[6]["uv"].tolist() == [[0, 0], [1, 0], [0, 1], [1, 1]]
+    assert seq[6]["uvidx"].tolist() == seq[5]["idx"].tolist()
+
+
+def test_unsupported_textures_are_errors(driver, tmp_path):
+    exe, obj, _ = driver
+    r = subprocess.run([exe, "noise", obj, str(tmp_path / "x.txt")], capture_output=True, text=True)
+    assert r.returncode == 1 and "NoiseTexture has no GPU lowering" in r.stderr
+    r = subprocess.run([exe, "temit", "x"], capture_output=True, text=True)
+    assert r.returncode == 1 and "constant colours only" in r.stderr
