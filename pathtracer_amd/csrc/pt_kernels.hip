@@ -340,135 +340,265 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
   uint32_t *cout = w.live(qi ^ 1, part);
   __shared__ uint32_t o_cnt[4][16], o_first[16];
   __shared__ uint16_t o_perm[256];
-  // The block reads its tiles of 256 entries into an LDS list and shades the
-  // list 256 entries at a time.  Under the reference estimator and NEE+MIS
-  // (without the material-key order) a miss is shaded as it is read, a few
-  // instructions, and only hits join the list, so that the material code
-  // runs on full waves rather than on the hits among a tile's misses.
-  __shared__ uint32_t o_list[512];
-  __shared__ uint32_t o_n;
+  // The block reads its tiles of 256 entries into two LDS lists and shades
+  // each list 256 entries at a time.  Under the reference estimator and
+  // NEE+MIS (without the material-key order) a miss is shaded as it is read,
+  // a few instructions, and only hits join a list; hair hits join the second
+  // list.  The material code then runs on full waves rather than on the hits
+  // among a tile's misses, and a wave runs the hair BSDF or the others, not
+  // both.  Shading of the Disney + hair grid frame: 88.6 ms with every entry
+  // in place, 86.4 with misses left out, 76.4 with hair apart; the glass /
+  // HDRI frame: 25.1 -> 21.4 ms.
+  __shared__ uint32_t o_list[2][512];
+  __shared__ uint32_t o_n[2];
   constexpr bool SPLIT = EST == PT_EST_REFERENCE_NEE || EST == PT_EST_NEE_MIS;
   const bool split = SPLIT && !w.sort_keys;  // uniform
-  if (threadIdx.x == 0) o_n = 0u;
+  if (threadIdx.x < 2u) o_n[threadIdx.x] = 0u;
   __syncthreads();
   for (uint32_t t0 = (blockIdx.x / PT_PARTS) * 256u;; t0 += nb * 256u) {
     const bool last = t0 + nb * 256u >= n;  // the block's last tile (uniform)
     if (t0 < n) {
       const uint32_t e = t0 + threadIdx.x;
       bool push = e < n;
-      if (split && push && in.prim[base + e] < 0) {
-        shade_miss<EST, BATCH>(s0, rp0, w, in, base + e, d, b);
-        push = false;
+      uint32_t cls = 0u;
+      if (split && push) {
+        const int pr = in.prim[base + e];
+        if (pr < 0) {
+          shade_miss<EST, BATCH>(s0, rp0, w, in, base + e, d, b);
+          push = false;
+        } else if (PT_IN(MS, PT_MAT_HAIR)) {
+          const DevScene &s = BATCH ? b.scenes[b.slot_inst[in.slot[base + e]]] : s0;
+          if (s.mat_kinds & (1u << PT_MAT_HAIR))
+            cls = s.mats[s.prim_info[pr].x & 0xffffffu].kind == PT_MAT_HAIR ? 1u : 0u;
+        }
       }
-      const unsigned long long m = __ballot(push);
-      uint32_t wb = 0u;
-      if (__lane_id() == 0u && m) wb = atomicAdd(&o_n, (uint32_t)__popcll(m));
-      wb = __shfl(wb, 0);
-      if (push) o_list[wb + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull))] = e;
+#pragma unroll
+      for (uint32_t c = 0; c < 2u; ++c) {
+        const bool pc = push && cls == c;
+        const unsigned long long m = __ballot(pc);
+        uint32_t wb = 0u;
+        if (__lane_id() == 0u && m) wb = atomicAdd(&o_n[c], (uint32_t)__popcll(m));
+        wb = __shfl(wb, 0);
+        if (pc) o_list[c][wb + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull))] = e;
+      }
     }
     __syncthreads();
-    uint32_t cnt = o_n;
+    const uint32_t cnt0 = o_n[0], cnt1 = o_n[1];
     __syncthreads();
-    while (cnt >= 256u || (last && cnt > 0u)) {
-      uint32_t jl = threadIdx.x;
-      if (w.sort_keys) {  // block-uniform
-        uint32_t key = 15u;
-        if (jl < cnt) {
-          const uint32_t e = base + o_list[jl];
-          const int pr = in.prim[e];
-          key = 0u;
-          if (pr >= 0) {
-            const DevScene &s = BATCH ? b.scenes[b.slot_inst[in.slot[e]]] : s0;
-            key = s.mats[s.prim_info[pr].x & 0xffffffu].kind & 15u;
-          }
-        }
-        jl = tile_order16(key, o_cnt, o_first, o_perm);
-      }
-      const bool active = jl < cnt;
-      const uint32_t i = base + (active ? o_list[jl] : 0u);
-      bool cont = false;
-      uint32_t slot = 0;
-      f3 p = s3(0.f), wi = s3(0.f), beta = s3(0.f), pprev = s3(0.f), pbeta = s3(0.f);
-      float ppdf = 0.f;
-      uint32_t fl = 0u;
-      if (active) {
-        slot = in.slot[i];
-        // the path's scene and render parameters (batched: its instance's)
-        const uint32_t inst = BATCH ? b.slot_inst[slot] : 0u;
-        const DevScene &s = BATCH ? b.scenes[inst] : s0;
-        const RenderParams &rp = BATCH ? b.params[inst] : rp0;
-        const uint32_t D = rp.max_depth;
-        const int N = s.n_lights;
-        uint32_t pixel, k;
-        if (BATCH) {
-          pixel = b.slot_pix[slot];
-          k = b.slot_k[slot];
-        } else {
-          uint64_t q = rp.chunk_first + slot;
-          pixel = (uint32_t)(q / rp.spp_count);
-          k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
-        }
-        uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
-        uint32_t g = 1u + k + rp.spp * (pj + rp.height * pi);
-        f3 o = mk3(in.ox[i], in.oy[i], in.oz[i]);
-        f3 dir = mk3(in.dx[i], in.dy[i], in.dz[i]);
-        float t = in.t[i];
-        int prim = in.prim[i];
-        beta = mk3(in.bx[i], in.by[i], in.bz[i]);
-        fl = in.flags[i];
-        ppdf = in.ppdf[i];
-        pprev = mk3(in.px[i], in.py[i], in.pz[i]);
-        f3 L = s3(0.f);  // radiance this vertex adds (emission); NEE goes through shadow rays
-        bool prev_spec = fl & 1u;
-        bool done = false;
-        // guided: the training record of vertex d - 1, completed here (the slot
-        // holds everything up to and including that vertex's direct light)
-        float *prec = nullptr;
-        if constexpr (EST == PT_EST_GUIDED) {
-          if (w.grec && d >= 1u && d - 1u < w.grec_verts && slot % w.grec_stride == 0u) {
-            prec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + (d - 1u)) * PT_G_REC;
-            prec[13] = w.lx[slot];
-            prec[14] = w.ly[slot];
-            prec[15] = w.lz[slot];
-            prec[17] = __uint_as_float(w.gpass);
-            prec[18] = prec[19] = prec[20] = 0.f;
-          }
-        }
-        if constexpr (EST == PT_EST_RESERVOIR) {
-          // getRayColor of maxdepth_reservoir_di.hpp:9-131
-          const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
-          const uint32_t ns = min(rp.light_samples, 4u);
-          (void)prev_spec;
-          // the previous vertex's resampled direct light, now that its candidates' visibility is known
-          f3 pend = s3(0.f);
-          if (fl & 2u) {
-            const uint32_t dp = d - 1u;
-            f3 sample = s3(0.f);
-            float W = 0.f;
-            for (uint32_t si = 0; si < ns; ++si) {
-              const size_t c0 = (size_t)(3u * si) * w.cap + slot;
-              f3 c = mk3(w.rc[c0], w.rc[c0 + w.cap], w.rc[c0 + 2 * (size_t)w.cap]);
-              f3 cand = scl3(c, w.rvis[4u * slot + si] ? 1.f : 0.f);
-              float wgt = len3(cand);
-              W += wgt;
-              if (pt_rng(g, rslot(dp, SLOT_RIS + si), rp.seed) < wgt / W) sample = cand;
+    for (uint32_t c = 0; c < 2u; ++c) {
+      uint32_t cnt = c ? cnt1 : cnt0;
+      while (cnt >= 256u || (last && cnt > 0u)) {
+        uint32_t jl = threadIdx.x;
+        if (w.sort_keys) {  // block-uniform
+          uint32_t key = 15u;
+          if (jl < cnt) {
+            const uint32_t e = base + o_list[c][jl];
+            const int pr = in.prim[e];
+            key = 0u;
+            if (pr >= 0) {
+              const DevScene &s = BATCH ? b.scenes[b.slot_inst[in.slot[e]]] : s0;
+              key = s.mats[s.prim_info[pr].x & 0xffffffu].kind & 15u;
             }
-            pend = mul3(mk3(in.pbx[i], in.pby[i], in.pbz[i]), scl3(sample, W / (float)ns));
           }
-          SurfHit h;
-          const pt_material_desc *mat = nullptr;
-          bool drop = false;
-          if (prim >= 0) {
-            h = surface_tex<MS>(s, prim, o, dir, t);
-            mat = &s.mats[h.mat];
-            drop = d > 0u && mat->kind == PT_MAT_EMITTER;  // :113-115: the vertex returns 0
+          jl = tile_order16(key, o_cnt, o_first, o_perm);
+        }
+        const bool active = jl < cnt;
+        const uint32_t i = base + (active ? o_list[c][jl] : 0u);
+        bool cont = false;
+        uint32_t slot = 0;
+        f3 p = s3(0.f), wi = s3(0.f), beta = s3(0.f), pprev = s3(0.f), pbeta = s3(0.f);
+        float ppdf = 0.f;
+        uint32_t fl = 0u;
+        if (active) {
+          slot = in.slot[i];
+          // the path's scene and render parameters (batched: its instance's)
+          const uint32_t inst = BATCH ? b.slot_inst[slot] : 0u;
+          const DevScene &s = BATCH ? b.scenes[inst] : s0;
+          const RenderParams &rp = BATCH ? b.params[inst] : rp0;
+          const uint32_t D = rp.max_depth;
+          const int N = s.n_lights;
+          uint32_t pixel, k;
+          if (BATCH) {
+            pixel = b.slot_pix[slot];
+            k = b.slot_k[slot];
+          } else {
+            uint64_t q = rp.chunk_first + slot;
+            pixel = (uint32_t)(q / rp.spp_count);
+            k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
           }
-          fl = 0u;
-          if (!drop) {
-            L = add3(L, pend);
-            if (prim < 0 || d >= D) {
+          uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
+          uint32_t g = 1u + k + rp.spp * (pj + rp.height * pi);
+          f3 o = mk3(in.ox[i], in.oy[i], in.oz[i]);
+          f3 dir = mk3(in.dx[i], in.dy[i], in.dz[i]);
+          float t = in.t[i];
+          int prim = in.prim[i];
+          beta = mk3(in.bx[i], in.by[i], in.bz[i]);
+          fl = in.flags[i];
+          ppdf = in.ppdf[i];
+          pprev = mk3(in.px[i], in.py[i], in.pz[i]);
+          f3 L = s3(0.f);  // radiance this vertex adds (emission); NEE goes through shadow rays
+          bool prev_spec = fl & 1u;
+          bool done = false;
+          // guided: the training record of vertex d - 1, completed here (the slot
+          // holds everything up to and including that vertex's direct light)
+          float *prec = nullptr;
+          if constexpr (EST == PT_EST_GUIDED) {
+            if (w.grec && d >= 1u && d - 1u < w.grec_verts && slot % w.grec_stride == 0u) {
+              prec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + (d - 1u)) * PT_G_REC;
+              prec[13] = w.lx[slot];
+              prec[14] = w.ly[slot];
+              prec[15] = w.lz[slot];
+              prec[17] = __uint_as_float(w.gpass);
+              prec[18] = prec[19] = prec[20] = 0.f;
+            }
+          }
+          if constexpr (EST == PT_EST_RESERVOIR) {
+            // getRayColor of maxdepth_reservoir_di.hpp:9-131
+            const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
+            const uint32_t ns = min(rp.light_samples, 4u);
+            (void)prev_spec;
+            // the previous vertex's resampled direct light, now that its candidates' visibility is known
+            f3 pend = s3(0.f);
+            if (fl & 2u) {
+              const uint32_t dp = d - 1u;
+              f3 sample = s3(0.f);
+              float W = 0.f;
+              for (uint32_t si = 0; si < ns; ++si) {
+                const size_t c0 = (size_t)(3u * si) * w.cap + slot;
+                f3 c = mk3(w.rc[c0], w.rc[c0 + w.cap], w.rc[c0 + 2 * (size_t)w.cap]);
+                f3 cand = scl3(c, w.rvis[4u * slot + si] ? 1.f : 0.f);
+                float wgt = len3(cand);
+                W += wgt;
+                if (pt_rng(g, rslot(dp, SLOT_RIS + si), rp.seed) < wgt / W) sample = cand;
+              }
+              pend = mul3(mk3(in.pbx[i], in.pby[i], in.pbz[i]), scl3(sample, W / (float)ns));
+            }
+            SurfHit h;
+            const pt_material_desc *mat = nullptr;
+            bool drop = false;
+            if (prim >= 0) {
+              h = surface_tex<MS>(s, prim, o, dir, t);
+              mat = &s.mats[h.mat];
+              drop = d > 0u && mat->kind == PT_MAT_EMITTER;  // :113-115: the vertex returns 0
+            }
+            fl = 0u;
+            if (!drop) {
+              L = add3(L, pend);
+              if (prim < 0 || d >= D) {
+                L = add3(L, mul3(beta, bg));
+              } else {
+                p = h.p;
+                float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
+                float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
+                float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
+                float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
+                f3 wo = neg3(norm3(dir));
+                Scatter sc;
+                if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
+                  L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+                } else {
+                  float pdf;
+                  if (sc.flags & BX_SPECULAR) {  // no pdf of its own: CosinePDF(n) (:39-41)
+                    wi = cosine_dir(h.sp.n, u0, u1);
+                    pdf = cosine_pdf(h.sp.n, wi);
+                  } else {
+                    wi = norm3(sc.wi);
+                    pdf = bsdf_pdf<MS>(*mat, wo, wi, h.sp);
+                  }
+                  const float cs = fabsf(dot3(h.sp.n, wi));
+                  if (N > 0 && ns > 0u) {
+                    for (uint32_t si = 0; si < ns; ++si) {
+                      uint32_t ii = si / rp.ly, jj = si % rp.ly;
+                      float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
+                      float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
+                      float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
+                      float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
+                      int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
+                      const DevLight &Lg = s.lights[li];
+                      f3 nl;
+                      f3 lp = light_point(Lg, ox, oy, nl);
+                      f3 ld = sub3(lp, p);
+                      f3 lwi = norm3(ld);
+                      float len = len3(ld);
+                      float noi = fmaxf(dot3(h.sp.n, lwi), 0.f);
+                      float nol = fmaxf(dot3(nl, neg3(lwi)), 0.f);
+                      float G = nol * noi / dot3(ld, ld);
+                      f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+                      f3 le = Lg.type == LIGHT_ENV ? env_lookup_uv(s, ox, oy) : Lg.le;
+                      float area = Lg.type == LIGHT_ENV ? INFINITY : Lg.area;
+                      f3 c = scl3(mul3(f, le), area * G);
+                      const size_t c0 = (size_t)(3u * si) * w.cap + slot;
+                      w.rc[c0] = c.x; w.rc[c0 + w.cap] = c.y; w.rc[c0 + 2 * (size_t)w.cap] = c.z;
+                      w.rvis[4u * slot + si] = 0u;
+                      // only the visibility of a non-zero candidate can matter
+                      if (c.x != 0.f || c.y != 0.f || c.z != 0.f || !finite3(c))
+                        emit_shadow(w, p, lwi, len - 0.001f, c, 4u * slot + si);
+                    }
+                    pbeta = beta;
+                    fl = 2u;
+                  }
+                  beta = scl3(mul3(beta, sc.att), cs / pdf);
+                  cont = d + 1u <= D;  // vertex D only decides whether this vertex counts
+                }
+              }
+            }
+          } else if constexpr (EST == PT_EST_VOLUME) {
+            // getRayColor of maxdepth_volume.hpp:72-118 with estimate_l_scatter1 (:10-70):
+            // single scattering in the global medium; the path ends here
+            if (s.has_medium) {
+              const f3 sa = mk3(s.med_sa[0], s.med_sa[1], s.med_sa[2]);
+              const f3 ss = mk3(s.med_ss[0], s.med_ss[1], s.med_ss[2]);
+              const f3 st = add3(sa, ss);
+              const float smax = fmaxf(st.x, fmaxf(st.y, st.z));
+              const float u = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 0u);
+              const float tf = -logf(1.f - u) / smax;
+              const float th = prim >= 0 ? t : 3.402823466e+38f;  // HitRecord::t = FLT_MAX on a miss
+              if (tf < th) {
+                const f3 ps = add3(o, scl3(dir, tf));
+                const f3 wo = neg3(norm3(dir));
+                const uint32_t ns = min(rp.light_samples, 4u);
+                // albedo * L_s1 / sigma_t: the reference divides by max(sigma_t) once more than
+                // the free-flight pdf asks for
+                const f3 alb = mk3(ss.x / st.x, ss.y / st.y, ss.z / st.z);
+                for (uint32_t si = 0; N > 0 && si < ns; ++si) {
+                  const int li = (int)(pt_rng(g, rslot(0u, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
+                  const DevLight &Lg = s.lights[li];
+                  f3 nl;
+                  const f3 lp = light_point(Lg, pt_rng(g, rslot(0u, SLOT_NEE_OFF + 2u * si), rp.seed),
+                                            pt_rng(g, rslot(0u, SLOT_NEE_OFF + 2u * si + 1u), rp.seed), nl);
+                  const f3 dv = sub3(ps, lp);
+                  const f3 dl = norm3(dv);
+                  const float dist = len3(dv);
+                  if (dist < 1e-3f) continue;  // gl::epsilon
+                  const f3 le = (dot3(nl, dl) > 0.f) ? Lg.le : s3(0.f);
+                  const float pdf = 1.f / (Lg.area * (float)N);
+                  const float cosl = fabsf(dot3(nl, dl));
+                  const float ph = hg_phase(dot3(wo, neg3(dl)), s.med_g);
+                  const f3 T = mk3(expf(-st.x * dist), expf(-st.y * dist), expf(-st.z * dist));
+                  f3 c = divs3(divs3(mul3(scl3(le, cosl * ph), T), pdf), dist * dist);
+                  c = divs3(mul3(alb, divs3(c, (float)ns)), smax);
+                  if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c))
+                    emit_shadow(w, lp, dl, dist - 0.001f, c, slot);
+                }
+              } else if (prim >= 0) {
+                const SurfHit h = surface_tex<MS>(s, prim, o, dir, t);
+                const pt_material_desc &m = s.mats[h.mat];
+                if (m.kind == PT_MAT_EMITTER) L = add3(L, bsdf_emit(m, dir, h.sp));
+              }
+            }
+            cont = false;
+            (void)prev_spec;
+          } else if constexpr (EST == PT_EST_NAIVE || EST == PT_EST_ROULETTE) {
+            // maxdepth_naive.hpp:9-75 / roulette_naive.hpp:9-78: no NEE; misses and
+            // the naive depth limit return bg_color (the environment is not consulted)
+            const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
+            const int NA = N - (s.has_env ? 1 : 0);  // area lights come first
+            (void)prev_spec;
+            if (prim < 0) {
               L = add3(L, mul3(beta, bg));
             } else {
+              SurfHit h = surface_tex<MS>(s, prim, o, dir, t);
+              const pt_material_desc *mat = &s.mats[h.mat];
               p = h.p;
               float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
               float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
@@ -479,412 +609,301 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
               if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
                 L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
               } else {
-                float pdf;
-                if (sc.flags & BX_SPECULAR) {  // no pdf of its own: CosinePDF(n) (:39-41)
-                  wi = cosine_dir(h.sp.n, u0, u1);
-                  pdf = cosine_pdf(h.sp.n, wi);
-                } else {
-                  wi = norm3(sc.wi);
-                  pdf = bsdf_pdf<MS>(*mat, wo, wi, h.sp);
+                float q = 1.f;
+                bool alive = true;
+                if (EST == PT_EST_ROULETTE) {
+                  q = fminf(fmaxf(maxc3(sc.att), 0.5f), 1.f);
+                  alive = !(pt_rng(g, rslot(d, SLOT_RR), rp.seed) > q);
                 }
-                const float cs = fabsf(dot3(h.sp.n, wi));
-                if (N > 0 && ns > 0u) {
-                  for (uint32_t si = 0; si < ns; ++si) {
-                    uint32_t ii = si / rp.ly, jj = si % rp.ly;
-                    float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
-                    float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
-                    float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
-                    float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
-                    int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
-                    const DevLight &Lg = s.lights[li];
-                    f3 nl;
-                    f3 lp = light_point(Lg, ox, oy, nl);
-                    f3 ld = sub3(lp, p);
-                    f3 lwi = norm3(ld);
-                    float len = len3(ld);
-                    float noi = fmaxf(dot3(h.sp.n, lwi), 0.f);
-                    float nol = fmaxf(dot3(nl, neg3(lwi)), 0.f);
-                    float G = nol * noi / dot3(ld, ld);
-                    f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
-                    f3 le = Lg.type == LIGHT_ENV ? env_lookup_uv(s, ox, oy) : Lg.le;
-                    float area = Lg.type == LIGHT_ENV ? INFINITY : Lg.area;
-                    f3 c = scl3(mul3(f, le), area * G);
-                    const size_t c0 = (size_t)(3u * si) * w.cap + slot;
-                    w.rc[c0] = c.x; w.rc[c0 + w.cap] = c.y; w.rc[c0 + 2 * (size_t)w.cap] = c.z;
-                    w.rvis[4u * slot + si] = 0u;
-                    // only the visibility of a non-zero candidate can matter
-                    if (c.x != 0.f || c.y != 0.f || c.z != 0.f || !finite3(c))
-                      emit_shadow(w, p, lwi, len - 0.001f, c, 4u * slot + si);
-                  }
-                  pbeta = beta;
-                  fl = 2u;
-                }
-                beta = scl3(mul3(beta, sc.att), cs / pdf);
-                cont = d + 1u <= D;  // vertex D only decides whether this vertex counts
-              }
-            }
-          }
-        } else if constexpr (EST == PT_EST_VOLUME) {
-          // getRayColor of maxdepth_volume.hpp:72-118 with estimate_l_scatter1 (:10-70):
-          // single scattering in the global medium; the path ends here
-          if (s.has_medium) {
-            const f3 sa = mk3(s.med_sa[0], s.med_sa[1], s.med_sa[2]);
-            const f3 ss = mk3(s.med_ss[0], s.med_ss[1], s.med_ss[2]);
-            const f3 st = add3(sa, ss);
-            const float smax = fmaxf(st.x, fmaxf(st.y, st.z));
-            const float u = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 0u);
-            const float tf = -logf(1.f - u) / smax;
-            const float th = prim >= 0 ? t : 3.402823466e+38f;  // HitRecord::t = FLT_MAX on a miss
-            if (tf < th) {
-              const f3 ps = add3(o, scl3(dir, tf));
-              const f3 wo = neg3(norm3(dir));
-              const uint32_t ns = min(rp.light_samples, 4u);
-              // albedo * L_s1 / sigma_t: the reference divides by max(sigma_t) once more than
-              // the free-flight pdf asks for
-              const f3 alb = mk3(ss.x / st.x, ss.y / st.y, ss.z / st.z);
-              for (uint32_t si = 0; N > 0 && si < ns; ++si) {
-                const int li = (int)(pt_rng(g, rslot(0u, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
-                const DevLight &Lg = s.lights[li];
-                f3 nl;
-                const f3 lp = light_point(Lg, pt_rng(g, rslot(0u, SLOT_NEE_OFF + 2u * si), rp.seed),
-                                          pt_rng(g, rslot(0u, SLOT_NEE_OFF + 2u * si + 1u), rp.seed), nl);
-                const f3 dv = sub3(ps, lp);
-                const f3 dl = norm3(dv);
-                const float dist = len3(dv);
-                if (dist < 1e-3f) continue;  // gl::epsilon
-                const f3 le = (dot3(nl, dl) > 0.f) ? Lg.le : s3(0.f);
-                const float pdf = 1.f / (Lg.area * (float)N);
-                const float cosl = fabsf(dot3(nl, dl));
-                const float ph = hg_phase(dot3(wo, neg3(dl)), s.med_g);
-                const f3 T = mk3(expf(-st.x * dist), expf(-st.y * dist), expf(-st.z * dist));
-                f3 c = divs3(divs3(mul3(scl3(le, cosl * ph), T), pdf), dist * dist);
-                c = divs3(mul3(alb, divs3(c, (float)ns)), smax);
-                if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c))
-                  emit_shadow(w, lp, dl, dist - 0.001f, c, slot);
-              }
-            } else if (prim >= 0) {
-              const SurfHit h = surface_tex<MS>(s, prim, o, dir, t);
-              const pt_material_desc &m = s.mats[h.mat];
-              if (m.kind == PT_MAT_EMITTER) L = add3(L, bsdf_emit(m, dir, h.sp));
-            }
-          }
-          cont = false;
-          (void)prev_spec;
-        } else if constexpr (EST == PT_EST_NAIVE || EST == PT_EST_ROULETTE) {
-          // maxdepth_naive.hpp:9-75 / roulette_naive.hpp:9-78: no NEE; misses and
-          // the naive depth limit return bg_color (the environment is not consulted)
-          const f3 bg = mk3(rp.bg[0], rp.bg[1], rp.bg[2]);
-          const int NA = N - (s.has_env ? 1 : 0);  // area lights come first
-          (void)prev_spec;
-          if (prim < 0) {
-            L = add3(L, mul3(beta, bg));
-          } else {
-            SurfHit h = surface_tex<MS>(s, prim, o, dir, t);
-            const pt_material_desc *mat = &s.mats[h.mat];
-            p = h.p;
-            float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
-            float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
-            float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
-            float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
-            f3 wo = neg3(norm3(dir));
-            Scatter sc;
-            if (!bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc)) {
-              L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
-            } else {
-              float q = 1.f;
-              bool alive = true;
-              if (EST == PT_EST_ROULETTE) {
-                q = fminf(fmaxf(maxc3(sc.att), 0.5f), 1.f);
-                alive = !(pt_rng(g, rslot(d, SLOT_RR), rp.seed) > q);
-              }
-              if (alive) {
-                if (sc.flags & BX_SPECULAR) {
-                  beta = mul3(beta, divs3(sc.att, q));
-                  wi = sc.wi;
-                } else {
-                  // MixedPDF of light_samples HittablePDFs and the material's pdf
-                  int li[4] = {0, 0, 0, 0};
-                  const uint32_t nl = NA > 0 ? min(rp.light_samples, 4u) : 0u;
-                  for (uint32_t si = 0; si < nl; ++si)
-                    li[si] = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)NA) % NA;
-                  const uint32_t ncomp = nl + 1u;
-                  const uint32_t comp =
-                      (uint32_t)floorf(pt_rng(g, rslot(d, SLOT_MIX), rp.seed) * (float)ncomp) % ncomp;
-                  if (comp < nl) {
-                    float e0 = pt_rng(g, rslot(d, SLOT_NEE_OFF), rp.seed);
-                    float e1 = pt_rng(g, rslot(d, SLOT_NEE_OFF + 1u), rp.seed);
-                    int lc = li[0];
-                    for (uint32_t si = 1; si < nl; ++si)
-                      if (si == comp) lc = li[si];
-                    wi = norm3(shape_sample(s.lights[lc], p, e0, e1));
+                if (alive) {
+                  if (sc.flags & BX_SPECULAR) {
+                    beta = mul3(beta, divs3(sc.att, q));
+                    wi = sc.wi;
                   } else {
-                    wi = norm3(sc.wi);
-                  }
-                  const float wgt = 1.f / (float)ncomp;
-                  float pdf = 0.f;
-                  for (uint32_t si = 0; si < nl; ++si) pdf += wgt * shape_pdf(s.lights[li[si]], p, wi);
-                  pdf += wgt * bsdf_pdf<MS>(*mat, wo, wi, h.sp);
-                  f3 f = divs3(bsdf_f<MS>(*mat, wo, wi, h.sp), q);
-                  float cs = fabsf(dot3(h.sp.n, wi));
-                  beta = scl3(mul3(beta, f), cs / pdf);
-                }
-                if (d + 1u < D) {
-                  cont = beta.x != 0.f || beta.y != 0.f || beta.z != 0.f;
-                } else if (EST == PT_EST_NAIVE) {
-                  L = add3(L, mul3(beta, bg));  // getRayColor(.., 0) returns bg_color
-                }
-                fl = 0u;
-              }
-            }
-          }
-        } else {
-          if (prim < 0) {
-            if (EST == 0) {
-              L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
-            } else if (prev_spec) {
-              if (d < D) L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
-            } else if (s.has_env) {
-              float pl = env_pdf(s, dir) / (float)N;
-              const f3 le = mul3(beta, env_lookup(s, dir));
-              const float wm = power_w(ppdf, pl);
-              L = add3(L, scl3(le, wm));
-              if (EST == PT_EST_GUIDED && prec) {  // the target is the unweighted radiance
-                prec[18] = le.x * (1.f - wm); prec[19] = le.y * (1.f - wm); prec[20] = le.z * (1.f - wm);
-              }
-            } else {
-              L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
-            }
-            done = true;
-          }
-          SurfHit h;
-          const pt_material_desc *mat = nullptr;
-          if (!done) {
-            h = surface_tex<MS>(s, prim, o, dir, t);
-            mat = &s.mats[h.mat];
-            if (mat->kind == PT_MAT_EMITTER) {
-              if (EST == 0) {
-                // A non-specular predecessor excludes emitter hits (maxdepth_nee.hpp:246-247);
-                // camera and specular chains take DiffuseEmitter::emit.
-                if (prev_spec) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
-              } else if (prev_spec || h.light < 0) {
-                if (d < D) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
-              } else {
-                const DevLight &Lg = s.lights[h.light];
-                // one-sided QuadLight/SphereLight::L_emit (light.hpp:157-165, 296-303): the quad's
-                // normal follows its vertex order, the sphere's points outwards
-                f3 nl = (Lg.type == LIGHT_QUAD) ? Lg.n : h.ng;
-                f3 le = (dot3(nl, neg3(norm3(dir))) > 0.f) ? Lg.le : s3(0.f);
-                float pl = light_pdf_sa(s, Lg, pprev, h.p) / (float)N;
-                const float wm = power_w(ppdf, pl);
-                L = add3(L, scl3(mul3(beta, le), wm));
-                if (EST == PT_EST_GUIDED && prec) {  // the target is the unweighted radiance
-                  const f3 r = scl3(mul3(beta, le), 1.f - wm);
-                  prec[18] = r.x; prec[19] = r.y; prec[20] = r.z;
-                }
-              }
-              done = true;
-            }
-          }
-          if (!done && d >= D) done = true;
-          if (!done) {
-            p = h.p;
-            float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
-            float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
-            float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
-            float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
-            f3 wo = neg3(norm3(dir));
-            Scatter sc;
-            // guided: the learned mixture of this point (c = 0: not guided); the
-            // technique (guide with probability c, else the BSDF) is drawn first
-            const float *mix = nullptr;
-            float gc = 0.f;
-            bool use_guide = false;
-            if constexpr (EST == PT_EST_GUIDED) {
-              mix = w.gmix + (size_t)i * PT_G_MIX;
-              gc = d < w.gverts ? mix[40] : 0.f;
-              use_guide = gc > 0.f && pt_rng(g, rslot(d, SLOT_GUIDE), rp.seed) < gc;
-            }
-            bool scat;
-            if (use_guide) {
-              sc.wi = g_sample(mix, pt_rng(g, rslot(d, SLOT_GUIDE + 1u), rp.seed),
-                               pt_rng(g, rslot(d, SLOT_GUIDE + 2u), rp.seed),
-                               pt_rng(g, rslot(d, SLOT_GUIDE + 3u), rp.seed));
-              sc.flags = 0u;
-              sc.att = s3(0.f);
-              sc.pdf = 0.f;
-              scat = true;
-            } else {
-              scat = bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc);
-            }
-            if (!scat) {
-              L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
-              done = true;
-            } else if (sc.flags & BX_SPECULAR) {
-              beta = mul3(beta, sc.att);
-              wi = sc.wi;
-              cont = d + 1u < D;
-              fl = 1u;
-            } else {
-              f3 wn = norm3(sc.wi);
-              float cs = fabsf(dot3(h.sp.n, wn));
-              if (EST == 0) {
-                // NEE with light_samples jittered offsets (maxdepth_nee.hpp:90-232)
-                if (N > 0) {
-                  float inv_ns = 1.f / (float)rp.light_samples;
-                  for (uint32_t si = 0; si < rp.light_samples && si < 4u; ++si) {
-                    uint32_t ii = si / rp.ly, jj = si % rp.ly;
-                    float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
-                    float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
-                    float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
-                    float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
-                    int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
-                    const DevLight &Lg = s.lights[li];
-                    f3 nl;
-                    f3 lp = light_point(Lg, ox, oy, nl);
-                    f3 c = s3(0.f), lwi = s3(0.f);
-                    float tmax = 0.f;
-                    bool valid = false;
-                    if (Lg.type == LIGHT_ENV) {
-                      lwi = norm3(sub3(lp, p));
-                      float pdf = (1.f / (float)N) * (1.f / (4.f * PT_PI));
-                      float fac = fmaxf(0.f, dot3(h.sp.n, lwi));
-                      f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
-                      c = divs3(scl3(mul3(f, env_lookup(s, lwi)), fac), pdf);
-                      tmax = 3.402823466e+38f;
-                      valid = true;
+                    // MixedPDF of light_samples HittablePDFs and the material's pdf
+                    int li[4] = {0, 0, 0, 0};
+                    const uint32_t nl = NA > 0 ? min(rp.light_samples, 4u) : 0u;
+                    for (uint32_t si = 0; si < nl; ++si)
+                      li[si] = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)NA) % NA;
+                    const uint32_t ncomp = nl + 1u;
+                    const uint32_t comp =
+                        (uint32_t)floorf(pt_rng(g, rslot(d, SLOT_MIX), rp.seed) * (float)ncomp) % ncomp;
+                    if (comp < nl) {
+                      float e0 = pt_rng(g, rslot(d, SLOT_NEE_OFF), rp.seed);
+                      float e1 = pt_rng(g, rslot(d, SLOT_NEE_OFF + 1u), rp.seed);
+                      int lc = li[0];
+                      for (uint32_t si = 1; si < nl; ++si)
+                        if (si == comp) lc = li[si];
+                      wi = norm3(shape_sample(s.lights[lc], p, e0, e1));
                     } else {
-                      f3 ld = sub3(lp, p);
-                      float len = len3(ld);
-                      lwi = divs3(ld, len);
-                      if (len >= 0.001f) {
-                        f3 le = (dot3(nl, neg3(lwi)) > 0.f) ? Lg.le : s3(0.f);
-                        float pdf = (Lg.area <= 1e-7f) ? 0.f : 1.f / (Lg.area * (float)N);
-                        float noi = fmaxf(0.f, dot3(h.sp.n, lwi));
-                        float nol = fmaxf(0.f, dot3(nl, neg3(lwi)));
-                        float d2 = len * len;
-                        float fac = (d2 < 1e-7f) ? 0.f : (nol * noi) / d2;
-                        if (pdf > 1e-7f) {
-                          f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
-                          c = divs3(scl3(mul3(f, le), fac), pdf);
-                          tmax = len - 0.001f;
-                          valid = true;
+                      wi = norm3(sc.wi);
+                    }
+                    const float wgt = 1.f / (float)ncomp;
+                    float pdf = 0.f;
+                    for (uint32_t si = 0; si < nl; ++si) pdf += wgt * shape_pdf(s.lights[li[si]], p, wi);
+                    pdf += wgt * bsdf_pdf<MS>(*mat, wo, wi, h.sp);
+                    f3 f = divs3(bsdf_f<MS>(*mat, wo, wi, h.sp), q);
+                    float cs = fabsf(dot3(h.sp.n, wi));
+                    beta = scl3(mul3(beta, f), cs / pdf);
+                  }
+                  if (d + 1u < D) {
+                    cont = beta.x != 0.f || beta.y != 0.f || beta.z != 0.f;
+                  } else if (EST == PT_EST_NAIVE) {
+                    L = add3(L, mul3(beta, bg));  // getRayColor(.., 0) returns bg_color
+                  }
+                  fl = 0u;
+                }
+              }
+            }
+          } else {
+            if (prim < 0) {
+              if (EST == 0) {
+                L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
+              } else if (prev_spec) {
+                if (d < D) L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
+              } else if (s.has_env) {
+                float pl = env_pdf(s, dir) / (float)N;
+                const f3 le = mul3(beta, env_lookup(s, dir));
+                const float wm = power_w(ppdf, pl);
+                L = add3(L, scl3(le, wm));
+                if (EST == PT_EST_GUIDED && prec) {  // the target is the unweighted radiance
+                  prec[18] = le.x * (1.f - wm); prec[19] = le.y * (1.f - wm); prec[20] = le.z * (1.f - wm);
+                }
+              } else {
+                L = add3(L, mul3(beta, miss_radiance(s, rp, dir)));
+              }
+              done = true;
+            }
+            SurfHit h;
+            const pt_material_desc *mat = nullptr;
+            if (!done) {
+              h = surface_tex<MS>(s, prim, o, dir, t);
+              mat = &s.mats[h.mat];
+              if (mat->kind == PT_MAT_EMITTER) {
+                if (EST == 0) {
+                  // A non-specular predecessor excludes emitter hits (maxdepth_nee.hpp:246-247);
+                  // camera and specular chains take DiffuseEmitter::emit.
+                  if (prev_spec) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+                } else if (prev_spec || h.light < 0) {
+                  if (d < D) L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+                } else {
+                  const DevLight &Lg = s.lights[h.light];
+                  // one-sided QuadLight/SphereLight::L_emit (light.hpp:157-165, 296-303): the quad's
+                  // normal follows its vertex order, the sphere's points outwards
+                  f3 nl = (Lg.type == LIGHT_QUAD) ? Lg.n : h.ng;
+                  f3 le = (dot3(nl, neg3(norm3(dir))) > 0.f) ? Lg.le : s3(0.f);
+                  float pl = light_pdf_sa(s, Lg, pprev, h.p) / (float)N;
+                  const float wm = power_w(ppdf, pl);
+                  L = add3(L, scl3(mul3(beta, le), wm));
+                  if (EST == PT_EST_GUIDED && prec) {  // the target is the unweighted radiance
+                    const f3 r = scl3(mul3(beta, le), 1.f - wm);
+                    prec[18] = r.x; prec[19] = r.y; prec[20] = r.z;
+                  }
+                }
+                done = true;
+              }
+            }
+            if (!done && d >= D) done = true;
+            if (!done) {
+              p = h.p;
+              float uc = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d);
+              float u0 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 1u);
+              float u1 = sample_dim(s, rp.sampler, rp.seed, rp.spp_x, rp.spp_y, pixel, k, g, 3u * d + 2u);
+              float aux = pt_rng(g, rslot(d, SLOT_AUX), rp.seed);
+              f3 wo = neg3(norm3(dir));
+              Scatter sc;
+              // guided: the learned mixture of this point (c = 0: not guided); the
+              // technique (guide with probability c, else the BSDF) is drawn first
+              const float *mix = nullptr;
+              float gc = 0.f;
+              bool use_guide = false;
+              if constexpr (EST == PT_EST_GUIDED) {
+                mix = w.gmix + (size_t)i * PT_G_MIX;
+                gc = d < w.gverts ? mix[40] : 0.f;
+                use_guide = gc > 0.f && pt_rng(g, rslot(d, SLOT_GUIDE), rp.seed) < gc;
+              }
+              bool scat;
+              if (use_guide) {
+                sc.wi = g_sample(mix, pt_rng(g, rslot(d, SLOT_GUIDE + 1u), rp.seed),
+                                 pt_rng(g, rslot(d, SLOT_GUIDE + 2u), rp.seed),
+                                 pt_rng(g, rslot(d, SLOT_GUIDE + 3u), rp.seed));
+                sc.flags = 0u;
+                sc.att = s3(0.f);
+                sc.pdf = 0.f;
+                scat = true;
+              } else {
+                scat = bsdf_scatter<MS>(*mat, wo, h.sp, uc, u0, u1, aux, sc);
+              }
+              if (!scat) {
+                L = add3(L, mul3(beta, bsdf_emit(*mat, dir, h.sp)));
+                done = true;
+              } else if (sc.flags & BX_SPECULAR) {
+                beta = mul3(beta, sc.att);
+                wi = sc.wi;
+                cont = d + 1u < D;
+                fl = 1u;
+              } else {
+                f3 wn = norm3(sc.wi);
+                float cs = fabsf(dot3(h.sp.n, wn));
+                if (EST == 0) {
+                  // NEE with light_samples jittered offsets (maxdepth_nee.hpp:90-232)
+                  if (N > 0) {
+                    float inv_ns = 1.f / (float)rp.light_samples;
+                    for (uint32_t si = 0; si < rp.light_samples && si < 4u; ++si) {
+                      uint32_t ii = si / rp.ly, jj = si % rp.ly;
+                      float a0 = div_rn((float)ii, (float)rp.lx), a1 = div_rn((float)(ii + 1u), (float)rp.lx);
+                      float b0 = div_rn((float)jj, (float)rp.ly), b1 = div_rn((float)(jj + 1u), (float)rp.ly);
+                      float ox = a0 + (a1 - a0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si), rp.seed);
+                      float oy = b0 + (b1 - b0) * pt_rng(g, rslot(d, SLOT_NEE_OFF + 2u * si + 1u), rp.seed);
+                      int li = (int)(pt_rng(g, rslot(d, SLOT_NEE_LIGHT + si), rp.seed) * (float)N) % N;
+                      const DevLight &Lg = s.lights[li];
+                      f3 nl;
+                      f3 lp = light_point(Lg, ox, oy, nl);
+                      f3 c = s3(0.f), lwi = s3(0.f);
+                      float tmax = 0.f;
+                      bool valid = false;
+                      if (Lg.type == LIGHT_ENV) {
+                        lwi = norm3(sub3(lp, p));
+                        float pdf = (1.f / (float)N) * (1.f / (4.f * PT_PI));
+                        float fac = fmaxf(0.f, dot3(h.sp.n, lwi));
+                        f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+                        c = divs3(scl3(mul3(f, env_lookup(s, lwi)), fac), pdf);
+                        tmax = 3.402823466e+38f;
+                        valid = true;
+                      } else {
+                        f3 ld = sub3(lp, p);
+                        float len = len3(ld);
+                        lwi = divs3(ld, len);
+                        if (len >= 0.001f) {
+                          f3 le = (dot3(nl, neg3(lwi)) > 0.f) ? Lg.le : s3(0.f);
+                          float pdf = (Lg.area <= 1e-7f) ? 0.f : 1.f / (Lg.area * (float)N);
+                          float noi = fmaxf(0.f, dot3(h.sp.n, lwi));
+                          float nol = fmaxf(0.f, dot3(nl, neg3(lwi)));
+                          float d2 = len * len;
+                          float fac = (d2 < 1e-7f) ? 0.f : (nol * noi) / d2;
+                          if (pdf > 1e-7f) {
+                            f3 f = bsdf_f<MS>(*mat, wo, lwi, h.sp);
+                            c = divs3(scl3(mul3(f, le), fac), pdf);
+                            tmax = len - 0.001f;
+                            valid = true;
+                          }
                         }
                       }
+                      c = scl3(mul3(beta, c), inv_ns);
+                      // zero contributions are dropped; non-finite ones are traced so the film's
+                      // NaN guard sees them
+                      if (valid && ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)))
+                        emit_shadow(w, p, lwi, tmax, c, slot);
                     }
-                    c = scl3(mul3(beta, c), inv_ns);
-                    // zero contributions are dropped; non-finite ones are traced so the film's
-                    // NaN guard sees them
-                    if (valid && ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)))
-                      emit_shadow(w, p, lwi, tmax, c, slot);
+                  }
+                } else if (N > 0) {
+                  int li = (int)(pt_rng(g, rslot(d, SLOT_MIS_LIGHT), rp.seed) * (float)N) % N;
+                  const DevLight &Lg = s.lights[li];
+                  float e0 = pt_rng(g, rslot(d, SLOT_NEE_OFF), rp.seed);
+                  float e1 = pt_rng(g, rslot(d, SLOT_NEE_OFF + 1u), rp.seed);
+                  f3 lwi = s3(0.f), le = s3(0.f);
+                  float pl = 0.f, tmax = 0.f;
+                  if (Lg.type == LIGHT_ENV) {
+                    lwi = env_sample(s, e0, e1, pl);
+                    le = env_lookup(s, lwi);
+                    tmax = 3.402823466e+38f;
+                  } else {
+                    f3 nl;
+                    f3 lp = light_point(Lg, e0, e1, nl);
+                    f3 ld = sub3(lp, p);
+                    float len = len3(ld);
+                    lwi = divs3(ld, len);
+                    float nol = dot3(nl, neg3(lwi));
+                    if (len >= 0.001f && nol > 0.f && Lg.area > 0.f) {
+                      pl = len * len / (Lg.area * nol);
+                      le = Lg.le;
+                      tmax = len - 0.001f;
+                    }
+                  }
+                  // a reflection-only BSDF takes no light from behind its surface (the NEE
+                  // integrator's max(0, n.wi), maxdepth_nee.hpp:137, 192)
+                  if (bsdf_reflect_only(mat->kind) && dot3(h.sp.n, lwi) <= 0.f) pl = 0.f;
+                  if (pl > 0.f) {
+                    pl /= (float)N;
+                    f3 f;
+                    float pb;
+                    bsdf_f_pdf<MS>(*mat, wo, lwi, h.sp, f, pb);
+                    if (EST == PT_EST_GUIDED && gc > 0.f) pb = gc * g_pdf(mix, lwi) + (1.f - gc) * pb;
+                    float cl = fabsf(dot3(h.sp.n, lwi));
+                    f3 c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
+                    if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)) emit_shadow(w, p, lwi, tmax, c, slot);
                   }
                 }
-              } else if (N > 0) {
-                int li = (int)(pt_rng(g, rslot(d, SLOT_MIS_LIGHT), rp.seed) * (float)N) % N;
-                const DevLight &Lg = s.lights[li];
-                float e0 = pt_rng(g, rslot(d, SLOT_NEE_OFF), rp.seed);
-                float e1 = pt_rng(g, rslot(d, SLOT_NEE_OFF + 1u), rp.seed);
-                f3 lwi = s3(0.f), le = s3(0.f);
-                float pl = 0.f, tmax = 0.f;
-                if (Lg.type == LIGHT_ENV) {
-                  lwi = env_sample(s, e0, e1, pl);
-                  le = env_lookup(s, lwi);
-                  tmax = 3.402823466e+38f;
+                float pm = sc.pdf;
+                if (EST == PT_EST_GUIDED && gc > 0.f) {
+                  // one-sample MIS over the two techniques (a guided direction behind a
+                  // reflection-only surface carries nothing)
+                  const bool below = bsdf_reflect_only(mat->kind) && dot3(h.sp.n, wn) <= 0.f;
+                  const f3 f = below ? s3(0.f) : bsdf_f<MS>(*mat, wo, wn, h.sp);
+                  const float pbs = bsdf_pdf<MS>(*mat, wo, wn, h.sp);
+                  pm = gc * g_pdf(mix, wn) + (1.f - gc) * pbs;
+                  if (w.grec && d < w.grec_verts && slot % w.grec_stride == 0u) {
+                    float *rec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + d) * PT_G_REC;
+                    for (int v = 0; v < 6; ++v) rec[v] = mix[41 + v];
+                    rec[6] = wn.x; rec[7] = wn.y; rec[8] = wn.z;
+                    rec[9] = pbs;
+                    rec[10] = beta.x; rec[11] = beta.y; rec[12] = beta.z;
+                    rec[16] = __uint_as_float(w.gpass);
+                  }
+                  beta = pm > 0.f ? scl3(mul3(beta, f), cs / pm) : s3(0.f);
                 } else {
-                  f3 nl;
-                  f3 lp = light_point(Lg, e0, e1, nl);
-                  f3 ld = sub3(lp, p);
-                  float len = len3(ld);
-                  lwi = divs3(ld, len);
-                  float nol = dot3(nl, neg3(lwi));
-                  if (len >= 0.001f && nol > 0.f && Lg.area > 0.f) {
-                    pl = len * len / (Lg.area * nol);
-                    le = Lg.le;
-                    tmax = len - 0.001f;
-                  }
+                  beta = scl3(mul3(beta, sc.att), cs / sc.pdf);
                 }
-                // a reflection-only BSDF takes no light from behind its surface (the NEE
-                // integrator's max(0, n.wi), maxdepth_nee.hpp:137, 192)
-                if (bsdf_reflect_only(mat->kind) && dot3(h.sp.n, lwi) <= 0.f) pl = 0.f;
-                if (pl > 0.f) {
-                  pl /= (float)N;
-                  f3 f;
-                  float pb;
-                  bsdf_f_pdf<MS>(*mat, wo, lwi, h.sp, f, pb);
-                  if (EST == PT_EST_GUIDED && gc > 0.f) pb = gc * g_pdf(mix, lwi) + (1.f - gc) * pb;
-                  float cl = fabsf(dot3(h.sp.n, lwi));
-                  f3 c = scl3(mul3(beta, mul3(f, le)), cl / pl * power_w(pl, pb));
-                  if ((c.x > 0.f || c.y > 0.f || c.z > 0.f) || !finite3(c)) emit_shadow(w, p, lwi, tmax, c, slot);
+                wi = wn;
+                fl = 0u;
+                if (EST == 0) {
+                  cont = d + 1u < D;  // the emitter check of the next hit happens at the next vertex
+                } else {
+                  cont = d + 1u <= D;
+                  if (EST == PT_EST_GUIDED) cont = cont && (beta.x != 0.f || beta.y != 0.f || beta.z != 0.f);
+                  ppdf = pm;
+                  pprev = p;
                 }
               }
-              float pm = sc.pdf;
-              if (EST == PT_EST_GUIDED && gc > 0.f) {
-                // one-sample MIS over the two techniques (a guided direction behind a
-                // reflection-only surface carries nothing)
-                const bool below = bsdf_reflect_only(mat->kind) && dot3(h.sp.n, wn) <= 0.f;
-                const f3 f = below ? s3(0.f) : bsdf_f<MS>(*mat, wo, wn, h.sp);
-                const float pbs = bsdf_pdf<MS>(*mat, wo, wn, h.sp);
-                pm = gc * g_pdf(mix, wn) + (1.f - gc) * pbs;
-                if (w.grec && d < w.grec_verts && slot % w.grec_stride == 0u) {
-                  float *rec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + d) * PT_G_REC;
-                  for (int v = 0; v < 6; ++v) rec[v] = mix[41 + v];
-                  rec[6] = wn.x; rec[7] = wn.y; rec[8] = wn.z;
-                  rec[9] = pbs;
-                  rec[10] = beta.x; rec[11] = beta.y; rec[12] = beta.z;
-                  rec[16] = __uint_as_float(w.gpass);
-                }
-                beta = pm > 0.f ? scl3(mul3(beta, f), cs / pm) : s3(0.f);
-              } else {
-                beta = scl3(mul3(beta, sc.att), cs / sc.pdf);
+              if (cont && rp.rr_depth && d >= rp.rr_depth) {
+                float qs = fminf(1.f, maxc3(beta));
+                if (!(pt_rng(g, rslot(d, SLOT_RR), rp.seed) < qs)) cont = false;
+                else beta = divs3(beta, qs);
               }
-              wi = wn;
-              fl = 0u;
-              if (EST == 0) {
-                cont = d + 1u < D;  // the emitter check of the next hit happens at the next vertex
-              } else {
-                cont = d + 1u <= D;
-                if (EST == PT_EST_GUIDED) cont = cont && (beta.x != 0.f || beta.y != 0.f || beta.z != 0.f);
-                ppdf = pm;
-                pprev = p;
-              }
-            }
-            if (cont && rp.rr_depth && d >= rp.rr_depth) {
-              float qs = fminf(1.f, maxc3(beta));
-              if (!(pt_rng(g, rslot(d, SLOT_RR), rp.seed) < qs)) cont = false;
-              else beta = divs3(beta, qs);
             }
           }
+          // this path's slot is written by no other lane; this vertex's shadow rays
+          // add to it only after this kernel
+          if (L.x != 0.f || L.y != 0.f || L.z != 0.f || !finite3(L)) {
+            w.lx[slot] += L.x;
+            w.ly[slot] += L.y;
+            w.lz[slot] += L.z;
+          }
         }
-        // this path's slot is written by no other lane; this vertex's shadow rays
-        // add to it only after this kernel
-        if (L.x != 0.f || L.y != 0.f || L.z != 0.f || !finite3(L)) {
-          w.lx[slot] += L.x;
-          w.ly[slot] += L.y;
-          w.lz[slot] += L.z;
+        // converged: append the continuing paths to the same partition of the other buffer
+        uint32_t j = base + wave_append(cout, cont);
+        if (cont) {
+          out.ox[j] = p.x; out.oy[j] = p.y; out.oz[j] = p.z;
+          out.dx[j] = wi.x; out.dy[j] = wi.y; out.dz[j] = wi.z;
+          out.bx[j] = beta.x; out.by[j] = beta.y; out.bz[j] = beta.z;
+          out.flags[j] = fl;
+          out.ppdf[j] = ppdf;
+          out.px[j] = pprev.x; out.py[j] = pprev.y; out.pz[j] = pprev.z;
+          out.slot[j] = slot;
+          if (EST == PT_EST_RESERVOIR) {
+            out.pbx[j] = pbeta.x; out.pby[j] = pbeta.y; out.pbz[j] = pbeta.z;
+          }
         }
+        // the list's first 256 entries are shaded: move the rest to the front
+        const uint32_t rem = cnt > 256u ? cnt - 256u : 0u;
+        __syncthreads();
+        const uint32_t mv = threadIdx.x < rem ? o_list[c][256u + threadIdx.x] : 0u;
+        __syncthreads();
+        if (threadIdx.x < rem) o_list[c][threadIdx.x] = mv;
+        if (threadIdx.x == 0) o_n[c] = rem;
+        __syncthreads();
+        cnt = rem;
       }
-      // converged: append the continuing paths to the same partition of the other buffer
-      uint32_t j = base + wave_append(cout, cont);
-      if (cont) {
-        out.ox[j] = p.x; out.oy[j] = p.y; out.oz[j] = p.z;
-        out.dx[j] = wi.x; out.dy[j] = wi.y; out.dz[j] = wi.z;
-        out.bx[j] = beta.x; out.by[j] = beta.y; out.bz[j] = beta.z;
-        out.flags[j] = fl;
-        out.ppdf[j] = ppdf;
-        out.px[j] = pprev.x; out.py[j] = pprev.y; out.pz[j] = pprev.z;
-        out.slot[j] = slot;
-        if (EST == PT_EST_RESERVOIR) {
-          out.pbx[j] = pbeta.x; out.pby[j] = pbeta.y; out.pbz[j] = pbeta.z;
-        }
-      }
-      // the list's first 256 entries are shaded: move the rest to the front
-      const uint32_t rem = cnt > 256u ? cnt - 256u : 0u;
-      __syncthreads();
-      const uint32_t mv = threadIdx.x < rem ? o_list[256u + threadIdx.x] : 0u;
-      __syncthreads();
-      if (threadIdx.x < rem) o_list[threadIdx.x] = mv;
-      if (threadIdx.x == 0) o_n = rem;
-      __syncthreads();
-      cnt = rem;
     }
     if (last) break;
   }
