@@ -159,8 +159,8 @@ def quality(scene, estimator, res=128, spp_side=8, ref_side=64):
 def any_hit_qnodes(scs):
     """Whether any-hit traversal reads the 64-byte quantised nodes: the
     library picks them for a tree whose 128-byte nodes exceed 4 MiB (a batch:
-    when any of its scenes does)."""
-    return any(sc.info.bvh_nodes * 128 > (4 << 20) for sc in scs)
+    when every one of its scenes does)."""
+    return all(sc.info.bvh_nodes * 128 > (4 << 20) for sc in scs)
 
 
 def kernel_roofline(cst, ms, launches, read_traffic_for=None, qnodes=True, paths=0):
@@ -510,6 +510,20 @@ def main_guided(args):
     print(json.dumps(res), flush=True)
 
 
+def copy_probe(mib):
+    """A device-to-device copy of a known size (hipMemcpy through torch's
+    copy_, the runtime's copy kernel): what the traffic counters read for it
+    scales their per-kernel figures to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_each_way": n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -533,7 +547,12 @@ def main():
                                                          "(PT_RENDER_SORT_HITS)")
     ap.add_argument("--build", default="sah", choices=("sah", "lbvh"),
                     help="BVH build: sah (host binned SAH, default) or lbvh (on the device)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device three times (the plain copy whose "
+                         "FETCH_SIZE / WRITE_SIZE calibrate the traffic counters), then exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the path would shard by pixel tiles with one film reduction; see DESIGN.md)",
