@@ -484,7 +484,8 @@ typedef struct pt_guide_stats {
 /* Renders cfg's frame (estimator PT_EST_NEE_MIS) with the guide: training
  * passes of growing spp, each followed by Adam steps on its samples, then
  * the rest of the frame with the trained network.  Every pass is unbiased,
- * so all count equally in the film.                                        */
+ * so all count equally in the film.  PT_RENDER_KEEP_SAMPLES is supported
+ * with PT_GUIDE_NO_TRAIN only (the frame is then one pass).                */
 pt_status pt_render_guided(pt_scene *scene, const pt_render_cfg *cfg, pt_guide *guide,
                            const pt_guide_cfg *gcfg, pt_stats *stats,
                            pt_guide_stats *gstats);

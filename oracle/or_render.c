@@ -35,8 +35,78 @@ static int reflect_only(uint32_t kind) {
 }
 
 enum { SLOT_LENS_X = 0, SLOT_LENS_Y = 1, SLOT_TIME = 2, SLOT_AUX = 4, SLOT_RR = 5,
-       SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_RIS = 16, SLOT_NEE_LIGHT = 40 };
+       SLOT_MIS_LIGHT = 6, SLOT_MIX = 7, SLOT_NEE_OFF = 8, SLOT_RIS = 16, SLOT_NEE_LIGHT = 40,
+       SLOT_GUIDE = 44 };
 static inline uint32_t slot(uint32_t d, uint32_t purpose) { return d * 64u + purpose; }
+
+/* ---- the guided estimator's mixture (no reference counterpart) ----
+ * Pinned by mathematics (tests/test_oracle_guide.py): the pdf integrates to 1
+ * over the sphere and equals the numpy guide oracle's mixture pdf; samples
+ * reproduce each lobe's closed-form mean cosine coth(kappa) - 1/kappa and the
+ * lobe weights. */
+int or_scene_set_guide(or_scene *s, const float *mix, uint32_t kinds, uint32_t verts) {
+  if (!s || !mix) return -1;
+  memcpy(s->gmix, mix, sizeof(s->gmix));
+  s->gkinds = kinds;
+  s->gverts = verts;
+  return 0;
+}
+
+/* Kinds in the mask whose lobes are all rough: a delta lobe takes no guided
+ * direction (conductor alphas p[6], p[7]; Disney roughness p[7]). */
+int or_guidable(const pt_material_desc *m, uint32_t kinds) {
+  if (!((kinds >> m->kind) & 1u)) return 0;
+  if (m->kind == PT_MAT_LAMBERTIAN) return 1;
+  if (m->kind == PT_MAT_CONDUCTOR) return fminf(m->p[6], m->p[7]) >= 1e-3f;
+  if (m->kind == PT_MAT_DISNEY) return m->p[7] >= 0.05f;
+  return 0;
+}
+
+/* p(w) = sum_i pi_i kappa_i / (2 pi (1 - e^{-2 kappa_i})) e^{kappa_i (mu_i.w - 1)},
+ * each term as exp(kappa_i mu_i . w + b_i) with b_i the lobe's stored log factor. */
+float or_guide_pdf(const float *mix, v3 w) {
+  float p = 0.f;
+  for (int i = 0; i < 8; ++i) {
+    const float *l = mix + 4 * i;
+    p += expf(l[0] * w.x + l[1] * w.y + l[2] * w.z + l[3]);
+  }
+  return p;
+}
+
+/* Lobe i by the weights' running sum against u0; then about mu_i the cosine
+ * 1 + log(u1 + (1 - u1) e^{-2 kappa}) / kappa (the inverse of the vMF's
+ * cosine CDF, Jakob 2012) and azimuth 2 pi u2, in the branchless orthonormal
+ * frame of Duff et al. 2017. */
+v3 or_guide_sample(const float *mix, float u0, float u1, float u2) {
+  int i = 0;
+  float acc = mix[32];
+  while (i < 7 && u0 >= acc) acc += mix[32 + ++i];
+  v3 m = V3(mix[4 * i], mix[4 * i + 1], mix[4 * i + 2]);
+  float kappa = vlen(m);
+  v3 mu = vdivs(m, kappa);
+  float ct = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kappa)) / kappa;
+  ct = fminf(1.f, fmaxf(-1.f, ct));
+  float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+  float ph = 2.f * (float)M_PI * u2;
+  float sg = copysignf(1.f, mu.z);
+  float a = -1.f / (sg + mu.z), b = mu.x * mu.y * a;
+  v3 t1 = V3(1.f + sg * mu.x * mu.x * a, sg * b, -sg * mu.x);
+  v3 t2 = V3(b, sg + mu.y * mu.y * a, -mu.y);
+  return vadd(vadd(vscale(t1, st * cosf(ph)), vscale(t2, st * sinf(ph))), vscale(mu, ct));
+}
+
+int or_guide_eval_batch(const float *mix, const float *dirs, const float *u, size_t n, float *pdf,
+                        float *sample_dir) {
+  for (size_t i = 0; i < n; ++i) {
+    const float *mx = mix + 48 * i;
+    if (dirs && pdf) pdf[i] = or_guide_pdf(mx, V3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    if (u && sample_dir) {
+      v3 w = or_guide_sample(mx, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+      sample_dir[3 * i] = w.x; sample_dir[3 * i + 1] = w.y; sample_dir[3 * i + 2] = w.z;
+    }
+  }
+  return 0;
+}
 
 typedef struct {
   const or_scene *s;
@@ -498,7 +568,8 @@ static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t
   or_hit h;
   int have = 0, hit = 0;
   const uint32_t D = cfg->max_depth;
-  const int mis = cfg->estimator == PT_EST_NEE_MIS;
+  const int guided = cfg->estimator == OR_EST_GUIDED;
+  const int mis = cfg->estimator == PT_EST_NEE_MIS || guided;
   int prev_spec = 1;
   float prev_pdf = 0.f;
   v3 prev_p = ro;
@@ -543,7 +614,16 @@ static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t
     float u1 = or_sample_dim(cfg->sampler, cfg->seed, cfg->spp_x, cfg->spp_y, pixel, k, g, 3 * d + 2);
     float aux = or_rng(g, slot(d, SLOT_AUX), cfg->seed);
     or_srec sr;
-    if (!or_scatter(m, rd, &h, uc, u0, u1, aux, &sr)) {
+    /* guided: the technique is drawn first, the mixture with probability c */
+    const float gc = (guided && d < s->gverts && or_guidable(m, s->gkinds)) ? s->gmix[40] : 0.f;
+    if (gc > 0.f && or_rng(g, slot(d, SLOT_GUIDE), cfg->seed) < gc) {
+      sr.wi = or_guide_sample(s->gmix, or_rng(g, slot(d, SLOT_GUIDE + 1), cfg->seed),
+                              or_rng(g, slot(d, SLOT_GUIDE + 2), cfg->seed),
+                              or_rng(g, slot(d, SLOT_GUIDE + 3), cfg->seed));
+      sr.flags = 0;
+      sr.att = vs(0.f);
+      sr.pdf = 0.f;
+    } else if (!or_scatter(m, rd, &h, uc, u0, u1, aux, &sr)) {
       L = vadd(L, vmul(beta, or_emit(m, rd, &h)));
       break;
     }
@@ -602,12 +682,23 @@ static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t
           pl /= (float)N;
           v3 f = or_bsdf_f(m, wo, lwi, &h);
           float pb = or_bsdf_pdf(m, wo, lwi, &h);
+          if (gc > 0.f) pb = gc * or_guide_pdf(s->gmix, lwi) + (1.f - gc) * pb;
           float w = power2(pl, pb);
           float cl = fabsf(vdot(h.n, lwi));
           L = vadd(L, vscale(vmul(beta, vmul(f, le)), cl / pl * w));
         }
       }
-      beta = vscale(vmul(beta, sr.att), cs / pdf);
+      if (gc > 0.f) {
+        /* one-sample MIS over the two techniques; a direction behind a
+         * reflection-only surface carries nothing */
+        const int below = reflect_only(m->kind) && vdot(h.n, wi) <= 0.f;
+        const v3 f = below ? vs(0.f) : or_bsdf_f(m, wo, wi, &h);
+        pdf = gc * or_guide_pdf(s->gmix, wi) + (1.f - gc) * or_bsdf_pdf(m, wo, wi, &h);
+        beta = pdf > 0.f ? vscale(vmul(beta, f), cs / pdf) : vs(0.f);
+        if (beta.x == 0.f && beta.y == 0.f && beta.z == 0.f) break;
+      } else {
+        beta = vscale(vmul(beta, sr.att), cs / pdf);
+      }
       prev_pdf = pdf;
       prev_p = vp;
       prev_spec = 0;

@@ -88,6 +88,10 @@ struct or_scene {
   int has_medium;
   v3 cam_h, cam_v, cam_horiz, cam_vert, cam_bl_rel;
   int committed;
+  /* the guided estimator's mixture (one for every guided point), the
+   * material kinds it guides and how many vertices (or_scene_set_guide) */
+  float gmix[48];
+  uint32_t gkinds, gverts;
 };
 
 int or_scene_intersect(const or_scene *s, v3 o, v3 d, float tmin, float tmax, int any,

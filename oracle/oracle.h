@@ -134,6 +134,23 @@ int or_camera_rays(const or_scene *s, const pt_render_cfg *cfg, const uint32_t *
  * samples, ...).  threads <= 0: all OpenMP threads.                       */
 int or_render(const or_scene *s, const pt_render_cfg *cfg, float *out_rgb, float *out_var,
               float *out_samples, uint64_t *counters, int threads);
+/* ---- the guided estimator (or_render.c) ----
+ * The device's learned-guiding estimator (no counterpart in the reference)
+ * restated with a fixed vMF mixture in place of the network: every point of a
+ * guided material kind at vertex d < verts samples the mixture with
+ * probability c = mix[40], else the BSDF, and weights the direction by the
+ * one-sample MIS pdf c p_guide + (1 - c) p_bsdf, which NEE's power heuristic
+ * and the next vertex's emitter MIS also use.  mix is the processed mixture
+ * (PT_GUIDE_MIX floats: per lobe (kappa mu, log(pi kappa / (2 pi (1 -
+ * e^{-2 kappa}))) - kappa), the 8 weights pi, c).  cfg->estimator
+ * OR_EST_GUIDED selects it in or_render.                                  */
+#define OR_EST_GUIDED 6
+int or_scene_set_guide(or_scene *s, const float *mix, uint32_t kinds, uint32_t verts);
+float or_guide_pdf(const float *mix, v3 w);
+v3 or_guide_sample(const float *mix, float u0, float u1, float u2);
+int or_guide_eval_batch(const float *mix, const float *dirs, const float *u, size_t n, float *pdf,
+                        float *sample_dir);
+int or_guidable(const pt_material_desc *m, uint32_t kinds);
 /* Renders only the path samples in [first, first+count) of the pixel-major
  * sample order and returns their summed luminance (for timing).          */
 double or_render_range(const or_scene *s, const pt_render_cfg *cfg, uint64_t first,

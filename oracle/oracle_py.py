@@ -66,6 +66,8 @@ def _load():
         "or_camera_rays": (ctypes.c_int, [_P, _P, _P, _P, sz, _P]),
         "or_render": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, ctypes.c_int]),
         "or_render_range": (f64, [_P, _P, u64, u64, ctypes.c_int, _P]),
+        "or_scene_set_guide": (ctypes.c_int, [_P, _P, u32, u32]),
+        "or_guide_eval_batch": (ctypes.c_int, [_P, _P, _P, sz, _P, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -239,6 +241,12 @@ class OracleScene:
         lib.or_env_sample_batch(self.h, ptr(u2), n, ptr(d), ptr(pdf))
         return d, pdf
 
+    def set_guide(self, mix, kinds, verts):
+        """The guided estimator (cfg estimator OR_EST_GUIDED = 6) with one fixed
+        processed mixture (48 floats) at every guided point."""
+        self._gmix = np.ascontiguousarray(mix, np.float32).reshape(48)
+        lib.or_scene_set_guide(self.h, ptr(self._gmix), kinds, verts)
+
     def env_pdf(self, dirs):
         dirs = np.ascontiguousarray(dirs, np.float32)
         return np.array([lib.or_env_pdf(self.h, ptr(dirs[i])) for i in range(dirs.shape[0])], np.float32)
@@ -279,3 +287,35 @@ def bsdf_sample(kind, params, wo, nrm, uc, u2, aux=None, inside=None):
     lib.or_bsdf_sample_batch(sref(m), ptr(wo), ptr(nrm), ptr(ins), ptr(uc), ptr(u2), ptr(ax), n, ptr(wi), ptr(f),
                              ptr(pdf), ptr(flags), ptr(ok))
     return wi, f, pdf, flags, ok
+
+
+EST_GUIDED = 6
+
+
+def guide_eval(mix, dirs=None, u=None):
+    """Per row of mix (n, 48): the mixture pdf at dirs (n, 3) and/or a sample
+    direction from uniforms u (n, 3)."""
+    mix = np.ascontiguousarray(mix, np.float32)
+    n = mix.shape[0]
+    d = None if dirs is None else np.ascontiguousarray(dirs, np.float32)
+    uu = None if u is None else np.ascontiguousarray(u, np.float32)
+    pdf = np.empty(n, np.float32) if d is not None else None
+    smp = np.empty((n, 3), np.float32) if uu is not None else None
+    lib.or_guide_eval_batch(ptr(mix), ptr(d), ptr(uu), n, ptr(pdf), ptr(smp))
+    return pdf, smp
+
+
+def processed_mixture(raw):
+    """The 48-float processed mixture of raw outputs (n, 41), from the numpy
+    guide oracle's pi, kappa, mu and c: per lobe (kappa mu, log(pi kappa /
+    (2 pi (1 - e^{-2 kappa}))) - kappa), then the 8 weights, then c."""
+    from oracle import guide_oracle as G
+    pi, kappa, mu, c = G.mixture(raw)
+    n = pi.shape[0]
+    out = np.zeros((n, 48), np.float64)
+    for i in range(8):
+        out[:, 4 * i:4 * i + 3] = kappa[:, i:i + 1] * mu[:, i]
+        out[:, 4 * i + 3] = np.log(pi[:, i]) + G.log_norm(kappa[:, i]) - kappa[:, i]
+    out[:, 32:40] = pi
+    out[:, 40] = c
+    return out.astype(np.float32)

@@ -1903,14 +1903,15 @@ extern "C" pt_status pt_render_guided(pt_scene *s, const pt_render_cfg *cfg, pt_
   if (!g) return fail(PT_ERR_INVALID, "pt_render_guided: NULL guide");
   if (cfg->estimator != PT_EST_NEE_MIS)
     return fail(PT_ERR_UNSUPPORTED, "pt_render_guided: guiding extends the PT_EST_NEE_MIS estimator");
-  if (cfg->flags & PT_RENDER_KEEP_SAMPLES)
-    return fail(PT_ERR_UNSUPPORTED, "pt_render_guided: per-sample radiance is not kept");
   st = ensure_stream(s);
   if (st != PT_OK) return st;
   if (g->device != s->device) return fail(PT_ERR_UNSUPPORTED, "pt_render_guided: guide and scene devices differ");
   const pt_guide_cfg c0{};
   const pt_guide_cfg &c = gc ? *gc : c0;
   const bool train = !(c.flags & PT_GUIDE_NO_TRAIN);
+  // a frame rendered in one pass (no training) keeps its per-sample radiance
+  if ((cfg->flags & PT_RENDER_KEEP_SAMPLES) && train)
+    return fail(PT_ERR_UNSUPPORTED, "pt_render_guided: per-sample radiance is kept only with PT_GUIDE_NO_TRAIN");
   const uint32_t train_spp = train ? std::min(sppc, c.train_spp ? c.train_spp : std::max(1u, sppc / 8u)) : 0u;
   const uint32_t batch = c.batch ? std::min(c.batch, 1u << 20) : 16384u;
   const uint32_t max_steps = c.max_steps ? c.max_steps : 64u;

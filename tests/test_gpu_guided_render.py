@@ -97,3 +97,66 @@ def test_trained_guide_reduces_error(gpu):
           f"untrained guide {untrained:.4g}")
     assert guided < 0.95 * plain
     assert guided < 0.9 * untrained
+
+
+def _constant_guide(scene, mix_raw):
+    """A guide whose network outputs mix_raw at every point: every weight zero,
+    the output layer's biases (the last 41 parameters) set to it."""
+    from pathtracer_amd import render as R, _lib as L
+    g = R.Guide(scene, seed=1)
+    p = np.zeros(L.GUIDE_NPARAMS, np.float32)
+    p[-L.GUIDE_OUTPUTS:] = mix_raw
+    g.set_params(p)
+    return g
+
+
+def _mix_raw(seed, c_logit):
+    rng = np.random.default_rng(seed)
+    raw = np.zeros(41, np.float32)
+    raw[0:8] = rng.normal(size=8)
+    raw[8] = c_logit
+    raw[9:17] = rng.uniform(0.0, 4.0, 8)                 # kappa ~ 4.5 .. 160
+    d = rng.normal(size=(8, 3))
+    d[0] = (-1.0, 0.1, 0.0)                               # towards the slit wall
+    raw[17:25], raw[25:33], raw[33:41] = d[:, 0], d[:, 1], d[:, 2]
+    return raw
+
+
+@pytest.mark.parametrize("scene_name,verts,c_logit", [("guiding_room", 2, 0.0), ("guiding_room", 16, 1.5),
+                                                      ("cornell", 8, -0.5)])
+def test_guided_estimator_per_sample_against_oracle(gpu, scene_name, verts, c_logit):
+    """The guided estimator (k_shade's guided variant) against the oracle's
+    restatement, sample for sample, with a network whose output is one fixed
+    mixture: technique choice, vMF sampling, the one-sample MIS pdf in the
+    throughput, NEE's power heuristic and the next vertex's emitter MIS.  The
+    device's processed mixture first matches the oracle's (numpy) processing
+    of the same raw outputs."""
+    from oracle import oracle_py as O
+    from pathtracer_amd import render as R, scenes, _lib as L
+    from tests.helpers import check_samples
+    spec = getattr(scenes, scene_name)(width=40, height=40, spp_x=2, spp_y=2)
+    sc = R.Scene(spec)
+    raw = _mix_raw(17 + verts, c_logit)
+    g = _constant_guide(sc, raw)
+    x = np.random.default_rng(0).uniform(0, 1, (64, 6)).astype(np.float32)
+    graw, gmix = g.forward(x)
+    np.testing.assert_allclose(graw, np.repeat(raw[None], 64, 0), rtol=0, atol=0)
+    omix = O.processed_mixture(raw[None].astype(np.float64))[0]
+    np.testing.assert_allclose(gmix[:, :41], np.repeat(omix[None, :41], 64, 0), rtol=2e-5, atol=2e-6)
+    kinds = (1 << L.MAT_LAMBERTIAN) | (1 << L.MAT_CONDUCTOR) | (1 << L.MAT_DISNEY)
+    gc = L.GuideCfg()
+    gc.flags = L.GUIDE_NO_TRAIN
+    gc.kinds = kinds
+    gc.guide_verts = verts
+    cfg = sc.cfg(flags=L.RENDER_KEEP_SAMPLES, estimator=L.EST_NEE_MIS)
+    g.render(cfg, gc)
+    gs = sc.samples(cfg)
+    g.close()
+    o = O.OracleScene(spec)
+    o.set_guide(omix, kinds, verts)
+    _, _, os_, _ = o.render(o.cfg(estimator=O.EST_GUIDED), want_samples=True)
+    check_samples(gs, os_, f"guided {scene_name} verts {verts} c_logit {c_logit}")
+    # the guide was used: the guided film differs from the plain tracer's samples
+    cfg_p = sc.cfg(flags=L.RENDER_KEEP_SAMPLES, estimator=L.EST_NEE_MIS)
+    sc.render(cfg_p)
+    assert not np.allclose(sc.samples(cfg_p), gs)

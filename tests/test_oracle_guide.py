@@ -145,3 +145,63 @@ def test_adam_first_step_is_signed_lr():
     p1, m, v = G.adam(p, np.zeros(5), np.zeros(5), g, 1, lr=1e-3)
     np.testing.assert_allclose(p1[:3], -1e-3 * np.sign(g[:3]), rtol=1e-4)
     assert p1[3] == 0.0
+
+
+# ---- the C oracle's mixture (the guided estimator's pdf and sampler) ----
+
+def _raw_mixture(rng, n, kappa_r=(-1.0, 4.0)):
+    raw = rng.normal(size=(n, G.OUT))
+    raw[:, 9:17] = rng.uniform(*kappa_r, (n, 8))
+    return raw
+
+
+def test_c_mixture_pdf_matches_numpy_and_integrates_to_one():
+    """or_guide_pdf on the processed mixture equals the numpy oracle's mixture
+    pdf (float64) and integrates to 1 over the sphere (Fibonacci points)."""
+    from oracle import oracle_py as O
+    rng = np.random.default_rng(7)
+    raw = _raw_mixture(rng, 64)
+    mix = O.processed_mixture(raw)
+    w = rng.normal(size=(64, 3))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    pc, _ = O.guide_eval(mix, dirs=w)
+    pn = G.guide_pdf(raw, w)
+    np.testing.assert_allclose(pc, pn, rtol=2e-5)
+    # quadrature on 2^18 Fibonacci points for moderate concentrations
+    raw = _raw_mixture(rng, 4, kappa_r=(-2.0, 1.0))
+    mix = O.processed_mixture(raw)
+    m = 1 << 18
+    k = np.arange(m) + 0.5
+    z = 1 - 2 * k / m
+    ph = np.pi * (1 + 5 ** 0.5) * k
+    r = np.sqrt(1 - z * z)
+    dirs = np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+    for row in mix:
+        p, _ = O.guide_eval(np.repeat(row[None], m, 0), dirs=dirs)
+        assert abs(p.astype(np.float64).mean() * 4 * np.pi - 1.0) < 1e-3
+
+
+def test_c_mixture_sampler_inverts_the_vmf_cdf():
+    """or_guide_sample: unit directions; the lobe the weights' CDF picks from
+    u0; cos(angle to that lobe's mean) = 1 + log(u1 + (1 - u1) e^{-2 kappa}) /
+    kappa, the vMF's inverse cosine CDF; the sample mean equals the closed form
+    sum_i pi_i (coth kappa_i - 1 / kappa_i) mu_i."""
+    from oracle import oracle_py as O
+    rng = np.random.default_rng(8)
+    raw = _raw_mixture(rng, 1)
+    mix = O.processed_mixture(raw)
+    pi, kappa, mu, _ = G.mixture(raw)
+    n = 400_000
+    u = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    _, w = O.guide_eval(np.repeat(mix, n, 0), u=u)
+    w = w.astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(w, axis=1), 1.0, atol=2e-6)
+    cdf = np.cumsum(mix[0, 32:40].astype(np.float32))
+    lobe = np.minimum(np.searchsorted(cdf, u[:, 0], side="right"), 7)
+    kap = kappa[0, lobe]
+    ct = 1 + np.log(u[:, 1] + (1 - u[:, 1]) * np.exp(-2 * kap)) / kap
+    got = np.einsum("ij,ij->i", w, mu[0, lobe])
+    assert np.abs(got - np.clip(ct, -1, 1)).max() < 2e-4
+    mean = (pi[0, :, None] * (1 / np.tanh(kappa[0]) - 1 / kappa[0])[:, None] * mu[0]).sum(0)
+    se = w.std(0) / np.sqrt(n)
+    assert np.all(np.abs(w.mean(0) - mean) < 5 * se + 1e-4), (w.mean(0), mean)

@@ -302,3 +302,62 @@ def test_reference_traversal_count_changes_no_sample():
     assert np.array_equal(a, b)
     assert ca[2] == cb[2]                      # shadow rays unchanged
     assert 1.4 * ca[1] < cb[1] < 2.0 * ca[1]   # camera rays are traversed once, continuations twice
+
+
+# ---- the guided estimator (a fixed vMF mixture in place of the network) ----
+
+def guide_mixture(c_logit=0.0, seed=3, toward=(0.0, 1.0, 0.0)):
+    """A processed mixture whose first lobe points along `toward` with
+    kappa ~ 31, the others at random directions with kappa from 4 to 160;
+    c = 0.05 + 0.45 sigmoid(c_logit)."""
+    from oracle import guide_oracle as G
+    rng = np.random.default_rng(seed)
+    raw = np.zeros((1, G.OUT))
+    raw[0, 0:8] = rng.normal(size=8)
+    raw[0, 8] = c_logit
+    raw[0, 9:17] = rng.uniform(0.0, 4.0, 8)
+    raw[0, 9] = 2.0
+    dirs = rng.normal(size=(8, 3))
+    dirs[0] = toward
+    raw[0, 17:25], raw[0, 25:33], raw[0, 33:41] = dirs[:, 0], dirs[:, 1], dirs[:, 2]
+    return O.processed_mixture(raw)[0]
+
+
+GUIDED_KINDS = (1 << 1) | (1 << 3) | (1 << 7)   # Lambertian, conductor, Disney
+
+
+@pytest.mark.parametrize("c_logit", [-1.0, 2.0])
+def test_guided_form_factor(c_logit):
+    """Direct light on a diffuse floor under a quad light: with guided
+    directions drawn at the floor point and weighted by the one-sample MIS pdf
+    (in the BSDF weight, NEE's power heuristic and the emitter hit's), the
+    estimate is the closed-form albedo * Le * form factor."""
+    spec, expect = floor_under_light()
+    o = O.OracleScene(spec)
+    o.set_guide(guide_mixture(c_logit), GUIDED_KINDS, 4)
+    img, var, _, _ = o.render(o.cfg(estimator=O.EST_GUIDED, max_depth=1, spp_x=128, spp_y=128), want_var=True)
+    se = np.sqrt(var[0, 0, 0] / (128 * 128))
+    assert abs(img[0, 0, 0] - expect) < max(4 * se, 2e-3 * expect)
+
+
+def test_guided_white_furnace():
+    """Every vertex guided inside the white furnace (Lambertian albedo 0.5
+    under a uniform environment of 1): the NEE+MIS value 0.5."""
+    spec = scenes.furnace(albedo=0.5, env_value=1.0)
+    o = O.OracleScene(spec)
+    o.set_guide(guide_mixture(1.0, seed=4), GUIDED_KINDS, 64)
+    img, _, _, _ = o.render(o.cfg(estimator=O.EST_GUIDED, max_depth=64, spp_x=32, spp_y=32))
+    assert abs(img[3:5, 3:5, 0].mean() - 0.5) < 0.01
+
+
+def test_guided_agrees_with_nee_mis_on_the_cornell_box():
+    """Guided (every vertex, rough conductor and Lambertian walls) and plain
+    NEE+MIS images of the Cornell box agree in their means."""
+    spec = scenes.cornell(width=16, height=16, spp_x=16, spp_y=16)
+    o = O.OracleScene(spec)
+    o.set_guide(guide_mixture(0.5, seed=5), GUIDED_KINDS, 8)
+    a, va, _, _ = o.render(o.cfg(estimator=1, rr_depth=0), want_var=True)
+    b, vb, _, _ = o.render(o.cfg(estimator=O.EST_GUIDED, rr_depth=0, seed=77), want_var=True)
+    ma, mb = a.mean((0, 1)), b.mean((0, 1))
+    se = np.sqrt((va.mean((0, 1)) + vb.mean((0, 1))) / (256 * 256))
+    assert np.all(np.abs(ma - mb) < 4 * se + 1e-3 * ma), (ma, mb, se)
