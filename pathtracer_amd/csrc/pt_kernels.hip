@@ -956,7 +956,13 @@ __global__ void __launch_bounds__(256) k_film_batch(Wave w, unsigned long long *
 // at most 64 and at most the pixel's samples): lane j sums the pixel's
 // samples j, j + T, j + 2T, ... (so consecutive lanes read consecutive
 // slots), then the T partial sums are added pairwise by an xor-shuffle tree.
-// The order is fixed by (samples per pixel, T), so films are deterministic.
+// The order is fixed by (samples per pixel, T), so the film is as
+// deterministic as the slots' radiance: with one shadow ray per vertex
+// (NEE+MIS, guided) each slot takes one add per shadow pass and is
+// reproducible bit for bit; the reference estimator with light_samples > 1 and
+// the volume estimator add up to four unoccluded contributions per slot in
+// k_shadow with float atomics in arbitrary order, so their slots (and films)
+// can differ in the last place between runs.
 // One lane per pixel strided through its pixel's samples and fetched 2.2 GB
 // per 8M-slot chunk for 100 MB of radiance.
 __global__ void __launch_bounds__(256) k_film_lanes(RenderParams rp, Wave w, float *film, float *samples,
