@@ -432,8 +432,8 @@ def main_guided(args):
     0.05-wide slit), 1024x1024, 256 spp, depth 16.  One step renders the frame
     with the learned guide, trained online from a fresh network inside the
     step (training passes of 1, 2, 4, ... spp up to 32 spp, 64 Adam steps after
-    each, then the remaining 224 spp with the trained network; the first two
-    vertices of a path are guided).  The plain NEE+MIS
+    each, then the remaining 224 spp with the trained network; the first
+    vertex of a path is guided).  The plain NEE+MIS
     tracer renders the same frame beside it.  Quality: relMSE against a
     32768-spp plain reference at 256x256 for both methods at 16, 64 and 256
     spp, and the time the guided method needs to reach the plain method's
@@ -495,7 +495,7 @@ def main_guided(args):
         "config": {"workload": "guided_room: slit-lit 4x3x3 room, Disney floor, 12.5k-tri conductor statue; "
                                "learned guide trained online inside the step",
                    "width": cfg.width, "height": cfg.height, "spp": cfg.spp_x * cfg.spp_y, "max_depth": cfg.max_depth,
-                   "train_spp": cfg.spp_x * cfg.spp_y // 8, "guided_vertices": 2,
+                   "train_spp": cfg.spp_x * cfg.spp_y // 8, "guided_vertices": 1,
                    "triangles": spec.n_triangles},
         "plain": {"value": round(plain_value, 2), "ms_per_step": round(dt_p / args.steps * 1e3, 2)},
         "guide": {k: gst[0][2][k] for k in ("passes", "steps", "samples", "last_loss", "ms_train")},

@@ -473,7 +473,7 @@ typedef struct pt_guide_cfg {
                               (0 = Lambertian, rough conductor, Disney)     */
   float lr;                /* Adam learning rate (0 = 1e-3)                 */
   uint32_t flags;          /* PT_GUIDE_*                                    */
-  uint32_t guide_verts;    /* vertices 0 .. guide_verts-1 are guided (0 = 2) */
+  uint32_t guide_verts;    /* vertices 0 .. guide_verts-1 are guided (0 = 1) */
 } pt_guide_cfg;
 typedef struct pt_guide_stats {
   uint32_t passes, steps;

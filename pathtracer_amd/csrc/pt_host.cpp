@@ -1925,8 +1925,10 @@ extern "C" pt_status pt_render_guided(pt_scene *s, const pt_render_cfg *cfg, pt_
   // Guiding deeper vertices compounds the throughput growth the defensive
   // mixture allows (up to 1 / (1 - c) per vertex): measured on the slit-lit
   // room at 256^2, 256 spp, guiding every vertex raised the relMSE 8x over the
-  // plain tracer while guiding the first two lowered it by a fifth.
-  gp.guided_verts = c.guide_verts ? c.guide_verts : 2u;
+  // plain tracer.  Guiding the first vertex alone lowers it as much as guiding
+  // the first two (trimmed relMSE 0.258 and 0.259 against 0.262 and 0.254 over
+  // two runs of three films each; plain 0.305) for half the network queries.
+  gp.guided_verts = c.guide_verts ? c.guide_verts : 1u;
   if (train) {
     g->tcap = 1u << 21;
     HIP_TRY(g->tset.alloc((size_t)g->tcap * PT_G_SAMPLE * sizeof(float)));
