@@ -92,6 +92,11 @@ PT_D float dmetal_pdf(const Disney &d, f3 wo, f3 wi, const ShadePoint &h) {
   Frame b = make_frame(h.n);
   return mf_pdf_at(d.dist, b, to_local(b, norm3(wo)), wi);
 }
+// ATT: the rough lobes' value at the sampled direction.  disney_scatter
+// re-evaluates every lobe there (disney_f_pdf) and keeps only the sampled
+// lobe's pdf for its cut-off, so it asks for the value only where the lobe is
+// specular.
+template <bool ATT>
 PT_D bool dmetal_scatter(const Disney &d, f3 wo, const ShadePoint &h, float u0, float u1, Scatter &s) {
   if (trd_smooth(d.dist)) {
     f3 wi = norm3(reflect_pbrt(wo, h.n));
@@ -112,7 +117,7 @@ PT_D bool dmetal_scatter(const Disney &d, f3 wo, const ShadePoint &h, float u0, 
   if (dot3(h.n, wi) <= 0.f) return false;
   s.wi = norm3(wi);
   s.flags = BX_GLOSSY | BX_REFL;
-  s.att = dmetal_f_c0(d, wo, wi, h, d.base);
+  s.att = ATT ? dmetal_f_c0(d, wo, wi, h, d.base) : s3(0.f);
   s.pdf = mf_pdf_at(d.dist, b, wo_l, wi);
   return true;
 }
@@ -141,6 +146,7 @@ PT_D float dglass_pdf(const Disney &d, f3 wo, f3 wi, const ShadePoint &h) {
   Frame b = make_frame(h.n);
   return mfdiel_pdf_local(d.eta, d.dist, to_local(b, norm3(wo)), to_local(b, norm3(wi)));
 }
+template <bool ATT>
 PT_D bool dglass_scatter(const Disney &d, f3 wo, const ShadePoint &h, float uc, float u0, float u1, Scatter &s) {
   float eta = d.eta;
   TRD t = d.dist;
@@ -183,7 +189,7 @@ PT_D bool dglass_scatter(const Disney &d, f3 wo, const ShadePoint &h, float uc, 
   bool is_refl = dot3(to_local(b, wi), to_local(b, wo)) > 0.f;
   s.wi = wi;
   s.flags = BX_GLOSSY | (is_refl ? BX_REFL : BX_TRANS);
-  s.att = dglass_f(d, wo, wi, h);
+  s.att = ATT ? dglass_f(d, wo, wi, h) : s3(0.f);
   s.pdf = mfdiel_pdf_local(eta, t, wo_l, to_local(b, norm3(wi)));
   return true;
 }
@@ -222,6 +228,7 @@ PT_D float dcc_pdf(const Disney &d, f3 wo, f3 wi, const ShadePoint &h) {
   Frame b = make_frame(h.n);
   return cc_pdf_at(d.cc_alpha, b, to_local(b, norm3(wo)), wi);
 }
+template <bool ATT>
 PT_D bool dcc_scatter(const Disney &d, f3 wo, const ShadePoint &h, float u0, float u1, Scatter &s) {
   Frame b = make_frame(h.n);
   f3 wo_l = to_local(b, norm3(wo));
@@ -238,7 +245,7 @@ PT_D bool dcc_scatter(const Disney &d, f3 wo, const ShadePoint &h, float u0, flo
   if (dot3(h.n, wi) <= 0.f) return false;
   s.wi = norm3(wi);
   s.flags = BX_GLOSSY | BX_REFL;
-  s.att = dcc_f(d, wo, wi, h);
+  s.att = ATT ? dcc_f(d, wo, wi, h) : s3(0.f);
   s.pdf = cc_pdf_at(a, b, wo_l, wi);
   return true;
 }
@@ -415,7 +422,7 @@ PT_D int sample_discrete(const float (&w)[N], float u, float *pmf, float *u_rema
 PT_D bool disney_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float uc, float u0, float u1,
                          Scatter &s) {
   Disney d = disney_params(m, h);
-  if (h.inside) return dglass_scatter(d, wo, h, uc, u0, u1, s);
+  if (h.inside) return dglass_scatter<true>(d, wo, h, uc, u0, u1, s);
   float w[4] = {d.dw, d.mw, d.gw, d.cw};
   float pmf, ucr;
   int lobe = sample_discrete<4>(w, uc, &pmf, &ucr);
@@ -427,14 +434,14 @@ PT_D bool disney_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float 
     s.wi = norm3(wiw);
     s.pdf = cosine_pdf(h.n, wiw);
     s.flags = BX_DIFFUSE | BX_REFL;
-    s.att = ddiffuse_f(d, wo, wiw, h);
+    s.att = s3(0.f);  // re-evaluated below (ddiffuse_f's value is not kept)
     ok = true;
   } else if (lobe == 1) {
-    ok = dmetal_scatter(d, wo, h, ucr, u1, s);
+    ok = dmetal_scatter<false>(d, wo, h, ucr, u1, s);
   } else if (lobe == 2) {
-    ok = dglass_scatter(d, wo, h, ucr, ucr, u1, s);
+    ok = dglass_scatter<false>(d, wo, h, ucr, ucr, u1, s);
   } else {
-    ok = dcc_scatter(d, wo, h, ucr, u1, s);
+    ok = dcc_scatter<false>(d, wo, h, ucr, u1, s);
   }
   if (!ok) return false;
   if (s.flags & BX_SPECULAR) return true;
