@@ -439,6 +439,7 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
   };
   bool have_cur = false, have_leaf = false;
   int leaf = 0;
+  uint32_t lq = 0, lend = 0;  // any hit: the primitive cursor of the current leaf
   // one inner-node step: test the 4 children, go to the nearest hit one and
   // push the others far to near, else pop
   auto node_step = [&]() {
@@ -512,6 +513,7 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
         cur = BATCH ? lane_root : s.root;
         have_cur = true;
         have_leaf = false;
+        lq = lend = 0;
         sp = 0;
         lo = 0;
       }
@@ -527,20 +529,22 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
       if (cur >= 0) {
         node_step();
       } else {
-        bool hit = false;
-        const uint32_t st = leaf_start(cur), cn = leaf_count(cur);
-        for (uint32_t q = st; q < st + cn; ++q) {
-          float t;
-          uint32_t id;
-          if (COUNT) ++n_prims;
-          if (hit_prim(BATCH ? lane_prims : s.prims, q, o, d, tmin, tb, t, id)) {
-            tb = t;
-            best = (int)id;
-            hit = true;
-            break;
-          }
+        // one primitive of the current leaf per iteration, so that a lane in a
+        // leaf holds the lanes stepping through nodes back by one test only
+        if (lq >= lend) {
+          lq = leaf_start(cur);
+          lend = lq + leaf_count(cur);
         }
-        have_cur = !hit && pop_next(cur);
+        float t;
+        uint32_t id;
+        if (COUNT) ++n_prims;
+        if (hit_prim(BATCH ? lane_prims : s.prims, lq, o, d, tmin, tb, t, id)) {
+          tb = t;
+          best = (int)id;
+          have_cur = false;
+        } else if (++lq >= lend) {
+          have_cur = pop_next(cur);
+        }
       }
       if (!have_cur) {
         finish((uint32_t)ray, best >= 0, tb, best);
