@@ -474,7 +474,14 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
     float t0 = h0 ? tn[0] : INFINITY, t1 = h1 ? tn[1] : INFINITY;
     float t2 = h2 ? tn[2] : INFINITY, t3 = h3 ? tn[3] : INFINITY;
     const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-    if (nh > 0) {
+    if (ANY && nh > 0) {
+      // any hit: every order finds an occluder when there is one; visit the
+      // first hit child in slot order and push the others unsorted
+      if (h3 && nh >= 2) push(c3, 0.f);
+      if (h2 && (h0 || h1)) push(c2, 0.f);
+      if (h1 && h0) push(c1, 0.f);
+      cur = h0 ? c0 : (h1 ? c1 : (h2 ? c2 : c3));
+    } else if (nh > 0) {
       PT_CSWAP(t0, c0, t1, c1);
       PT_CSWAP(t2, c2, t3, c3);
       PT_CSWAP(t0, c0, t2, c2);
