@@ -31,7 +31,7 @@ constexpr SobolTable make_sobol() {
   }
   return t;
 }
-__constant__ static const SobolTable kSobol = make_sobol();
+__constant__ static const SobolNibbles kSobol = sobol_nibbles(make_sobol());
 
 // One sampler dimension of path sample k of `pixel` (Halton index g).
 PT_D float sample_dim(const DevScene &s, uint32_t sampler, uint32_t seed, uint32_t spp_x, uint32_t spp_y,

@@ -136,6 +136,25 @@ PT_D float halton_plain(uint32_t base_index, uint32_t a) {
 struct SobolTable {
   uint32_t v[4][32];
 };
+// The same generator matrices in 4-bit slices: n[d][p][b] is the XOR of
+// v[d][4p + j] over the set bits j of b, so a matrix-vector product over
+// GF(2) is 8 lookups instead of one step per index bit (XOR is associative
+// and commutative: the same bits).
+struct SobolNibbles {
+  uint32_t n[4][8][16];
+};
+constexpr SobolNibbles sobol_nibbles(const SobolTable &t) {
+  SobolNibbles r{};
+  for (int d = 0; d < 4; ++d)
+    for (int p = 0; p < 8; ++p)
+      for (int b = 0; b < 16; ++b) {
+        uint32_t x = 0;
+        for (int j = 0; j < 4; ++j)
+          if ((b >> j) & 1) x ^= t.v[d][4 * p + j];
+        r.n[d][p][b] = x;
+      }
+  return r;
+}
 
 PT_HD uint32_t reverse_bits32(uint32_t x) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -158,14 +177,14 @@ PT_HD uint32_t nested_uniform_scramble(uint32_t x, uint32_t seed) {
   x ^= x * 0x8d22f6e6u;
   return reverse_bits32(x);
 }
-PT_HD uint32_t sobol_bits(const SobolTable &t, uint32_t dim, uint32_t index) {
+PT_HD uint32_t sobol_bits(const SobolNibbles &t, uint32_t dim, uint32_t index) {
+  const uint32_t (*r)[16] = t.n[dim & 3u];
   uint32_t x = 0;
-  const uint32_t *v = t.v[dim & 3u];
-  for (int k = 0; index; index >>= 1, ++k)
-    if (index & 1u) x ^= v[k];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) x ^= r[p][(index >> (4 * p)) & 15u];
   return x;
 }
-PT_HD float sobol_owen(const SobolTable &t, uint32_t index, uint32_t dim, uint32_t seed) {
+PT_HD float sobol_owen(const SobolNibbles &t, uint32_t index, uint32_t dim, uint32_t seed) {
   uint32_t group = dim >> 2;
   uint32_t shuffled = nested_uniform_scramble(index, pt_hash32(seed ^ pt_hash32(group * 0x68bc21ebu + 1u)));
   uint32_t x = sobol_bits(t, dim & 3u, shuffled);
