@@ -152,24 +152,37 @@ PT_D void g_process(const f32x4 (&o)[4], uint32_t g, float c_lo, float c_span, L
 
 // ------------------------------------------------------------------ inference
 #define PT_GQ_STRIDE 7  // per point in LDS: 6 inputs + guided flag
-__global__ void __launch_bounds__(256) k_guide_infer(DevScene s, Wave w, int qi, GuideView gv, uint32_t kinds) {
+// A block of PT_GI_BLOCK threads shares one LDS copy of the weights; each of
+// its waves evaluates PT_GI_T tiles of 16 points, so a block iteration covers
+// PT_GI_PTS points.  Two tiles per wave keep the kernel within 128 registers
+// (4 waves per SIMD; four tiles needed 202), and 512-thread blocks keep the
+// weights' 56 KB of LDS to two copies per CU.
+#ifndef PT_GI_BLOCK
+#define PT_GI_BLOCK 512
+#endif
+#ifndef PT_GI_T
+#define PT_GI_T 2
+#endif
+#define PT_GI_PTS (PT_GI_BLOCK / 64 * 16 * PT_GI_T)
+static_assert(PT_GI_PTS <= PT_GI_BLOCK, "one input thread per point");
+__global__ void __launch_bounds__(PT_GI_BLOCK) k_guide_infer(DevScene s, Wave w, int qi, GuideView gv, uint32_t kinds) {
   __shared__ bf16x8 wf[PT_G_FWD_VALUES / 8];
   __shared__ float bias[PT_G_BIAS_VALUES];
-  __shared__ float qin[256 * PT_GQ_STRIDE];
+  __shared__ float qin[PT_GI_PTS * PT_GQ_STRIDE];
   const uint32_t tid = threadIdx.x;
   {
     const uint4 *src = reinterpret_cast<const uint4 *>(gv.fwd);
     uint4 *dst = reinterpret_cast<uint4 *>(wf);
-    for (uint32_t i = tid; i < PT_G_FWD_VALUES / 8; i += 256) dst[i] = src[i];
-    for (uint32_t i = tid; i < PT_G_BIAS_VALUES; i += 256) bias[i] = gv.bias[i];
+    for (uint32_t i = tid; i < PT_G_FWD_VALUES / 8; i += PT_GI_BLOCK) dst[i] = src[i];
+    for (uint32_t i = tid; i < PT_G_BIAS_VALUES; i += PT_GI_BLOCK) bias[i] = gv.bias[i];
   }
   __syncthreads();
   const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
   const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
   const PathState &ps = w.st[qi];
   const uint32_t lane = tid & 63u, wv = tid >> 6, q = lane & 15u, g = lane >> 4;
-  for (uint32_t i0 = (blockIdx.x / PT_PARTS) * 256u; i0 < n; i0 += nb * 256u) {
-    {
+  for (uint32_t i0 = (blockIdx.x / PT_PARTS) * PT_GI_PTS; i0 < n; i0 += nb * PT_GI_PTS) {
+    if (tid < PT_GI_PTS) {
       const uint32_t i = i0 + tid;
       float x[PT_G_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       float ok = 0.f;
@@ -197,18 +210,18 @@ __global__ void __launch_bounds__(256) k_guide_infer(DevScene s, Wave w, int qi,
       qin[tid * PT_GQ_STRIDE + 6] = ok;
     }
     __syncthreads();
-    float xin[4][PT_G_IN];
+    float xin[PT_GI_T][PT_G_IN];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < PT_GI_T; ++t)
 #pragma unroll
-      for (int v = 0; v < PT_G_IN; ++v) xin[t][v] = qin[(64u * wv + 16u * t + q) * PT_GQ_STRIDE + v];
-    f32x4 acc[4][4];
-    g_network<4>(wf, bias, xin, lane, acc);
+      for (int v = 0; v < PT_G_IN; ++v) xin[t][v] = qin[(16u * PT_GI_T * wv + 16u * t + q) * PT_GQ_STRIDE + v];
+    f32x4 acc[PT_GI_T][4];
+    g_network<PT_GI_T>(wf, bias, xin, lane, acc);
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < PT_GI_T; ++t) {
       LobeQuad r;
       g_process(acc[t], g, gv.c_lo, gv.c_span, r);
-      const uint32_t pl = 64u * wv + 16u * t + q;  // point of the block
+      const uint32_t pl = 16u * PT_GI_T * wv + 16u * t + q;  // point of the block
       const uint32_t i = i0 + pl;
       if (i < n) {
         const bool ok = qin[pl * PT_GQ_STRIDE + 6] != 0.f;
@@ -688,7 +701,7 @@ static uint32_t blocks_of(size_t n, uint32_t bs) { return (uint32_t)((n + bs - 1
 
 void launch_guide_infer(hipStream_t st, const DevScene &s, const Wave &w, int qi, const GuideView &gv,
                         uint32_t kinds, uint32_t grid) {
-  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(256), 0, st, s, w, qi, gv, kinds);
+  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(PT_GI_BLOCK), 0, st, s, w, qi, gv, kinds);
 }
 void launch_guide_points(hipStream_t st, const GuideView &gv, const float *xs, size_t n, float *raw, float *mix) {
   if (n == 0) return;
