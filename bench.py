@@ -323,28 +323,35 @@ def time_to_reach(curve, target):
     return None
 
 
-def guided_quality(scenes, R, L, q):
-    """relMSE against a 32768-spp plain reference at q x q for the plain and the
-    guided method at 16, 64, 256 spp, in full and with the 0.1% largest pixel
-    errors rejected, and the guided time to reach the plain method's 256-spp
-    error (log-log interpolation) by each."""
+def guided_quality(scenes, R, L, q, scene=None, seeds=2):
+    """relMSE against a plain reference at q x q (8192 spp: four 2048-spp
+    films, the frame size a render takes) for the plain and the guided method
+    at 16, 64, 256 spp, in full and with the 0.1% largest pixel errors
+    rejected, and the guided time to reach the plain method's 256-spp error
+    (log-log interpolation) by each.  Measured at the workload's own
+    resolution: the guide's training costs a fixed number of Adam steps per
+    frame, which a smaller film would over-weight.  `seeds` films of each
+    method per spp are averaged."""
     import numpy as np
-    qs = R.Scene(scenes.guiding_room(width=q, height=q))
+    qs = scene or R.Scene(scenes.guiding_room(width=q, height=q))
     ref = np.zeros((q, q, 3))
-    for k in range(8):
-        img, _ = qs.render(qs.cfg(spp_x=64, spp_y=64, seed=900 + k))
-        ref += img / 8
+    nref = 4
+    for k in range(nref):
+        img, _ = qs.render(qs.cfg(spp_x=64, spp_y=32, seed=900 + k))
+        ref += img / nref
     curve = {"plain": [], "guided": []}
     for side in (4, 8, 16):
         pe, pt, ge, gt, pe_t, ge_t = [], [], [], [], [], []
-        for k in range(3):  # three films of each method, averaged
+        for k in range(seeds):
             c = qs.cfg(spp_x=side, spp_y=side, seed=11 + k)
+            L.check(L.lib.pt_device_synchronize(), "sync")
             t = time.perf_counter()
             img, _ = qs.render(c)
             pt.append(time.perf_counter() - t)
             pe.append(relmse(img, ref))
             pe_t.append(relmse(img, ref, 0.999))
             g = R.Guide(qs, seed=5 + k)
+            L.check(L.lib.pt_device_synchronize(), "sync")
             t = time.perf_counter()
             img, _, _ = g.render(c, L.GuideCfg())
             gt.append(time.perf_counter() - t)
@@ -431,14 +438,14 @@ def main_guided(args):
     displaced conductor statue, a quad light of radiance 200 seen through a
     0.05-wide slit), 1024x1024, 256 spp, depth 16.  One step renders the frame
     with the learned guide, trained online from a fresh network inside the
-    step (training passes of 1, 2, 4, ... spp up to 32 spp, 64 Adam steps after
-    each, then the remaining 224 spp with the trained network; the first
-    vertex of a path is guided).  The plain NEE+MIS
-    tracer renders the same frame beside it.  Quality: relMSE against a
-    32768-spp plain reference at 256x256 for both methods at 16, 64 and 256
-    spp, and the time the guided method needs to reach the plain method's
-    relMSE at 256 spp.  relMSE is taken on films scaled to the reference's
-    unit mean."""
+    step (training passes of 4, 8 and 4 spp, 96 Adam steps after each, then
+    the remaining 240 spp with the trained network; the first vertex of a
+    path is guided).  The plain NEE+MIS
+    tracer renders the same frame beside it.  Quality: relMSE against an
+    8192-spp plain reference at the frame's resolution for both methods at
+    16, 64 and 256 spp, and the time the guided method needs to reach the
+    plain method's relMSE at 256 spp.  relMSE is taken on films scaled to the
+    reference's unit mean."""
     import numpy as np
     from pathtracer_amd import _lib as L, render as R, scenes
     t0 = time.perf_counter()
@@ -481,11 +488,11 @@ def main_guided(args):
     plain_value = total * args.steps / dt_p / 1e6
     log(f"guided {value:.1f} Mpath/s, plain {plain_value:.1f} Mpath/s")
 
-    # quality at 256x256 against a 32768-spp plain reference
-    q = 256
+    # quality at the frame's own resolution against an 8192-spp plain reference
+    q = res_px
     curve, reach = None, None
     if not args.no_quality:
-        curve, reach = guided_quality(scenes, R, L, q)
+        curve, reach = guided_quality(scenes, R, L, q, scene=scene)
     cst = gst[0][1]
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpath-samples/s", "n_gpus": 1,
@@ -495,12 +502,12 @@ def main_guided(args):
         "config": {"workload": "guided_room: slit-lit 4x3x3 room, Disney floor, 12.5k-tri conductor statue; "
                                "learned guide trained online inside the step",
                    "width": cfg.width, "height": cfg.height, "spp": cfg.spp_x * cfg.spp_y, "max_depth": cfg.max_depth,
-                   "train_spp": cfg.spp_x * cfg.spp_y // 8, "guided_vertices": 1,
+                   "train_spp": cfg.spp_x * cfg.spp_y // 16, "guided_vertices": 1,
                    "triangles": spec.n_triangles},
         "plain": {"value": round(plain_value, 2), "ms_per_step": round(dt_p / args.steps * 1e3, 2)},
         "guide": {k: gst[0][2][k] for k in ("passes", "steps", "samples", "last_loss", "ms_train")},
         "quality": None if curve is None else {
-            "resolution": f"{q}x{q}", "ref_spp": 32768, "curve": curve, "reach": reach},
+            "resolution": f"{q}x{q}", "ref_spp": 8192, "curve": curve, "reach": reach},
         "segments": cst["segments"], "shadow_rays": cst["shadow_rays"],
         "cpu_baseline": None,
     }

@@ -463,10 +463,10 @@ enum {
 };
 typedef struct pt_guide_cfg {
   uint32_t train_spp;      /* samples per pixel rendered while training
-                              (0 = an eighth of the frame's)                */
-  uint32_t first_pass_spp; /* training passes: 1, 2, 4, ... spp (0 = 1)     */
+                              (0 = a sixteenth of the frame's)            */
+  uint32_t first_pass_spp; /* training passes: f, 2f, 4f, ... spp (0: f = 4) */
   uint32_t batch;          /* training batch (0 = 16384)                    */
-  uint32_t max_steps;      /* Adam steps after each training pass (0 = 64)  */
+  uint32_t max_steps;      /* Adam steps after each training pass (0 = 96)  */
   uint32_t record_stride;  /* one path in this many records (0 = 1)         */
   uint32_t record_verts;   /* vertices recorded per path (0 = 4)            */
   uint32_t kinds;          /* material kinds guided, bit per pt_mat_kind
