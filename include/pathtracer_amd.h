@@ -468,7 +468,7 @@ typedef struct pt_guide_cfg {
   uint32_t batch;          /* training batch (0 = 16384)                    */
   uint32_t max_steps;      /* Adam steps after each training pass (0 = 96)  */
   uint32_t record_stride;  /* one path in this many records (0 = 1)         */
-  uint32_t record_verts;   /* vertices recorded per path (0 = 4)            */
+  uint32_t record_verts;   /* vertices recorded per path (0 = guide_verts)  */
   uint32_t kinds;          /* material kinds guided, bit per pt_mat_kind
                               (0 = Lambertian, rough conductor, Disney)     */
   float lr;                /* Adam learning rate (0 = 1e-3)                 */

@@ -1926,7 +1926,7 @@ extern "C" pt_status pt_render_guided(pt_scene *s, const pt_render_cfg *cfg, pt_
   gp.kinds = c.kinds ? c.kinds
                      : ((1u << PT_MAT_LAMBERTIAN) | (1u << PT_MAT_CONDUCTOR) | (1u << PT_MAT_DISNEY));
   gp.stride = c.record_stride ? c.record_stride : 1u;
-  gp.verts = std::min(c.record_verts ? c.record_verts : 4u, 64u);
+
   // Guiding deeper vertices compounds the throughput growth the defensive
   // mixture allows (up to 1 / (1 - c) per vertex): measured on the slit-lit
   // room at 256^2, 256 spp, guiding every vertex raised the relMSE 8x over the
@@ -1934,6 +1934,8 @@ extern "C" pt_status pt_render_guided(pt_scene *s, const pt_render_cfg *cfg, pt_
   // the first two (trimmed relMSE 0.258 and 0.259 against 0.262 and 0.254 over
   // two runs of three films each; plain 0.305) for half the network queries.
   gp.guided_verts = c.guide_verts ? c.guide_verts : 1u;
+  // records are written at guided vertices only: by default one per guided vertex
+  gp.verts = std::min(c.record_verts ? c.record_verts : gp.guided_verts, 64u);
   if (train) {
     g->tcap = 1u << 21;
     HIP_TRY(g->tset.alloc((size_t)g->tcap * PT_G_SAMPLE * sizeof(float)));
