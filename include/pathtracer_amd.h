@@ -404,8 +404,9 @@ pt_status pt_batch_render(pt_batch *batch, const pt_batch_cfg *cfg, pt_stats *st
 /* Film of render `index` (width*height*3 per-pixel means, as pt_film_read). */
 pt_status pt_batch_film_read(pt_batch *batch, uint32_t index, float *out_rgb);
 pt_status pt_batch_set_tuning(pt_batch *batch, const pt_tuning *tuning);
-/* Picks the wavefront size (and grid sizes) for the batch as pt_autotune
- * does for one render; leaf sizes are not tuned here.                     */
+/* Picks the wavefront size, grid sizes, sort-key bits and (PT_TUNE_LEAF)
+ * one BVH leaf size for the batch as pt_autotune does for one render; the
+ * leaf size re-commits the batch's scenes with it.                       */
 pt_status pt_batch_autotune(pt_batch *batch, const pt_autotune_cfg *tc, pt_tuning *out);
 
 /* ------------------------------------------------------------------ */

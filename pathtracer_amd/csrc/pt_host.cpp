@@ -2533,7 +2533,8 @@ extern "C" pt_status pt_autotune(pt_scene *s, const pt_render_cfg *cfg, const pt
 
 extern "C" pt_status pt_batch_autotune(pt_batch *b, const pt_autotune_cfg *tc, pt_tuning *out) {
   if (!b) return fail(PT_ERR_INVALID, "pt_batch_autotune: NULL batch");
-  const uint32_t flags = (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS | PT_TUNE_SORT);
+  const uint32_t flags =
+      (tc && tc->flags) ? tc->flags : (PT_TUNE_WAVEFRONT | PT_TUNE_GRIDS | PT_TUNE_LEAF | PT_TUNE_SORT);
   const uint32_t div = (tc && tc->spp_div) ? tc->spp_div : 8u;
   const uint32_t repeats = (tc && tc->repeats) ? tc->repeats : 1u;
   pt_tuning &t = b->tuning;
@@ -2557,6 +2558,44 @@ extern "C" pt_status pt_batch_autotune(pt_batch *b, const pt_autotune_cfg *tc, p
     best = ms;
   }
   pt_status st = PT_OK;
+  if (flags & PT_TUNE_LEAF) {
+    // one BVH leaf size for every scene of the batch: each candidate
+    // re-commits the batch's distinct scenes (their other build options kept)
+    std::vector<pt_scene *> uniq;
+    for (auto &in : b->insts)
+      if (std::find(uniq.begin(), uniq.end(), in->scene) == uniq.end()) uniq.push_back(in->scene);
+    auto commit_all = [&](uint32_t leaf) -> pt_status {
+      for (pt_scene *sc : uniq) {
+        pt_build_cfg bcfg = sc->build_cfg;
+        if ((bcfg.leaf_size ? bcfg.leaf_size : 4u) == leaf) continue;
+        bcfg.leaf_size = leaf;
+        pt_status r = pt_scene_commit(sc, &bcfg);
+        if (r != PT_OK) return r;
+      }
+      return PT_OK;
+    };
+    const uint32_t leaf0 = t.leaf_size ? t.leaf_size : 4u;
+    uint32_t best_leaf = leaf0;
+    for (uint32_t cand : kTuneLeaf) {
+      if (cand == leaf0) continue;
+      st = commit_all(cand);
+      if (st != PT_OK) return st;
+      double ms = 1e300;
+      for (uint32_t r = 0; r < repeats; ++r) {
+        double x = 0;
+        st = time_one(x);
+        if (st != PT_OK) return st;
+        ms = std::min(ms, x);
+      }
+      if (ms < best) {
+        best = ms;
+        best_leaf = cand;
+      }
+    }
+    st = commit_all(best_leaf);
+    if (st != PT_OK) return st;
+    t.leaf_size = best_leaf;
+  }
   if (flags & PT_TUNE_WAVEFRONT) {
     st = tune_param(&t.wavefront, kTuneWavefront, sizeof(kTuneWavefront) / 4, repeats, time_one, best);
     if (st != PT_OK) return st;

@@ -148,8 +148,18 @@ def test_autotune(gpu):
     assert abs(relit.mean() - before.mean()) < 1e-3 * before.mean()
 
     b = R.Batch()
-    for r, c in _mixed_renders(1):
+    renders = _mixed_renders(1)
+    for r, c in renders:
         b.add(r, c)
+    b.render()
+    films = [b.film(k) for k in range(len(renders))]
     bt = b.autotune(spp_div=2)
     assert bt["wavefront"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
+    assert bt["leaf_size"] in (1, 2, 4, 8)
+    # the batch's scenes stay committed with the chosen leaf size: the same
+    # films up to ties between equal hits
+    b.render()
+    for k in range(len(renders)):
+        f = b.film(k)
+        assert abs(f.mean() - films[k].mean()) <= 1e-3 * films[k].mean() + 1e-6
     b.close()
