@@ -460,7 +460,12 @@ pt_status pt_guide_train_step(pt_guide *guide, const float *samples, size_t n, f
                               float *loss);
 
 enum {
-  PT_GUIDE_NO_TRAIN = 1u << 0    /* render with the current network only     */
+  PT_GUIDE_NO_TRAIN = 1u << 0,   /* render with the current network only     */
+  PT_GUIDE_PATH_QUERIES = 1u << 1 /* query the network at every path's camera
+                                    vertex; by default the camera vertices of
+                                    a pixel share one query (at the hit of
+                                    the pixel's first sample in each chunk),
+                                    which leaves the estimator unbiased     */
 };
 typedef struct pt_guide_cfg {
   uint32_t train_spp;      /* samples per pixel rendered while training

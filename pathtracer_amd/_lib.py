@@ -32,6 +32,7 @@ BUILD_DEVICE_LBVH, BUILD_NO_LDS_NODES = 1, 2
 BUILD_ANY_HIT_QNODES, BUILD_ANY_HIT_FULL_NODES = 4, 8
 INFO_ANY_HIT_QNODES = 1
 GUIDE_NO_TRAIN = 1
+GUIDE_PATH_QUERIES = 2
 GUIDE_INPUTS, GUIDE_OUTPUTS, GUIDE_NPARAMS, GUIDE_MIX = 6, 41, 27497, 48
 PRIM_FLIP_FACE, PRIM_FORCE_INSIDE, PRIM_NOT_LIGHT = 1, 2, 4
 NPARAMS = 30

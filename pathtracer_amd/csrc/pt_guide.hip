@@ -165,7 +165,23 @@ PT_D void g_process(const f32x4 (&o)[4], uint32_t g, float c_lo, float c_span, L
 #endif
 #define PT_GI_PTS (PT_GI_BLOCK / 64 * 16 * PT_GI_T)
 static_assert(PT_GI_PTS <= PT_GI_BLOCK, "one input thread per point");
-__global__ void __launch_bounds__(PT_GI_BLOCK) k_guide_infer(DevScene s, Wave w, int qi, GuideView gv, uint32_t kinds) {
+// With pix (the camera vertex of a guided render), the items are the
+// chunk's pixels rather than its live entries: pixel i of the chunk is
+// evaluated at the hit of its first sample in the chunk, found in the queue
+// where k_generate placed that slot, and its mixture goes to gmix[48 i]
+// (a sample that hit an unguided material still gives the pixel a mixture,
+// from its point's inputs; one that missed, from zero inputs).
+struct InferPixels {
+  uint64_t chunk_first;  // the chunk's first path sample
+  uint32_t sppc, n;      // samples per pixel of the pass, path samples of the chunk
+  uint32_t on;
+};
+PT_D uint32_t entry_of_slot(const Wave &w, uint32_t slot) {  // k_generate's slot -> entry layout
+  const uint32_t run = slot / PT_PART_RUN;
+  return (run % PT_PARTS) * w.part_cap + (run / PT_PARTS) * PT_PART_RUN + slot % PT_PART_RUN;
+}
+__global__ void __launch_bounds__(PT_GI_BLOCK) k_guide_infer(DevScene s, Wave w, int qi, GuideView gv, uint32_t kinds,
+                                                             InferPixels px) {
   __shared__ bf16x8 wf[PT_G_FWD_VALUES / 8];
   __shared__ float bias[PT_G_BIAS_VALUES];
   __shared__ float qin[PT_GI_PTS * PT_GQ_STRIDE];
@@ -178,30 +194,39 @@ __global__ void __launch_bounds__(PT_GI_BLOCK) k_guide_infer(DevScene s, Wave w,
   }
   __syncthreads();
   const uint32_t part = blockIdx.x % PT_PARTS, nb = gridDim.x / PT_PARTS;
-  const uint32_t n = *w.live(qi, part), base = part * w.part_cap;
+  const uint64_t pix0 = px.on ? px.chunk_first / px.sppc : 0u;
+  // items: the partition's live entries, or the chunk's pixels (all blocks)
+  const uint32_t n = px.on ? (uint32_t)((px.chunk_first + px.n - 1u) / px.sppc - pix0 + 1u) : *w.live(qi, part);
+  const uint32_t base = px.on ? 0u : part * w.part_cap;
+  const uint32_t first = px.on ? blockIdx.x * PT_GI_PTS : (blockIdx.x / PT_PARTS) * PT_GI_PTS;
+  const uint32_t step = px.on ? gridDim.x * PT_GI_PTS : nb * PT_GI_PTS;
   const PathState &ps = w.st[qi];
   const uint32_t lane = tid & 63u, wv = tid >> 6, q = lane & 15u, g = lane >> 4;
-  for (uint32_t i0 = (blockIdx.x / PT_PARTS) * PT_GI_PTS; i0 < n; i0 += nb * PT_GI_PTS) {
+  for (uint32_t i0 = first; i0 < n; i0 += step) {
     if (tid < PT_GI_PTS) {
       const uint32_t i = i0 + tid;
       float x[PT_G_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       float ok = 0.f;
       if (i < n) {
-        const uint32_t e = base + i;
+        uint32_t e = base + i;
+        if (px.on) {
+          const uint64_t q0 = (pix0 + i) * px.sppc;
+          e = entry_of_slot(w, (uint32_t)((q0 > px.chunk_first ? q0 : px.chunk_first) - px.chunk_first));
+        }
         const int prim = ps.prim[e];
         if (prim >= 0) {
           const f3 o = mk3(ps.ox[e], ps.oy[e], ps.oz[e]);
           const f3 d = mk3(ps.dx[e], ps.dy[e], ps.dz[e]);
           const SurfHit h = surface(s, prim, o, d, ps.t[e]);
           const pt_material_desc &m = s.mats[h.mat];
-          if (g_guidable(m.kind, m.p, kinds)) {
+          ok = g_guidable(m.kind, m.p, kinds) ? 1.f : 0.f;
+          if (ok != 0.f || px.on) {
             x[0] = fminf(1.f, fmaxf(0.f, (h.p.x - gv.lo[0]) * gv.inv_ext[0]));
             x[1] = fminf(1.f, fmaxf(0.f, (h.p.y - gv.lo[1]) * gv.inv_ext[1]));
             x[2] = fminf(1.f, fmaxf(0.f, (h.p.z - gv.lo[2]) * gv.inv_ext[2]));
             x[3] = fminf(1.f, fmaxf(0.f, (h.sp.n.x + 1.f) * 0.5f));
             x[4] = fminf(1.f, fmaxf(0.f, (h.sp.n.y + 1.f) * 0.5f));
             x[5] = fminf(1.f, fmaxf(0.f, (h.sp.n.z + 1.f) * 0.5f));
-            ok = 1.f;
           }
         }
       }
@@ -226,17 +251,15 @@ __global__ void __launch_bounds__(PT_GI_BLOCK) k_guide_infer(DevScene s, Wave w,
       if (i < n) {
         const bool ok = qin[pl * PT_GQ_STRIDE + 6] != 0.f;
         float4 *mix = reinterpret_cast<float4 *>(w.gmix + (size_t)(base + i) * PT_G_MIX);
-        if (g < 2u && ok) {
+        if (g < 2u && (ok || px.on)) {
 #pragma unroll
           for (int j = 0; j < 4; ++j)
             mix[4 * g + j] = make_float4(r.kappa[j] * r.mux[j], r.kappa[j] * r.muy[j], r.kappa[j] * r.muz[j], r.b[j]);
           mix[8 + g] = make_float4(r.pi[0], r.pi[1], r.pi[2], r.pi[3]);
         } else if (g == 2u) {
-          // c, then the point's inputs (the training records copy them)
-          mix[10] = make_float4(ok ? r.c : 0.f, qin[pl * PT_GQ_STRIDE + 0], qin[pl * PT_GQ_STRIDE + 1],
-                                qin[pl * PT_GQ_STRIDE + 2]);
-          mix[11] = make_float4(qin[pl * PT_GQ_STRIDE + 3], qin[pl * PT_GQ_STRIDE + 4],
-                                qin[pl * PT_GQ_STRIDE + 5], 0.f);
+          // c; per entry, 0 where the point is not guided (a pixel's shared
+          // query keeps c: each path decides by its own material)
+          mix[10] = make_float4(ok || px.on ? r.c : 0.f, 0.f, 0.f, 0.f);
         }
       }
     }
@@ -700,8 +723,10 @@ __global__ void k_guide_collect(Wave w, uint32_t n_rec, uint32_t verts, uint32_t
 static uint32_t blocks_of(size_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 void launch_guide_infer(hipStream_t st, const DevScene &s, const Wave &w, int qi, const GuideView &gv,
-                        uint32_t kinds, uint32_t grid) {
-  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(PT_GI_BLOCK), 0, st, s, w, qi, gv, kinds);
+                        uint32_t kinds, uint32_t grid, bool per_pixel, uint64_t chunk_first, uint32_t sppc,
+                        uint32_t n) {
+  InferPixels px{chunk_first, sppc, n, per_pixel && n > 0u && sppc > 0u ? 1u : 0u};
+  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(PT_GI_BLOCK), 0, st, s, w, qi, gv, kinds, px);
 }
 void launch_guide_points(hipStream_t st, const GuideView &gv, const float *xs, size_t n, float *raw, float *mix) {
   if (n == 0) return;

@@ -25,8 +25,12 @@ struct GuideTrainBufs {
 };
 #define PT_G_SAMPLE 12
 
+// per_pixel: one query per pixel of the chunk (chunk_first, sppc, n: the
+// chunk's first path sample, samples per pixel and path samples), at the hit
+// of the pixel's first sample in the chunk; else one per live entry.
 void launch_guide_infer(hipStream_t st, const DevScene &s, const Wave &w, int qi, const GuideView &gv,
-                        uint32_t kinds, uint32_t grid);
+                        uint32_t kinds, uint32_t grid, bool per_pixel, uint64_t chunk_first, uint32_t sppc,
+                        uint32_t n);
 void launch_guide_points(hipStream_t st, const GuideView &gv, const float *xs, size_t n, float *raw, float *mix);
 void launch_guide_mix(hipStream_t st, const float *mix, const float *dirs, const float *u, size_t n, float *pdf,
                       float *out_dir, float *out_pdf);

@@ -1579,8 +1579,20 @@ static void launch_chunk(hipStream_t st, const DevScene &s, const DevCamera &cam
     launch_extend(st, s, w, qi, n, pl.grids.trav, pl.count, b);
     if (ev) ev->end(e);
     e = ev ? ev->begin(2) : -1;
-    if (pl.gv && d < pl.gverts) launch_guide_infer(st, s, w, qi, *pl.gv, pl.gkinds, pl.infer_grid);
-    launch_shade(st, s, rp, w, qi, d, n, pl.grids.shade, b, pl.mat_kinds);
+    if (pl.gv && d < pl.gverts)
+      launch_guide_infer(st, s, w, qi, *pl.gv, pl.gkinds, pl.infer_grid, d == 0u && w.gpix, rp.chunk_first,
+                         rp.spp_count, n);
+    if (pl.gv && d >= pl.gverts && (!w.grec || d > w.grec_verts)) {
+      // a guided render past its guided vertices, with no training record
+      // left to complete: the vertex is NEE+MIS with the path's last
+      // mixture pdf (carried in the path state), which the plain kernel
+      // shades with fewer registers and less code
+      RenderParams rpm = rp;
+      rpm.estimator = PT_EST_NEE_MIS;
+      launch_shade(st, s, rpm, w, qi, d, n, pl.grids.shade, b, pl.mat_kinds);
+    } else {
+      launch_shade(st, s, rp, w, qi, d, n, pl.grids.shade, b, pl.mat_kinds);
+    }
     if (ev) ev->end(e);
     e = ev ? ev->begin(3) : -1;
     if (pl.estimator != PT_EST_NAIVE && pl.estimator != PT_EST_ROULETTE)  // those trace no shadow rays
@@ -1634,6 +1646,7 @@ struct GuidePass {
   bool record;
   uint32_t stride, verts;
   uint32_t guided_verts;  // vertices 0 .. guided_verts-1 use the guide
+  bool per_pixel;         // one query per pixel at the camera vertex
 };
 
 static pt_status render_frame(pt_scene *s, const pt_render_cfg *cfg, pt_stats *stats, const GuidePass *gp);
@@ -1718,6 +1731,12 @@ static pt_status render_frame(pt_scene *s, const pt_render_cfg *cfg, pt_stats *s
     pl.gkinds = gp->kinds;
     pl.gverts = gp->guided_verts;
     wave.gverts = gp->guided_verts;
+    wave.gpix = gp->per_pixel ? 1u : 0u;
+    wave.gkinds = gp->kinds;
+    for (int a = 0; a < 3; ++a) {
+      wave.glo[a] = gview.lo[a];
+      wave.ginv[a] = gview.inv_ext[a];
+    }
     pl.estimator = PT_EST_GUIDED;
     rp.estimator = PT_EST_GUIDED;
     // the network kernel holds 66 KB of LDS: two resident blocks per CU
@@ -1934,6 +1953,13 @@ extern "C" pt_status pt_render_guided(pt_scene *s, const pt_render_cfg *cfg, pt_
   // the first two (trimmed relMSE 0.258 and 0.259 against 0.262 and 0.254 over
   // two runs of three films each; plain 0.305) for half the network queries.
   gp.guided_verts = c.guide_verts ? c.guide_verts : 1u;
+  // The camera vertices of a pixel lie within a pixel's footprint of one
+  // another, where the network's input (position and normal, 32 bins per
+  // axis) barely changes: they share one query, at the pixel's first sample
+  // in the chunk.  Each path still decides guidability by its own material,
+  // and samples and weights with the mixture it uses, so the estimate stays
+  // unbiased; the training records hold each path's own inputs.
+  gp.per_pixel = !(c.flags & PT_GUIDE_PATH_QUERIES);
   // records are written at guided vertices only: by default one per guided vertex
   gp.verts = std::min(c.record_verts ? c.record_verts : gp.guided_verts, 64u);
   if (train) {

@@ -298,10 +298,12 @@ PT_D uint32_t tile_order16(uint32_t key, uint32_t (*cnt)[16], uint32_t *first, u
 #ifndef PT_SHADE_WAVES_ALL
 #define PT_SHADE_WAVES_ALL 4
 #endif
-// A path whose extension ray left the scene, under the reference estimator
-// or NEE+MIS: the environment or background radiance it adds (under NEE+MIS
-// weighted against the previous vertex's light sample), and the path ends.
-// k_shade's own miss case, for the entries its tiles shade as they read them.
+// A path whose extension ray left the scene, under the reference estimator,
+// NEE+MIS or the guided estimator: the environment or background radiance it
+// adds (weighted against the previous vertex's light sample but for the
+// reference estimator), and the path ends; guided, the previous vertex's
+// training record is completed as k_shade completes it.  k_shade's own miss
+// case, for the entries its tiles shade as they read them.
 template <int EST, bool BATCH>
 PT_D void shade_miss(const DevScene &s0, const RenderParams &rp0, const Wave &w, const PathState &in, uint32_t i,
                      uint32_t d, const BatchView &b) {
@@ -312,6 +314,17 @@ PT_D void shade_miss(const DevScene &s0, const RenderParams &rp0, const Wave &w,
   const f3 dir = mk3(in.dx[i], in.dy[i], in.dz[i]);
   const f3 beta = mk3(in.bx[i], in.by[i], in.bz[i]);
   const bool prev_spec = in.flags[i] & 1u;
+  float *prec = nullptr;
+  if constexpr (EST == PT_EST_GUIDED) {
+    if (w.grec && d >= 1u && d - 1u < w.grec_verts && slot % w.grec_stride == 0u) {
+      prec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + (d - 1u)) * PT_G_REC;
+      prec[13] = w.lx[slot];
+      prec[14] = w.ly[slot];
+      prec[15] = w.lz[slot];
+      prec[17] = __uint_as_float(w.gpass);
+      prec[18] = prec[19] = prec[20] = 0.f;
+    }
+  }
   f3 L = s3(0.f);
   if (EST == PT_EST_REFERENCE_NEE) {
     L = mul3(beta, miss_radiance(s, rp, dir));
@@ -319,7 +332,12 @@ PT_D void shade_miss(const DevScene &s0, const RenderParams &rp0, const Wave &w,
     if (d < rp.max_depth) L = mul3(beta, miss_radiance(s, rp, dir));
   } else if (s.has_env) {
     const float pl = env_pdf(s, dir) / (float)s.n_lights;
-    L = scl3(mul3(beta, env_lookup(s, dir)), power_w(in.ppdf[i], pl));
+    const f3 le = mul3(beta, env_lookup(s, dir));
+    const float wm = power_w(in.ppdf[i], pl);
+    L = scl3(le, wm);
+    if (EST == PT_EST_GUIDED && prec) {  // the target is the unweighted radiance
+      prec[18] = le.x * (1.f - wm); prec[19] = le.y * (1.f - wm); prec[20] = le.z * (1.f - wm);
+    }
   } else {
     L = mul3(beta, miss_radiance(s, rp, dir));
   }
@@ -341,17 +359,17 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
   __shared__ uint32_t o_cnt[4][16], o_first[16];
   __shared__ uint16_t o_perm[256];
   // The block reads its tiles of 256 entries into two LDS lists and shades
-  // each list 256 entries at a time.  Under the reference estimator and
-  // NEE+MIS (without the material-key order) a miss is shaded as it is read,
-  // a few instructions, and only hits join a list; hair hits join the second
-  // list.  The material code then runs on full waves rather than on the hits
+  // each list 256 entries at a time.  Under the reference estimator, NEE+MIS
+  // and the guided estimator (without the material-key order) a miss is
+  // shaded as it is read, a few instructions, and only hits join a list; hair
+  // hits join the second list.  The material code then runs on full waves rather than on the hits
   // among a tile's misses, and a wave runs the hair BSDF or the others, not
   // both.  Shading of the Disney + hair grid frame: 88.6 ms with every entry
   // in place, 86.4 with misses left out, 76.4 with hair apart; the glass /
   // HDRI frame: 25.1 -> 21.4 ms.
   __shared__ uint32_t o_list[2][512];
   __shared__ uint32_t o_n[2];
-  constexpr bool SPLIT = EST == PT_EST_REFERENCE_NEE || EST == PT_EST_NEE_MIS;
+  constexpr bool SPLIT = EST == PT_EST_REFERENCE_NEE || EST == PT_EST_NEE_MIS || EST == PT_EST_GUIDED;
   const bool split = SPLIT && !w.sort_keys;  // uniform
   if (threadIdx.x < 2u) o_n[threadIdx.x] = 0u;
   __syncthreads();
@@ -718,8 +736,17 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
               float gc = 0.f;
               bool use_guide = false;
               if constexpr (EST == PT_EST_GUIDED) {
-                mix = w.gmix + (size_t)i * PT_G_MIX;
-                gc = d < w.gverts ? mix[40] : 0.f;
+                if (d < w.gverts) {
+                  if (d == 0u && w.gpix) {
+                    // the query the camera vertices of this pixel share; whether
+                    // to guide follows this path's own material
+                    mix = w.gmix + (size_t)(pixel - (uint32_t)(rp.chunk_first / rp.spp_count)) * PT_G_MIX;
+                    gc = g_guidable(mat->kind, mat->p, w.gkinds) ? mix[40] : 0.f;
+                  } else {
+                    mix = w.gmix + (size_t)i * PT_G_MIX;
+                    gc = mix[40];
+                  }
+                }
                 use_guide = gc > 0.f && pt_rng(g, rslot(d, SLOT_GUIDE), rp.seed) < gc;
               }
               bool scat;
@@ -844,7 +871,13 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
                   pm = gc * g_pdf(mix, wn) + (1.f - gc) * pbs;
                   if (w.grec && d < w.grec_verts && slot % w.grec_stride == 0u) {
                     float *rec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + d) * PT_G_REC;
-                    for (int v = 0; v < 6; ++v) rec[v] = mix[41 + v];
+                    // the network's inputs at this point (k_guide_infer's)
+                    rec[0] = fminf(1.f, fmaxf(0.f, (p.x - w.glo[0]) * w.ginv[0]));
+                    rec[1] = fminf(1.f, fmaxf(0.f, (p.y - w.glo[1]) * w.ginv[1]));
+                    rec[2] = fminf(1.f, fmaxf(0.f, (p.z - w.glo[2]) * w.ginv[2]));
+                    rec[3] = fminf(1.f, fmaxf(0.f, (h.sp.n.x + 1.f) * 0.5f));
+                    rec[4] = fminf(1.f, fmaxf(0.f, (h.sp.n.y + 1.f) * 0.5f));
+                    rec[5] = fminf(1.f, fmaxf(0.f, (h.sp.n.z + 1.f) * 0.5f));
                     rec[6] = wn.x; rec[7] = wn.y; rec[8] = wn.z;
                     rec[9] = pbs;
                     rec[10] = beta.x; rec[11] = beta.y; rec[12] = beta.z;

@@ -307,13 +307,17 @@ struct Wave {
   float *rc;
   uint8_t *rvis;
   // path guiding (allocated for guided renders only): the processed mixture
-  // of live entry e at gmix[48 e]; training records of vertex v of slot
-  // r * grec_stride at grec[20 (r * grec_verts + v)]; the pass number tags
-  // the records this pass wrote
+  // of live entry e at gmix[48 e] (with gpix, the camera vertex's mixture of
+  // pixel p at gmix[48 (p - the chunk's first pixel)]); training records of
+  // vertex v of slot r * grec_stride at grec[24 (r * grec_verts + v)]; the
+  // pass number tags the records this pass wrote
   float *gmix;
   float *grec;
   uint32_t grec_stride, grec_verts, gpass;
   uint32_t gverts;  // vertices 0 .. gverts-1 are guided
+  uint32_t gpix;    // 1: one network query per pixel at the camera vertex
+  uint32_t gkinds;  // material kinds guided (g_guidable)
+  float glo[3], ginv[3];  // the network's position normalisation
   uint32_t cap, part_cap, shadow_part_cap;
   uint32_t sort_keys;  // 1: shading orders each tile by material kind
   PT_HD uint32_t *live(uint32_t b, uint32_t p) const { return counts + (b * PT_PARTS + p) * PT_COUNTER_STRIDE; }
