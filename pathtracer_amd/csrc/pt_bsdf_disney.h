@@ -17,11 +17,22 @@
 namespace pt {
 
 PT_D float rgb_luminance(f3 c) { return dot3(c, mk3(0.2126f, 0.7152f, 0.0722f)); }
+// x^5 rounded to float: three double products (relative error below 2^-51)
+// round to the float nearest x^5, the value a correctly rounded pow gives,
+// unless x^5 lies within 2^-51 of the midpoint between two floats.  A tenth
+// of the instructions of powf, which the Schlick weights call several times
+// per Disney evaluation.
+PT_D float pow5_rn(double x) {
+  const double x2 = x * x;
+  return (float)(x2 * x2 * x);
+}
+// materialMath.hpp:55-63: fresnelSchlick raises the double 1.0 - cos to the
+// fifth power, hackedSchlick the float 1.f - cos
 PT_D f3 fresnel_schlick(float c, f3 F0) {
-  float w = powf(1.f - c, 5.f);
+  float w = pow5_rn(1.0 - (double)c);
   return mk3(F0.x + (1.f - F0.x) * w, F0.y + (1.f - F0.y) * w, F0.z + (1.f - F0.z) * w);
 }
-PT_D float hacked_schlick(float c, float F90) { return 1.f + (F90 - 1.f) * powf(1.f - c, 5.f); }
+PT_D float hacked_schlick(float c, float F90) { return 1.f + (F90 - 1.f) * pow5_rn((double)(1.f - c)); }
 PT_D TRD trd_from_roughness(float roughness, float aniso) {
   aniso = fmaxf(0.0f, fminf(aniso, 1.0f / 0.9f - 0.00001f));
   float aspect = safe_sqrtf(1.0f - 0.9f * aniso);
@@ -257,7 +268,7 @@ PT_D f3 dsheen_f(const Disney &d, f3 wo_w, f3 wi_w, const ShadePoint &h) {
   float lum = rgb_luminance(d.base);
   f3 ct = lum > 0.f ? divs3(d.base, lum) : s3(1.f);
   f3 cs = add3(s3(1.f), scl3(sub3(ct, s3(1.f)), d.sheen_tint));
-  return scl3(cs, powf(1 - a, 5.f));
+  return scl3(cs, pow5_rn((double)(1 - a)));
 }
 
 PT_D f3 disney_f(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h) {
@@ -369,7 +380,7 @@ PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint
     float lum = rgb_luminance(d.base);
     f3 ct = lum > 0.f ? divs3(d.base, lum) : s3(1.f);
     f3 cs = add3(s3(1.f), scl3(sub3(ct, s3(1.f)), d.sheen_tint));
-    fs = scl3(cs, powf(1 - a, 5.f));
+    fs = scl3(cs, pow5_rn((double)(1 - a)));
   }
   f3 r = scl3(fd, d.dw);
   r = add3(r, scl3(fm, d.mw));
