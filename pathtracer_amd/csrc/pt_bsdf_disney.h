@@ -17,15 +17,23 @@
 namespace pt {
 
 PT_D float rgb_luminance(f3 c) { return dot3(c, mk3(0.2126f, 0.7152f, 0.0722f)); }
-// x^5 rounded to float: three double products (relative error below 2^-51)
-// round to the float nearest x^5, the value a correctly rounded pow gives,
-// unless x^5 lies within 2^-51 of the midpoint between two floats.  A tenth
-// of the instructions of powf, which the Schlick weights call several times
-// per Disney evaluation.
-PT_D float pow5_rn(double x) {
-  const double x2 = x * x;
-  return (float)(x2 * x2 * x);
+// x^N for a constant integer N by repeated squaring in double.  Rounded to
+// float, the product (relative error below N 2^-53) is the float nearest x^N,
+// the value a correctly rounded pow gives, unless x^N lies within that error
+// of the midpoint between two floats.  A fraction of the instructions of
+// powf, which the Schlick weights call several times per Disney evaluation.
+template <int N>
+PT_D double powi_d(double x) {
+  if constexpr (N == 1) {
+    return x;
+  } else if constexpr (N % 2 == 0) {
+    const double h = powi_d<N / 2>(x);
+    return h * h;
+  } else {
+    return powi_d<N - 1>(x) * x;
+  }
 }
+PT_D float pow5_rn(double x) { return (float)powi_d<5>(x); }
 // materialMath.hpp:55-63: fresnelSchlick raises the double 1.0 - cos to the
 // fifth power, hackedSchlick the float 1.f - cos
 PT_D f3 fresnel_schlick(float c, f3 F0) {
@@ -477,11 +485,11 @@ PT_D Hair hair_params(const pt_material_desc &m) {
   H.h = m.p[3];
   H.eta = m.p[4];
   float bm = m.p[5], bn = m.p[6], alpha = m.p[7];
-  H.v[0] = sqrf(0.726f * bm + 0.812f * sqrf(bm) + 3.7f * powf(bm, 20.f));
+  H.v[0] = sqrf(0.726f * bm + 0.812f * sqrf(bm) + 3.7f * (float)powi_d<20>((double)bm));
   H.v[1] = 0.25f * H.v[0];
   H.v[2] = 4 * H.v[0];
   H.v[3] = H.v[2];
-  H.s = 0.626657069f * (0.265f * bn + 1.194f * sqrf(bn) + 5.372f * powf(bn, 22.f));
+  H.s = 0.626657069f * (0.265f * bn + 1.194f * sqrf(bn) + 5.372f * (float)powi_d<22>((double)bn));
   H.sin2k[0] = sinf(alpha);
   H.cos2k[0] = safe_sqrtf(1 - sqrf(H.sin2k[0]));
 #pragma unroll

@@ -298,6 +298,22 @@ PT_D uint32_t tile_order16(uint32_t key, uint32_t (*cnt)[16], uint32_t *first, u
 #ifndef PT_SHADE_WAVES_ALL
 #define PT_SHADE_WAVES_ALL 4
 #endif
+// The pixel, sample index and random-stream index of a path slot.
+template <bool BATCH>
+PT_D void path_index(const RenderParams &rp, const BatchView &b, uint32_t slot, uint32_t &pixel, uint32_t &k,
+                     uint32_t &g) {
+  if (BATCH) {
+    pixel = b.slot_pix[slot];
+    k = b.slot_k[slot];
+  } else {
+    const uint64_t q = rp.chunk_first + slot;
+    pixel = (uint32_t)(q / rp.spp_count);
+    k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
+  }
+  const uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
+  g = 1u + k + rp.spp * (pj + rp.height * pi);
+}
+
 // A path whose extension ray left the scene, under the reference estimator,
 // NEE+MIS or the guided estimator: the environment or background radiance it
 // adds (weighted against the previous vertex's light sample but for the
@@ -435,17 +451,8 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
           const RenderParams &rp = BATCH ? b.params[inst] : rp0;
           const uint32_t D = rp.max_depth;
           const int N = s.n_lights;
-          uint32_t pixel, k;
-          if (BATCH) {
-            pixel = b.slot_pix[slot];
-            k = b.slot_k[slot];
-          } else {
-            uint64_t q = rp.chunk_first + slot;
-            pixel = (uint32_t)(q / rp.spp_count);
-            k = rp.spp_begin + (uint32_t)(q % rp.spp_count);
-          }
-          uint32_t pi = pixel % rp.width, pj = pixel / rp.width;
-          uint32_t g = 1u + k + rp.spp * (pj + rp.height * pi);
+          uint32_t pixel, k, g;
+          path_index<BATCH>(rp, b, slot, pixel, k, g);
           f3 o = mk3(in.ox[i], in.oy[i], in.oz[i]);
           f3 dir = mk3(in.dx[i], in.dy[i], in.dz[i]);
           float t = in.t[i];
