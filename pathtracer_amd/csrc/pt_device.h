@@ -648,8 +648,8 @@ PT_D f3 smooth_normal(const float4 *r, f3 o, f3 d, f3 ng) {
 
 PT_D SurfHit surface(const DevScene &s, int prim, f3 o, f3 d, float t) {
   SurfHit h;
-  uint2 info = s.prim_info[prim];
-  float4 g = s.prim_geom[prim];
+  uint2 info = prim_info(s, prim);
+  float4 g = prim_geom(s, prim);
   uint32_t type = (info.x >> 24) & 3u, fl = info.x >> 28;
   h.mat = info.x & 0xffffffu;
   h.light = (type == PRIM_TRI || type == PRIM_CURVE) ? -1 : (int)info.y;
@@ -736,9 +736,9 @@ PT_D f3 texture_lookup(const DevScene &s, uint32_t id, float u, float v) {
 //   curve   (0, 0): Curve sets none
 #pragma clang fp contract(off)
 PT_D float2 hit_uv(const DevScene &s, int prim, const SurfHit &h, f3 o, f3 d) {
-  const uint2 info = s.prim_info[prim];
+  const uint2 info = prim_info(s, prim);
   const uint32_t type = (info.x >> 24) & 3u;
-  const float4 g = s.prim_geom[prim];
+  const float4 g = prim_geom(s, prim);
   if (type == PRIM_SPHERE) {
     const f3 p = norm3(sub3(h.p, mk3(g.x, g.y, g.z)));
     const float phi = atan2f(p.z, p.x), theta = asinf(p.y);

@@ -105,7 +105,7 @@ struct pt_scene {
   // device
   hipStream_t stream = nullptr;
   int device = -1;
-  DevBuf d_nodes, d_qnodes, d_prims, d_info, d_geom, d_smooth, d_curve, d_mats, d_lights, d_env, d_env_func, d_env_cdf,
+  DevBuf d_nodes, d_qnodes, d_prims, d_prec, d_smooth, d_curve, d_mats, d_lights, d_env, d_env_func, d_env_cdf,
       d_env_mcdf, d_env_guide;
   DevBuf d_perm, d_perm_off;
   DevBuf d_tex, d_tex_desc, d_prim_tex, d_tri_uv, d_mesh_box;  // textures and texture coordinates
@@ -1227,8 +1227,16 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   }
 
   // -- upload
-  HIP_TRY(upload(s->d_info, info.data(), info.size() * sizeof(uint2)));
-  HIP_TRY(upload(s->d_geom, geom.data(), geom.size() * sizeof(float4)));
+  {
+    // one 32-byte record per primitive: geom, then info and padding
+    std::vector<float4> rec(2 * geom.size());
+    for (size_t i = 0; i < geom.size(); ++i) {
+      rec[2 * i] = geom[i];
+      uint32_t w[4] = {info[i].x, info[i].y, 0u, 0u};
+      std::memcpy(&rec[2 * i + 1], w, sizeof(w));
+    }
+    HIP_TRY(upload(s->d_prec, rec.data(), rec.size() * sizeof(float4)));
+  }
   HIP_TRY(upload(s->d_curve, cgeom.data(), cgeom.size() * sizeof(float4)));
   if (smooth.empty())
     HIP_TRY(s->d_smooth.alloc(sizeof(float4)));
@@ -1251,8 +1259,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   ds.nodes = s->d_nodes.as<float4>();
   ds.qnodes = s->d_qnodes.as<float4>();
   ds.prims = s->d_prims.as<float4>();
-  ds.prim_info = s->d_info.as<uint2>();
-  ds.prim_geom = s->d_geom.as<float4>();
+  ds.prim_rec = s->d_prec.as<float4>();
   ds.tri_smooth = s->d_smooth.as<float4>();
   ds.curve_geom = s->d_curve.as<float4>();
   ds.mats = s->d_mats.as<pt_material_desc>();
@@ -1298,7 +1305,7 @@ extern "C" pt_status pt_scene_commit(pt_scene *s, const pt_build_cfg *cfg) {
   in.bvh_nodes = n_nodes4;
   in.bvh_depth = depth4;
   in.bvh_ms = bvh_ms;
-  in.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_prims.bytes + s->d_info.bytes + s->d_geom.bytes + s->d_smooth.bytes +
+  in.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_prims.bytes + s->d_prec.bytes + s->d_smooth.bytes +
                     s->d_env.bytes +
                     s->d_env_func.bytes + s->d_env_cdf.bytes;
   in.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -403,7 +403,7 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
         } else if (PT_IN(MS, PT_MAT_HAIR)) {
           const DevScene &s = BATCH ? b.scenes[b.slot_inst[in.slot[base + e]]] : s0;
           if (s.mat_kinds & (1u << PT_MAT_HAIR))
-            cls = s.mats[s.prim_info[pr].x & 0xffffffu].kind == PT_MAT_HAIR ? 1u : 0u;
+            cls = s.mats[prim_info(s, pr).x & 0xffffffu].kind == PT_MAT_HAIR ? 1u : 0u;
         }
       }
 #pragma unroll
@@ -431,7 +431,7 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
             key = 0u;
             if (pr >= 0) {
               const DevScene &s = BATCH ? b.scenes[b.slot_inst[in.slot[e]]] : s0;
-              key = s.mats[s.prim_info[pr].x & 0xffffffu].kind & 15u;
+              key = s.mats[prim_info(s, pr).x & 0xffffffu].kind & 15u;
             }
           }
           jl = tile_order16(key, o_cnt, o_first, o_perm);

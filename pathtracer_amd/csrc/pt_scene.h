@@ -14,13 +14,15 @@
 //                rect      (k, d0min, d0max, id|type) (d1min, d1max, axis, -) (-)
 //              e1 = p1-p0, e2 = p2-p0 are formed on the host exactly as
 //              MeshBVHNode::hitTriangle forms them (meshBVH.cpp:93-94).
-//   prim_info  per global primitive id: x = material | type<<24 | flags<<28,
-//              y = light index (spheres, rects) or shading-normal record
-//              (triangles), -1 for none.
+//   prim_rec   per global primitive id, 32 B (one cache line holds four):
+//                geom  triangle geometric normal, sphere centre+radius,
+//                      rect axis (float4)
+//                info  x = material | type<<24 | flags<<28, y = light index
+//                      (spheres, rects) or shading-normal record
+//                      (triangles), -1 for none (uint2), then 8 B of padding
+//              Shading reads both halves of a hit's record: one line.
 //   tri_smooth per triangle with shading normals, 80 B: the vertex p0 and
 //              edges e1, e2 (for the hit's barycentrics) and three normals.
-//   prim_geom  per global primitive id: triangle geometric normal,
-//              sphere centre+radius, rect axis.
 #pragma once
 #include <stdint.h>
 #include "pt_math.h"
@@ -183,8 +185,7 @@ struct DevScene {
   const float4 *qnodes;  // 64-byte quantised nodes (any hit when qany)
   uint32_t qany;         // any-hit traversal reads qnodes
   const float4 *prims;
-  const uint2 *prim_info;
-  const float4 *prim_geom;
+  const float4 *prim_rec;  // 2 float4 per primitive: geom, info
   // textures (only when a material is textured, else null): texel pool,
   // 2 float4 records per texture, and the data texture coordinates need
   // (pt_host.cpp commit_textures)
@@ -217,6 +218,10 @@ struct DevScene {
   const uint32_t *perm_off;
   uint32_t perm_seed;
 };
+PT_HD float4 prim_geom(const DevScene &s, int prim) { return s.prim_rec[2 * (size_t)prim]; }
+PT_HD uint2 prim_info(const DevScene &s, int prim) {
+  return *reinterpret_cast<const uint2 *>(s.prim_rec + 2 * (size_t)prim + 1);
+}
 
 struct DevCamera {
   f3 pos, pos1, h, v, horiz, vert, lookat;
