@@ -655,11 +655,13 @@ static v3 trace_path(const rctx *c, uint32_t i, uint32_t j, uint32_t k, uint64_t
         float pl = 0.f;
         int vis = 0;
         if (Lg->type == OR_LIGHT_ENV) {
-          lwi = or_env_sample_dir(s, e0, e1, &pl);
+          float euv[2];
+          lwi = or_env_sample_dir(s, e0, e1, &pl, euv);
           if (pl > 0.f) {
             if (cnt) cnt[2]++;
             vis = !or_scene_intersect(s, h.p, lwi, 0.001f, FLT_MAX, 1, NULL, NULL);
-            le = or_env_lookup(s, lwi);
+            /* the radiance at the sampled texture coordinates (the direction's own) */
+            le = euv[0] >= 0.f ? or_env_lookup_uv(s, euv[0], euv[1]) : or_env_lookup(s, lwi);
           }
         } else {
           v3 nl;

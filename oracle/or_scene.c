@@ -926,10 +926,11 @@ static int find_interval(const float *cdf, int n, float u) {
 }
 
 /* 2-D piecewise-constant sampling of the environment (new). */
-v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf) {
+v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf, float *uv) {
   int W = s->env_w, H = s->env_h;
   if (!(s->env_integral > 0.f)) {
     *pdf = (float)(1.0 / (4.0 * M_PI));
+    if (uv) uv[0] = -1.f, uv[1] = 0.f;
     return or_uniform_sphere(u0, u1);
   }
   int y = find_interval(s->env_mcdf, H, u1);
@@ -942,6 +943,7 @@ v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf) {
   float wx = cdf[x + 1] - cdf[x];
   if (wx > 0.f) dx /= wx;
   float u = ((float)x + dx) / (float)W, v = ((float)y + dy) / (float)H;
+  if (uv) uv[0] = u, uv[1] = v;
   float theta = v * (float)M_PI, phi = u * 2.f * (float)M_PI;
   float st = sinf(theta);
   v3 dir = V3(-st * cosf(phi), -cosf(theta), st * sinf(phi));
@@ -971,7 +973,7 @@ float or_env_pdf(const or_scene *s, const float *dir) { return or_env_pdf_dir(s,
 
 int or_env_sample_batch(const or_scene *s, const float *u2, size_t n, float *dir, float *pdf) {
   for (size_t i = 0; i < n; ++i) {
-    v3 d = or_env_sample_dir(s, u2[2 * i], u2[2 * i + 1], &pdf[i]);
+    v3 d = or_env_sample_dir(s, u2[2 * i], u2[2 * i + 1], &pdf[i], NULL);
     dir[3 * i] = d.x; dir[3 * i + 1] = d.y; dir[3 * i + 2] = d.z;
   }
   return 0;

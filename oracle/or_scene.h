@@ -100,7 +100,8 @@ int or_prim_light(const or_scene *s, int prim);
 v3 or_env_lookup(const or_scene *s, v3 dir);
 v3 or_env_lookup_uv(const or_scene *s, float u, float v);
 v3 or_uniform_sphere(float u1, float u2);
-v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf);
+/* uv (may be NULL): the sampled texture coordinates, uv[0] < 0 for uniform sampling */
+v3 or_env_sample_dir(const or_scene *s, float u0, float u1, float *pdf, float *uv);
 float or_env_pdf_dir(const or_scene *s, v3 dir);
 v3 or_light_point(const or_scene *s, const or_light *L, float u, float v, v3 *nrm);
 float or_shape_pdf(const or_scene *s, const or_light *L, v3 o, v3 wi);

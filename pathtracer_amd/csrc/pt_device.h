@@ -875,10 +875,14 @@ PT_D uint32_t find_interval_guided(const float *cdf, uint32_t n, float u, const 
 }
 // 2-D piecewise-constant environment sampling (the reference samples the
 // environment uniformly, texture.hpp:120-131; this is the MIS estimator's).
-PT_D f3 env_sample(const DevScene &s, float u0, float u1, float &pdf) {
+// uv (optional) receives the sampled texture coordinates, which the returned
+// direction maps back to, so that the radiance along it is read there
+// without the inverse mapping; x < 0 where the map is sampled uniformly.
+PT_D f3 env_sample(const DevScene &s, float u0, float u1, float &pdf, float2 *uv = nullptr) {
   uint32_t W = s.env_w, H = s.env_h;
   if (!(s.env_integral > 0.f)) {
     pdf = 1.f / (4.f * PT_PI);
+    if (uv) *uv = make_float2(-1.f, 0.f);
     return uniform_sphere(u0, u1);
   }
   uint32_t y = find_interval_guided(s.env_mcdf, H, u1, s.env_mguide, H);
@@ -889,6 +893,7 @@ PT_D f3 env_sample(const DevScene &s, float u0, float u1, float &pdf) {
   float dx = u0 - cdf[x], wx = cdf[x + 1] - cdf[x];
   if (wx > 0.f) dx /= wx;
   float u = ((float)x + dx) / (float)W, v = ((float)y + dy) / (float)H;
+  if (uv) *uv = make_float2(u, v);
   float th = v * PT_PI, ph = u * 2.f * PT_PI;
   float st = sinf(th);
   float f = s.env_func[(size_t)y * W + x];

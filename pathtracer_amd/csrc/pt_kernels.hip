@@ -838,8 +838,9 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
                   f3 lwi = s3(0.f), le = s3(0.f);
                   float pl = 0.f, tmax = 0.f;
                   if (Lg.type == LIGHT_ENV) {
-                    lwi = env_sample(s, e0, e1, pl);
-                    le = env_lookup(s, lwi);
+                    float2 euv;
+                    lwi = env_sample(s, e0, e1, pl, &euv);
+                    le = euv.x >= 0.f ? env_lookup_uv(s, euv.x, euv.y) : env_lookup(s, lwi);
                     tmax = 3.402823466e+38f;
                   } else {
                     f3 nl;
