@@ -116,7 +116,8 @@ PT_D float dmetal_pdf(const Disney &d, f3 wo, f3 wi, const ShadePoint &h) {
 // lobe's pdf for its cut-off, so it asks for the value only where the lobe is
 // specular.
 template <bool ATT>
-PT_D bool dmetal_scatter(const Disney &d, f3 wo, const ShadePoint &h, float u0, float u1, Scatter &s) {
+PT_D bool dmetal_scatter(const Disney &d, const Frame &b, f3 wo_l, f3 wo, const ShadePoint &h, float u0, float u1,
+                         Scatter &s) {
   if (trd_smooth(d.dist)) {
     f3 wi = norm3(reflect_pbrt(wo, h.n));
     if (dot3(h.n, wi) <= 0.f) return false;
@@ -126,8 +127,6 @@ PT_D bool dmetal_scatter(const Disney &d, f3 wo, const ShadePoint &h, float u0, 
     s.pdf = 0.f;
     return true;
   }
-  Frame b = make_frame(h.n);
-  f3 wo_l = to_local(b, norm3(wo));
   f3 ml = trd_sample_wm(d.dist, wo_l, u0, u1);
   f3 wi_l = reflect_pbrt(wo_l, ml);
   if (!same_hemi(wo_l, wi_l)) wi_l.z = -wo_l.z;
@@ -166,7 +165,8 @@ PT_D float dglass_pdf(const Disney &d, f3 wo, f3 wi, const ShadePoint &h) {
   return mfdiel_pdf_local(d.eta, d.dist, to_local(b, norm3(wo)), to_local(b, norm3(wi)));
 }
 template <bool ATT>
-PT_D bool dglass_scatter(const Disney &d, f3 wo, const ShadePoint &h, float uc, float u0, float u1, Scatter &s) {
+PT_D bool dglass_scatter(const Disney &d, const Frame &b, f3 wo_l, f3 wo, const ShadePoint &h, float uc, float u0,
+                         float u1, Scatter &s) {
   float eta = d.eta;
   TRD t = d.dist;
   if (eta == 1.f || trd_smooth(t)) {
@@ -189,8 +189,6 @@ PT_D bool dglass_scatter(const Disney &d, f3 wo, const ShadePoint &h, float uc, 
     s.pdf = T;
     return true;
   }
-  Frame b = make_frame(h.n);
-  f3 wo_l = to_local(b, norm3(wo));
   f3 ml = trd_sample_wm(t, wo_l, u0, u1);
   float R = fresnel_diel(dot3(wo_l, ml), eta), T = 1.f - R;
   if (R + T == 0.f) return false;
@@ -248,9 +246,8 @@ PT_D float dcc_pdf(const Disney &d, f3 wo, f3 wi, const ShadePoint &h) {
   return cc_pdf_at(d.cc_alpha, b, to_local(b, norm3(wo)), wi);
 }
 template <bool ATT>
-PT_D bool dcc_scatter(const Disney &d, f3 wo, const ShadePoint &h, float u0, float u1, Scatter &s) {
-  Frame b = make_frame(h.n);
-  f3 wo_l = to_local(b, norm3(wo));
+PT_D bool dcc_scatter(const Disney &d, const Frame &b, f3 wo_l, f3 wo, const ShadePoint &h, float u0, float u1,
+                      Scatter &s) {
   float a = d.cc_alpha;
   float upper = 1 - powf(sqrf(a), 1 - u0);
   float lower = 1 - sqrf(a);
@@ -301,7 +298,7 @@ PT_D float disney_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint 
 }
 // Glass lobe f and pdf from the normalised local directions (dglass_f and
 // dglass_pdf's operations, shared terms once).
-PT_D void dglass_f_pdf_local(const Disney &d, f3 wo_l, f3 wi_l, f3 &f, float &pdf) {
+PT_D void dglass_f_pdf_local(const Disney &d, f3 wo_l, f3 wi_l, float lo, f3 &f, float &pdf) {
   f = s3(0.f);
   pdf = 0.f;
   const float eta = d.eta;
@@ -316,7 +313,7 @@ PT_D void dglass_f_pdf_local(const Disney &d, f3 wo_l, f3 wi_l, f3 &f, float &pd
   wm = face_fwd(norm3(wm), mk3(0.f, 0.f, 1.f));
   if (dot3(wm, wi_l) * ci < 0.f || dot3(wm, wo_l) * co < 0.f) return;
   const float F = fresnel_diel(dot3(wo_l, wm), eta), T = 1.f - F;
-  const float D = trd_D(t, wm), lo = trd_lambda(t, wo_l);
+  const float D = trd_D(t, wm);  // lo = trd_lambda(t, wo_l)
   const float G = 1.f / (1.f + lo + trd_lambda(t, wi_l));
   const float Dw = 1.f / (1.f + lo) / abs_cost(wo_l) * D * fabsf(dot3(wo_l, wm));
   if (refl) {
@@ -333,24 +330,43 @@ PT_D void dglass_f_pdf_local(const Disney &d, f3 wo_l, f3 wi_l, f3 &f, float &pd
   }
 }
 
-// disney_f and disney_pdf in one pass: the parameters, the shading frame, the
-// local directions (as given and normalised, as the lobes use them), the half
-// vectors and the shared microfacet terms once.  Every lobe keeps its own
-// early-outs and operation order, so f and pdf equal the separate functions'
-// bit for bit.
-PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
-  const Disney d = disney_params(m, h);
-  const Frame b = make_frame(h.n);
-  const f3 wo_n = to_local(b, norm3(wo)), wi_n = to_local(b, norm3(wi));
+// What the scatter and every evaluation of one shading point share: the
+// parameters, the shading frame, wo in it (as given and normalised, as the
+// lobes use them) and wo's masking terms.
+struct DisneyCtx {
+  Disney d;
+  Frame b;
+  f3 wo_n, wo_r;
+  float lam_n, lam_r, cc_lam_r;  // trd_lambda(wo_n), trd_lambda(wo_r), cc_lambda(wo_r)
+};
+PT_D DisneyCtx disney_ctx(const pt_material_desc &m, const ShadePoint &h, f3 wo) {
+  DisneyCtx c;
+  c.d = disney_params(m, h);
+  c.b = make_frame(h.n);
+  c.wo_n = to_local(c.b, norm3(wo));
+  c.wo_r = to_local(c.b, wo);
+  c.lam_n = trd_lambda(c.d.dist, c.wo_n);
+  c.lam_r = trd_lambda(c.d.dist, c.wo_r);
+  c.cc_lam_r = cc_lambda(c.wo_r);
+  return c;
+}
+// disney_f and disney_pdf in one pass: the half vectors and the shared
+// microfacet terms once, wo's terms from the context.  Every lobe keeps its
+// own early-outs and operation order, so f and pdf equal the separate
+// functions' bit for bit.
+PT_D void disney_f_pdf_ctx(const DisneyCtx &c, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
+  const Disney &d = c.d;
+  const Frame &b = c.b;
+  const f3 wo_n = c.wo_n, wi_n = to_local(b, norm3(wi));
   f3 gf;
   float gp;
-  dglass_f_pdf_local(d, wo_n, wi_n, gf, gp);
+  dglass_f_pdf_local(d, wo_n, wi_n, c.lam_n, gf, gp);
   if (h.inside) {
     f = gf;
     pdf = gp;
     return;
   }
-  const f3 wo_r = to_local(b, wo), wi_r = to_local(b, wi);  // ddiffuse_f, dmetal_f_c0, dcc_f, dsheen_f
+  const f3 wo_r = c.wo_r, wi_r = to_local(b, wi);  // ddiffuse_f, dmetal_f_c0, dcc_f, dsheen_f
   const f3 wh = norm3(add3(wo_r, wi_r));
   const float whwi = dot3(wh, wi_r);
   // diffuse (ddiffuse_f)
@@ -364,13 +380,13 @@ PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint
     bs = scl3(bs, hacked_schlick(aci, fss90) * hacked_schlick(aco, fss90) * (1.f / (aci + aco) - 0.5f) + 0.5f);
     fd = add3(scl3(bd, 1.f - d.subsurface), scl3(bs, d.subsurface));
   }
-  // metal (dmetal_f_variant -> dmetal_f_c0)
+  // metal (dmetal_f_variant -> dmetal_f_c0; trd_G with wo's lambda)
   f3 fm = s3(0.f);
   if (!(wi_r.z <= 0.f || wo_r.z <= 0.f || trd_smooth(d.dist))) {
     f3 ks = add3(s3(0.5f), scl3(d.base, 0.5f));
     f3 c0 = add3(scl3(scl3(ks, 0.f * (1.f - 0.5f)), 0.5f), scl3(d.base, 0.5f));
     f3 F = fresnel_schlick(fabsf(whwi), c0);
-    float D = trd_D(d.dist, wh), G = trd_G(d.dist, wo_r, wi_r);
+    float D = trd_D(d.dist, wh), G = 1.f / (1.f + c.lam_r + trd_lambda(d.dist, wi_r));
     fm = divs3(scl3(scl3(F, D), G), fabsf(4.f * wi_r.z * wo_r.z));
   }
   // clearcoat (dcc_f)
@@ -378,7 +394,7 @@ PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint
   if (!(wi_r.z <= 0.f || wo_r.z <= 0.f)) {
     float r0 = sqrf((1.5f - 1) / (1.5f + 1));
     f3 Fc = fresnel_schlick(fabsf(whwi), s3(r0));
-    float Dc = cc_D(d.cc_alpha, wh), Gc = 1.f / (1.f + cc_lambda(wo_r) + cc_lambda(wi_r));
+    float Dc = cc_D(d.cc_alpha, wh), Gc = 1.f / (1.f + c.cc_lam_r + cc_lambda(wi_r));
     fc = divs3(scl3(scl3(Fc, Dc), Gc), fabsf(4.f * wi_r.z * wo_r.z));
   }
   // sheen (dsheen_f)
@@ -400,9 +416,10 @@ PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint
   float pm = 0.f, pc = 0.f;
   if (wi_n.z > 0.f) {
     const f3 hn = norm3(add3(wo_n, wi_n));
-    if (!trd_smooth(d.dist)) {  // dmetal_pdf -> mf_pdf_at
+    if (!trd_smooth(d.dist)) {  // dmetal_pdf -> mf_pdf_at -> trd_Dw with wo's G1
       float den = 4.f * fabsf(dot3(wo_n, hn));
-      if (!(den < 1e-6f)) pm = trd_Dw(d.dist, wo_n, hn) / den;
+      if (!(den < 1e-6f))
+        pm = 1.f / (1.f + c.lam_n) / abs_cost(wo_n) * trd_D(d.dist, hn) * fabsf(dot3(wo_n, hn)) / den;
     }
     {  // dcc_pdf -> cc_pdf_at
       float den = 4.f * fabsf(dot3(wi_n, hn));
@@ -415,6 +432,9 @@ PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint
   p += (d.gw / sum) * gp;
   p += (d.cw / sum) * pc;
   pdf = p;
+}
+PT_D void disney_f_pdf(const pt_material_desc &m, f3 wo, f3 wi, const ShadePoint &h, f3 &f, float &pdf) {
+  disney_f_pdf_ctx(disney_ctx(m, h, wo), wi, h, f, pdf);
 }
 
 // gl::sampleDiscrete (utility.hpp:155-190)
@@ -438,36 +458,39 @@ PT_D int sample_discrete(const float (&w)[N], float u, float *pmf, float *u_rema
   }
   return off;
 }
-PT_D bool disney_scatter(const pt_material_desc &m, f3 wo, ShadePoint &h, float uc, float u0, float u1,
-                         Scatter &s) {
-  Disney d = disney_params(m, h);
-  if (h.inside) return dglass_scatter<true>(d, wo, h, uc, u0, u1, s);
+PT_D bool disney_scatter_ctx(const DisneyCtx &c, f3 wo, const ShadePoint &h, float uc, float u0, float u1,
+                             Scatter &s) {
+  const Disney &d = c.d;
+  if (h.inside) return dglass_scatter<true>(d, c.b, c.wo_n, wo, h, uc, u0, u1, s);
   float w[4] = {d.dw, d.mw, d.gw, d.cw};
   float pmf, ucr;
   int lobe = sample_discrete<4>(w, uc, &pmf, &ucr);
   if (pmf <= 1e-6f) return false;
   bool ok;
   if (lobe == 0) {
-    Frame b = make_frame(h.n);
-    f3 wiw = to_world(b, cosine_hemi(ucr, u1));
+    f3 wiw = to_world(c.b, cosine_hemi(ucr, u1));
     s.wi = norm3(wiw);
     s.pdf = cosine_pdf(h.n, wiw);
     s.flags = BX_DIFFUSE | BX_REFL;
     s.att = s3(0.f);  // re-evaluated below (ddiffuse_f's value is not kept)
     ok = true;
   } else if (lobe == 1) {
-    ok = dmetal_scatter<false>(d, wo, h, ucr, u1, s);
+    ok = dmetal_scatter<false>(d, c.b, c.wo_n, wo, h, ucr, u1, s);
   } else if (lobe == 2) {
-    ok = dglass_scatter<false>(d, wo, h, ucr, ucr, u1, s);
+    ok = dglass_scatter<false>(d, c.b, c.wo_n, wo, h, ucr, ucr, u1, s);
   } else {
-    ok = dcc_scatter<false>(d, wo, h, ucr, u1, s);
+    ok = dcc_scatter<false>(d, c.b, c.wo_n, wo, h, ucr, u1, s);
   }
   if (!ok) return false;
   if (s.flags & BX_SPECULAR) return true;
   if (s.pdf <= 1e-6f) return false;
   f3 wiw = norm3(s.wi);
-  disney_f_pdf(m, wo, wiw, h, s.att, s.pdf);
+  disney_f_pdf_ctx(c, wiw, h, s.att, s.pdf);
   return s.pdf > 1e-7f;
+}
+PT_D bool disney_scatter(const pt_material_desc &m, f3 wo, const ShadePoint &h, float uc, float u0, float u1,
+                         Scatter &s) {
+  return disney_scatter_ctx(disney_ctx(m, h, wo), wo, h, uc, u0, u1, s);
 }
 
 // ============================== HairMarschner

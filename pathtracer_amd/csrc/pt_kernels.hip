@@ -874,8 +874,10 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
                   // one-sample MIS over the two techniques (a guided direction behind a
                   // reflection-only surface carries nothing)
                   const bool below = bsdf_reflect_only(mat->kind) && dot3(h.sp.n, wn) <= 0.f;
-                  const f3 f = below ? s3(0.f) : bsdf_f<MS>(*mat, wo, wn, h.sp);
-                  const float pbs = bsdf_pdf<MS>(*mat, wo, wn, h.sp);
+                  f3 fg;
+                  float pbs;
+                  bsdf_f_pdf<MS>(*mat, wo, wn, h.sp, fg, pbs);
+                  const f3 f = below ? s3(0.f) : fg;
                   pm = gc * g_pdf(mix, wn) + (1.f - gc) * pbs;
                   if (w.grec && d < w.grec_verts && slot % w.grec_stride == 0u) {
                     float *rec = w.grec + ((size_t)(slot / w.grec_stride) * w.grec_verts + d) * PT_G_REC;
