@@ -175,6 +175,12 @@ PT_D bool hit_prim(const float4 *prims, uint32_t p, f3 o, f3 d, float tmin, floa
 #define PT_BLOCK 256
 // LDS words per block for a stack of K entries (codes, plus distances); at
 // least one word so that the declaration stays valid for K = 0.
+// Entries per wave below which a traversal launch gives every wave a fixed
+// share instead of sharing a work queue (the Cornell frame's 32k-entry
+// partitions: 0.61 ms of closest hit with fixed shares, 0.66 ms queued).
+#ifndef PT_QUEUE_MIN
+#define PT_QUEUE_MIN 256
+#endif
 #define PT_TRAV_LDS(K, WITH_T) ((K) > 0 ? (K) * PT_BLOCK * ((WITH_T) ? 2 : 1) : 1)
 // Copies the scene's top nodes into the block's LDS (every thread calls it):
 // the quantised nodes for any-hit traversal (QN), else the 128-byte nodes.
@@ -358,21 +364,31 @@ PT_D bool traverse(const DevScene &s, int *lds, f3 o, f3 d, float tmin, float tm
 
 // ---------------------------------------------------------------------
 // Persistent traversal with per-lane ray refill and speculative while-while
-// loops (Aila & Laine, HPG 2009, on 64-wide waves).  A wave owns the
-// contiguous ray range [begin, end); each iteration runs a node phase and a
-// leaf phase (below), and lanes whose rays are finished take the next rays
-// of the range once REFILL of them are idle, so lanes do not idle while the
-// wave's longest ray finishes.  The per-ray results equal traverse<>'s (the
-// closest hit, or whether any hit exists) up to the winner of exact ties:
-// a postponed leaf is tested after some of the nodes traverse<> would visit
-// after it, which changes the order of tests and culls fewer subtrees.
-// fetch(i, o, d, tmin, tmax, scene) loads ray i (and, batched, points scene
-// at the ray's instance); finish(i, hit, t, prim) stores it.  begin, end must
-// be the same in every lane of the wave.
+// loops (Aila & Laine, HPG 2009, on 64-wide waves).  The rays [0, n) of a
+// queue partition are shared by its nwaves waves: wave wid first takes a
+// fixed contiguous share of the first ST/8 of them, then chunks of CH
+// consecutive entries of the rest through the partition's counter work[0]
+// (on the 1.3M-triangle glass frame, ST = 0 against fixed shares alone:
+// closest hit 30.6 -> 25.9 ms, any hit 17.5 -> 15.8 ms; ST = 4: 26.0, 17.0).
+// Each iteration runs a node phase and a leaf phase (below), and lanes whose
+// rays are finished take the next rays of the wave's range (a new chunk when
+// it runs dry) once REFILL of them are idle, so lanes do not idle while the
+// wave's longest ray finishes, and waves that drew cheap rays take more of
+// the rest instead of idling while others finish their share; the fixed
+// shares keep the counter's round trips to the tail of the launch.  The
+// per-ray results equal traverse<>'s (the closest hit, or whether any hit
+// exists) up to the winner of exact ties: a postponed leaf is tested after
+// some of the nodes traverse<> would visit after it, which changes the order
+// of tests and culls fewer subtrees.  work[0] must be zero at launch: a kernel
+// between two launches rewinds it (a count of finished waves that lets the
+// last one rewind it needs a device-scope release per wave, which writes back
+// the L2 and cost a launch of the Cornell frame 0.13 ms).  fetch(i, o, d, tmin,
+// tmax, scene) loads ray i (and, batched, points scene at the ray's instance);
+// finish(i, hit, t, prim) stores it.
 // ---------------------------------------------------------------------
-template <bool ANY, bool QN, bool COUNT, int K, int REFILL, bool BATCH, class Fetch, class Finish>
-PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t begin, uint32_t end, Fetch fetch,
-                      Finish finish, uint32_t &n_nodes, uint32_t &n_prims) {
+template <bool ANY, bool QN, bool COUNT, int K, int REFILL, int ST, int CH, bool BATCH, class Fetch, class Finish>
+PT_D void trace_queue(const DevScene &s, int *lds, const float4 *top, uint32_t n, uint32_t *work, uint32_t wid,
+                      uint32_t nwaves, Fetch fetch, Finish finish, uint32_t &n_nodes, uint32_t &n_prims) {
   // nodes [0, ntop) are read from the block's LDS copy `top` (single scenes)
   const uint32_t ntop = BATCH ? 0u : s.top_nodes;
   // BATCH: fetch names each ray's scene, whose tree the lane then walks
@@ -388,7 +404,14 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
   lds_float *const lt = (lds_float *)(lds + K * PT_BLOCK) + threadIdx.x;
   const uint32_t lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
-  uint32_t next = begin;
+  // the fixed shares, then the entries [dyn, n) that the counter hands out;
+  // below PT_QUEUE_MIN entries per wave every entry is in a fixed share (a
+  // launch that short would wait on the counter's round trip)
+  const bool shared = n >= (uint32_t)PT_QUEUE_MIN * nwaves;
+  const uint32_t per = shared ? (uint32_t)((uint64_t)n * (uint32_t)ST / (8ull * nwaves)) : (n + nwaves - 1u) / nwaves;
+  const uint32_t dyn = shared ? per * nwaves : n;
+  uint32_t next = min(n, wid * per), wend = min(n, next + per);  // the wave's unclaimed entries
+  bool more = dyn < n;  // wave-uniform: the counter may still hand out entries
   int ray = -1;
   f3 o = s3(0.f), d = s3(0.f);
   float ix = 0.f, iy = 0.f, iz = 0.f, oix = 0.f, oiy = 0.f, oiz = 0.f, tmin = 0.f, tb = 0.f;
@@ -498,33 +521,51 @@ PT_D void trace_range(const DevScene &s, int *lds, const float4 *top, uint32_t b
   while (true) {
     // refill the lanes that have no ray once at least REFILL of them are idle
     // (or all are): waiting for a group keeps the new rays of a wave adjacent
-    // (wave-uniform bookkeeping)
+    // (wave-uniform bookkeeping); the rest of the chunk first, then a new one
     const unsigned long long idle = __ballot(ray < 0);
-    const int n_idle = __popcll(idle);
-    if (next < end && (n_idle >= REFILL || n_idle == 64)) {
-      const uint32_t mine = next + (uint32_t)__popcll(idle & below);
-      if (ray < 0 && mine < end) {
-        ray = (int)mine;
-        float tmax;
-        const DevScene *sc = &s;
-        fetch(mine, o, d, tmin, tmax, sc);
-        if (BATCH) {
-          lane_nodes = QN ? sc->qnodes : sc->nodes;
-          lane_prims = sc->prims;
-          lane_root = sc->root;
-        }
-        ix = rcp_fast(d.x); iy = rcp_fast(d.y); iz = rcp_fast(d.z);
-        oix = o.x * ix; oiy = o.y * iy; oiz = o.z * iz;
-        tb = tmax;
-        best = -1;
-        cur = BATCH ? lane_root : s.root;
-        have_cur = true;
-        have_leaf = false;
-        lq = lend = 0;
-        sp = 0;
-        lo = 0;
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if ((next < wend || more) && (n_idle >= (uint32_t)REFILL || n_idle == 64u)) {
+      const uint32_t rem = wend - next;
+      uint32_t g = 0u, gn = 0u;
+      if (rem < n_idle && more) {
+        uint32_t v = 0u;
+        if (lane == (uint32_t)__builtin_amdgcn_readfirstlane((int)lane)) v = atomicAdd(work, (uint32_t)CH);
+        g = dyn + (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+        gn = g < n ? min((uint32_t)CH, n - g) : 0u;
+        more = g + (uint32_t)CH < n;
       }
-      next = min(end, next + (uint32_t)__popcll(idle));
+      if (ray < 0) {
+        const uint32_t k = (uint32_t)__popcll(idle & below);
+        const bool got = k < rem || k - rem < gn;
+        if (got) {
+          const uint32_t mine = k < rem ? next + k : g + (k - rem);
+          ray = (int)mine;
+          float tmax;
+          const DevScene *sc = &s;
+          fetch(mine, o, d, tmin, tmax, sc);
+          if (BATCH) {
+            lane_nodes = QN ? sc->qnodes : sc->nodes;
+            lane_prims = sc->prims;
+            lane_root = sc->root;
+          }
+          ix = rcp_fast(d.x); iy = rcp_fast(d.y); iz = rcp_fast(d.z);
+          oix = o.x * ix; oiy = o.y * iy; oiz = o.z * iz;
+          tb = tmax;
+          best = -1;
+          cur = BATCH ? lane_root : s.root;
+          have_cur = true;
+          have_leaf = false;
+          lq = lend = 0;
+          sp = 0;
+          lo = 0;
+        }
+      }
+      if (rem >= n_idle) {
+        next += n_idle;
+      } else {
+        next = g + min(gn, n_idle - rem);
+        wend = g + gn;
+      }
     }
     if (__ballot(ray >= 0) == 0ull) break;
     if (ray < 0) continue;

@@ -65,6 +65,7 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s0, DevCamera cam0, R
     const uint32_t extra = rest > part * PT_PART_RUN ? min(rest - part * PT_PART_RUN, (uint32_t)PT_PART_RUN) : 0u;
     *w.live(0, part) = full * PT_PART_RUN + extra;
     *w.live(1, part) = 0u;
+    *w.work(0, part) = 0u;  // the first extension launch's queue
   }
   const uint32_t slot = ((l / PT_PART_RUN) * PT_PARTS + part) * PT_PART_RUN + l % PT_PART_RUN;
   if (slot >= n) return;
@@ -132,6 +133,20 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s0, DevCamera cam0, R
 #ifndef PT_REFILL_SHADOW
 #define PT_REFILL_SHADOW 8
 #endif
+// Traversal work sharing: eighths of a partition's entries split into fixed
+// per-wave shares, and the entries a wave claims at a time from the rest.
+#ifndef PT_STATIC_EXT
+#define PT_STATIC_EXT 0
+#endif
+#ifndef PT_STATIC_SHADOW
+#define PT_STATIC_SHADOW 0
+#endif
+#ifndef PT_CHUNK_EXT
+#define PT_CHUNK_EXT 64
+#endif
+#ifndef PT_CHUNK_SHADOW
+#define PT_CHUNK_SHADOW 64
+#endif
 template <bool COUNT, bool BATCH>
 __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRAV_WAVES))) k_extend(DevScene s, Wave w, int qi, BatchView b) {
   __shared__ int lds[PT_TRAV_LDS(PT_K_EXT, true)];
@@ -146,13 +161,9 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
     *w.shadow(threadIdx.x) = 0u;
   }
   uint32_t nn = 0, np = 0, segs = 0;
-  // this wave's contiguous share of the partition
-  const uint32_t wpb = blockDim.x / 64u, nw = nb * wpb;
-  const uint32_t wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
-  const uint32_t per = (n + nw - 1) / nw;
-  const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
-  trace_range<false, false, COUNT, PT_K_EXT, PT_REFILL_EXT, BATCH>(
-      s, lds, top, begin, end,
+  const uint32_t wpb = blockDim.x / 64u, wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
+  trace_queue<false, false, COUNT, PT_K_EXT, PT_REFILL_EXT, PT_STATIC_EXT, PT_CHUNK_EXT, BATCH>(
+      s, lds, top, n, w.work(0, part), wid, nb * wpb,
       [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
         if (BATCH) sc = &b.scenes[b.slot_inst[ps.slot[i]]];
@@ -188,12 +199,9 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   const uint32_t n = min(*w.shadow(p), w.shadow_part_cap);
   const uint32_t base = p * w.shadow_part_cap;
   uint32_t nn = 0, np = 0, rays = 0;
-  const uint32_t wpb = blockDim.x / 64u, nw = nb * wpb;
-  const uint32_t wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
-  const uint32_t per = (n + nw - 1) / nw;
-  const uint32_t begin = min(n, wid * per), end = min(n, begin + per);
-  trace_range<true, QN, COUNT, PT_K_SHADOW, PT_REFILL_SHADOW, BATCH>(
-      s, lds, top, begin, end,
+  const uint32_t wpb = blockDim.x / 64u, wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
+  trace_queue<true, QN, COUNT, PT_K_SHADOW, PT_REFILL_SHADOW, PT_STATIC_SHADOW, PT_CHUNK_SHADOW, BATCH>(
+      s, lds, top, n, w.work(1, p), wid, nb * wpb,
       [&](uint32_t i, f3 &o, f3 &d, float &tmin, float &tmax, const DevScene *&sc) {
         i += base;
         if (BATCH) sc = &b.scenes[b.slot_inst[VIS ? w.sslot[i] / 4u : w.sslot[i]]];
@@ -372,6 +380,9 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
   const PathState &in = w.st[qi];
   const PathState &out = w.st[qi ^ 1];
   uint32_t *cout = w.live(qi ^ 1, part);
+  // the traversal queues of the launches that follow (the shadow rays this
+  // vertex emits, the next extension) start from zero; the ones before ended
+  if (blockIdx.x == 0 && threadIdx.x < 2 * PT_PARTS) *w.work(threadIdx.x / PT_PARTS, threadIdx.x % PT_PARTS) = 0u;
   __shared__ uint32_t o_cnt[4][16], o_first[16];
   __shared__ uint16_t o_perm[256];
   // The block reads its tiles of 256 entries into two LDS lists and shades
