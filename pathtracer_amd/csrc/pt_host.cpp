@@ -1389,7 +1389,7 @@ static pt_status ensure_wave(WaveStore &ws, uint32_t cap) {
   if (!wave_layout(cap, l)) return fail(PT_ERR_INVALID, "pt_render: wavefront too large");
   HIP_TRY(ws.buf.alloc(l.bytes));
   HIP_TRY(ws.counts.alloc(PT_COUNTER_WORDS * sizeof(uint32_t)));
-  // the traversal work queues start at zero (each launch rewinds its own)
+  // the work queues start at zero (the kernels rewind them between launches)
   HIP_TRY(hipMemset(ws.counts.p, 0, PT_COUNTER_WORDS * sizeof(uint32_t)));
   HIP_TRY(ws.stats.alloc(CNT_N * sizeof(unsigned long long)));
   carve_wave(ws.buf.as<char>(), l, cap, ws.counts.as<uint32_t>(), ws.stats.as<unsigned long long>(), ws.wave);
@@ -1879,8 +1879,8 @@ extern "C" pt_status pt_render_noalloc(pt_scene *s, const pt_render_cfg *cfg, fl
   const auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipMemsetAsync(film, 0, npix * 3 * sizeof(float), s->stream));
   HIP_TRY(hipMemsetAsync(wave.stats, 0, CNT_N * sizeof(unsigned long long), s->stream));
-  // the traversal work queues start at zero (each launch rewinds its own)
-  HIP_TRY(hipMemsetAsync(wave.work(0, 0), 0, 2 * PT_PARTS * PT_COUNTER_STRIDE * sizeof(uint32_t), s->stream));
+  // the work queues start at zero (the kernels rewind them between launches)
+  HIP_TRY(hipMemsetAsync(wave.work(0, 0), 0, 3 * PT_PARTS * PT_COUNTER_STRIDE * sizeof(uint32_t), s->stream));
   launch_pixel_offsets(s->stream, cfg->spp_x, cfg->spp_y, cfg->seed, off);
   RenderParams rp = make_params(cfg, spp, sppc);
   const DevCamera cam = make_camera(s->cam);

@@ -124,14 +124,15 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s0, DevCamera cam0, R
 #define PT_TRAV_WAVES 8
 #endif
 // Idle lanes a wave waits for before refilling them (64: only when all are).
-// Swept on the 1.3M-triangle glass frame: closest hit {64, 32, 16, 8} ->
-// {50.0, 47.3, 48.3, 49.4} ms, any hit {1, 8, 16, 32} -> {40.7, 39.9, 39.8,
-// 40.1} ms; the grid is one resident round (8 blocks per CU).
+// Swept on the 1.3M-triangle glass frame with the work queues: closest hit
+// {16, 32, 48} -> {26.2, 25.7, 25.9} ms, any hit {2, 8, 16} -> {16.9, 15.7,
+// 15.2} ms (the Disney + hair grid: any hit 12.0 -> 11.0 ms at 16); the grid
+// is one resident round (8 blocks per CU).
 #ifndef PT_REFILL_EXT
 #define PT_REFILL_EXT 32
 #endif
 #ifndef PT_REFILL_SHADOW
-#define PT_REFILL_SHADOW 8
+#define PT_REFILL_SHADOW 16
 #endif
 // Traversal work sharing: eighths of a partition's entries split into fixed
 // per-wave shares, and the entries a wave claims at a time from the rest.
@@ -140,6 +141,13 @@ __global__ void __launch_bounds__(256) k_generate(DevScene s0, DevCamera cam0, R
 #endif
 #ifndef PT_STATIC_SHADOW
 #define PT_STATIC_SHADOW 0
+#endif
+// Shading: tiles claimed through a counter (1) or dealt round robin (0).
+#ifndef PT_SHADE_QUEUE
+#define PT_SHADE_QUEUE 1
+#endif
+#ifndef PT_SHADE_QUEUE_MIN
+#define PT_SHADE_QUEUE_MIN 4
 #endif
 #ifndef PT_CHUNK_EXT
 #define PT_CHUNK_EXT 64
@@ -159,6 +167,7 @@ __global__ void __launch_bounds__(PT_BLOCK) __attribute__((amdgpu_waves_per_eu(P
   if (blockIdx.x == 0 && threadIdx.x < PT_PARTS) {
     *w.live(qi ^ 1, threadIdx.x) = 0u;
     *w.shadow(threadIdx.x) = 0u;
+    *w.work(2, threadIdx.x) = 0u;  // the shading launch's tile queue
   }
   uint32_t nn = 0, np = 0, segs = 0;
   const uint32_t wpb = blockDim.x / 64u, wid = (blockIdx.x / PT_PARTS) * wpb + threadIdx.x / 64u;
@@ -398,10 +407,22 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
   __shared__ uint32_t o_n[2];
   constexpr bool SPLIT = EST == PT_EST_REFERENCE_NEE || EST == PT_EST_NEE_MIS || EST == PT_EST_GUIDED;
   const bool split = SPLIT && !w.sort_keys;  // uniform
+  // The partition's tiles go to its blocks through the counter work(2, part)
+  // (rewound by the extension kernel before): a block claims its next tile
+  // while it reads the current one, so blocks that drew cheap tiles take more.
+  // Below PT_SHADE_QUEUE_MIN tiles per block the tiles are dealt round robin
+  // (a launch that short would wait on the counter).  Measured against round
+  // robin: shading of the Disney + hair grid frame 60.4 -> 54.5 ms, of the
+  // glass / HDRI frame 20.3 -> 19.5 ms.
+  __shared__ uint32_t o_tile[2];
+  uint32_t *const tq = w.work(2, part);
+  const bool queued = PT_SHADE_QUEUE && n >= (uint32_t)PT_SHADE_QUEUE_MIN * 256u * nb;  // uniform
   if (threadIdx.x < 2u) o_n[threadIdx.x] = 0u;
+  if (queued && threadIdx.x == 0) o_tile[0] = atomicAdd(tq, 256u);
   __syncthreads();
-  for (uint32_t t0 = (blockIdx.x / PT_PARTS) * 256u;; t0 += nb * 256u) {
-    const bool last = t0 + nb * 256u >= n;  // the block's last tile (uniform)
+  uint32_t t0 = queued ? o_tile[0] : (blockIdx.x / PT_PARTS) * 256u;
+  for (uint32_t it = 0;; ++it) {
+    if (queued && threadIdx.x == 0) o_tile[(it + 1u) & 1u] = atomicAdd(tq, 256u);
     if (t0 < n) {
       const uint32_t e = t0 + threadIdx.x;
       bool push = e < n;
@@ -429,6 +450,8 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
     }
     __syncthreads();
     const uint32_t cnt0 = o_n[0], cnt1 = o_n[1];
+    const uint32_t t_next = queued ? o_tile[(it + 1u) & 1u] : t0 + nb * 256u;
+    const bool last = t_next >= n;  // the block's last tile (uniform)
     __syncthreads();
     for (uint32_t c = 0; c < 2u; ++c) {
       uint32_t cnt = c ? cnt1 : cnt0;
@@ -960,6 +983,7 @@ __attribute__((amdgpu_waves_per_eu(MS == 0xffffffffu ? PT_SHADE_WAVES_ALL : PT_S
       }
     }
     if (last) break;
+    t0 = t_next;
   }
 }
 

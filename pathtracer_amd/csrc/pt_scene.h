@@ -327,13 +327,13 @@ struct Wave {
   uint32_t sort_keys;  // 1: shading orders each tile by material kind
   PT_HD uint32_t *live(uint32_t b, uint32_t p) const { return counts + (b * PT_PARTS + p) * PT_COUNTER_STRIDE; }
   PT_HD uint32_t *shadow(uint32_t p) const { return counts + (2 * PT_PARTS + p) * PT_COUNTER_STRIDE; }
-  // traversal work queue of partition p (k = 0: closest hit, 1: any hit):
-  // the next unclaimed entry beyond the waves' fixed shares.  k_generate
-  // rewinds the closest-hit counters and k_shade both kinds, so every
-  // traversal launch starts from zero.
+  // work queue of partition p (k = 0: closest hit, 1: any hit, 2: shading):
+  // the next unclaimed entry.  k_generate rewinds the closest-hit counters,
+  // k_shade both traversal kinds and k_extend the shading ones, so every
+  // launch starts from zero.
   PT_HD uint32_t *work(uint32_t k, uint32_t p) const { return counts + ((3 + k) * PT_PARTS + p) * PT_COUNTER_STRIDE; }
 };
-#define PT_COUNTER_WORDS (5 * PT_PARTS * PT_COUNTER_STRIDE)
+#define PT_COUNTER_WORDS (6 * PT_PARTS * PT_COUNTER_STRIDE)
 
 enum {
   CNT_Q0 = 0, CNT_Q1 = 1, CNT_SEGMENTS = 3, CNT_SHADOW_RAYS = 4,
