@@ -1516,9 +1516,15 @@ static uint32_t n_vertices(const pt_render_cfg *cfg) {
 struct Grids {
   uint32_t shade, trav;
 };
+// Default shading grid cap, blocks per CU.  With the tile queue, 6, 8, 16 and
+// 64 were within 1.2% of 32 on the 1.3M-triangle glass frame and the Disney +
+// hair grid, and 8 lost 3% on the Cornell frame (fewer blocks than tiles).
+#ifndef PT_SHADE_BLOCKS_PER_CU
+#define PT_SHADE_BLOCKS_PER_CU 32u
+#endif
 static Grids make_grids(uint32_t n_cu, const pt_tuning &t) {
   const uint32_t tb = t.trav_blocks_per_cu ? t.trav_blocks_per_cu : 8u;
-  const uint32_t sb = t.shade_blocks_per_cu ? t.shade_blocks_per_cu : 32u;
+  const uint32_t sb = t.shade_blocks_per_cu ? t.shade_blocks_per_cu : PT_SHADE_BLOCKS_PER_CU;
   Grids g;
   g.shade = std::max<uint32_t>(PT_PARTS, n_cu * sb / PT_PARTS * PT_PARTS);
   g.trav = std::max<uint32_t>(PT_PARTS, n_cu * tb / PT_PARTS * PT_PARTS);
